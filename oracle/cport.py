@@ -1,0 +1,81 @@
+"""Test infrastructure: ctypes wrapper of the C restatement (librbchem_oracle.so)."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "librbchem_oracle.so")
+
+
+class _Grid(C.Structure):
+    _fields_ = [("nlon", C.c_int), ("nlat", C.c_int), ("niv", C.c_int), ("nivbas", C.c_int),
+                ("nivhau", C.c_int), ("niter", C.c_int)]
+
+
+_lib = None
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", HERE, "restatement"], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = C.CDLL(LIB)
+        dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+        L.rbo_chem_step.argtypes = [C.POINTER(_Grid), C.c_double, dp, dp, dp, dp, dp, dp, dp, dp,
+                                    dp, dp, dp, dp, C.c_int]
+        L.rbo_chem_step.restype = C.c_int
+        L.rbo_conserv_floor.argtypes = [C.POINTER(_Grid), dp, dp, C.c_int]
+        L.rbo_troe_constants.argtypes = [dp]
+        L.rbo_troe.argtypes = [C.c_double] * 6
+        L.rbo_troe.restype = C.c_double
+        L.rbo_rates.argtypes = [C.c_double, C.c_double, C.c_double, dp]
+        L.rbo_rates.restype = C.c_int
+        L.rbo_rate_name.argtypes = [C.c_int]
+        L.rbo_rate_name.restype = C.c_char_p
+        L.rbo_phot_box.argtypes = [dp, dp, C.c_double, C.c_double, C.c_double, dp]
+        L.rbo_column.argtypes = [C.c_int, dp, dp, dp, dp, C.c_int, dp, dp, dp]
+        L.rbo_jno_box.argtypes = [C.c_double] * 5
+        L.rbo_jno_box.restype = C.c_double
+        L.rbo_density.argtypes = [C.c_double] * 3
+        L.rbo_density.restype = C.c_double
+        L.rbo_aerosol_box.argtypes = [C.c_double, C.c_double, C.c_double, dp, dp, dp, dp, dp,
+                                      np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")]
+        _lib = L
+    return _lib
+
+
+def chem_step(inp, ajl, o3up, niter: int = 5, nthreads: int = 0, conserv: bool = True) -> dict:
+    """Run the restatement on a synth.StepInputs; returns outputs like refrun."""
+    g = inp.grid
+    G = _Grid(g.nlon, g.nlat, g.niv, g.nivbas, g.nivhau, niter)
+    q = np.ascontiguousarray(inp.qj1, dtype=np.float64).copy()
+    h = np.ascontiguousarray(inp.hc, dtype=np.float64).copy()
+    surf = np.zeros(g.shape3)
+    shb = (g.nivbas, g.nlat, g.nlon)
+    vsed, p2o, pno3 = np.zeros(shb), np.zeros(shb), np.zeros(shb)
+    c = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+    ajl_f = np.ascontiguousarray(np.asarray(ajl, dtype=np.float64).transpose(3, 2, 1, 0))
+    rc = lib().rbo_chem_step(C.byref(G), float(inp.dt), c(inp.T), c(inp.p), c(inp.sza), c(inp.alt),
+                             ajl_f, c(o3up), q, h, surf, vsed, p2o, pno3, nthreads)
+    if rc != 0:
+        raise RuntimeError(f"rbo_chem_step failed: {rc}")
+    out = {"qj1_chem": q.copy(), "hc_chem": h.copy(), "surfarea": surf, "vsed3d": vsed,
+           "parth2osed": p2o, "parthno3sed": pno3}
+    if conserv:
+        lib().rbo_conserv_floor(C.byref(G), q, h, nthreads)
+        out["qj1"], out["hc"] = q, h
+    return out
+
+
+def fortran_ajl(ajl):
+    """ajl (40,101,27,7) in Fortran index order -> contiguous Fortran bytes."""
+    return np.ascontiguousarray(np.asarray(ajl, dtype=np.float64).transpose(3, 2, 1, 0))

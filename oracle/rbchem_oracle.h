@@ -1,0 +1,65 @@
+/* Test infrastructure: CPU restatement of the REPROBUS per-box chemistry step.
+ *
+ * Follows src/reprobus_l137_20230723.f (chem3d F:601-2412, analytic F:2620-3361,
+ * hetero F:3365-4029, density F:4033-4073, phot F:4996-5149, total F:5153-5186,
+ * const Troe block F:5927-5948, conserv F:2526-2616, floor F:431-440) and
+ * src/jno.f90 (J:1-249).  Used only by tests, smoke checks and the CPU baseline;
+ * the product path never links it.
+ *
+ * Arrays use the model's Fortran layout: a field is (nlon, nlat, niv) with lon
+ * fastest; species fields are stacked last.  Level 1 is the model top.
+ */
+#ifndef RBCHEM_ORACLE_H
+#define RBCHEM_ORACLE_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+    int nlon, nlat, niv;   /* array extents */
+    int nivbas;            /* chemistry acts on levels 1..nivbas */
+    int nivhau;            /* heterogeneous chemistry on levels nivhau..nivbas */
+    int niter;             /* sweeps per step (reference: 5) */
+} rbo_grid;
+
+/* One chemistry step over the whole grid (chem3d for every row). qj1 (43
+ * fields) and hc (15 fields) are updated in place; surfarea (niv levels) is
+ * written on levels nivhau..nivbas; vsed3d/parth2osed/parthno3sed (nivbas
+ * levels) everywhere.  ajl is (40,101,27,7), o3up (80).  nthreads <= 0: all. */
+int rbo_chem_step(const rbo_grid *g, double dt, const double *T, const double *p,
+                  const double *sza, const double *alt, const double *ajl,
+                  const double *o3up, double *qj1, double *hc, double *surfarea,
+                  double *vsed3d, double *parth2osed, double *parthno3sed, int nthreads);
+
+/* conserv (F:2526-2616) on levels 1..nivbas, then the 1e-30 floor of
+ * species 1..42 on all niv levels (F:431-440). */
+void rbo_conserv_floor(const rbo_grid *g, double *qj1, double *hc, int nthreads);
+
+/* ---- pieces exposed for unit tests ---- */
+/* Troe prefactors of const (F:5927-5948), 18 values in COMMON order. */
+void rbo_troe_constants(double out18[18]);
+/* Falloff rate as chem3d evaluates it (e.g. F:930-935). */
+double rbo_troe(double a0, double a1, double n0, double n1, double t, double hnm);
+/* 103 thermal rates in the order of rbo_rate_names(); returns the count. */
+int rbo_rates(double t, double hnm, double pm, double *out);
+const char *rbo_rate_name(int i);
+/* phot trilinear lookup for one box: tj[0..39]. */
+void rbo_phot_box(const double *ajl, const double *o3up, double alt, double sza,
+                  double o3t, double *tj40);
+/* Column pre-pass for one column of nlev levels (stride between levels `st`):
+ * o3t (total), colo2 and colno (jno prefix sums, before the sec(sza) factor). */
+void rbo_column(int nlev, const double *p, const double *T, const double *hc_o3,
+                const double *hc_no, int st, double *o3t, double *colo2, double *colno);
+/* J(NO) for one box from its column sums (jno inner loop). */
+double rbo_jno_box(double colo2, double colno, double colo3, double hnm, double sza_top);
+/* density (F:4033-4073) */
+double rbo_density(double ws, double wn, double t);
+/* analytic + hetero for one box (het levels): outputs het[9], parthno3, parthno3sl,
+ * parth2o, parthcl, surfarea, vsed, plus diagnostics flags[3] = lnat, lice, condliq. */
+void rbo_aerosol_box(double t, double p, double hnm, const double *ck_1based,
+                     double het[9], double part[5], double *surf, double *vsed, int flags[3]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

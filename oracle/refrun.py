@@ -1,0 +1,104 @@
+"""Test infrastructure: run the flang-built reference chemistry on a small grid.
+
+The reference computes SZA itself (``zenith``, src/reprobus_l137_20230723.f:
+5308-5406) from trajectory longitude/latitude and the day number.  To realise a
+requested per-box SZA we put every box on the equator (``trajlat = 0``) and
+solve the hour-angle relation of that routine for the longitude; the reference
+then reports the SZA it actually used in ``sza3d``, and that array is what the
+GPU path and the C restatement receive.
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REF_BIN = os.path.join(HERE, "_ref", "ref_harness")
+NLON_REF = 180
+
+DAYNUM = 80.5      # fixed day number: declination near zero, ut = 12 h
+
+
+def available() -> bool:
+    return os.path.exists(REF_BIN)
+
+
+def _zenith_terms(day):
+    """Declination (rad) and equation of time (h) as zenith computes them (F:5336-5376)."""
+    pi = 2.0 * np.arcsin(1.0)
+    tz = 2.0 * pi * day / 365.0
+    s, c = np.sin(tz), np.cos(tz)
+    s2, c2 = 2.0 * s * c, c * c - s * s
+    s3, c3 = s * c2 + c * s2, c * c2 - s * s2
+    rdecl = (0.006918 - 0.399912 * c + 0.070257 * s - 0.006758 * c2 + 0.000907 * s2
+             - 0.002697 * c3 + 0.001480 * s3)
+    eqr = 0.000075 + 0.001868 * c - 0.032077 * s - 0.014615 * c2 - 0.040849 * s2
+    eqh = eqr * 24.0 / (2.0 * pi)
+    return rdecl, eqh
+
+
+def trajectories_for_sza(sza_deg: np.ndarray, day: float = DAYNUM):
+    """(trajlon, trajlat) placing each box on the equator at the hour angle giving ``sza``."""
+    pi = 2.0 * np.arcsin(1.0)
+    rdecl, eqh = _zenith_terms(day)
+    ut = (day - np.trunc(day)) * 24.0
+    cz = np.cos(np.radians(sza_deg)) / np.cos(rdecl)
+    cz = np.clip(cz, -1.0, 1.0)
+    zpt = np.arccos(cz)                  # hour angle (rad), afternoon branch
+    lzut = zpt * 180.0 / pi
+    lbut = ut - lzut / 15.0
+    lon = (12.0 - eqh - lbut) * 360.0 / 24.0
+    return lon, np.zeros_like(lon)
+
+
+def run_reference(inp, nrep: int = 1, day: float = DAYNUM, workdir: str | None = None,
+                  keep: bool = False) -> dict:
+    """Run the reference on ``inp`` (a synth.StepInputs with nlat*nlon <= 16380 columns)."""
+    from reprobus_amd import jtable
+
+    if not available():
+        raise RuntimeError(f"reference harness not built: {REF_BIN} (run `make -C oracle ref`)")
+    g = inp.grid
+    ncol, nlev = g.nlat * g.nlon, g.nivbas
+    lon, lat = trajectories_for_sza(inp.sza[:nlev])
+    base = workdir or os.path.join(HERE, "_ref")
+    os.makedirs(base, exist_ok=True)
+    run = tempfile.mkdtemp(prefix="run_", dir=base)
+    try:
+        ajl, o3up = jtable.default_table()
+        jtable.write_jstrato(os.path.join(run, "jstrato.txt"), ajl, o3up)
+        with open(os.path.join(run, "relief.txt"), "w") as f:
+            f.write(f"{180 * 91}*0.0\n")
+        fin, fout = os.path.join(run, "in.bin"), os.path.join(run, "out.bin")
+        with open(fin, "wb") as f:
+            np.array([ncol, nlev, nrep], dtype=np.int32).tofile(f)
+            np.array([day, inp.dt], dtype=np.float64).tofile(f)
+            for a in (lon, lat, inp.p[:nlev], inp.T[:nlev], inp.alt[:nlev]):
+                np.ascontiguousarray(a, dtype=np.float64).tofile(f)
+            np.ascontiguousarray(inp.qj1[:, :nlev], dtype=np.float64).tofile(f)
+            np.ascontiguousarray(inp.hc[:, :nlev], dtype=np.float64).tofile(f)
+        r = subprocess.run([REF_BIN, fin, fout], cwd=run, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"reference harness failed ({r.returncode}): {r.stdout}\n{r.stderr}")
+        with open(fout, "rb") as f:
+            hdr = np.fromfile(f, dtype=np.int32, count=3)
+            secs = float(np.fromfile(f, dtype=np.float64, count=1)[0])
+            shp = (nlev, g.nlat, g.nlon)
+
+            def rd(nf):
+                a = np.fromfile(f, dtype=np.float64, count=nf * ncol * nlev)
+                return a.reshape((nf,) + shp) if nf > 1 else a.reshape(shp)
+
+            out = {"ncalls": int(hdr[2]), "seconds": secs,
+                   "qj1_chem": rd(43), "hc_chem": rd(15), "surfarea": rd(1), "sza3d": rd(1),
+                   "vsed3d": rd(1), "parth2osed": rd(1), "parthno3sed": rd(1),
+                   "qj1": rd(43), "hc": rd(15)}
+        out["boxes_per_call"] = NLON_REF * 94
+        return out
+    finally:
+        if not keep:
+            shutil.rmtree(run, ignore_errors=True)

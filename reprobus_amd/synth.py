@@ -1,0 +1,273 @@
+"""Synthetic chemistry-step inputs for the benchmark and test grids.
+
+Arrays follow the model's Fortran layout (``qj1(nlon,nlat,niv,nbcon)``: lon
+fastest, then lat, level, species), held here as C-ordered numpy arrays of shape
+``(nspecies, niv, nlat, nlon)`` -- the same bytes.  Level 1 is the model top.
+
+Grids
+  * ``c0``: 64 columns x 60 levels (3,840 boxes), one 900 s step; T uniform
+    185-270 K per box, cos SZA uniform -0.3..1 per box, sulfate SAD log-uniform
+    0.1-10 um2/cm3.
+  * ``c1``: 144 lon x 91 lat x 60 levels (786,240 boxes); T from a latitude
+    profile (200 K polar winter to 270 K stratopause) plus 5 K Gaussian noise;
+    SZA from latitude, longitude and a fixed solar declination.
+Both use 60 levels log-spaced from 1 hPa (top) to 100 hPa, ``alt`` from a 7 km
+scale height, and species drawn log-uniformly within stratospheric ranges with
+the Cly / NOy / Bry family totals fixed per box and partitioned among members.
+
+Surface area is not an input of the reference step: it derives liquid-aerosol
+area from H2SO4 (``analytic``, src/reprobus_l137_20230723.f:2919-3096).  The
+requested SAD is therefore realised by choosing H2SO4 from the binary
+H2SO4/H2O aerosol relation of that routine (aliq proportional to qH2SO4**(2/3) at
+fixed T and water vapour); boxes that go ternary (T < 215 K) or form NAT/ice
+realise a different area, which the step reports in ``surfarea``.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+NBCON, NCM = 43, 15
+
+# Long-lived slots, src/reprobus_l137_20230723.f:720-730 (1-based there).
+LONG = ["n2o", "ch4", "h2o", "noy", "hno3", "n2o5", "cly", "ox", "co", "oclo",
+        "ox_passive", "h2so4", "hcl", "clono2", "hocl", "cl2", "h2o2", "clno2", "hbr",
+        "brono2", "nox", "ho2no2", "clox", "brox", "cl2o2", "hobr", "brcl", "ch2o",
+        "ch3o2", "ch3o2h", "cfc11", "cfc12", "cfc113", "ccl4", "ch3ccl3", "ch3cl",
+        "hcfc22", "ch3br", "h1211", "h1301", "bry", "ch2br2", "hno3_gas"]
+# Short-lived slots, F:743-746.
+SHORT = ["o1d", "oh", "cl", "o3p", "o3", "ho2", "no2", "no", "br", "n", "clo", "bro",
+         "no3", "h", "ch3"]
+L = {n: i for i, n in enumerate(LONG)}
+S = {n: i for i, n in enumerate(SHORT)}
+
+NLEV = 60
+P_TOP, P_BOT = 1.0, 100.0          # hPa
+NIVHAU = 14                        # first level with heterogeneous chemistry (F:2425)
+
+# log-uniform ranges (mixing ratio) for species not tied to a family
+_RANGES = {
+    "n2o": (2e-8, 3.2e-7), "ch4": (2e-7, 1.8e-6), "h2o": (2e-6, 8e-6),
+    "co": (1e-8, 6e-8), "h2o2": (1e-12, 1e-10), "ch2o": (1e-12, 1e-10),
+    "ch3o2": (1e-15, 1e-11), "ch3o2h": (1e-12, 1e-10),
+    "cfc11": (1e-13, 2.6e-10), "cfc12": (1e-11, 5.3e-10), "cfc113": (1e-13, 8e-11),
+    "ccl4": (1e-14, 1e-10), "ch3ccl3": (1e-14, 2e-11), "ch3cl": (1e-12, 5.5e-10),
+    "hcfc22": (1e-11, 2.5e-10), "ch3br": (1e-14, 7e-12), "h1211": (1e-14, 4e-12),
+    "h1301": (1e-14, 3.5e-12), "ch2br2": (1e-14, 5e-12),
+}
+O3_RANGE = (1e-7, 1e-5)
+NOY_RANGE = (1e-10, 2e-8)
+CLY_RANGE = (1e-10, 3.5e-9)
+BRY_RANGE = (1e-12, 2.2e-11)
+RADICAL_RANGE = (1e-15, 1e-10)
+
+
+@dataclass
+class Grid:
+    nlon: int
+    nlat: int
+    niv: int
+    nivbas: int
+    nivhau: int = NIVHAU
+
+    @property
+    def nbox(self) -> int:
+        return self.nlon * self.nlat * self.nivbas
+
+    @property
+    def shape3(self) -> tuple:
+        return (self.niv, self.nlat, self.nlon)
+
+
+@dataclass
+class StepInputs:
+    """One chemistry step's inputs (numpy, Fortran byte order)."""
+    grid: Grid
+    dt: float
+    T: np.ndarray            # (niv, nlat, nlon) K
+    p: np.ndarray            # (niv, nlat, nlon) hPa
+    sza: np.ndarray          # (niv, nlat, nlon) deg
+    alt: np.ndarray          # (niv, nlat, nlon) km
+    qj1: np.ndarray          # (43, niv, nlat, nlon) mixing ratio
+    hc: np.ndarray           # (15, niv, nlat, nlon) mixing ratio
+    sad_target: np.ndarray | None = None   # (niv, nlat, nlon) cm2/cm3
+    meta: dict = field(default_factory=dict)
+
+    def copy(self) -> "StepInputs":
+        return StepInputs(self.grid, self.dt, self.T.copy(), self.p.copy(), self.sza.copy(),
+                          self.alt.copy(), self.qj1.copy(), self.hc.copy(),
+                          None if self.sad_target is None else self.sad_target.copy(),
+                          dict(self.meta))
+
+
+def level_pressures(nlev: int = NLEV) -> np.ndarray:
+    """Levels log-spaced from P_TOP (level 1) to P_BOT (level nlev), hPa."""
+    return np.exp(np.linspace(np.log(P_TOP), np.log(P_BOT), nlev))
+
+
+def altitude_km(p_hpa: np.ndarray) -> np.ndarray:
+    """Geometric altitude from pressure with a 7 km scale height."""
+    return 7.0 * np.log(1013.25 / p_hpa)
+
+
+def _logu(rng, lo, hi, shape):
+    return np.exp(rng.uniform(np.log(lo), np.log(hi), size=shape))
+
+
+def _partition(rng, total, nmember, shape, lo=0.02):
+    """Split ``total`` into ``nmember`` positive parts with log-uniform weights."""
+    w = _logu(rng, lo, 1.0, (nmember,) + shape)
+    return total[None] * w / w.sum(axis=0, keepdims=True)
+
+
+def draw_species(rng: np.random.Generator, shape: tuple) -> tuple[np.ndarray, np.ndarray]:
+    """Long-lived (43,*shape) and short-lived (15,*shape) mixing ratios."""
+    q = np.zeros((NBCON,) + shape)
+    h = np.zeros((NCM,) + shape)
+    for name, (lo, hi) in _RANGES.items():
+        q[L[name]] = _logu(rng, lo, hi, shape)
+
+    # Bry = HBr + BrONO2 + BrOx + HOBr + BrCl   (conserv, F:2591-2598)
+    bry = _logu(rng, *BRY_RANGE, shape)
+    hbr, brono2, brox, hobr, brcl = _partition(rng, bry, 5, shape)
+    q[L["bry"]], q[L["hbr"]], q[L["brono2"]] = bry, hbr, brono2
+    q[L["brox"]], q[L["hobr"]], q[L["brcl"]] = brox, hobr, brcl
+
+    # Cly - BrCl = OClO + HCl + ClONO2 + HOCl + 2 Cl2 + ClNO2 + ClOx + 2 Cl2O2  (F:2541-2551)
+    cly = _logu(rng, *CLY_RANGE, shape)
+    parts = _partition(rng, cly - brcl, 8, shape)
+    oclo, hcl, clono2, hocl, cl2x2, clno2, clox, cl2o2x2 = parts
+
+    # NOy - ClONO2 - ClNO2 - BrONO2 = HNO3 + 2 N2O5 + NOx + HNO4  (F:2566-2580);
+    # keep the chlorine/bromine nitrates under half of NOy so the NOy members stay positive.
+    noy = _logu(rng, *NOY_RANGE, shape)
+    nitr = clono2 + clno2 + brono2
+    cap = 0.5 * noy
+    scale = np.where(nitr > cap, np.maximum(cap - brono2, 0.0) / np.maximum(clono2 + clno2, 1e-300), 1.0)
+    scale = np.minimum(scale, 1.0)
+    moved = (clono2 + clno2) * (1.0 - scale)
+    clono2, clno2, hcl = clono2 * scale, clno2 * scale, hcl + moved
+    rest = noy - clono2 - clno2 - brono2
+    hno3, n2o5x2, nox, hno4 = _partition(rng, rest, 4, shape)
+
+    q[L["cly"]], q[L["oclo"]], q[L["hcl"]], q[L["clono2"]] = cly, oclo, hcl, clono2
+    q[L["hocl"]], q[L["cl2"]], q[L["clno2"]], q[L["clox"]] = hocl, 0.5 * cl2x2, clno2, clox
+    q[L["cl2o2"]] = 0.5 * cl2o2x2
+    q[L["noy"]], q[L["hno3"]], q[L["n2o5"]] = noy, hno3, 0.5 * n2o5x2
+    q[L["nox"]], q[L["ho2no2"]] = nox, hno4
+    q[L["hno3_gas"]] = hno3
+
+    # Ox = O3 + O(3P) (F:1400-1408); passive Ox starts equal to Ox.
+    o3 = _logu(rng, *O3_RANGE, shape)
+    for name in SHORT:
+        h[S[name]] = _logu(rng, *RADICAL_RANGE, shape)
+    h[S["o3"]] = o3
+    h[S["no2"]] = nox * rng.uniform(0.3, 0.9, shape)
+    h[S["no"]] = np.maximum(nox - h[S["no2"]], 0.0) * rng.uniform(0.5, 1.0, shape)
+    h[S["clo"]] = clox * rng.uniform(0.5, 1.0, shape)
+    h[S["bro"]] = brox * rng.uniform(0.3, 1.0, shape)
+    q[L["ox"]] = o3 + h[S["o3p"]]
+    q[L["ox_passive"]] = q[L["ox"]]
+    return q, h
+
+
+# ---- SAD -> H2SO4 (binary liquid aerosol relation of the reference, see module doc) ----
+_KS = (-21.661, 2724.2, 51.81, -15732.0, 47.004, -6969.0, -4.6183)   # F:2725, Carslaw 1995
+_DENS_X = (2.393284E-02, -4.359335E-05, 7.961181E-08, 0.0, -0.198716351,
+           1.39564574E-03, -2.020633E-06, 0.51684706, -3.0539E-03, 4.505475E-06)
+
+
+def _binary_density(ws, T):
+    x = _DENS_X
+    v1 = x[0] + x[1] * T + x[2] * T ** 2 + x[3] * T ** 3
+    vs = x[4] + x[5] * T + x[6] * T ** 2 + (x[7] + x[8] * T + x[9] * T ** 2) * ws
+    # the third density term multiplies w*w with (X11..X13); fold it in:
+    vs = vs + (-0.30119511 + 1.840408E-03 * T - 2.7221253742E-06 * T ** 2) * ws * ws
+    vmcal = (1.0 - ws) / 18.0160 * v1 + vs * ws / 98.080
+    return 1.0 / vmcal * 0.001
+
+
+def h2so4_for_sad(T, p, qh2o, sad):
+    """H2SO4 mixing ratio giving binary liquid-aerosol area ``sad`` (cm2/cm3)."""
+    pw0 = np.minimum(qh2o * p / 1013.0, 2e-3 / 1013.0)
+    tice = 2668.70 / (10.4310 - (np.log(pw0) + np.log(760.0)) / np.log(10.0))
+    tt = np.minimum(T, 273.0)
+    ph2o = 10.0 ** (-2668.7 / tt + 10.4310) / 760.0
+    parth2o = np.where(tt <= tice, np.clip(1.0 - np.maximum(pw0 - ph2o, 0.0) / pw0, 0, 1), 1.0)
+    t = np.maximum(T, np.maximum(tice - 3.0, 185.0))
+    pw = np.clip(qh2o * parth2o * p / 1013.0, 2e-5 / 1013.0, 2e-3 / 1013.0)
+    ks = _KS
+    a = ks[2] + ks[3] / t
+    b = ks[0] + ks[1] / t
+    c = ks[4] + ks[5] / t + ks[6] * np.log(t) - np.log(pw)
+    det = np.maximum(b * b - 4 * a * c, 1e-30)
+    xsb = (-b - np.sqrt(det)) / (2 * a)
+    msb = 55.51 * xsb / (1.0 - xsb)
+    ws = np.maximum(msb * 0.098076 / (1.0 + msb * 0.098076), 0.005)
+    dens = _binary_density(ws, T)
+    ns_per_q = p * 100.0 / 8.314 / t
+    vliq_per_q = ns_per_q * 98.076e-6 / ws / dens
+    ls2 = np.log(1.8) ** 2
+    ndrop = 10.0
+    # aliq = 4 pi ndrop e^{2 ls2} * (3 vliq / (4 pi ndrop) e^{-4.5 ls2})^{2/3}
+    k = 4 * np.pi * ndrop * np.exp(2 * ls2) * (3 * vliq_per_q / (4 * np.pi * ndrop) * np.exp(-4.5 * ls2)) ** (2.0 / 3.0)
+    return (sad / k) ** 1.5
+
+
+def make_c0(seed: int = 0) -> StepInputs:
+    """Grid c0: 64 columns x 60 levels, one 900 s step."""
+    rng = np.random.default_rng(seed)
+    g = Grid(nlon=64, nlat=1, niv=NLEV, nivbas=NLEV)
+    shape = g.shape3
+    p = np.broadcast_to(level_pressures()[:, None, None], shape).copy()
+    T = rng.uniform(185.0, 270.0, shape)
+    cossza = rng.uniform(-0.3, 1.0, shape)
+    sza = np.degrees(np.arccos(cossza))
+    q, h = draw_species(rng, shape)
+    sad = _logu(rng, 0.1, 10.0, shape) * 1e-8
+    q[L["h2so4"]] = h2so4_for_sad(T, p, q[L["h2o"]], sad)
+    return StepInputs(g, 900.0, T, p, sza, altitude_km(p), q, h, sad,
+                      {"name": "c0", "seed": seed, "cossza": cossza})
+
+
+def _latlon(nlon, nlat):
+    lat = np.linspace(-90.0, 90.0, nlat)
+    lon = np.arange(nlon) * (360.0 / nlon)
+    return lat, lon
+
+
+def solar_zenith(lat_deg, lon_deg, decl_deg=20.0, ut_hours=12.0):
+    """SZA (deg) from latitude, longitude and a fixed declination (noon at lon 0 at 12 UT)."""
+    phi = np.radians(lat_deg)
+    dec = np.radians(decl_deg)
+    h = np.radians((ut_hours - 12.0) * 15.0 + lon_deg)
+    c = np.sin(phi) * np.sin(dec) + np.cos(phi) * np.cos(dec) * np.cos(h)
+    return np.degrees(np.arccos(np.clip(c, -1.0, 1.0)))
+
+
+def temperature_profile(lat_deg, p_hpa):
+    """200 K at the winter (south) pole / ~215 K elsewhere at 100 hPa, 270 K at 1 hPa."""
+    s = np.log(P_BOT / p_hpa) / np.log(P_BOT / P_TOP)           # 0 at 100 hPa, 1 at 1 hPa
+    t_low = 215.0 - 15.0 * np.maximum(-np.sin(np.radians(lat_deg)), 0.0) ** 2 + 5.0 * np.maximum(np.sin(np.radians(lat_deg)), 0.0)
+    return t_low + (270.0 - t_low) * s
+
+
+def make_c1(seed: int = 0, nlon: int = 144, nlat: int = 91, decl_deg: float = 20.0,
+            ut_hours: float = 12.0) -> StepInputs:
+    """Grid c1: 144 x 91 x 60 boxes, one step (smaller nlon/nlat allowed for tests)."""
+    rng = np.random.default_rng(seed)
+    g = Grid(nlon=nlon, nlat=nlat, niv=NLEV, nivbas=NLEV)
+    shape = g.shape3
+    lat, lon = _latlon(nlon, nlat)
+    p1 = level_pressures()
+    p = np.broadcast_to(p1[:, None, None], shape).copy()
+    LAT = np.broadcast_to(lat[None, :, None], shape)
+    LON = np.broadcast_to(lon[None, None, :], shape)
+    T = temperature_profile(LAT, p) + rng.normal(0.0, 5.0, shape)
+    sza = np.broadcast_to(solar_zenith(LAT[0], LON[0], decl_deg, ut_hours)[None], shape).copy()
+    q, h = draw_species(rng, shape)
+    sad = _logu(rng, 0.1, 10.0, shape) * 1e-8
+    q[L["h2so4"]] = h2so4_for_sad(T, p, q[L["h2o"]], sad)
+    return StepInputs(g, 900.0, T, p, sza, altitude_km(p), q, h, sad,
+                      {"name": "c1", "seed": seed, "decl_deg": decl_deg, "lat": lat, "lon": lon})
