@@ -1,0 +1,104 @@
+/* rbchem -- MI355X (gfx950) chemistry step for REPROBUS, C ABI.
+ *
+ * Replaces the model's chemistry sub-step (src/reprobus_l137_20230723.f:398-440):
+ * the row loop `do ilat; call chem3d(...)` (F:398-415, signature F:601-616),
+ * `call conserv(qj1, hc)` (F:427, body F:2526-2616) and the 1e-30 floor
+ * (F:431-440).  Plain pointers and sizes only; every entry returns 0 on success
+ * or a negative RB_E* code, with a message from rb_last_error().  Nothing here
+ * calls exit(); nothing retains caller pointers after return.
+ *
+ * Array layout is the model's Fortran layout: a field is (nlon, nlat, niv)
+ * with lon fastest; species fields are stacked last (qj1: 43 fields, hc: 15).
+ * T and p are the `trajtem` / `trajpre` slice of the current sub-step
+ * (F:802-806), i.e. one (nlon, nlat, niv) field each.  SZA is given per box in
+ * degrees (the model computes it in `zenith`, F:845; pass `sza3d`).
+ *
+ * One context owns one device, one HIP stream and the device copies of the
+ * photolysis table and work buffers.  Calls on one context are serialised on its
+ * stream; a context is not re-entrant.
+ */
+#ifndef RBCHEM_H
+#define RBCHEM_H
+#include <stddef.h>
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RB_OK 0
+#define RB_EINVAL (-1)     /* bad argument (null pointer, bad grid) */
+#define RB_EHIP (-2)       /* HIP runtime error */
+#define RB_ENOTABLE (-3)   /* rb_load_jtable not called */
+#define RB_ENONFINITE (-4) /* non-finite values in the output (count in rb_stats) */
+#define RB_ENOMEM (-5)     /* device allocation failed */
+
+typedef struct rb_ctx rb_ctx;
+
+/* Grid extents (Fortran order) and the level ranges the reference hard-codes:
+ * chemistry on levels 1..nivbas (F:17, nivbas = 94), heterogeneous chemistry on
+ * levels nivhau..nivbas (F:2425, nivhau = 14).  Level 1 is the model top. */
+typedef struct {
+    int32_t nlon, nlat, niv;
+    int32_t nivbas, nivhau;
+} rb_grid;
+
+typedef struct {
+    int32_t niter;          /* fixed sweeps per step; the reference uses 5 (F:630) */
+    int32_t check_finite;   /* 1: count non-finite outputs and return RB_ENONFINITE */
+    int32_t block;          /* threads per block of the per-box kernel (0: default) */
+    int32_t profile;        /* 1: time every launch of the per-box kernel with HIP events */
+} rb_config;
+
+typedef struct {
+    int64_t boxes;          /* boxes processed by the last step */
+    int64_t nonfinite;      /* non-finite outputs of the last checked step */
+    double last_kernel_ms;  /* device time of the last step's kernels */
+    double h2d_ms, d2h_ms;  /* copy times of the last host-array call */
+    double chem_ms_total;   /* profile mode: summed device time of the per-box kernel ... */
+    int64_t chem_launches;  /* ... over this many launches since the previous rb_get_stats */
+} rb_stats_t;
+
+/* Create a context on HIP device `device`; cfg may be NULL (defaults).
+ * Replaces the Troe set-up of `const` (F:5927-5948). */
+int rb_init(rb_ctx **ctx, int device, const rb_config *cfg);
+
+/* Load the photolysis table: ajl(40,101,27,7) and o3up(0:79), Fortran order.
+ * Replaces the jstrato.txt read of `const` (F:5953-5973, COMMON /photodis/ F:714). */
+int rb_load_jtable(rb_ctx *ctx, const double *ajl, const double *o3up);
+
+/* Use `stream` (a hipStream_t) for subsequent calls; NULL restores the context's own. */
+int rb_set_stream(rb_ctx *ctx, void *stream);
+
+/* One chemistry step on host (caller-owned, Fortran-order) arrays: copies in,
+ * runs, copies back.  qj1 and hc are updated in place; surfarea (niv levels)
+ * is written on levels nivhau..nivbas only, like `analytic` (F:3240-3246);
+ * vsed3d, parth2osed, parthno3sed are (nlon, nlat, nivbas) fields (F:906-912).
+ * Replaces the chem3d row loop F:398-415. */
+int rb_chem_step(rb_ctx *ctx, const rb_grid *g, double dt, const double *T, const double *p,
+                 const double *sza, const double *alt, double *qj1, double *hc,
+                 double *surfarea, double *vsed3d, double *parth2osed, double *parthno3sed);
+
+/* conserv (F:2526-2616, levels 1..nivbas) then the floor max(q, 1e-30) of
+ * species 1..42 on all niv levels (F:431-440), on host arrays. */
+int rb_conserv_floor(rb_ctx *ctx, const rb_grid *g, double *qj1, double *hc);
+
+/* Device-resident variants: the same contracts on device pointers, enqueued on
+ * the context's stream (asynchronous; use rb_sync).  `fuse_conserv` = 1 also
+ * applies conserv + floor to the chemistry levels in the same pass (exact when
+ * nothing -- e.g. sedimentation, F:419-423 -- runs in between). */
+int rb_chem_step_dev(rb_ctx *ctx, const rb_grid *g, double dt, const double *T, const double *p,
+                     const double *sza, const double *alt, double *qj1, double *hc,
+                     double *surfarea, double *vsed3d, double *parth2osed, double *parthno3sed,
+                     int fuse_conserv);
+int rb_conserv_floor_dev(rb_ctx *ctx, const rb_grid *g, double *qj1, double *hc);
+
+int rb_sync(rb_ctx *ctx);
+/* Waits for the timed launches, fills *out and resets the profile accumulators. */
+int rb_get_stats(rb_ctx *ctx, rb_stats_t *out);
+const char *rb_last_error(rb_ctx *ctx);
+int rb_finalize(rb_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

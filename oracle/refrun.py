@@ -102,3 +102,55 @@ def run_reference(inp, nrep: int = 1, day: float = DAYNUM, workdir: str | None =
     finally:
         if not keep:
             shutil.rmtree(run, ignore_errors=True)
+
+
+def time_reference(inp, nrep: int, nproc: int, day: float = DAYNUM) -> dict:
+    """CPU baseline: `nproc` concurrent single-threaded reference processes, each
+    running the chem3d row sweep of ``inp`` ``nrep`` times (successive steps).
+
+    Throughput sums the per-process rates measured inside the harness around the
+    chem3d calls (process start-up and file I/O excluded), counting every box
+    chem3d computes (180 longitudes x 94 levels per row call).
+    """
+    import time
+
+    from reprobus_amd import jtable
+
+    g = inp.grid
+    ncol, nlev = g.nlat * g.nlon, g.nivbas
+    lon, lat = trajectories_for_sza(inp.sza[:nlev])
+    run = tempfile.mkdtemp(prefix="bench_", dir=os.path.join(HERE, "_ref"))
+    try:
+        ajl, o3up = jtable.default_table()
+        jtable.write_jstrato(os.path.join(run, "jstrato.txt"), ajl, o3up)
+        with open(os.path.join(run, "relief.txt"), "w") as f:
+            f.write(f"{180 * 91}*0.0\n")
+        fin = os.path.join(run, "in.bin")
+        with open(fin, "wb") as f:
+            np.array([ncol, nlev, nrep], dtype=np.int32).tofile(f)
+            np.array([day, inp.dt], dtype=np.float64).tofile(f)
+            for a in (lon, lat, inp.p[:nlev], inp.T[:nlev], inp.alt[:nlev]):
+                np.ascontiguousarray(a, dtype=np.float64).tofile(f)
+            np.ascontiguousarray(inp.qj1[:, :nlev], dtype=np.float64).tofile(f)
+            np.ascontiguousarray(inp.hc[:, :nlev], dtype=np.float64).tofile(f)
+        t0 = time.perf_counter()
+        procs = [subprocess.Popen([REF_BIN, fin, os.path.join(run, f"out{i}.bin")], cwd=run,
+                                  stdout=subprocess.DEVNULL, stderr=subprocess.PIPE)
+                 for i in range(nproc)]
+        errs = [p.wait() for p in procs]
+        wall = time.perf_counter() - t0
+        if any(errs):
+            raise RuntimeError(f"reference baseline run failed: {errs}")
+        ncalls, secs, rate = 0, 0.0, 0.0
+        for i in range(nproc):
+            with open(os.path.join(run, f"out{i}.bin"), "rb") as f:
+                hdr = np.fromfile(f, dtype=np.int32, count=3)
+                s_i = float(np.fromfile(f, dtype=np.float64, count=1)[0])
+            ncalls += int(hdr[2])
+            secs += s_i
+            rate += int(hdr[2]) * NLON_REF * 94 / s_i
+        boxes = ncalls * NLON_REF * 94
+        return {"wall_s": wall, "chem3d_s": secs, "calls": ncalls, "boxes": boxes,
+                "box_steps_per_s": rate, "per_core_box_steps_per_s": boxes / secs}
+    finally:
+        shutil.rmtree(run, ignore_errors=True)

@@ -1,0 +1,77 @@
+"""ctypes binding of librbchem.so (the C ABI declared in include/rbchem.h).
+
+The library is built in-tree by ``__graft_entry__.build()`` (or
+``make -C reprobus_amd/csrc``).  There is no fallback: if the library is missing
+or fails to load, every entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "librbchem.so")
+
+RB_OK, RB_EINVAL, RB_EHIP, RB_ENOTABLE, RB_ENONFINITE, RB_ENOMEM = 0, -1, -2, -3, -4, -5
+ERROR_NAMES = {RB_EINVAL: "RB_EINVAL", RB_EHIP: "RB_EHIP", RB_ENOTABLE: "RB_ENOTABLE",
+               RB_ENONFINITE: "RB_ENONFINITE", RB_ENOMEM: "RB_ENOMEM"}
+
+# every symbol include/rbchem.h declares
+EXPORTS = ["rb_init", "rb_load_jtable", "rb_set_stream", "rb_chem_step", "rb_conserv_floor",
+           "rb_chem_step_dev", "rb_conserv_floor_dev", "rb_sync", "rb_get_stats",
+           "rb_last_error", "rb_finalize"]
+
+
+class Grid(C.Structure):
+    _fields_ = [("nlon", C.c_int32), ("nlat", C.c_int32), ("niv", C.c_int32),
+                ("nivbas", C.c_int32), ("nivhau", C.c_int32)]
+
+
+class Config(C.Structure):
+    _fields_ = [("niter", C.c_int32), ("check_finite", C.c_int32), ("block", C.c_int32),
+                ("profile", C.c_int32)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("boxes", C.c_int64), ("nonfinite", C.c_int64), ("last_kernel_ms", C.c_double),
+                ("h2d_ms", C.c_double), ("d2h_ms", C.c_double), ("chem_ms_total", C.c_double),
+                ("chem_launches", C.c_int64)]
+
+
+class RbError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{ERROR_NAMES.get(code, code)}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    """Load librbchem.so and declare its signatures (raises if it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is not built; run __graft_entry__.build() "
+                           "or `make -C reprobus_amd/csrc`")
+    L = C.CDLL(LIB_PATH)
+    P = C.c_void_p
+    d = C.c_double
+    L.rb_init.argtypes = [C.POINTER(P), C.c_int, C.POINTER(Config)]
+    L.rb_load_jtable.argtypes = [P, P, P]
+    L.rb_set_stream.argtypes = [P, P]
+    L.rb_chem_step.argtypes = [P, C.POINTER(Grid), d] + [P] * 10
+    L.rb_conserv_floor.argtypes = [P, C.POINTER(Grid), P, P]
+    L.rb_chem_step_dev.argtypes = [P, C.POINTER(Grid), d] + [P] * 10 + [C.c_int]
+    L.rb_conserv_floor_dev.argtypes = [P, C.POINTER(Grid), P, P]
+    L.rb_sync.argtypes = [P]
+    L.rb_get_stats.argtypes = [P, C.POINTER(Stats)]
+    L.rb_last_error.argtypes = [P]
+    L.rb_last_error.restype = C.c_char_p
+    L.rb_finalize.argtypes = [P]
+    for name in EXPORTS:
+        if name != "rb_last_error":
+            getattr(L, name).restype = C.c_int
+    _lib = L
+    return L

@@ -1,0 +1,127 @@
+"""Host-side interface of the GPU chemistry step.
+
+Mirrors the model's chemistry sub-step (src/reprobus_l137_20230723.f:398-440):
+``chem3d`` over every latitude row (F:398-415), then ``conserv`` (F:427) and the
+1e-30 floor (F:431-440).  Arrays are in the model's Fortran layout, held as
+C-ordered numpy arrays / torch tensors of shape (nspecies, niv, nlat, nlon).
+
+Two ways in:
+  * host arrays (numpy float64): ``chem3d`` / ``conserv`` copy to the device, run
+    and copy back, updating the caller's arrays in place like the Fortran routines;
+  * device-resident torch tensors: ``step_device`` enqueues the kernels on the
+    current torch stream, the path the model's time loop and the benchmark use.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import Config, Grid, RbError, Stats
+
+
+def _ptr(a) -> int:
+    if isinstance(a, np.ndarray):
+        if a.dtype != np.float64 or not a.flags.c_contiguous:
+            raise TypeError("arrays must be C-contiguous float64")
+        return a.ctypes.data
+    # torch tensor
+    if a.dtype.__repr__() != "torch.float64" or not a.is_contiguous() or a.device.type != "cuda":
+        raise TypeError("tensors must be contiguous float64 on a GPU")
+    return a.data_ptr()
+
+
+def make_grid(nlon: int, nlat: int, niv: int, nivbas: int, nivhau: int) -> Grid:
+    return Grid(nlon, nlat, niv, nivbas, nivhau)
+
+
+class ChemistryStep:
+    """One library context: device, stream, photolysis table, work buffers."""
+
+    def __init__(self, ajl: np.ndarray, o3up: np.ndarray, device: int = 0, niter: int = 5,
+                 block: int = 256, check_finite: bool = False, profile: bool = False):
+        self._L = _lib.load()
+        self._ctx = C.c_void_p()
+        cfg = Config(niter, 1 if check_finite else 0, block, 1 if profile else 0)
+        rc = self._L.rb_init(C.byref(self._ctx), device, C.byref(cfg))
+        if rc != 0:
+            raise RbError(rc, f"rb_init(device={device}) failed")
+        ajl_f = np.ascontiguousarray(np.asarray(ajl, dtype=np.float64).transpose(3, 2, 1, 0))
+        o3 = np.ascontiguousarray(o3up, dtype=np.float64)
+        if ajl_f.size != 40 * 101 * 27 * 7 or o3.size != 80:
+            raise ValueError("ajl must be (40,101,27,7) and o3up (80,)")
+        self._check(self._L.rb_load_jtable(self._ctx, ajl_f.ctypes.data, o3.ctypes.data))
+
+    # -- plumbing -------------------------------------------------------------------
+    def _check(self, rc: int) -> None:
+        if rc != 0:
+            msg = self._L.rb_last_error(self._ctx)
+            raise RbError(rc, msg.decode() if msg else "")
+
+    def close(self) -> None:
+        if self._ctx:
+            self._L.rb_finalize(self._ctx)
+            self._ctx = C.c_void_p()
+
+    def __del__(self):  # pragma: no cover
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def stats(self) -> dict:
+        s = Stats()
+        self._check(self._L.rb_get_stats(self._ctx, C.byref(s)))
+        return {"boxes": s.boxes, "nonfinite": s.nonfinite, "kernel_ms": s.last_kernel_ms,
+                "h2d_ms": s.h2d_ms, "d2h_ms": s.d2h_ms, "chem_ms_total": s.chem_ms_total,
+                "chem_launches": s.chem_launches}
+
+    def set_stream(self, stream_handle: int | None) -> None:
+        self._check(self._L.rb_set_stream(self._ctx, stream_handle))
+
+    def sync(self) -> None:
+        self._check(self._L.rb_sync(self._ctx))
+
+    # -- host arrays (drop-in for the Fortran calls) ------------------------------
+    def chem3d(self, grid: Grid, dt: float, T, p, sza, alt, qj1, hc, surfarea, vsed3d,
+               parth2osed, parthno3sed) -> None:
+        """All rows of chem3d (F:398-415) on host arrays, in place."""
+        self._check(self._L.rb_chem_step(
+            self._ctx, C.byref(grid), float(dt), *[_ptr(a) for a in (
+                T, p, sza, alt, qj1, hc, surfarea, vsed3d, parth2osed, parthno3sed)]))
+
+    def conserv(self, grid: Grid, qj1, hc) -> None:
+        """conserv (F:2526-2616) + floor (F:431-440) on host arrays, in place."""
+        self._check(self._L.rb_conserv_floor(self._ctx, C.byref(grid), _ptr(qj1), _ptr(hc)))
+
+    # -- device tensors -----------------------------------------------------------
+    def step_device(self, grid: Grid, dt: float, T, p, sza, alt, qj1, hc, surfarea, vsed3d,
+                    parth2osed, parthno3sed, fuse_conserv: bool = True) -> None:
+        """Enqueue one step on device tensors (conserv + floor fused when asked)."""
+        self._check(self._L.rb_chem_step_dev(
+            self._ctx, C.byref(grid), float(dt), *[_ptr(a) for a in (
+                T, p, sza, alt, qj1, hc, surfarea, vsed3d, parth2osed, parthno3sed)],
+            1 if fuse_conserv else 0))
+
+    def conserv_device(self, grid: Grid, qj1, hc) -> None:
+        self._check(self._L.rb_conserv_floor_dev(self._ctx, C.byref(grid), _ptr(qj1), _ptr(hc)))
+
+    # -- convenience on synth.StepInputs -------------------------------------------
+    def run_inputs(self, inp, conserv: bool = True) -> dict:
+        """Run one step on a synth.StepInputs via the host-array path; returns outputs."""
+        g = inp.grid
+        grid = make_grid(g.nlon, g.nlat, g.niv, g.nivbas, g.nivhau)
+        c = lambda a: np.ascontiguousarray(a, dtype=np.float64).copy()
+        q, h = c(inp.qj1), c(inp.hc)
+        surf = np.zeros(g.shape3)
+        shb = (g.nivbas, g.nlat, g.nlon)
+        vsed, p2o, pno3 = np.zeros(shb), np.zeros(shb), np.zeros(shb)
+        self.chem3d(grid, inp.dt, c(inp.T), c(inp.p), c(inp.sza), c(inp.alt), q, h, surf, vsed,
+                    p2o, pno3)
+        out = {"qj1_chem": q.copy(), "hc_chem": h.copy(), "surfarea": surf, "vsed3d": vsed,
+               "parth2osed": p2o, "parthno3sed": pno3}
+        if conserv:
+            self.conserv(grid, q, h)
+            out["qj1"], out["hc"] = q, h
+        return out

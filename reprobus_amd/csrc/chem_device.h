@@ -1,0 +1,55 @@
+// Device-side declarations shared by the chemistry kernels and the C ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace rb {
+
+// Long-lived (transported) slots, src/reprobus_l137_20230723.f:720-730.
+enum Long : int {
+    N2O, CH4, H2O, NOY, HNO3, N2O5, CLY, OX, CO, OCLO, OXP, H2SO4, HCL, CLONO2, HOCL, CL2,
+    H2O2, CLNO2, HBR, BRONO2, NOX, HO2NO2, CLOX, BROX, CL2O2, HOBR, BRCL, CH2O, CH3O2,
+    CH3O2H, CFC11, CFC12, CFC113, CCL4, CH3CCL3, CH3CL, HCFC22, CH3BR, H1211, H1301, BRY,
+    CH2BR2, HNO3G, NLONG
+};
+// Short-lived (photochemical-equilibrium) slots, F:743-746.
+enum Short : int { O1D, OH, CL, O3P, O3, HO2, NO2, NO, BR, N, CLO, BRO, NO3, H, CH3, NSHORT };
+// Photolysis slots, F:754-794 (0-based here).
+enum Jslot : int {
+    JO2, JNO2, JO3_O1D, JO3_O3P, JNO3_NO, JNO3_NO2, JN2O5, JHNO3, JHNO4, JH2O2, JH2O, JN2O,
+    JHOCL, JCLONO2_CL, JCLONO2_CLO, JCL2O2, JHCL, JCL2, JCLNO2, JBRO, JBRONO2, JHOBR, JOCLO,
+    JBRCL, JCH3O2H, JCH2O_HCO, JCH2O_CO, JCFC11, JCFC12, JCFC113, JCCL4, JCH3CCL3, JCH3CL,
+    JHCFC22, JCH3BR, JH1211, JH1301, JCO2, JCH2BR2, JHBR, JNO, NJ
+};
+
+constexpr int NZ = 101, NSZA = 27, NOZO = 7, NJTAB = 40;
+
+// Falloff prefactors k0*300^n and kinf*300^m (const, F:5927-5948).
+struct Troe {
+    double a0hox, a1hox, a0nox1, a1nox1, a0nox2, a1nox2, a0nox3, a1nox3, a0clx1, a1clx1,
+        a0clx2, a1clx2, a0clx3, a1clx3, a0brx, a1brx, a0c1, a1c1;
+};
+
+struct StepArgs {
+    const double *T, *p, *sza, *alt;   // (plane * niv) fields
+    const double *col;                 // 3 fields of plane*nivbas: o3 column, O2 and NO sums
+    double *qj1, *hc;                  // 43 / 15 fields of plane*niv
+    double *surfarea;                  // plane*niv
+    double *vsed, *p2o, *pno3;         // plane*nivbas
+    const double *ajl, *o3up;
+    int64_t plane, field, nbox;        // plane = nlon*nlat, field = plane*niv, nbox = plane*nivbas
+    int32_t nivbas, nivhau, niter, fuse_conserv;
+    double dt;
+    Troe K;
+};
+
+Troe troe_constants();
+hipError_t launch_columns(const StepArgs &a, double *col, hipStream_t s);
+hipError_t launch_chem(const StepArgs &a, int block, hipStream_t s);
+// conserv on boxes below plane*nivbas and the floor on every box, for boxes [start, field)
+hipError_t launch_conserv_floor(double *qj1, double *hc, int64_t plane, int64_t field,
+                                int32_t nivbas, int64_t start, hipStream_t s);
+hipError_t launch_count_nonfinite(const double *x, int64_t n, unsigned long long *count,
+                                  hipStream_t s);
+
+}  // namespace rb

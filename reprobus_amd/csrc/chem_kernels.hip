@@ -1,0 +1,1107 @@
+// REPROBUS chemistry step on gfx950: one box per lane, species-major SoA.
+//
+// Reference: src/reprobus_l137_20230723.f ("F") and src/jno.f90 ("J").
+//   k_columns       : O3 column (total, F:5153-5186) and the O2 / NO column sums of
+//                     jno (J:103-126) -- the only vertical coupling of the step; one
+//                     lane per column walks its levels top-down.
+//   k_chem          : everything else per box: J-values (phot F:4996-5149, jno
+//                     J:128-246), PSC/aerosol and heterogeneous rates (analytic
+//                     F:2620-3361, hetero F:3365-4029, density F:4033-4073), thermal
+//                     rates (F:920-1306), rain-out (F:1315-1338), the fixed-sweep
+//                     QSSA + linearised backward-Euler integration (F:1348-2378) and
+//                     the write-back (F:2380-2410); optionally conserv + floor.
+//   k_conserv_floor : conserv (F:2526-2616) and the 1e-30 floor (F:431-440).
+// Expressions keep the reference's evaluation order (the library is built with
+// -ffp-contract=off) so the step reproduces the reference to a few ulps per box.
+#include "chem_device.h"
+
+#include <math.h>
+
+namespace rb {
+
+__device__ __forceinline__ double sq(double x) { return x * x; }
+__device__ __forceinline__ double cube(double x) { return x * x * x; }
+
+// ------------------------------------------------------------------ rates
+struct Rates {
+    double a1, a1et, a2, a5, a6, a6b, a7, a17, a19, a26, a27, a36;
+    double b3, b4, b6, b7, b9, b12, b22, b23, b24, b27, b28, b32, b38, b39;
+    double c2, c4, c5, c7, c8, c9, c10, c14, c15, c17a, c17b, c18, c19;
+    double d2, d3, d4, d5, d6, d7, d8, d9, d10, d11, d31, d32, d33, d34, d36, d37, d60, d61,
+        d62, d94, d95, d97;
+    double e2, e3, e4, e5a, e5b, e5c, e6, e7, e8, e10, e11, e13, e15, e16, e51, e54;
+    double hk1, hk2, hk3, hk4, hk5;
+};
+// temperature-independent rates (F:964-972, 982-985, 1061-1068, 1082, 1163, 1196-1208,
+// 1245, 1276-1281) and the chem3d scalars a3et, a23a/b/c
+constexpr double A3ET = 1.2e-10, A23A = 7.2e-11, A23B = 6.9e-12, A23C = 1.6e-12;
+constexpr double A30 = 1.8e-12, A31 = 1.8e-12, C1A = 1.31e-10, C1B = 0.35e-10, C1C = 0.09e-10,
+                 C3 = 1.1e-10, C12 = 5.2e-12, D35 = 1.7e-13, D90 = 2.3e-10, D91 = 1.4e-10,
+                 D92 = 2.0e-10, D93 = 3.3e-10, D96 = 1.0e-10, E9 = 1.5e-10, E50 = 1.8e-10,
+                 E52 = 1.5e-10, E53 = 1.0e-10;
+
+// JPL falloff, Fc = 0.6 (F:930-935 and its 8 siblings).
+__device__ __forceinline__ double falloff(double a0, double a1, double n0, double n1,
+                                          double lnt, double hnm, double xln6)
+{
+    const double k0 = a0 * exp(lnt * n0);
+    const double ki = a1 * exp(lnt * n1);
+    const double rate = (k0 * hnm) / (1. + k0 * hnm / ki);
+    const double x = 1. / (1. + sq(log10((k0 * hnm) / ki)));
+    return rate * exp(xln6 * x);
+}
+
+__device__ void thermal_rates(const Troe &K, double t, double hnm, double pm, Rates &r)
+{
+    const double xln6 = log(0.6);
+    const double lnt = log(t);
+    r.a1 = falloff(K.a0hox, K.a1hox, -1.3, 0.2, lnt, hnm, xln6);
+    r.a1et = 1.63e-10 * exp(60. / t);
+    r.a2 = 1.4e-10 * exp(-470. / t);
+    r.a5 = 1.8e-11 * exp(180. / t);
+    r.a6 = 1.7e-12 * exp(-940. / t);
+    r.a6b = 1.0e-14 * exp(-490. / t);
+    r.a19 = 2.8e-12 * exp(-1800. / t);
+    r.a27 = 1.5e-12 * exp(19. / t) + 2.1e-33 * exp(920. / t) * hnm;
+    r.a36 = 1.5e-13 * (1. + .6 * pm / 1013.);
+
+    r.b3 = 5.1e-12 * exp(210. / t);
+    r.b4 = 3.0e-12 * exp(-1500. / t);
+    r.b6 = 2.1e-11 * exp(100. / t);
+    r.b7 = 1.5e-11 * exp(-3600. / t);
+    r.b9 = 1.2e-13 * exp(-2450. / t);
+    r.b12 = falloff(K.a0nox1, K.a1nox1, -4.4, -0.7, lnt, hnm, xln6);
+    r.b22 = falloff(K.a0nox2, K.a1nox2, -3.0, 0.0, lnt, hnm, xln6);
+    r.b23 = falloff(K.a0nox3, K.a1nox3, -3.4, -1.1, lnt, hnm, xln6);
+    r.b24 = r.b23 / (2.1e-27 * exp(10900. / t) * hnm);          // via equilibrium constant
+    {
+        const double aux1 = 2.7e-17 * exp(2199. / t);
+        const double aux2 = 1. + aux1 / (6.5e-34 * exp(1335. / t) * hnm);
+        r.b27 = 2.4e-14 * exp(460. / t) + aux1 / aux2;
+    }
+    r.b28 = 1.3e-12 * exp(380. / t);
+    r.b32 = r.b12 / (2.7e-27 * exp(11000. / t) * hnm);
+    r.b38 = 4.63e-11 * exp(20. / t);
+    r.b39 = 7.25e-11 * exp(20. / t);
+
+    r.c2 = 2.45e-12 * exp(-1775. / t);
+    r.c4 = falloff(K.a0c1, K.a1c1, -3.6, 1.1, lnt, hnm, xln6);
+    r.c5 = 2.8e-12 * exp(300. / t);
+    r.c7 = 3.8e-13 * exp(800. / t);
+    r.c8 = 5.5e-12 * exp(125. / t);
+    r.c10 = 8.1e-11 * exp(-30. / t);
+    r.c14 = 9.5e-14 * exp(390. / t);
+    r.c15 = 3.9e-14 * exp(-900. / t);
+    {
+        const double e200 = exp(200. / t);
+        r.c17a = 3.8e-12 * e200 * 0.7;
+        r.c17b = 3.8e-12 * e200 * 0.3;
+        r.e11 = 5.5e-12 * e200;
+        r.a7 = 3.0e-11 * e200;
+    }
+    r.c18 = 3.3e-12 * exp(-115. / t);
+    r.c19 = 1.7e-11 * exp(-800. / t);
+
+    r.d2 = 2.3e-11 * exp(-200. / t);
+    r.d3 = 2.8e-11 * exp(85. / t);
+    {
+        const double e290 = exp(290. / t);
+        r.d4 = 6.4e-12 * e290;
+        r.d33 = 2.6e-12 * e290;
+        r.e5c = 4.1e-13 * e290;
+    }
+    r.d5 = 7.3e-12 * exp(-1280. / t);
+    r.d6 = 3.05e-11 * exp(-2270. / t);
+    r.d7 = 1.4e-11 * exp(269. / t);
+    {
+        const double e270 = exp(270. / t);
+        r.d8 = 7.4e-12 * e270;
+        r.a26 = 3.3e-12 * e270;
+    }
+    {
+        const double e230 = exp(230. / t);
+        r.d9 = 6.0e-13 * e230;
+        r.e3 = 1.9e-11 * e230;
+        r.e6 = 1.5e-12 * e230;
+    }
+    r.d10 = 3.6e-11 * exp(-375. / t);
+    r.d11 = 1.8e-12 * exp(-250. / t);
+    r.d31 = falloff(K.a0clx1, K.a1clx1, -3.4, -1.9, lnt, hnm, xln6);
+    {
+        const double em840 = exp(-840. / t);
+        r.d32 = 3.6e-12 * em840;
+        r.e54 = 2.0e-12 * em840;
+    }
+    r.d34 = 3.0e-12 * exp(-500. / t);
+    r.d36 = falloff(K.a0clx2, K.a1clx2, -2., -1., lnt, hnm, xln6);
+    r.d37 = 3.4e-12 * exp(-130. / t);
+    r.d60 = falloff(K.a0clx3, K.a1clx3, -4.5, -2.0, lnt, hnm, xln6);
+    r.d61 = r.d60 / (1.72e-27 * exp(8649. / t) * hnm);
+    r.d62 = 1.0e-12 * exp(-1590. / t) + 3.0e-11 * exp(-2450. / t);
+    r.d94 = 1.64e-12 * exp(-1520. / t);
+    r.d95 = 2.4e-12 * exp(-1250. / t);
+    {
+        const double em1600 = exp(-1600. / t);
+        r.d97 = 1.05e-12 * em1600;
+        r.c9 = 3.4e-11 * em1600;
+    }
+
+    r.e2 = 1.6e-11 * exp(-780. / t);
+    {
+        const double e260 = exp(260. / t);
+        r.e4 = 8.8e-12 * e260;
+        r.e5b = 2.3e-12 * e260;
+    }
+    r.e5a = 9.5e-13 * exp(550. / t);
+    r.e7 = 4.8e-12 * exp(-310. / t);
+    r.e8 = 1.9e-11 * exp(215. / t);
+    {
+        const double e250 = exp(250. / t);
+        r.e10 = 1.7e-11 * e250;
+        r.a17 = 4.8e-11 * e250;
+    }
+    r.e13 = falloff(K.a0brx, K.a1brx, -3.2, -2.9, lnt, hnm, xln6);
+    {
+        const double e460 = exp(460. / t);
+        r.e15 = 4.5e-12 * e460;
+    }
+    r.e16 = 1.2e-10 * exp(-430. / t);
+    r.e51 = 2.35e-12 * exp(-1300. / t);
+
+    r.hk1 = 4.23e-28 * hnm / (t * t);
+    r.hk2 = 6.0e-34 * pow(300. / t, 2.4) * hnm;
+    r.hk3 = 8.e-12 * exp(-2060. / t);
+    r.hk4 = 2.15e-11 * exp(110. / t);
+    r.hk5 = 3.3e-11 * exp(55. / t);
+}
+
+// ------------------------------------------------------------------ photolysis
+__constant__ double c_dsza[NSZA] = {0.,  5.,  10., 15., 20., 25., 30., 35., 40.,
+                                     45., 50., 55., 60., 65., 70., 75., 80., 82.,
+                                     84., 86., 88., 90., 91., 92., 93., 94., 95.};
+__constant__ double c_xv3[NOZO] = {0.4, 0.6, 0.8, 1.0, 1.2, 1.4, 1.6};
+
+// Trilinear lookup in (altitude, SZA, O3-column ratio) (phot F:5043-5132); J = 0 above 95 deg.
+__device__ void table_j(const double *__restrict__ ajl, const double *__restrict__ o3up,
+                        double alt, double sza, double o3t, double *tj)
+{
+    if (!(sza <= 95.)) {
+#pragma unroll
+        for (int n = 0; n < NJTAB; ++n) tj[n] = 0.;
+        return;
+    }
+    // SZA bracket: first node >= sza, minus one (F:5060-5068)
+    int is = NSZA - 1;
+#pragma unroll
+    for (int i = NSZA; i >= 1; --i)
+        if (c_dsza[i - 1] >= sza) is = i - 1;
+    // alt < 0 or NaN would index o3up out of bounds (undefined in the reference too)
+    const double zin = fmin(fmax(alt, 0.0), 78.9);
+    const int iz = (int)zin + 1;
+    const int i0 = (int)zin;
+    const double v3std = o3up[i0] - (zin - (double)i0) * (o3up[i0] - o3up[i0 + 1]);
+    const double stepo3 = (c_xv3[NOZO - 1] - c_xv3[0]) / (double)(NOZO - 1);
+    double v3rat = o3t / v3std;
+    v3rat = fmin(fmax(v3rat, c_xv3[0]), c_xv3[NOZO - 1]);
+    int iv = (int)trunc((v3rat - c_xv3[0]) / stepo3) + 1;
+    iv = iv < NOZO - 1 ? iv : NOZO - 1;
+    const int j = iz > 1 ? iz : 1, k = is > 1 ? is : 1, l = iv > 1 ? iv : 1;
+    const double cz = (zin - (double)(j - 1) * 1.) / 1.;
+    const double cs = (sza - c_dsza[k - 1]) / (c_dsza[k] - c_dsza[k - 1]);
+    const double co = (v3rat - c_xv3[l - 1]) / (c_xv3[l] - c_xv3[l - 1]);
+    const double w0 = (1. - cz) * (1. - cs) * (1. - co), w1 = (1. - cz) * (1. - cs) * co;
+    const double w2 = (1. - cz) * cs * (1. - co), w3 = (1. - cz) * cs * co;
+    const double w4 = cz * (1. - cs) * (1. - co), w5 = cz * (1. - cs) * co;
+    const double w6 = cz * cs * (1. - co), w7 = cz * cs * co;
+    // 40 J-values of one node are contiguous (Fortran ajl(40,101,27,7))
+    auto node = [&](int jj, int kk, int ll) {
+        return ajl + NJTAB * ((jj - 1) + NZ * ((kk - 1) + NSZA * (ll - 1)));
+    };
+    const double *p0 = node(j, k, l), *p1 = node(j, k, l + 1), *p2 = node(j, k + 1, l),
+                 *p3 = node(j, k + 1, l + 1), *p4 = node(j + 1, k, l),
+                 *p5 = node(j + 1, k, l + 1), *p6 = node(j + 1, k + 1, l),
+                 *p7 = node(j + 1, k + 1, l + 1);
+#pragma unroll
+    for (int n = 0; n < NJTAB; ++n)
+        tj[n] = w0 * p0[n] + w1 * p1[n] + w2 * p2[n] + w3 * p3[n] + w4 * p4[n] + w5 * p5[n] +
+                w6 * p6[n] + w7 * p7[n];
+}
+
+// NO photolysis, Minschwaner & Siskind (jno J:128-246), from the box's column sums.
+__constant__ double c_sig_o2[3][6] = {
+    {1.12e-23, 2.45e-23, 7.19e-23, 3.04e-22, 1.75e-21, 1.11e-20},
+    {1.35e-22, 2.99e-22, 7.33e-22, 3.07e-21, 1.69e-20, 1.66e-19},
+    {2.97e-22, 5.83e-22, 2.05e-21, 8.19e-21, 4.80e-20, 2.66e-19}};
+__constant__ double c_sig_no1[3][6] = {
+    {0.00e+00, 1.32e-18, 6.35e-19, 7.09e-19, 2.18e-19, 4.67e-19},
+    {0.00e+00, 0.00e+00, 3.05e-21, 5.76e-19, 2.29e-18, 2.21e-18},
+    {1.80e-18, 1.50e-18, 5.01e-19, 7.20e-20, 6.72e-20, 1.49e-21}};
+__constant__ double c_sig_no2[3][6] = {
+    {0.00e+00, 4.41e-17, 4.45e-17, 4.50e-17, 2.94e-17, 4.35e-17},
+    {0.00e+00, 0.00e+00, 3.20e-21, 5.71e-17, 9.09e-17, 6.00e-17},
+    {1.40e-16, 1.52e-16, 7.00e-17, 2.83e-17, 2.73e-17, 6.57e-18}};
+__constant__ double c_w_no1[3][6] = {
+    {0.00e+00, 5.12e-02, 1.36e-01, 1.65e-01, 1.41e-01, 4.50e-02},
+    {0.00e+00, 0.00e+00, 1.93e-03, 9.73e-02, 9.75e-02, 3.48e-02},
+    {4.50e-02, 1.80e-01, 2.25e-01, 2.25e-01, 1.80e-01, 4.50e-02}};
+__constant__ double c_w_no2[3][6] = {
+    {0.00e+00, 5.68e-03, 1.52e-02, 1.83e-02, 1.57e-02, 5.00e-03},
+    {0.00e+00, 0.00e+00, 2.14e-04, 1.08e-02, 1.08e-02, 3.86e-03},
+    {5.00e-03, 2.00e-02, 2.50e-02, 2.50e-02, 2.00e-02, 5.00e-03}};
+
+__device__ double j_no(double colo2, double colno, double colo3, double hnm, double sza_top)
+{
+    if (!(sza_top <= 89.0)) return 0.;
+    constexpr double dl[3] = {2.3, 1.5, 1.5}, fl[3] = {3.98e+11, 2.21e+11, 2.30e+11},
+                     so3[3] = {4.6e-19, 6.7e-19, 7.1e-19};
+    const double sec = 1. / cos(sza_top * (3.141592 / 180.));
+    colo2 *= sec;
+    colno *= sec;
+    colo3 *= sec;
+    const double prob = 1.65e+09 / (5.1e+07 + 1.65e+09 + 1.5e-09 * (hnm * 0.78));
+    double band[3];
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+        double s = 0.;
+#pragma unroll
+        for (int i = 0; i < 6; ++i) {
+            const double term =
+                exp(-c_sig_o2[b][i] * colo2) *
+                (c_w_no1[b][i] * c_sig_no1[b][i] * exp(-c_sig_no1[b][i] * colno) +
+                 c_w_no2[b][i] * c_sig_no2[b][i] * exp(-c_sig_no2[b][i] * colno));
+            s = i == 0 ? term : s + term;
+        }
+        band[b] = dl[b] * fl[b] * exp(-so3[b] * colo3) * prob * s;
+    }
+    return fmax(band[0] + band[1] + band[2], 1.e-30);
+}
+
+// ------------------------------------------------------------------ aerosol
+struct Het {
+    double k[9];                    // het1..het9
+    double pno3, pno3sl, ph2o, phcl; // gas fractions
+    double surf, vsed;
+};
+
+__device__ __forceinline__ double solid_uptake(double g, double t, double mw, double r, double n,
+                                               double ptot, double z)
+{
+    // R = g*pi*r^2*cbar*n/(1+3g r/(4L)) in the Turco form of F:3815-3880
+    return g * 4.56E4 * sqrt(t / mw) * (r * r) * n / (1.0 + 3.3E4 * g * r * ptot / t) / z;
+}
+
+__device__ double ternary_density(double ws, double wn, double t)
+{
+    // density F:4059-4069
+    const double w = ws + wn, wh = 1.0 - w, t2 = t * t, t3 = t * t * t;
+    const double v1 = 2.393284E-02 + -4.359335E-05 * t + 7.961181E-08 * t2 + 0.0 * t3;
+    const double vs = -0.198716351 + 1.39564574E-03 * t + -2.020633E-06 * t2 +
+                      (0.51684706 + -3.0539E-03 * t + 4.505475E-06 * t2) * w +
+                      (-0.30119511 + 1.840408E-03 * t + -2.7221253742E-06 * t2) * w * w;
+    const double vn = -0.11331674116 + 8.47763E-04 * t + -1.22336185E-06 * t2 +
+                      (0.3455282 + -2.2111E-03 * t + 3.503768245E-06 * t2) * w +
+                      (-0.2315332 + 1.60074E-03 * t + -2.5827835E-06 * t2) * w * w;
+    const double vm = wh / 18.0160 * v1 + vs * ws / 98.080 + vn * wn / 63.0160;
+    return 1.0 / vm * 0.001;
+}
+
+// analytic (NAT F:2795-2857, ice F:2861-2913, liquid F:2921-3228, area F:3240, HNO3
+// partition F:3252, settling F:3264-3348) then hetero (F:3436-4023) for one box.
+__device__ void psc_and_het(double temp, double ptot, double hnm, const double *ck, Het &o)
+{
+    constexpr double R = 8.205e-5, NDROP = 10., NNAT_SMALL = 1., RNATS = 0.5e-4,
+                     RNATL = 6.5e-4, RICE = 10.e-4, AMU = 1.66e-24;
+    const double pi = 2. * asin(1.0);
+    double nnats = 0., nnatl = 0., anat = 0., nice = 0., aice = 0., vnat = 0., vnatl = 0.;
+    double pno3l = 1., pno3s = 1.;
+    int lnat = 0, lice = 0, condliq;
+    o.vsed = 0.;
+    o.pno3 = 1.;
+    o.pno3sl = 1.;
+    o.ph2o = 1.;
+    o.phcl = 1.;
+
+    double pw = fmin(ck[H2O] / hnm * ptot / 1013.0, 2.e-3 / 1013.);
+    double pn0 = ck[HNO3] / hnm * ptot / 1013.0;
+    double pn;
+    const double tm = fmin(temp, 273.);
+
+    // NAT (Hanson & Mauersberger)
+    {
+        const double zmt = -2.7836 - 0.00088 * tm;
+        const double zbt = 38.9855 - 11397.0 / tm + 0.009179 * tm;
+        const double eq = pow(10.0, zmt * log10(pw * 1013. * 0.75) + zbt);
+        if (pn0 * 1013. * 0.75 / eq >= 1.) {
+            lnat = 1;
+            pn = eq / (0.75 * 1013.);
+            const double hno3s = fmax((pn0 - pn) * 1013. * hnm / ptot, 0.);
+            vnat = hno3s * AMU * 117. / 1.35;
+            const double vsmall = NNAT_SMALL * (4. / 3. * pi * pow(RNATS, 3.));
+            if (vnat > vsmall) {
+                nnats = NNAT_SMALL;
+                nnatl = (vnat - vsmall) / (4. / 3. * pi * pow(RNATL, 3.));
+                vnatl = vnat - vsmall;
+            } else {
+                nnats = vnat / (4. / 3. * pi * pow(RNATS, 3.));
+                nnatl = 0.;
+                vnatl = 0.;
+            }
+            anat = nnats * 4 * pi * (RNATS * RNATS) + nnatl * 4 * pi * (RNATL * RNATL);
+            pno3s = fmin(fmax(1.0 - (pn0 - pn) / pn0, 0.), 1.);
+            o.pno3sl = fmin(fmax(1.0 - (pn0 - pn) * vnatl / vnat / pn0, 0.), 1.);
+        }
+    }
+    // ice
+    const double tice = 2668.70 / (10.4310 - (log(pw) + log(760.0)) / log(10.0));
+    if (tm <= tice) {
+        lice = 1;
+        lnat = 0;
+        anat = 0.;
+        const double ph2o = pow(10.0, -2668.7 / tm + 10.4310) / 760.0;
+        const double h2os = fmax(pw - ph2o, 0.) * 1013. * hnm / ptot;
+        const double vice = h2os * AMU * 18. / 0.928;
+        nice = vice / (4. / 3. * pi * pow(RICE, 3.));
+        aice = nice * 4 * pi * (RICE * RICE);
+        o.ph2o = fmin(fmax(1.0 - fmax(pw - ph2o, 0.) / pw, 0.), 1.);
+        const double zmt = -2.7836 - 0.00088 * tm;
+        const double zbt = 38.9855 - 11397.0 / tm + 0.009179 * tm;
+        const double eq = pow(10.0, zmt * log10(pw * o.ph2o * 1013. * 0.75) + zbt);
+        pn = eq / (0.75 * 1013.);
+        pno3s = fmin(fmax(1.0 - (pn0 - pn) / pn0, 0.), 1.);
+        o.pno3sl = pno3s;
+    }
+    // liquid binary H2SO4/H2O then ternary (Carslaw et al. 1995)
+    const double t = fmax(temp, fmax(tice - 3.0, 185.));
+    pw = fmax(fmin(ck[H2O] * o.ph2o / hnm * ptot / 1013.0, 2.e-3 / 1013.), 2.e-5 / 1013.);
+    const double qh2so4 = fmax(ck[H2SO4] / hnm, 1.e-18);
+    const double ns = ptot * 100.0 * qh2so4 / 8.314 / t;
+    pn0 = ptot / 1013.0 * (ck[HNO3] * pno3s / hnm);
+    double ms, ws, wn = 0.0, msb;
+    condliq = 1;
+    {
+        const double a = 51.81 + -15732.0 / t;
+        const double b = -21.661 + 2724.2 / t;
+        const double c = 47.004 + -6969.0 / t + -4.6183 * log(t) - log(pw);
+        const double det = (b * b) - 4. * a * c;
+        if (det > 0.) {
+            const double xsb = (-b - sqrt(det)) / (2. * a);
+            msb = 55.51 * xsb / (1.0 - xsb);
+        } else {
+            msb = 0.;
+            condliq = 0;
+        }
+    }
+    ms = msb;
+    ws = msb * 0.098076 / (1.0 + msb * 0.098076);
+    {
+        // H2SO4 saturation pressure (Ayers et al. 1980, Giauque et al. 1960)
+        const double wts = fmax(41., ws * 100.);
+        const double mu = 4.184 * (1.514e4 - 286. * (wts - 40.) + 1.08 * sq(wts - 40.) -
+                                   3941. / pow(wts - 40., 0.1));
+        const double ph2so4 = exp(-10156. / t + 16.2590 - mu / (8.314 * t));
+        if (qh2so4 * ptot / 1013. <= ph2so4) {
+            ms = 0.;
+            ws = 0.;
+            condliq = 0;
+        }
+    }
+    if (condliq == 1 && t < 215.) {
+        constexpr double qn[10] = {14.5734, 0.0615994, -1.14895, 0.691693, -0.098863,
+                                   0.0051579, 0.123472, -0.115574, 0.0110113, 0.0097914};
+        constexpr double qs[10] = {14.4700, 0.0638795, -3.29597, 1.778224, -0.223244,
+                                   0.0086486, 0.536695, -0.335164, 0.0265153, 0.0157550};
+        constexpr double kn[7] = {-39.136, 6358.4, 83.29, -17650.0, 198.53, -11948, -28.469};
+        const double tt = R * t * ns, tr = 1.0e4 / t - 43.4782608, pr = log(pw) + 18.4;
+        const double tr2 = tr * tr, tr3 = tr * tr * tr, pr2 = pr * pr, pr3 = pr * pr * pr;
+        const double hsb = exp(qs[0] + qs[1] * tr2 +
+                               (qs[2] + qs[3] * tr + qs[4] * tr2 + qs[5] * tr3) * pr +
+                               (qs[6] + qs[7] * tr + qs[8] * tr2) * pr2 + qs[9] * tr * pr3);
+        const double kk = kn[0] + kn[1] / t;
+        const double xnb = 1.0 / (2.0 * (kn[2] + kn[3] / t)) *
+                           (-kn[0] - kn[1] / t -
+                            pow(kk * kk - 4.0 * (kn[2] + kn[3] / t) *
+                                              (kn[4] + kn[5] / t + kn[6] * log(t) - log(pw)),
+                                0.5));
+        const double mnb = 55.51 * xnb / (1.0 - xnb);
+        const double hnb = exp(qn[0] + qn[1] * tr2 +
+                               (qn[2] + qn[3] * tr + qn[4] * tr2 + qn[5] * tr3) * pr +
+                               (qn[6] + qn[7] * tr + qn[8] * tr2) * pr2 + qn[9] * tr * pr3);
+        const double A = (tt * hnb * (mnb * mnb) - tt * hsb * mnb * msb - 2.0 * (mnb * mnb) * msb +
+                          mnb * (msb * msb) + hnb * mnb * msb * pn0 - hsb * (msb * msb) * pn0) /
+                         ((mnb * mnb) - mnb * msb);
+        const double B =
+            msb * (-2.0 * tt * hnb * mnb + tt * hsb * msb + mnb * msb - hnb * msb * pn0) /
+            (mnb - msb);
+        const double C = (tt * hnb * mnb * (msb * msb)) / (mnb - msb);
+        const double u = (A * A) - 3.0 * B;
+        const double v = -2.0 * (A * A * A) + 9.0 * A * B - 27.0 * C;
+        double phi = atan(sqrt(fmax(4.0 * (u * u * u) - (v * v), 0.)) / v);
+        if (phi < 0.) phi = phi + pi;
+        const double msf = fmax(-1.0 / 3.0 * (A + 2.0 * sqrt((A * A) - 3.0 * B) * cos((pi + phi) / 3.0)), 0.);
+        const double mnf = fmax(mnb * (1.0 - msf / msb), 0.);
+        pn = mnf / (hnb * mnf / (mnf + msf) + hsb * msf / (mnf + msf));
+        ms = msf;
+        ws = msf * 0.098076 / (1.0 + msf * 0.098076 + mnf * 0.063012);
+        wn = mnf * 0.063012 / (1.0 + msf * 0.098076 + mnf * 0.063012);
+        pno3l = fmin(1.0 - (pn0 - pn) / pn0, 1.);
+    }
+    ws = fmax(ws, 0.005);
+    const double dens = ternary_density(ws, wn, temp);
+
+    double aliq = 0., rmean = 0., aw = 0., wt = 0., hhcl = 0., hclono2 = 0., hhocl = 0.,
+           hhobr = 0.;
+    if (condliq == 1) {
+        const double ls2 = sq(log(1.8));
+        const double vliq = ns * 98.076e-6 / ws / dens;
+        const double rmode = pow(3.0 * vliq / (4. * pi * NDROP) * exp(-9.0 / 2.0 * ls2), 1. / 3.);
+        rmean = rmode * exp(0.5 * ls2);
+        aliq = 4.0 * pi * (rmode * rmode) * NDROP * exp(2.0 * ls2);
+        const double ph2o0 = exp(18.452406985 - 3505.1578807 / t - 330918.55082 / (t * t) +
+                                 12725068.262 / (t * t * t));
+        aw = fmin(pw * 1013. / ph2o0, 1.);
+        double y1, y2;
+        if (aw <= 0.05) {
+            y1 = 12.37208932 * pow(aw, -0.16125516114) + -30.490657554 * aw + -2.1133114241;
+            y2 = 13.455394705 * pow(aw, -0.1921312255) + -34.285174607 * aw + -1.7620073078;
+        } else if (aw < 0.85) {
+            y1 = 11.820654354 * pow(aw, -0.20786404244) + -4.807306373 * aw + -5.1727540348;
+            y2 = 12.891938068 * pow(aw, -0.232338847708) + -6.4261237757 * aw + -4.9005471319;
+        } else {
+            y1 = -180.06541028 * pow(aw, -0.38601102592) + -93.317846778 * aw + 273.88132245;
+            y2 = -176.95814097 * pow(aw, -0.36257048154) + -90.469744201 * aw + 267.45509988;
+        }
+        const double m = y1 + (t - 190.) * (y2 - y1) / 70.;
+        wt = fmax(9800. * m / (98. * m + 1000.), 0.5);
+        const double mh2so4 = dens * wt / 9.8;
+        const double bigx = wt / (wt + (100. - wt) * 98. / 18.);
+        hhcl = (0.094 - 0.61 * bigx + 1.2 * (bigx * bigx)) *
+               exp(-8.68 + (8515. - 10718. * pow(bigx, 0.7)) / t);
+        const double phcl0 = ck[HCL] / hnm * ptot / 1013.0;
+        const double phcl1 = (phcl0 / (R * t)) / (1. / (R * t) + hhcl * ns / ms);
+        o.phcl = 1.0 - (phcl0 - phcl1) / phcl0;
+        hclono2 = 1.6e-6 * exp(4710. / t) * exp(-(0.306 + 24. / t) * mh2so4);
+        hhocl = 1.91e-6 * exp(5862.4 / t) * exp(-(0.0776 + 59.18 / t) * mh2so4);
+        hhobr = 4.6e-4 * exp(4.5e3 / t);
+    }
+    o.surf = aliq + anat * (double)lnat + aice * (double)lice;
+    o.pno3 = fmin(pno3l * pno3s, 1.);
+
+    // settling velocity of NAT / ice particles (Pruppacher & Klett; Larsen 1991)
+    if (lice == 1 || lnat == 1) {
+        const double rpart = fmin(((double)lice * RICE + (double)lnat * RNATL) * 1.e-2, 535.e-6);
+        if (rpart >= 0.5e-6) {
+            constexpr double bn[7] = {-3.18657, 0.992696, -0.153193e-2, -0.987059e-3,
+                                      -0.578878e-3, 0.855176e-4, -0.327815e-5};
+            const double rhopart = (double)lice * (0.928 * 1000.0) + (double)lnat * (1.350 * 1000.0);
+            const double rhoair = (hnm * 28.97 / 6.022e23) * 1000.0;
+            const double tcel = temp - 273.15;
+            const double eta = (1.718 + 0.0049 * tcel - 1.2e-5 * tcel * tcel) * 1.e-5;
+            const double cd = 32.0 * (rpart * rpart * rpart) * (rhopart - rhoair) * rhoair * 9.80665 /
+                              (3.0 * (eta * eta));
+            const double lx = log(cd), l2 = lx * lx, l4 = l2 * l2;
+            const double nre = exp(bn[0] + bn[1] * lx + bn[2] * l2 + bn[3] * (lx * lx * lx) +
+                                   bn[4] * l4 + bn[5] * (l4 * lx) + bn[6] * (l4 * l2));
+            if (nre >= 1.e-2) {
+                o.vsed = eta * nre / (2 * rhoair * rpart);
+            } else {
+                const double lam = 6.6e-8 * (1013.25 / ptot) * (temp / 293.15);
+                const double us = 2.0 * (rpart * rpart) * 9.80665 * (rhopart - rhoair) / (9.0 * eta);
+                o.vsed = us / 1.2 *
+                         (1.0 + 1.246 * lam / rpart + 0.42 * lam / rpart * exp(-0.87 * rpart / lam));
+            }
+        }
+    }
+
+    // heterogeneous second-order rates: liquid (JPL 2000 / Shi et al.), NAT, ice
+    double kl[9] = {0., 0., 0., 0., 0., 0., 0., 0., 0.};
+    double ks[9] = {0., 0., 0., 0., 0., 0., 0., 0., 0.};
+    if (condliq == 1) {
+        const double T = temp, sT = sqrt(T);
+        const double zh2o = fmax(ck[H2O], 1.E-08 * hnm);
+        const double zhcl = fmax(ck[HCL], 1.E-13 * hnm);
+        const double zclono2 = fmax(ck[CLONO2], 1.E-13 * hnm);
+        const double zhobr = fmax(ck[HOBR], 1.E-15 * hnm);
+        const double phcl = (zhcl / hnm) * ptot / 1013.;
+        const double pclono2 = (zclono2 / hnm) * ptot / 1013.;
+        const double phobr = (zhobr / hnm) * ptot / 1013.;
+        const double wt2 = wt * wt, wt3 = wt * wt * wt;
+        const double t0 = 144.11 + 0.166 * wt - 0.015 * wt2 + 2.18e-4 * wt3;
+        const double biga = 169.5 + 5.18 * wt - 0.0825 * wt2 + 3.27e-3 * wt3;
+        const double eta = biga * pow(T, -1.43) * exp(448. / (T - t0));
+        const double ah = exp(60.51 - 0.095 * wt + 0.0077 * wt2 - 1.61e-5 * wt3 -
+                              (1.76 + 2.52e-4 * wt2) * sT + (-805.89 + 253.05 * pow(wt, 0.076)) / sT);
+        // ClONO2 + H2O / HCl
+        const double dclono2 = 5.e-8 * T / eta;
+        const double cclono2 = 1474. * sT;
+        const double khydr = 1.95e10 * exp(-2800. / T) * aw + 1.22e12 * exp(-6200. / T) * ah * aw;
+        const double gbh2o = 4 * hclono2 * 0.082 * T * sqrt(dclono2 * khydr) / cclono2;
+        const double mhcl = hhcl * phcl;
+        const double khcl = 7.9e11 * ah * dclono2 * mhcl;
+        const double lcl = sqrt(dclono2 / (khydr + khcl));
+        const double fcl = 1. / tanh(rmean / lcl) - lcl / rmean;
+        double grxn = fcl * gbh2o * sqrt(1. + khcl / khydr);
+        const double gbhcl = grxn * khcl / (khcl + khydr);
+        const double gs = 66.12 * hclono2 * mhcl * exp(-1374. / T);
+        const double fhcl = 1. / (1 + 0.612 * (gs + gbhcl) * pclono2 / phcl);
+        const double gps = fhcl * gs, gbhclp = fhcl * gbhcl;
+        const double gb = gbhclp + grxn * khydr / (khcl + khydr);
+        const double g = 1. / (1. + 1. / (gps + gb));
+        const double ghcl = g * (gps + gbhclp) / (gps + gb);
+        kl[0] = 0.25 * (g - ghcl) * cclono2 * aliq / zh2o;
+        kl[1] = 0.25 * ghcl * cclono2 * aliq / zhcl;
+        // HOCl + HCl
+        const double chocl = 2009. * sT;
+        const double dhocl = 6.4e-8 * T / eta;
+        const double khh = 1.25e9 * ah * dhocl * mhcl;
+        const double lhocl = sqrt(dhocl / khh);
+        const double fhocl = 1. / tanh(rmean / lhocl) - lhocl / rmean;
+        grxn = 4. * hhocl * 0.082 * T * sqrt(dhocl * khh) / chocl;
+        kl[4] = 0.25 * (1. / (1. + 1. / (fhocl * grxn * fhcl))) * chocl * aliq / zhcl;
+        // HOBr + HCl (Waschewsky & Abbatt 1999), limiting reagent switch
+        const double mhobr = hhobr * phobr;
+        const double k2 = exp(0.542 * wt - 6.44e3 / T + 10.3);
+        if (mhobr / mhcl < 1.) {
+            const double k1 = k2 * mhcl, dd = 6.2e-8 * T / eta, ll = sqrt(dd / k1);
+            const double ff = 1. / tanh(rmean / ll) - ll / rmean, cb = 1477. * sT;
+            const double gg = 4. * hhobr * 0.082 * T * sqrt(dd * k1) * ff / cb;
+            kl[6] = 0.25 * gg * cb * aliq / zhcl;
+        } else {
+            const double k1 = k2 * mhobr, dd = 7.8e-8 * T / eta, ll = sqrt(dd / k1);
+            const double ff = 1. / tanh(rmean / ll) - ll / rmean, cb = 2408. * sT;
+            const double gg = 4. * hhcl * 0.082 * T * sqrt(dd * k1) * ff / cb;
+            kl[6] = 0.25 * gg * cb * aliq / zhobr;
+        }
+        // N2O5 + H2O (gamma 0.1), BrONO2 + H2O (JPL 2000)
+        kl[2] = 0.25 * 0.1 * (1400.1 * sT) * aliq / zh2o;
+        const double gxrn = exp(29.24 - 0.396 * 100. * ws) + 0.114;
+        kl[5] = 0.25 * (1. / (1. / 0.805 + 1. / gxrn)) * (1221.4 * sT) * aliq / zh2o;
+    }
+    if (lnat == 1) {
+        const double T = temp;
+        const double zh2o = fmax(ck[H2O], 1.E-08 * hnm), zhcl = fmax(ck[HCL], 1.E-13 * hnm);
+        auto nat = [&](double g, double mw, double z) {
+            return solid_uptake(g, T, mw, RNATS, nnats, ptot, z) +
+                   solid_uptake(g, T, mw, RNATL, nnatl, ptot, z);
+        };
+        ks[0] = nat(0.004, 97.45, zh2o);
+        ks[1] = nat(0.2, 97.45, zhcl);
+        ks[2] = nat(4.0E-4, 108.0, zh2o);
+        ks[3] = nat(3.0E-3, 108.0, zhcl);
+        ks[4] = nat(0.1, 52.5, zhcl);
+        ks[5] = nat(0., 142.0, zh2o);
+        ks[6] = nat(0., 96.9, zhcl);
+    }
+    if (lice == 1) {
+        const double T = temp;
+        const double zh2o = fmax(ck[H2O], 1.E-08 * hnm), zhcl = fmax(ck[HCL], 1.E-13 * hnm),
+                     zhobr = fmax(ck[HOBR], 1.E-13 * hnm);
+        ks[0] = solid_uptake(0.3, T, 97.45, RICE, nice, ptot, zh2o);
+        ks[1] = solid_uptake(0.3, T, 97.45, RICE, nice, ptot, zhcl);
+        ks[2] = solid_uptake(0.02, T, 108.0, RICE, nice, ptot, zh2o);
+        ks[3] = solid_uptake(0.03, T, 108.0, RICE, nice, ptot, zhcl);
+        ks[4] = solid_uptake(0.2, T, 52.5, RICE, nice, ptot, zhcl);
+        ks[5] = solid_uptake(0.3, T, 142.0, RICE, nice, ptot, zh2o);
+        ks[6] = solid_uptake(0.3, T, 96.9, RICE, nice, ptot, zhcl);
+        ks[7] = solid_uptake(0.1, T, 96.9, RICE, nice, ptot, zhobr);
+    }
+    const double nsol = (double)(lnat + lice);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) o.k[i] = kl[i] + nsol * ks[i];
+    o.k[8] = 0.;
+}
+
+// ------------------------------------------------------------------ the sweeps
+// Fixed-count family/QSSA iteration of chem3d (F:1348-2378). ck0 is the start-of-step
+// state, ck/cc the iterates (number densities), updated in place.
+struct BoxEnv {
+    double hnm, n2, o2, co2, h2, dt;
+    double het1, het2, het3, het4, het5, het6, het7, het8, het9;
+    double pno3, ph2o, phcl, pbr;
+    double rain_hno3, rain_hno4, rain_hcl, rain_hbr, rain_h2o2, rain_ch2o, rain_ch3o2h;
+    bool day;
+};
+
+__device__ void sweep(const Rates &R, const double *tj, const BoxEnv &e, const double *ck0,
+                      double *ck, double *cc, double &ch3o, double &hco)
+{
+    const double hnm = e.hnm, n2 = e.n2, o2 = e.o2, co2 = e.co2, h2 = e.h2;
+    double day;
+    if (e.day) {
+        day = 1.;
+        cc[CH3] = ((R.c2 * cc[OH] + C1A * cc[O1D] + R.d5 * cc[CL]) * ck[CH4]) /
+                  (R.c4 * o2 + C3 * cc[O3P]);
+        ch3o = (C1B * cc[O1D] * ck[CH4] +
+                (R.c5 * cc[NO] + 2. * R.c14 * ck[CH3O2] + R.c18 * cc[CLO]) * ck[CH3O2] +
+                tj[JCH3O2H] * ck[CH3O2H]) /
+               (R.c15 * o2);
+        hco = (R.c8 * cc[OH] + R.c9 * cc[O3P] + R.c10 * cc[CL] + R.c19 * cc[BR] + tj[JCH2O_HCO]) *
+              ck[CH2O] / (C12 * o2);
+        const double r1 = (tj[JO3_O1D] + tj[JO3_O3P] + R.a2 * cc[H] + R.a6 * cc[OH] +
+                           R.a6b * cc[HO2] + R.b4 * cc[NO] + R.d2 * cc[CL]) /
+                          (R.hk2 * o2);
+        cc[O3] = 1. / (1. + r1) * ck[OX];
+        cc[O3P] = r1 * cc[O3];
+        cc[O1D] = tj[JO3_O1D] / (R.hk4 * n2 + R.hk5 * o2) * cc[O3];
+        const double r4 = (R.a5 * cc[O3P] + R.a36 * ck[CO] + R.a19 * h2) /
+                          (R.a1 * o2 + R.a2 * cc[O3] + (A23A + A23B + A23C) * cc[HO2]);
+        cc[H] = r4 * cc[OH];
+        const double r7 = tj[JNO] / (R.b7 * o2 + R.b6 * cc[NO]);
+        const double r8 = (R.b9 * cc[O3]) / (tj[JNO3_NO] + tj[JNO3_NO2] + R.b12 * cc[NO2]);
+        const double r5 = (tj[JNO2] + R.b3 * cc[O3P] + tj[JNO3_NO] * r8) /
+                          (R.b4 * cc[O3] + R.a26 * cc[HO2] + R.d4 * cc[CLO] + R.e4 * cc[BRO] +
+                           R.c5 * ck[CH3O2]);
+        cc[NO2] = ck[NOX] / (1. + r5 + r7 * r5 + r8);
+        cc[NO] = r5 * cc[NO2];
+        cc[N] = r7 * cc[NO];
+        cc[NO3] = r8 * cc[NO2];
+        const double r6 = (R.a6 * cc[O3] + A30 * ck[H2O2] + R.c8 * ck[CH2O] + R.d8 * cc[CLO] +
+                           R.e10 * cc[BRO] + R.a1 * o2 * r4) /
+                          ((A23A + A23B + A23C) * cc[H] + R.a6b * cc[O3] + R.a7 * cc[O3P] +
+                           R.a26 * cc[NO] + 2. * R.a27 * cc[HO2] + R.a17 * cc[OH] +
+                           R.d7 * cc[CL] + R.d10 * cc[CL] + R.d33 * cc[CLO] + R.e7 * cc[BR] +
+                           R.e15 * cc[BRO]);
+        cc[HO2] = .5 * cc[HO2] + .5 * cc[OH] * r6;        // 0.5 under-relaxation
+        const double r13 = R.d31 * cc[NO2] /
+                           (tj[JCLONO2_CL] + tj[JCLONO2_CLO] + R.d32 * cc[O3P] +
+                            e.het1 * ck[H2O] + e.het2 * ck[HCL]);
+        const double r9 = (R.d33 * cc[HO2] + e.het1 * ck[H2O] * r13) /
+                          (e.het5 * ck[HCL] + D35 * cc[O3P] + tj[JHOCL]);
+        const double r10 = (R.d60 * cc[CLO]) / (R.d61 * hnm + tj[JCL2O2]);
+        const double r12 = R.e5c * cc[BRO] / tj[JBRCL];
+        const double r11 = (R.d3 * cc[O3P] + R.d4 * cc[NO] + R.d8 * cc[OH] + R.c18 * ck[CH3O2] +
+                            R.e5b * cc[BRO] + tj[JCLONO2_CL] * r13 + tj[JHOCL] * r9 +
+                            2. * tj[JCL2O2] * r10 + tj[JBRCL] * r12) /
+                           (R.d2 * cc[O3] + R.d5 * ck[CH4] + R.d7 * cc[HO2] + R.d10 * cc[HO2] +
+                            R.d37 * ck[HOCL]);
+        cc[CLO] = .5 * cc[CLO] + .5 * ck[CLOX] / (1. + r11);
+        cc[CL] = r11 * cc[CLO];
+        const double r27 = R.e13 * cc[NO2] / (tj[JBRONO2] + R.e8 * cc[O3P]);
+        const double r28 = R.e15 * cc[HO2] / (tj[JHOBR] + R.e16 * cc[O3P]);
+        const double r29 = R.e5c * cc[CLO] / tj[JBRCL];
+        const double r30 = (R.e3 * cc[O3P] + R.e4 * cc[NO] + R.e5a * cc[CLO] + R.e5b * cc[CLO] +
+                            2. * R.e6 * cc[BRO] + R.e10 * cc[OH] + tj[JBRO] +
+                            tj[JBRONO2] * 0.85 * r27 + tj[JHOBR] * r28 + tj[JBRCL] * r29) /
+                           (R.e2 * cc[O3] + R.e7 * cc[HO2] + R.c19 * ck[CH2O]);
+        cc[BRO] = .5 * cc[BRO] + .5 * ck[BROX] / (1. + r30);
+        cc[BR] = r30 * cc[BRO];
+        // OH from its quadratic budget (F:1600-1665)
+        const double coea = 2. * A31 + 2. * (R.a17 + R.a27 * r6 + (A23B + A23C) * r4) * r6;
+        const double coeb = R.b27 * ck[HNO3] * e.pno3 + R.b22 * cc[NO2] + R.b28 * ck[HO2NO2] +
+                            R.b23 * cc[NO2] * r6 + R.c2 * ck[CH4] + R.c7 * ck[CH3O2] * r6 +
+                            R.c8 * ck[CH2O] + R.c17a * ck[CH3O2H] + R.d7 * cc[CL] * r6 +
+                            R.d9 * cc[CLO] + R.d11 * ck[HCL] * e.phcl + R.d33 * cc[CLO] * r6 +
+                            R.d34 * ck[HOCL] + R.e11 * ck[HBR] * e.pbr + R.e7 * cc[BR] * r6 +
+                            R.e15 * cc[BRO] * r6;
+        const double coec =
+            2. * (R.a1et * cc[O1D] + tj[JH2O]) * ck[H2O] * e.ph2o +
+            ((C1A + C1B) * ck[CH4] + 2. * A3ET * h2) * cc[O1D] + C3 * cc[CH3] * cc[O3P] +
+            tj[JCH3O2H] * ck[CH3O2H] + R.c15 * ch3o * o2 +
+            (tj[JCH2O_HCO] + R.c9 * cc[O3P]) * ck[CH2O] + C12 * hco * o2 +
+            tj[JHNO3] * ck[HNO3] * e.pno3 + (R.b24 * hnm + tj[JHNO4]) * ck[HO2NO2] +
+            (D35 * cc[O3P] + R.d37 * cc[CL] + tj[JHOCL]) * ck[HOCL] + 2. * tj[JH2O2] * ck[H2O2] +
+            R.d6 * h2 * cc[CL] + tj[JHOBR] * ck[HOBR] + R.e16 * cc[O3P] * ck[HOBR] +
+            tj[JHBR] * ck[HBR];
+        const double argum = fmax(coeb * coeb + 4. * coea * coec, 0.);
+        cc[OH] = .5 * cc[OH] + .5 * (sqrt(argum) - coeb) / (2. * coea);
+    } else {
+        // night (F:1671-1746)
+        day = 0.;
+        cc[H] = 0.;
+        cc[CH3] = 0.;
+        ch3o = 0.;
+        hco = 0.;
+        cc[O1D] = 0.;
+        cc[O3P] = 0.;
+        cc[O3] = ck[OX];
+        cc[OH] = 0.;
+        cc[CL] = 0.;
+        cc[HO2] = 0.;
+        cc[NO] = 0.;
+        const double r8 = R.b9 * cc[O3] / (R.b12 * fmax(cc[NO2], 1.e-30));
+        cc[NO2] = ck[NOX] / (1. + r8);
+        cc[NO3] = r8 * cc[NO2];
+        cc[N] = 0.;
+        cc[CLO] = fmax(ck[CLOX], 0.);
+        cc[BR] = 0.;
+        cc[BRO] = fmax(ck[BROX], 0.);
+    }
+
+    // production P and loss L (F:1758-2358), then ck = (ck0 + dt P)/(1 + dt L)
+    const double dt = e.dt;
+    double nk[NLONG];
+#define UPDATE(s, P, Lr) nk[s] = (ck0[s] + dt * (P)) / (1. + dt * (Lr))
+    UPDATE(N2O, 0.e0, tj[JN2O] + (R.b38 + R.b39) * cc[O1D]);
+    UPDATE(CH4, 0.e0, R.c2 * cc[OH] + C1A * cc[O1D] + C1B * cc[O1D] + C1C * cc[O1D] + R.d5 * cc[CL]);
+    UPDATE(H2O,
+           R.c2 * cc[OH] * ck[CH4] + R.c8 * cc[OH] * ck[CH2O] + R.c17a * ck[CH3O2H] * cc[OH] +
+               R.c17b * ck[CH3O2H] * cc[OH] + (R.a17 * cc[OH] + A23C * cc[H]) * cc[HO2] +
+               R.a19 * cc[OH] * h2 + A30 * cc[OH] * ck[H2O2] + A31 * cc[OH] * cc[OH] +
+               R.b27 * cc[OH] * ck[HNO3] * e.pno3 + R.d11 * cc[OH] * ck[HCL] * e.phcl,
+           (tj[JH2O] + R.a1et * cc[O1D]) * e.ph2o);
+    UPDATE(NOY, 2. * R.b39 * cc[O1D] * ck[N2O],
+           (2. * R.b6 * cc[N] * cc[NO] + e.rain_hno3 * ck[HNO3] + e.rain_hno4 * ck[HO2NO2]) / ck[NOY]);
+    UPDATE(HNO3,
+           R.b22 * cc[OH] * cc[NO2] + e.het1 * ck[CLONO2] * ck[H2O] + e.het2 * ck[CLONO2] * ck[HCL] +
+               2. * e.het3 * ck[N2O5] * ck[H2O] + e.het4 * ck[N2O5] * ck[HCL] +
+               e.het6 * ck[BRONO2] * ck[H2O],
+           (tj[JHNO3] + R.b27 * cc[OH]) * e.pno3 + e.rain_hno3);
+    UPDATE(N2O5, R.b12 * cc[NO3] * cc[NO2],
+           tj[JN2O5] + R.b32 * hnm + e.het3 * ck[H2O] + e.het4 * ck[HCL]);
+    UPDATE(CLY,
+           3. * (tj[JCFC11] + D90 * cc[O1D]) * ck[CFC11] + 2. * (tj[JCFC12] + D91 * cc[O1D]) * ck[CFC12] +
+               3. * (tj[JCFC113] + D92 * cc[O1D]) * ck[CFC113] +
+               4. * (tj[JCCL4] + D93 * cc[O1D]) * ck[CCL4] +
+               3. * (tj[JCH3CCL3] + R.d94 * cc[OH]) * ck[CH3CCL3] +
+               1. * (tj[JCH3CL] + R.d95 * cc[OH]) * ck[CH3CL] +
+               1. * (tj[JHCFC22] + D96 * cc[O1D] + R.d97 * cc[OH]) * ck[HCFC22] +
+               1. * (tj[JH1211] + E52 * cc[O1D]) * ck[H1211],
+           e.rain_hcl * ck[HCL] / ck[CLY]);
+    {
+        // odd oxygen (F:1864-1950)
+        const double ratio = tj[JCL2O2] / (tj[JCL2O2] + R.d61 * hnm);
+        const double fno3 = tj[JNO3_NO] / fmax(tj[JNO3_NO] + tj[JNO3_NO2], 1.e-30);
+        const double loss =
+            2. * R.hk1 * cc[O3P] * cc[O3P] + 2. * R.hk3 * cc[O3] * cc[O3P] +
+            2. * R.b3 * cc[NO2] * cc[O3P] + 2. * R.d3 * cc[CLO] * cc[O3P] +
+            2. * R.d60 * ratio * cc[CLO] * cc[CLO] +
+            fmin(tj[JHOCL] * ck[HOCL], R.d33 * cc[HO2] * cc[CLO]) +
+            2. * fmin(R.e5c * cc[BRO] * cc[CLO], tj[JBRCL] * ck[BRCL]) +
+            2. * R.e5b * cc[BRO] * cc[CLO] * day + fmin(R.e15 * cc[BRO] * cc[HO2], tj[JHOBR] * ck[HOBR]) +
+            2. * R.e6 * cc[BRO] * cc[BRO] * day + 2. * fno3 * tj[JCLONO2_CL] * ck[CLONO2] +
+            ((R.a2 * cc[H] + R.a6 * cc[OH] + R.a6b * cc[HO2]) * cc[O3] +
+             (R.a5 * cc[OH] + R.a7 * cc[HO2]) * cc[O3P]) +
+            2. * R.b9 * cc[NO2] * cc[O3] * tj[JNO3_NO] / fmax(tj[JNO3_NO] + tj[JNO3_NO2], 1.e-30);
+        UPDATE(OX, 2. * tj[JO2] * o2 + R.a26 * cc[HO2] * cc[NO] + R.c5 * cc[NO] * ck[CH3O2],
+               loss / ck[OX]);
+    }
+    UPDATE(CO, tj[JCO2] * co2 + tj[JCH2O_CO] * ck[CH2O] + C12 * hco * o2, R.a36 * cc[OH]);
+    UPDATE(OCLO, R.e5a * cc[BRO] * cc[CLO], tj[JOCLO]);
+    UPDATE(OXP, 0., 0.);
+    UPDATE(H2SO4, 0., 0.);
+    UPDATE(HCL,
+           (R.c10 * ck[CH2O] + R.d5 * ck[CH4] + R.d7 * cc[HO2] + R.d6 * h2) * cc[CL] +
+               R.d9 * cc[OH] * cc[CLO],
+           (tj[JHCL] + R.d11 * cc[OH]) * e.phcl + e.het2 * ck[CLONO2] + e.het4 * ck[N2O5] +
+               e.het5 * ck[HOCL] + e.het7 * ck[HOBR] + e.rain_hcl);
+    UPDATE(CLONO2, R.d31 * cc[NO2] * cc[CLO],
+           tj[JCLONO2_CL] + tj[JCLONO2_CLO] + R.d32 * cc[O3P] + e.het1 * ck[H2O] + e.het2 * ck[HCL]);
+    UPDATE(HOCL, R.d33 * cc[HO2] * cc[CLO] + e.het1 * ck[CLONO2] * ck[H2O],
+           tj[JHOCL] + R.d34 * cc[OH] + D35 * cc[O3P] + R.d37 * cc[CL] + e.het5 * ck[HCL] +
+               e.het9 * ck[HBR]);
+    UPDATE(CL2,
+           R.d37 * ck[HOCL] * cc[CL] + R.d62 * cc[CLO] * cc[CLO] + e.het2 * ck[CLONO2] * ck[HCL] +
+               e.het5 * ck[HOCL] * ck[HCL],
+           tj[JCL2]);
+    UPDATE(H2O2, R.a27 * cc[HO2] * cc[HO2], tj[JH2O2] + A30 * cc[OH] + e.rain_h2o2);
+    UPDATE(CLNO2, R.d36 * cc[CL] * cc[NO2] + e.het4 * ck[N2O5] * ck[HCL], tj[JCLNO2]);
+    UPDATE(HBR, R.e7 * cc[HO2] * cc[BR] + R.c19 * ck[CH2O] * cc[BR],
+           (R.e11 * cc[OH] + E9 * cc[O1D] + tj[JHBR]) * e.pbr + e.het8 * ck[HOBR] +
+               e.het9 * ck[HOCL] + e.rain_hbr);
+    UPDATE(BRONO2, R.e13 * cc[BRO] * cc[NO2], tj[JBRONO2] + R.e8 * cc[O3P] + e.het6 * ck[H2O]);
+    UPDATE(NOX,
+           (tj[JHNO4] + R.b28 * cc[OH] + R.b24 * hnm) * ck[HO2NO2] + tj[JHNO3] * ck[HNO3] * e.pno3 +
+               2. * (tj[JN2O5] + R.b32 * hnm) * ck[N2O5] + 2. * R.b39 * cc[O1D] * ck[N2O] +
+               tj[JCLNO2] * ck[CLNO2] + R.b27 * ck[HNO3] * e.pno3 * cc[OH] +
+               R.d32 * ck[CLONO2] * cc[O3P] + (tj[JCLONO2_CL] + tj[JCLONO2_CLO]) * ck[CLONO2] +
+               tj[JBRONO2] * ck[BRONO2] + R.e8 * ck[BRONO2] * cc[O3P],
+           ((R.d31 * cc[CLO] + R.b22 * cc[OH] + R.b23 * cc[HO2] + 2. * R.b12 * cc[NO3] +
+             R.d36 * cc[CL] + R.e13 * cc[BRO]) * cc[NO2] +
+            2. * R.b6 * cc[N] * cc[NO]) / ck[NOX]);
+    UPDATE(HO2NO2, R.b23 * cc[HO2] * cc[NO2], tj[JHNO4] + R.b28 * cc[OH] + R.b24 * hnm + e.rain_hno4);
+    UPDATE(CLOX,
+           2. * (R.d61 * hnm + tj[JCL2O2]) * ck[CL2O2] +
+               (tj[JCLONO2_CL] + tj[JCLONO2_CLO] + R.d32 * cc[O3P]) * ck[CLONO2] +
+               (tj[JHOCL] + R.d34 * cc[OH] + D35 * cc[O3P]) * ck[HOCL] +
+               (tj[JOCLO] * ck[OCLO] + (tj[JHCL] + R.d11 * cc[OH]) * ck[HCL] * e.phcl) +
+               2. * tj[JCL2] * ck[CL2] + tj[JCLNO2] * ck[CLNO2] + tj[JBRCL] * ck[BRCL] +
+               3. * (tj[JCFC11] + D90 * cc[O1D]) * ck[CFC11] +
+               2. * (tj[JCFC12] + D91 * cc[O1D]) * ck[CFC12] +
+               3. * (tj[JCFC113] + D92 * cc[O1D]) * ck[CFC113] +
+               4. * (tj[JCCL4] + D93 * cc[O1D]) * ck[CCL4] +
+               3. * (tj[JCH3CCL3] + R.d94 * cc[OH]) * ck[CH3CCL3] +
+               1. * (tj[JCH3CL] + R.d95 * cc[OH]) * ck[CH3CL] +
+               1. * (tj[JHCFC22] + D96 * cc[O1D] + R.d97 * cc[OH]) * ck[HCFC22],
+           ((R.d31 * cc[NO2] + 2. * R.d60 * cc[CLO] + 2. * R.d62 * cc[CLO] + R.e5a * cc[BRO] +
+             R.e5c * cc[BRO] + R.d9 * cc[OH] + R.d33 * cc[HO2]) * cc[CLO] +
+            (R.c10 * ck[CH2O] + R.d5 * ck[CH4] + R.d7 * cc[HO2] + R.d6 * h2 + R.d36 * cc[NO2] +
+             R.d37 * ck[HOCL]) * cc[CL]) / ck[CLOX]);
+    UPDATE(BROX,
+           tj[JHOBR] * ck[HOBR] + tj[JBRONO2] * ck[BRONO2] + R.e8 * ck[BRONO2] * cc[O3P] +
+               R.e16 * ck[HOBR] * cc[O3P] + tj[JBRCL] * ck[BRCL] +
+               (R.e11 * cc[OH] + E9 * cc[O1D] + tj[JHBR]) * ck[HBR] * e.pbr +
+               2. * e.het8 * ck[HBR] * ck[HOBR],
+           ((R.e13 * cc[NO2] + R.e5c * cc[CLO] + R.e15 * cc[HO2]) * cc[BRO] +
+            R.c19 * cc[BR] * ck[CH2O] + R.e7 * cc[BR] * cc[HO2]) / ck[BROX]);
+    UPDATE(CL2O2, R.d60 * cc[CLO] * cc[CLO], tj[JCL2O2] + R.d61 * hnm);
+    UPDATE(HOBR, R.e15 * cc[BRO] * cc[HO2] + e.het6 * ck[BRONO2] * ck[H2O],
+           tj[JHOBR] + R.e16 * cc[O3P] + e.het7 * ck[HCL] + e.het8 * ck[HBR]);
+    UPDATE(BRCL,
+           R.e5c * cc[CLO] * cc[BRO] + e.het7 * ck[HOBR] * ck[HCL] + e.het9 * ck[HBR] * ck[HOCL],
+           tj[JBRCL]);
+    UPDATE(CH2O,
+           (C1C * cc[O1D] * ck[CH4] + C3 * cc[O3P] * cc[CH3] + R.c15 * ch3o * o2 +
+            R.c17b * cc[OH] * ck[CH3O2H]),
+           R.c8 * cc[OH] + R.c9 * cc[O3P] + R.c10 * cc[CL] + R.c19 * cc[BR] + tj[JCH2O_HCO] +
+               tj[JCH2O_CO] + e.rain_ch2o);
+    UPDATE(CH3O2, R.c4 * cc[CH3] * o2 + R.c17a * cc[OH] * ck[CH3O2H],
+           2. * R.c14 * ck[CH3O2] + R.c7 * cc[HO2] + R.c5 * cc[NO] + R.c18 * cc[CLO]);
+    UPDATE(CH3O2H, R.c7 * ck[CH3O2] * cc[HO2],
+           R.c17a * cc[OH] + R.c17b * cc[OH] + tj[JCH3O2H] + e.rain_ch3o2h);
+    UPDATE(CFC11, 0., tj[JCFC11] + D90 * cc[O1D]);
+    UPDATE(CFC12, 0., tj[JCFC12] + D91 * cc[O1D]);
+    UPDATE(CFC113, 0., tj[JCFC113] + D92 * cc[O1D]);
+    UPDATE(CCL4, 0., tj[JCCL4] + D93 * cc[O1D]);
+    UPDATE(CH3CCL3, 0., tj[JCH3CCL3] + R.d94 * cc[OH]);
+    UPDATE(CH3CL, 0., tj[JCH3CL] + R.d95 * cc[OH]);
+    UPDATE(HCFC22, 0., tj[JHCFC22] + D96 * cc[O1D] + R.d97 * cc[OH]);
+    UPDATE(CH3BR, 0., tj[JCH3BR] + E50 * cc[O1D] + R.e51 * cc[OH]);
+    UPDATE(H1211, 0., tj[JH1211] + E52 * cc[O1D]);
+    UPDATE(H1301, 0., tj[JH1301] + E53 * cc[O1D]);
+    UPDATE(BRY,
+           (tj[JCH3BR] + E50 * cc[O1D] + R.e51 * cc[OH]) * ck[CH3BR] +
+               (tj[JH1211] + E52 * cc[O1D]) * ck[H1211] + (tj[JH1301] + E53 * cc[O1D]) * ck[H1301] +
+               2. * (R.e54 * cc[OH] + tj[JCH2BR2]) * ck[CH2BR2],
+           e.rain_hbr * ck[HBR] / ck[BRY]);
+    UPDATE(CH2BR2, 0., tj[JCH2BR2] + R.e54 * cc[OH]);
+#undef UPDATE
+#pragma unroll
+    for (int s = 0; s < NLONG - 1; ++s) ck[s] = nk[s];
+}
+
+// ------------------------------------------------------------------ kernels
+__global__ void k_columns(const double *__restrict__ p, const double *__restrict__ T,
+                          const double *__restrict__ hc, double *__restrict__ col, int64_t plane,
+                          int64_t field, int32_t nivbas)
+{
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= plane) return;
+    const int64_t nb = plane * (int64_t)nivbas;
+    const double zcmt_t = 2.120e20, zcmt_j = 287.0 / (1.38e-23 * 9.81), r_o2 = 0.2095;
+    const double *o3 = hc + (int64_t)O3 * field, *no = hc + (int64_t)NO * field;
+    double o3t = 0., co2 = 0., cno = 0., hprev = 0., nprev = 0., pprev = 0., o3prev = 0.;
+    for (int k = 0; k < nivbas; ++k) {
+        const int64_t b = (int64_t)k * plane + c;
+        const double pk = p[b], hnm = pk / (T[b] * 1.38e-19), ccno = no[b] * hnm, o3k = o3[b];
+        if (k == 0) {
+            o3t = o3k * hnm * 6.8e5;                       // total, F:5170
+            co2 = 6.8e+05 * r_o2 * hnm;                     // jno, J:109
+            cno = 6.8e+05 * ccno;                           // jno, J:110
+        } else {
+            const double dp = fmax((pk - pprev) * 100., 0.);
+            o3t = o3t + zcmt_t * (o3k + o3prev) * .5 * dp;  // F:5176-5181
+            co2 = co2 + zcmt_j * (r_o2 + r_o2) * 0.5 * dp * 1.e-4;
+            cno = cno + zcmt_j * (nprev / hprev + ccno / hnm) * 0.5 * dp * 1.e-4;
+        }
+        col[b] = o3t;
+        col[nb + b] = co2;
+        col[2 * nb + b] = cno;
+        hprev = hnm;
+        nprev = ccno;
+        pprev = pk;
+        o3prev = o3k;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_chem(StepArgs a)
+{
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= a.nbox) return;
+    const int64_t c = b % a.plane;
+    const int k = (int)(b / a.plane);
+
+    const double t = a.T[b], pm = a.p[b], sza = a.sza[b];
+    BoxEnv e;
+    e.hnm = pm / (t * 1.38e-19);
+    e.n2 = 0.78 * e.hnm;
+    e.o2 = 0.21 * e.hnm;
+    e.co2 = 360.e-6 * e.hnm;
+    e.h2 = 5.e-7 * e.hnm;
+    e.dt = a.dt;
+    e.day = sza <= 95.;
+
+    double ck0[NLONG], ck[NLONG], cc[NSHORT];
+#pragma unroll
+    for (int s = 0; s < NLONG; ++s) {
+        ck0[s] = a.qj1[(int64_t)s * a.field + b] * e.hnm;
+        ck[s] = ck0[s];
+    }
+#pragma unroll
+    for (int s = 0; s < NSHORT; ++s) cc[s] = a.hc[(int64_t)s * a.field + b] * e.hnm;
+
+    double tj[NJ];
+    table_j(a.ajl, a.o3up, a.alt[b], sza, a.col[b], tj);
+    tj[JNO] = j_no(a.col[a.nbox + b], a.col[2 * a.nbox + b], a.col[b], e.hnm, a.sza[c]);
+
+    Het h;
+    const bool het_level = (k + 1 >= a.nivhau) && (k + 1 <= a.nivbas);
+    if (het_level) {
+        psc_and_het(t, pm, e.hnm, ck, h);
+    } else {
+#pragma unroll
+        for (int i = 0; i < 9; ++i) h.k[i] = 0.;
+        h.pno3 = h.pno3sl = h.ph2o = h.phcl = 1.;
+        h.vsed = 0.;
+    }
+    e.het1 = h.k[0]; e.het2 = h.k[1]; e.het3 = h.k[2]; e.het4 = h.k[3]; e.het5 = h.k[4];
+    e.het6 = h.k[5]; e.het7 = h.k[6]; e.het8 = h.k[7]; e.het9 = h.k[8];
+    e.pno3 = h.pno3;
+    e.ph2o = h.ph2o;
+    e.phcl = h.phcl;
+    e.pbr = 1.;
+    {
+        // rain-out above 100 ppmv water (F:1315-1338)
+        const bool wet = ck[H2O] / e.hnm >= 100.e-6;
+        const double rr = wet ? 1.e-6 : 0.;
+        e.rain_hno3 = rr; e.rain_hcl = rr; e.rain_hbr = rr; e.rain_h2o2 = rr;
+        e.rain_hno4 = 0.; e.rain_ch2o = 0.; e.rain_ch3o2h = 0.;
+    }
+
+    Rates R;
+    thermal_rates(a.K, t, e.hnm, pm, R);
+
+    double ch3o = 0., hco = 0.;
+    for (int it = 0; it < a.niter; ++it) sweep(R, tj, e, ck0, ck, cc, ch3o, hco);
+
+    // write-back (F:2380-2410)
+    double q[NLONG], hs[NSHORT];
+#pragma unroll
+    for (int s = 0; s < NLONG - 1; ++s) q[s] = ck[s] / e.hnm;
+    q[HNO3G] = ck[HNO3] * e.pno3 / e.hnm;
+#pragma unroll
+    for (int s = 0; s < NSHORT; ++s) hs[s] = cc[s] / e.hnm;
+
+    if (a.fuse_conserv) {
+        // conserv (F:2541-2598), then the floor (F:431-440)
+        const double rcly = (q[CLY] - q[BRCL]) /
+                            (q[OCLO] + q[HCL] + q[CLONO2] + q[HOCL] + 2. * q[CL2] + q[CLNO2] +
+                             q[CLOX] + 2. * q[CL2O2]);
+        hs[CL] *= rcly; hs[CLO] *= rcly;
+        q[OCLO] *= rcly; q[HCL] *= rcly; q[CLONO2] *= rcly; q[HOCL] *= rcly; q[CL2] *= rcly;
+        q[CLNO2] *= rcly; q[CLOX] *= rcly; q[CL2O2] *= rcly;
+        const double rnoy = fmax((q[NOY] - q[CLONO2] - q[CLNO2] - q[BRONO2]), 0.) /
+                            (q[HNO3] + 2. * q[N2O5] + q[NOX] + q[HO2NO2]);
+        hs[NO2] *= rnoy; hs[NO] *= rnoy; hs[N] *= rnoy; hs[NO3] *= rnoy;
+        q[HNO3] *= rnoy; q[N2O5] *= rnoy; q[NOX] *= rnoy; q[HO2NO2] *= rnoy;
+        const double rbry = q[BRY] / (q[HBR] + q[BRONO2] + q[BROX] + q[HOBR] + q[BRCL]);
+        hs[BR] *= rbry; hs[BRO] *= rbry;
+        q[HBR] *= rbry; q[BRONO2] *= rbry; q[BROX] *= rbry; q[HOBR] *= rbry; q[BRCL] *= rbry;
+#pragma unroll
+        for (int s = 0; s < NLONG - 1; ++s) q[s] = fmax(q[s], 1.e-30);
+    }
+#pragma unroll
+    for (int s = 0; s < NLONG; ++s) a.qj1[(int64_t)s * a.field + b] = q[s];
+#pragma unroll
+    for (int s = 0; s < NSHORT; ++s) a.hc[(int64_t)s * a.field + b] = hs[s];
+    if (het_level) a.surfarea[b] = h.surf;
+    a.vsed[b] = h.vsed;
+    a.p2o[b] = h.ph2o;
+    a.pno3[b] = h.pno3sl;
+}
+
+__global__ void k_conserv_floor(double *__restrict__ qj1, double *__restrict__ hc, int64_t plane,
+                                int64_t field, int32_t nivbas, int64_t start)
+{
+    const int64_t b = start + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= field) return;
+    double *q = qj1 + b, *h = hc + b;
+#define Q(s) q[(int64_t)(s) * field]
+#define HS(s) h[(int64_t)(s) * field]
+    if (b < plane * (int64_t)nivbas) {
+        const double rcly = (Q(CLY) - Q(BRCL)) /
+                            (Q(OCLO) + Q(HCL) + Q(CLONO2) + Q(HOCL) + 2. * Q(CL2) + Q(CLNO2) +
+                             Q(CLOX) + 2. * Q(CL2O2));
+        HS(CL) *= rcly; HS(CLO) *= rcly;
+        Q(OCLO) *= rcly; Q(HCL) *= rcly; Q(CLONO2) *= rcly; Q(HOCL) *= rcly; Q(CL2) *= rcly;
+        Q(CLNO2) *= rcly; Q(CLOX) *= rcly; Q(CL2O2) *= rcly;
+        const double rnoy = fmax((Q(NOY) - Q(CLONO2) - Q(CLNO2) - Q(BRONO2)), 0.) /
+                            (Q(HNO3) + 2. * Q(N2O5) + Q(NOX) + Q(HO2NO2));
+        HS(NO2) *= rnoy; HS(NO) *= rnoy; HS(N) *= rnoy; HS(NO3) *= rnoy;
+        Q(HNO3) *= rnoy; Q(N2O5) *= rnoy; Q(NOX) *= rnoy; Q(HO2NO2) *= rnoy;
+        const double rbry = Q(BRY) / (Q(HBR) + Q(BRONO2) + Q(BROX) + Q(HOBR) + Q(BRCL));
+        HS(BR) *= rbry; HS(BRO) *= rbry;
+        Q(HBR) *= rbry; Q(BRONO2) *= rbry; Q(BROX) *= rbry; Q(HOBR) *= rbry; Q(BRCL) *= rbry;
+    }
+#pragma unroll
+    for (int s = 0; s < NLONG - 1; ++s) Q(s) = fmax(Q(s), 1.e-30);
+#undef Q
+#undef HS
+}
+
+__global__ void k_count_nonfinite(const double *__restrict__ x, int64_t n,
+                                  unsigned long long *__restrict__ count)
+{
+    unsigned long long local = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        local += !isfinite(x[i]);
+    // wave-level reduction, then one atomic per wave
+    for (int off = warpSize / 2; off > 0; off >>= 1) local += __shfl_down(local, off);
+    if ((threadIdx.x & (warpSize - 1)) == 0 && local) atomicAdd(count, local);
+}
+
+// ------------------------------------------------------------------ host launchers
+Troe troe_constants()
+{
+    Troe c;
+    c.a0hox = 4.4e-32 * pow(300., 1.3);
+    c.a1hox = 7.5e-11 * pow(300., -0.2);
+    c.a0nox1 = 2.0e-30 * pow(300., 4.4);
+    c.a1nox1 = 1.4e-12 * pow(300., 0.7);
+    c.a0nox2 = 1.8e-30 * pow(300., 3.0);
+    c.a1nox2 = 2.8e-11 * pow(300., 0.0);
+    c.a0nox3 = 2.0e-31 * pow(300., 3.4);
+    c.a1nox3 = 2.9e-12 * pow(300., 1.1);
+    c.a0c1 = 4.0e-31 * pow(300., 3.6);
+    c.a1c1 = 1.2e-12 * pow(300., -1.1);
+    c.a0clx1 = 1.8e-31 * pow(300., 3.4);
+    c.a1clx1 = 1.5e-11 * pow(300., 1.9);
+    c.a0clx2 = 1.8e-31 * pow(300., 2.0);
+    c.a1clx2 = 1.0e-10 * pow(300., 1.0);
+    c.a0clx3 = 1.6e-32 * pow(300., 4.5);
+    c.a1clx3 = 3.0e-12 * pow(300., 2.0);
+    c.a0brx = 5.2e-31 * pow(300., 3.2);
+    c.a1brx = 6.9e-12 * pow(300., 2.9);
+    return c;
+}
+
+hipError_t launch_columns(const StepArgs &a, double *col, hipStream_t s)
+{
+    const int blk = 64;
+    const int64_t grid = (a.plane + blk - 1) / blk;
+    hipLaunchKernelGGL(k_columns, dim3((unsigned)grid), dim3(blk), 0, s, a.p, a.T, a.hc, col,
+                       a.plane, a.field, a.nivbas);
+    return hipGetLastError();
+}
+
+hipError_t launch_chem(const StepArgs &a, int block, hipStream_t s)
+{
+    const int blk = block > 0 ? block : 256;
+    const int64_t grid = (a.nbox + blk - 1) / blk;
+    hipLaunchKernelGGL(k_chem, dim3((unsigned)grid), dim3(blk), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_conserv_floor(double *qj1, double *hc, int64_t plane, int64_t field,
+                                int32_t nivbas, int64_t start, hipStream_t s)
+{
+    const int blk = 256;
+    const int64_t n = field - start;
+    if (n <= 0) return hipSuccess;
+    const int64_t grid = (n + blk - 1) / blk;
+    hipLaunchKernelGGL(k_conserv_floor, dim3((unsigned)grid), dim3(blk), 0, s, qj1, hc, plane,
+                       field, nivbas, start);
+    return hipGetLastError();
+}
+
+hipError_t launch_count_nonfinite(const double *x, int64_t n, unsigned long long *count,
+                                  hipStream_t s)
+{
+    const int blk = 256;
+    int64_t grid = (n + blk - 1) / blk;
+    if (grid > 4096) grid = 4096;
+    if (grid < 1) grid = 1;
+    hipLaunchKernelGGL(k_count_nonfinite, dim3((unsigned)grid), dim3(blk), 0, s, x, n, count);
+    return hipGetLastError();
+}
+
+}  // namespace rb

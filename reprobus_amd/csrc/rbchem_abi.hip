@@ -1,0 +1,339 @@
+// C ABI of the chemistry step (declared in include/rbchem.h).
+#include "../../include/rbchem.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "chem_device.h"
+
+struct rb_ctx {
+    int device = 0;
+    hipStream_t own = nullptr;
+    hipStream_t stream = nullptr;
+    rb_config cfg{5, 0, 256, 0};
+    rb::Troe K{};
+    double *d_ajl = nullptr, *d_o3up = nullptr;
+    bool table = false;
+    double *d_col = nullptr;          // column pre-pass output: 3 * nbox
+    size_t col_cap = 0;
+    double *d_stage = nullptr;        // staging for the host-array entry points
+    size_t stage_cap = 0;
+    unsigned long long *d_count = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_h2d = nullptr, ev_d2h = nullptr;
+    bool timed = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> prof;   // profile-mode event pairs
+    size_t prof_used = 0;
+    rb_stats_t stats{};
+    std::string err;
+};
+
+namespace {
+
+int fail(rb_ctx *c, int code, const std::string &msg)
+{
+    if (c) c->err = msg;
+    return code;
+}
+
+int hip_fail(rb_ctx *c, hipError_t e, const char *what)
+{
+    return fail(c, RB_EHIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIPCHK(call)                                          \
+    do {                                                      \
+        hipError_t _e = (call);                               \
+        if (_e != hipSuccess) return hip_fail(ctx, _e, #call); \
+    } while (0)
+
+int check_grid(rb_ctx *ctx, const rb_grid *g)
+{
+    if (!g) return fail(ctx, RB_EINVAL, "grid is NULL");
+    if (g->nlon < 1 || g->nlat < 1 || g->niv < 1)
+        return fail(ctx, RB_EINVAL, "grid extents must be positive");
+    if (g->nivbas < 1 || g->nivbas > g->niv)
+        return fail(ctx, RB_EINVAL, "nivbas must be in 1..niv");
+    if (g->nivhau < 1) return fail(ctx, RB_EINVAL, "nivhau must be >= 1");
+    return RB_OK;
+}
+
+int ensure(rb_ctx *ctx, double **buf, size_t *cap, size_t n)
+{
+    if (*cap >= n) return RB_OK;
+    if (*buf) (void)hipFree(*buf);
+    *buf = nullptr;
+    *cap = 0;
+    hipError_t e = hipMalloc((void **)buf, n * sizeof(double));
+    if (e != hipSuccess) {
+        *buf = nullptr;
+        return fail(ctx, RB_ENOMEM, std::string("device allocation of ") +
+                                        std::to_string(n * sizeof(double)) + " bytes failed: " +
+                                        hipGetErrorString(e));
+    }
+    *cap = n;
+    return RB_OK;
+}
+
+bool any_null(std::initializer_list<const void *> ps)
+{
+    for (const void *p : ps)
+        if (!p) return true;
+    return false;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rb_init(rb_ctx **out, int device, const rb_config *cfg)
+{
+    if (!out) return RB_EINVAL;
+    *out = nullptr;
+    rb_ctx *ctx = new (std::nothrow) rb_ctx();
+    if (!ctx) return RB_ENOMEM;
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n == 0 || device < 0 || device >= n) {
+        delete ctx;
+        return RB_EHIP;
+    }
+    ctx->device = device;
+    if (cfg) ctx->cfg = *cfg;
+    if (ctx->cfg.niter < 1) ctx->cfg.niter = 5;
+    if (ctx->cfg.block <= 0) ctx->cfg.block = 256;
+    ctx->K = rb::troe_constants();
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&ctx->own) != hipSuccess ||
+        hipMalloc((void **)&ctx->d_count, sizeof(unsigned long long)) != hipSuccess ||
+        hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess ||
+        hipEventCreate(&ctx->ev_h2d) != hipSuccess || hipEventCreate(&ctx->ev_d2h) != hipSuccess) {
+        rb_finalize(ctx);
+        return RB_EHIP;
+    }
+    ctx->stream = ctx->own;
+    *out = ctx;
+    return RB_OK;
+}
+
+int rb_load_jtable(rb_ctx *ctx, const double *ajl, const double *o3up)
+{
+    if (!ctx) return RB_EINVAL;
+    if (any_null({ajl, o3up})) return fail(ctx, RB_EINVAL, "ajl/o3up is NULL");
+    HIPCHK(hipSetDevice(ctx->device));
+    const size_t n = (size_t)rb::NJTAB * rb::NZ * rb::NSZA * rb::NOZO;
+    if (!ctx->d_ajl) HIPCHK(hipMalloc((void **)&ctx->d_ajl, n * sizeof(double)));
+    if (!ctx->d_o3up) HIPCHK(hipMalloc((void **)&ctx->d_o3up, 80 * sizeof(double)));
+    HIPCHK(hipMemcpyAsync(ctx->d_ajl, ajl, n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->d_o3up, o3up, 80 * sizeof(double), hipMemcpyHostToDevice,
+                          ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    ctx->table = true;
+    return RB_OK;
+}
+
+int rb_set_stream(rb_ctx *ctx, void *stream)
+{
+    if (!ctx) return RB_EINVAL;
+    ctx->stream = stream ? (hipStream_t)stream : ctx->own;
+    return RB_OK;
+}
+
+int rb_chem_step_dev(rb_ctx *ctx, const rb_grid *g, double dt, const double *T, const double *p,
+                     const double *sza, const double *alt, double *qj1, double *hc,
+                     double *surfarea, double *vsed3d, double *parth2osed, double *parthno3sed,
+                     int fuse_conserv)
+{
+    if (!ctx) return RB_EINVAL;
+    int rc = check_grid(ctx, g);
+    if (rc) return rc;
+    if (!ctx->table) return fail(ctx, RB_ENOTABLE, "rb_load_jtable has not been called");
+    if (any_null({T, p, sza, alt, qj1, hc, surfarea, vsed3d, parth2osed, parthno3sed}))
+        return fail(ctx, RB_EINVAL, "a device array is NULL");
+    if (!(dt > 0.) || !std::isfinite(dt)) return fail(ctx, RB_EINVAL, "dt must be positive");
+    HIPCHK(hipSetDevice(ctx->device));
+    const int64_t plane = (int64_t)g->nlon * g->nlat;
+    const int64_t field = plane * g->niv;
+    const int64_t nbox = plane * g->nivbas;
+    if ((rc = ensure(ctx, &ctx->d_col, &ctx->col_cap, (size_t)(3 * nbox)))) return rc;
+
+    rb::StepArgs a{};
+    a.T = T; a.p = p; a.sza = sza; a.alt = alt; a.col = ctx->d_col;
+    a.qj1 = qj1; a.hc = hc; a.surfarea = surfarea;
+    a.vsed = vsed3d; a.p2o = parth2osed; a.pno3 = parthno3sed;
+    a.ajl = ctx->d_ajl; a.o3up = ctx->d_o3up;
+    a.plane = plane; a.field = field; a.nbox = nbox;
+    a.nivbas = g->nivbas; a.nivhau = g->nivhau; a.niter = ctx->cfg.niter;
+    a.fuse_conserv = fuse_conserv ? 1 : 0;
+    a.dt = dt;
+    a.K = ctx->K;
+
+    HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
+    HIPCHK(rb::launch_columns(a, ctx->d_col, ctx->stream));
+    if (ctx->cfg.profile) {
+        if (ctx->prof_used == ctx->prof.size()) {
+            hipEvent_t e0, e1;
+            HIPCHK(hipEventCreate(&e0));
+            HIPCHK(hipEventCreate(&e1));
+            ctx->prof.emplace_back(e0, e1);
+        }
+        auto &pe = ctx->prof[ctx->prof_used++];
+        HIPCHK(hipEventRecord(pe.first, ctx->stream));
+        HIPCHK(rb::launch_chem(a, ctx->cfg.block, ctx->stream));
+        HIPCHK(hipEventRecord(pe.second, ctx->stream));
+    } else {
+        HIPCHK(rb::launch_chem(a, ctx->cfg.block, ctx->stream));
+    }
+    if (fuse_conserv && field > nbox)   // floor of the levels below the chemistry
+        HIPCHK(rb::launch_conserv_floor(qj1, hc, plane, field, g->nivbas, nbox, ctx->stream));
+    HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
+    ctx->timed = true;
+    ctx->stats.boxes = nbox;
+
+    if (ctx->cfg.check_finite) {
+        HIPCHK(hipMemsetAsync(ctx->d_count, 0, sizeof(unsigned long long), ctx->stream));
+        HIPCHK(rb::launch_count_nonfinite(qj1, 43 * field, ctx->d_count, ctx->stream));
+        HIPCHK(rb::launch_count_nonfinite(hc, 15 * field, ctx->d_count, ctx->stream));
+        unsigned long long cnt = 0;
+        HIPCHK(hipMemcpyAsync(&cnt, ctx->d_count, sizeof cnt, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        ctx->stats.nonfinite = (int64_t)cnt;
+        if (cnt) return fail(ctx, RB_ENONFINITE, std::to_string(cnt) + " non-finite outputs");
+    }
+    return RB_OK;
+}
+
+int rb_conserv_floor_dev(rb_ctx *ctx, const rb_grid *g, double *qj1, double *hc)
+{
+    if (!ctx) return RB_EINVAL;
+    int rc = check_grid(ctx, g);
+    if (rc) return rc;
+    if (any_null({qj1, hc})) return fail(ctx, RB_EINVAL, "a device array is NULL");
+    HIPCHK(hipSetDevice(ctx->device));
+    const int64_t plane = (int64_t)g->nlon * g->nlat;
+    HIPCHK(rb::launch_conserv_floor(qj1, hc, plane, plane * g->niv, g->nivbas, 0, ctx->stream));
+    return RB_OK;
+}
+
+int rb_chem_step(rb_ctx *ctx, const rb_grid *g, double dt, const double *T, const double *p,
+                 const double *sza, const double *alt, double *qj1, double *hc, double *surfarea,
+                 double *vsed3d, double *parth2osed, double *parthno3sed)
+{
+    if (!ctx) return RB_EINVAL;
+    int rc = check_grid(ctx, g);
+    if (rc) return rc;
+    if (any_null({T, p, sza, alt, qj1, hc, surfarea, vsed3d, parth2osed, parthno3sed}))
+        return fail(ctx, RB_EINVAL, "a host array is NULL");
+    HIPCHK(hipSetDevice(ctx->device));
+    const size_t field = (size_t)g->nlon * g->nlat * g->niv;
+    const size_t nb = (size_t)g->nlon * g->nlat * g->nivbas;
+    const size_t total = 4 * field + 43 * field + 15 * field + field + 3 * nb;
+    if ((rc = ensure(ctx, &ctx->d_stage, &ctx->stage_cap, total))) return rc;
+    double *d = ctx->d_stage;
+    double *dT = d, *dp = dT + field, *dsza = dp + field, *dalt = dsza + field;
+    double *dq = dalt + field, *dh = dq + 43 * field, *dsurf = dh + 15 * field;
+    double *dv = dsurf + field, *dp2o = dv + nb, *dpno3 = dp2o + nb;
+    const size_t B = sizeof(double);
+    hipStream_t s = ctx->stream;
+    HIPCHK(hipEventRecord(ctx->ev_h2d, s));
+    HIPCHK(hipMemcpyAsync(dT, T, field * B, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(dp, p, field * B, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(dsza, sza, field * B, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(dalt, alt, field * B, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(dq, qj1, 43 * field * B, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(dh, hc, 15 * field * B, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(dsurf, surfarea, field * B, hipMemcpyHostToDevice, s));
+    rc = rb_chem_step_dev(ctx, g, dt, dT, dp, dsza, dalt, dq, dh, dsurf, dv, dp2o, dpno3, 0);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(qj1, dq, 43 * field * B, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(hc, dh, 15 * field * B, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(surfarea, dsurf, field * B, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(vsed3d, dv, nb * B, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(parth2osed, dp2o, nb * B, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(parthno3sed, dpno3, nb * B, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipEventRecord(ctx->ev_d2h, s));
+    HIPCHK(hipStreamSynchronize(s));
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, ctx->ev_h2d, ctx->ev0) == hipSuccess) ctx->stats.h2d_ms = ms;
+    if (hipEventElapsedTime(&ms, ctx->ev1, ctx->ev_d2h) == hipSuccess) ctx->stats.d2h_ms = ms;
+    return RB_OK;
+}
+
+int rb_conserv_floor(rb_ctx *ctx, const rb_grid *g, double *qj1, double *hc)
+{
+    if (!ctx) return RB_EINVAL;
+    int rc = check_grid(ctx, g);
+    if (rc) return rc;
+    if (any_null({qj1, hc})) return fail(ctx, RB_EINVAL, "a host array is NULL");
+    HIPCHK(hipSetDevice(ctx->device));
+    const size_t field = (size_t)g->nlon * g->nlat * g->niv;
+    if ((rc = ensure(ctx, &ctx->d_stage, &ctx->stage_cap, 58 * field))) return rc;
+    double *dq = ctx->d_stage, *dh = dq + 43 * field;
+    const size_t B = sizeof(double);
+    hipStream_t s = ctx->stream;
+    HIPCHK(hipMemcpyAsync(dq, qj1, 43 * field * B, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(dh, hc, 15 * field * B, hipMemcpyHostToDevice, s));
+    if ((rc = rb_conserv_floor_dev(ctx, g, dq, dh))) return rc;
+    HIPCHK(hipMemcpyAsync(qj1, dq, 43 * field * B, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(hc, dh, 15 * field * B, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return RB_OK;
+}
+
+int rb_sync(rb_ctx *ctx)
+{
+    if (!ctx) return RB_EINVAL;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return RB_OK;
+}
+
+int rb_get_stats(rb_ctx *ctx, rb_stats_t *out)
+{
+    if (!ctx || !out) return RB_EINVAL;
+    if (ctx->timed) {
+        HIPCHK(hipEventSynchronize(ctx->ev1));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+        ctx->stats.last_kernel_ms = ms;
+    }
+    double total = 0.;
+    for (size_t i = 0; i < ctx->prof_used; ++i) {
+        float ms = 0.f;
+        HIPCHK(hipEventSynchronize(ctx->prof[i].second));
+        HIPCHK(hipEventElapsedTime(&ms, ctx->prof[i].first, ctx->prof[i].second));
+        total += ms;
+    }
+    ctx->stats.chem_ms_total = total;
+    ctx->stats.chem_launches = (int64_t)ctx->prof_used;
+    ctx->prof_used = 0;
+    *out = ctx->stats;
+    return RB_OK;
+}
+
+const char *rb_last_error(rb_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int rb_finalize(rb_ctx *ctx)
+{
+    if (!ctx) return RB_EINVAL;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->own) (void)hipStreamSynchronize(ctx->own);
+    for (double *b : {ctx->d_ajl, ctx->d_o3up, ctx->d_col, ctx->d_stage})
+        if (b) (void)hipFree(b);
+    if (ctx->d_count) (void)hipFree(ctx->d_count);
+    for (hipEvent_t e : {ctx->ev0, ctx->ev1, ctx->ev_h2d, ctx->ev_d2h})
+        if (e) (void)hipEventDestroy(e);
+    for (auto &pe : ctx->prof) {
+        (void)hipEventDestroy(pe.first);
+        (void)hipEventDestroy(pe.second);
+    }
+    if (ctx->own) (void)hipStreamDestroy(ctx->own);
+    delete ctx;
+    return RB_OK;
+}
+
+}  // extern "C"

@@ -60,7 +60,17 @@ O3_RANGE = (1e-7, 1e-5)
 NOY_RANGE = (1e-10, 2e-8)
 CLY_RANGE = (1e-10, 3.5e-9)
 BRY_RANGE = (1e-12, 2.2e-11)
-RADICAL_RANGE = (1e-15, 1e-10)
+# Short-lived radicals, log-uniform.  The nominal band is 1e-15..1e-10; O(1D), H, CH3
+# and N sit below it and O(3P), Cl, Br are narrowed to their stratospheric
+# magnitudes, because the first sweep uses the incoming values before it
+# recomputes them (F:1358-1370): an O(1D) of 1e-10 (10^6 x physical) makes the
+# CH3 -> CH3O2 -> HOx chain run away to mixing ratios of order 100.
+RADICAL_RANGES = {
+    "o1d": (1e-19, 1e-16), "oh": (1e-14, 1e-11), "cl": (1e-15, 1e-12), "o3p": (1e-12, 1e-9),
+    "ho2": (1e-13, 1e-10), "br": (1e-15, 1e-12), "n": (1e-18, 1e-14), "no3": (1e-14, 1e-10),
+    "h": (1e-18, 1e-14), "ch3": (1e-20, 1e-16), "no2": (1e-15, 1e-10), "no": (1e-15, 1e-10),
+    "clo": (1e-15, 1e-10), "bro": (1e-15, 1e-10), "o3": O3_RANGE,
+}
 
 
 @dataclass
@@ -161,7 +171,7 @@ def draw_species(rng: np.random.Generator, shape: tuple) -> tuple[np.ndarray, np
     # Ox = O3 + O(3P) (F:1400-1408); passive Ox starts equal to Ox.
     o3 = _logu(rng, *O3_RANGE, shape)
     for name in SHORT:
-        h[S[name]] = _logu(rng, *RADICAL_RANGE, shape)
+        h[S[name]] = _logu(rng, *RADICAL_RANGES[name], shape)
     h[S["o3"]] = o3
     h[S["no2"]] = nox * rng.uniform(0.3, 0.9, shape)
     h[S["no"]] = np.maximum(nox - h[S["no2"]], 0.0) * rng.uniform(0.5, 1.0, shape)

@@ -1,0 +1,171 @@
+"""GPU parity: the HIP chemistry step against the reference golden vectors and the
+C restatement, plus size-independent properties on the full 786,240-box grid.
+
+Tolerance (from the specification of the step): 1e-9 relative per species above a
+1e-30 mole-fraction floor; conserv identities to 1e-12.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from conftest import FLOOR, PARITY_TOL, load_fixture, max_rel_by_species, rel_err
+
+pytestmark = pytest.mark.gpu
+
+FIXTURES = ["c0_seed0_16col", "psc_seed7_8col"]
+KEYS = ("qj1_chem", "hc_chem", "qj1", "hc")
+AUX = ("surfarea", "vsed3d", "parth2osed", "parthno3sed")
+
+
+def _report(tag, out, ref):
+    worst = {}
+    for key in KEYS:
+        worst[key] = float(max_rel_by_species(out[key], ref[key]).max())
+    for key in AUX:
+        worst[key] = float(rel_err(out[key], ref[key], floor=1e-300).max())
+    print(f"[parity] {tag}: " + ", ".join(f"{k}={v:.2e}" for k, v in worst.items()), flush=True)
+    return worst
+
+
+@pytest.fixture(scope="module")
+def step(table):
+    from reprobus_amd.chem import ChemistryStep
+
+    s = ChemistryStep(*table, device=0)
+    yield s
+    s.close()
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_gpu_matches_reference_golden(step, name):
+    inp, ref = load_fixture(name)
+    out = step.run_inputs(inp)
+    worst = _report(name, out, ref)
+    for k, v in worst.items():
+        assert v < PARITY_TOL, (k, v)
+
+
+def test_gpu_matches_restatement_c0(step, table):
+    import cport
+    from reprobus_amd import synth
+
+    inp = synth.make_c0(0)
+    out = step.run_inputs(inp)
+    ref = cport.chem_step(inp, *table)
+    worst = _report("c0 full vs restatement", out, ref)
+    for k, v in worst.items():
+        assert v < PARITY_TOL, (k, v)
+
+
+def test_gpu_matches_restatement_c1_subgrid(step, table):
+    import cport
+    from reprobus_amd import synth
+
+    inp = synth.make_c1(seed=1, nlon=24, nlat=13)
+    out = step.run_inputs(inp)
+    ref = cport.chem_step(inp, *table)
+    worst = _report("c1 24x13x60 vs restatement", out, ref)
+    for k, v in worst.items():
+        assert v < PARITY_TOL, (k, v)
+
+
+def _device_inputs(inp):
+    import torch
+
+    dev = torch.device("cuda:0")
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).to(dev)
+    g = inp.grid
+    d = {"T": t(inp.T), "p": t(inp.p), "sza": t(inp.sza), "alt": t(inp.alt), "qj1": t(inp.qj1),
+         "hc": t(inp.hc), "surfarea": torch.zeros(g.shape3, dtype=torch.float64, device=dev)}
+    shb = (g.nivbas, g.nlat, g.nlon)
+    for k in ("vsed3d", "parth2osed", "parthno3sed"):
+        d[k] = torch.zeros(shb, dtype=torch.float64, device=dev)
+    return d
+
+
+def _run_device(step, inp, fuse):
+    import torch
+    from reprobus_amd.chem import make_grid
+
+    g = inp.grid
+    grid = make_grid(g.nlon, g.nlat, g.niv, g.nivbas, g.nivhau)
+    d = _device_inputs(inp)
+    step.set_stream(torch.cuda.current_stream().cuda_stream)
+    step.step_device(grid, inp.dt, d["T"], d["p"], d["sza"], d["alt"], d["qj1"], d["hc"],
+                     d["surfarea"], d["vsed3d"], d["parth2osed"], d["parthno3sed"],
+                     fuse_conserv=fuse)
+    out = {}
+    if not fuse:
+        out["qj1_chem"], out["hc_chem"] = d["qj1"].cpu().numpy(), d["hc"].cpu().numpy()
+        step.conserv_device(grid, d["qj1"], d["hc"])
+    torch.cuda.synchronize()
+    out["qj1"], out["hc"] = d["qj1"].cpu().numpy(), d["hc"].cpu().numpy()
+    for k in ("surfarea", "vsed3d", "parth2osed", "parthno3sed"):
+        out[k] = d[k].cpu().numpy()
+    step.set_stream(None)
+    return out
+
+
+def test_fused_device_path_is_bit_identical_to_host_path(step):
+    from reprobus_amd import synth
+
+    inp = synth.make_c1(seed=2, nlon=16, nlat=7)
+    host = step.run_inputs(inp)
+    fused = _run_device(step, inp, fuse=True)
+    split = _run_device(step, inp, fuse=False)
+    for k in ("qj1", "hc", "surfarea", "vsed3d", "parth2osed", "parthno3sed"):
+        assert np.array_equal(fused[k], host[k]), k
+        assert np.array_equal(split[k], host[k]), k
+    assert np.array_equal(split["qj1_chem"], host["qj1_chem"])
+
+
+def test_full_grid_properties():
+    """786,240 boxes (144 x 91 x 60): finite, floored, conserv identities, night zeros."""
+    import torch
+    from reprobus_amd import jtable, synth
+    from reprobus_amd.chem import ChemistryStep
+    from test_oracle import family_residuals
+
+    inp = synth.make_c1(seed=0)
+    s = ChemistryStep(*jtable.default_table(), device=0)
+    try:
+        out = _run_device(s, inp, fuse=False)
+    finally:
+        s.close()
+    q, h = out["qj1"], out["hc"]
+    assert np.isfinite(q).all() and np.isfinite(h).all()
+    assert (q[:42] >= 1e-30).all()
+    res = family_residuals(out["qj1_chem"], q)
+    print("[props] family residual max:", {k: float(v.max()) for k, v in res.items()}, flush=True)
+    for k, v in res.items():
+        assert v.max() < 1e-12, k
+    night = inp.sza > 95.0
+    assert night.any()
+    for name in ("oh", "ho2", "cl", "no", "o3p", "o1d"):
+        assert np.all(out["hc_chem"][synth.S[name]][night] == 0.0), name
+
+
+def test_error_paths(step):
+    from reprobus_amd import _lib
+    from reprobus_amd._lib import RbError
+    from reprobus_amd.chem import make_grid
+
+    bad = make_grid(4, 1, 10, 20, 14)            # nivbas > niv
+    z = np.zeros(10 * 4)
+    with pytest.raises(RbError) as e:
+        step.chem3d(bad, 900.0, z, z, z, z, np.zeros(43 * 40), np.zeros(15 * 40), z, z, z, z)
+    assert e.value.code == _lib.RB_EINVAL
+    L = _lib.load()
+    ctx = C.c_void_p()
+    assert L.rb_init(C.byref(ctx), 0, None) == 0
+    try:
+        g = make_grid(4, 1, 10, 10, 14)
+        one = np.ones(40)
+        rc = L.rb_chem_step(ctx, C.byref(g), 900.0, *([one.ctypes.data] * 4),
+                            np.ones(43 * 40).ctypes.data, np.ones(15 * 40).ctypes.data,
+                            *([one.ctypes.data] * 4))
+        assert rc == _lib.RB_ENOTABLE
+        assert b"rb_load_jtable" in L.rb_last_error(ctx)
+    finally:
+        L.rb_finalize(ctx)

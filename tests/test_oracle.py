@@ -1,0 +1,223 @@
+"""CPU tests of the oracle (C restatement) and the host-side pieces.
+
+The restatement is pinned (1) against golden vectors of the flang-built
+reference and (2) against closed forms and invariants chosen so that a
+dropped term, a wrong sign/index or a transposed operand fails one of them.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from conftest import FLOOR, load_fixture, max_rel_by_species, rel_err
+
+cport = pytest.importorskip("cport")
+
+FIXTURES = ["c0_seed0_16col", "psc_seed7_8col"]
+
+
+@pytest.fixture(scope="module")
+def lib():
+    return cport.lib()
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_restatement_matches_reference_golden(name, table):
+    inp, ref = load_fixture(name)
+    out = cport.chem_step(inp, *table, nthreads=4)
+    for key in ("qj1_chem", "hc_chem", "qj1", "hc"):
+        worst = max_rel_by_species(out[key], ref[key]).max()
+        assert worst < 1e-11, (key, worst)
+    for key in ("surfarea", "vsed3d", "parth2osed", "parthno3sed"):
+        assert rel_err(out[key], ref[key], floor=1e-300).max() < 1e-11, key
+
+
+def test_psc_fixture_exercises_all_aerosol_branches(table):
+    """The PSC fixture must hold NAT, ice and ternary-liquid boxes (else it pins little)."""
+    inp, _ = load_fixture("psc_seed7_8col")
+    L = cport.lib()
+    flags = np.zeros(3, dtype=np.int32)
+    counts = np.zeros(3, dtype=int)
+    ternary = 0
+    g = inp.grid
+    for k in range(g.nivhau - 1, g.nivbas):
+        for c in range(g.nlon):
+            t, p = inp.T[k, 0, c], inp.p[k, 0, c]
+            hnm = p / (t * 1.38e-19)
+            ck = np.concatenate([[0.0], inp.qj1[:, k, 0, c] * hnm])
+            het, part, surf, vsed = np.zeros(9), np.zeros(5), np.zeros(1), np.zeros(1)
+            L.rbo_aerosol_box(t, p, hnm, ck, het, part, surf, vsed, flags)
+            counts += flags
+            ternary += int(flags[2] == 1 and t < 215.0)
+    assert counts[0] > 0 and counts[1] > 0 and ternary > 0, (counts, ternary)
+
+
+def test_troe_limits(lib):
+    """Falloff: low-pressure limit k0*M, high-pressure limit kinf, and 0.3*kinf at k0*M = kinf."""
+    a0, a1, n0, n1, t = 1e-31 * 300 ** 3.0, 1e-11 * 300 ** 1.0, -3.0, -1.0, 230.0
+    k0 = a0 * t ** n0
+    kinf = a1 * t ** n1
+    lo = lib.rbo_troe(a0, a1, n0, n1, t, 1e3)
+    assert lo == pytest.approx(k0 * 1e3, rel=1e-12)
+    hi = lib.rbo_troe(a0, a1, n0, n1, t, 1e40)
+    assert hi == pytest.approx(kinf, rel=1e-6)
+    m_mid = kinf / k0
+    assert lib.rbo_troe(a0, a1, n0, n1, t, m_mid) == pytest.approx(0.3 * kinf, rel=1e-12)
+
+
+def test_troe_constants_are_prefactors(lib):
+    """COMMON /constant/ holds k*300**n: dividing by 300**n recovers the JPL k300 values."""
+    out = np.zeros(18)
+    lib.rbo_troe_constants(out)
+    k300 = [4.4e-32, 7.5e-11, 2.0e-30, 1.4e-12, 1.8e-30, 2.8e-11, 2.0e-31, 2.9e-12,
+            1.8e-31, 1.5e-11, 1.8e-31, 1.0e-10, 1.6e-32, 3.0e-12, 5.2e-31, 6.9e-12, 4.0e-31, 1.2e-12]
+    expo = [1.3, -0.2, 4.4, 0.7, 3.0, 0.0, 3.4, 1.1, 3.4, 1.9, 2.0, 1.0, 4.5, 2.0, 3.2, 2.9, 3.6, -1.1]
+    for v, k, e in zip(out, k300, expo):
+        assert v / 300.0 ** e == pytest.approx(k, rel=1e-13)
+
+
+def test_arrhenius_rates_at_300K(lib):
+    """Pure Arrhenius rates reduce to A*exp(-E/300) -- checked on a ratio that needs the sign of E."""
+    out = np.zeros(200)
+    n = lib.rbo_rates(300.0, 2.4e19, 1000.0, out)
+    names = [lib.rbo_rate_name(i).decode() for i in range(n)]
+    r = dict(zip(names, out[:n]))
+    # ratio of the same reaction at two temperatures fixes E (sign and magnitude)
+    out2 = np.zeros(200)
+    lib.rbo_rates(200.0, 2.4e19, 1000.0, out2)
+    r2 = dict(zip(names, out2[:n]))
+    assert r2["b4"] / r["b4"] == pytest.approx(math.exp(-1500 / 200 + 1500 / 300), rel=1e-12)
+    assert r2["a5"] / r["a5"] == pytest.approx(math.exp(180 / 200 - 180 / 300), rel=1e-12)
+    assert r["c17a"] / r["c17b"] == pytest.approx(0.7 / 0.3, rel=1e-14)
+    # equilibrium-constant decompositions: b24 * Keq * M = b23
+    assert r["b24"] * 2.1e-27 * math.exp(10900 / 300) * 2.4e19 == pytest.approx(r["b23"], rel=1e-13)
+    assert r["d61"] * 1.72e-27 * math.exp(8649 / 300) * 2.4e19 == pytest.approx(r["d60"], rel=1e-13)
+    # hk2 ~ 6e-34 (300/T)^2.4 M
+    assert r2["hk2"] == pytest.approx(6.0e-34 * 1.5 ** 2.4 * 2.4e19, rel=1e-13)
+    # 99 rate planes plus the 4 chem3d scalars a3et, a23a/b/c = the 103 thermal rates
+    assert n == 99
+
+
+def test_trilinear_reproduces_affine_table(lib):
+    """An affine table in (z, sza, ratio) must be interpolated exactly inside the cell grid."""
+    from reprobus_amd import jtable
+
+    z = np.arange(101.0)[None, :, None, None]
+    s = jtable.DSZA[None, None, :, None]
+    r = jtable.XV3[None, None, None, :]
+    coef = np.arange(1, 41, dtype=float)[:, None, None, None]
+    ajl = 1e-6 * coef * (3.0 + 0.5 * z + 0.01 * s + 2.0 * r)
+    o3up = np.full(80, 1e18)
+    ajl_f = cport.fortran_ajl(ajl)
+    tj = np.zeros(40)
+    rng = np.random.default_rng(3)
+    for _ in range(200):
+        alt, sza, ratio = rng.uniform(0.5, 78.0), rng.uniform(0.0, 95.0), rng.uniform(0.41, 1.59)
+        lib.rbo_phot_box(ajl_f, o3up, alt, sza, ratio * 1e18, tj)
+        expect = 1e-6 * np.arange(1, 41) * (3.0 + 0.5 * alt + 0.01 * sza + 2.0 * ratio)
+        np.testing.assert_allclose(tj, expect, rtol=1e-13)
+    lib.rbo_phot_box(ajl_f, o3up, 30.0, 95.5, 1e18, tj)
+    assert np.all(tj == 0.0)
+
+
+def test_jno_top_of_atmosphere_closed_form(lib):
+    """With zero overhead columns J(NO) = sum_b dl_b I0_b P sum_i (w1 s1 + w2 s2)."""
+    from math import isclose
+
+    hnm = 1e16
+    prob = 1.65e9 / (5.1e7 + 1.65e9 + 1.5e-9 * hnm * 0.78)
+    dl, i0 = [2.3, 1.5, 1.5], [3.98e11, 2.21e11, 2.30e11]
+    w1 = [[0, 5.12e-2, 1.36e-1, 1.65e-1, 1.41e-1, 4.5e-2], [0, 0, 1.93e-3, 9.73e-2, 9.75e-2, 3.48e-2],
+          [4.5e-2, 1.8e-1, 2.25e-1, 2.25e-1, 1.8e-1, 4.5e-2]]
+    s1 = [[0, 1.32e-18, 6.35e-19, 7.09e-19, 2.18e-19, 4.67e-19], [0, 0, 3.05e-21, 5.76e-19, 2.29e-18, 2.21e-18],
+          [1.8e-18, 1.5e-18, 5.01e-19, 7.2e-20, 6.72e-20, 1.49e-21]]
+    w2 = [[0, 5.68e-3, 1.52e-2, 1.83e-2, 1.57e-2, 5e-3], [0, 0, 2.14e-4, 1.08e-2, 1.08e-2, 3.86e-3],
+          [5e-3, 2e-2, 2.5e-2, 2.5e-2, 2e-2, 5e-3]]
+    s2 = [[0, 4.41e-17, 4.45e-17, 4.5e-17, 2.94e-17, 4.35e-17], [0, 0, 3.2e-21, 5.71e-17, 9.09e-17, 6e-17],
+          [1.4e-16, 1.52e-16, 7e-17, 2.83e-17, 2.73e-17, 6.57e-18]]
+    expect = sum(dl[b] * i0[b] * prob * sum(w1[b][i] * s1[b][i] + w2[b][i] * s2[b][i] for i in range(6))
+                 for b in range(3))
+    got = lib.rbo_jno_box(0.0, 0.0, 0.0, hnm, 30.0)
+    assert isclose(got, expect, rel_tol=1e-13)
+    # the O2 column only attenuates: J must fall monotonically with it; night -> 0
+    assert lib.rbo_jno_box(1e20, 0.0, 0.0, hnm, 30.0) < got
+    assert lib.rbo_jno_box(0.0, 0.0, 0.0, hnm, 89.5) == 0.0
+
+
+def test_column_prefix_sums(lib):
+    """total/jno columns: constant O3 mixing ratio q gives o3t(k) = top + zcmt*q*(p_k - p_1)*100."""
+    n = 10
+    p = np.linspace(1.0, 50.0, n)
+    T = np.full(n, 230.0)
+    q = 4e-6
+    o3 = np.full(n, q)
+    no = np.full(n, 2e-9)
+    o3t, co2, cno = np.zeros(n), np.zeros(n), np.zeros(n)
+    lib.rbo_column(n, p, T, o3, no, 1, o3t, co2, cno)
+    hnm0 = p[0] / (230.0 * 1.38e-19)
+    np.testing.assert_allclose(o3t, q * hnm0 * 6.8e5 + 2.12e20 * q * (p - p[0]) * 100, rtol=1e-12)
+    zc = 287.0 / (1.38e-23 * 9.81)
+    np.testing.assert_allclose(co2, 6.8e5 * 0.2095 * hnm0 + zc * 0.2095 * (p - p[0]) * 100 * 1e-4, rtol=1e-12)
+    np.testing.assert_allclose(cno, 6.8e5 * 2e-9 * hnm0 + zc * 2e-9 * (p - p[0]) * 100 * 1e-4, rtol=1e-12)
+
+
+def test_density_limits(lib):
+    """Pure water near 1 g/cm3 at 273 K; 50 wt% H2SO4 near 1.4 g/cm3 (Carslaw-type fit range)."""
+    assert lib.rbo_density(0.0, 0.0, 273.0) == pytest.approx(1.0, abs=0.01)
+    assert lib.rbo_density(0.5, 0.0, 220.0) == pytest.approx(1.42, abs=0.05)
+    assert lib.rbo_density(0.3, 0.2, 200.0) > lib.rbo_density(0.0, 0.0, 200.0)
+
+
+def family_residuals(q_chem, q):
+    """Relative residuals of the identities conserv (F:2541-2598) establishes.
+
+    conserv rescales Cly members, then NOy members, then Bry members, in place:
+    Bry holds exactly; Cly and NOy hold with the BrCl / BrONO2 values from before
+    the Bry rescaling (taken from the post-chem3d state ``q_chem``).
+    """
+    from reprobus_amd import synth
+
+    L = synth.L
+    brcl0, brono20 = q_chem[L["brcl"]], q_chem[L["brono2"]]
+    cly = (q[L["oclo"]] + q[L["hcl"]] + q[L["clono2"]] + q[L["hocl"]] + 2 * q[L["cl2"]]
+           + q[L["clno2"]] + q[L["clox"]] + 2 * q[L["cl2o2"]] + brcl0)
+    bry = q[L["hbr"]] + q[L["brono2"]] + q[L["brox"]] + q[L["hobr"]] + q[L["brcl"]]
+    noy = (q[L["hno3"]] + 2 * q[L["n2o5"]] + q[L["nox"]] + q[L["ho2no2"]] + q[L["clono2"]]
+           + q[L["clno2"]] + brono20)
+    members = ["oclo", "hcl", "clono2", "hocl", "cl2", "clno2", "clox", "cl2o2", "hno3", "n2o5",
+               "nox", "ho2no2", "hbr", "brono2", "brox", "hobr", "brcl"]
+    # a negative rescale factor leaves members at the 1e-30 floor: no identity there
+    clipped = np.any([q[L[m]] <= 1e-30 for m in members], axis=0)
+    res = {"cly": np.abs(cly / q[L["cly"]] - 1), "bry": np.abs(bry / q[L["bry"]] - 1),
+           "noy": np.abs(noy / q[L["noy"]] - 1)}
+    return {k: np.where(clipped, 0.0, v) for k, v in res.items()}
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_conserv_identities_on_reference_output(name):
+    """The reference output satisfies the conserv identities to 1e-12."""
+    inp, ref = load_fixture(name)
+    for fam, r in family_residuals(ref["qj1_chem"], ref["qj1"]).items():
+        assert r.max() < 1e-12, (fam, r.max())
+
+
+def test_night_branch_and_floor(table):
+    inp, ref = load_fixture("c0_seed0_16col")
+    from reprobus_amd import synth
+
+    night = inp.sza > 95.0
+    assert night.any() and (~night).any()
+    h = ref["hc_chem"]
+    for s in ("o1d", "oh", "cl", "o3p", "ho2", "no", "br", "n", "h", "ch3"):
+        assert np.all(h[synth.S[s]][night] == 0.0), s
+    assert np.all(ref["qj1"][:42] >= 1e-30)
+    assert np.all(np.isfinite(ref["qj1"]))
+
+
+def test_restatement_threads_agree(table):
+    """OpenMP column split must not change any bit."""
+    inp, _ = load_fixture("c0_seed0_16col")
+    a = cport.chem_step(inp, *table, nthreads=1)
+    b = cport.chem_step(inp, *table, nthreads=4)
+    for k in a:
+        assert np.array_equal(a[k], b[k]), k
