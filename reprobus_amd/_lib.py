@@ -55,6 +55,16 @@ def load() -> C.CDLL:
     if not os.path.exists(LIB_PATH):
         raise RuntimeError(f"{LIB_PATH} is not built; run __graft_entry__.build() "
                            "or `make -C reprobus_amd/csrc`")
+    # torch ships its own HIP runtime with the same soname (libamdhip64.so.7).  Loading
+    # it first makes this library bind to that runtime, so the process holds one HIP
+    # runtime and device pointers from torch tensors are valid in our launches.
+    try:
+        import torch
+
+        if torch.cuda.is_available():
+            torch.cuda.init()
+    except ImportError:
+        pass
     L = C.CDLL(LIB_PATH)
     P = C.c_void_p
     d = C.c_double

@@ -14,6 +14,7 @@
 // Expressions keep the reference's evaluation order (the library is built with
 // -ffp-contract=off) so the step reproduces the reference to a few ulps per box.
 #include "chem_device.h"
+#include "crmath.h"
 
 #include <math.h>
 
@@ -307,11 +308,15 @@ __device__ double ternary_density(double ws, double wn, double t)
 
 // analytic (NAT F:2795-2857, ice F:2861-2913, liquid F:2921-3228, area F:3240, HNO3
 // partition F:3252, settling F:3264-3348) then hetero (F:3436-4023) for one box.
+// Up to the liquid composition (ws, wn, pn) the elementary functions are the correctly
+// rounded ones of crmath.h: the ternary solution is ill-conditioned near its binary
+// limit and amplifies last-bit differences there (see crmath.h).  Area, solubilities,
+// settling and the uptake rates downstream are well conditioned and use the library.
 __device__ void psc_and_het(double temp, double ptot, double hnm, const double *ck, Het &o)
 {
     constexpr double R = 8.205e-5, NDROP = 10., NNAT_SMALL = 1., RNATS = 0.5e-4,
                      RNATL = 6.5e-4, RICE = 10.e-4, AMU = 1.66e-24;
-    const double pi = 2. * asin(1.0);
+    const double pi = 3.141592653589793;   // 2*asin(1.0) (F:2769), exactly
     double nnats = 0., nnatl = 0., anat = 0., nice = 0., aice = 0., vnat = 0., vnatl = 0.;
     double pno3l = 1., pno3s = 1.;
     int lnat = 0, lice = 0, condliq;
@@ -330,19 +335,19 @@ __device__ void psc_and_het(double temp, double ptot, double hnm, const double *
     {
         const double zmt = -2.7836 - 0.00088 * tm;
         const double zbt = 38.9855 - 11397.0 / tm + 0.009179 * tm;
-        const double eq = pow(10.0, zmt * log10(pw * 1013. * 0.75) + zbt);
+        const double eq = crm::pow_cr(10.0, zmt * crm::log10_cr(pw * 1013. * 0.75) + zbt);
         if (pn0 * 1013. * 0.75 / eq >= 1.) {
             lnat = 1;
             pn = eq / (0.75 * 1013.);
             const double hno3s = fmax((pn0 - pn) * 1013. * hnm / ptot, 0.);
             vnat = hno3s * AMU * 117. / 1.35;
-            const double vsmall = NNAT_SMALL * (4. / 3. * pi * pow(RNATS, 3.));
+            const double vsmall = NNAT_SMALL * (4. / 3. * pi * crm::pow_cr(RNATS, 3.));
             if (vnat > vsmall) {
                 nnats = NNAT_SMALL;
-                nnatl = (vnat - vsmall) / (4. / 3. * pi * pow(RNATL, 3.));
+                nnatl = (vnat - vsmall) / (4. / 3. * pi * crm::pow_cr(RNATL, 3.));
                 vnatl = vnat - vsmall;
             } else {
-                nnats = vnat / (4. / 3. * pi * pow(RNATS, 3.));
+                nnats = vnat / (4. / 3. * pi * crm::pow_cr(RNATS, 3.));
                 nnatl = 0.;
                 vnatl = 0.;
             }
@@ -352,20 +357,20 @@ __device__ void psc_and_het(double temp, double ptot, double hnm, const double *
         }
     }
     // ice
-    const double tice = 2668.70 / (10.4310 - (log(pw) + log(760.0)) / log(10.0));
+    const double tice = 2668.70 / (10.4310 - (crm::log_cr(pw) + crm::log_cr(760.0)) / crm::log_cr(10.0));
     if (tm <= tice) {
         lice = 1;
         lnat = 0;
         anat = 0.;
-        const double ph2o = pow(10.0, -2668.7 / tm + 10.4310) / 760.0;
+        const double ph2o = crm::pow_cr(10.0, -2668.7 / tm + 10.4310) / 760.0;
         const double h2os = fmax(pw - ph2o, 0.) * 1013. * hnm / ptot;
         const double vice = h2os * AMU * 18. / 0.928;
-        nice = vice / (4. / 3. * pi * pow(RICE, 3.));
+        nice = vice / (4. / 3. * pi * crm::pow_cr(RICE, 3.));
         aice = nice * 4 * pi * (RICE * RICE);
         o.ph2o = fmin(fmax(1.0 - fmax(pw - ph2o, 0.) / pw, 0.), 1.);
         const double zmt = -2.7836 - 0.00088 * tm;
         const double zbt = 38.9855 - 11397.0 / tm + 0.009179 * tm;
-        const double eq = pow(10.0, zmt * log10(pw * o.ph2o * 1013. * 0.75) + zbt);
+        const double eq = crm::pow_cr(10.0, zmt * crm::log10_cr(pw * o.ph2o * 1013. * 0.75) + zbt);
         pn = eq / (0.75 * 1013.);
         pno3s = fmin(fmax(1.0 - (pn0 - pn) / pn0, 0.), 1.);
         o.pno3sl = pno3s;
@@ -381,7 +386,7 @@ __device__ void psc_and_het(double temp, double ptot, double hnm, const double *
     {
         const double a = 51.81 + -15732.0 / t;
         const double b = -21.661 + 2724.2 / t;
-        const double c = 47.004 + -6969.0 / t + -4.6183 * log(t) - log(pw);
+        const double c = 47.004 + -6969.0 / t + -4.6183 * crm::log_cr(t) - crm::log_cr(pw);
         const double det = (b * b) - 4. * a * c;
         if (det > 0.) {
             const double xsb = (-b - sqrt(det)) / (2. * a);
@@ -397,8 +402,8 @@ __device__ void psc_and_het(double temp, double ptot, double hnm, const double *
         // H2SO4 saturation pressure (Ayers et al. 1980, Giauque et al. 1960)
         const double wts = fmax(41., ws * 100.);
         const double mu = 4.184 * (1.514e4 - 286. * (wts - 40.) + 1.08 * sq(wts - 40.) -
-                                   3941. / pow(wts - 40., 0.1));
-        const double ph2so4 = exp(-10156. / t + 16.2590 - mu / (8.314 * t));
+                                   3941. / crm::pow_cr(wts - 40., 0.1));
+        const double ph2so4 = crm::exp_cr(-10156. / t + 16.2590 - mu / (8.314 * t));
         if (qh2so4 * ptot / 1013. <= ph2so4) {
             ms = 0.;
             ws = 0.;
@@ -411,19 +416,19 @@ __device__ void psc_and_het(double temp, double ptot, double hnm, const double *
         constexpr double qs[10] = {14.4700, 0.0638795, -3.29597, 1.778224, -0.223244,
                                    0.0086486, 0.536695, -0.335164, 0.0265153, 0.0157550};
         constexpr double kn[7] = {-39.136, 6358.4, 83.29, -17650.0, 198.53, -11948, -28.469};
-        const double tt = R * t * ns, tr = 1.0e4 / t - 43.4782608, pr = log(pw) + 18.4;
+        const double tt = R * t * ns, tr = 1.0e4 / t - 43.4782608, pr = crm::log_cr(pw) + 18.4;
         const double tr2 = tr * tr, tr3 = tr * tr * tr, pr2 = pr * pr, pr3 = pr * pr * pr;
-        const double hsb = exp(qs[0] + qs[1] * tr2 +
+        const double hsb = crm::exp_cr(qs[0] + qs[1] * tr2 +
                                (qs[2] + qs[3] * tr + qs[4] * tr2 + qs[5] * tr3) * pr +
                                (qs[6] + qs[7] * tr + qs[8] * tr2) * pr2 + qs[9] * tr * pr3);
         const double kk = kn[0] + kn[1] / t;
         const double xnb = 1.0 / (2.0 * (kn[2] + kn[3] / t)) *
                            (-kn[0] - kn[1] / t -
-                            pow(kk * kk - 4.0 * (kn[2] + kn[3] / t) *
-                                              (kn[4] + kn[5] / t + kn[6] * log(t) - log(pw)),
+                            crm::pow_cr(kk * kk - 4.0 * (kn[2] + kn[3] / t) *
+                                              (kn[4] + kn[5] / t + kn[6] * crm::log_cr(t) - crm::log_cr(pw)),
                                 0.5));
         const double mnb = 55.51 * xnb / (1.0 - xnb);
-        const double hnb = exp(qn[0] + qn[1] * tr2 +
+        const double hnb = crm::exp_cr(qn[0] + qn[1] * tr2 +
                                (qn[2] + qn[3] * tr + qn[4] * tr2 + qn[5] * tr3) * pr +
                                (qn[6] + qn[7] * tr + qn[8] * tr2) * pr2 + qn[9] * tr * pr3);
         const double A = (tt * hnb * (mnb * mnb) - tt * hsb * mnb * msb - 2.0 * (mnb * mnb) * msb +
@@ -435,9 +440,9 @@ __device__ void psc_and_het(double temp, double ptot, double hnm, const double *
         const double C = (tt * hnb * mnb * (msb * msb)) / (mnb - msb);
         const double u = (A * A) - 3.0 * B;
         const double v = -2.0 * (A * A * A) + 9.0 * A * B - 27.0 * C;
-        double phi = atan(sqrt(fmax(4.0 * (u * u * u) - (v * v), 0.)) / v);
+        double phi = crm::atan_cr(sqrt(fmax(4.0 * (u * u * u) - (v * v), 0.)) / v);
         if (phi < 0.) phi = phi + pi;
-        const double msf = fmax(-1.0 / 3.0 * (A + 2.0 * sqrt((A * A) - 3.0 * B) * cos((pi + phi) / 3.0)), 0.);
+        const double msf = fmax(-1.0 / 3.0 * (A + 2.0 * sqrt((A * A) - 3.0 * B) * crm::cos_cr((pi + phi) / 3.0)), 0.);
         const double mnf = fmax(mnb * (1.0 - msf / msb), 0.);
         pn = mnf / (hnb * mnf / (mnf + msf) + hsb * msf / (mnf + msf));
         ms = msf;
