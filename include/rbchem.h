@@ -10,8 +10,9 @@
  * Array layout is the model's Fortran layout: a field is (nlon, nlat, niv)
  * with lon fastest; species fields are stacked last (qj1: 43 fields, hc: 15).
  * T and p are the `trajtem` / `trajpre` slice of the current sub-step
- * (F:802-806), i.e. one (nlon, nlat, niv) field each.  SZA is given per box in
- * degrees (the model computes it in `zenith`, F:845; pass `sza3d`).
+ * (F:802-806); T, p, SZA and alt are read on levels 1..nivbas only, so any
+ * (nlon, nlat, n >= nivbas) field works.  SZA is given per box in degrees (the
+ * model computes it in `zenith`, F:845; pass `sza3d`).
  *
  * One context owns one device, one HIP stream and the device copies of the
  * photolysis table and work buffers.  Calls on one context are serialised on its

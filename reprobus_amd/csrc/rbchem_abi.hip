@@ -232,19 +232,20 @@ int rb_chem_step(rb_ctx *ctx, const rb_grid *g, double dt, const double *T, cons
     HIPCHK(hipSetDevice(ctx->device));
     const size_t field = (size_t)g->nlon * g->nlat * g->niv;
     const size_t nb = (size_t)g->nlon * g->nlat * g->nivbas;
-    const size_t total = 4 * field + 43 * field + 15 * field + field + 3 * nb;
+    // T, p, SZA and alt are read on the chemistry levels only (nb boxes)
+    const size_t total = 4 * nb + 43 * field + 15 * field + field + 3 * nb;
     if ((rc = ensure(ctx, &ctx->d_stage, &ctx->stage_cap, total))) return rc;
     double *d = ctx->d_stage;
-    double *dT = d, *dp = dT + field, *dsza = dp + field, *dalt = dsza + field;
-    double *dq = dalt + field, *dh = dq + 43 * field, *dsurf = dh + 15 * field;
+    double *dT = d, *dp = dT + nb, *dsza = dp + nb, *dalt = dsza + nb;
+    double *dq = dalt + nb, *dh = dq + 43 * field, *dsurf = dh + 15 * field;
     double *dv = dsurf + field, *dp2o = dv + nb, *dpno3 = dp2o + nb;
     const size_t B = sizeof(double);
     hipStream_t s = ctx->stream;
     HIPCHK(hipEventRecord(ctx->ev_h2d, s));
-    HIPCHK(hipMemcpyAsync(dT, T, field * B, hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(dp, p, field * B, hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(dsza, sza, field * B, hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(dalt, alt, field * B, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(dT, T, nb * B, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(dp, p, nb * B, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(dsza, sza, nb * B, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(dalt, alt, nb * B, hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(dq, qj1, 43 * field * B, hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(dh, hc, 15 * field * B, hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(dsurf, surfarea, field * B, hipMemcpyHostToDevice, s));
