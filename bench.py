@@ -38,6 +38,9 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-procs", type=int, default=16)
     ap.add_argument("--cpu-nrep", type=int, default=8)
+    ap.add_argument("--calib-copy-mb", type=int, default=0,
+                    help="after the timed region, one device copy of this many MB (a kernel of "
+                         "known traffic, to scale profiler byte counters)")
     return ap.parse_args()
 
 
@@ -113,6 +116,13 @@ def main():
     st = step.stats()
     finite = bool(torch.isfinite(q).all().item() and torch.isfinite(h).all().item())
     step.close()
+    if args.calib_copy_mb > 0:
+        n = args.calib_copy_mb * (1 << 20) // 8
+        src = torch.ones(n, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     nbox = g.nbox
     ms_per_step = ms_total / args.steps

@@ -9,6 +9,8 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "librbchem_oracle.so")
+# the restatement with correctly rounded math in the PSC block (built with -DRBO_CR_PSC)
+LIB_CR = os.path.join(HERE, "librbchem_oracle_cr.so")
 
 
 class _Grid(C.Structure):
@@ -17,18 +19,25 @@ class _Grid(C.Structure):
 
 
 _lib = None
+_libs = {}
 
 
 def build() -> None:
     subprocess.run(["make", "-s", "-C", HERE, "restatement"], check=True)
 
 
-def lib():
+def lib(cr_psc: bool = False):
+    """The restatement library; cr_psc selects the correctly rounded PSC variant."""
     global _lib
-    if _lib is None:
-        if not os.path.exists(LIB):
+    path = LIB_CR if cr_psc else LIB
+    if path == LIB and _lib is not None:
+        return _lib
+    if path in _libs:
+        return _libs[path]
+    if True:
+        if not os.path.exists(path):
             build()
-        L = C.CDLL(LIB)
+        L = C.CDLL(path)
         dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
         L.rbo_chem_step.argtypes = [C.POINTER(_Grid), C.c_double, dp, dp, dp, dp, dp, dp, dp, dp,
                                     dp, dp, dp, dp, C.c_int]
@@ -49,11 +58,14 @@ def lib():
         L.rbo_density.restype = C.c_double
         L.rbo_aerosol_box.argtypes = [C.c_double, C.c_double, C.c_double, dp, dp, dp, dp, dp,
                                       np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")]
-        _lib = L
-    return _lib
+        _libs[path] = L
+        if path == LIB:
+            _lib = L
+    return L
 
 
-def chem_step(inp, ajl, o3up, niter: int = 5, nthreads: int = 0, conserv: bool = True) -> dict:
+def chem_step(inp, ajl, o3up, niter: int = 5, nthreads: int = 0, conserv: bool = True,
+              cr_psc: bool = False) -> dict:
     """Run the restatement on a synth.StepInputs; returns outputs like refrun."""
     g = inp.grid
     G = _Grid(g.nlon, g.nlat, g.niv, g.nivbas, g.nivhau, niter)
@@ -64,14 +76,14 @@ def chem_step(inp, ajl, o3up, niter: int = 5, nthreads: int = 0, conserv: bool =
     vsed, p2o, pno3 = np.zeros(shb), np.zeros(shb), np.zeros(shb)
     c = lambda a: np.ascontiguousarray(a, dtype=np.float64)
     ajl_f = np.ascontiguousarray(np.asarray(ajl, dtype=np.float64).transpose(3, 2, 1, 0))
-    rc = lib().rbo_chem_step(C.byref(G), float(inp.dt), c(inp.T), c(inp.p), c(inp.sza), c(inp.alt),
+    rc = lib(cr_psc).rbo_chem_step(C.byref(G), float(inp.dt), c(inp.T), c(inp.p), c(inp.sza), c(inp.alt),
                              ajl_f, c(o3up), q, h, surf, vsed, p2o, pno3, nthreads)
     if rc != 0:
         raise RuntimeError(f"rbo_chem_step failed: {rc}")
     out = {"qj1_chem": q.copy(), "hc_chem": h.copy(), "surfarea": surf, "vsed3d": vsed,
            "parth2osed": p2o, "parthno3sed": pno3}
     if conserv:
-        lib().rbo_conserv_floor(C.byref(G), q, h, nthreads)
+        lib(cr_psc).rbo_conserv_floor(C.byref(G), q, h, nthreads)
         out["qj1"], out["hc"] = q, h
     return out
 

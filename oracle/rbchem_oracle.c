@@ -402,6 +402,45 @@ double rbo_density(double ws, double wn, double t)
 }
 
 /* ---------------------------------------------------------------- analytic + hetero */
+/* Elementary functions of the PSC block up to the liquid composition (NAT, ice,
+ * binary and ternary liquid).  Default: the C library, as the reference.  Built
+ * with -DRBO_CR_PSC: binary128 evaluation rounded once (correctly rounded), with
+ * log10 in the fdlibm form the C library uses -- the variant that isolates the
+ * reference's library rounding from the algorithm (the ternary solve amplifies
+ * last-bit differences by up to 1e10). */
+#ifdef RBO_CR_PSC
+#include <quadmath.h>
+static double psc_log(double x) { return (double)logq((__float128)x); }
+static double psc_exp(double x) { return (double)expq((__float128)x); }
+static double psc_pow(double x, double y) { return (double)powq((__float128)x, (__float128)y); }
+static double psc_atan(double x) { return (double)atanq((__float128)x); }
+static double psc_cos(double x) { return (double)cosq((__float128)x); }
+static double psc_log10(double x)
+{
+    const double ivln10 = 4.34294481903251816668e-01, log10_2hi = 3.01029995663611771306e-01,
+                 log10_2lo = 3.69423907715893078616e-13;
+    union { double d; unsigned long long u; } v;
+    int hx, k, i;
+    double y, z;
+    v.d = x;
+    hx = (int)(v.u >> 32);
+    if (!(x > 0.) || hx < 0x00100000 || hx >= 0x7ff00000) return log10(x);
+    k = (hx >> 20) - 1023;
+    i = (int)(((unsigned)k & 0x80000000u) >> 31);
+    hx = (hx & 0x000fffff) | ((0x3ff - i) << 20);
+    y = (double)(k + i);
+    v.u = ((unsigned long long)(unsigned)hx << 32) | (v.u & 0xffffffffull);
+    z = y * log10_2lo + ivln10 * psc_log(v.d);
+    return z + y * log10_2hi;
+}
+#else
+#define psc_log log
+#define psc_exp exp
+#define psc_pow pow
+#define psc_atan atan
+#define psc_cos cos
+#define psc_log10 log10
+#endif
 typedef struct {
     double het[10];               /* het[1..9] */
     double parthno3, parthno3sl, parth2o, parthcl;
@@ -468,7 +507,7 @@ static void aerosol_box(double temp, double ptot, double hnm, const double *ck, 
         double pn0t = pn0 * 1013. * 0.75;
         double zmt = -2.7836 - 0.00088 * tt;
         double zbt = 38.9855 - 11397.0 / tt + 0.009179 * tt;
-        double zhno3eq = pow(10.0, zmt * log10(pw * 1013. * 0.75) + zbt);
+        double zhno3eq = psc_pow(10.0, zmt * psc_log10(pw * 1013. * 0.75) + zbt);
         if (pn0t / zhno3eq >= 1.) {
             double hno3s, vnat_small_max;
             lnat = 1;
@@ -476,13 +515,13 @@ static void aerosol_box(double temp, double ptot, double hnm, const double *ck, 
             hno3s = (pn0 - pn) * 1013. * hnm / ptot;
             hno3s = fmax(hno3s, 0.);
             vnat = hno3s * amu * 117. / 1.35;
-            vnat_small_max = nnat_small_max * (4. / 3. * pi * pow(rnats, 3.));
+            vnat_small_max = nnat_small_max * (4. / 3. * pi * psc_pow(rnats, 3.));
             if (vnat > vnat_small_max) {
                 nnats = nnat_small_max;
-                nnatl = (vnat - vnat_small_max) / (4. / 3. * pi * pow(rnatl, 3.));
+                nnatl = (vnat - vnat_small_max) / (4. / 3. * pi * psc_pow(rnatl, 3.));
                 vnatl = vnat - vnat_small_max;
             } else {
-                nnats = vnat / (4. / 3. * pi * pow(rnats, 3.));
+                nnats = vnat / (4. / 3. * pi * psc_pow(rnats, 3.));
                 nnatl = 0.;
                 vnatl = 0.;
             }
@@ -499,24 +538,24 @@ static void aerosol_box(double temp, double ptot, double hnm, const double *ck, 
     /* ice, F:2861-2913 */
     {
         double tt = fmin(temp, 273.);
-        tice = 2668.70 / (10.4310 - (log(pw) + log(760.0)) / log(10.0));
+        tice = 2668.70 / (10.4310 - (psc_log(pw) + psc_log(760.0)) / psc_log(10.0));
         if (tt <= tice) {
             double zh2oeq, ph2o, h2os, zmt, zbt, zhno3eq;
             lice = 1;
             lnat = 0;
             anat = 0.;
-            zh2oeq = pow(10.0, -2668.7 / tt + 10.4310);
+            zh2oeq = psc_pow(10.0, -2668.7 / tt + 10.4310);
             ph2o = zh2oeq / 760.0;
             h2os = fmax(pw - ph2o, 0.) * 1013. * hnm / ptot;
             vice = h2os * amu * 18. / 0.928;
-            nice = vice / (4. / 3. * pi * pow(rice, 3.));
+            nice = vice / (4. / 3. * pi * psc_pow(rice, 3.));
             aice = nice * 4 * pi * (rice * rice);
             o->parth2o = 1.0 - fmax(pw - ph2o, 0.) / pw;
             o->parth2o = fmax(o->parth2o, 0.);
             o->parth2o = fmin(o->parth2o, 1.);
             zmt = -2.7836 - 0.00088 * tt;
             zbt = 38.9855 - 11397.0 / tt + 0.009179 * tt;
-            zhno3eq = pow(10.0, zmt * log10(pw * o->parth2o * 1013. * 0.75) + zbt);
+            zhno3eq = psc_pow(10.0, zmt * psc_log10(pw * o->parth2o * 1013. * 0.75) + zbt);
             pn = zhno3eq / (0.75 * 1013.);
             parthno3s = 1.0 - (pn0 - pn) / pn0;
             parthno3s = fmax(parthno3s, 0.);
@@ -541,7 +580,7 @@ static void aerosol_box(double temp, double ptot, double hnm, const double *ck, 
 
         a = ks[2] + ks[3] / t;
         b = ks[0] + ks[1] / t;
-        c = ks[4] + ks[5] / t + ks[6] * log(t) - log(pw);
+        c = ks[4] + ks[5] / t + ks[6] * psc_log(t) - psc_log(pw);
         det = (b * b) - 4. * a * c;
         if (det > 0.) {
             double xsb = (-b - sqrt(det)) / (2. * a);
@@ -557,8 +596,8 @@ static void aerosol_box(double temp, double ptot, double hnm, const double *ck, 
 
         wts = fmax(41., ws * 100.);
         muh2so4 = 4.184 * (1.514e4 - 286. * (wts - 40.) + 1.08 * ((wts - 40.) * (wts - 40.)) -
-                           3941. / pow(wts - 40., 0.1));
-        ph2so4 = exp(-10156. / t + 16.2590 - muh2so4 / (8.314 * t));
+                           3941. / psc_pow(wts - 40., 0.1));
+        ph2so4 = psc_exp(-10156. / t + 16.2590 - muh2so4 / (8.314 * t));
         if (qh2so4 * ptot / 1013. <= ph2so4) {
             ms = 0.;
             ws = 0.;
@@ -566,25 +605,25 @@ static void aerosol_box(double temp, double ptot, double hnm, const double *ck, 
         }
 
         if (condliq == 1 && t < 215.) {
-            double tf = t, tt = r * tf * ns, tr = 1.0e4 / tf - 43.4782608, pr = log(pw) + 18.4;
+            double tf = t, tt = r * tf * ns, tr = 1.0e4 / tf - 43.4782608, pr = psc_log(pw) + 18.4;
             double hsb, xnb, mnb, hnb, phi, msf, mnf, wsf, wnf, A, B, C, q;
             hsb = qs[0] + qs[1] * (tr * tr) +
                   (qs[2] + qs[3] * tr + qs[4] * (tr * tr) + qs[5] * (tr * tr * tr)) * pr +
                   (qs[6] + qs[7] * tr + qs[8] * (tr * tr)) * (pr * pr) +
                   qs[9] * tr * (pr * pr * pr);
-            hsb = exp(hsb);
+            hsb = psc_exp(hsb);
             q = (kn[0] + kn[1] / tf);
             xnb = 1.0 / (2.0 * (kn[2] + kn[3] / tf)) *
                   (-kn[0] - kn[1] / tf -
-                   pow(q * q - 4.0 * (kn[2] + kn[3] / tf) *
-                                   (kn[4] + kn[5] / tf + kn[6] * log(tf) - log(pw)),
+                   psc_pow(q * q - 4.0 * (kn[2] + kn[3] / tf) *
+                                   (kn[4] + kn[5] / tf + kn[6] * psc_log(tf) - psc_log(pw)),
                        0.5));
             mnb = 55.51 * xnb / (1.0 - xnb);
             hnb = qn[0] + qn[1] * (tr * tr) +
                   (qn[2] + qn[3] * tr + qn[4] * (tr * tr) + qn[5] * (tr * tr * tr)) * pr +
                   (qn[6] + qn[7] * tr + qn[8] * (tr * tr)) * (pr * pr) +
                   qn[9] * tr * (pr * pr * pr);
-            hnb = exp(hnb);
+            hnb = psc_exp(hnb);
 
             A = (tt * hnb * (mnb * mnb) - tt * hsb * mnb * msb - 2.0 * (mnb * mnb) * msb +
                  mnb * (msb * msb) + hnb * mnb * msb * pn0 - hsb * (msb * msb) * pn0) /
@@ -596,10 +635,10 @@ static void aerosol_box(double temp, double ptot, double hnm, const double *ck, 
             {
                 double u = (A * A) - 3.0 * B;
                 double v = -2.0 * (A * A * A) + 9.0 * A * B - 27.0 * C;
-                phi = atan(sqrt(fmax(4.0 * (u * u * u) - (v * v), 0.)) / v);
+                phi = psc_atan(sqrt(fmax(4.0 * (u * u * u) - (v * v), 0.)) / v);
             }
             if (phi < 0.) phi = phi + pi;
-            msf = -1.0 / 3.0 * (A + 2.0 * sqrt((A * A) - 3.0 * B) * cos((pi + phi) / 3.0));
+            msf = -1.0 / 3.0 * (A + 2.0 * sqrt((A * A) - 3.0 * B) * psc_cos((pi + phi) / 3.0));
             msf = fmax(msf, 0.);
             mnf = mnb * (1.0 - msf / msb);
             mnf = fmax(mnf, 0.);

@@ -56,124 +56,127 @@ __device__ void thermal_rates(const Troe &K, double t, double hnm, double pm, Ra
 {
     const double xln6 = log(0.6);
     const double lnt = log(t);
+    // one reciprocal for the Arrhenius arguments: c*rt is within an ulp of c/t and the
+    // rates feed only well-conditioned parts of the step
+    const double rt = 1. / t;
     r.a1 = falloff(K.a0hox, K.a1hox, -1.3, 0.2, lnt, hnm, xln6);
-    r.a1et = 1.63e-10 * exp(60. / t);
-    r.a2 = 1.4e-10 * exp(-470. / t);
-    r.a5 = 1.8e-11 * exp(180. / t);
-    r.a6 = 1.7e-12 * exp(-940. / t);
-    r.a6b = 1.0e-14 * exp(-490. / t);
-    r.a19 = 2.8e-12 * exp(-1800. / t);
-    r.a27 = 1.5e-12 * exp(19. / t) + 2.1e-33 * exp(920. / t) * hnm;
+    r.a1et = 1.63e-10 * exp(60. * rt);
+    r.a2 = 1.4e-10 * exp(-470. * rt);
+    r.a5 = 1.8e-11 * exp(180. * rt);
+    r.a6 = 1.7e-12 * exp(-940. * rt);
+    r.a6b = 1.0e-14 * exp(-490. * rt);
+    r.a19 = 2.8e-12 * exp(-1800. * rt);
+    r.a27 = 1.5e-12 * exp(19. * rt) + 2.1e-33 * exp(920. * rt) * hnm;
     r.a36 = 1.5e-13 * (1. + .6 * pm / 1013.);
 
-    r.b3 = 5.1e-12 * exp(210. / t);
-    r.b4 = 3.0e-12 * exp(-1500. / t);
-    r.b6 = 2.1e-11 * exp(100. / t);
-    r.b7 = 1.5e-11 * exp(-3600. / t);
-    r.b9 = 1.2e-13 * exp(-2450. / t);
+    r.b3 = 5.1e-12 * exp(210. * rt);
+    r.b4 = 3.0e-12 * exp(-1500. * rt);
+    r.b6 = 2.1e-11 * exp(100. * rt);
+    r.b7 = 1.5e-11 * exp(-3600. * rt);
+    r.b9 = 1.2e-13 * exp(-2450. * rt);
     r.b12 = falloff(K.a0nox1, K.a1nox1, -4.4, -0.7, lnt, hnm, xln6);
     r.b22 = falloff(K.a0nox2, K.a1nox2, -3.0, 0.0, lnt, hnm, xln6);
     r.b23 = falloff(K.a0nox3, K.a1nox3, -3.4, -1.1, lnt, hnm, xln6);
-    r.b24 = r.b23 / (2.1e-27 * exp(10900. / t) * hnm);          // via equilibrium constant
+    r.b24 = r.b23 / (2.1e-27 * exp(10900. * rt) * hnm);          // via equilibrium constant
     {
-        const double aux1 = 2.7e-17 * exp(2199. / t);
-        const double aux2 = 1. + aux1 / (6.5e-34 * exp(1335. / t) * hnm);
-        r.b27 = 2.4e-14 * exp(460. / t) + aux1 / aux2;
+        const double aux1 = 2.7e-17 * exp(2199. * rt);
+        const double aux2 = 1. + aux1 / (6.5e-34 * exp(1335. * rt) * hnm);
+        r.b27 = 2.4e-14 * exp(460. * rt) + aux1 / aux2;
     }
-    r.b28 = 1.3e-12 * exp(380. / t);
-    r.b32 = r.b12 / (2.7e-27 * exp(11000. / t) * hnm);
-    r.b38 = 4.63e-11 * exp(20. / t);
-    r.b39 = 7.25e-11 * exp(20. / t);
+    r.b28 = 1.3e-12 * exp(380. * rt);
+    r.b32 = r.b12 / (2.7e-27 * exp(11000. * rt) * hnm);
+    r.b38 = 4.63e-11 * exp(20. * rt);
+    r.b39 = 7.25e-11 * exp(20. * rt);
 
-    r.c2 = 2.45e-12 * exp(-1775. / t);
+    r.c2 = 2.45e-12 * exp(-1775. * rt);
     r.c4 = falloff(K.a0c1, K.a1c1, -3.6, 1.1, lnt, hnm, xln6);
-    r.c5 = 2.8e-12 * exp(300. / t);
-    r.c7 = 3.8e-13 * exp(800. / t);
-    r.c8 = 5.5e-12 * exp(125. / t);
-    r.c10 = 8.1e-11 * exp(-30. / t);
-    r.c14 = 9.5e-14 * exp(390. / t);
-    r.c15 = 3.9e-14 * exp(-900. / t);
+    r.c5 = 2.8e-12 * exp(300. * rt);
+    r.c7 = 3.8e-13 * exp(800. * rt);
+    r.c8 = 5.5e-12 * exp(125. * rt);
+    r.c10 = 8.1e-11 * exp(-30. * rt);
+    r.c14 = 9.5e-14 * exp(390. * rt);
+    r.c15 = 3.9e-14 * exp(-900. * rt);
     {
-        const double e200 = exp(200. / t);
+        const double e200 = exp(200. * rt);
         r.c17a = 3.8e-12 * e200 * 0.7;
         r.c17b = 3.8e-12 * e200 * 0.3;
         r.e11 = 5.5e-12 * e200;
         r.a7 = 3.0e-11 * e200;
     }
-    r.c18 = 3.3e-12 * exp(-115. / t);
-    r.c19 = 1.7e-11 * exp(-800. / t);
+    r.c18 = 3.3e-12 * exp(-115. * rt);
+    r.c19 = 1.7e-11 * exp(-800. * rt);
 
-    r.d2 = 2.3e-11 * exp(-200. / t);
-    r.d3 = 2.8e-11 * exp(85. / t);
+    r.d2 = 2.3e-11 * exp(-200. * rt);
+    r.d3 = 2.8e-11 * exp(85. * rt);
     {
-        const double e290 = exp(290. / t);
+        const double e290 = exp(290. * rt);
         r.d4 = 6.4e-12 * e290;
         r.d33 = 2.6e-12 * e290;
         r.e5c = 4.1e-13 * e290;
     }
-    r.d5 = 7.3e-12 * exp(-1280. / t);
-    r.d6 = 3.05e-11 * exp(-2270. / t);
-    r.d7 = 1.4e-11 * exp(269. / t);
+    r.d5 = 7.3e-12 * exp(-1280. * rt);
+    r.d6 = 3.05e-11 * exp(-2270. * rt);
+    r.d7 = 1.4e-11 * exp(269. * rt);
     {
-        const double e270 = exp(270. / t);
+        const double e270 = exp(270. * rt);
         r.d8 = 7.4e-12 * e270;
         r.a26 = 3.3e-12 * e270;
     }
     {
-        const double e230 = exp(230. / t);
+        const double e230 = exp(230. * rt);
         r.d9 = 6.0e-13 * e230;
         r.e3 = 1.9e-11 * e230;
         r.e6 = 1.5e-12 * e230;
     }
-    r.d10 = 3.6e-11 * exp(-375. / t);
-    r.d11 = 1.8e-12 * exp(-250. / t);
+    r.d10 = 3.6e-11 * exp(-375. * rt);
+    r.d11 = 1.8e-12 * exp(-250. * rt);
     r.d31 = falloff(K.a0clx1, K.a1clx1, -3.4, -1.9, lnt, hnm, xln6);
     {
-        const double em840 = exp(-840. / t);
+        const double em840 = exp(-840. * rt);
         r.d32 = 3.6e-12 * em840;
         r.e54 = 2.0e-12 * em840;
     }
-    r.d34 = 3.0e-12 * exp(-500. / t);
+    r.d34 = 3.0e-12 * exp(-500. * rt);
     r.d36 = falloff(K.a0clx2, K.a1clx2, -2., -1., lnt, hnm, xln6);
-    r.d37 = 3.4e-12 * exp(-130. / t);
+    r.d37 = 3.4e-12 * exp(-130. * rt);
     r.d60 = falloff(K.a0clx3, K.a1clx3, -4.5, -2.0, lnt, hnm, xln6);
-    r.d61 = r.d60 / (1.72e-27 * exp(8649. / t) * hnm);
-    r.d62 = 1.0e-12 * exp(-1590. / t) + 3.0e-11 * exp(-2450. / t);
-    r.d94 = 1.64e-12 * exp(-1520. / t);
-    r.d95 = 2.4e-12 * exp(-1250. / t);
+    r.d61 = r.d60 / (1.72e-27 * exp(8649. * rt) * hnm);
+    r.d62 = 1.0e-12 * exp(-1590. * rt) + 3.0e-11 * exp(-2450. * rt);
+    r.d94 = 1.64e-12 * exp(-1520. * rt);
+    r.d95 = 2.4e-12 * exp(-1250. * rt);
     {
-        const double em1600 = exp(-1600. / t);
+        const double em1600 = exp(-1600. * rt);
         r.d97 = 1.05e-12 * em1600;
         r.c9 = 3.4e-11 * em1600;
     }
 
-    r.e2 = 1.6e-11 * exp(-780. / t);
+    r.e2 = 1.6e-11 * exp(-780. * rt);
     {
-        const double e260 = exp(260. / t);
+        const double e260 = exp(260. * rt);
         r.e4 = 8.8e-12 * e260;
         r.e5b = 2.3e-12 * e260;
     }
-    r.e5a = 9.5e-13 * exp(550. / t);
-    r.e7 = 4.8e-12 * exp(-310. / t);
-    r.e8 = 1.9e-11 * exp(215. / t);
+    r.e5a = 9.5e-13 * exp(550. * rt);
+    r.e7 = 4.8e-12 * exp(-310. * rt);
+    r.e8 = 1.9e-11 * exp(215. * rt);
     {
-        const double e250 = exp(250. / t);
+        const double e250 = exp(250. * rt);
         r.e10 = 1.7e-11 * e250;
         r.a17 = 4.8e-11 * e250;
     }
     r.e13 = falloff(K.a0brx, K.a1brx, -3.2, -2.9, lnt, hnm, xln6);
     {
-        const double e460 = exp(460. / t);
+        const double e460 = exp(460. * rt);
         r.e15 = 4.5e-12 * e460;
     }
-    r.e16 = 1.2e-10 * exp(-430. / t);
-    r.e51 = 2.35e-12 * exp(-1300. / t);
+    r.e16 = 1.2e-10 * exp(-430. * rt);
+    r.e51 = 2.35e-12 * exp(-1300. * rt);
 
     r.hk1 = 4.23e-28 * hnm / (t * t);
     r.hk2 = 6.0e-34 * pow(300. / t, 2.4) * hnm;
-    r.hk3 = 8.e-12 * exp(-2060. / t);
-    r.hk4 = 2.15e-11 * exp(110. / t);
-    r.hk5 = 3.3e-11 * exp(55. / t);
+    r.hk3 = 8.e-12 * exp(-2060. * rt);
+    r.hk4 = 2.15e-11 * exp(110. * rt);
+    r.hk5 = 3.3e-11 * exp(55. * rt);
 }
 
 // ------------------------------------------------------------------ photolysis
@@ -278,6 +281,14 @@ __device__ double j_no(double colo2, double colno, double colo3, double hnm, dou
 }
 
 // ------------------------------------------------------------------ aerosol
+// Correctly rounded values of the reference's constant libm calls (checked in
+// tests/test_crmath.py): log(760), log(10) (F:2863), rnats**3, rnatl**3, rice**3.
+constexpr double LOG760 = 6.633318433280377, LOG10 = 2.302585092994046;
+constexpr double RNATS3 = 1.2500000000000002e-13, RNATL3 = 2.7462499999999997e-10, RICE3 = 1e-09;
+
+// x**0.5: the correctly rounded power equals the correctly rounded square root.
+__device__ __forceinline__ double sqrt_pow(double x) { return x == -INFINITY ? INFINITY : sqrt(x) + 0.; }
+
 struct Het {
     double k[9];                    // het1..het9
     double pno3, pno3sl, ph2o, phcl; // gas fractions
@@ -335,19 +346,19 @@ __device__ void psc_and_het(double temp, double ptot, double hnm, const double *
     {
         const double zmt = -2.7836 - 0.00088 * tm;
         const double zbt = 38.9855 - 11397.0 / tm + 0.009179 * tm;
-        const double eq = crm::pow_cr(10.0, zmt * crm::log10_cr(pw * 1013. * 0.75) + zbt);
+        const double eq = crm::pow10_cr(zmt * crm::log10_fdlibm(pw * 1013. * 0.75) + zbt);
         if (pn0 * 1013. * 0.75 / eq >= 1.) {
             lnat = 1;
             pn = eq / (0.75 * 1013.);
             const double hno3s = fmax((pn0 - pn) * 1013. * hnm / ptot, 0.);
             vnat = hno3s * AMU * 117. / 1.35;
-            const double vsmall = NNAT_SMALL * (4. / 3. * pi * crm::pow_cr(RNATS, 3.));
+            const double vsmall = NNAT_SMALL * (4. / 3. * pi * RNATS3);
             if (vnat > vsmall) {
                 nnats = NNAT_SMALL;
-                nnatl = (vnat - vsmall) / (4. / 3. * pi * crm::pow_cr(RNATL, 3.));
+                nnatl = (vnat - vsmall) / (4. / 3. * pi * RNATL3);
                 vnatl = vnat - vsmall;
             } else {
-                nnats = vnat / (4. / 3. * pi * crm::pow_cr(RNATS, 3.));
+                nnats = vnat / (4. / 3. * pi * RNATS3);
                 nnatl = 0.;
                 vnatl = 0.;
             }
@@ -357,20 +368,20 @@ __device__ void psc_and_het(double temp, double ptot, double hnm, const double *
         }
     }
     // ice
-    const double tice = 2668.70 / (10.4310 - (crm::log_cr(pw) + crm::log_cr(760.0)) / crm::log_cr(10.0));
+    const double tice = 2668.70 / (10.4310 - (crm::log_cr(pw) + LOG760) / LOG10);
     if (tm <= tice) {
         lice = 1;
         lnat = 0;
         anat = 0.;
-        const double ph2o = crm::pow_cr(10.0, -2668.7 / tm + 10.4310) / 760.0;
+        const double ph2o = crm::pow10_cr(-2668.7 / tm + 10.4310) / 760.0;
         const double h2os = fmax(pw - ph2o, 0.) * 1013. * hnm / ptot;
         const double vice = h2os * AMU * 18. / 0.928;
-        nice = vice / (4. / 3. * pi * crm::pow_cr(RICE, 3.));
+        nice = vice / (4. / 3. * pi * RICE3);
         aice = nice * 4 * pi * (RICE * RICE);
         o.ph2o = fmin(fmax(1.0 - fmax(pw - ph2o, 0.) / pw, 0.), 1.);
         const double zmt = -2.7836 - 0.00088 * tm;
         const double zbt = 38.9855 - 11397.0 / tm + 0.009179 * tm;
-        const double eq = crm::pow_cr(10.0, zmt * crm::log10_cr(pw * o.ph2o * 1013. * 0.75) + zbt);
+        const double eq = crm::pow10_cr(zmt * crm::log10_fdlibm(pw * o.ph2o * 1013. * 0.75) + zbt);
         pn = eq / (0.75 * 1013.);
         pno3s = fmin(fmax(1.0 - (pn0 - pn) / pn0, 0.), 1.);
         o.pno3sl = pno3s;
@@ -378,6 +389,7 @@ __device__ void psc_and_het(double temp, double ptot, double hnm, const double *
     // liquid binary H2SO4/H2O then ternary (Carslaw et al. 1995)
     const double t = fmax(temp, fmax(tice - 3.0, 185.));
     pw = fmax(fmin(ck[H2O] * o.ph2o / hnm * ptot / 1013.0, 2.e-3 / 1013.), 2.e-5 / 1013.);
+    const double lt = crm::log_cr(t), lpw = crm::log_cr(pw);   // shared by binary and ternary
     const double qh2so4 = fmax(ck[H2SO4] / hnm, 1.e-18);
     const double ns = ptot * 100.0 * qh2so4 / 8.314 / t;
     pn0 = ptot / 1013.0 * (ck[HNO3] * pno3s / hnm);
@@ -386,7 +398,7 @@ __device__ void psc_and_het(double temp, double ptot, double hnm, const double *
     {
         const double a = 51.81 + -15732.0 / t;
         const double b = -21.661 + 2724.2 / t;
-        const double c = 47.004 + -6969.0 / t + -4.6183 * crm::log_cr(t) - crm::log_cr(pw);
+        const double c = 47.004 + -6969.0 / t + -4.6183 * lt - lpw;
         const double det = (b * b) - 4. * a * c;
         if (det > 0.) {
             const double xsb = (-b - sqrt(det)) / (2. * a);
@@ -416,7 +428,7 @@ __device__ void psc_and_het(double temp, double ptot, double hnm, const double *
         constexpr double qs[10] = {14.4700, 0.0638795, -3.29597, 1.778224, -0.223244,
                                    0.0086486, 0.536695, -0.335164, 0.0265153, 0.0157550};
         constexpr double kn[7] = {-39.136, 6358.4, 83.29, -17650.0, 198.53, -11948, -28.469};
-        const double tt = R * t * ns, tr = 1.0e4 / t - 43.4782608, pr = crm::log_cr(pw) + 18.4;
+        const double tt = R * t * ns, tr = 1.0e4 / t - 43.4782608, pr = lpw + 18.4;
         const double tr2 = tr * tr, tr3 = tr * tr * tr, pr2 = pr * pr, pr3 = pr * pr * pr;
         const double hsb = crm::exp_cr(qs[0] + qs[1] * tr2 +
                                (qs[2] + qs[3] * tr + qs[4] * tr2 + qs[5] * tr3) * pr +
@@ -424,9 +436,8 @@ __device__ void psc_and_het(double temp, double ptot, double hnm, const double *
         const double kk = kn[0] + kn[1] / t;
         const double xnb = 1.0 / (2.0 * (kn[2] + kn[3] / t)) *
                            (-kn[0] - kn[1] / t -
-                            crm::pow_cr(kk * kk - 4.0 * (kn[2] + kn[3] / t) *
-                                              (kn[4] + kn[5] / t + kn[6] * crm::log_cr(t) - crm::log_cr(pw)),
-                                0.5));
+                            sqrt_pow(kk * kk - 4.0 * (kn[2] + kn[3] / t) *
+                                                   (kn[4] + kn[5] / t + kn[6] * lt - lpw)));
         const double mnb = 55.51 * xnb / (1.0 - xnb);
         const double hnb = crm::exp_cr(qn[0] + qn[1] * tr2 +
                                (qn[2] + qn[3] * tr + qn[4] * tr2 + qn[5] * tr3) * pr +
@@ -626,6 +637,8 @@ struct BoxEnv {
     bool day;
 };
 
+// ck0 is the start-of-step state in LDS, one column per lane: ck0[s * CK0_STRIDE].
+template <int CK0_STRIDE>
 __device__ void sweep(const Rates &R, const double *tj, const BoxEnv &e, const double *ck0,
                       double *ck, double *cc, double &ch3o, double &hco)
 {
@@ -734,7 +747,7 @@ __device__ void sweep(const Rates &R, const double *tj, const BoxEnv &e, const d
     // production P and loss L (F:1758-2358), then ck = (ck0 + dt P)/(1 + dt L)
     const double dt = e.dt;
     double nk[NLONG];
-#define UPDATE(s, P, Lr) nk[s] = (ck0[s] + dt * (P)) / (1. + dt * (Lr))
+#define UPDATE(s, P, Lr) nk[s] = (ck0[(s) * CK0_STRIDE] + dt * (P)) / (1. + dt * (Lr))
     UPDATE(N2O, 0.e0, tj[JN2O] + (R.b38 + R.b39) * cc[O1D]);
     UPDATE(CH4, 0.e0, R.c2 * cc[OH] + C1A * cc[O1D] + C1B * cc[O1D] + C1C * cc[O1D] + R.d5 * cc[CL]);
     UPDATE(H2O,
@@ -907,9 +920,13 @@ __global__ void k_columns(const double *__restrict__ p, const double *__restrict
     }
 }
 
-__global__ void __launch_bounds__(256) k_chem(StepArgs a)
+template <int BLK>
+__global__ void __launch_bounds__(BLK) k_chem(StepArgs a)
 {
-    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    // start-of-step state of the block's boxes, species-major: lane-private columns,
+    // conflict-free, and off the register file that the sweep needs for rates and J
+    __shared__ double s_ck0[(NLONG - 1) * BLK];
+    const int64_t b = (int64_t)blockIdx.x * BLK + threadIdx.x;
     if (b >= a.nbox) return;
     const int64_t c = b % a.plane;
     const int k = (int)(b / a.plane);
@@ -924,11 +941,12 @@ __global__ void __launch_bounds__(256) k_chem(StepArgs a)
     e.dt = a.dt;
     e.day = sza <= 95.;
 
-    double ck0[NLONG], ck[NLONG], cc[NSHORT];
+    double ck[NLONG], cc[NSHORT];
+    double *ck0 = s_ck0 + threadIdx.x;
 #pragma unroll
     for (int s = 0; s < NLONG; ++s) {
-        ck0[s] = a.qj1[(int64_t)s * a.field + b] * e.hnm;
-        ck[s] = ck0[s];
+        ck[s] = a.qj1[(int64_t)s * a.field + b] * e.hnm;
+        if (s < NLONG - 1) ck0[s * BLK] = ck[s];
     }
 #pragma unroll
     for (int s = 0; s < NSHORT; ++s) cc[s] = a.hc[(int64_t)s * a.field + b] * e.hnm;
@@ -965,7 +983,11 @@ __global__ void __launch_bounds__(256) k_chem(StepArgs a)
     thermal_rates(a.K, t, e.hnm, pm, R);
 
     double ch3o = 0., hco = 0.;
-    for (int it = 0; it < a.niter; ++it) sweep(R, tj, e, ck0, ck, cc, ch3o, hco);
+    for (int it = 0; it < a.niter; ++it) {
+        // keeps the LDS reads of ck0 inside the loop (not hoisted into registers)
+        asm volatile("" ::: "memory");
+        sweep<BLK>(R, tj, e, ck0, ck, cc, ch3o, hco);
+    }
 
     // write-back (F:2380-2410)
     double q[NLONG], hs[NSHORT];
@@ -1082,7 +1104,12 @@ hipError_t launch_chem(const StepArgs &a, int block, hipStream_t s)
 {
     const int blk = block > 0 ? block : 256;
     const int64_t grid = (a.nbox + blk - 1) / blk;
-    hipLaunchKernelGGL(k_chem, dim3((unsigned)grid), dim3(blk), 0, s, a);
+    switch (blk) {
+    case 64: hipLaunchKernelGGL(k_chem<64>, dim3((unsigned)grid), dim3(64), 0, s, a); break;
+    case 128: hipLaunchKernelGGL(k_chem<128>, dim3((unsigned)grid), dim3(128), 0, s, a); break;
+    case 256: hipLaunchKernelGGL(k_chem<256>, dim3((unsigned)grid), dim3(256), 0, s, a); break;
+    default: return hipErrorInvalidConfiguration;
+    }
     return hipGetLastError();
 }
 

@@ -94,6 +94,26 @@ constexpr double INV_LN2 = 1.44269504088896338700e+00;
 constexpr double PIO2_HI = 1.57079632679489655800e+00, PIO2_LO = 6.12323399573676603587e-17;
 constexpr double PIO2_LO2 = -1.49738490485916983e-33;
 constexpr double INV_LN10_HI = 4.34294481903251816668e-01, INV_LN10_LO = 1.09831965021676510e-17;
+constexpr double LN10_HI = 2.302585092994046, LN10_LO = -2.1707562233822494e-16;
+// 1/j (j = 1..9) and 1/(j(j-1)) (j = 2..19) as double-double: the Taylor steps
+// multiply instead of divide.
+constexpr double INVJ_HI[10] = {0., 1.0, 0.5, 0.3333333333333333, 0.25, 0.2, 0.16666666666666666,
+                                0.14285714285714285, 0.125, 0.1111111111111111};
+constexpr double INVJ_LO[10] = {0., 0.0, 0.0, 1.850371707708594e-17, 0.0, -1.1102230246251566e-17,
+                                9.25185853854297e-18, 7.93016446160826e-18, 0.0,
+                                6.1679056923619804e-18};
+constexpr double INVJJ_HI[20] = {
+    0., 0., 0.5, 0.16666666666666666, 0.08333333333333333, 0.05, 0.03333333333333333,
+    0.023809523809523808, 0.017857142857142856, 0.013888888888888888, 0.011111111111111112,
+    0.00909090909090909, 0.007575757575757576, 0.00641025641025641, 0.005494505494505495,
+    0.004761904761904762, 0.004166666666666667, 0.003676470588235294, 0.0032679738562091504,
+    0.0029239766081871343};
+constexpr double INVJJ_LO[20] = {
+    0., 0., 0.0, 9.25185853854297e-18, 4.625929269271485e-18, -2.7755575615628915e-18,
+    4.625929269271486e-19, 1.32169407693471e-18, 9.912705577010326e-19, 7.709882115452476e-19,
+    -4.2404351634988616e-19, 4.415659757031872e-19, -2.1026951223961299e-19,
+    2.2240044563805217e-19, -4.2891514515910067e-19, -4.295505750037808e-19,
+    5.782411586589357e-20, 5.102127870520021e-20, -9.920804192677818e-20, 1.6231330769373633e-19};
 
 // exp of a double-double, ~2^-100 relative.  Reduction x = n ln2 + r, then
 // exp(r/256) by Taylor (9 terms, |r/256| < 1.4e-3) and 8 squarings.
@@ -107,7 +127,10 @@ CR_HD dd exp_dd(dd a)
     r = add_d(r, -n * LN2_LO);
     r = scale(r, -8);
     dd s = {1., 0.};
-    for (int j = 9; j >= 1; --j) s = add_d(div_d(mul(s, r), (double)j), 1.);
+#pragma unroll
+    for (int j = 9; j >= 2; --j) s = add_d(mul(mul(s, r), dd{INVJ_HI[j], INVJ_LO[j]}), 1.);
+    s = add_d(mul(s, r), 1.);
+#pragma unroll
     for (int i = 0; i < 8; ++i) s = mul(s, s);
     // 2^n in two steps so that n near the exponent limits cannot overflow the factor
     const int ni = (int)n;
@@ -148,6 +171,44 @@ CR_HD double log10_cr(double x)
     return r.hi + r.lo;
 }
 
+// log10 as the reference's C library evaluates it: the classic fdlibm splitting
+// (published with fdlibm 5.3, e_log10.c) x = 2^k x', log10 x = k log10(2) + log(x')/ln 10
+// with log10(2) in two parts -- not correctly rounded (~3 % of arguments differ from the
+// correctly rounded value by an ulp), so the reference's results follow this form and
+// so do ours.  The natural log inside is the correctly rounded one.
+CR_HD double log10_fdlibm(double x)
+{
+    const double ivln10 = 4.34294481903251816668e-01, log10_2hi = 3.01029995663611771306e-01,
+                 log10_2lo = 3.69423907715893078616e-13, two54 = 1.80143985094819840000e+16;
+    union { double d; unsigned long long u; } v;
+    v.d = x;
+    int hx = (int)(v.u >> 32);
+    int k = 0;
+    if (hx < 0x00100000) {
+        if (((hx & 0x7fffffff) | (unsigned)(v.u & 0xffffffffu)) == 0) return -INFINITY;
+        if (hx < 0) return NAN;
+        k -= 54;
+        v.d = x * two54;
+        hx = (int)(v.u >> 32);
+    }
+    if (hx >= 0x7ff00000) return x + x;
+    k += (hx >> 20) - 1023;
+    const int i = (int)(((unsigned)k & 0x80000000u) >> 31);
+    hx = (hx & 0x000fffff) | ((0x3ff - i) << 20);
+    const double y = (double)(k + i);
+    v.u = ((unsigned long long)(unsigned)hx << 32) | (v.u & 0xffffffffull);
+    const double z = y * log10_2lo + ivln10 * log_cr(v.d);
+    return z + y * log10_2hi;
+}
+
+// 10**y, correctly rounded: exp of y*ln10 with ln10 in double-double.
+CR_HD double pow10_cr(double y)
+{
+    if (y != y) return y;
+    const dd r = exp_dd(mul_d(dd{LN10_HI, LN10_LO}, y));
+    return r.hi + r.lo;
+}
+
 CR_HD double pow_cr(double x, double y)
 {
     if (y == 0. || x == 1.) return 1.;
@@ -168,11 +229,13 @@ CR_HD void sincos_dd(dd a, dd &s, dd &c)
     r = scale(r, -3);
     const dd r2 = mul(r, r);
     dd ts = {1., 0.}, tc = {1., 0.};
+#pragma unroll
     for (int j = 19; j >= 3; j -= 2)
-        ts = add_d(neg(div_d(mul(ts, r2), (double)(j * (j - 1)))), 1.);
+        ts = add_d(neg(mul(mul(ts, r2), dd{INVJJ_HI[j], INVJJ_LO[j]})), 1.);
     ts = mul(ts, r);
+#pragma unroll
     for (int j = 18; j >= 2; j -= 2)
-        tc = add_d(neg(div_d(mul(tc, r2), (double)(j * (j - 1)))), 1.);
+        tc = add_d(neg(mul(mul(tc, r2), dd{INVJJ_HI[j], INVJJ_LO[j]})), 1.);
     for (int i = 0; i < 3; ++i) {
         const dd s2 = mul_d(mul(ts, tc), 2.);
         const dd c2 = add_d(neg(mul_d(mul(ts, ts), 2.)), 1.);
@@ -220,6 +283,7 @@ CR_HD double tanh_cr(double x)
         // expm1(2ax) by Taylor (13 terms), tanh = em / (em + 2)
         const dd z = {2. * ax, 0.};
         dd s = {1., 0.};
+#pragma unroll
         for (int j = 13; j >= 2; --j) s = add_d(div_d(mul(s, z), (double)j), 1.);
         const dd em = mul(s, z);
         const dd q = div(em, add_d(em, 2.));

@@ -6,7 +6,9 @@ extern "C" {
 double rb_cr_exp(double x) { return crm::exp_cr(x); }
 double rb_cr_log(double x) { return crm::log_cr(x); }
 double rb_cr_log10(double x) { return crm::log10_cr(x); }
+double rb_fdlibm_log10(double x) { return crm::log10_fdlibm(x); }
 double rb_cr_pow(double x, double y) { return crm::pow_cr(x, y); }
+double rb_cr_pow10(double y) { return crm::pow10_cr(y); }
 double rb_cr_atan(double x) { return crm::atan_cr(x); }
 double rb_cr_cos(double x) { return crm::cos_cr(x); }
 double rb_cr_tanh(double x) { return crm::tanh_cr(x); }

@@ -108,6 +108,10 @@ int rb_init(rb_ctx **out, int device, const rb_config *cfg)
     if (cfg) ctx->cfg = *cfg;
     if (ctx->cfg.niter < 1) ctx->cfg.niter = 5;
     if (ctx->cfg.block <= 0) ctx->cfg.block = 256;
+    if (ctx->cfg.block != 64 && ctx->cfg.block != 128 && ctx->cfg.block != 256) {
+        delete ctx;
+        return RB_EINVAL;   // the per-box kernel is instantiated for 64, 128 and 256 threads
+    }
     ctx->K = rb::troe_constants();
     if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&ctx->own) != hipSuccess ||
         hipMalloc((void **)&ctx->d_count, sizeof(unsigned long long)) != hipSuccess ||

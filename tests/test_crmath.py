@@ -27,6 +27,7 @@ def libs():
             f.argtypes, f.restype = [C.c_double], C.c_double
         f = getattr(L, pre + "pow")
         f.argtypes, f.restype = [C.c_double, C.c_double], C.c_double
+    H.rb_cr_pow10.argtypes, H.rb_cr_pow10.restype = [C.c_double], C.c_double
     return H, Q
 
 
@@ -69,3 +70,24 @@ def test_special_values(libs):
     assert np.isnan(H.rb_cr_log(-1.0)) and H.rb_cr_exp(-1e4) == 0.0 and np.isinf(H.rb_cr_exp(1e4))
     assert H.rb_cr_atan(float("inf")) == np.pi / 2 and H.rb_cr_tanh(-30.0) == -1.0
     assert H.rb_cr_pow(7.0, 0.0) == 1.0 and H.rb_cr_pow(1.0, 123.4) == 1.0
+
+
+def test_pow10(libs):
+    H, Q = libs
+    ys = np.random.default_rng(5).uniform(-30, 30, 20000)
+    bad = [y for y in ys if H.rb_cr_pow10(float(y)) != Q.qr_pow(10.0, float(y))]
+    assert not bad, bad[:5]
+
+
+def test_kernel_constants_are_correctly_rounded(libs):
+    """The kernel hoists the reference's constant libm calls as literals."""
+    import re
+
+    _, Q = libs
+    src = open(os.path.join(ROOT, "reprobus_amd", "csrc", "chem_kernels.hip")).read()
+    vals = dict(re.findall(r"(LOG760|LOG10|RNATS3|RNATL3|RICE3) = ([0-9.e+-]+)", src))
+    assert float(vals["LOG760"]) == Q.qr_log(760.0)
+    assert float(vals["LOG10"]) == Q.qr_log(10.0)
+    assert float(vals["RNATS3"]) == Q.qr_pow(0.5e-4, 3.0)
+    assert float(vals["RNATL3"]) == Q.qr_pow(6.5e-4, 3.0)
+    assert float(vals["RICE3"]) == Q.qr_pow(10.0e-4, 3.0)

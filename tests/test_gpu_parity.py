@@ -46,16 +46,23 @@ def test_gpu_matches_reference_golden(step, name):
         assert v < PARITY_TOL, (k, v)
 
 
+# GPU vs the restatement built with correctly rounded math in the PSC block: the
+# algorithm is the same, so the gap is rounding noise of well-conditioned parts.
+TIGHT = 1e-11
+
+
 def test_gpu_matches_restatement_c0(step, table):
     import cport
     from reprobus_amd import synth
 
     inp = synth.make_c0(0)
     out = step.run_inputs(inp)
-    ref = cport.chem_step(inp, *table)
-    worst = _report("c0 full vs restatement", out, ref)
+    worst = _report("c0 full vs restatement", out, cport.chem_step(inp, *table))
     for k, v in worst.items():
         assert v < PARITY_TOL, (k, v)
+    worst = _report("c0 full vs CR-PSC restatement", out, cport.chem_step(inp, *table, cr_psc=True))
+    for k, v in worst.items():
+        assert v < TIGHT, (k, v)
 
 
 def test_gpu_matches_restatement_c1_subgrid(step, table):
@@ -64,10 +71,40 @@ def test_gpu_matches_restatement_c1_subgrid(step, table):
 
     inp = synth.make_c1(seed=1, nlon=24, nlat=13)
     out = step.run_inputs(inp)
-    ref = cport.chem_step(inp, *table)
-    worst = _report("c1 24x13x60 vs restatement", out, ref)
+    worst = _report("c1 24x13x60 vs restatement", out, cport.chem_step(inp, *table))
     for k, v in worst.items():
         assert v < PARITY_TOL, (k, v)
+    worst = _report("c1 24x13x60 vs CR-PSC restatement", out,
+                    cport.chem_step(inp, *table, cr_psc=True))
+    for k, v in worst.items():
+        assert v < TIGHT, (k, v)
+
+
+def test_gpu_full_grid_parity(step, table):
+    """All 786,240 boxes of c1 against both restatements.
+
+    Against the correctly rounded PSC variant: 1e-11 everywhere.  Against the C-library
+    variant (the reference's rounding): the ternary PSC solve amplifies the library's
+    last-bit misroundings (glibc 2.35 atan/cos/exp/pow misround ~0.1 % of calls), so a
+    few ternary boxes may exceed 1e-9; they must be rare and all ternary (T < 215 K).
+    """
+    import cport
+    from reprobus_amd import synth
+
+    inp = synth.make_c1(seed=0)
+    out = step.run_inputs(inp)
+    cr = cport.chem_step(inp, *table, cr_psc=True)
+    worst = _report("c1 full vs CR-PSC restatement", out, cr)
+    for k, v in worst.items():
+        assert v < TIGHT, (k, v)
+    lib = cport.chem_step(inp, *table)
+    d = np.maximum(rel_err(out["qj1"], lib["qj1"]).max(axis=0),
+                   rel_err(out["hc"], lib["hc"]).max(axis=0))
+    over = d > PARITY_TOL
+    print(f"[parity] c1 full vs C-library restatement: max {d.max():.2e}, "
+          f"{int(over.sum())} of {d.size} boxes above {PARITY_TOL:g}", flush=True)
+    assert over.sum() <= 1e-4 * d.size
+    assert np.all(inp.T[over] < 215.0)
 
 
 def _device_inputs(inp):
