@@ -93,6 +93,21 @@ int rb_chem_step_dev(rb_ctx *ctx, const rb_grid *g, double dt, const double *T, 
                      int fuse_conserv);
 int rb_conserv_floor_dev(rb_ctx *ctx, const rb_grid *g, double *qj1, double *hc);
 
+/* Converged mode (no counterpart in the reference, which always stops after niter = 5
+ * sweeps, F:630): the same per-box algorithm swept to its fixed point -- until the largest
+ * relative change of any long- or short-lived species over a sweep is below tol (values
+ * under the 1e-30 mole-fraction floor compared at the floor) or maxiter sweeps.  A
+ * persistent kernel takes boxes from a device work queue, so converged boxes free their
+ * lanes at once.  guess_qj1 / guess_hc (device, mixing ratios, state layout, or both
+ * NULL) give the initial iterate instead of the incoming state -- the learned initial
+ * guess; the start-of-step state ck0 is always qj1.  iters (device, nlon*nlat*nivbas
+ * int32, or NULL) receives the sweep count of each box. */
+int rb_chem_converge_dev(rb_ctx *ctx, const rb_grid *g, double dt, double tol, int32_t maxiter,
+                         const double *T, const double *p, const double *sza, const double *alt,
+                         double *qj1, double *hc, double *surfarea, double *vsed3d,
+                         double *parth2osed, double *parthno3sed, const double *guess_qj1,
+                         const double *guess_hc, int32_t *iters, int fuse_conserv);
+
 int rb_sync(rb_ctx *ctx);
 /* Waits for the timed launches, fills *out and resets the profile accumulators. */
 int rb_get_stats(rb_ctx *ctx, rb_stats_t *out);

@@ -43,6 +43,10 @@ def lib(cr_psc: bool = False):
                                     dp, dp, dp, dp, C.c_int]
         L.rbo_chem_step.restype = C.c_int
         L.rbo_conserv_floor.argtypes = [C.POINTER(_Grid), dp, dp, C.c_int]
+        L.rbo_chem_converge.argtypes = [C.POINTER(_Grid), C.c_double, C.c_double] + [dp] * 12 + [
+            np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS"), C.c_void_p, C.c_void_p,
+            C.c_int]
+        L.rbo_chem_converge.restype = C.c_int
         L.rbo_troe_constants.argtypes = [dp]
         L.rbo_troe.argtypes = [C.c_double] * 6
         L.rbo_troe.restype = C.c_double
@@ -91,3 +95,35 @@ def chem_step(inp, ajl, o3up, niter: int = 5, nthreads: int = 0, conserv: bool =
 def fortran_ajl(ajl):
     """ajl (40,101,27,7) in Fortran index order -> contiguous Fortran bytes."""
     return np.ascontiguousarray(np.asarray(ajl, dtype=np.float64).transpose(3, 2, 1, 0))
+
+
+def chem_converge(inp, ajl, o3up, tol: float = 1e-10, maxiter: int = 2000, nthreads: int = 0,
+                  conserv: bool = True, cr_psc: bool = False, guess=None) -> dict:
+    """Converged mode of the restatement: sweeps to the fixed point (per-box counts in 'iters').
+    guess = (qj1, hc) mixing ratios in state layout replaces the initial iterate."""
+    g = inp.grid
+    G = _Grid(g.nlon, g.nlat, g.niv, g.nivbas, g.nivhau, maxiter)
+    q = np.ascontiguousarray(inp.qj1, dtype=np.float64).copy()
+    h = np.ascontiguousarray(inp.hc, dtype=np.float64).copy()
+    surf = np.zeros(g.shape3)
+    shb = (g.nivbas, g.nlat, g.nlon)
+    vsed, p2o, pno3 = np.zeros(shb), np.zeros(shb), np.zeros(shb)
+    iters = np.zeros(shb, dtype=np.int32)
+    c = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+    L = lib(cr_psc)
+    gq = gh = None
+    if guess is not None:
+        gq, gh = c(guess[0]), c(guess[1])
+        assert gq.shape == q.shape and gh.shape == h.shape
+    rc = L.rbo_chem_converge(C.byref(G), float(inp.dt), float(tol), c(inp.T), c(inp.p), c(inp.sza),
+                             c(inp.alt), fortran_ajl(ajl), c(o3up), q, h, surf, vsed, p2o, pno3,
+                             iters, None if gq is None else gq.ctypes.data,
+                             None if gh is None else gh.ctypes.data, nthreads)
+    if rc != 0:
+        raise RuntimeError(f"rbo_chem_converge failed: {rc}")
+    out = {"qj1_chem": q.copy(), "hc_chem": h.copy(), "surfarea": surf, "vsed3d": vsed,
+           "parth2osed": p2o, "parthno3sed": pno3, "iters": iters}
+    if conserv:
+        L.rbo_conserv_floor(C.byref(G), q, h, nthreads)
+        out["qj1"], out["hc"] = q, h
+    return out

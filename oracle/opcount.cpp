@@ -45,6 +45,7 @@ inline cd atan(cd a) { ++C.trig; return cd(std::atan(a.v)); }
 inline cd cos(cd a) { ++C.trig; return cd(std::cos(a.v)); }
 inline cd asin(cd a) { ++C.trig; return cd(std::asin(a.v)); }
 inline cd trunc(cd a) { return cd(std::trunc(a.v)); }
+inline cd fabs(cd a) { return cd(std::fabs(a.v)); }
 inline cd fmin(cd a, cd b) { ++C.minmax; return cd(std::fmin(a.v, b.v)); }
 inline cd fmax(cd a, cd b) { ++C.minmax; return cd(std::fmax(a.v, b.v)); }
 }  // namespace opc
@@ -52,7 +53,7 @@ inline cd fmax(cd a, cd b) { ++C.minmax; return cd(std::fmax(a.v, b.v)); }
 using opc::cd;
 using opc::exp; using opc::log; using opc::log10; using opc::pow; using opc::sqrt;
 using opc::tanh; using opc::atan; using opc::cos; using opc::asin; using opc::trunc;
-using opc::fmin; using opc::fmax;
+using opc::fmin; using opc::fmax; using opc::fabs;
 
 // system headers first, so the macro below never reaches them
 #include <math.h>

@@ -906,7 +906,8 @@ typedef struct {
  * q[1..43] and h[1..15] are mixing ratios, updated in place. */
 static void box_step(const troe_t *K, const double *ajl, const double *o3up, const boxenv_t *e,
                      double dt, int niter, double *q, double *h, double *surf, double *vsed,
-                     double *p2osed, double *pno3sed, int *surf_written)
+                     double *p2osed, double *pno3sed, int *surf_written, double tol, int *iters,
+                     const double *gq, const double *gh)
 {
     double ck0[NBCON + 1], ck[NBCON + 1], cc[NCM + 1], tj[NPHOT + 1], pr[NBCON + 1],
         pe[NBCON + 1];
@@ -963,7 +964,19 @@ static void box_step(const troe_t *K, const double *ajl, const double *o3up, con
         }
     }
 
+    /* converged mode with an initial guess: the iterate (not ck0) starts from it */
+    if (gq) {
+        for (n = 1; n <= NBCON - 1; ++n) ck[n] = gq[n] * hnm;
+        for (n = 1; n <= NCM; ++n) cc[n] = gh[n] * hnm;
+    }
+
+    *iters = niter;
     for (int iter = 1; iter <= niter; ++iter) {
+        double prev[NBCON + NCM + 1];
+        if (tol > 0.) {
+            for (n = 1; n <= NBCON - 1; ++n) prev[n] = ck[n];
+            for (n = 1; n <= NCM; ++n) prev[NBCON - 1 + n] = cc[n];
+        }
         double r1, r2, r3, r4, r5, r6, r7, r8, r9, r10, r11, r12, r13, r27, r28, r29, r30;
         double a, b, coea, coeb, coec, argum;
         double t01, t02, t03, t04, t05, t06, t07, t08, t09, t10, t11, t12, t13, t14, t15;
@@ -1235,6 +1248,25 @@ static void box_step(const troe_t *K, const double *ajl, const double *o3up, con
 
         /* F:2364-2372 */
         for (n = 1; n <= NBCON - 1; ++n) ck[n] = (ck0[n] + dt * pr[n]) / (1. + dt * pe[n]);
+
+        /* converged mode: stop at the fixed point of the sweep map (relative change of
+         * every species below tol, above the 1e-30 mole-fraction floor) */
+        if (tol > 0.) {
+            const double fl = 1.e-30 * hnm;
+            double worst = 0.;
+            for (n = 1; n <= NBCON - 1; ++n) {
+                double d = fabs(ck[n] - prev[n]) / fmax(fabs(ck[n]), fl);
+                if (!(d <= worst)) worst = d;
+            }
+            for (n = 1; n <= NCM; ++n) {
+                double d = fabs(cc[n] - prev[NBCON - 1 + n]) / fmax(fabs(cc[n]), fl);
+                if (!(d <= worst)) worst = d;
+            }
+            if (worst < tol) {
+                *iters = iter;
+                break;
+            }
+        }
     }
 
     /* write-back F:2380-2410 */
@@ -1244,10 +1276,37 @@ static void box_step(const troe_t *K, const double *ajl, const double *o3up, con
 }
 
 /* ---------------------------------------------------------------- grid driver */
+static int chem_grid(const rbo_grid *g, double dt, const double *T, const double *p,
+                     const double *sza, const double *alt, const double *ajl, const double *o3up,
+                     double *qj1, double *hc, double *surfarea, double *vsed3d,
+                     double *parth2osed, double *parthno3sed, int nthreads, double tol,
+                     int *iters, const double *guess_qj1, const double *guess_hc);
+
 int rbo_chem_step(const rbo_grid *g, double dt, const double *T, const double *p,
                   const double *sza, const double *alt, const double *ajl, const double *o3up,
                   double *qj1, double *hc, double *surfarea, double *vsed3d,
                   double *parth2osed, double *parthno3sed, int nthreads)
+{
+    return chem_grid(g, dt, T, p, sza, alt, ajl, o3up, qj1, hc, surfarea, vsed3d, parth2osed,
+                     parthno3sed, nthreads, 0., NULL, NULL, NULL);
+}
+
+int rbo_chem_converge(const rbo_grid *g, double dt, double tol, const double *T, const double *p,
+                      const double *sza, const double *alt, const double *ajl, const double *o3up,
+                      double *qj1, double *hc, double *surfarea, double *vsed3d,
+                      double *parth2osed, double *parthno3sed, int *iters,
+                      const double *guess_qj1, const double *guess_hc, int nthreads)
+{
+    if ((guess_qj1 == NULL) != (guess_hc == NULL)) return -1;
+    return chem_grid(g, dt, T, p, sza, alt, ajl, o3up, qj1, hc, surfarea, vsed3d, parth2osed,
+                     parthno3sed, nthreads, tol, iters, guess_qj1, guess_hc);
+}
+
+static int chem_grid(const rbo_grid *g, double dt, const double *T, const double *p,
+                     const double *sza, const double *alt, const double *ajl, const double *o3up,
+                     double *qj1, double *hc, double *surfarea, double *vsed3d,
+                     double *parth2osed, double *parthno3sed, int nthreads, double tol,
+                     int *iters, const double *guess_qj1, const double *guess_hc)
 {
     const size_t plane = (size_t)g->nlon * g->nlat;          /* one level of a field */
     const size_t field = plane * g->niv;                       /* one species field */
@@ -1261,7 +1320,7 @@ int rbo_chem_step(const rbo_grid *g, double dt, const double *T, const double *p
 #endif
     for (int c = 0; c < ncol; ++c) {
         double o3t[256], colo2[256], colno[256];
-        double q[NBCON + 1], h[NCM + 1];
+        double q[NBCON + 1], h[NCM + 1], gq[NBCON + 1], gh[NCM + 1];
         if (nlev > 256) { bad++; continue; }
         rbo_column(nlev, p + c, T + c, hc + 4 * field + c, hc + 7 * field + c, (int)plane, o3t,
                    colo2, colno);
@@ -1269,7 +1328,7 @@ int rbo_chem_step(const rbo_grid *g, double dt, const double *T, const double *p
             size_t b = (size_t)k * plane + c;
             boxenv_t e;
             double surf = 0., vsed, p2o, pno3;
-            int sw;
+            int sw, it;
             e.t = T[b];
             e.pm = p[b];
             e.sza = sza[b];
@@ -1281,7 +1340,13 @@ int rbo_chem_step(const rbo_grid *g, double dt, const double *T, const double *p
             e.het_level = (k + 1 >= g->nivhau && k + 1 <= g->nivbas);
             for (int n = 1; n <= NBCON; ++n) q[n] = qj1[(size_t)(n - 1) * field + b];
             for (int n = 1; n <= NCM; ++n) h[n] = hc[(size_t)(n - 1) * field + b];
-            box_step(&K, ajl, o3up, &e, dt, g->niter, q, h, &surf, &vsed, &p2o, &pno3, &sw);
+            if (guess_qj1) {
+                for (int n = 1; n <= NBCON; ++n) gq[n] = guess_qj1[(size_t)(n - 1) * field + b];
+                for (int n = 1; n <= NCM; ++n) gh[n] = guess_hc[(size_t)(n - 1) * field + b];
+            }
+            box_step(&K, ajl, o3up, &e, dt, g->niter, q, h, &surf, &vsed, &p2o, &pno3, &sw, tol,
+                     &it, guess_qj1 ? gq : NULL, guess_qj1 ? gh : NULL);
+            if (iters) iters[b] = it;
             for (int n = 1; n <= NBCON; ++n) qj1[(size_t)(n - 1) * field + b] = q[n];
             for (int n = 1; n <= NCM; ++n) hc[(size_t)(n - 1) * field + b] = h[n];
             if (sw) surfarea[b] = surf;

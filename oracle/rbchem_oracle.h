@@ -31,6 +31,16 @@ int rbo_chem_step(const rbo_grid *g, double dt, const double *T, const double *p
                   const double *o3up, double *qj1, double *hc, double *surfarea,
                   double *vsed3d, double *parth2osed, double *parthno3sed, int nthreads);
 
+/* Converged mode: the same per-box algorithm, sweeping until the relative change of
+ * every long- and short-lived species is below tol (values under the 1e-30 mole-fraction
+ * floor compared at the floor) or g->niter sweeps; iters[box] receives the count.
+ * guess_qj1/guess_hc (state layout, or both NULL) replace the initial iterate. */
+int rbo_chem_converge(const rbo_grid *g, double dt, double tol, const double *T, const double *p,
+                      const double *sza, const double *alt, const double *ajl, const double *o3up,
+                      double *qj1, double *hc, double *surfarea, double *vsed3d,
+                      double *parth2osed, double *parthno3sed, int *iters,
+                      const double *guess_qj1, const double *guess_hc, int nthreads);
+
 /* conserv (F:2526-2616) on levels 1..nivbas, then the 1e-30 floor of
  * species 1..42 on all niv levels (F:431-440). */
 void rbo_conserv_floor(const rbo_grid *g, double *qj1, double *hc, int nthreads);

@@ -18,8 +18,8 @@ ERROR_NAMES = {RB_EINVAL: "RB_EINVAL", RB_EHIP: "RB_EHIP", RB_ENOTABLE: "RB_ENOT
 
 # every symbol include/rbchem.h declares
 EXPORTS = ["rb_init", "rb_load_jtable", "rb_set_stream", "rb_chem_step", "rb_conserv_floor",
-           "rb_chem_step_dev", "rb_conserv_floor_dev", "rb_sync", "rb_get_stats",
-           "rb_last_error", "rb_finalize"]
+           "rb_chem_step_dev", "rb_conserv_floor_dev", "rb_chem_converge_dev", "rb_sync",
+           "rb_get_stats", "rb_last_error", "rb_finalize"]
 
 
 class Grid(C.Structure):
@@ -75,6 +75,7 @@ def load() -> C.CDLL:
     L.rb_conserv_floor.argtypes = [P, C.POINTER(Grid), P, P]
     L.rb_chem_step_dev.argtypes = [P, C.POINTER(Grid), d] + [P] * 10 + [C.c_int]
     L.rb_conserv_floor_dev.argtypes = [P, C.POINTER(Grid), P, P]
+    L.rb_chem_converge_dev.argtypes = [P, C.POINTER(Grid), d, d, C.c_int32] + [P] * 13 + [C.c_int]
     L.rb_sync.argtypes = [P]
     L.rb_get_stats.argtypes = [P, C.POINTER(Stats)]
     L.rb_last_error.argtypes = [P]

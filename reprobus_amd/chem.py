@@ -104,6 +104,18 @@ class ChemistryStep:
                 T, p, sza, alt, qj1, hc, surfarea, vsed3d, parth2osed, parthno3sed)],
             1 if fuse_conserv else 0))
 
+    def converge_device(self, grid: Grid, dt: float, tol: float, maxiter: int, T, p, sza, alt,
+                        qj1, hc, surfarea, vsed3d, parth2osed, parthno3sed, guess_qj1=None,
+                        guess_hc=None, iters=None, fuse_conserv: bool = True) -> None:
+        """Converged mode: sweep every box to the fixed point (optionally from a guess)."""
+        g_q = _ptr(guess_qj1) if guess_qj1 is not None else None
+        g_h = _ptr(guess_hc) if guess_hc is not None else None
+        it = iters.data_ptr() if iters is not None else None
+        self._check(self._L.rb_chem_converge_dev(
+            self._ctx, C.byref(grid), float(dt), float(tol), int(maxiter),
+            *[_ptr(a) for a in (T, p, sza, alt, qj1, hc, surfarea, vsed3d, parth2osed,
+                                parthno3sed)], g_q, g_h, it, 1 if fuse_conserv else 0))
+
     def conserv_device(self, grid: Grid, qj1, hc) -> None:
         self._check(self._L.rb_conserv_floor_dev(self._ctx, C.byref(grid), _ptr(qj1), _ptr(hc)))
 

@@ -41,11 +41,23 @@ struct StepArgs {
     int32_t nivbas, nivhau, niter, fuse_conserv;
     double dt;
     Troe K;
+    const double *guess_q, *guess_h;   // converged mode: optional initial iterate (mixing ratios)
+};
+
+struct ConvArgs {
+    double tol;                        // fixed-point tolerance (relative change per sweep)
+    int32_t maxiter;                   // sweep cap per box
+    unsigned long long *counter;       // work-queue head, zero at launch
+    int32_t *iters;                    // per-box sweep count (nbox) or null
 };
 
 Troe troe_constants();
 hipError_t launch_columns(const StepArgs &a, double *col, hipStream_t s);
 hipError_t launch_chem(const StepArgs &a, int block, hipStream_t s);
+hipError_t launch_chem_conv(const StepArgs &a, const ConvArgs &cv, int block, int nblocks,
+                            hipStream_t s);
+int conv_blocks_per_cu(int block);
+hipError_t launch_zero_counter(unsigned long long *p, hipStream_t s);
 // conserv on boxes below plane*nivbas and the floor on every box, for boxes [start, field)
 hipError_t launch_conserv_floor(double *qj1, double *hc, int64_t plane, int64_t field,
                                 int32_t nivbas, int64_t start, hipStream_t s);

@@ -637,17 +637,35 @@ struct BoxEnv {
     bool day;
 };
 
-// ck0 is the start-of-step state in LDS, one column per lane: ck0[s * CK0_STRIDE].
-template <int CK0_STRIDE>
-__device__ void sweep(const Rates &R, const double *tj, const BoxEnv &e, const double *ck0,
-                      double *ck, double *cc, double &ch3o, double &hco)
+// Running maximum of the relative change |v - old| / max(|v|, fl); a NaN propagates, so
+// a box that goes non-finite never passes the convergence test.
+__device__ __forceinline__ double track_change(double chg, double v, double old, double fl)
 {
+    const double d = fabs(v - old) / fmax(fabs(v), fl);
+    return (d <= chg) ? chg : d;
+}
+
+// ck0 is the start-of-step state in LDS, one column per lane: ck0[s * CK0_STRIDE].
+// With TRACK, chg receives the largest relative change of any long- or short-lived
+// species over this sweep (values under the floor fl compared at fl): the converged
+// mode's fixed-point test.
+template <int CK0_STRIDE, bool TRACK>
+__device__ void sweep(const Rates &R, const double *tj, const BoxEnv &e,
+                                      const double *ck0, double *ck, double *cc, double &ch3o,
+                                      double &hco, double &chg, double fl)
+{
+#define CCSET(sl, expr)                                                              \
+    do {                                                                             \
+        const double v_ = (expr);                                                    \
+        if (TRACK) chg = track_change(chg, v_, cc[sl], fl);                          \
+        cc[sl] = v_;                                                                 \
+    } while (0)
     const double hnm = e.hnm, n2 = e.n2, o2 = e.o2, co2 = e.co2, h2 = e.h2;
     double day;
     if (e.day) {
         day = 1.;
-        cc[CH3] = ((R.c2 * cc[OH] + C1A * cc[O1D] + R.d5 * cc[CL]) * ck[CH4]) /
-                  (R.c4 * o2 + C3 * cc[O3P]);
+        CCSET(CH3, ((R.c2 * cc[OH] + C1A * cc[O1D] + R.d5 * cc[CL]) * ck[CH4]) /
+                  (R.c4 * o2 + C3 * cc[O3P]));
         ch3o = (C1B * cc[O1D] * ck[CH4] +
                 (R.c5 * cc[NO] + 2. * R.c14 * ck[CH3O2] + R.c18 * cc[CLO]) * ck[CH3O2] +
                 tj[JCH3O2H] * ck[CH3O2H]) /
@@ -657,28 +675,28 @@ __device__ void sweep(const Rates &R, const double *tj, const BoxEnv &e, const d
         const double r1 = (tj[JO3_O1D] + tj[JO3_O3P] + R.a2 * cc[H] + R.a6 * cc[OH] +
                            R.a6b * cc[HO2] + R.b4 * cc[NO] + R.d2 * cc[CL]) /
                           (R.hk2 * o2);
-        cc[O3] = 1. / (1. + r1) * ck[OX];
-        cc[O3P] = r1 * cc[O3];
-        cc[O1D] = tj[JO3_O1D] / (R.hk4 * n2 + R.hk5 * o2) * cc[O3];
+        CCSET(O3, 1. / (1. + r1) * ck[OX]);
+        CCSET(O3P, r1 * cc[O3]);
+        CCSET(O1D, tj[JO3_O1D] / (R.hk4 * n2 + R.hk5 * o2) * cc[O3]);
         const double r4 = (R.a5 * cc[O3P] + R.a36 * ck[CO] + R.a19 * h2) /
                           (R.a1 * o2 + R.a2 * cc[O3] + (A23A + A23B + A23C) * cc[HO2]);
-        cc[H] = r4 * cc[OH];
+        CCSET(H, r4 * cc[OH]);
         const double r7 = tj[JNO] / (R.b7 * o2 + R.b6 * cc[NO]);
         const double r8 = (R.b9 * cc[O3]) / (tj[JNO3_NO] + tj[JNO3_NO2] + R.b12 * cc[NO2]);
         const double r5 = (tj[JNO2] + R.b3 * cc[O3P] + tj[JNO3_NO] * r8) /
                           (R.b4 * cc[O3] + R.a26 * cc[HO2] + R.d4 * cc[CLO] + R.e4 * cc[BRO] +
                            R.c5 * ck[CH3O2]);
-        cc[NO2] = ck[NOX] / (1. + r5 + r7 * r5 + r8);
-        cc[NO] = r5 * cc[NO2];
-        cc[N] = r7 * cc[NO];
-        cc[NO3] = r8 * cc[NO2];
+        CCSET(NO2, ck[NOX] / (1. + r5 + r7 * r5 + r8));
+        CCSET(NO, r5 * cc[NO2]);
+        CCSET(N, r7 * cc[NO]);
+        CCSET(NO3, r8 * cc[NO2]);
         const double r6 = (R.a6 * cc[O3] + A30 * ck[H2O2] + R.c8 * ck[CH2O] + R.d8 * cc[CLO] +
                            R.e10 * cc[BRO] + R.a1 * o2 * r4) /
                           ((A23A + A23B + A23C) * cc[H] + R.a6b * cc[O3] + R.a7 * cc[O3P] +
                            R.a26 * cc[NO] + 2. * R.a27 * cc[HO2] + R.a17 * cc[OH] +
                            R.d7 * cc[CL] + R.d10 * cc[CL] + R.d33 * cc[CLO] + R.e7 * cc[BR] +
                            R.e15 * cc[BRO]);
-        cc[HO2] = .5 * cc[HO2] + .5 * cc[OH] * r6;        // 0.5 under-relaxation
+        CCSET(HO2, .5 * cc[HO2] + .5 * cc[OH] * r6);        // 0.5 under-relaxation
         const double r13 = R.d31 * cc[NO2] /
                            (tj[JCLONO2_CL] + tj[JCLONO2_CLO] + R.d32 * cc[O3P] +
                             e.het1 * ck[H2O] + e.het2 * ck[HCL]);
@@ -691,8 +709,8 @@ __device__ void sweep(const Rates &R, const double *tj, const BoxEnv &e, const d
                             2. * tj[JCL2O2] * r10 + tj[JBRCL] * r12) /
                            (R.d2 * cc[O3] + R.d5 * ck[CH4] + R.d7 * cc[HO2] + R.d10 * cc[HO2] +
                             R.d37 * ck[HOCL]);
-        cc[CLO] = .5 * cc[CLO] + .5 * ck[CLOX] / (1. + r11);
-        cc[CL] = r11 * cc[CLO];
+        CCSET(CLO, .5 * cc[CLO] + .5 * ck[CLOX] / (1. + r11));
+        CCSET(CL, r11 * cc[CLO]);
         const double r27 = R.e13 * cc[NO2] / (tj[JBRONO2] + R.e8 * cc[O3P]);
         const double r28 = R.e15 * cc[HO2] / (tj[JHOBR] + R.e16 * cc[O3P]);
         const double r29 = R.e5c * cc[CLO] / tj[JBRCL];
@@ -700,8 +718,8 @@ __device__ void sweep(const Rates &R, const double *tj, const BoxEnv &e, const d
                             2. * R.e6 * cc[BRO] + R.e10 * cc[OH] + tj[JBRO] +
                             tj[JBRONO2] * 0.85 * r27 + tj[JHOBR] * r28 + tj[JBRCL] * r29) /
                            (R.e2 * cc[O3] + R.e7 * cc[HO2] + R.c19 * ck[CH2O]);
-        cc[BRO] = .5 * cc[BRO] + .5 * ck[BROX] / (1. + r30);
-        cc[BR] = r30 * cc[BRO];
+        CCSET(BRO, .5 * cc[BRO] + .5 * ck[BROX] / (1. + r30));
+        CCSET(BR, r30 * cc[BRO]);
         // OH from its quadratic budget (F:1600-1665)
         const double coea = 2. * A31 + 2. * (R.a17 + R.a27 * r6 + (A23B + A23C) * r4) * r6;
         const double coeb = R.b27 * ck[HNO3] * e.pno3 + R.b22 * cc[NO2] + R.b28 * ck[HO2NO2] +
@@ -720,28 +738,28 @@ __device__ void sweep(const Rates &R, const double *tj, const BoxEnv &e, const d
             R.d6 * h2 * cc[CL] + tj[JHOBR] * ck[HOBR] + R.e16 * cc[O3P] * ck[HOBR] +
             tj[JHBR] * ck[HBR];
         const double argum = fmax(coeb * coeb + 4. * coea * coec, 0.);
-        cc[OH] = .5 * cc[OH] + .5 * (sqrt(argum) - coeb) / (2. * coea);
+        CCSET(OH, .5 * cc[OH] + .5 * (sqrt(argum) - coeb) / (2. * coea));
     } else {
         // night (F:1671-1746)
         day = 0.;
-        cc[H] = 0.;
-        cc[CH3] = 0.;
+        CCSET(H, 0.);
+        CCSET(CH3, 0.);
         ch3o = 0.;
         hco = 0.;
-        cc[O1D] = 0.;
-        cc[O3P] = 0.;
-        cc[O3] = ck[OX];
-        cc[OH] = 0.;
-        cc[CL] = 0.;
-        cc[HO2] = 0.;
-        cc[NO] = 0.;
+        CCSET(O1D, 0.);
+        CCSET(O3P, 0.);
+        CCSET(O3, ck[OX]);
+        CCSET(OH, 0.);
+        CCSET(CL, 0.);
+        CCSET(HO2, 0.);
+        CCSET(NO, 0.);
         const double r8 = R.b9 * cc[O3] / (R.b12 * fmax(cc[NO2], 1.e-30));
-        cc[NO2] = ck[NOX] / (1. + r8);
-        cc[NO3] = r8 * cc[NO2];
-        cc[N] = 0.;
-        cc[CLO] = fmax(ck[CLOX], 0.);
-        cc[BR] = 0.;
-        cc[BRO] = fmax(ck[BROX], 0.);
+        CCSET(NO2, ck[NOX] / (1. + r8));
+        CCSET(NO3, r8 * cc[NO2]);
+        CCSET(N, 0.);
+        CCSET(CLO, fmax(ck[CLOX], 0.));
+        CCSET(BR, 0.);
+        CCSET(BRO, fmax(ck[BROX], 0.));
     }
 
     // production P and loss L (F:1758-2358), then ck = (ck0 + dt P)/(1 + dt L)
@@ -883,7 +901,11 @@ __device__ void sweep(const Rates &R, const double *tj, const BoxEnv &e, const d
     UPDATE(CH2BR2, 0., tj[JCH2BR2] + R.e54 * cc[OH]);
 #undef UPDATE
 #pragma unroll
-    for (int s = 0; s < NLONG - 1; ++s) ck[s] = nk[s];
+    for (int s = 0; s < NLONG - 1; ++s) {
+        if (TRACK) chg = track_change(chg, nk[s], ck[s], fl);
+        ck[s] = nk[s];
+    }
+#undef CCSET
 }
 
 // ------------------------------------------------------------------ kernels
@@ -920,6 +942,120 @@ __global__ void k_columns(const double *__restrict__ p, const double *__restrict
     }
 }
 
+// Per-box state between set-up, sweeps and write-back.
+struct Box {
+    BoxEnv e;
+    Rates R;
+    double tj[NJ];
+    double ck[NLONG], cc[NSHORT];
+    double ch3o, hco;
+    Het h;
+    bool het_level;
+};
+
+// Densities, J-values, PSC/heterogeneous rates, rain-out and thermal rates of box b
+// (F:800-1338); the start-of-step state goes to the LDS column ck0 (stride BLK).  With
+// a guess, the iterate starts from it instead of the start-of-step state.
+template <int BLK, bool GUESS>
+__device__ __forceinline__ void box_setup(const StepArgs &a, int64_t b, Box &x, double *ck0)
+{
+    const int64_t c = b % a.plane;
+    const int k = (int)(b / a.plane);
+    const double t = a.T[b], pm = a.p[b], sza = a.sza[b];
+    BoxEnv &e = x.e;
+    e.hnm = pm / (t * 1.38e-19);
+    e.n2 = 0.78 * e.hnm;
+    e.o2 = 0.21 * e.hnm;
+    e.co2 = 360.e-6 * e.hnm;
+    e.h2 = 5.e-7 * e.hnm;
+    e.dt = a.dt;
+    e.day = sza <= 95.;
+#pragma unroll
+    for (int s = 0; s < NLONG; ++s) {
+        x.ck[s] = a.qj1[(int64_t)s * a.field + b] * e.hnm;
+        if (s < NLONG - 1) ck0[s * BLK] = x.ck[s];
+    }
+#pragma unroll
+    for (int s = 0; s < NSHORT; ++s) x.cc[s] = a.hc[(int64_t)s * a.field + b] * e.hnm;
+
+    table_j(a.ajl, a.o3up, a.alt[b], sza, a.col[b], x.tj);
+    x.tj[JNO] = j_no(a.col[a.nbox + b], a.col[2 * a.nbox + b], a.col[b], e.hnm, a.sza[c]);
+
+    Het &h = x.h;
+    x.het_level = (k + 1 >= a.nivhau) && (k + 1 <= a.nivbas);
+    if (x.het_level) {
+        psc_and_het(t, pm, e.hnm, x.ck, h);
+    } else {
+#pragma unroll
+        for (int i = 0; i < 9; ++i) h.k[i] = 0.;
+        h.pno3 = h.pno3sl = h.ph2o = h.phcl = 1.;
+        h.vsed = 0.;
+    }
+    e.het1 = h.k[0]; e.het2 = h.k[1]; e.het3 = h.k[2]; e.het4 = h.k[3]; e.het5 = h.k[4];
+    e.het6 = h.k[5]; e.het7 = h.k[6]; e.het8 = h.k[7]; e.het9 = h.k[8];
+    e.pno3 = h.pno3;
+    e.ph2o = h.ph2o;
+    e.phcl = h.phcl;
+    e.pbr = 1.;
+    {
+        // rain-out above 100 ppmv water (F:1315-1338)
+        const bool wet = x.ck[H2O] / e.hnm >= 100.e-6;
+        const double rr = wet ? 1.e-6 : 0.;
+        e.rain_hno3 = rr; e.rain_hcl = rr; e.rain_hbr = rr; e.rain_h2o2 = rr;
+        e.rain_hno4 = 0.; e.rain_ch2o = 0.; e.rain_ch3o2h = 0.;
+    }
+    thermal_rates(a.K, t, e.hnm, pm, x.R);
+    x.ch3o = 0.;
+    x.hco = 0.;
+    if (GUESS && a.guess_q) {
+        // initial iterate of the converged mode (mixing ratios, state layout)
+#pragma unroll
+        for (int s = 0; s < NLONG - 1; ++s) x.ck[s] = a.guess_q[(int64_t)s * a.field + b] * e.hnm;
+#pragma unroll
+        for (int s = 0; s < NSHORT; ++s) x.cc[s] = a.guess_h[(int64_t)s * a.field + b] * e.hnm;
+    }
+}
+
+// Write-back (F:2380-2410), optionally conserv (F:2541-2598) and the floor (F:431-440).
+__device__ __forceinline__ void box_writeback(const StepArgs &a, int64_t b, const Box &x)
+{
+    const double hnm = x.e.hnm;
+    double q[NLONG], hs[NSHORT];
+#pragma unroll
+    for (int s = 0; s < NLONG - 1; ++s) q[s] = x.ck[s] / hnm;
+    q[HNO3G] = x.ck[HNO3] * x.e.pno3 / hnm;
+#pragma unroll
+    for (int s = 0; s < NSHORT; ++s) hs[s] = x.cc[s] / hnm;
+    if (a.fuse_conserv) {
+        const double rcly = (q[CLY] - q[BRCL]) /
+                            (q[OCLO] + q[HCL] + q[CLONO2] + q[HOCL] + 2. * q[CL2] + q[CLNO2] +
+                             q[CLOX] + 2. * q[CL2O2]);
+        hs[CL] *= rcly; hs[CLO] *= rcly;
+        q[OCLO] *= rcly; q[HCL] *= rcly; q[CLONO2] *= rcly; q[HOCL] *= rcly; q[CL2] *= rcly;
+        q[CLNO2] *= rcly; q[CLOX] *= rcly; q[CL2O2] *= rcly;
+        const double rnoy = fmax((q[NOY] - q[CLONO2] - q[CLNO2] - q[BRONO2]), 0.) /
+                            (q[HNO3] + 2. * q[N2O5] + q[NOX] + q[HO2NO2]);
+        hs[NO2] *= rnoy; hs[NO] *= rnoy; hs[N] *= rnoy; hs[NO3] *= rnoy;
+        q[HNO3] *= rnoy; q[N2O5] *= rnoy; q[NOX] *= rnoy; q[HO2NO2] *= rnoy;
+        const double rbry = q[BRY] / (q[HBR] + q[BRONO2] + q[BROX] + q[HOBR] + q[BRCL]);
+        hs[BR] *= rbry; hs[BRO] *= rbry;
+        q[HBR] *= rbry; q[BRONO2] *= rbry; q[BROX] *= rbry; q[HOBR] *= rbry; q[BRCL] *= rbry;
+#pragma unroll
+        for (int s = 0; s < NLONG - 1; ++s) q[s] = fmax(q[s], 1.e-30);
+    }
+#pragma unroll
+    for (int s = 0; s < NLONG; ++s) a.qj1[(int64_t)s * a.field + b] = q[s];
+#pragma unroll
+    for (int s = 0; s < NSHORT; ++s) a.hc[(int64_t)s * a.field + b] = hs[s];
+    if (x.het_level) a.surfarea[b] = x.h.surf;
+    a.vsed[b] = x.h.vsed;
+    a.p2o[b] = x.h.ph2o;
+    a.pno3[b] = x.h.pno3sl;
+}
+
+// Fixed-sweep step (the reference scheme, F:1348 niter = 5): one lane per box.  Kept
+// as one body: factoring it into the shared set-up and write-back adds ~1 KB of scratch
+// per lane and costs about half again in time.
 template <int BLK>
 __global__ void __launch_bounds__(BLK) k_chem(StepArgs a)
 {
@@ -982,11 +1118,11 @@ __global__ void __launch_bounds__(BLK) k_chem(StepArgs a)
     Rates R;
     thermal_rates(a.K, t, e.hnm, pm, R);
 
-    double ch3o = 0., hco = 0.;
+    double ch3o = 0., hco = 0., nochg = 0.;
     for (int it = 0; it < a.niter; ++it) {
         // keeps the LDS reads of ck0 inside the loop (not hoisted into registers)
         asm volatile("" ::: "memory");
-        sweep<BLK>(R, tj, e, ck0, ck, cc, ch3o, hco);
+        sweep<BLK, false>(R, tj, e, ck0, ck, cc, ch3o, hco, nochg, 0.);
     }
 
     // write-back (F:2380-2410)
@@ -1023,6 +1159,57 @@ __global__ void __launch_bounds__(BLK) k_chem(StepArgs a)
     a.vsed[b] = h.vsed;
     a.p2o[b] = h.ph2o;
     a.pno3[b] = h.pno3sl;
+}
+
+// Converged mode: sweeps to the fixed point of the sweep map.  Persistent: each lane
+// takes boxes from a device work queue (one atomic per wave for all lanes that need
+// work), sweeps its box until the largest relative change of any species falls under
+// tol (or maxiter sweeps), writes it back and takes the next box, so a wave never waits
+// for its slowest box.  Every lane leaves once the queue is exhausted, so every wave
+// reaches the exit.
+template <int BLK>
+__global__ void __launch_bounds__(BLK) k_chem_conv(StepArgs a, ConvArgs cv)
+{
+    __shared__ double s_ck0[(NLONG - 1) * BLK];
+    double *ck0 = s_ck0 + threadIdx.x;
+    const int lane = threadIdx.x & (warpSize - 1);
+    bool active = false, finished = false;
+    int64_t b = 0;
+    int it = 0;
+    Box x;
+    for (;;) {
+        const unsigned long long need = __ballot(!active && !finished);
+        if (need) {
+            const int leader = __ffsll((long long)need) - 1;
+            unsigned long long base = 0;
+            if (lane == leader) base = atomicAdd(cv.counter, (unsigned long long)__popcll(need));
+            base = __shfl(base, leader);
+            if (!active && !finished) {
+                const unsigned long long below = need & ((1ull << lane) - 1ull);
+                b = (int64_t)(base + (unsigned long long)__popcll(below));
+                if (b < a.nbox) {
+                    box_setup<BLK, true>(a, b, x, ck0);
+                    active = true;
+                    it = 0;
+                } else {
+                    finished = true;
+                }
+            }
+        }
+        if (__ballot(active) == 0ull) break;
+        if (active) {
+            asm volatile("" ::: "memory");
+            double chg = 0.;
+            sweep<BLK, true>(x.R, x.tj, x.e, ck0, x.ck, x.cc, x.ch3o, x.hco, chg,
+                             1.e-30 * x.e.hnm);
+            ++it;
+            if (chg < cv.tol || it >= cv.maxiter) {
+                box_writeback(a, b, x);
+                if (cv.iters) cv.iters[b] = it;
+                active = false;
+            }
+        }
+    }
 }
 
 __global__ void k_conserv_floor(double *__restrict__ qj1, double *__restrict__ hc, int64_t plane,
@@ -1110,6 +1297,37 @@ hipError_t launch_chem(const StepArgs &a, int block, hipStream_t s)
     case 256: hipLaunchKernelGGL(k_chem<256>, dim3((unsigned)grid), dim3(256), 0, s, a); break;
     default: return hipErrorInvalidConfiguration;
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_chem_conv(const StepArgs &a, const ConvArgs &cv, int block, int nblocks,
+                            hipStream_t s)
+{
+    const int blk = block > 0 ? block : 256;
+    switch (blk) {
+    case 64: hipLaunchKernelGGL(k_chem_conv<64>, dim3(nblocks), dim3(64), 0, s, a, cv); break;
+    case 128: hipLaunchKernelGGL(k_chem_conv<128>, dim3(nblocks), dim3(128), 0, s, a, cv); break;
+    case 256: hipLaunchKernelGGL(k_chem_conv<256>, dim3(nblocks), dim3(256), 0, s, a, cv); break;
+    default: return hipErrorInvalidConfiguration;
+    }
+    return hipGetLastError();
+}
+
+int conv_blocks_per_cu(int block)
+{
+    int n = 0;
+    hipError_t e = hipErrorInvalidValue;
+    if (block == 64) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_chem_conv<64>, 64, 0);
+    if (block == 128) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_chem_conv<128>, 128, 0);
+    if (block == 256) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_chem_conv<256>, 256, 0);
+    return (e == hipSuccess && n > 0) ? n : 1;
+}
+
+__global__ void k_zero_u64(unsigned long long *p) { *p = 0ull; }
+
+hipError_t launch_zero_counter(unsigned long long *p, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, s, p);
     return hipGetLastError();
 }
 

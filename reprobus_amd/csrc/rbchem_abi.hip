@@ -26,6 +26,8 @@ struct rb_ctx {
     double *d_stage = nullptr;        // staging for the host-array entry points
     size_t stage_cap = 0;
     unsigned long long *d_count = nullptr;
+    unsigned long long *d_queue = nullptr;   // converged-mode work-queue head
+    int n_cu = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_h2d = nullptr, ev_d2h = nullptr;
     bool timed = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> prof;   // profile-mode event pairs
@@ -115,6 +117,9 @@ int rb_init(rb_ctx **out, int device, const rb_config *cfg)
     ctx->K = rb::troe_constants();
     if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&ctx->own) != hipSuccess ||
         hipMalloc((void **)&ctx->d_count, sizeof(unsigned long long)) != hipSuccess ||
+        hipMalloc((void **)&ctx->d_queue, sizeof(unsigned long long)) != hipSuccess ||
+        hipDeviceGetAttribute(&ctx->n_cu, hipDeviceAttributeMultiprocessorCount, device) !=
+            hipSuccess ||
         hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess ||
         hipEventCreate(&ctx->ev_h2d) != hipSuccess || hipEventCreate(&ctx->ev_d2h) != hipSuccess) {
         rb_finalize(ctx);
@@ -209,6 +214,69 @@ int rb_chem_step_dev(rb_ctx *ctx, const rb_grid *g, double dt, const double *T, 
         ctx->stats.nonfinite = (int64_t)cnt;
         if (cnt) return fail(ctx, RB_ENONFINITE, std::to_string(cnt) + " non-finite outputs");
     }
+    return RB_OK;
+}
+
+int rb_chem_converge_dev(rb_ctx *ctx, const rb_grid *g, double dt, double tol, int32_t maxiter,
+                         const double *T, const double *p, const double *sza, const double *alt,
+                         double *qj1, double *hc, double *surfarea, double *vsed3d,
+                         double *parth2osed, double *parthno3sed, const double *guess_qj1,
+                         const double *guess_hc, int32_t *iters, int fuse_conserv)
+{
+    if (!ctx) return RB_EINVAL;
+    int rc = check_grid(ctx, g);
+    if (rc) return rc;
+    if (!ctx->table) return fail(ctx, RB_ENOTABLE, "rb_load_jtable has not been called");
+    if (any_null({T, p, sza, alt, qj1, hc, surfarea, vsed3d, parth2osed, parthno3sed}))
+        return fail(ctx, RB_EINVAL, "a device array is NULL");
+    if ((guess_qj1 == nullptr) != (guess_hc == nullptr))
+        return fail(ctx, RB_EINVAL, "guess_qj1 and guess_hc must be both set or both NULL");
+    if (!(dt > 0.) || !std::isfinite(dt)) return fail(ctx, RB_EINVAL, "dt must be positive");
+    if (!(tol > 0.) || maxiter < 1) return fail(ctx, RB_EINVAL, "tol must be > 0, maxiter >= 1");
+    HIPCHK(hipSetDevice(ctx->device));
+    const int64_t plane = (int64_t)g->nlon * g->nlat;
+    const int64_t field = plane * g->niv;
+    const int64_t nbox = plane * g->nivbas;
+    if ((rc = ensure(ctx, &ctx->d_col, &ctx->col_cap, (size_t)(3 * nbox)))) return rc;
+
+    rb::StepArgs a{};
+    a.T = T; a.p = p; a.sza = sza; a.alt = alt; a.col = ctx->d_col;
+    a.qj1 = qj1; a.hc = hc; a.surfarea = surfarea;
+    a.vsed = vsed3d; a.p2o = parth2osed; a.pno3 = parthno3sed;
+    a.ajl = ctx->d_ajl; a.o3up = ctx->d_o3up;
+    a.plane = plane; a.field = field; a.nbox = nbox;
+    a.nivbas = g->nivbas; a.nivhau = g->nivhau; a.niter = maxiter;
+    a.fuse_conserv = fuse_conserv ? 1 : 0;
+    a.dt = dt;
+    a.K = ctx->K;
+    a.guess_q = guess_qj1;
+    a.guess_h = guess_hc;
+    rb::ConvArgs cv{tol, maxiter, ctx->d_queue, iters};
+    // one resident wave set: blocks per CU from the occupancy calculator
+    const int nblocks = ctx->n_cu * rb::conv_blocks_per_cu(ctx->cfg.block);
+
+    HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
+    HIPCHK(rb::launch_columns(a, ctx->d_col, ctx->stream));
+    HIPCHK(rb::launch_zero_counter(ctx->d_queue, ctx->stream));
+    if (ctx->cfg.profile) {
+        if (ctx->prof_used == ctx->prof.size()) {
+            hipEvent_t e0, e1;
+            HIPCHK(hipEventCreate(&e0));
+            HIPCHK(hipEventCreate(&e1));
+            ctx->prof.emplace_back(e0, e1);
+        }
+        auto &pe = ctx->prof[ctx->prof_used++];
+        HIPCHK(hipEventRecord(pe.first, ctx->stream));
+        HIPCHK(rb::launch_chem_conv(a, cv, ctx->cfg.block, nblocks, ctx->stream));
+        HIPCHK(hipEventRecord(pe.second, ctx->stream));
+    } else {
+        HIPCHK(rb::launch_chem_conv(a, cv, ctx->cfg.block, nblocks, ctx->stream));
+    }
+    if (fuse_conserv && field > nbox)
+        HIPCHK(rb::launch_conserv_floor(qj1, hc, plane, field, g->nivbas, nbox, ctx->stream));
+    HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
+    ctx->timed = true;
+    ctx->stats.boxes = nbox;
     return RB_OK;
 }
 
@@ -330,6 +398,7 @@ int rb_finalize(rb_ctx *ctx)
     for (double *b : {ctx->d_ajl, ctx->d_o3up, ctx->d_col, ctx->d_stage})
         if (b) (void)hipFree(b);
     if (ctx->d_count) (void)hipFree(ctx->d_count);
+    if (ctx->d_queue) (void)hipFree(ctx->d_queue);
     for (hipEvent_t e : {ctx->ev0, ctx->ev1, ctx->ev_h2d, ctx->ev_d2h})
         if (e) (void)hipEventDestroy(e);
     for (auto &pe : ctx->prof) {

@@ -1,0 +1,133 @@
+"""GPU converged mode (the same per-box algorithm swept to its fixed point) against the
+restatement's converged mode, and the initial-guess entry.
+
+Both sides stop a box at the first sweep whose largest relative change of any species
+(values under the 1e-30 mole-fraction floor compared at the floor) is below tol, so they
+take the same sweep counts; where last-bit differences move the stop by one sweep, the
+extra sweep changes no species by more than tol.  Bar: 1e-9 relative per species above
+the floor, as for the 5-sweep step.
+"""
+import numpy as np
+import pytest
+
+from conftest import PARITY_TOL, max_rel_by_species, rel_err
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-10
+MAXITER = 5000
+
+
+@pytest.fixture(scope="module")
+def step(table):
+    from reprobus_amd.chem import ChemistryStep
+
+    s = ChemistryStep(*table, device=0)
+    yield s
+    s.close()
+
+
+def _dev(inp):
+    import torch
+
+    dev = torch.device("cuda:0")
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).to(dev)
+    g = inp.grid
+    d = {"T": t(inp.T), "p": t(inp.p), "sza": t(inp.sza), "alt": t(inp.alt), "qj1": t(inp.qj1),
+         "hc": t(inp.hc), "surfarea": torch.zeros(g.shape3, dtype=torch.float64, device=dev)}
+    shb = (g.nivbas, g.nlat, g.nlon)
+    for k in ("vsed3d", "parth2osed", "parthno3sed"):
+        d[k] = torch.zeros(shb, dtype=torch.float64, device=dev)
+    d["iters"] = torch.zeros(shb, dtype=torch.int32, device=dev)
+    return d
+
+
+def converge_gpu(step, inp, tol=TOL, maxiter=MAXITER, guess=None, fuse=True):
+    import torch
+    from reprobus_amd.chem import make_grid
+
+    g = inp.grid
+    grid = make_grid(g.nlon, g.nlat, g.niv, g.nivbas, g.nivhau)
+    d = _dev(inp)
+    gq = gh = None
+    if guess is not None:
+        gq = torch.from_numpy(np.ascontiguousarray(guess[0])).to("cuda:0")
+        gh = torch.from_numpy(np.ascontiguousarray(guess[1])).to("cuda:0")
+    step.set_stream(torch.cuda.current_stream().cuda_stream)
+    step.converge_device(grid, inp.dt, tol, maxiter, d["T"], d["p"], d["sza"], d["alt"], d["qj1"],
+                         d["hc"], d["surfarea"], d["vsed3d"], d["parth2osed"], d["parthno3sed"],
+                         guess_qj1=gq, guess_hc=gh, iters=d["iters"], fuse_conserv=fuse)
+    torch.cuda.synchronize()
+    step.set_stream(None)
+    return {k: v.cpu().numpy() for k, v in d.items()}
+
+
+def _worst(out, ref):
+    return max(float(max_rel_by_species(out[k], ref[k]).max()) for k in ("qj1", "hc"))
+
+
+@pytest.mark.parametrize("cr_psc", [True, False])
+def test_converged_matches_restatement(step, table, cr_psc):
+    import cport
+    from reprobus_amd import synth
+
+    inp = synth.make_c1(seed=1, nlon=24, nlat=13)
+    out = converge_gpu(step, inp)
+    ref = cport.chem_converge(inp, *table, tol=TOL, maxiter=MAXITER, cr_psc=cr_psc)
+    it, rit = out["iters"], ref["iters"]
+    same = float((it == rit).mean())
+    w = _worst(out, ref)
+    print(f"[converge] cr_psc={cr_psc}: sweeps median {np.median(it):.0f} p99 "
+          f"{np.percentile(it, 99):.0f} max {it.max()}, same count {same:.4%}, "
+          f"max |dcount| {int(np.abs(it - rit).max())}, worst rel {w:.2e}", flush=True)
+    assert it.min() >= 1 and it.max() <= MAXITER
+    assert same > 0.999
+    if cr_psc:
+        assert w < PARITY_TOL
+    else:
+        # the C-library variant: only the ill-conditioned ternary PSC boxes may differ
+        d = np.maximum(rel_err(out["qj1"], ref["qj1"]).max(axis=0),
+                       rel_err(out["hc"], ref["hc"]).max(axis=0))
+        over = d > PARITY_TOL
+        assert over.sum() <= 1e-3 * d.size and np.all(inp.T[over] < 215.0)
+
+
+def test_converged_from_guess(step, table):
+    """A perturbed fixed point as the initial iterate (the learned-guess entry): the GPU
+    matches the restatement started from the same guess, converges back to the fixed point
+    reached from the incoming state, and needs fewer sweeps.  Raw chemistry output (no
+    conserv), since conserv moves the state off the fixed point."""
+    import cport
+    from reprobus_amd import synth
+
+    inp = synth.make_c1(seed=3, nlon=16, nlat=9)
+    tol = 1e-12
+    base = converge_gpu(step, inp, tol=tol, fuse=False)
+    rng = np.random.default_rng(0)
+    gq = base["qj1"] * np.exp(rng.normal(0, 1e-3, base["qj1"].shape))
+    gh = base["hc"] * np.exp(rng.normal(0, 1e-3, base["hc"].shape))
+    out = converge_gpu(step, inp, tol=tol, guess=(gq, gh), fuse=False)
+    ref = cport.chem_converge(inp, *table, tol=tol, maxiter=MAXITER, conserv=False, cr_psc=True,
+                              guess=(gq, gh))
+    w_ref = max(float(max_rel_by_species(out[k], ref[k + "_chem"]).max()) for k in ("qj1", "hc"))
+    same = float((out["iters"] == ref["iters"]).mean())
+    live = base["iters"] < MAXITER
+    d = np.maximum(rel_err(out["qj1"], base["qj1"]).max(axis=0),
+                   rel_err(out["hc"], base["hc"]).max(axis=0))
+    print(f"[converge] guess: vs restatement {w_ref:.2e} (same count {same:.4%}); vs unguessed "
+          f"fixed point {float(d[live].max()):.2e}; mean sweeps {base['iters'][live].mean():.1f} -> "
+          f"{out['iters'][live].mean():.1f}", flush=True)
+    assert w_ref < PARITY_TOL and same > 0.99
+    assert d[live].max() < PARITY_TOL
+    assert out["iters"][live].mean() < base["iters"][live].mean()
+
+
+def test_converged_rejects_bad_arguments(step):
+    from reprobus_amd._lib import RbError
+    from reprobus_amd import synth
+
+    inp = synth.make_c1(seed=3, nlon=4, nlat=3)
+    with pytest.raises(RbError):
+        converge_gpu(step, inp, tol=0.0)
+    with pytest.raises(RbError):
+        converge_gpu(step, inp, maxiter=0)

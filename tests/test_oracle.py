@@ -235,3 +235,47 @@ def test_library_rounding_matters_only_in_ternary_boxes(table):
     d = np.maximum(rel_err(a["qj1"], b["qj1"]).max(axis=0), rel_err(a["hc"], b["hc"]).max(axis=0))
     warm = inp.T >= 215.0
     assert d[warm].max() < 1e-12, d[warm].max()
+
+
+def test_converged_mode_is_the_step_with_niter_raised(table):
+    """Converged mode with a tolerance only an exact floating-point fixed point meets is
+    the 5-sweep step with niter raised (bit for bit; a box that stops early has reached a
+    state the sweep maps to itself), and the count 5 reproduces the reference step."""
+    import cport
+    from reprobus_amd import synth
+
+    inp = synth.make_c1(seed=4, nlon=6, nlat=5)
+    for n in (5, 17):
+        step = cport.chem_step(inp, *table, niter=n)
+        conv = cport.chem_converge(inp, *table, tol=1e-300, maxiter=n)
+        assert conv["iters"].max() == n
+        for k in ("qj1", "hc", "surfarea", "vsed3d", "parth2osed", "parthno3sed"):
+            assert np.array_equal(conv[k], step[k]), (n, k)
+
+
+def test_converged_state_is_a_fixed_point(table):
+    """Started from its own converged state, the sweep stops after one sweep and moves no
+    species by more than the tolerance; started from a perturbed state it returns."""
+    import cport
+    from reprobus_amd import synth
+
+    inp = synth.make_c1(seed=5, nlon=6, nlat=5)
+    tol = 1e-12
+    base = cport.chem_converge(inp, *table, tol=tol, maxiter=20000, conserv=False)
+    live = base["iters"] < 20000
+    assert live.mean() > 0.98
+    again = cport.chem_converge(inp, *table, tol=tol, maxiter=20000, conserv=False,
+                                guess=(base["qj1_chem"], base["hc_chem"]))
+    assert np.all(again["iters"][live] == 1)
+    for k in ("qj1_chem", "hc_chem"):
+        d = rel_err(again[k][:42] if k == "qj1_chem" else again[k],
+                    base[k][:42] if k == "qj1_chem" else base[k], floor=1e-30)
+        assert d[:, live].max() < tol
+    rng = np.random.default_rng(1)
+    gq = base["qj1_chem"] * np.exp(rng.normal(0, 1e-2, base["qj1_chem"].shape))
+    gh = base["hc_chem"] * np.exp(rng.normal(0, 1e-2, base["hc_chem"].shape))
+    back = cport.chem_converge(inp, *table, tol=tol, maxiter=20000, conserv=False, guess=(gq, gh))
+    for k in ("qj1_chem", "hc_chem"):
+        d = rel_err(back[k][:42] if k == "qj1_chem" else back[k],
+                    base[k][:42] if k == "qj1_chem" else base[k], floor=1e-30)
+        assert d[:, live].max() < 1e-9, k
