@@ -1266,6 +1266,10 @@ static void box_step(const troe_t *K, const double *ajl, const double *o3up, con
                 *iters = iter;
                 break;
             }
+            if (!(worst <= 1.7976931348623157e308) || iter == niter) {
+                *iters = -iter;       /* diverged or out of sweeps */
+                break;
+            }
         }
     }
 
@@ -1346,6 +1350,15 @@ static int chem_grid(const rbo_grid *g, double dt, const double *T, const double
             }
             box_step(&K, ajl, o3up, &e, dt, g->niter, q, h, &surf, &vsed, &p2o, &pno3, &sw, tol,
                      &it, guess_qj1 ? gq : NULL, guess_qj1 ? gh : NULL);
+            if (it < 0) {
+                /* converged mode without convergence: the reference answer, RBO_NITER_REF
+                 * sweeps from the incoming state */
+                for (int n = 1; n <= NBCON; ++n) q[n] = qj1[(size_t)(n - 1) * field + b];
+                for (int n = 1; n <= NCM; ++n) h[n] = hc[(size_t)(n - 1) * field + b];
+                int it5;
+                box_step(&K, ajl, o3up, &e, dt, RBO_NITER_REF, q, h, &surf, &vsed, &p2o, &pno3,
+                         &sw, 0., &it5, NULL, NULL);
+            }
             if (iters) iters[b] = it;
             for (int n = 1; n <= NBCON; ++n) qj1[(size_t)(n - 1) * field + b] = q[n];
             for (int n = 1; n <= NCM; ++n) hc[(size_t)(n - 1) * field + b] = h[n];

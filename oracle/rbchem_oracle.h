@@ -33,8 +33,11 @@ int rbo_chem_step(const rbo_grid *g, double dt, const double *T, const double *p
 
 /* Converged mode: the same per-box algorithm, sweeping until the relative change of
  * every long- and short-lived species is below tol (values under the 1e-30 mole-fraction
- * floor compared at the floor) or g->niter sweeps; iters[box] receives the count.
+ * floor compared at the floor); iters[box] receives the count.  A box whose change goes
+ * non-finite, or that has not converged after g->niter sweeps, gets the reference answer
+ * (RBO_NITER_REF sweeps from the incoming state) and iters[box] = -(sweeps tried).
  * guess_qj1/guess_hc (state layout, or both NULL) replace the initial iterate. */
+#define RBO_NITER_REF 5
 int rbo_chem_converge(const rbo_grid *g, double dt, double tol, const double *T, const double *p,
                       const double *sza, const double *alt, const double *ajl, const double *o3up,
                       double *qj1, double *hc, double *surfarea, double *vsed3d,

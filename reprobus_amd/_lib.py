@@ -20,6 +20,11 @@ ERROR_NAMES = {RB_EINVAL: "RB_EINVAL", RB_EHIP: "RB_EHIP", RB_ENOTABLE: "RB_ENOT
 EXPORTS = ["rb_init", "rb_load_jtable", "rb_set_stream", "rb_chem_step", "rb_conserv_floor",
            "rb_chem_step_dev", "rb_conserv_floor_dev", "rb_chem_converge_dev", "rb_sync",
            "rb_get_stats", "rb_last_error", "rb_finalize"]
+# and include/rbmlp.h (the learned initial guess)
+MLP_EXPORTS = ["rb_mlp_create", "rb_mlp_destroy", "rb_mlp_set_stream", "rb_mlp_last_error",
+               "rb_mlp_param_count", "rb_mlp_set_params", "rb_mlp_get_params", "rb_mlp_set_norm",
+               "rb_mlp_forward_dev", "rb_mlp_train_step_dev", "rb_mlp_gradients_dev",
+               "rb_mlp_features_dev", "rb_mlp_guess_dev"]
 
 
 class Grid(C.Structure):
@@ -84,5 +89,25 @@ def load() -> C.CDLL:
     for name in EXPORTS:
         if name != "rb_last_error":
             getattr(L, name).restype = C.c_int
+    f = C.c_float
+    i64 = C.c_int64
+    for name in MLP_EXPORTS:
+        getattr(L, name).restype = C.c_int
+    L.rb_mlp_create.argtypes = [C.POINTER(P), C.c_int]
+    L.rb_mlp_destroy.argtypes = [P]
+    L.rb_mlp_destroy.restype = None
+    L.rb_mlp_set_stream.argtypes = [P, P]
+    L.rb_mlp_last_error.argtypes = [P]
+    L.rb_mlp_last_error.restype = C.c_char_p
+    L.rb_mlp_param_count.argtypes = []
+    L.rb_mlp_param_count.restype = i64
+    L.rb_mlp_set_params.argtypes = [P, P]
+    L.rb_mlp_get_params.argtypes = [P, P]
+    L.rb_mlp_set_norm.argtypes = [P, P, P, P, P]
+    L.rb_mlp_forward_dev.argtypes = [P, P, i64, P]
+    L.rb_mlp_train_step_dev.argtypes = [P, P, P, i64, f, C.POINTER(f)]
+    L.rb_mlp_gradients_dev.argtypes = [P, P, P, i64, P, C.POINTER(f)]
+    L.rb_mlp_features_dev.argtypes = [P, C.POINTER(Grid), d] + [P] * 6
+    L.rb_mlp_guess_dev.argtypes = [P, C.POINTER(Grid), d] + [P] * 7
     _lib = L
     return L

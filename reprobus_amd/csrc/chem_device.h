@@ -48,7 +48,9 @@ struct ConvArgs {
     double tol;                        // fixed-point tolerance (relative change per sweep)
     int32_t maxiter;                   // sweep cap per box
     unsigned long long *counter;       // work-queue head, zero at launch
-    int32_t *iters;                    // per-box sweep count (nbox) or null
+    int32_t *iters;                    // per-box sweep count (nbox) or null; -n: the box
+                                       // did not converge in n sweeps and holds the
+                                       // fixed-sweep (niter) answer instead
 };
 
 Troe troe_constants();

@@ -955,9 +955,10 @@ struct Box {
 
 // Densities, J-values, PSC/heterogeneous rates, rain-out and thermal rates of box b
 // (F:800-1338); the start-of-step state goes to the LDS column ck0 (stride BLK).  With
-// a guess, the iterate starts from it instead of the start-of-step state.
-template <int BLK, bool GUESS>
-__device__ __forceinline__ void box_setup(const StepArgs &a, int64_t b, Box &x, double *ck0)
+// use_guess and a guess, the iterate starts from it instead of the start-of-step state.
+template <int BLK>
+__device__ __forceinline__ void box_setup(const StepArgs &a, int64_t b, Box &x, double *ck0,
+                                          bool use_guess)
 {
     const int64_t c = b % a.plane;
     const int k = (int)(b / a.plane);
@@ -1007,7 +1008,7 @@ __device__ __forceinline__ void box_setup(const StepArgs &a, int64_t b, Box &x, 
     thermal_rates(a.K, t, e.hnm, pm, x.R);
     x.ch3o = 0.;
     x.hco = 0.;
-    if (GUESS && a.guess_q) {
+    if (use_guess && a.guess_q) {
         // initial iterate of the converged mode (mixing ratios, state layout)
 #pragma unroll
         for (int s = 0; s < NLONG - 1; ++s) x.ck[s] = a.guess_q[(int64_t)s * a.field + b] * e.hnm;
@@ -1164,8 +1165,10 @@ __global__ void __launch_bounds__(BLK) k_chem(StepArgs a)
 // Converged mode: sweeps to the fixed point of the sweep map.  Persistent: each lane
 // takes boxes from a device work queue (one atomic per wave for all lanes that need
 // work), sweeps its box until the largest relative change of any species falls under
-// tol (or maxiter sweeps), writes it back and takes the next box, so a wave never waits
-// for its slowest box.  Every lane leaves once the queue is exhausted, so every wave
+// tol, writes it back and takes the next box, so a wave never waits for its slowest box.
+// A box whose iteration goes non-finite or reaches maxiter sweeps without converging is
+// set up again and given the fixed-sweep answer (a.niter sweeps from the incoming state),
+// its count stored negated.  Every lane leaves once the queue is exhausted, so every wave
 // reaches the exit.
 template <int BLK>
 __global__ void __launch_bounds__(BLK) k_chem_conv(StepArgs a, ConvArgs cv)
@@ -1173,28 +1176,34 @@ __global__ void __launch_bounds__(BLK) k_chem_conv(StepArgs a, ConvArgs cv)
     __shared__ double s_ck0[(NLONG - 1) * BLK];
     double *ck0 = s_ck0 + threadIdx.x;
     const int lane = threadIdx.x & (warpSize - 1);
-    bool active = false, finished = false;
+    bool active = false, finished = false, fallback = false, restart = false;
     int64_t b = 0;
-    int it = 0;
+    int it = 0, tried = 0;
     Box x;
     for (;;) {
-        const unsigned long long need = __ballot(!active && !finished);
+        const unsigned long long need = __ballot(!active && !finished && !restart);
+        bool setup = restart;
         if (need) {
             const int leader = __ffsll((long long)need) - 1;
             unsigned long long base = 0;
             if (lane == leader) base = atomicAdd(cv.counter, (unsigned long long)__popcll(need));
             base = __shfl(base, leader);
-            if (!active && !finished) {
+            if (!active && !finished && !restart) {
                 const unsigned long long below = need & ((1ull << lane) - 1ull);
                 b = (int64_t)(base + (unsigned long long)__popcll(below));
                 if (b < a.nbox) {
-                    box_setup<BLK, true>(a, b, x, ck0);
-                    active = true;
-                    it = 0;
+                    setup = true;
+                    fallback = false;
                 } else {
                     finished = true;
                 }
             }
+        }
+        if (setup) {
+            box_setup<BLK>(a, b, x, ck0, !fallback);
+            active = true;
+            restart = false;
+            it = 0;
         }
         if (__ballot(active) == 0ull) break;
         if (active) {
@@ -1203,9 +1212,21 @@ __global__ void __launch_bounds__(BLK) k_chem_conv(StepArgs a, ConvArgs cv)
             sweep<BLK, true>(x.R, x.tj, x.e, ck0, x.ck, x.cc, x.ch3o, x.hco, chg,
                              1.e-30 * x.e.hnm);
             ++it;
-            if (chg < cv.tol || it >= cv.maxiter) {
+            if (fallback) {
+                if (it >= a.niter) {
+                    box_writeback(a, b, x);
+                    if (cv.iters) cv.iters[b] = -tried;
+                    active = false;
+                }
+            } else if (chg < cv.tol) {
                 box_writeback(a, b, x);
                 if (cv.iters) cv.iters[b] = it;
+                active = false;
+            } else if (!(chg <= 1.7976931348623157e308) || it >= cv.maxiter) {
+                // diverged (NaN or infinite change) or out of sweeps
+                tried = it;
+                fallback = true;
+                restart = true;
                 active = false;
             }
         }

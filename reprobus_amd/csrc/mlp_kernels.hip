@@ -1,0 +1,397 @@
+// Learned initial guess for the converged chemistry step: a 60 -> 256 -> 256 -> 256 -> 55
+// MLP with GELU activations.  Every GEMM runs on the gfx950 matrix cores in BF16 with FP32
+// accumulation (v_mfma_f32_16x16x32_bf16); master weights, biases, Adam moments, losses and
+// gradients are FP32.
+//
+// Layouts (row-major, batch rows): activations [n][64 or 256]; weights W_l [in][out] so a
+// forward layer is act * W; the input 60 and the output 55 are zero-padded to 64 columns.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "mlp_device.h"
+
+namespace rbm {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// -------------------------------------------------------------------- GELU (tanh form)
+__device__ __forceinline__ float gelu(float z)
+{
+    const float u = 0.7978845608028654f * (z + 0.044715f * z * z * z);
+    return 0.5f * z * (1.f + tanhf(u));
+}
+
+__device__ __forceinline__ float gelu_grad(float z)
+{
+    const float u = 0.7978845608028654f * (z + 0.044715f * z * z * z);
+    const float t = tanhf(u);
+    return 0.5f * (1.f + t) +
+           0.5f * z * (1.f - t * t) * 0.7978845608028654f * (1.f + 3.f * 0.044715f * z * z);
+}
+
+// -------------------------------------------------------------------- tiled GEMM
+// C[M x N] = A[M x K] * B[K x N]; A(m,k) = A[m*sam + k*sak], B(k,n) = B[k*sbk + n*sbn], so
+// transposed operands are strides, not copies.  A 256-thread workgroup owns a 64 x 64
+// tile of C; each of its four waves owns a 32 x 32 quarter = 2 x 2 MFMA 16x16 tiles.  The
+// K loop stages 64 x 32 slabs of A and B through LDS, k-contiguous per row, so that each
+// lane fetches its eight operand values with one 16-byte LDS read.
+constexpr int TM = 64, TN = 64, TK = 32, LDS_K = TK + 8;   // +8: spreads rows over banks
+
+__global__ void __launch_bounds__(256) k_gemm(GemmArgs g)
+{
+    __shared__ __attribute__((aligned(16))) __bf16 sA[TM * LDS_K];
+    __shared__ __attribute__((aligned(16))) __bf16 sB[TN * LDS_K];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int64_t m0 = (int64_t)blockIdx.y * TM;
+    const int n0 = blockIdx.x * TN;
+    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+
+    f32x4 acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+    const __bf16 zero = (__bf16)0.f;
+    // split-K: workgroup z sums k in [kbeg, kend) (multiples of TK)
+    const int kchunk = ((g.K + g.ksplit - 1) / g.ksplit + TK - 1) / TK * TK;
+    const int kbeg = blockIdx.z * kchunk;
+    const int kend = min(g.K, kbeg + kchunk);
+    for (int k0 = kbeg; k0 < kend; k0 += TK) {
+        // stage: 64 x 32 = 2048 values of each operand, 8 per thread
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            const int idx = tid + r * 256;
+            // A: consecutive threads walk k when A is k-contiguous, else m
+            int mm, kk;
+            if (g.sak == 1) { mm = idx >> 5; kk = idx & 31; } else { kk = idx >> 6; mm = idx & 63; }
+            const int64_t gm = m0 + mm;
+            const int gk = k0 + kk;
+            sA[mm * LDS_K + kk] = (gm < g.M && gk < kend) ? g.A[gm * g.sam + (int64_t)gk * g.sak] : zero;
+            int nn, kb;
+            if (g.sbn == 1) { kb = idx >> 6; nn = idx & 63; } else { nn = idx >> 5; kb = idx & 31; }
+            const int gn = n0 + nn;
+            const int gkb = k0 + kb;
+            sB[nn * LDS_K + kb] = (gn < g.N && gkb < kend) ? g.B[(int64_t)gkb * g.sbk + (int64_t)gn * g.sbn] : zero;
+        }
+        __syncthreads();
+        // lane l supplies row (l % 16) of its A tile and column (l % 16) of its B tile,
+        // k = 8 * (l / 16) .. + 7 of this 32-wide slab
+        const int r16 = lane & 15, kq = (lane >> 4) * 8;
+        bf16x8 va[2], vb[2];
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+            va[i] = *reinterpret_cast<const bf16x8 *>(&sA[(wm + i * 16 + r16) * LDS_K + kq]);
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+            vb[j] = *reinterpret_cast<const bf16x8 *>(&sB[(wn + j * 16 + r16) * LDS_K + kq]);
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(va[i], vb[j], acc[i][j], 0, 0, 0);
+        __syncthreads();
+    }
+
+    // epilogue: lane l holds C rows 4 * (l / 16) + e, e = 0..3, column l % 16 of each tile
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int64_t m = m0 + wm + i * 16 + (lane >> 4) * 4 + e;
+                const int n = n0 + wn + j * 16 + (lane & 15);
+                if (m >= g.M || n >= g.N) continue;
+                float v = acc[i][j][e];
+                switch (g.mode) {
+                case EPI_F32:
+                    if (g.ksplit > 1)
+                        g.part[((int64_t)blockIdx.z * g.M + m) * g.N + n] = v;
+                    else
+                        g.Cf[m * g.ldc + n] = v;
+                    break;
+                case EPI_HIDDEN_FWD: {
+                    v += g.bias[n];
+                    if (g.Cf) g.Cf[m * g.ldc + n] = v;
+                    g.Cb[m * g.ldc + n] = (__bf16)gelu(v);
+                    break;
+                }
+                case EPI_OUT_FWD:
+                    g.Cf[m * g.ldc + n] = (n < g.n_valid) ? v + g.bias[n] : 0.f;
+                    break;
+                case EPI_HIDDEN_BWD: {
+                    v *= gelu_grad(g.Z[m * g.ldc + n]);
+                    g.Cf[m * g.ldc + n] = v;
+                    g.Cb[m * g.ldc + n] = (__bf16)v;
+                    break;
+                }
+                }
+            }
+}
+
+// sums the ksplit partials (M x N each) into C (leading dimension ldc)
+__global__ void k_splitk_reduce(const float *__restrict__ part, int ksplit, int64_t M, int N,
+                                float *__restrict__ C, int64_t ldc)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= M * N) return;
+    float acc = 0.f;
+    for (int z = 0; z < ksplit; ++z) acc += part[z * M * N + i];
+    C[(i / N) * ldc + i % N] = acc;
+}
+
+hipError_t launch_gemm(const GemmArgs &gin, hipStream_t s)
+{
+    GemmArgs g = gin;
+    if (g.M <= 0 || g.N <= 0 || g.K <= 0) return hipErrorInvalidValue;
+    if (g.ksplit < 1 || g.mode != EPI_F32 || !g.part) g.ksplit = 1;
+    if (g.ksplit > MAX_KSPLIT) return hipErrorInvalidValue;
+    dim3 grid((unsigned)((g.N + TN - 1) / TN), (unsigned)((g.M + TM - 1) / TM), (unsigned)g.ksplit);
+    hipLaunchKernelGGL(k_gemm, grid, dim3(256), 0, s, g);
+    if (g.ksplit > 1)
+        hipLaunchKernelGGL(k_splitk_reduce, dim3((unsigned)((g.M * g.N + 255) / 256)), dim3(256), 0, s,
+                           g.part, g.ksplit, g.M, g.N, g.Cf, g.ldc);
+    return hipGetLastError();
+}
+
+// -------------------------------------------------------------------- pack / loss / Adam
+// Raw features (n x 60, FP32) -> normalised BF16 rows of 64 (zero pad).
+__global__ void k_pack_inputs(const float *__restrict__ X, int64_t n, const float *__restrict__ mu,
+                              const float *__restrict__ rsd, __bf16 *__restrict__ A0)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n * MLP_PAD) return;
+    const int64_t r = i / MLP_PAD;
+    const int c = (int)(i % MLP_PAD);
+    const float v = (c < MLP_NIN) ? (X[r * MLP_NIN + c] - mu[c]) * rsd[c] : 0.f;
+    A0[i] = (__bf16)v;
+}
+
+// Normalised prediction (n x 64) -> raw outputs (n x 55).
+__global__ void k_unpack_outputs(const float *__restrict__ Yn, int64_t n, const float *__restrict__ mu,
+                                 const float *__restrict__ sd, float *__restrict__ Y)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n * MLP_NOUT) return;
+    const int64_t r = i / MLP_NOUT;
+    const int c = (int)(i % MLP_NOUT);
+    Y[i] = Yn[r * MLP_PAD + c] * sd[c] + mu[c];
+}
+
+// Mean-squared error over the 55 normalised outputs: dY = 2 (y - t) / (n * 55) as BF16,
+// and the per-block partial sums of the squared error.
+__global__ void k_loss_grad(const float *__restrict__ Yn, const float *__restrict__ T, int64_t n,
+                            const float *__restrict__ mu, const float *__restrict__ rsd,
+                            __bf16 *__restrict__ dY, float *__restrict__ dYf,
+                            float *__restrict__ partial)
+{
+    __shared__ float red[256];
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    float se = 0.f;
+    if (i < n * MLP_PAD) {
+        const int64_t r = i / MLP_PAD;
+        const int c = (int)(i % MLP_PAD);
+        float d = 0.f;
+        if (c < MLP_NOUT) d = Yn[i] - (T[r * MLP_NOUT + c] - mu[c]) * rsd[c];
+        se = d * d;
+        const float gd = 2.f * d / (float)(n * MLP_NOUT);
+        dY[i] = (__bf16)gd;
+        dYf[i] = gd;
+    }
+    red[threadIdx.x] = se;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+// Column sums of an FP32 [n x ncol] matrix (bias gradients): block (x, y) sums 64
+// columns over row chunk y into part[y][ncol]; k_colsum_reduce adds the chunks in order.
+__global__ void k_colsum(const float *__restrict__ D, int64_t n, int ncol, float *__restrict__ part)
+{
+    __shared__ float red[4][64];
+    const int c = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int q = threadIdx.x >> 6;
+    const int64_t rows = (n + gridDim.y - 1) / gridDim.y;
+    const int64_t r0 = blockIdx.y * rows, r1 = min(n, r0 + rows);
+    float s = 0.f;
+    if (c < ncol)
+        for (int64_t r = r0 + q; r < r1; r += 4) s += D[r * ncol + c];
+    red[q][threadIdx.x & 63] = s;
+    __syncthreads();
+    if (q == 0 && c < ncol)
+        part[(int64_t)blockIdx.y * ncol + c] =
+            red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x];
+}
+
+__global__ void k_colsum_reduce(const float *__restrict__ part, int chunks, int ncol,
+                                float *__restrict__ out)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncol) return;
+    float s = 0.f;
+    for (int y = 0; y < chunks; ++y) s += part[(int64_t)y * ncol + c];
+    out[c] = s;
+}
+
+__global__ void k_sum(const float *__restrict__ x, int n, float *__restrict__ out)
+{
+    __shared__ float red[256];
+    float s = 0.f;
+    for (int i = threadIdx.x; i < n; i += 256) s += x[i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = red[0];
+}
+
+// Adam (Kingma & Ba 2015) on the FP32 master parameters; refreshes the BF16 copies of the
+// weight matrices (wbf maps a parameter index to its BF16 slot, -1 for biases).
+__global__ void k_adam(float *__restrict__ p, const float *__restrict__ g, float *__restrict__ m,
+                       float *__restrict__ v, int64_t n, float lr, float b1, float b2, float eps,
+                       float c1, float c2, const int32_t *__restrict__ wbf, __bf16 *__restrict__ pb)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    const float pi = p[i] - lr * (mi * c1) / (sqrtf(vi * c2) + eps);
+    p[i] = pi;
+    const int32_t j = wbf[i];
+    if (j >= 0) pb[j] = (__bf16)pi;
+}
+
+__global__ void k_to_bf16(const float *__restrict__ p, const int32_t *__restrict__ wbf, int64_t n,
+                          __bf16 *__restrict__ pb)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int32_t j = wbf[i];
+    if (j >= 0) pb[j] = (__bf16)p[i];
+}
+
+// -------------------------------------------------------------------- chemistry features
+// Feature vector of box b (FP32, 60): log mixing ratios (floored at 1e-30) of the 40
+// integrated long-lived and 15 short-lived species, T, log p, cos SZA, log H2SO4 (the
+// aerosol loading from which the liquid surface area follows), log dt.
+__global__ void k_features(const double *__restrict__ T, const double *__restrict__ p,
+                           const double *__restrict__ sza, const double *__restrict__ qj1,
+                           const double *__restrict__ hc, int64_t field, int64_t nbox, double dt,
+                           float *__restrict__ X)
+{
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nbox) return;
+    float *x = X + b * MLP_NIN;
+    for (int s = 0; s < MLP_NLONG; ++s)
+        x[s] = (float)log(fmax(qj1[(int64_t)long_slot(s) * field + b], 1.e-30));
+    for (int s = 0; s < MLP_NSHORT; ++s)
+        x[MLP_NLONG + s] = (float)log(fmax(hc[(int64_t)s * field + b], 1.e-30));
+    x[55] = (float)T[b];
+    x[56] = (float)log(p[b]);
+    x[57] = (float)cos(sza[b] * 0.017453292519943295);
+    x[58] = (float)log(fmax(qj1[(int64_t)SLOT_H2SO4 * field + b], 1.e-30));
+    x[59] = (float)log(dt);
+}
+
+// Initial iterate from the predicted log increments y (raw, n x 55):
+// guess = max(x, 1e-30) * exp(y) for the 55 species; the two inert slots and the
+// diagnostic HNO3-gas slot are copied.
+__global__ void k_apply_guess(const double *__restrict__ qj1, const double *__restrict__ hc,
+                              const float *__restrict__ Y, int64_t field, int64_t nbox,
+                              double *__restrict__ gq, double *__restrict__ gh)
+{
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nbox) return;
+    const float *y = Y + b * MLP_NOUT;
+    gq[(int64_t)SLOT_OX_PASSIVE * field + b] = qj1[(int64_t)SLOT_OX_PASSIVE * field + b];
+    gq[(int64_t)SLOT_H2SO4 * field + b] = qj1[(int64_t)SLOT_H2SO4 * field + b];
+    gq[(int64_t)SLOT_HNO3G * field + b] = qj1[(int64_t)SLOT_HNO3G * field + b];
+    for (int s = 0; s < MLP_NLONG; ++s) {
+        const int64_t o = (int64_t)long_slot(s) * field + b;
+        gq[o] = fmax(qj1[o], 1.e-30) * exp((double)y[s]);
+    }
+    for (int s = 0; s < MLP_NSHORT; ++s) {
+        const int64_t o = (int64_t)s * field + b;
+        gh[o] = fmax(hc[o], 1.e-30) * exp((double)y[MLP_NLONG + s]);
+    }
+}
+
+// -------------------------------------------------------------------- launchers
+static inline unsigned blocks(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
+
+hipError_t launch_pack_inputs(const float *X, int64_t n, const float *mu, const float *rsd,
+                              __bf16 *A0, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_pack_inputs, dim3(blocks(n * MLP_PAD, 256)), dim3(256), 0, s, X, n, mu, rsd, A0);
+    return hipGetLastError();
+}
+
+hipError_t launch_unpack_outputs(const float *Yn, int64_t n, const float *mu, const float *sd,
+                                 float *Y, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_unpack_outputs, dim3(blocks(n * MLP_NOUT, 256)), dim3(256), 0, s, Yn, n, mu, sd, Y);
+    return hipGetLastError();
+}
+
+hipError_t launch_loss_grad(const float *Yn, const float *T, int64_t n, const float *mu,
+                            const float *rsd, __bf16 *dY, float *dYf, float *partial, float *loss,
+                            hipStream_t s)
+{
+    const unsigned nb = blocks(n * MLP_PAD, 256);
+    hipLaunchKernelGGL(k_loss_grad, dim3(nb), dim3(256), 0, s, Yn, T, n, mu, rsd, dY, dYf, partial);
+    hipLaunchKernelGGL(k_sum, dim3(1), dim3(256), 0, s, partial, (int)nb, loss);
+    return hipGetLastError();
+}
+
+hipError_t launch_colsum(const float *D, int64_t n, int ncol, float *part, float *out, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_colsum, dim3(blocks(ncol, 64), COLSUM_CHUNKS), dim3(256), 0, s, D, n, ncol, part);
+    hipLaunchKernelGGL(k_colsum_reduce, dim3(blocks(ncol, 256)), dim3(256), 0, s, part, COLSUM_CHUNKS,
+                       ncol, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_adam(float *p, const float *g, float *m, float *v, int64_t n, float lr, float b1,
+                       float b2, float eps, float c1, float c2, const int32_t *wbf, __bf16 *pb,
+                       hipStream_t s)
+{
+    hipLaunchKernelGGL(k_adam, dim3(blocks(n, 256)), dim3(256), 0, s, p, g, m, v, n, lr, b1, b2, eps,
+                       c1, c2, wbf, pb);
+    return hipGetLastError();
+}
+
+hipError_t launch_to_bf16(const float *p, const int32_t *wbf, int64_t n, __bf16 *pb, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_to_bf16, dim3(blocks(n, 256)), dim3(256), 0, s, p, wbf, n, pb);
+    return hipGetLastError();
+}
+
+hipError_t launch_features(const double *T, const double *p, const double *sza, const double *qj1,
+                           const double *hc, int64_t field, int64_t nbox, double dt, float *X,
+                           hipStream_t s)
+{
+    hipLaunchKernelGGL(k_features, dim3(blocks(nbox, 256)), dim3(256), 0, s, T, p, sza, qj1, hc,
+                       field, nbox, dt, X);
+    return hipGetLastError();
+}
+
+hipError_t launch_apply_guess(const double *qj1, const double *hc, const float *Y, int64_t field,
+                              int64_t nbox, double *gq, double *gh, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_apply_guess, dim3(blocks(nbox, 256)), dim3(256), 0, s, qj1, hc, Y, field,
+                       nbox, gq, gh);
+    return hipGetLastError();
+}
+
+}  // namespace rbm

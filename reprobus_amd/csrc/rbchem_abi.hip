@@ -245,7 +245,7 @@ int rb_chem_converge_dev(rb_ctx *ctx, const rb_grid *g, double dt, double tol, i
     a.vsed = vsed3d; a.p2o = parth2osed; a.pno3 = parthno3sed;
     a.ajl = ctx->d_ajl; a.o3up = ctx->d_o3up;
     a.plane = plane; a.field = field; a.nbox = nbox;
-    a.nivbas = g->nivbas; a.nivhau = g->nivhau; a.niter = maxiter;
+    a.nivbas = g->nivbas; a.nivhau = g->nivhau; a.niter = ctx->cfg.niter;   // fallback
     a.fuse_conserv = fuse_conserv ? 1 : 0;
     a.dt = dt;
     a.K = ctx->K;

@@ -281,3 +281,36 @@ def make_c1(seed: int = 0, nlon: int = 144, nlat: int = 91, decl_deg: float = 20
     q[L["h2so4"]] = h2so4_for_sad(T, p, q[L["h2o"]], sad)
     return StepInputs(g, 900.0, T, p, sza, altitude_km(p), q, h, sad,
                       {"name": "c1", "seed": seed, "decl_deg": decl_deg, "lat": lat, "lon": lon})
+
+
+C2_STEPS = 96                 # one day of 900 s steps
+
+
+def c2_sza(meta: dict, step: int) -> np.ndarray:
+    """SZA (niv, nlat, nlon) of step `step` of the c2 day: 15 min per step from 0 UT."""
+    lat, lon = meta["lat"], meta["lon"]
+    LAT = np.broadcast_to(lat[:, None], (lat.size, lon.size))
+    LON = np.broadcast_to(lon[None, :], (lat.size, lon.size))
+    s2 = solar_zenith(LAT, LON, meta["decl_deg"], step * 0.25)
+    return np.broadcast_to(s2[None], (NLEV,) + s2.shape).copy()
+
+
+def make_c2(seed: int = 0, nlon: int = 144, nlat: int = 91, decl_deg: float = 0.0) -> StepInputs:
+    """Grid c2 at step 0 of its day: c1's state with a polar vortex poleward of 60 S on the
+    heterogeneous levels -- T ~ U(183, 195) K and liquid-PSC surface area
+    logU(1, 50) um2/cm3 (realised through H2SO4) -- and a diurnal SZA (c2_sza) at the
+    equinox (decl 0), so the vortex sees both sunlit and dark hours."""
+    rng = np.random.default_rng(seed)
+    inp = make_c1(seed, nlon, nlat, decl_deg, 0.0)
+    g = inp.grid
+    lat = inp.meta["lat"]
+    vortex = np.zeros(g.shape3, bool)
+    vortex[NIVHAU - 1:, lat <= -60.0, :] = True
+    nv = int(vortex.sum())
+    inp.T[vortex] = rng.uniform(183.0, 195.0, nv)
+    sad = inp.sad_target
+    sad[vortex] = _logu(rng, 1.0, 50.0, (nv,)) * 1e-8
+    inp.qj1[L["h2so4"]] = h2so4_for_sad(inp.T, inp.p, inp.qj1[L["h2o"]], sad)
+    inp.sza = c2_sza(inp.meta, 0)
+    inp.meta.update({"name": "c2", "vortex": vortex})
+    return inp

@@ -1,5 +1,5 @@
 """CPU checks of the C-ABI library: it loads without a GPU and exports every
-symbol include/rbchem.h declares (no compute calls here)."""
+symbol include/rbchem.h and include/rbmlp.h declare (no compute calls here)."""
 import os
 import re
 
@@ -8,15 +8,17 @@ import pytest
 from conftest import ROOT
 
 
-def declared_symbols():
-    text = open(os.path.join(ROOT, "include", "rbchem.h")).read()
-    return sorted(set(re.findall(r"^(?:int|const char \*)\s*\*?\s*(rb_\w+)\s*\(", text, re.M)))
+def declared_symbols(header="rbchem.h"):
+    text = open(os.path.join(ROOT, "include", header)).read()
+    return sorted(set(re.findall(r"^(?:int|int64_t|void|const char \*)\s*\*?\s*(rb_\w+)\s*\(",
+                                 text, re.M)))
 
 
 def test_header_declares_the_documented_entry_points():
     from reprobus_amd import _lib
 
     assert declared_symbols() == sorted(_lib.EXPORTS)
+    assert declared_symbols("rbmlp.h") == sorted(_lib.MLP_EXPORTS)
 
 
 def test_library_exports_every_declared_symbol():
@@ -25,8 +27,9 @@ def test_library_exports_every_declared_symbol():
     if not os.path.exists(_lib.LIB_PATH):
         pytest.fail("librbchem.so is not built (run __graft_entry__.build())")
     L = _lib.load()
-    for name in declared_symbols():
+    for name in declared_symbols() + declared_symbols("rbmlp.h"):
         assert hasattr(L, name), name
+    assert L.rb_mlp_param_count() == 64 * 256 + 256 + 2 * (256 * 256 + 256) + 256 * 64 + 64
 
 
 def test_struct_layouts_match_header():
