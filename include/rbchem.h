@@ -79,8 +79,9 @@ int rb_chem_step(rb_ctx *ctx, const rb_grid *g, double dt, const double *T, cons
                  const double *sza, const double *alt, double *qj1, double *hc,
                  double *surfarea, double *vsed3d, double *parth2osed, double *parthno3sed);
 
-/* conserv (F:2526-2616, levels 1..nivbas) then the floor max(q, 1e-30) of
- * species 1..42 on all niv levels (F:431-440), on host arrays. */
+/* conserv (F:2526-2616, levels 1..nivbas), the floor max(q, 1e-30) of species
+ * 1..42 on all niv levels (F:431-440), and HNO3 gas (slot 43) = HNO3 (slot 5) on levels
+ * nivbas+1..niv (F:444-450), on host arrays. */
 int rb_conserv_floor(rb_ctx *ctx, const rb_grid *g, double *qj1, double *hc);
 
 /* Device-resident variants: the same contracts on device pointers, enqueued on
@@ -92,6 +93,22 @@ int rb_chem_step_dev(rb_ctx *ctx, const rb_grid *g, double dt, const double *T, 
                      double *surfarea, double *vsed3d, double *parth2osed, double *parthno3sed,
                      int fuse_conserv);
 int rb_conserv_floor_dev(rb_ctx *ctx, const rb_grid *g, double *qj1, double *hc);
+
+/* Sedimentation (replaces sedimentation(trajpre, trajtem, irapp, qj1, vsed3d, nivh2oecmwf,
+ * parth2osed, parthno3sed, dtsedi), F:2416-2522, called at F:419-423 between the
+ * chemistry and rb_conserv_floor on the sub-steps where irapp == nrapp).  Moves H2O
+ * (qj1 slot 3, levels nivh2o-1 .. nivhau) and NOy/HNO3 (slots 4, 5, levels nivbas ..
+ * nivhau) down with the settling velocity vsed3d and the condensed fractions
+ * (1 - parth2osed), (1 - parthno3sed) from rb_chem_step, over dt (the model's dtsedi).
+ * T and p are the step's fields (levels 1..nivbas read).  Needs nivhau >= 2 and
+ * nivh2o <= nivbas + 1 (the model: nivh2oecmwf = 59).  rb_sediment takes host arrays,
+ * rb_sediment_dev device arrays (stream-ordered, no synchronisation). */
+int rb_sediment(rb_ctx *ctx, const rb_grid *g, double dt, int32_t nivh2o, const double *T,
+                const double *p, double *qj1, const double *vsed3d, const double *parth2osed,
+                const double *parthno3sed);
+int rb_sediment_dev(rb_ctx *ctx, const rb_grid *g, double dt, int32_t nivh2o, const double *T,
+                    const double *p, double *qj1, const double *vsed3d,
+                    const double *parth2osed, const double *parthno3sed);
 
 /* Converged mode (no counterpart in the reference, which always stops after niter = 5
  * sweeps, F:630): the same per-box algorithm swept to its fixed point -- until the largest

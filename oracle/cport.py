@@ -43,6 +43,8 @@ def lib(cr_psc: bool = False):
                                     dp, dp, dp, dp, C.c_int]
         L.rbo_chem_step.restype = C.c_int
         L.rbo_conserv_floor.argtypes = [C.POINTER(_Grid), dp, dp, C.c_int]
+        L.rbo_sediment.argtypes = [C.POINTER(_Grid), C.c_double, C.c_int, dp, dp, dp, dp, dp, dp]
+        L.rbo_sediment.restype = None
         L.rbo_chem_converge.argtypes = [C.POINTER(_Grid), C.c_double, C.c_double] + [dp] * 12 + [
             np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS"), C.c_void_p, C.c_void_p,
             C.c_int]
@@ -127,3 +129,15 @@ def chem_converge(inp, ajl, o3up, tol: float = 1e-10, maxiter: int = 2000, nthre
         L.rbo_conserv_floor(C.byref(G), q, h, nthreads)
         out["qj1"], out["hc"] = q, h
     return out
+
+
+def sediment(inp, qj1, vsed3d, parth2osed, parthno3sed, dt: float, nivh2o: int,
+             cr_psc: bool = False) -> np.ndarray:
+    """Sedimentation of the restatement on a copy of qj1 (state layout)."""
+    g = inp.grid
+    G = _Grid(g.nlon, g.nlat, g.niv, g.nivbas, g.nivhau, 5)
+    q = np.ascontiguousarray(qj1, dtype=np.float64).copy()
+    c = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+    lib(cr_psc).rbo_sediment(C.byref(G), float(dt), int(nivh2o), c(inp.p), c(inp.T), q, c(vsed3d),
+                             c(parth2osed), c(parthno3sed))
+    return q

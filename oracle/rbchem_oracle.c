@@ -1425,6 +1425,45 @@ void rbo_conserv_floor(const rbo_grid *g, double *qj1, double *hc, int nthreads)
 #endif
     for (size_t b = 0; b < na; ++b)
         for (int n = 1; n <= NBCON - 1; ++n) Q(n) = fmax(Q(n), 1.e-30);
+    /* HNO3 gas = HNO3 below the chemistry levels, F:444-450 */
+    for (size_t b = nb; b < na; ++b) Q(NBCON) = Q(5);
 #undef Q
 #undef H
+}
+
+/* sedimentation F:2416-2522: H2O (slot 3) on levels nivh2o-1 .. nivhau and NOy, HNO3
+ * (slots 4, 5) on nivbas .. nivhau, bottom-up, each level from the not yet updated
+ * values of itself and the level above (so the update is the same in any order). */
+void rbo_sediment(const rbo_grid *g, double dt, int nivh2o, const double *p, const double *T,
+                  double *qj1, const double *vsed3d, const double *parth2osed,
+                  const double *parthno3sed)
+{
+    const size_t plane = (size_t)g->nlon * g->nlat, field = plane * g->niv;
+    double *q3 = qj1 + 2 * field, *q4 = qj1 + 3 * field, *q5 = qj1 + 4 * field;
+    for (size_t c = 0; c < plane; ++c) {
+#define HNM(k) (p[(size_t)((k) - 1) * plane + c] / (T[(size_t)((k) - 1) * plane + c] * 1.38e-19))
+#define AT(a, k) a[(size_t)((k) - 1) * plane + c]
+        for (int k = nivh2o - 1; k >= g->nivhau; --k) {
+            double dz = 7000. * log(AT(p, k) / AT(p, k - 1));
+            double fracup = AT(vsed3d, k - 1) * dt / dz;
+            double fracdown = AT(vsed3d, k) * dt / dz;
+            fracup = fmin(fracup, 1.0);
+            fracdown = fmin(fracdown, 1.0);
+            AT(q3, k) = AT(q3, k) + AT(q3, k - 1) * (1. - AT(parth2osed, k - 1)) * HNM(k - 1) / HNM(k) * fracup -
+                        AT(q3, k) * (1. - AT(parth2osed, k)) * fracdown;
+        }
+        for (int k = g->nivbas; k >= g->nivhau; --k) {
+            double dz = 7000. * log(AT(p, k) / AT(p, k - 1));
+            double fracup = AT(vsed3d, k - 1) * dt / dz;
+            double fracdown = AT(vsed3d, k) * dt / dz;
+            fracup = fmin(fracup, 1.0);
+            fracdown = fmin(fracdown, 1.0);
+            AT(q4, k) = AT(q4, k) + AT(q5, k - 1) * (1. - AT(parthno3sed, k - 1)) * HNM(k - 1) / HNM(k) * fracup -
+                        AT(q5, k) * (1. - AT(parthno3sed, k)) * fracdown;
+            AT(q5, k) = AT(q5, k) + AT(q5, k - 1) * (1. - AT(parthno3sed, k - 1)) * HNM(k - 1) / HNM(k) * fracup -
+                        AT(q5, k) * (1. - AT(parthno3sed, k)) * fracdown;
+        }
+#undef HNM
+#undef AT
+    }
 }

@@ -45,8 +45,16 @@ int rbo_chem_converge(const rbo_grid *g, double dt, double tol, const double *T,
                       const double *guess_qj1, const double *guess_hc, int nthreads);
 
 /* conserv (F:2526-2616) on levels 1..nivbas, then the 1e-30 floor of
- * species 1..42 on all niv levels (F:431-440). */
+ * species 1..42 on all niv levels (F:431-440), then HNO3 gas = HNO3 on levels
+ * nivbas+1..niv (F:444-450). */
 void rbo_conserv_floor(const rbo_grid *g, double *qj1, double *hc, int nthreads);
+
+/* sedimentation (F:2416-2522) of H2O (levels nivh2o-1..nivhau) and NOy/HNO3 (levels
+ * nivbas..nivhau) over a step dt; vsed3d, parth2osed, parthno3sed are the nivbas-level
+ * outputs of the chemistry step; p, T as for rbo_chem_step.  Requires nivhau >= 2. */
+void rbo_sediment(const rbo_grid *g, double dt, int nivh2o, const double *p, const double *T,
+                  double *qj1, const double *vsed3d, const double *parth2osed,
+                  const double *parthno3sed);
 
 /* ---- pieces exposed for unit tests ---- */
 /* Troe prefactors of const (F:5927-5948), 18 values in COMMON order. */

@@ -7,16 +7,18 @@
 ! same sequence as the model's sub-step F:398-440 without transport forcing:
 !   const (F:294, reads jstrato.txt and relief.txt from the working directory)
 !   chem3d for each used latitude row (F:398-415)
+!   sedimentation (F:419-423; only if dtsedi > 0, which needs nlev = 94)
 !   conserv (F:427) and the 1e-30 floor (F:431-440).
 ! Unused longitudes of a row and levels below nlev replicate real columns /
 ! the lowest real level, so every box the reference touches is well defined;
 ! with equal pressures below nlev their column increments are zero.
 !
-! Input  (stream, native endian): int32 ncol, nlev, nrep; real64 daynum, dt;
+! Input  (stream, native endian): int32 ncol, nlev, nrep; real64 daynum, dt, dtsedi;
 !        then real64 arrays (ncol,nlev): trajlon, trajlat, p, T, alt;
 !        qj1 (ncol,nlev,43); hc (ncol,nlev,15).
 ! Output (stream): int32 ncol, nlev, ncalls; real64 seconds in chem3d (wall);
 !        after chem3d: qj1, hc, surfarea, sza3d, vsed3d, parth2osed, parthno3sed;
+!        if dtsedi > 0, after sedimentation: qj1;
 !        after conserv + floor: qj1, hc.
 ! With nrep > 1 the chem3d row sweep is repeated nrep times (successive steps)
 ! for timing; outputs are then those of the last step.
@@ -25,7 +27,8 @@ program ref_harness
   integer, parameter :: nlon = 180, nlat = 91, niv = 137, nivbas = 94
   integer, parameter :: nbcon = 43, ncm = 15, nrappmax = 12
   integer(4) :: ncol, nlev, nrep
-  real(8) :: daynum, dt
+  real(8) :: daynum, dt, dtsedi
+  integer, parameter :: nivh2oecmwf = 59   ! the model's parameter, F:18
   real(8), allocatable :: clon(:,:), clat(:,:), cp(:,:), ct(:,:), calt(:,:)
   real(8), allocatable :: cq(:,:,:), ch(:,:,:)
   real(8), allocatable :: qj1(:,:,:,:), hc(:,:,:,:), alt(:,:,:)
@@ -43,7 +46,11 @@ program ref_harness
   call get_command_argument(2, fout)
 
   open(10, file=trim(fin), access='stream', form='unformatted', status='old')
-  read(10) ncol, nlev, nrep, daynum, dt
+  read(10) ncol, nlev, nrep, daynum, dt, dtsedi
+  if (dtsedi > 0.d0 .and. nlev /= nivbas) then
+     print *, 'ref_harness: sedimentation needs nlev =', nivbas
+     stop 2
+  end if
   if (ncol < 1 .or. nlev < 1 .or. nlev > nivbas .or. ncol > nlon*nlat) then
      print *, 'ref_harness: bad grid', ncol, nlev
      stop 2
@@ -122,6 +129,13 @@ program ref_harness
   write(11) obuf(:,:,1)
   call gather(parthno3sed, 1, nivbas)
   write(11) obuf(:,:,1)
+
+  if (dtsedi > 0.d0) then
+     call sedimentation(trajpre, trajtem, irapp, qj1, vsed3d, nivh2oecmwf, parth2osed, &
+                        parthno3sed, dtsedi)
+     call gather(qj1, nbcon, niv)
+     write(11) obuf(:,:,1:nbcon)
+  end if
 
   call conserv(qj1, hc)
   do n = 1, nbcon - 1

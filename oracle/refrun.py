@@ -56,8 +56,10 @@ def trajectories_for_sza(sza_deg: np.ndarray, day: float = DAYNUM):
 
 
 def run_reference(inp, nrep: int = 1, day: float = DAYNUM, workdir: str | None = None,
-                  keep: bool = False) -> dict:
-    """Run the reference on ``inp`` (a synth.StepInputs with nlat*nlon <= 16380 columns)."""
+                  keep: bool = False, dtsedi: float = 0.0) -> dict:
+    """Run the reference on ``inp`` (a synth.StepInputs with nlat*nlon <= 16380 columns).
+    dtsedi > 0 also runs the reference's sedimentation (F:419-423; needs nivbas = 94) and
+    returns qj1 after it as 'qj1_sedi'."""
     from reprobus_amd import jtable
 
     if not available():
@@ -76,7 +78,7 @@ def run_reference(inp, nrep: int = 1, day: float = DAYNUM, workdir: str | None =
         fin, fout = os.path.join(run, "in.bin"), os.path.join(run, "out.bin")
         with open(fin, "wb") as f:
             np.array([ncol, nlev, nrep], dtype=np.int32).tofile(f)
-            np.array([day, inp.dt], dtype=np.float64).tofile(f)
+            np.array([day, inp.dt, dtsedi], dtype=np.float64).tofile(f)
             for a in (lon, lat, inp.p[:nlev], inp.T[:nlev], inp.alt[:nlev]):
                 np.ascontiguousarray(a, dtype=np.float64).tofile(f)
             np.ascontiguousarray(inp.qj1[:, :nlev], dtype=np.float64).tofile(f)
@@ -95,8 +97,10 @@ def run_reference(inp, nrep: int = 1, day: float = DAYNUM, workdir: str | None =
 
             out = {"ncalls": int(hdr[2]), "seconds": secs,
                    "qj1_chem": rd(43), "hc_chem": rd(15), "surfarea": rd(1), "sza3d": rd(1),
-                   "vsed3d": rd(1), "parth2osed": rd(1), "parthno3sed": rd(1),
-                   "qj1": rd(43), "hc": rd(15)}
+                   "vsed3d": rd(1), "parth2osed": rd(1), "parthno3sed": rd(1)}
+            if dtsedi > 0:
+                out["qj1_sedi"] = rd(43)
+            out["qj1"], out["hc"] = rd(43), rd(15)
         out["boxes_per_call"] = NLON_REF * 94
         return out
     finally:
@@ -128,7 +132,7 @@ def time_reference(inp, nrep: int, nproc: int, day: float = DAYNUM) -> dict:
         fin = os.path.join(run, "in.bin")
         with open(fin, "wb") as f:
             np.array([ncol, nlev, nrep], dtype=np.int32).tofile(f)
-            np.array([day, inp.dt], dtype=np.float64).tofile(f)
+            np.array([day, inp.dt, 0.0], dtype=np.float64).tofile(f)
             for a in (lon, lat, inp.p[:nlev], inp.T[:nlev], inp.alt[:nlev]):
                 np.ascontiguousarray(a, dtype=np.float64).tofile(f)
             np.ascontiguousarray(inp.qj1[:, :nlev], dtype=np.float64).tofile(f)

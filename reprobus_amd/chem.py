@@ -116,6 +116,21 @@ class ChemistryStep:
             *[_ptr(a) for a in (T, p, sza, alt, qj1, hc, surfarea, vsed3d, parth2osed,
                                 parthno3sed)], g_q, g_h, it, 1 if fuse_conserv else 0))
 
+    def sediment(self, grid: Grid, dt: float, nivh2o: int, T, p, qj1, vsed3d, parth2osed,
+                 parthno3sed) -> None:
+        """Sedimentation on host arrays (qj1 updated in place), F:2416-2522."""
+        for a in (T, p, qj1, vsed3d, parth2osed, parthno3sed):
+            assert a.dtype == np.float64 and a.flags.c_contiguous
+        self._check(self._L.rb_sediment(self._ctx, C.byref(grid), float(dt), int(nivh2o),
+                                        *[a.ctypes.data for a in (T, p, qj1, vsed3d, parth2osed,
+                                                                  parthno3sed)]))
+
+    def sediment_device(self, grid: Grid, dt: float, nivh2o: int, T, p, qj1, vsed3d, parth2osed,
+                        parthno3sed) -> None:
+        self._check(self._L.rb_sediment_dev(self._ctx, C.byref(grid), float(dt), int(nivh2o),
+                                            *[_ptr(a) for a in (T, p, qj1, vsed3d, parth2osed,
+                                                                parthno3sed)]))
+
     def conserv_device(self, grid: Grid, qj1, hc) -> None:
         self._check(self._L.rb_conserv_floor_dev(self._ctx, C.byref(grid), _ptr(qj1), _ptr(hc)))
 

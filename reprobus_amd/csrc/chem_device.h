@@ -61,6 +61,11 @@ hipError_t launch_chem_conv(const StepArgs &a, const ConvArgs &cv, int block, in
 int conv_blocks_per_cu(int block);
 hipError_t launch_zero_counter(unsigned long long *p, hipStream_t s);
 // conserv on boxes below plane*nivbas and the floor on every box, for boxes [start, field)
+// h2o, noy, hno3: the qj1 fields of those slots
+hipError_t launch_sediment(const double *p, const double *T, double *h2o, double *noy,
+                           double *hno3, const double *vsed, const double *p2o, const double *pno3,
+                           int64_t plane, int nivbas, int nivhau, int nivh2o, double dt,
+                           hipStream_t s);
 hipError_t launch_conserv_floor(double *qj1, double *hc, int64_t plane, int64_t field,
                                 int32_t nivbas, int64_t start, hipStream_t s);
 hipError_t launch_count_nonfinite(const double *x, int64_t n, unsigned long long *count,

@@ -1258,8 +1258,56 @@ __global__ void k_conserv_floor(double *__restrict__ qj1, double *__restrict__ h
     }
 #pragma unroll
     for (int s = 0; s < NLONG - 1; ++s) Q(s) = fmax(Q(s), 1.e-30);
+    // HNO3 gas = HNO3 below the chemistry levels (F:444-450)
+    if (b >= plane * (int64_t)nivbas) Q(HNO3G) = Q(HNO3);
 #undef Q
 #undef HS
+}
+
+// Sedimentation (F:2416-2522): one thread per column walks its levels bottom-up; each
+// level's update reads the not yet updated values of itself and the level above, which
+// is the reference's loop order.  Levels are 1-based as in the reference.
+__global__ void k_sediment(const double *__restrict__ p, const double *__restrict__ T,
+                           double *__restrict__ h2o, double *__restrict__ noy,
+                           double *__restrict__ hno3, const double *__restrict__ vsed,
+                           const double *__restrict__ p2o, const double *__restrict__ pno3,
+                           int64_t plane, int32_t nivbas, int32_t nivhau, int32_t nivh2o,
+                           double dt)
+{
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= plane) return;
+    double *q3 = h2o + c, *q4 = noy + c, *q5 = hno3 + c;
+    auto at = [&](int k) { return (int64_t)(k - 1) * plane; };
+    auto hnm = [&](int k) { return p[at(k) + c] / (T[at(k) + c] * 1.38e-19); };
+    for (int k = nivh2o - 1; k >= nivhau; --k) {
+        const int64_t o = at(k), u = at(k - 1);
+        const double dz = 7000. * log(p[o + c] / p[u + c]);
+        const double fracup = fmin(vsed[u + c] * dt / dz, 1.0);
+        const double fracdown = fmin(vsed[o + c] * dt / dz, 1.0);
+        q3[o] = q3[o] + q3[u] * (1. - p2o[u + c]) * hnm(k - 1) / hnm(k) * fracup -
+                q3[o] * (1. - p2o[o + c]) * fracdown;
+    }
+    for (int k = nivbas; k >= nivhau; --k) {
+        const int64_t o = at(k), u = at(k - 1);
+        const double dz = 7000. * log(p[o + c] / p[u + c]);
+        const double fracup = fmin(vsed[u + c] * dt / dz, 1.0);
+        const double fracdown = fmin(vsed[o + c] * dt / dz, 1.0);
+        const double h5u = q5[u], h5 = q5[o];
+        q4[o] = q4[o] + h5u * (1. - pno3[u + c]) * hnm(k - 1) / hnm(k) * fracup -
+                h5 * (1. - pno3[o + c]) * fracdown;
+        q5[o] = h5 + h5u * (1. - pno3[u + c]) * hnm(k - 1) / hnm(k) * fracup -
+                h5 * (1. - pno3[o + c]) * fracdown;
+    }
+}
+
+hipError_t launch_sediment(const double *p, const double *T, double *h2o, double *noy,
+                           double *hno3, const double *vsed, const double *p2o, const double *pno3,
+                           int64_t plane, int nivbas, int nivhau, int nivh2o, double dt,
+                           hipStream_t s)
+{
+    hipLaunchKernelGGL(k_sediment, dim3((unsigned)((plane + 255) / 256)), dim3(256), 0, s, p, T,
+                       h2o, noy, hno3, vsed, p2o, pno3, plane, nivbas, nivhau, nivh2o, dt);
+    return hipGetLastError();
 }
 
 __global__ void k_count_nonfinite(const double *__restrict__ x, int64_t n,

@@ -358,6 +358,65 @@ int rb_conserv_floor(rb_ctx *ctx, const rb_grid *g, double *qj1, double *hc)
     return RB_OK;
 }
 
+static int check_sediment(rb_ctx *ctx, const rb_grid *g, double dt, int32_t nivh2o)
+{
+    int rc = check_grid(ctx, g);
+    if (rc) return rc;
+    if (g->nivhau < 2) return fail(ctx, RB_EINVAL, "sedimentation needs nivhau >= 2");
+    if (nivh2o > g->nivbas + 1) return fail(ctx, RB_EINVAL, "nivh2o must be <= nivbas + 1");
+    if (!(dt >= 0.) || !std::isfinite(dt)) return fail(ctx, RB_EINVAL, "dt must be >= 0");
+    return RB_OK;
+}
+
+int rb_sediment_dev(rb_ctx *ctx, const rb_grid *g, double dt, int32_t nivh2o, const double *T,
+                    const double *p, double *qj1, const double *vsed3d, const double *parth2osed,
+                    const double *parthno3sed)
+{
+    if (!ctx) return RB_EINVAL;
+    int rc = check_sediment(ctx, g, dt, nivh2o);
+    if (rc) return rc;
+    if (any_null({T, p, qj1, vsed3d, parth2osed, parthno3sed}))
+        return fail(ctx, RB_EINVAL, "a device array is NULL");
+    HIPCHK(hipSetDevice(ctx->device));
+    const int64_t plane = (int64_t)g->nlon * g->nlat;
+    const int64_t field = plane * g->niv;
+    HIPCHK(rb::launch_sediment(p, T, qj1 + rb::H2O * field, qj1 + rb::NOY * field,
+                               qj1 + rb::HNO3 * field, vsed3d, parth2osed, parthno3sed, plane,
+                               g->nivbas, g->nivhau, nivh2o, dt, ctx->stream));
+    return RB_OK;
+}
+
+int rb_sediment(rb_ctx *ctx, const rb_grid *g, double dt, int32_t nivh2o, const double *T,
+                const double *p, double *qj1, const double *vsed3d, const double *parth2osed,
+                const double *parthno3sed)
+{
+    if (!ctx) return RB_EINVAL;
+    int rc = check_sediment(ctx, g, dt, nivh2o);
+    if (rc) return rc;
+    if (any_null({T, p, qj1, vsed3d, parth2osed, parthno3sed}))
+        return fail(ctx, RB_EINVAL, "a host array is NULL");
+    HIPCHK(hipSetDevice(ctx->device));
+    const size_t plane = (size_t)g->nlon * g->nlat, field = plane * g->niv;
+    const size_t nb = plane * g->nivbas;
+    // staging: H2O, NOy, HNO3 fields (slots 3-5, contiguous), T, p, vsed, the partitions
+    if ((rc = ensure(ctx, &ctx->d_stage, &ctx->stage_cap, 3 * field + 5 * nb))) return rc;
+    double *dq = ctx->d_stage, *dT = dq + 3 * field, *dp = dT + nb, *dv = dp + nb,
+           *d2 = dv + nb, *d3 = d2 + nb;
+    const size_t B = sizeof(double);
+    hipStream_t s = ctx->stream;
+    HIPCHK(hipMemcpyAsync(dq, qj1 + 2 * field, 3 * field * B, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(dT, T, nb * B, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(dp, p, nb * B, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(dv, vsed3d, nb * B, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(d2, parth2osed, nb * B, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(d3, parthno3sed, nb * B, hipMemcpyHostToDevice, s));
+    HIPCHK(rb::launch_sediment(dp, dT, dq, dq + field, dq + 2 * field, dv, d2, d3, (int64_t)plane,
+                               g->nivbas, g->nivhau, nivh2o, dt, s));
+    HIPCHK(hipMemcpyAsync(qj1 + 2 * field, dq, 3 * field * B, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return RB_OK;
+}
+
 int rb_sync(rb_ctx *ctx)
 {
     if (!ctx) return RB_EINVAL;

@@ -29,6 +29,9 @@ def load_fixture(name):
                            z["hc_in"], None, {"name": name})
     ref = {k: z[k] for k in ("qj1_chem", "hc_chem", "surfarea", "vsed3d", "parth2osed",
                              "parthno3sed", "qj1", "hc")}
+    if "qj1_sedi" in z.files:      # the reference ran sedimentation before conserv
+        ref["qj1_sedi"] = z["qj1_sedi"]
+        ref["dtsedi"], ref["nivh2o"] = float(z["dtsedi"]), int(z["nivh2o"])
     return inp, ref
 
 

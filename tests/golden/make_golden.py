@@ -6,7 +6,8 @@ Run from the repository root where /root/reference is present:
     make -C oracle ref && python tests/golden/make_golden.py
 
 Each fixture stores the step inputs (with the SZA the reference realised,
-``sza3d``) and the reference outputs after chem3d and after conserv + floor.
+``sza3d``) and the reference outputs after chem3d and after conserv + floor; the
+sedimentation fixture also holds qj1 after the reference's sedimentation.
 """
 from __future__ import annotations
 
@@ -54,6 +55,31 @@ def psc_grid(seed: int = 7):
     return inp
 
 
+def sedi_grid(seed: int = 9, ncol: int = 8, nlev: int = 94):
+    """The reference's own level count (nivbas = 94, heterogeneous chemistry from level
+    14, H2O sedimented above level nivh2oecmwf = 59): cold, wet, HNO3-rich columns so that
+    NAT/ice particles settle."""
+    rng = np.random.default_rng(seed)
+    g = synth.Grid(nlon=ncol, nlat=1, niv=nlev, nivbas=nlev)
+    shape = g.shape3
+    p = np.broadcast_to(synth.level_pressures(nlev)[:, None, None], shape).copy()
+    T = rng.uniform(183.0, 200.0, shape)
+    sza = np.degrees(np.arccos(rng.uniform(-0.3, 1.0, shape)))
+    q, h = synth.draw_species(rng, shape)
+    q[synth.L["h2o"]] = rng.uniform(3e-6, 8e-6, shape)
+    scale = rng.uniform(1.0, 10.0, shape)
+    for n in ("noy", "hno3", "n2o5", "nox", "ho2no2", "clono2", "clno2", "brono2", "hno3_gas"):
+        q[synth.L[n]] *= scale
+    sad = np.exp(rng.uniform(np.log(0.1), np.log(10.0), shape)) * 1e-8
+    q[synth.L["h2so4"]] = synth.h2so4_for_sad(T, p, q[synth.L["h2o"]], sad)
+    return synth.StepInputs(g, 900.0, T, p, sza, synth.altitude_km(p), q, h, sad,
+                            {"name": "sedi", "seed": seed})
+
+
+DTSEDI = 10800.0       # 12 sub-steps of 900 s
+NIVH2O = 59            # nivh2oecmwf, F:18
+
+
 def save(name, inp, ref):
     g = inp.grid
     np.savez_compressed(
@@ -62,7 +88,9 @@ def save(name, inp, ref):
         T=inp.T, p=inp.p, sza=ref["sza3d"], alt=inp.alt, qj1_in=inp.qj1, hc_in=inp.hc,
         qj1_chem=ref["qj1_chem"], hc_chem=ref["hc_chem"], surfarea=ref["surfarea"],
         vsed3d=ref["vsed3d"], parth2osed=ref["parth2osed"], parthno3sed=ref["parthno3sed"],
-        qj1=ref["qj1"], hc=ref["hc"])
+        qj1=ref["qj1"], hc=ref["hc"],
+        **({"qj1_sedi": ref["qj1_sedi"], "dtsedi": DTSEDI, "nivh2o": NIVH2O}
+           if "qj1_sedi" in ref else {}))
     print("wrote", name, g.nlon * g.nlat * g.nivbas, "boxes")
 
 
@@ -74,6 +102,8 @@ def main():
     save("c0_seed0_16col", sub, refrun.run_reference(sub))
     psc = subset_columns(psc_grid(), list(range(8)))
     save("psc_seed7_8col", psc, refrun.run_reference(psc))
+    sedi = sedi_grid()
+    save("sedi_seed9_8col", sedi, refrun.run_reference(sedi, dtsedi=DTSEDI))
 
 
 if __name__ == "__main__":

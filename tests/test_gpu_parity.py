@@ -13,7 +13,7 @@ from conftest import FLOOR, PARITY_TOL, load_fixture, max_rel_by_species, rel_er
 
 pytestmark = pytest.mark.gpu
 
-FIXTURES = ["c0_seed0_16col", "psc_seed7_8col"]
+FIXTURES = ["c0_seed0_16col", "psc_seed7_8col"]   # (sedi_seed9_8col: its own test)
 KEYS = ("qj1_chem", "hc_chem", "qj1", "hc")
 AUX = ("surfarea", "vsed3d", "parth2osed", "parthno3sed")
 
@@ -44,6 +44,54 @@ def test_gpu_matches_reference_golden(step, name):
     worst = _report(name, out, ref)
     for k, v in worst.items():
         assert v < PARITY_TOL, (k, v)
+
+
+def _sediment_host(step, inp, q, vsed, p2o, pno3, dtsedi, nivh2o):
+    from reprobus_amd.chem import make_grid
+
+    g = inp.grid
+    grid = make_grid(g.nlon, g.nlat, g.niv, g.nivbas, g.nivhau)
+    c = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+    q = c(q).copy()
+    step.sediment(grid, dtsedi, nivh2o, c(inp.T), c(inp.p), q, c(vsed), c(p2o), c(pno3))
+    return grid, q
+
+
+def test_gpu_sedimentation_matches_reference_golden(step):
+    """The model's sub-step with sedimentation (F:398-450) on the reference's 94 levels:
+    chemistry, sedimentation, conserv + floor, each against the reference."""
+    import torch
+
+    inp, ref = load_fixture("sedi_seed9_8col")
+    dts, nh = ref["dtsedi"], ref["nivh2o"]
+    # sedimentation alone, on the reference's own chemistry output
+    _, q = _sediment_host(step, inp, ref["qj1_chem"], ref["vsed3d"], ref["parth2osed"],
+                          ref["parthno3sed"], dts, nh)
+    alone = float(max_rel_by_species(q, ref["qj1_sedi"]).max())
+    # the whole sub-step on the GPU
+    out = step.run_inputs(inp, conserv=False)
+    grid, q = _sediment_host(step, inp, out["qj1_chem"], out["vsed3d"], out["parth2osed"],
+                             out["parthno3sed"], dts, nh)
+    chain = float(max_rel_by_species(q, ref["qj1_sedi"]).max())
+    h = out["hc_chem"].copy()
+    step.conserv(grid, q, h)
+    final = max(float(max_rel_by_species(q, ref["qj1"]).max()),
+                float(max_rel_by_species(h, ref["hc"]).max()))
+    # device entry point: same numbers
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).to("cuda:0")
+    dq = dev(out["qj1_chem"])
+    step.set_stream(torch.cuda.current_stream().cuda_stream)
+    step.sediment_device(grid, dts, nh, dev(inp.T), dev(inp.p), dq, dev(out["vsed3d"]),
+                         dev(out["parth2osed"]), dev(out["parthno3sed"]))
+    torch.cuda.synchronize()
+    step.set_stream(None)
+    _, qh = _sediment_host(step, inp, out["qj1_chem"], out["vsed3d"], out["parth2osed"],
+                           out["parthno3sed"], dts, nh)
+    print(f"[parity] sedimentation: alone {alone:.2e}, chem+sedi {chain:.2e}, "
+          f"+conserv {final:.2e}", flush=True)
+    assert alone < 1e-13
+    assert chain < PARITY_TOL and final < PARITY_TOL
+    assert np.array_equal(dq.cpu().numpy(), qh)
 
 
 # GPU vs the restatement built with correctly rounded math in the PSC block: every

@@ -13,7 +13,7 @@ from conftest import FLOOR, load_fixture, max_rel_by_species, rel_err
 
 cport = pytest.importorskip("cport")
 
-FIXTURES = ["c0_seed0_16col", "psc_seed7_8col"]
+FIXTURES = ["c0_seed0_16col", "psc_seed7_8col", "sedi_seed9_8col"]
 
 
 @pytest.fixture(scope="module")
@@ -21,10 +21,91 @@ def lib():
     return cport.lib()
 
 
+def sediment_then_conserv(inp, out, ref):
+    """The model's sequence F:419-427 on the restatement's chemistry output."""
+    import ctypes
+
+    g = inp.grid
+    q = cport.sediment(inp, out["qj1_chem"], out["vsed3d"], out["parth2osed"],
+                       out["parthno3sed"], ref["dtsedi"], ref["nivh2o"])
+    h = out["hc_chem"].copy()
+    G = cport._Grid(g.nlon, g.nlat, g.niv, g.nivbas, g.nivhau, 5)
+    cport.lib().rbo_conserv_floor(ctypes.byref(G), q, h, 0)
+    return dict(out, qj1=q, hc=h)
+
+
+def test_sedimentation_matches_reference_golden():
+    """The restatement's sedimentation on the reference's own chemistry output reproduces
+    the reference's sedimentation (94 levels, NAT/ice settling), and conserv + floor after
+    it the reference's final state."""
+    import ctypes
+
+    inp, ref = load_fixture("sedi_seed9_8col")
+    q = cport.sediment(inp, ref["qj1_chem"], ref["vsed3d"], ref["parth2osed"],
+                       ref["parthno3sed"], ref["dtsedi"], ref["nivh2o"])
+    moved = rel_err(ref["qj1_sedi"], ref["qj1_chem"]).reshape(43, -1).max(axis=1)
+    assert moved[2] > 1e-3 and moved[3] > 1e-3 and moved[4] > 1e-3   # H2O, NOy, HNO3 move
+    assert np.all(moved[[i for i in range(43) if i not in (2, 3, 4)]] == 0)
+    assert max_rel_by_species(q, ref["qj1_sedi"]).max() < 1e-13
+    h = ref["hc_chem"].copy()
+    g = inp.grid
+    G = cport._Grid(g.nlon, g.nlat, g.niv, g.nivbas, g.nivhau, 5)
+    cport.lib().rbo_conserv_floor(ctypes.byref(G), q, h, 0)
+    assert max_rel_by_species(q, ref["qj1"]).max() < 1e-13
+    assert max_rel_by_species(h, ref["hc"]).max() < 1e-13
+
+
+def test_sedimentation_moves_mass_downward_only():
+    """With the condensed fraction of one level only, H2O leaves that level and arrives
+    only at the level below, and the column content (q * hnm * dz weighting aside) is
+    unchanged where nothing settles."""
+    from reprobus_amd import synth
+
+    g = synth.Grid(1, 1, 20, 20, 2)
+    p = synth.level_pressures(20)[:, None, None].copy()
+    T = np.full_like(p, 200.0)
+    inp = synth.StepInputs(g, 900.0, T, p, T * 0, T * 0, np.full((43, 20, 1, 1), 1e-6),
+                           np.zeros((15, 20, 1, 1)))
+    vsed = np.zeros((20, 1, 1))
+    p2o = np.ones((20, 1, 1))
+    pno3 = np.ones((20, 1, 1))
+    vsed[9] = 1e-3
+    p2o[9] = 0.5
+    q = cport.sediment(inp, inp.qj1, vsed, p2o, pno3, 10800.0, 21)
+    d = q - inp.qj1
+    changed = np.nonzero(np.abs(d).reshape(43, -1).max(axis=0))[0]
+    assert list(changed) == [9, 10]                  # level 10 (1-based) loses, 11 gains
+    assert d[2, 9, 0, 0] < 0 < d[2, 10, 0, 0]
+    assert np.all(d[3:5] == 0)                       # NOy/HNO3: nothing condensed
+    hnm = p / (T * 1.38e-19)
+    dz = 7000.0 * np.log(p[10, 0, 0] / p[9, 0, 0])
+    frac = min(1e-3 * 10800.0 / dz, 1.0)
+    assert abs(d[2, 10, 0, 0] - 1e-6 * 0.5 * hnm[9, 0, 0] / hnm[10, 0, 0] * frac) < 1e-20
+    fd = min(1e-3 * 10800.0 / (7000.0 * np.log(p[9, 0, 0] / p[8, 0, 0])), 1.0)
+    assert abs(d[2, 9, 0, 0] + 1e-6 * 0.5 * fd) < 1e-20
+
+
+def test_tropospheric_hno3_gas_is_hno3():
+    """F:444-450: below the chemistry levels HNO3 gas takes the (floored) HNO3 value."""
+    import ctypes
+    from reprobus_amd import synth
+
+    rng = np.random.default_rng(0)
+    q = rng.uniform(0, 1e-9, (43, 10, 2, 3))
+    h = rng.uniform(0, 1e-9, (15, 10, 2, 3))
+    G = cport._Grid(3, 2, 10, 7, 3, 5)
+    q0 = q.copy()
+    cport.lib().rbo_conserv_floor(ctypes.byref(G), q, h, 0)
+    assert np.array_equal(q[42, 7:], q[4, 7:])
+    assert np.array_equal(q[42, :7], q0[42, :7])
+
+
 @pytest.mark.parametrize("name", FIXTURES)
 def test_restatement_matches_reference_golden(name, table):
     inp, ref = load_fixture(name)
     out = cport.chem_step(inp, *table, nthreads=4)
+    if "qj1_sedi" in ref:
+        out = sediment_then_conserv(inp, out, ref)
     for key in ("qj1_chem", "hc_chem", "qj1", "hc"):
         worst = max_rel_by_species(out[key], ref[key]).max()
         assert worst < 1e-11, (key, worst)
