@@ -94,6 +94,33 @@ int rb_chem_step_dev(rb_ctx *ctx, const rb_grid *g, double dt, const double *T, 
                      int fuse_conserv);
 int rb_conserv_floor_dev(rb_ctx *ctx, const rb_grid *g, double *qj1, double *hc);
 
+/* Input producers (replace zenith(trajlon, trajlat, irapp, ilat, daynum, sza), F:5308-5406,
+ * called from chem3d at F:845, and altitude(nlon, nlat, niv, relief, psol, aaa, bbb, t,
+ * h2o, trajlon, trajlat, trajpre, alt), src/altitude.f90:1-148, called at F:391), so the
+ * driver hands over only the trajectory lon/lat/p, T and the day number.
+ *
+ * rb_zenith: SZA (deg) of levels 1..nivbas of the trajectory points (fields of the grid's
+ * shape; sza receives nlon*nlat*nivbas values) on day number `day` (fractional part =
+ * UT).  rb_altitude: geometric altitude (km) of every trajectory point (grid nlon x nlat x
+ * niv, the same shape as the model's regular grid) from the regular grid's relief [km],
+ * surface pressure psol [hPa], hybrid coefficients (p = aaa + bbb psol, niv each), T and
+ * H2O mixing ratio; the regular grid has spacing dlon x dlat deg (the model: 2 x 2, with
+ * nearest-column indices nint(lon/dlon)+1, nint((90-lat)/dlat)+1, clamped into the grid).
+ * Host-array and device-array (_dev, stream-ordered) variants. */
+int rb_zenith(rb_ctx *ctx, const rb_grid *g, double day, const double *trajlon,
+              const double *trajlat, double *sza);
+int rb_zenith_dev(rb_ctx *ctx, const rb_grid *g, double day, const double *trajlon,
+                  const double *trajlat, double *sza);
+int rb_altitude(rb_ctx *ctx, const rb_grid *g, double dlon, double dlat, const double *relief,
+                const double *psol, const double *aaa, const double *bbb, const double *t,
+                const double *h2o, const double *trajlon, const double *trajlat,
+                const double *trajpre, double *alt);
+int rb_altitude_dev(rb_ctx *ctx, const rb_grid *g, double dlon, double dlat,
+                    const double *relief, const double *psol, const double *aaa,
+                    const double *bbb, const double *t, const double *h2o,
+                    const double *trajlon, const double *trajlat, const double *trajpre,
+                    double *alt);
+
 /* Sedimentation (replaces sedimentation(trajpre, trajtem, irapp, qj1, vsed3d, nivh2oecmwf,
  * parth2osed, parthno3sed, dtsedi), F:2416-2522, called at F:419-423 between the
  * chemistry and rb_conserv_floor on the sub-steps where irapp == nrapp).  Moves H2O

@@ -45,6 +45,10 @@ def lib(cr_psc: bool = False):
         L.rbo_conserv_floor.argtypes = [C.POINTER(_Grid), dp, dp, C.c_int]
         L.rbo_sediment.argtypes = [C.POINTER(_Grid), C.c_double, C.c_int, dp, dp, dp, dp, dp, dp]
         L.rbo_sediment.restype = None
+        L.rbo_zenith.argtypes = [C.c_int64, C.c_double, dp, dp, dp]
+        L.rbo_zenith.restype = None
+        L.rbo_altitude.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double] + [dp] * 10
+        L.rbo_altitude.restype = None
         L.rbo_chem_converge.argtypes = [C.POINTER(_Grid), C.c_double, C.c_double] + [dp] * 12 + [
             np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS"), C.c_void_p, C.c_void_p,
             C.c_int]
@@ -141,3 +145,21 @@ def sediment(inp, qj1, vsed3d, parth2osed, parthno3sed, dt: float, nivh2o: int,
     lib(cr_psc).rbo_sediment(C.byref(G), float(dt), int(nivh2o), c(inp.p), c(inp.T), q, c(vsed3d),
                              c(parth2osed), c(parthno3sed))
     return q
+
+
+def zenith(day: float, lon, lat) -> np.ndarray:
+    c = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+    lon, lat = c(lon), c(lat)
+    out = np.empty_like(lon)
+    lib().rbo_zenith(lon.size, float(day), lon, lat, out)
+    return out
+
+
+def altitude(relief, psol, aaa, bbb, t, h2o, tlon, tlat, tpre, dlon=2.0, dlat=2.0) -> np.ndarray:
+    """Regular-grid arrays in numpy order: relief/psol (mlat, mlon), t/h2o/traj (mniv, mlat, mlon)."""
+    c = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+    mniv, mlat, mlon = np.shape(t)
+    out = np.empty((mniv, mlat, mlon))
+    lib().rbo_altitude(mlon, mlat, mniv, dlon, dlat, c(relief), c(psol), c(aaa), c(bbb), c(t),
+                       c(h2o), c(tlon), c(tlat), c(tpre), out)
+    return out

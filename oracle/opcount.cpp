@@ -63,6 +63,7 @@ using opc::fmin; using opc::fmax; using opc::fabs;
 // The restatement, with every double counted.  Its public entry points take
 // double* arrays; they are renamed so the counting build can wrap them.
 #define double cd
+#define RBO_OPCOUNT 1   /* the chemistry step only */
 #define rbo_chem_step rbo_chem_step_counted_impl
 #define rbo_conserv_floor rbo_conserv_floor_counted_impl
 #define rbo_troe_constants opc_troe_constants

@@ -15,6 +15,8 @@
 
 #include <math.h>
 #include <stddef.h>
+#include <stdint.h>
+#include <stdlib.h>
 #include <string.h>
 #ifdef _OPENMP
 #include <omp.h>
@@ -1467,3 +1469,82 @@ void rbo_sediment(const rbo_grid *g, double dt, int nivh2o, const double *p, con
 #undef AT
     }
 }
+
+/* ---------------------------------------------------------------- input producers */
+#ifndef RBO_OPCOUNT
+/* zenith F:5308-5406 for n trajectory points (degrees), day number `day` */
+void rbo_zenith(int64_t n, double day, const double *lon, const double *lat, double *sza)
+{
+    const double pi = 2. * asin(1.0);
+    const double ut = (day - trunc(day)) * 24.;
+    const double tz = 2. * pi * day / 365.;
+    const double sintz = sin(tz), costz = cos(tz);
+    const double sin2tz = 2. * sintz * costz;
+    const double cos2tz = costz * costz - sintz * sintz;
+    const double sin3tz = sintz * cos2tz + costz * sin2tz;
+    const double cos3tz = costz * cos2tz - sintz * sin2tz;
+    const double rdecl = 0.006918 - 0.399912 * costz + 0.070257 * sintz - 0.006758 * cos2tz +
+                         0.000907 * sin2tz - 0.002697 * cos3tz + 0.001480 * sin3tz;
+    const double eqr = 0.000075 + 0.001868 * costz - 0.032077 * sintz - 0.014615 * cos2tz -
+                       0.040849 * sin2tz;
+    const double eqh = eqr * 24. / (2. * pi);
+    for (int64_t i = 0; i < n; ++i) {
+        const double rlt = lat[i] * pi / 180.;
+        const double lbut = 12. - eqh - lon[i] * 24. / 360.;
+        const double lzut = 15. * (ut - lbut);
+        const double zpt = lzut * pi / 180.;
+        const double csz = sin(rlt) * sin(rdecl) + cos(rlt) * cos(rdecl) * cos(zpt);
+        const double zr = acos(csz);
+        sza[i] = fmax(zr * 180. / pi, 0.);
+    }
+}
+
+/* altitude A:1-148 on a (mlon, mlat, mniv) regular grid with spacing dlon x dlat degrees
+ * (the model: 2 x 2); arrays in Fortran order, lon fastest.  Nearest-column indices are
+ * clamped into the grid (the reference assumes they fall inside). */
+void rbo_altitude(int mlon, int mlat, int mniv, double dlon, double dlat, const double *relief,
+                  const double *psol, const double *aaa, const double *bbb, const double *t,
+                  const double *h2o, const double *tlon, const double *tlat, const double *tpre,
+                  double *alt)
+{
+    const size_t plane = (size_t)mlon * mlat, field = plane * mniv;
+    const double ctegas = 287.04 / 9.80665;
+    const double pi = 2. * asin(1.);
+    double *p = (double *)malloc(field * sizeof(double));
+    double *tv = (double *)malloc(field * sizeof(double));
+    double *geo = (double *)malloc(field * sizeof(double));
+#define IX(i, k) ((size_t)(k) * plane + (i))
+    for (size_t c = 0; c < plane; ++c) {
+        for (int k = 0; k < mniv; ++k) {
+            p[IX(c, k)] = aaa[k] + bbb[k] * psol[c];
+            tv[IX(c, k)] = t[IX(c, k)] * (1. + 0.608 * h2o[IX(c, k)] * 18. / 28.97);
+        }
+        geo[IX(c, mniv - 1)] = relief[c] + ctegas * tv[IX(c, mniv - 1)] * log(psol[c] / p[IX(c, mniv - 1)]) * 1.e-3;
+        for (int k = mniv - 2; k >= 0; --k)
+            geo[IX(c, k)] = geo[IX(c, k + 1)] + ctegas * ((tv[IX(c, k + 1)] + tv[IX(c, k)]) / 2.) *
+                                                    log(p[IX(c, k + 1)] / p[IX(c, k)]) * 1.e-3;
+    }
+    for (size_t b = 0; b < field; ++b) {
+        int il = (int)round(tlon[b] / dlon) + 1;
+        if (il > mlon) il = 1;
+        int jl = (int)round((90. - tlat[b]) / dlat) + 1;
+        if (il < 1) il = 1;
+        if (jl < 1) jl = 1;
+        if (jl > mlat) jl = mlat;
+        const size_t c = (size_t)(jl - 1) * mlon + (il - 1);
+        const double pirr = fmin(tpre[b], p[IX(c, mniv - 1)]);
+        int kr = mniv;   /* smallest 1-based level >= 2 with p >= pirr (the reference's scan) */
+        for (int k = 2; k <= mniv; ++k)
+            if (p[IX(c, k - 1)] >= pirr) { kr = k; break; }
+        const double dz = ctegas * (tv[IX(c, kr - 1)] + tv[IX(c, kr - 2)]) / 2. *
+                          log(p[IX(c, kr - 1)] / pirr) * 1.e-3;
+        const double g = geo[IX(c, kr - 1)] + dz;
+        const double xlat = tlat[b] * pi / 180.;
+        alt[b] = (1. + 0.002644 * cos(2. * xlat)) * g + (1. + 0.0089 * cos(2. * xlat)) * (g * g) / 6245.;
+    }
+#undef IX
+    free(p);
+    free(tv);
+    free(geo);
+}
+#endif /* RBO_OPCOUNT */

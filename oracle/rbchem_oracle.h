@@ -11,6 +11,7 @@
  */
 #ifndef RBCHEM_ORACLE_H
 #define RBCHEM_ORACLE_H
+#include <stdint.h>
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -55,6 +56,17 @@ void rbo_conserv_floor(const rbo_grid *g, double *qj1, double *hc, int nthreads)
 void rbo_sediment(const rbo_grid *g, double dt, int nivh2o, const double *p, const double *T,
                   double *qj1, const double *vsed3d, const double *parth2osed,
                   const double *parthno3sed);
+
+/* zenith (F:5308-5406): SZA (deg) of n trajectory points (lon, lat in deg) on day `day`. */
+void rbo_zenith(int64_t n, double day, const double *lon, const double *lat, double *sza);
+/* altitude (src/altitude.f90, A:1-148): geometric altitude (km) of trajectory points
+ * (tlon, tlat, tpre on the same (mlon, mlat, mniv) shape) from the regular grid's
+ * relief, surface pressure, hybrid coefficients p = aaa + bbb psol, T and H2O; the
+ * regular grid has spacing dlon x dlat deg (the model: 2 x 2). */
+void rbo_altitude(int mlon, int mlat, int mniv, double dlon, double dlat, const double *relief,
+                  const double *psol, const double *aaa, const double *bbb, const double *t,
+                  const double *h2o, const double *tlon, const double *tlat, const double *tpre,
+                  double *alt);
 
 /* ---- pieces exposed for unit tests ---- */
 /* Troe prefactors of const (F:5927-5948), 18 values in COMMON order. */

@@ -158,3 +158,37 @@ def time_reference(inp, nrep: int, nproc: int, day: float = DAYNUM) -> dict:
                 "box_steps_per_s": rate, "per_core_box_steps_per_s": boxes / secs}
     finally:
         shutil.rmtree(run, ignore_errors=True)
+
+
+REF_INPUTS_BIN = os.path.join(HERE, "_ref", "ref_inputs")
+
+
+def run_inputs_reference(days, zlon, zlat, relief, psol, aaa, bbb, t, h2o, tlon, tlat, tpre,
+                         workdir: str | None = None) -> tuple:
+    """The reference's zenith (on trajectories zlon/zlat, numpy (nz, 180), for each day
+    number) and altitude (regular grid (mniv, mlat, mlon) arrays in numpy order).
+    Returns (sza (ndays, nz, 180), alt (mniv, mlat, mlon))."""
+    if not os.path.exists(REF_INPUTS_BIN):
+        raise RuntimeError(f"{REF_INPUTS_BIN} not built (run `make -C oracle ref`)")
+    zlon = np.ascontiguousarray(zlon, dtype=np.float64)
+    nz = zlon.shape[0]
+    assert zlon.shape == (nz, 180)
+    mniv, mlat, mlon = np.shape(t)
+    base = workdir or os.path.join(HERE, "_ref")
+    os.makedirs(base, exist_ok=True)
+    run = tempfile.mkdtemp(prefix="inp_", dir=base)
+    try:
+        fin, fout = os.path.join(run, "in.bin"), os.path.join(run, "out.bin")
+        with open(fin, "wb") as f:
+            np.array([nz, len(days), mlon, mlat, mniv], dtype=np.int32).tofile(f)
+            for a in (days, zlon, zlat, relief, psol, aaa, bbb, t, h2o, tlon, tlat, tpre):
+                np.ascontiguousarray(a, dtype=np.float64).tofile(f)
+        r = subprocess.run([REF_INPUTS_BIN, fin, fout], cwd=run, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"ref_inputs failed ({r.returncode}): {r.stdout}\n{r.stderr}")
+        with open(fout, "rb") as f:
+            sza = np.fromfile(f, dtype=np.float64, count=len(days) * nz * 180)
+            alt = np.fromfile(f, dtype=np.float64, count=mniv * mlat * mlon)
+        return sza.reshape(len(days), nz, 180), alt.reshape(mniv, mlat, mlon)
+    finally:
+        shutil.rmtree(run, ignore_errors=True)

@@ -19,7 +19,8 @@ ERROR_NAMES = {RB_EINVAL: "RB_EINVAL", RB_EHIP: "RB_EHIP", RB_ENOTABLE: "RB_ENOT
 # every symbol include/rbchem.h declares
 EXPORTS = ["rb_init", "rb_load_jtable", "rb_set_stream", "rb_chem_step", "rb_conserv_floor",
            "rb_chem_step_dev", "rb_conserv_floor_dev", "rb_chem_converge_dev", "rb_sediment",
-           "rb_sediment_dev", "rb_sync", "rb_get_stats", "rb_last_error", "rb_finalize"]
+           "rb_sediment_dev", "rb_zenith", "rb_zenith_dev", "rb_altitude", "rb_altitude_dev",
+           "rb_sync", "rb_get_stats", "rb_last_error", "rb_finalize"]
 # and include/rbmlp.h (the learned initial guess)
 MLP_EXPORTS = ["rb_mlp_create", "rb_mlp_destroy", "rb_mlp_set_stream", "rb_mlp_last_error",
                "rb_mlp_param_count", "rb_mlp_set_params", "rb_mlp_get_params", "rb_mlp_set_norm",
@@ -83,6 +84,10 @@ def load() -> C.CDLL:
     L.rb_chem_converge_dev.argtypes = [P, C.POINTER(Grid), d, d, C.c_int32] + [P] * 13 + [C.c_int]
     L.rb_sediment.argtypes = [P, C.POINTER(Grid), d, C.c_int32] + [P] * 6
     L.rb_sediment_dev.argtypes = [P, C.POINTER(Grid), d, C.c_int32] + [P] * 6
+    L.rb_zenith.argtypes = [P, C.POINTER(Grid), d, P, P, P]
+    L.rb_zenith_dev.argtypes = [P, C.POINTER(Grid), d, P, P, P]
+    L.rb_altitude.argtypes = [P, C.POINTER(Grid), d, d] + [P] * 10
+    L.rb_altitude_dev.argtypes = [P, C.POINTER(Grid), d, d] + [P] * 10
     L.rb_sync.argtypes = [P]
     L.rb_get_stats.argtypes = [P, C.POINTER(Stats)]
     L.rb_last_error.argtypes = [P]

@@ -131,6 +131,33 @@ class ChemistryStep:
                                             *[_ptr(a) for a in (T, p, qj1, vsed3d, parth2osed,
                                                                 parthno3sed)]))
 
+    def zenith(self, grid: Grid, day: float, trajlon, trajlat, sza) -> None:
+        """SZA of levels 1..nivbas (host arrays), F:5308-5406."""
+        for a in (trajlon, trajlat, sza):
+            assert a.dtype == np.float64 and a.flags.c_contiguous
+        assert sza.size >= grid.nlon * grid.nlat * grid.nivbas
+        self._check(self._L.rb_zenith(self._ctx, C.byref(grid), float(day), trajlon.ctypes.data,
+                                      trajlat.ctypes.data, sza.ctypes.data))
+
+    def zenith_device(self, grid: Grid, day: float, trajlon, trajlat, sza) -> None:
+        self._check(self._L.rb_zenith_dev(self._ctx, C.byref(grid), float(day), _ptr(trajlon),
+                                          _ptr(trajlat), _ptr(sza)))
+
+    def altitude(self, grid: Grid, relief, psol, aaa, bbb, t, h2o, trajlon, trajlat, trajpre, alt,
+                 dlon: float = 2.0, dlat: float = 2.0) -> None:
+        """Geometric altitude of the trajectory points (host arrays), altitude.f90."""
+        arrs = (relief, psol, aaa, bbb, t, h2o, trajlon, trajlat, trajpre, alt)
+        for a in arrs:
+            assert a.dtype == np.float64 and a.flags.c_contiguous
+        self._check(self._L.rb_altitude(self._ctx, C.byref(grid), float(dlon), float(dlat),
+                                        *[a.ctypes.data for a in arrs]))
+
+    def altitude_device(self, grid: Grid, relief, psol, aaa, bbb, t, h2o, trajlon, trajlat, trajpre,
+                        alt, dlon: float = 2.0, dlat: float = 2.0) -> None:
+        arrs = (relief, psol, aaa, bbb, t, h2o, trajlon, trajlat, trajpre, alt)
+        self._check(self._L.rb_altitude_dev(self._ctx, C.byref(grid), float(dlon), float(dlat),
+                                            *[_ptr(a) for a in arrs]))
+
     def conserv_device(self, grid: Grid, qj1, hc) -> None:
         self._check(self._L.rb_conserv_floor_dev(self._ctx, C.byref(grid), _ptr(qj1), _ptr(hc)))
 

@@ -61,6 +61,20 @@ hipError_t launch_chem_conv(const StepArgs &a, const ConvArgs &cv, int block, in
 int conv_blocks_per_cu(int block);
 hipError_t launch_zero_counter(unsigned long long *p, hipStream_t s);
 // conserv on boxes below plane*nivbas and the floor on every box, for boxes [start, field)
+// day-dependent terms of zenith (F:5334-5380), computed once per call on the host
+struct ZenithTerms {
+    double pi, ut, eqh, sin_decl, cos_decl;
+};
+ZenithTerms zenith_terms(double day);
+hipError_t launch_zenith(const double *lon, const double *lat, int64_t n, const ZenithTerms &z,
+                         double *sza, hipStream_t s);
+// work: 3 * mlon * mlat * niv doubles (level pressure, virtual temperature, geopotential)
+hipError_t launch_altitude(const double *relief, const double *psol, const double *aaa,
+                           const double *bbb, const double *t, const double *h2o,
+                           const double *tlon, const double *tlat, const double *tpre, int mlon,
+                           int mlat, int niv, double dlon, double dlat, double *work, double *alt,
+                           hipStream_t s);
+
 // h2o, noy, hno3: the qj1 fields of those slots
 hipError_t launch_sediment(const double *p, const double *T, double *h2o, double *noy,
                            double *hno3, const double *vsed, const double *p2o, const double *pno3,

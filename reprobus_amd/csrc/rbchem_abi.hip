@@ -27,6 +27,8 @@ struct rb_ctx {
     size_t stage_cap = 0;
     unsigned long long *d_count = nullptr;
     unsigned long long *d_queue = nullptr;   // converged-mode work-queue head
+    double *d_work = nullptr;                // altitude: level pressure, Tv, geopotential
+    size_t work_cap = 0;
     int n_cu = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_h2d = nullptr, ev_d2h = nullptr;
     bool timed = false;
@@ -358,6 +360,99 @@ int rb_conserv_floor(rb_ctx *ctx, const rb_grid *g, double *qj1, double *hc)
     return RB_OK;
 }
 
+int rb_zenith_dev(rb_ctx *ctx, const rb_grid *g, double day, const double *trajlon,
+                  const double *trajlat, double *sza)
+{
+    if (!ctx) return RB_EINVAL;
+    int rc = check_grid(ctx, g);
+    if (rc) return rc;
+    if (any_null({trajlon, trajlat, sza})) return fail(ctx, RB_EINVAL, "a device array is NULL");
+    if (!std::isfinite(day)) return fail(ctx, RB_EINVAL, "day must be finite");
+    HIPCHK(hipSetDevice(ctx->device));
+    const int64_t n = (int64_t)g->nlon * g->nlat * g->nivbas;
+    HIPCHK(rb::launch_zenith(trajlon, trajlat, n, rb::zenith_terms(day), sza, ctx->stream));
+    return RB_OK;
+}
+
+int rb_zenith(rb_ctx *ctx, const rb_grid *g, double day, const double *trajlon,
+              const double *trajlat, double *sza)
+{
+    if (!ctx) return RB_EINVAL;
+    int rc = check_grid(ctx, g);
+    if (rc) return rc;
+    if (any_null({trajlon, trajlat, sza})) return fail(ctx, RB_EINVAL, "a host array is NULL");
+    HIPCHK(hipSetDevice(ctx->device));
+    const size_t n = (size_t)g->nlon * g->nlat * g->nivbas;
+    if ((rc = ensure(ctx, &ctx->d_stage, &ctx->stage_cap, 3 * n))) return rc;
+    double *dl = ctx->d_stage, *dt = dl + n, *ds = dt + n;
+    const size_t B = sizeof(double);
+    HIPCHK(hipMemcpyAsync(dl, trajlon, n * B, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(dt, trajlat, n * B, hipMemcpyHostToDevice, ctx->stream));
+    if ((rc = rb_zenith_dev(ctx, g, day, dl, dt, ds))) return rc;
+    HIPCHK(hipMemcpyAsync(sza, ds, n * B, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return RB_OK;
+}
+
+static int check_altitude(rb_ctx *ctx, const rb_grid *g, double dlon, double dlat)
+{
+    if (!g || g->nlon < 1 || g->nlat < 1 || g->niv < 2) return fail(ctx, RB_EINVAL, "bad grid");
+    if (!(dlon > 0.) || !(dlat > 0.)) return fail(ctx, RB_EINVAL, "dlon, dlat must be > 0");
+    return RB_OK;
+}
+
+int rb_altitude_dev(rb_ctx *ctx, const rb_grid *g, double dlon, double dlat, const double *relief,
+                    const double *psol, const double *aaa, const double *bbb, const double *t,
+                    const double *h2o, const double *trajlon, const double *trajlat,
+                    const double *trajpre, double *alt)
+{
+    if (!ctx) return RB_EINVAL;
+    int rc = check_altitude(ctx, g, dlon, dlat);
+    if (rc) return rc;
+    if (any_null({relief, psol, aaa, bbb, t, h2o, trajlon, trajlat, trajpre, alt}))
+        return fail(ctx, RB_EINVAL, "a device array is NULL");
+    HIPCHK(hipSetDevice(ctx->device));
+    const size_t field = (size_t)g->nlon * g->nlat * g->niv;
+    if ((rc = ensure(ctx, &ctx->d_work, &ctx->work_cap, 3 * field))) return rc;
+    HIPCHK(rb::launch_altitude(relief, psol, aaa, bbb, t, h2o, trajlon, trajlat, trajpre, g->nlon,
+                               g->nlat, g->niv, dlon, dlat, ctx->d_work, alt, ctx->stream));
+    return RB_OK;
+}
+
+int rb_altitude(rb_ctx *ctx, const rb_grid *g, double dlon, double dlat, const double *relief,
+                const double *psol, const double *aaa, const double *bbb, const double *t,
+                const double *h2o, const double *trajlon, const double *trajlat,
+                const double *trajpre, double *alt)
+{
+    if (!ctx) return RB_EINVAL;
+    int rc = check_altitude(ctx, g, dlon, dlat);
+    if (rc) return rc;
+    if (any_null({relief, psol, aaa, bbb, t, h2o, trajlon, trajlat, trajpre, alt}))
+        return fail(ctx, RB_EINVAL, "a host array is NULL");
+    HIPCHK(hipSetDevice(ctx->device));
+    const size_t plane = (size_t)g->nlon * g->nlat, field = plane * g->niv;
+    if ((rc = ensure(ctx, &ctx->d_stage, &ctx->stage_cap, 6 * field + 2 * plane + 2 * g->niv)))
+        return rc;
+    double *d = ctx->d_stage;
+    double *dt_ = d, *dh = d + field, *dlo = d + 2 * field, *dla = d + 3 * field,
+           *dpr = d + 4 * field, *dal = d + 5 * field, *dre = d + 6 * field, *dps = dre + plane,
+           *da = dps + plane, *db = da + g->niv;
+    const size_t B = sizeof(double);
+    hipStream_t s = ctx->stream;
+    const std::pair<double *, const double *> in[] = {
+        {dt_, t}, {dh, h2o}, {dlo, trajlon}, {dla, trajlat}, {dpr, trajpre}};
+    for (auto &io : in) HIPCHK(hipMemcpyAsync(io.first, io.second, field * B, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(dre, relief, plane * B, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(dps, psol, plane * B, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(da, aaa, g->niv * B, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(db, bbb, g->niv * B, hipMemcpyHostToDevice, s));
+    if ((rc = rb_altitude_dev(ctx, g, dlon, dlat, dre, dps, da, db, dt_, dh, dlo, dla, dpr, dal)))
+        return rc;
+    HIPCHK(hipMemcpyAsync(alt, dal, field * B, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return RB_OK;
+}
+
 static int check_sediment(rb_ctx *ctx, const rb_grid *g, double dt, int32_t nivh2o)
 {
     int rc = check_grid(ctx, g);
@@ -458,6 +553,7 @@ int rb_finalize(rb_ctx *ctx)
         if (b) (void)hipFree(b);
     if (ctx->d_count) (void)hipFree(ctx->d_count);
     if (ctx->d_queue) (void)hipFree(ctx->d_queue);
+    if (ctx->d_work) (void)hipFree(ctx->d_work);
     for (hipEvent_t e : {ctx->ev0, ctx->ev1, ctx->ev_h2d, ctx->ev_d2h})
         if (e) (void)hipEventDestroy(e);
     for (auto &pe : ctx->prof) {

@@ -80,6 +80,32 @@ DTSEDI = 10800.0       # 12 sub-steps of 900 s
 NIVH2O = 59            # nivh2oecmwf, F:18
 
 
+def input_producers(seed: int = 11) -> dict:
+    """Inputs for the reference's zenith and altitude.  zenith: 20 levels x 180
+    trajectory points anywhere on the globe, four day numbers.  altitude: a 16 x 16 x 20
+    regular grid (the model's 2-degree spacing, the polar cap 60-90 N, 0-30 E) with hybrid
+    levels p = a + b psol, and trajectory points in that cap, pressures from above the top
+    level to below the bottom one (both clamps)."""
+    rng = np.random.default_rng(seed)
+    r = lambda a, q: np.round(a / q) * q          # short decimals: the fixture compresses
+    d = {"days": np.array([1.0, 80.5, 172.25, 355.9]),
+         "zlon": r(rng.uniform(0.0, 360.0, (20, 180)), 1e-4),
+         "zlat": r(rng.uniform(-90.0, 90.0, (20, 180)), 1e-4)}
+    m, nl = 16, 20
+    eta = (np.arange(nl) + 0.5) / nl
+    d["bbb"] = eta ** 3
+    d["aaa"] = 0.5 + 150.0 * eta * (1.0 - eta)
+    d["psol"] = r(rng.uniform(950.0, 1030.0, (m, m)), 1e-3)
+    d["relief"] = r(rng.uniform(0.0, 2.0, (m, m)), 1e-4)
+    p = d["aaa"][:, None, None] + d["bbb"][:, None, None] * d["psol"][None]
+    d["t"] = r(210.0 + 80.0 * eta[:, None, None] + rng.normal(0, 4, (nl, m, m)), 1e-3)
+    d["h2o"] = r(np.exp(rng.uniform(np.log(2e-6), np.log(1e-2), (nl, m, m))), 1e-9)
+    d["tlon"] = r(rng.uniform(0.0, 30.9, (nl, m, m)), 1e-4)
+    d["tlat"] = r(rng.uniform(59.1, 90.0, (nl, m, m)), 1e-4)
+    d["tpre"] = r(np.exp(rng.uniform(np.log(0.3), np.log(1100.0), (nl, m, m))), 1e-4)
+    return d
+
+
 def save(name, inp, ref):
     g = inp.grid
     np.savez_compressed(
@@ -104,6 +130,12 @@ def main():
     save("psc_seed7_8col", psc, refrun.run_reference(psc))
     sedi = sedi_grid()
     save("sedi_seed9_8col", sedi, refrun.run_reference(sedi, dtsedi=DTSEDI))
+    d = input_producers()
+    sza, alt = refrun.run_inputs_reference(d["days"], d["zlon"], d["zlat"], d["relief"],
+                                           d["psol"], d["aaa"], d["bbb"], d["t"], d["h2o"],
+                                           d["tlon"], d["tlat"], d["tpre"])
+    np.savez_compressed(os.path.join(OUT, "inputs_seed11.npz"), sza=sza, alt=alt, **d)
+    print("wrote inputs_seed11", sza.size, "SZA values,", alt.size, "altitudes")
 
 
 if __name__ == "__main__":

@@ -395,3 +395,71 @@ def test_converged_state_is_a_fixed_point(table):
         d = rel_err(back[k][:42] if k == "qj1_chem" else back[k],
                     base[k][:42] if k == "qj1_chem" else base[k], floor=1e-30)
         assert d[:, live].max() < 1e-9, k
+
+
+def _inputs_fixture():
+    import os
+
+    from conftest import GOLDEN
+
+    return np.load(os.path.join(GOLDEN, "inputs_seed11.npz"), allow_pickle=False)
+
+
+def test_zenith_matches_reference_golden():
+    z = _inputs_fixture()
+    for d, day in enumerate(z["days"]):
+        sza = cport.zenith(day, z["zlon"], z["zlat"])
+        assert np.array_equal(sza, z["sza"][d]), day
+
+
+def test_zenith_closed_forms():
+    """Sub-solar point: the SZA is 0 where lat = declination and the hour angle is 0; the
+    antipode is at 180; on the terminator great circle cos SZA = 0."""
+    day = 172.25
+    rdecl, eqh = _decl_eqh(day)
+    ut = (day - np.trunc(day)) * 24.0
+    lon0 = (12.0 - eqh - ut) * 360.0 / 24.0 % 360.0        # hour angle 0
+    lat0 = np.degrees(rdecl)
+    sza = cport.zenith(day, np.array([lon0, lon0 + 180.0, lon0 + 90.0]),
+                       np.array([lat0, -lat0, 0.0]))
+    assert sza[0] < 1e-5 and abs(sza[1] - 180.0) < 1e-5
+    assert abs(np.cos(np.radians(sza[2])) - np.cos(np.pi / 2)) < 1e-12
+
+
+def _decl_eqh(day):
+    """Spencer (1971) Fourier series for declination and the equation of time."""
+    g = 2.0 * np.pi * day / 365.0
+    decl = (0.006918 - 0.399912 * np.cos(g) + 0.070257 * np.sin(g) - 0.006758 * np.cos(2 * g)
+            + 0.000907 * np.sin(2 * g) - 0.002697 * np.cos(3 * g) + 0.00148 * np.sin(3 * g))
+    eqt = 0.000075 + 0.001868 * np.cos(g) - 0.032077 * np.sin(g) - 0.014615 * np.cos(2 * g) \
+        - 0.040849 * np.sin(2 * g)
+    return decl, eqt * 12.0 / np.pi
+
+
+def test_altitude_matches_reference_golden():
+    z = _inputs_fixture()
+    alt = cport.altitude(z["relief"], z["psol"], z["aaa"], z["bbb"], z["t"], z["h2o"], z["tlon"],
+                         z["tlat"], z["tpre"])
+    assert np.max(np.abs(alt - z["alt"]) / np.abs(z["alt"])) < 1e-14
+
+
+def test_altitude_isothermal_dry_closed_form():
+    """Dry isothermal atmosphere: geopotential height = relief + (R T / g) ln(psol / p); the
+    trajectory point at a level's own pressure sits at that level's height exactly."""
+    m, nl = 4, 6
+    eta = (np.arange(nl) + 0.5) / nl
+    bbb, aaa = eta ** 2, 1.0 + 50.0 * eta * (1 - eta)
+    psol = np.full((m, m), 1000.0)
+    relief = np.full((m, m), 0.3)
+    t = np.full((nl, m, m), 250.0)
+    h2o = np.zeros((nl, m, m))
+    p = aaa[:, None, None] + bbb[:, None, None] * psol[None]
+    tlon = np.zeros((nl, m, m))
+    tlat = np.full((nl, m, m), 90.0)
+    k = np.arange(nl)[:, None, None] * np.ones((1, m, m))
+    alt = cport.altitude(relief, psol, aaa, bbb, t, h2o, tlon, tlat, p)
+    H = 287.04 / 9.80665 * 250.0 * 1e-3
+    geo = 0.3 + H * np.log(1000.0 / p)
+    x = np.radians(90.0)
+    expect = (1 + 0.002644 * np.cos(2 * x)) * geo + (1 + 0.0089 * np.cos(2 * x)) * geo ** 2 / 6245.0
+    assert np.max(np.abs(alt[1:] - expect[1:])) < 1e-12
