@@ -192,3 +192,38 @@ def run_inputs_reference(days, zlon, zlat, relief, psol, aaa, bbb, t, h2o, tlon,
         return sza.reshape(len(days), nz, 180), alt.reshape(mniv, mlat, mlon)
     finally:
         shutil.rmtree(run, ignore_errors=True)
+
+
+REF_ECMWF_BIN = os.path.join(HERE, "_ref", "ref_ecmwf")
+
+
+def run_ecmwf_reference(forcing_file: str, ian: int, day: float, idx, ntime: int = 4,
+                        nintecmwf: int = 12) -> dict:
+    """The reference's ECMWF reader (F:4713-4935) on `forcing_file` (named
+    ecmwf_YYYYMMDD): the two analysis times it loads, sampled at the 1-based positions
+    idx of a 180 x 91 x 137 field, and the sums of every field."""
+    if not os.path.exists(REF_ECMWF_BIN):
+        raise RuntimeError(f"{REF_ECMWF_BIN} not built (run `make -C oracle ref`)")
+    idx = np.ascontiguousarray(idx, dtype=np.int32)
+    base = os.path.join(HERE, "_ref")
+    run = tempfile.mkdtemp(prefix="ecmwf_", dir=base)
+    try:
+        os.symlink(os.path.abspath(forcing_file), os.path.join(run, os.path.basename(forcing_file)))
+        fin, fout = os.path.join(run, "in.bin"), os.path.join(run, "out.bin")
+        # the driver runs with big-endian unformatted I/O throughout, its own files too
+        with open(fin, "wb") as f:
+            np.array([ian, ntime, nintecmwf, idx.size], dtype=">i4").tofile(f)
+            np.array([day], dtype=">f8").tofile(f)
+            idx.astype(">i4").tofile(f)
+        r = subprocess.run([REF_ECMWF_BIN, fin, fout], cwd=run, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"ref_ecmwf failed ({r.returncode}): {r.stdout}\n{r.stderr}")
+        o = np.fromfile(fout, dtype=">f8").astype(np.float64)
+        n = idx.size
+        out = {"daym1": o[0], "dayp1": o[1]}
+        vals = o[2:2 + 12 * n].reshape(2, 6, n)
+        out["m1"], out["p1"] = vals[0], vals[1]          # rows: ps, t, w, h, u, v
+        out["sums"] = o[2 + 12 * n:].reshape(2, 6)
+        return out
+    finally:
+        shutil.rmtree(run, ignore_errors=True)

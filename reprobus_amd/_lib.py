@@ -12,9 +12,9 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "librbchem.so")
 
-RB_OK, RB_EINVAL, RB_EHIP, RB_ENOTABLE, RB_ENONFINITE, RB_ENOMEM = 0, -1, -2, -3, -4, -5
+RB_OK, RB_EINVAL, RB_EHIP, RB_ENOTABLE, RB_ENONFINITE, RB_ENOMEM, RB_EIO = 0, -1, -2, -3, -4, -5, -6
 ERROR_NAMES = {RB_EINVAL: "RB_EINVAL", RB_EHIP: "RB_EHIP", RB_ENOTABLE: "RB_ENOTABLE",
-               RB_ENONFINITE: "RB_ENONFINITE", RB_ENOMEM: "RB_ENOMEM"}
+               RB_ENONFINITE: "RB_ENONFINITE", RB_ENOMEM: "RB_ENOMEM", RB_EIO: "RB_EIO"}
 
 # every symbol include/rbchem.h declares
 EXPORTS = ["rb_init", "rb_load_jtable", "rb_set_stream", "rb_chem_step", "rb_conserv_floor",
@@ -26,6 +26,9 @@ MLP_EXPORTS = ["rb_mlp_create", "rb_mlp_destroy", "rb_mlp_set_stream", "rb_mlp_l
                "rb_mlp_param_count", "rb_mlp_set_params", "rb_mlp_get_params", "rb_mlp_set_norm",
                "rb_mlp_forward_dev", "rb_mlp_train_step_dev", "rb_mlp_gradients_dev",
                "rb_mlp_features_dev", "rb_mlp_guess_dev"]
+# and include/rbio.h (history and forcing records, host only)
+IO_EXPORTS = ["rb_history_write", "rb_history_read", "rb_ecmwf_write_time", "rb_ecmwf_read_time",
+              "rb_io_last_error"]
 
 
 class Grid(C.Structure):
@@ -116,5 +119,14 @@ def load() -> C.CDLL:
     L.rb_mlp_gradients_dev.argtypes = [P, P, P, i64, P, C.POINTER(f)]
     L.rb_mlp_features_dev.argtypes = [P, C.POINTER(Grid), d] + [P] * 6
     L.rb_mlp_guess_dev.argtypes = [P, C.POINTER(Grid), d] + [P] * 7
+    for name in IO_EXPORTS:
+        getattr(L, name).restype = C.c_int
+    L.rb_io_last_error.argtypes = []
+    L.rb_io_last_error.restype = C.c_char_p
+    i32 = C.c_int32
+    L.rb_history_write.argtypes = [C.c_char_p, i32, i32, i32] + [P] * 9
+    L.rb_history_read.argtypes = [C.c_char_p, i32, i32, i32] + [P] * 9
+    L.rb_ecmwf_write_time.argtypes = [C.c_char_p, i32, i32, i32, i32] + [P] * 8
+    L.rb_ecmwf_read_time.argtypes = [C.c_char_p, i32, i32, i32, i32] + [P] * 7
     _lib = L
     return L

@@ -136,6 +136,27 @@ def main():
                                            d["tlon"], d["tlat"], d["tpre"])
     np.savez_compressed(os.path.join(OUT, "inputs_seed11.npz"), sza=sza, alt=alt, **d)
     print("wrote inputs_seed11", sza.size, "SZA values,", alt.size, "altitudes")
+    ecmwf_fixture()
+
+
+def ecmwf_fixture(seed: int = 13, nsamp: int = 3000):
+    """The reference's ECMWF reader on a forcing file written by the library's writer
+    (three analysis times, 0/3/6 h; the model at 03 UT reads the 3 h and 6 h ones)."""
+    import tempfile
+
+    sys.path.insert(0, os.path.dirname(OUT))
+    import forcing_synth as fs
+
+    y, m, d = fs.DATE
+    with tempfile.TemporaryDirectory() as tmp:
+        path = os.path.join(tmp, f"ecmwf_{y:04d}{m:02d}{d:02d}")
+        fs.write_file(path, seed, 3)
+        idx = np.random.default_rng(seed).integers(1, 180 * 91 * 137 + 1, nsamp)
+        day = (d - 1) + 3.0 / 24.0          # gregtojul(2023, 1, 15, 3, 0)
+        ref = refrun.run_ecmwf_reference(path, y, day, idx)
+    np.savez_compressed(os.path.join(OUT, f"ecmwf_seed{seed}.npz"), idx=idx, day=day, ian=y,
+                        seed=seed, **ref)
+    print(f"wrote ecmwf_seed{seed}: days {ref['daym1']} {ref['dayp1']}")
 
 
 if __name__ == "__main__":

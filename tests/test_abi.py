@@ -1,5 +1,6 @@
 """CPU checks of the C-ABI library: it loads without a GPU and exports every
-symbol include/rbchem.h and include/rbmlp.h declare (no compute calls here)."""
+symbol include/rbchem.h, include/rbmlp.h and include/rbio.h declare (no compute calls
+here)."""
 import os
 import re
 
@@ -19,6 +20,7 @@ def test_header_declares_the_documented_entry_points():
 
     assert declared_symbols() == sorted(_lib.EXPORTS)
     assert declared_symbols("rbmlp.h") == sorted(_lib.MLP_EXPORTS)
+    assert declared_symbols("rbio.h") == sorted(_lib.IO_EXPORTS)
 
 
 def test_library_exports_every_declared_symbol():
@@ -27,7 +29,7 @@ def test_library_exports_every_declared_symbol():
     if not os.path.exists(_lib.LIB_PATH):
         pytest.fail("librbchem.so is not built (run __graft_entry__.build())")
     L = _lib.load()
-    for name in declared_symbols() + declared_symbols("rbmlp.h"):
+    for name in declared_symbols() + declared_symbols("rbmlp.h") + declared_symbols("rbio.h"):
         assert hasattr(L, name), name
     assert L.rb_mlp_param_count() == 64 * 256 + 256 + 2 * (256 * 256 + 256) + 256 * 64 + 64
 
