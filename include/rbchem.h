@@ -121,6 +121,27 @@ int rb_altitude_dev(rb_ctx *ctx, const rb_grid *g, double dlon, double dlat,
                     const double *trajlon, const double *trajlat, const double *trajpre,
                     double *alt);
 
+/* A day of sub-steps on device-resident state (the model's loop F:300-440 over one day of
+ * chemistry, without transport): for k = 0..nsteps-1, rb_zenith_dev at day number
+ * day0 + k dt / 86400 into sza (device, nlon*nlat*nivbas), then rb_chem_step_dev (or, with
+ * converged = 1, rb_chem_converge_dev with tol/maxiter; iters may be NULL) with conserv and
+ * the floor fused.  use_graph = 1 captures the whole day once into a HIP graph (kept in
+ * the context and rebuilt when any argument or the configuration changes) and replays
+ * it with one launch; results are identical to use_graph = 0.  No counterpart in the
+ * reference. */
+typedef struct {
+    int32_t nsteps;
+    int32_t use_graph;
+    int32_t converged;
+    int32_t maxiter;
+    double day0, dt, tol;
+} rb_day_config;
+
+int rb_run_day_dev(rb_ctx *ctx, const rb_grid *g, const rb_day_config *dc, const double *trajlon,
+                   const double *trajlat, const double *T, const double *p, const double *alt,
+                   double *sza, double *qj1, double *hc, double *surfarea, double *vsed3d,
+                   double *parth2osed, double *parthno3sed, int32_t *iters);
+
 /* Sedimentation (replaces sedimentation(trajpre, trajtem, irapp, qj1, vsed3d, nivh2oecmwf,
  * parth2osed, parthno3sed, dtsedi), F:2416-2522, called at F:419-423 between the
  * chemistry and rb_conserv_floor on the sub-steps where irapp == nrapp).  Moves H2O

@@ -20,7 +20,7 @@ ERROR_NAMES = {RB_EINVAL: "RB_EINVAL", RB_EHIP: "RB_EHIP", RB_ENOTABLE: "RB_ENOT
 EXPORTS = ["rb_init", "rb_load_jtable", "rb_set_stream", "rb_chem_step", "rb_conserv_floor",
            "rb_chem_step_dev", "rb_conserv_floor_dev", "rb_chem_converge_dev", "rb_sediment",
            "rb_sediment_dev", "rb_zenith", "rb_zenith_dev", "rb_altitude", "rb_altitude_dev",
-           "rb_sync", "rb_get_stats", "rb_last_error", "rb_finalize"]
+           "rb_run_day_dev", "rb_sync", "rb_get_stats", "rb_last_error", "rb_finalize"]
 # and include/rbmlp.h (the learned initial guess)
 MLP_EXPORTS = ["rb_mlp_create", "rb_mlp_destroy", "rb_mlp_set_stream", "rb_mlp_last_error",
                "rb_mlp_param_count", "rb_mlp_set_params", "rb_mlp_get_params", "rb_mlp_set_norm",
@@ -39,6 +39,12 @@ class Grid(C.Structure):
 class Config(C.Structure):
     _fields_ = [("niter", C.c_int32), ("check_finite", C.c_int32), ("block", C.c_int32),
                 ("profile", C.c_int32)]
+
+
+class DayConfig(C.Structure):
+    _fields_ = [("nsteps", C.c_int32), ("use_graph", C.c_int32), ("converged", C.c_int32),
+                ("maxiter", C.c_int32), ("day0", C.c_double), ("dt", C.c_double),
+                ("tol", C.c_double)]
 
 
 class Stats(C.Structure):
@@ -91,6 +97,7 @@ def load() -> C.CDLL:
     L.rb_zenith_dev.argtypes = [P, C.POINTER(Grid), d, P, P, P]
     L.rb_altitude.argtypes = [P, C.POINTER(Grid), d, d] + [P] * 10
     L.rb_altitude_dev.argtypes = [P, C.POINTER(Grid), d, d] + [P] * 10
+    L.rb_run_day_dev.argtypes = [P, C.POINTER(Grid), C.POINTER(DayConfig)] + [P] * 13
     L.rb_sync.argtypes = [P]
     L.rb_get_stats.argtypes = [P, C.POINTER(Stats)]
     L.rb_last_error.argtypes = [P]

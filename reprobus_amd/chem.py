@@ -18,7 +18,7 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from ._lib import Config, Grid, RbError, Stats
+from ._lib import Config, DayConfig, Grid, RbError, Stats
 
 
 def _ptr(a) -> int:
@@ -157,6 +157,20 @@ class ChemistryStep:
         arrs = (relief, psol, aaa, bbb, t, h2o, trajlon, trajlat, trajpre, alt)
         self._check(self._L.rb_altitude_dev(self._ctx, C.byref(grid), float(dlon), float(dlat),
                                             *[_ptr(a) for a in arrs]))
+
+    def run_day_device(self, grid: Grid, nsteps: int, day0: float, dt: float, trajlon, trajlat,
+                       T, p, alt, sza, qj1, hc, surfarea, vsed3d, parth2osed, parthno3sed,
+                       use_graph: bool = True, converged: bool = False, tol: float = 1e-10,
+                       maxiter: int = 2000, iters=None) -> None:
+        """nsteps sub-steps (device zenith + chemistry, conserv fused), optionally as one
+        captured HIP graph."""
+        dc = DayConfig(int(nsteps), 1 if use_graph else 0, 1 if converged else 0, int(maxiter),
+                       float(day0), float(dt), float(tol))
+        self._check(self._L.rb_run_day_dev(
+            self._ctx, C.byref(grid), C.byref(dc),
+            *[_ptr(a) for a in (trajlon, trajlat, T, p, alt, sza, qj1, hc, surfarea, vsed3d,
+                                parth2osed, parthno3sed)],
+            iters.data_ptr() if iters is not None else None))
 
     def conserv_device(self, grid: Grid, qj1, hc) -> None:
         self._check(self._L.rb_conserv_floor_dev(self._ctx, C.byref(grid), _ptr(qj1), _ptr(hc)))

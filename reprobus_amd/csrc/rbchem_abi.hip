@@ -30,6 +30,10 @@ struct rb_ctx {
     double *d_work = nullptr;                // altitude: level pressure, Tv, geopotential
     size_t work_cap = 0;
     int n_cu = 0;
+    bool capturing = false;                  // inside a graph capture: no event records
+    hipGraph_t day_graph = nullptr;          // the captured day and what it was built for
+    hipGraphExec_t day_exec = nullptr;
+    std::vector<char> day_key;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_h2d = nullptr, ev_d2h = nullptr;
     bool timed = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> prof;   // profile-mode event pairs
@@ -184,9 +188,9 @@ int rb_chem_step_dev(rb_ctx *ctx, const rb_grid *g, double dt, const double *T, 
     a.dt = dt;
     a.K = ctx->K;
 
-    HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
+    if (!ctx->capturing) HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
     HIPCHK(rb::launch_columns(a, ctx->d_col, ctx->stream));
-    if (ctx->cfg.profile) {
+    if (ctx->cfg.profile && !ctx->capturing) {
         if (ctx->prof_used == ctx->prof.size()) {
             hipEvent_t e0, e1;
             HIPCHK(hipEventCreate(&e0));
@@ -202,12 +206,14 @@ int rb_chem_step_dev(rb_ctx *ctx, const rb_grid *g, double dt, const double *T, 
     }
     if (fuse_conserv && field > nbox)   // floor of the levels below the chemistry
         HIPCHK(rb::launch_conserv_floor(qj1, hc, plane, field, g->nivbas, nbox, ctx->stream));
-    HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
-    ctx->timed = true;
+    if (!ctx->capturing) {
+        HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
+        ctx->timed = true;
+    }
     ctx->stats.boxes = nbox;
 
-    if (ctx->cfg.check_finite) {
-        HIPCHK(hipMemsetAsync(ctx->d_count, 0, sizeof(unsigned long long), ctx->stream));
+    if (ctx->cfg.check_finite && !ctx->capturing) {
+        HIPCHK(rb::launch_zero_counter(ctx->d_count, ctx->stream));
         HIPCHK(rb::launch_count_nonfinite(qj1, 43 * field, ctx->d_count, ctx->stream));
         HIPCHK(rb::launch_count_nonfinite(hc, 15 * field, ctx->d_count, ctx->stream));
         unsigned long long cnt = 0;
@@ -257,10 +263,10 @@ int rb_chem_converge_dev(rb_ctx *ctx, const rb_grid *g, double dt, double tol, i
     // one resident wave set: blocks per CU from the occupancy calculator
     const int nblocks = ctx->n_cu * rb::conv_blocks_per_cu(ctx->cfg.block);
 
-    HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
+    if (!ctx->capturing) HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
     HIPCHK(rb::launch_columns(a, ctx->d_col, ctx->stream));
     HIPCHK(rb::launch_zero_counter(ctx->d_queue, ctx->stream));
-    if (ctx->cfg.profile) {
+    if (ctx->cfg.profile && !ctx->capturing) {
         if (ctx->prof_used == ctx->prof.size()) {
             hipEvent_t e0, e1;
             HIPCHK(hipEventCreate(&e0));
@@ -276,8 +282,8 @@ int rb_chem_converge_dev(rb_ctx *ctx, const rb_grid *g, double dt, double tol, i
     }
     if (fuse_conserv && field > nbox)
         HIPCHK(rb::launch_conserv_floor(qj1, hc, plane, field, g->nivbas, nbox, ctx->stream));
-    HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
-    ctx->timed = true;
+    if (!ctx->capturing) HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
+    ctx->timed = !ctx->capturing;
     ctx->stats.boxes = nbox;
     return RB_OK;
 }
@@ -512,6 +518,86 @@ int rb_sediment(rb_ctx *ctx, const rb_grid *g, double dt, int32_t nivh2o, const 
     return RB_OK;
 }
 
+// one sub-step of a day: SZA of the step's time, then the chemistry step
+static int day_step(rb_ctx *ctx, const rb_grid *g, const rb_day_config *dc, int k,
+                    const double *trajlon, const double *trajlat, const double *T,
+                    const double *p, const double *alt, double *sza, double *qj1, double *hc,
+                    double *surfarea, double *vsed3d, double *p2o, double *pno3, int32_t *iters)
+{
+    int rc = rb_zenith_dev(ctx, g, dc->day0 + k * dc->dt / 86400., trajlon, trajlat, sza);
+    if (rc) return rc;
+    if (dc->converged)
+        return rb_chem_converge_dev(ctx, g, dc->dt, dc->tol, dc->maxiter, T, p, sza, alt, qj1, hc,
+                                    surfarea, vsed3d, p2o, pno3, nullptr, nullptr, iters, 1);
+    return rb_chem_step_dev(ctx, g, dc->dt, T, p, sza, alt, qj1, hc, surfarea, vsed3d, p2o, pno3, 1);
+}
+
+int rb_run_day_dev(rb_ctx *ctx, const rb_grid *g, const rb_day_config *dc, const double *trajlon,
+                   const double *trajlat, const double *T, const double *p, const double *alt,
+                   double *sza, double *qj1, double *hc, double *surfarea, double *vsed3d,
+                   double *parth2osed, double *parthno3sed, int32_t *iters)
+{
+    if (!ctx) return RB_EINVAL;
+    int rc = check_grid(ctx, g);
+    if (rc) return rc;
+    if (!dc || dc->nsteps < 1 || !(dc->dt > 0.) || !std::isfinite(dc->day0))
+        return fail(ctx, RB_EINVAL, "bad day configuration");
+    if (dc->converged && (!(dc->tol > 0.) || dc->maxiter < 1))
+        return fail(ctx, RB_EINVAL, "converged day needs tol > 0 and maxiter >= 1");
+    if (any_null({trajlon, trajlat, T, p, alt, sza, qj1, hc, surfarea, vsed3d, parth2osed,
+                  parthno3sed}))
+        return fail(ctx, RB_EINVAL, "a device array is NULL");
+    if (!ctx->table) return fail(ctx, RB_ENOTABLE, "rb_load_jtable has not been called");
+    HIPCHK(hipSetDevice(ctx->device));
+    auto steps = [&]() -> int {
+        for (int k = 0; k < dc->nsteps; ++k) {
+            int r = day_step(ctx, g, dc, k, trajlon, trajlat, T, p, alt, sza, qj1, hc, surfarea,
+                             vsed3d, parth2osed, parthno3sed, iters);
+            if (r) return r;
+        }
+        return RB_OK;
+    };
+    if (!dc->use_graph) return steps();
+
+    // scratch the steps use must exist before capture (no allocation inside it)
+    const size_t nbox = (size_t)g->nlon * g->nlat * g->nivbas;
+    if ((rc = ensure(ctx, &ctx->d_col, &ctx->col_cap, 3 * nbox))) return rc;
+    // the graph is rebuilt when anything it was captured with changes
+    std::vector<char> key;
+    auto add = [&](const void *v, size_t n) {
+        const char *c = (const char *)v;
+        key.insert(key.end(), c, c + n);
+    };
+    const void *ptrs[] = {trajlon, trajlat, T, p, alt, sza, qj1, hc, surfarea, vsed3d,
+                          parth2osed, parthno3sed, iters, ctx->stream, ctx->d_col};
+    add(g, sizeof *g);
+    add(dc, sizeof *dc);
+    add(ptrs, sizeof ptrs);
+    add(&ctx->cfg, sizeof ctx->cfg);
+    if (!ctx->day_exec || key != ctx->day_key) {
+        if (ctx->day_exec) (void)hipGraphExecDestroy(ctx->day_exec);
+        if (ctx->day_graph) (void)hipGraphDestroy(ctx->day_graph);
+        ctx->day_exec = nullptr;
+        ctx->day_graph = nullptr;
+        HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        ctx->capturing = true;
+        rc = steps();
+        ctx->capturing = false;
+        hipGraph_t graph = nullptr;
+        const hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
+        if (rc) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return rc;
+        }
+        if (e != hipSuccess) return hip_fail(ctx, e, "hipStreamEndCapture");
+        ctx->day_graph = graph;
+        HIPCHK(hipGraphInstantiate(&ctx->day_exec, graph, nullptr, nullptr, 0));
+        ctx->day_key = key;
+    }
+    HIPCHK(hipGraphLaunch(ctx->day_exec, ctx->stream));
+    return RB_OK;
+}
+
 int rb_sync(rb_ctx *ctx)
 {
     if (!ctx) return RB_EINVAL;
@@ -554,6 +640,8 @@ int rb_finalize(rb_ctx *ctx)
     if (ctx->d_count) (void)hipFree(ctx->d_count);
     if (ctx->d_queue) (void)hipFree(ctx->d_queue);
     if (ctx->d_work) (void)hipFree(ctx->d_work);
+    if (ctx->day_exec) (void)hipGraphExecDestroy(ctx->day_exec);
+    if (ctx->day_graph) (void)hipGraphDestroy(ctx->day_graph);
     for (hipEvent_t e : {ctx->ev0, ctx->ev1, ctx->ev_h2d, ctx->ev_d2h})
         if (e) (void)hipEventDestroy(e);
     for (auto &pe : ctx->prof) {

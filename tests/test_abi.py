@@ -42,3 +42,4 @@ def test_struct_layouts_match_header():
     assert C.sizeof(_lib.Grid) == 5 * 4
     assert C.sizeof(_lib.Config) == 4 * 4
     assert C.sizeof(_lib.Stats) == 8 * 7
+    assert C.sizeof(_lib.DayConfig) == 4 * 4 + 3 * 8

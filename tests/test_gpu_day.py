@@ -1,0 +1,106 @@
+"""A day of sub-steps as one captured HIP graph: identical to the same steps launched one
+by one, on every replay (replays interleaved with the program's own eager work)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def step(table):
+    from reprobus_amd.chem import ChemistryStep
+
+    s = ChemistryStep(*table, device=0)
+    yield s
+    s.close()
+
+
+def _state(seed=8, nlon=24, nlat=13):
+    import torch
+    from reprobus_amd import synth
+    from reprobus_amd.chem import make_grid
+
+    inp = synth.make_c2(seed, nlon, nlat)
+    g = inp.grid
+    dev = torch.device("cuda:0")
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).to(dev)
+    lat, lon = inp.meta["lat"], inp.meta["lon"]
+    d = {"grid": make_grid(g.nlon, g.nlat, g.niv, g.nivbas, g.nivhau), "dt": inp.dt,
+         "trajlon": t(np.broadcast_to(lon[None, None, :], g.shape3)),
+         "trajlat": t(np.broadcast_to(lat[None, :, None], g.shape3)),
+         "T": t(inp.T), "p": t(inp.p), "alt": t(inp.alt), "q0": t(inp.qj1), "h0": t(inp.hc)}
+    shb = (g.nivbas, g.nlat, g.nlon)
+    d["sza"] = torch.zeros(shb, dtype=torch.float64, device=dev)
+    d["surf"] = torch.zeros(g.shape3, dtype=torch.float64, device=dev)
+    d["aux"] = [torch.zeros(shb, dtype=torch.float64, device=dev) for _ in range(3)]
+    d["iters"] = torch.zeros(shb, dtype=torch.int32, device=dev)
+    d["q"], d["h"] = d["q0"].clone(), d["h0"].clone()
+    return d
+
+
+def _reset(d):
+    d["q"].copy_(d["q0"])
+    d["h"].copy_(d["h0"])
+    d["surf"].zero_()
+    for a in d["aux"]:
+        a.zero_()
+
+
+def _day(step, d, use_graph, converged=False, nsteps=8):
+    step.run_day_device(d["grid"], nsteps, 172.0, d["dt"], d["trajlon"], d["trajlat"], d["T"],
+                        d["p"], d["alt"], d["sza"], d["q"], d["h"], d["surf"], *d["aux"],
+                        use_graph=use_graph, converged=converged, tol=1e-10, maxiter=500,
+                        iters=d["iters"] if converged else None)
+
+
+def _snap(d):
+    import torch
+
+    torch.cuda.synchronize()
+    return [d[k].cpu().numpy().copy() for k in ("q", "h", "surf", "sza")] + \
+        [a.cpu().numpy().copy() for a in d["aux"]]
+
+
+@pytest.mark.parametrize("converged", [False, True])
+def test_graph_day_equals_eager_day_on_every_replay(step, converged):
+    import torch
+
+    d = _state()
+    s = torch.cuda.Stream()
+    step.set_stream(s.cuda_stream)
+    with torch.cuda.stream(s):
+        _reset(d)
+        _day(step, d, use_graph=False, converged=converged)
+        eager = _snap(d)
+        for rep in range(3):
+            _reset(d)                                    # eager work between replays
+            d["q"].mul_(1.0)
+            _day(step, d, use_graph=True, converged=converged)
+            got = _snap(d)
+            for a, b in zip(got, eager):
+                assert np.array_equal(a, b), rep
+    step.set_stream(None)
+    assert np.isfinite(eager[0]).all()
+
+
+def test_day_equals_explicit_steps(step):
+    """The day runner's sub-step is device zenith at day0 + k dt/86400, then the step."""
+    import torch
+
+    d = _state(seed=9)
+    s = torch.cuda.Stream()
+    step.set_stream(s.cuda_stream)
+    with torch.cuda.stream(s):
+        _reset(d)
+        _day(step, d, use_graph=True, nsteps=4)
+        day = _snap(d)
+        _reset(d)
+        for k in range(4):
+            step.zenith_device(d["grid"], 172.0 + k * d["dt"] / 86400.0, d["trajlon"],
+                               d["trajlat"], d["sza"])
+            step.step_device(d["grid"], d["dt"], d["T"], d["p"], d["sza"], d["alt"], d["q"],
+                             d["h"], d["surf"], *d["aux"], fuse_conserv=True)
+        explicit = _snap(d)
+    step.set_stream(None)
+    for a, b in zip(day, explicit):
+        assert np.array_equal(a, b)
