@@ -1,10 +1,17 @@
 """Benchmark: REPROBUS chemistry step on one MI355X.
 
-Workload: the full 144 x 91 x 60 grid (786,240 boxes), one 900 s chemistry step
-per timed step (fixed 5 sweeps in FP64, conserv + floor fused), state resident
-in HBM; successive steps integrate the evolving state.  Prints one JSON line.
+Headline (`value`): the full 144 x 91 x 60 grid (786,240 boxes), one 900 s chemistry
+step per timed step -- the reference scheme, 5 fixed sweeps in FP64, conserv + floor
+fused -- state resident in HBM; successive steps integrate the evolving state.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+Beside it on the same line: the converged mode (sweeps to the fixed point) from the
+incoming state ("plain") and from the learned initial guess ("mlp_guess") on the
+786,240-box c2 grid (polar vortex), with their agreement; a 96-step day of
+(device zenith + step) as one HIP graph against the same steps launched one by one;
+the autotuned launch shapes; the roofline of the per-box kernel; and the reference
+Fortran timed on the host cores.  Prints one JSON line.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--no-extras]
 """
 from __future__ import annotations
 
@@ -26,6 +33,16 @@ FP64_PEAK_TFLOPS = 78.6        # MI355X FP64 (vector and matrix)
 # column sums, 43 qj1 and 15 hc; writes 43 qj1, 15 hc, surfarea and the 3
 # sedimentation fields = 65 + 62 doubles.
 BYTES_PER_BOX = (65 + 62) * 8
+# FP64 operations per box-step of the reference algorithm, counted exactly by running the
+# restatement with a counting number type (oracle/opcount.cpp): 2,201 add, 3,544 mul,
+# 595 div, 13 sqrt; plus 166 calls of exp/log/pow/trig, counted as one operation each.
+FLOP_PER_BOX = 2201 + 3544 + 595 + 13 + 166
+# Measured HBM bytes per k_chem<256> launch on this workload (FETCH_SIZE and WRITE_SIZE
+# scaled by a 1 GiB copy of known traffic; profiles/k_chem_c1_pmc.md).
+TRAFFIC_BYTES = {256: (1101882 + 1358277) * 1024}
+# FP64 VALU instructions per k_chem launch (SQ_INSTS_VALU_*_F64, same profile): add + mul
+# + transcendental + 2 x fma, per lane
+FP64_INSTR_FLOP = (68.80e6 + 55.70e6 + 7.10e6 + 2 * 72.58e6) * 64
 
 
 def parse():
@@ -34,10 +51,12 @@ def parse():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--block", type=int, default=256)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-procs", type=int, default=16)
     ap.add_argument("--cpu-nrep", type=int, default=8)
+    ap.add_argument("--no-extras", action="store_true",
+                    help="only the headline step (for profiler runs)")
+    ap.add_argument("--block", type=int, default=0, help="0: autotuned")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="after the timed region, one device copy of this many MB (a kernel of "
                          "known traffic, to scale profiler byte counters)")
@@ -73,15 +92,156 @@ def cpu_baseline(inp, nproc, nrep):
             "per_core": r["per_core_box_steps_per_s"]}
 
 
+def _device_state(inp, dev):
+    import torch
+
+    g = inp.grid
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).to(dev)
+    st = {"T": t(inp.T), "p": t(inp.p), "sza": t(inp.sza), "alt": t(inp.alt), "qj1": t(inp.qj1),
+          "hc": t(inp.hc), "surfarea": torch.zeros(g.shape3, dtype=torch.float64, device=dev),
+          "dt": inp.dt}
+    shb = (g.nivbas, g.nlat, g.nlon)
+    for k in ("vsed3d", "parth2osed", "parthno3sed"):
+        st[k] = torch.zeros(shb, dtype=torch.float64, device=dev)
+    return st
+
+
+def _timed(fn, reps, stream):
+    import torch
+
+    ms = []
+    for _ in range(reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms.append(e0.elapsed_time(e1))
+    return sorted(ms)[len(ms) // 2]
+
+
+def converged_section(table, block_conv, dev, tol=1e-10, maxiter=2000):
+    """Converged mode on the c2 grid (seed 1, the guess network's evaluation seed) from
+    the incoming state and from the learned guess; agreement over boxes that converged
+    on both paths."""
+    import torch
+
+    from reprobus_amd import guess, synth
+    from reprobus_amd.chem import ChemistryStep, make_grid
+    from reprobus_amd.mlp import GuessMLP
+
+    inp = synth.make_c2(1)
+    g = inp.grid
+    grid = make_grid(g.nlon, g.nlat, g.niv, g.nivbas, g.nivhau)
+    st = _device_state(inp, dev)
+    step = ChemistryStep(*table, device=0, block=block_conv)
+    net = GuessMLP(0)
+    if not guess.load(net):
+        return {"error": "no trained guess weights (reprobus_amd/data/guess_mlp.npz)"}
+    stream = torch.cuda.current_stream()
+    step.set_stream(stream.cuda_stream)
+    net.set_stream(stream.cuda_stream)
+    q0, h0 = st["qj1"].clone(), st["hc"].clone()
+    shb = (g.nivbas, g.nlat, g.nlon)
+    it_p = torch.zeros(shb, dtype=torch.int32, device=dev)
+    it_g = torch.zeros_like(it_p)
+    gq, gh = torch.zeros_like(q0), torch.zeros_like(h0)
+    args = [st[k] for k in ("T", "p", "sza", "alt")]
+    aux = [st[k] for k in ("surfarea", "vsed3d", "parth2osed", "parthno3sed")]
+
+    def plain():
+        st["qj1"].copy_(q0)
+        st["hc"].copy_(h0)
+        step.converge_device(grid, inp.dt, tol, maxiter, *args, st["qj1"], st["hc"], *aux,
+                             iters=it_p)
+
+    def guessed():
+        st["qj1"].copy_(q0)
+        st["hc"].copy_(h0)
+        net.guess(grid, inp.dt, st["T"], st["p"], st["sza"], st["qj1"], st["hc"], gq, gh)
+        step.converge_device(grid, inp.dt, tol, maxiter, *args, st["qj1"], st["hc"], *aux,
+                             guess_qj1=gq, guess_hc=gh, iters=it_g)
+
+    plain()
+    torch.cuda.synchronize()
+    ms_p = _timed(plain, 3, stream)
+    qp, hp = st["qj1"].clone(), st["hc"].clone()
+    guessed()
+    torch.cuda.synchronize()
+    ms_g = _timed(guessed, 3, stream)
+    nb = g.nivbas
+    rel = lambda a, b: ((a - b).abs() / torch.maximum(a.abs(), b.abs()).clamp(min=1e-30)).amax(0)
+    d = torch.maximum(rel(st["qj1"][:42, :nb], qp[:42, :nb]), rel(st["hc"][:, :nb], hp[:, :nb]))
+    live = (it_p > 0) & (it_g > 0)
+    dl = d[live]
+    nbox = g.nbox
+    out = {"workload": "c2 seed 1 (786,240 boxes, polar vortex), incoming state, dt 900 s",
+           "tol": tol, "maxiter": maxiter, "block": block_conv,
+           "plain": {"ms": ms_p, "box_steps_per_s": nbox / (ms_p / 1e3),
+                     "sweeps_mean": float(it_p.abs().float().mean()),
+                     "sweeps_max": int(it_p.abs().max()), "fallback_boxes": int((it_p < 0).sum())},
+           "mlp_guess": {"ms": ms_g, "box_steps_per_s": nbox / (ms_g / 1e3),
+                         "sweeps_mean": float(it_g.abs().float().mean()),
+                         "fallback_boxes": int((it_g < 0).sum())},
+           "speedup_mlp_over_plain": ms_p / ms_g,
+           "agreement": {"max_rel": float(dl.max()), "p99_rel": float(torch.quantile(dl[:2_000_000].float(), 0.99)),
+                         "boxes_over_1e-9": int((dl > 1e-9).sum()), "boxes_compared": int(live.sum())}}
+    step.close()
+    net.close()
+    return out
+
+
+def day_section(table, block, dev, nsteps=96):
+    """96 sub-steps (device zenith at each step's time + the 5-sweep step, conserv fused) of
+    the c2 day on the 786,240-box grid: one HIP graph against the launches one by one."""
+    import torch
+
+    from reprobus_amd import synth
+    from reprobus_amd.chem import ChemistryStep, make_grid
+
+    inp = synth.make_c2(0)
+    g = inp.grid
+    grid = make_grid(g.nlon, g.nlat, g.niv, g.nivbas, g.nivhau)
+    st = _device_state(inp, dev)
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).to(dev)
+    lat, lon = inp.meta["lat"], inp.meta["lon"]
+    tlon = t(np.broadcast_to(lon[None, None, :], g.shape3))
+    tlat = t(np.broadcast_to(lat[None, :, None], g.shape3))
+    sza = torch.zeros((g.nivbas, g.nlat, g.nlon), dtype=torch.float64, device=dev)
+    stream = torch.cuda.Stream()
+    step = ChemistryStep(*table, device=0, block=block)
+    step.set_stream(stream.cuda_stream)
+    q0, h0 = st["qj1"].clone(), st["hc"].clone()
+    aux = [st[k] for k in ("surfarea", "vsed3d", "parth2osed", "parthno3sed")]
+    res = {}
+    with torch.cuda.stream(stream):
+        for graph in (False, True, True):
+            st["qj1"].copy_(q0)
+            st["hc"].copy_(h0)
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            step.run_day_device(grid, nsteps, 265.0, inp.dt, tlon, tlat, st["T"], st["p"],
+                                st["alt"], sza, st["qj1"], st["hc"], *aux, use_graph=graph)
+            e1.record(stream)
+            torch.cuda.synchronize()
+            res["graph" if graph else "eager"] = e0.elapsed_time(e1)   # graph: the replay
+    step.close()
+    return {"workload": f"c2 seed 0 day: {nsteps} x (zenith + 5-sweep step), 786,240 boxes",
+            "ms_eager": res["eager"], "ms_graph": res["graph"],
+            "box_steps_per_s_graph": g.nbox * nsteps / (res["graph"] / 1e3)}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
               "(the step shards by whole columns with no communication)", file=sys.stderr)
         sys.exit(2)
+    import tempfile
+
     import torch
 
-    from reprobus_amd import jtable, synth
+    from reprobus_amd import autotune, jtable, synth
     from reprobus_amd.chem import ChemistryStep, make_grid
 
     t_gen = time.time()
@@ -89,14 +249,21 @@ def main():
     g = inp.grid
     grid = make_grid(g.nlon, g.nlat, g.niv, g.nivbas, g.nivhau)
     dev = torch.device("cuda:0")
-    t = lambda a: torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).to(dev)
-    T, p, sza, alt, q, h = t(inp.T), t(inp.p), t(inp.sza), t(inp.alt), t(inp.qj1), t(inp.hc)
-    surf = torch.zeros(g.shape3, dtype=torch.float64, device=dev)
-    shb = (g.nivbas, g.nlat, g.nlon)
-    vsed, p2o, pno3 = (torch.zeros(shb, dtype=torch.float64, device=dev) for _ in range(3))
+    st = _device_state(inp, dev)
     gen_s = time.time() - t_gen
+    table = jtable.default_table()
 
-    step = ChemistryStep(*jtable.default_table(), device=0, block=args.block, profile=True)
+    tuned = {}
+    block = args.block
+    if block == 0:
+        cache = os.path.join(tempfile.gettempdir(), "reprobus_amd_tune.json")
+        r = autotune.tune(table, grid, st, mode="plain", cache_path=cache)
+        block = r.block
+        tuned["plain"] = {"block": r.block, "ms": r.ms}
+
+    T, p, sza, alt, q, h = (st[k] for k in ("T", "p", "sza", "alt", "qj1", "hc"))
+    surf, vsed, p2o, pno3 = (st[k] for k in ("surfarea", "vsed3d", "parth2osed", "parthno3sed"))
+    step = ChemistryStep(*table, device=0, block=block, profile=True)
     stream = torch.cuda.current_stream()
     step.set_stream(stream.cuda_stream)
     run = lambda: step.step_device(grid, inp.dt, T, p, sza, alt, q, h, surf, vsed, p2o, pno3,
@@ -113,7 +280,7 @@ def main():
     e1.record(stream)
     torch.cuda.synchronize()
     ms_total = e0.elapsed_time(e1)
-    st = step.stats()
+    st_ = step.stats()
     finite = bool(torch.isfinite(q).all().item() and torch.isfinite(h).all().item())
     step.close()
     if args.calib_copy_mb > 0:
@@ -127,11 +294,35 @@ def main():
     nbox = g.nbox
     ms_per_step = ms_total / args.steps
     value = nbox * args.steps / (ms_total / 1e3)
-    chem_ms = st["chem_ms_total"] / max(st["chem_launches"], 1)
+    chem_ms = st_["chem_ms_total"] / max(st_["chem_launches"], 1)
     achieved_gbs = nbox * BYTES_PER_BOX / (chem_ms / 1e3) / 1e9
+    traffic = TRAFFIC_BYTES.get(block)
     roofline = {"bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": None,
-                "kernel": "k_chem", "kernel_ms": chem_ms, "bytes_per_box": BYTES_PER_BOX}
+                "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": traffic,
+                "kernel": f"k_chem<{block}>", "kernel_ms": chem_ms, "bytes_per_box": BYTES_PER_BOX,
+                "traffic_gbs": (traffic / (chem_ms / 1e3) / 1e9) if traffic else None,
+                "fp64": {"algorithmic_tflops": nbox * FLOP_PER_BOX / (chem_ms / 1e3) / 1e12,
+                         "instruction_tflops": (FP64_INSTR_FLOP / (chem_ms / 1e3) / 1e12
+                                                if block == 256 else None),
+                         "peak_tflops": FP64_PEAK_TFLOPS, "flop_per_box": FLOP_PER_BOX,
+                         "note": "instruction_tflops from the SQ_INSTS_VALU_*_F64 counters of "
+                                 "the block-256 kernel (profiles/k_chem_c1_pmc.md)"}}
+
+    extras = {}
+    if not args.no_extras:
+        try:
+            cache = os.path.join(tempfile.gettempdir(), "reprobus_amd_tune.json")
+            c2 = _device_state(synth.make_c2(1), dev)
+            r = autotune.tune(table, grid, c2, mode="converged", reps=2, cache_path=cache)
+            tuned["converged"] = {"block": r.block, "ms": r.ms}
+            del c2
+            extras["converged"] = converged_section(table, r.block, dev)
+        except Exception as exc:  # reported beside the headline, never instead of it
+            extras["converged"] = {"error": repr(exc)}
+        try:
+            extras["day"] = day_section(table, block, dev)
+        except Exception as exc:
+            extras["day"] = {"error": repr(exc)}
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -145,11 +336,12 @@ def main():
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
             "data": "synthetic",
             "config": {"workload": "c1 plain FP64 chemistry step: 144 lon x 91 lat x 60 levels "
-                                   "= 786,240 boxes, dt 900 s, 5 fixed sweeps, conserv+floor fused, "
-                                   "state resident in HBM",
+                                   "= 786,240 boxes, dt 900 s, 5 fixed sweeps (the reference "
+                                   "scheme), conserv+floor fused, state resident in HBM",
                        "boxes": nbox, "solver": "plain", "niter": 5, "seed": args.seed,
-                       "block": args.block, "finite": finite, "gen_s": round(gen_s, 1)},
-            "roofline": roofline, "cpu_baseline": cpu}
+                       "block": block, "autotune": tuned, "finite": finite,
+                       "gen_s": round(gen_s, 1)},
+            "roofline": roofline, "cpu_baseline": cpu, **extras}
     if cpu and "value" in cpu:
         line["gpu_over_cpu"] = value / cpu["value"]
     print(json.dumps(line))
