@@ -1251,24 +1251,25 @@ static void box_step(const troe_t *K, const double *ajl, const double *o3up, con
         /* F:2364-2372 */
         for (n = 1; n <= NBCON - 1; ++n) ck[n] = (ck0[n] + dt * pr[n]) / (1. + dt * pe[n]);
 
-        /* converged mode: stop at the fixed point of the sweep map (relative change of
-         * every species below tol, above the 1e-30 mole-fraction floor) */
+        /* converged mode: stop at the fixed point of the sweep map -- every species
+         * changed by less than tol relative to its new value (values under the 1e-30
+         * mole-fraction floor compared at the floor): |v - old| < tol max(|v|, floor) */
         if (tol > 0.) {
             const double fl = 1.e-30 * hnm;
-            double worst = 0.;
+            int conv = 1, fin = 1;
             for (n = 1; n <= NBCON - 1; ++n) {
-                double d = fabs(ck[n] - prev[n]) / fmax(fabs(ck[n]), fl);
-                if (!(d <= worst)) worst = d;
+                conv = conv && (fabs(ck[n] - prev[n]) < tol * fmax(fabs(ck[n]), fl));
+                fin = fin && (fabs(ck[n]) <= 1.7976931348623157e308);
             }
             for (n = 1; n <= NCM; ++n) {
-                double d = fabs(cc[n] - prev[NBCON - 1 + n]) / fmax(fabs(cc[n]), fl);
-                if (!(d <= worst)) worst = d;
+                conv = conv && (fabs(cc[n] - prev[NBCON - 1 + n]) < tol * fmax(fabs(cc[n]), fl));
+                fin = fin && (fabs(cc[n]) <= 1.7976931348623157e308);
             }
-            if (worst < tol) {
+            if (conv && fin) {
                 *iters = iter;
                 break;
             }
-            if (!(worst <= 1.7976931348623157e308) || iter == niter) {
+            if (!fin || iter == niter) {
                 *iters = -iter;       /* diverged or out of sweeps */
                 break;
             }

@@ -637,27 +637,32 @@ struct BoxEnv {
     bool day;
 };
 
-// Running maximum of the relative change |v - old| / max(|v|, fl); a NaN propagates, so
-// a box that goes non-finite never passes the convergence test.
-__device__ __forceinline__ double track_change(double chg, double v, double old, double fl)
+// The converged mode's fixed-point test, without divisions: a sweep has converged when
+// every long- and short-lived species changed by less than tol relative to its new value
+// (values under the floor fl compared at fl): |v - old| < tol max(|v|, fl); `fin` stays true
+// while every new value is finite.  A NaN fails both.
+struct Track {
+    bool conv, fin;
+    double tol, fl;
+};
+
+__device__ __forceinline__ void track(Track &t, double v, double old)
 {
-    const double d = fabs(v - old) / fmax(fabs(v), fl);
-    return (d <= chg) ? chg : d;
+    t.conv = t.conv && (fabs(v - old) < t.tol * fmax(fabs(v), t.fl));
+    t.fin = t.fin && (fabs(v) <= 1.7976931348623157e308);
 }
 
 // ck0 is the start-of-step state in LDS, one column per lane: ck0[s * CK0_STRIDE].
-// With TRACK, chg receives the largest relative change of any long- or short-lived
-// species over this sweep (values under the floor fl compared at fl): the converged
-// mode's fixed-point test.
+// With TRACK, tr accumulates the fixed-point test over this sweep.
 template <int CK0_STRIDE, bool TRACK>
 __device__ void sweep(const Rates &R, const double *tj, const BoxEnv &e,
                                       const double *ck0, double *ck, double *cc, double &ch3o,
-                                      double &hco, double &chg, double fl)
+                                      double &hco, Track &tr)
 {
 #define CCSET(sl, expr)                                                              \
     do {                                                                             \
         const double v_ = (expr);                                                    \
-        if (TRACK) chg = track_change(chg, v_, cc[sl], fl);                          \
+        if (TRACK) track(tr, v_, cc[sl]);                                            \
         cc[sl] = v_;                                                                 \
     } while (0)
     const double hnm = e.hnm, n2 = e.n2, o2 = e.o2, co2 = e.co2, h2 = e.h2;
@@ -902,7 +907,7 @@ __device__ void sweep(const Rates &R, const double *tj, const BoxEnv &e,
 #undef UPDATE
 #pragma unroll
     for (int s = 0; s < NLONG - 1; ++s) {
-        if (TRACK) chg = track_change(chg, nk[s], ck[s], fl);
+        if (TRACK) track(tr, nk[s], ck[s]);
         ck[s] = nk[s];
     }
 #undef CCSET
@@ -1119,11 +1124,12 @@ __global__ void __launch_bounds__(BLK) k_chem(StepArgs a)
     Rates R;
     thermal_rates(a.K, t, e.hnm, pm, R);
 
-    double ch3o = 0., hco = 0., nochg = 0.;
+    double ch3o = 0., hco = 0.;
+    Track notrack{true, true, 0., 0.};
     for (int it = 0; it < a.niter; ++it) {
         // keeps the LDS reads of ck0 inside the loop (not hoisted into registers)
         asm volatile("" ::: "memory");
-        sweep<BLK, false>(R, tj, e, ck0, ck, cc, ch3o, hco, nochg, 0.);
+        sweep<BLK, false>(R, tj, e, ck0, ck, cc, ch3o, hco, notrack);
     }
 
     // write-back (F:2380-2410)
@@ -1208,9 +1214,8 @@ __global__ void __launch_bounds__(BLK) k_chem_conv(StepArgs a, ConvArgs cv)
         if (__ballot(active) == 0ull) break;
         if (active) {
             asm volatile("" ::: "memory");
-            double chg = 0.;
-            sweep<BLK, true>(x.R, x.tj, x.e, ck0, x.ck, x.cc, x.ch3o, x.hco, chg,
-                             1.e-30 * x.e.hnm);
+            Track tr{true, true, cv.tol, 1.e-30 * x.e.hnm};
+            sweep<BLK, true>(x.R, x.tj, x.e, ck0, x.ck, x.cc, x.ch3o, x.hco, tr);
             ++it;
             if (fallback) {
                 if (it >= a.niter) {
@@ -1218,12 +1223,12 @@ __global__ void __launch_bounds__(BLK) k_chem_conv(StepArgs a, ConvArgs cv)
                     if (cv.iters) cv.iters[b] = -tried;
                     active = false;
                 }
-            } else if (chg < cv.tol) {
+            } else if (tr.conv && tr.fin) {
                 box_writeback(a, b, x);
                 if (cv.iters) cv.iters[b] = it;
                 active = false;
-            } else if (!(chg <= 1.7976931348623157e308) || it >= cv.maxiter) {
-                // diverged (NaN or infinite change) or out of sweeps
+            } else if (!tr.fin || it >= cv.maxiter) {
+                // diverged (a non-finite value) or out of sweeps
                 tried = it;
                 fallback = true;
                 restart = true;
