@@ -39,7 +39,7 @@ BYTES_PER_BOX = (65 + 62) * 8
 FLOP_PER_BOX = 2201 + 3544 + 595 + 13 + 166
 # Measured HBM bytes per k_chem<256> launch on this workload (FETCH_SIZE and WRITE_SIZE
 # scaled by a 1 GiB copy of known traffic; profiles/k_chem_c1_pmc.md).
-TRAFFIC_BYTES = {256: (1101882 + 1358277) * 1024}
+TRAFFIC_BYTES = {256: (1101882 + 1358277) * 1024, 128: (1097924 + 1334544) * 1024}
 # FP64 VALU instructions per k_chem launch (SQ_INSTS_VALU_*_F64, same profile): add + mul
 # + transcendental + 2 x fma, per lane
 FP64_INSTR_FLOP = (68.80e6 + 55.70e6 + 7.10e6 + 2 * 72.58e6) * 64
