@@ -8,7 +8,7 @@ module rbchem_iface
   implicit none
 
   integer(c_int), parameter :: RB_OK = 0, RB_EINVAL = -1, RB_EHIP = -2, RB_ENOTABLE = -3, &
-                               RB_ENONFINITE = -4, RB_ENOMEM = -5
+                               RB_ENONFINITE = -4, RB_ENOMEM = -5, RB_EIO = -6
 
   type, bind(C) :: rb_grid
      integer(c_int32_t) :: nlon, nlat, niv, nivbas, nivhau
@@ -17,6 +17,12 @@ module rbchem_iface
   type, bind(C) :: rb_config
      integer(c_int32_t) :: niter = 5, check_finite = 0, block = 256, profile = 0
   end type rb_config
+
+  ! rbio.h: the history record's header (namexp is 6 characters + NUL)
+  type, bind(C) :: rb_history_header
+     character(kind=c_char) :: namexp(7)
+     integer(c_int32_t) :: ian, imois, ijour, iheure, imin
+  end type rb_history_header
 
   interface
      integer(c_int) function rb_init(ctx, device, cfg) bind(C, name="rb_init")
@@ -49,6 +55,72 @@ module rbchem_iface
        type(rb_grid), intent(in) :: g
        real(c_double), intent(inout) :: qj1(*), hc(*)
      end function rb_conserv_floor
+
+     ! sedimentation(trajpre, trajtem, irapp, qj1, vsed3d, nivh2oecmwf, parth2osed,
+     ! parthno3sed, dtsedi), F:2416-2522
+     integer(c_int) function rb_sediment(ctx, g, dt, nivh2o, t, p, qj1, vsed3d, parth2osed, &
+                                         parthno3sed) bind(C, name="rb_sediment")
+       import :: c_int, c_int32_t, c_ptr, c_double, rb_grid
+       type(c_ptr), value :: ctx
+       type(rb_grid), intent(in) :: g
+       real(c_double), value :: dt
+       integer(c_int32_t), value :: nivh2o
+       real(c_double), intent(in) :: t(*), p(*), vsed3d(*), parth2osed(*), parthno3sed(*)
+       real(c_double), intent(inout) :: qj1(*)
+     end function rb_sediment
+
+     ! zenith, F:5308-5406: SZA of levels 1..nivbas of the trajectory points
+     integer(c_int) function rb_zenith(ctx, g, day, trajlon, trajlat, sza) bind(C, name="rb_zenith")
+       import :: c_int, c_ptr, c_double, rb_grid
+       type(c_ptr), value :: ctx
+       type(rb_grid), intent(in) :: g
+       real(c_double), value :: day
+       real(c_double), intent(in) :: trajlon(*), trajlat(*)
+       real(c_double), intent(out) :: sza(*)
+     end function rb_zenith
+
+     ! altitude (src/altitude.f90): geometric altitude of the trajectory points
+     integer(c_int) function rb_altitude(ctx, g, dlon, dlat, relief, psol, aaa, bbb, t, h2o, &
+                                         trajlon, trajlat, trajpre, alt) bind(C, name="rb_altitude")
+       import :: c_int, c_ptr, c_double, rb_grid
+       type(c_ptr), value :: ctx
+       type(rb_grid), intent(in) :: g
+       real(c_double), value :: dlon, dlat
+       real(c_double), intent(in) :: relief(*), psol(*), aaa(*), bbb(*), t(*), h2o(*)
+       real(c_double), intent(in) :: trajlon(*), trajlat(*), trajpre(*)
+       real(c_double), intent(out) :: alt(*)
+     end function rb_altitude
+
+     ! rbio.h: the history/restart record (F:587-588, F:231-233)
+     integer(c_int) function rb_history_write(path, nlon, nlat, niv, h, pj1, uj1, vj1, alt, tj1, &
+                                              qj1, surfarea, hc) bind(C, name="rb_history_write")
+       import :: c_int, c_int32_t, c_char, c_double, rb_history_header
+       character(kind=c_char), intent(in) :: path(*)
+       integer(c_int32_t), value :: nlon, nlat, niv
+       type(rb_history_header), intent(in) :: h
+       real(c_double), intent(in) :: pj1(*), uj1(*), vj1(*), alt(*), tj1(*), qj1(*)
+       real(c_double), intent(in) :: surfarea(*), hc(*)
+     end function rb_history_write
+
+     integer(c_int) function rb_history_read(path, nlon, nlat, niv, h, pj1, uj1, vj1, alt, tj1, &
+                                             qj1, surfarea, hc) bind(C, name="rb_history_read")
+       import :: c_int, c_int32_t, c_char, c_double, rb_history_header
+       character(kind=c_char), intent(in) :: path(*)
+       integer(c_int32_t), value :: nlon, nlat, niv
+       type(rb_history_header), intent(out) :: h
+       real(c_double), intent(out) :: pj1(*), uj1(*), vj1(*), alt(*), tj1(*), qj1(*)
+       real(c_double), intent(out) :: surfarea(*), hc(*)
+     end function rb_history_read
+
+     ! rbio.h: one analysis time of the ECMWF forcing file (F:4713-4935)
+     integer(c_int) function rb_ecmwf_read_time(path, itime, nlon, nlat, niv, idate, ps, t, w, h, &
+                                                u, v) bind(C, name="rb_ecmwf_read_time")
+       import :: c_int, c_int32_t, c_char, c_double
+       character(kind=c_char), intent(in) :: path(*)
+       integer(c_int32_t), value :: itime, nlon, nlat, niv
+       integer(c_int32_t), intent(out) :: idate(18)
+       real(c_double), intent(out) :: ps(*), t(*), w(*), h(*), u(*), v(*)
+     end function rb_ecmwf_read_time
 
      type(c_ptr) function rb_last_error(ctx) bind(C, name="rb_last_error")
        import :: c_ptr
