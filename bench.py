@@ -149,43 +149,46 @@ def converged_section(table, block_conv, dev, tol=1e-10, maxiter=2000):
     args = [st[k] for k in ("T", "p", "sza", "alt")]
     aux = [st[k] for k in ("surfarea", "vsed3d", "parth2osed", "parthno3sed")]
 
-    def plain():
-        st["qj1"].copy_(q0)
-        st["hc"].copy_(h0)
-        step.converge_device(grid, inp.dt, tol, maxiter, *args, st["qj1"], st["hc"], *aux,
-                             iters=it_p)
-
-    def guessed():
-        st["qj1"].copy_(q0)
-        st["hc"].copy_(h0)
-        net.guess(grid, inp.dt, st["T"], st["p"], st["sza"], st["qj1"], st["hc"], gq, gh)
-        step.converge_device(grid, inp.dt, tol, maxiter, *args, st["qj1"], st["hc"], *aux,
-                             guess_qj1=gq, guess_hc=gh, iters=it_g)
-
-    plain()
-    torch.cuda.synchronize()
-    ms_p = _timed(plain, 3, stream)
-    qp, hp = st["qj1"].clone(), st["hc"].clone()
-    guessed()
-    torch.cuda.synchronize()
-    ms_g = _timed(guessed, 3, stream)
     nb = g.nivbas
     rel = lambda a, b: ((a - b).abs() / torch.maximum(a.abs(), b.abs()).clamp(min=1e-30)).amax(0)
-    d = torch.maximum(rel(st["qj1"][:42, :nb], qp[:42, :nb]), rel(st["hc"][:, :nb], hp[:, :nb]))
-    live = (it_p > 0) & (it_g > 0)
-    dl = d[live]
     nbox = g.nbox
+    res, ref = {}, None
+    # (name, Anderson depth, learned guess)
+    for name, accel, use_guess in (("plain", 0, False), ("mlp_guess", 0, True),
+                                   ("accelerated", 2, False), ("mlp_guess_accelerated", 2, True)):
+        it = it_g if use_guess else it_p
+
+        def solve():
+            st["qj1"].copy_(q0)
+            st["hc"].copy_(h0)
+            if use_guess:
+                net.guess(grid, inp.dt, st["T"], st["p"], st["sza"], st["qj1"], st["hc"], gq, gh)
+            step.converge_device(grid, inp.dt, tol, maxiter, *args, st["qj1"], st["hc"], *aux,
+                                 guess_qj1=gq if use_guess else None,
+                                 guess_hc=gh if use_guess else None, iters=it, accel=accel)
+
+        solve()
+        torch.cuda.synchronize()
+        ms = _timed(solve, 3, stream)
+        r = {"ms": ms, "box_steps_per_s": nbox / (ms / 1e3),
+             "sweeps_mean": float(it.abs().float().mean()), "sweeps_max": int(it.abs().max()),
+             "fallback_boxes": int((it < 0).sum())}
+        if ref is None:
+            ref = (st["qj1"].clone(), st["hc"].clone(), it.clone())
+        else:
+            d = torch.maximum(rel(st["qj1"][:42, :nb], ref[0][:42, :nb]),
+                              rel(st["hc"][:, :nb], ref[1][:, :nb]))
+            live = (ref[2] > 0) & (it > 0)
+            dl = d[live]
+            r["agreement_with_plain"] = {
+                "max_rel": float(dl.max()), "p99_rel": float(torch.quantile(dl[:2_000_000].float(), 0.99)),
+                "boxes_over_1e-9": int((dl > 1e-9).sum()), "boxes_compared": int(live.sum())}
+            r["speedup_over_plain"] = res["plain"]["ms"] / ms
+        res[name] = r
     out = {"workload": "c2 seed 1 (786,240 boxes, polar vortex), incoming state, dt 900 s",
            "tol": tol, "maxiter": maxiter, "block": block_conv,
-           "plain": {"ms": ms_p, "box_steps_per_s": nbox / (ms_p / 1e3),
-                     "sweeps_mean": float(it_p.abs().float().mean()),
-                     "sweeps_max": int(it_p.abs().max()), "fallback_boxes": int((it_p < 0).sum())},
-           "mlp_guess": {"ms": ms_g, "box_steps_per_s": nbox / (ms_g / 1e3),
-                         "sweeps_mean": float(it_g.abs().float().mean()),
-                         "fallback_boxes": int((it_g < 0).sum())},
-           "speedup_mlp_over_plain": ms_p / ms_g,
-           "agreement": {"max_rel": float(dl.max()), "p99_rel": float(torch.quantile(dl[:2_000_000].float(), 0.99)),
-                         "boxes_over_1e-9": int((dl > 1e-9).sum()), "boxes_compared": int(live.sum())}}
+           "anderson": {"depth": 2, "from_sweep": 60}, **res,
+           "speedup_mlp_over_plain": res["plain"]["ms"] / res["mlp_guess"]["ms"]}
     step.close()
     net.close()
     return out

@@ -48,6 +48,10 @@ typedef struct {
     int32_t check_finite;   /* 1: count non-finite outputs and return RB_ENONFINITE */
     int32_t block;          /* threads per block of the per-box kernel: 64, 128, 256 (0: 256) */
     int32_t profile;        /* 1: time every launch of the per-box kernel with HIP events */
+    int32_t conv_refill;    /* converged mode: idle lanes of a wave that take new boxes
+                               together (1..64; 0: 16) -- one set-up pass serves them all */
+    int32_t conv_aa_start;  /* converged mode with accel: plain sweeps before this sweep
+                               number, Anderson mixing from it on (0: from the first) */
 } rb_config;
 
 typedef struct {
@@ -135,6 +139,8 @@ typedef struct {
     int32_t converged;
     int32_t maxiter;
     double day0, dt, tol;
+    int32_t accel;          /* converged days: Anderson depth, 0 or 2 */
+    int32_t pad_;
 } rb_day_config;
 
 int rb_run_day_dev(rb_ctx *ctx, const rb_grid *g, const rb_day_config *dc, const double *trajlon,
@@ -166,12 +172,18 @@ int rb_sediment_dev(rb_ctx *ctx, const rb_grid *g, double dt, int32_t nivh2o, co
  * lanes at once.  guess_qj1 / guess_hc (device, mixing ratios, state layout, or both
  * NULL) give the initial iterate instead of the incoming state -- the learned initial
  * guess; the start-of-step state ck0 is always qj1.  iters (device, nlon*nlat*nivbas
- * int32, or NULL) receives the sweep count of each box. */
+ * int32, or NULL) receives the sweep count of each box.  accel = 2 accelerates the
+ * iteration (Anderson mixing of depth 2 over the last three sweeps, Walker & Ni 2011,
+ * from sweep rb_config.conv_aa_start on): the same fixed point and stopping test, fewer
+ * sweeps; 0 sweeps plainly.  A box that
+ * goes non-finite or does not converge in maxiter sweeps gets the fixed-sweep answer
+ * (niter sweeps from the incoming state) and a negative count. */
 int rb_chem_converge_dev(rb_ctx *ctx, const rb_grid *g, double dt, double tol, int32_t maxiter,
-                         const double *T, const double *p, const double *sza, const double *alt,
-                         double *qj1, double *hc, double *surfarea, double *vsed3d,
-                         double *parth2osed, double *parthno3sed, const double *guess_qj1,
-                         const double *guess_hc, int32_t *iters, int fuse_conserv);
+                         int32_t accel, const double *T, const double *p, const double *sza,
+                         const double *alt, double *qj1, double *hc, double *surfarea,
+                         double *vsed3d, double *parth2osed, double *parthno3sed,
+                         const double *guess_qj1, const double *guess_hc, int32_t *iters,
+                         int fuse_conserv);
 
 int rb_sync(rb_ctx *ctx);
 /* Waits for the timed launches, fills *out and resets the profile accumulators. */

@@ -53,6 +53,11 @@ def lib(cr_psc: bool = False):
             np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS"), C.c_void_p, C.c_void_p,
             C.c_int]
         L.rbo_chem_converge.restype = C.c_int
+        L.rbo_chem_converge_aa.argtypes = [C.POINTER(_Grid), C.c_double, C.c_double, C.c_int,
+                                           C.c_int] + \
+            [dp] * 12 + [np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS"), C.c_void_p,
+                         C.c_void_p, C.c_int]
+        L.rbo_chem_converge_aa.restype = C.c_int
         L.rbo_troe_constants.argtypes = [dp]
         L.rbo_troe.argtypes = [C.c_double] * 6
         L.rbo_troe.restype = C.c_double
@@ -104,7 +109,8 @@ def fortran_ajl(ajl):
 
 
 def chem_converge(inp, ajl, o3up, tol: float = 1e-10, maxiter: int = 2000, nthreads: int = 0,
-                  conserv: bool = True, cr_psc: bool = False, guess=None) -> dict:
+                  conserv: bool = True, cr_psc: bool = False, guess=None, accel: int = 0,
+                  aa_start: int = 0) -> dict:
     """Converged mode of the restatement: sweeps to the fixed point (per-box counts in 'iters').
     guess = (qj1, hc) mixing ratios in state layout replaces the initial iterate."""
     g = inp.grid
@@ -121,10 +127,12 @@ def chem_converge(inp, ajl, o3up, tol: float = 1e-10, maxiter: int = 2000, nthre
     if guess is not None:
         gq, gh = c(guess[0]), c(guess[1])
         assert gq.shape == q.shape and gh.shape == h.shape
-    rc = L.rbo_chem_converge(C.byref(G), float(inp.dt), float(tol), c(inp.T), c(inp.p), c(inp.sza),
-                             c(inp.alt), fortran_ajl(ajl), c(o3up), q, h, surf, vsed, p2o, pno3,
-                             iters, None if gq is None else gq.ctypes.data,
-                             None if gh is None else gh.ctypes.data, nthreads)
+    rc = L.rbo_chem_converge_aa(C.byref(G), float(inp.dt), float(tol), int(accel), int(aa_start),
+                                c(inp.T),
+                                c(inp.p), c(inp.sza), c(inp.alt), fortran_ajl(ajl), c(o3up), q, h,
+                                surf, vsed, p2o, pno3, iters,
+                                None if gq is None else gq.ctypes.data,
+                                None if gh is None else gh.ctypes.data, nthreads)
     if rc != 0:
         raise RuntimeError(f"rbo_chem_converge failed: {rc}")
     out = {"qj1_chem": q.copy(), "hc_chem": h.copy(), "surfarea": surf, "vsed3d": vsed,

@@ -903,13 +903,93 @@ typedef struct {
     int het_level;
 } boxenv_t;
 
+/* Anderson acceleration (Walker & Ni 2011, type II, depth m <= AA_MAX) of the sweep map F
+ * on x = (ck[1..42], cc[1..15]): from x_k and f = F(x_k), the next iterate is
+ * f - dF gamma, gamma minimising || w (g_k - dG gamma) || with g = F(x) - x, dF/dG the
+ * differences of the last m pairs, weights w_i = 1/max(|f_i|, floor) (relative scale).
+ * Components that would turn non-positive keep f_i. */
+#define AA_MAX 4
+#define NX (NBCON - 1 + NCM)
+typedef struct {
+    int m, n;                       /* depth; stored pairs (<= m + 1, ring) */
+    int head;
+    double f[AA_MAX + 1][NX], g[AA_MAX + 1][NX];
+} aa_t;
+
+static void aa_update(aa_t *A, double *x, const double *xprev, double fl)
+{
+    /* x holds f = F(x_k) on entry and x_{k+1} on exit */
+    double w[NX], gk[NX];
+    for (int i = 0; i < NX; ++i) {
+        gk[i] = x[i] - xprev[i];
+        const double s = fmax(fabs(x[i]), fl);
+        w[i] = 1. / (s * s);
+    }
+    const int slot = A->head;
+    memcpy(A->f[slot], x, sizeof(double) * NX);
+    memcpy(A->g[slot], gk, sizeof(double) * NX);
+    A->head = (A->head + 1) % (A->m + 1);
+    if (A->n < A->m + 1) A->n++;
+    const int mk = A->n - 1;
+    if (mk < 1) return;
+    /* columns j = 1..mk: newest minus the one before it */
+    double dg[AA_MAX][NX], df[AA_MAX][NX];
+    for (int j = 0; j < mk; ++j) {
+        const int a = (slot - j + (A->m + 1)) % (A->m + 1);
+        const int b = (slot - j - 1 + 2 * (A->m + 1)) % (A->m + 1);
+        for (int i = 0; i < NX; ++i) {
+            dg[j][i] = A->g[a][i] - A->g[b][i];
+            df[j][i] = A->f[a][i] - A->f[b][i];
+        }
+    }
+    /* normal equations (dG^T W dG) gamma = dG^T W g_k, Gaussian elimination with pivoting */
+    double M[AA_MAX][AA_MAX + 1];
+    for (int r = 0; r < mk; ++r) {
+        for (int c = 0; c < mk; ++c) {
+            double acc = 0.;
+            for (int i = 0; i < NX; ++i) acc += dg[r][i] * w[i] * dg[c][i];
+            M[r][c] = acc;
+        }
+        double acc = 0.;
+        for (int i = 0; i < NX; ++i) acc += dg[r][i] * w[i] * gk[i];
+        M[r][mk] = acc;
+    }
+    for (int r = 0; r < mk; ++r) M[r][r] *= (1. + 1.e-10);      /* Tikhonov nudge */
+    for (int c = 0; c < mk; ++c) {
+        int piv = c;
+        for (int r = c + 1; r < mk; ++r)
+            if (fabs(M[r][c]) > fabs(M[piv][c])) piv = r;
+        if (!(fabs(M[piv][c]) > 0.)) return;                    /* singular: plain step */
+        if (piv != c)
+            for (int k = 0; k <= mk; ++k) {
+                const double t_ = M[c][k]; M[c][k] = M[piv][k]; M[piv][k] = t_;
+            }
+        for (int r = c + 1; r < mk; ++r) {
+            const double f_ = M[r][c] / M[c][c];
+            for (int k = c; k <= mk; ++k) M[r][k] -= f_ * M[c][k];
+        }
+    }
+    double gam[AA_MAX];
+    for (int r = mk - 1; r >= 0; --r) {
+        double acc = M[r][mk];
+        for (int k = r + 1; k < mk; ++k) acc -= M[r][k] * gam[k];
+        gam[r] = acc / M[r][r];
+        if (!(fabs(gam[r]) <= 1.e6)) return;                    /* wild: plain step */
+    }
+    for (int i = 0; i < NX; ++i) {
+        double v = x[i];
+        for (int j = 0; j < mk; ++j) v -= gam[j] * df[j][i];
+        if (v > 0. && v <= 1.7976931348623157e308) x[i] = v;
+    }
+}
+
 /* chem3d for one box: F:800-833 (densities), F:857-912 (J, aerosol), F:920-1306
  * (rates), F:1315-1338 (rain-out), F:1348-2378 (sweeps), F:2380-2410 (write-back).
  * q[1..43] and h[1..15] are mixing ratios, updated in place. */
 static void box_step(const troe_t *K, const double *ajl, const double *o3up, const boxenv_t *e,
                      double dt, int niter, double *q, double *h, double *surf, double *vsed,
                      double *p2osed, double *pno3sed, int *surf_written, double tol, int *iters,
-                     const double *gq, const double *gh)
+                     const double *gq, const double *gh, int accel, int aa_start)
 {
     double ck0[NBCON + 1], ck[NBCON + 1], cc[NCM + 1], tj[NPHOT + 1], pr[NBCON + 1],
         pe[NBCON + 1];
@@ -972,6 +1052,10 @@ static void box_step(const troe_t *K, const double *ajl, const double *o3up, con
         for (n = 1; n <= NCM; ++n) cc[n] = gh[n] * hnm;
     }
 
+    aa_t aa;
+    aa.m = accel < AA_MAX ? accel : AA_MAX;
+    aa.n = 0;
+    aa.head = 0;
     *iters = niter;
     for (int iter = 1; iter <= niter; ++iter) {
         double prev[NBCON + NCM + 1];
@@ -1273,6 +1357,14 @@ static void box_step(const troe_t *K, const double *ajl, const double *o3up, con
                 *iters = -iter;       /* diverged or out of sweeps */
                 break;
             }
+            if (accel > 0 && iter >= aa_start) {
+                double xv[NX];
+                for (n = 1; n <= NBCON - 1; ++n) xv[n - 1] = ck[n];
+                for (n = 1; n <= NCM; ++n) xv[NBCON - 2 + n] = cc[n];
+                aa_update(&aa, xv, prev + 1, fl);
+                for (n = 1; n <= NBCON - 1; ++n) ck[n] = xv[n - 1];
+                for (n = 1; n <= NCM; ++n) cc[n] = xv[NBCON - 2 + n];
+            }
         }
     }
 
@@ -1287,7 +1379,8 @@ static int chem_grid(const rbo_grid *g, double dt, const double *T, const double
                      const double *sza, const double *alt, const double *ajl, const double *o3up,
                      double *qj1, double *hc, double *surfarea, double *vsed3d,
                      double *parth2osed, double *parthno3sed, int nthreads, double tol,
-                     int *iters, const double *guess_qj1, const double *guess_hc);
+                     int *iters, const double *guess_qj1, const double *guess_hc, int accel,
+                     int aa_start);
 
 int rbo_chem_step(const rbo_grid *g, double dt, const double *T, const double *p,
                   const double *sza, const double *alt, const double *ajl, const double *o3up,
@@ -1295,7 +1388,7 @@ int rbo_chem_step(const rbo_grid *g, double dt, const double *T, const double *p
                   double *parth2osed, double *parthno3sed, int nthreads)
 {
     return chem_grid(g, dt, T, p, sza, alt, ajl, o3up, qj1, hc, surfarea, vsed3d, parth2osed,
-                     parthno3sed, nthreads, 0., NULL, NULL, NULL);
+                     parthno3sed, nthreads, 0., NULL, NULL, NULL, 0, 0);
 }
 
 int rbo_chem_converge(const rbo_grid *g, double dt, double tol, const double *T, const double *p,
@@ -1304,16 +1397,29 @@ int rbo_chem_converge(const rbo_grid *g, double dt, double tol, const double *T,
                       double *parth2osed, double *parthno3sed, int *iters,
                       const double *guess_qj1, const double *guess_hc, int nthreads)
 {
-    if ((guess_qj1 == NULL) != (guess_hc == NULL)) return -1;
+    return rbo_chem_converge_aa(g, dt, tol, 0, 0, T, p, sza, alt, ajl, o3up, qj1, hc, surfarea,
+                                vsed3d, parth2osed, parthno3sed, iters, guess_qj1, guess_hc,
+                                nthreads);
+}
+
+int rbo_chem_converge_aa(const rbo_grid *g, double dt, double tol, int accel, int aa_start,
+                         const double *T, const double *p, const double *sza, const double *alt,
+                         const double *ajl, const double *o3up, double *qj1, double *hc,
+                         double *surfarea, double *vsed3d, double *parth2osed,
+                         double *parthno3sed, int *iters, const double *guess_qj1,
+                         const double *guess_hc, int nthreads)
+{
+    if ((guess_qj1 == NULL) != (guess_hc == NULL) || accel < 0 || accel > AA_MAX) return -1;
     return chem_grid(g, dt, T, p, sza, alt, ajl, o3up, qj1, hc, surfarea, vsed3d, parth2osed,
-                     parthno3sed, nthreads, tol, iters, guess_qj1, guess_hc);
+                     parthno3sed, nthreads, tol, iters, guess_qj1, guess_hc, accel, aa_start);
 }
 
 static int chem_grid(const rbo_grid *g, double dt, const double *T, const double *p,
                      const double *sza, const double *alt, const double *ajl, const double *o3up,
                      double *qj1, double *hc, double *surfarea, double *vsed3d,
                      double *parth2osed, double *parthno3sed, int nthreads, double tol,
-                     int *iters, const double *guess_qj1, const double *guess_hc)
+                     int *iters, const double *guess_qj1, const double *guess_hc, int accel,
+                     int aa_start)
 {
     const size_t plane = (size_t)g->nlon * g->nlat;          /* one level of a field */
     const size_t field = plane * g->niv;                       /* one species field */
@@ -1352,7 +1458,7 @@ static int chem_grid(const rbo_grid *g, double dt, const double *T, const double
                 for (int n = 1; n <= NCM; ++n) gh[n] = guess_hc[(size_t)(n - 1) * field + b];
             }
             box_step(&K, ajl, o3up, &e, dt, g->niter, q, h, &surf, &vsed, &p2o, &pno3, &sw, tol,
-                     &it, guess_qj1 ? gq : NULL, guess_qj1 ? gh : NULL);
+                     &it, guess_qj1 ? gq : NULL, guess_qj1 ? gh : NULL, accel, aa_start);
             if (it < 0) {
                 /* converged mode without convergence: the reference answer, RBO_NITER_REF
                  * sweeps from the incoming state */
@@ -1360,7 +1466,7 @@ static int chem_grid(const rbo_grid *g, double dt, const double *T, const double
                 for (int n = 1; n <= NCM; ++n) h[n] = hc[(size_t)(n - 1) * field + b];
                 int it5;
                 box_step(&K, ajl, o3up, &e, dt, RBO_NITER_REF, q, h, &surf, &vsed, &p2o, &pno3,
-                         &sw, 0., &it5, NULL, NULL);
+                         &sw, 0., &it5, NULL, NULL, 0, 0);
             }
             if (iters) iters[b] = it;
             for (int n = 1; n <= NBCON; ++n) qj1[(size_t)(n - 1) * field + b] = q[n];

@@ -45,6 +45,16 @@ int rbo_chem_converge(const rbo_grid *g, double dt, double tol, const double *T,
                       double *parth2osed, double *parthno3sed, int *iters,
                       const double *guess_qj1, const double *guess_hc, int nthreads);
 
+/* Converged mode with Anderson acceleration of depth accel (0 = plain sweeps, <= 4) on the
+ * sweep map from sweep aa_start on (plain sweeps before it): the same fixed point and
+ * stopping test (the plain map's residual), a different sequence of iterates. */
+int rbo_chem_converge_aa(const rbo_grid *g, double dt, double tol, int accel, int aa_start,
+                         const double *T, const double *p, const double *sza, const double *alt,
+                         const double *ajl, const double *o3up, double *qj1, double *hc,
+                         double *surfarea, double *vsed3d, double *parth2osed,
+                         double *parthno3sed, int *iters, const double *guess_qj1,
+                         const double *guess_hc, int nthreads);
+
 /* conserv (F:2526-2616) on levels 1..nivbas, then the 1e-30 floor of
  * species 1..42 on all niv levels (F:431-440), then HNO3 gas = HNO3 on levels
  * nivbas+1..niv (F:444-450). */

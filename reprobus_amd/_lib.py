@@ -38,13 +38,13 @@ class Grid(C.Structure):
 
 class Config(C.Structure):
     _fields_ = [("niter", C.c_int32), ("check_finite", C.c_int32), ("block", C.c_int32),
-                ("profile", C.c_int32)]
+                ("profile", C.c_int32), ("conv_refill", C.c_int32), ("conv_aa_start", C.c_int32)]
 
 
 class DayConfig(C.Structure):
     _fields_ = [("nsteps", C.c_int32), ("use_graph", C.c_int32), ("converged", C.c_int32),
                 ("maxiter", C.c_int32), ("day0", C.c_double), ("dt", C.c_double),
-                ("tol", C.c_double)]
+                ("tol", C.c_double), ("accel", C.c_int32), ("pad_", C.c_int32)]
 
 
 class Stats(C.Structure):
@@ -90,7 +90,7 @@ def load() -> C.CDLL:
     L.rb_conserv_floor.argtypes = [P, C.POINTER(Grid), P, P]
     L.rb_chem_step_dev.argtypes = [P, C.POINTER(Grid), d] + [P] * 10 + [C.c_int]
     L.rb_conserv_floor_dev.argtypes = [P, C.POINTER(Grid), P, P]
-    L.rb_chem_converge_dev.argtypes = [P, C.POINTER(Grid), d, d, C.c_int32] + [P] * 13 + [C.c_int]
+    L.rb_chem_converge_dev.argtypes = [P, C.POINTER(Grid), d, d, C.c_int32, C.c_int32] + [P] * 13 + [C.c_int]
     L.rb_sediment.argtypes = [P, C.POINTER(Grid), d, C.c_int32] + [P] * 6
     L.rb_sediment_dev.argtypes = [P, C.POINTER(Grid), d, C.c_int32] + [P] * 6
     L.rb_zenith.argtypes = [P, C.POINTER(Grid), d, P, P, P]

@@ -40,10 +40,12 @@ class ChemistryStep:
     """One library context: device, stream, photolysis table, work buffers."""
 
     def __init__(self, ajl: np.ndarray, o3up: np.ndarray, device: int = 0, niter: int = 5,
-                 block: int = 256, check_finite: bool = False, profile: bool = False):
+                 block: int = 256, check_finite: bool = False, profile: bool = False,
+                 conv_refill: int = 16, conv_aa_start: int = 60):
         self._L = _lib.load()
         self._ctx = C.c_void_p()
-        cfg = Config(niter, 1 if check_finite else 0, block, 1 if profile else 0)
+        cfg = Config(niter, 1 if check_finite else 0, block, 1 if profile else 0, conv_refill,
+                     conv_aa_start)
         rc = self._L.rb_init(C.byref(self._ctx), device, C.byref(cfg))
         if rc != 0:
             raise RbError(rc, f"rb_init(device={device}) failed")
@@ -106,13 +108,15 @@ class ChemistryStep:
 
     def converge_device(self, grid: Grid, dt: float, tol: float, maxiter: int, T, p, sza, alt,
                         qj1, hc, surfarea, vsed3d, parth2osed, parthno3sed, guess_qj1=None,
-                        guess_hc=None, iters=None, fuse_conserv: bool = True) -> None:
-        """Converged mode: sweep every box to the fixed point (optionally from a guess)."""
+                        guess_hc=None, iters=None, fuse_conserv: bool = True,
+                        accel: int = 0) -> None:
+        """Converged mode: sweep every box to the fixed point (optionally from a guess;
+        accel = 2: Anderson-accelerated sweeps)."""
         g_q = _ptr(guess_qj1) if guess_qj1 is not None else None
         g_h = _ptr(guess_hc) if guess_hc is not None else None
         it = iters.data_ptr() if iters is not None else None
         self._check(self._L.rb_chem_converge_dev(
-            self._ctx, C.byref(grid), float(dt), float(tol), int(maxiter),
+            self._ctx, C.byref(grid), float(dt), float(tol), int(maxiter), int(accel),
             *[_ptr(a) for a in (T, p, sza, alt, qj1, hc, surfarea, vsed3d, parth2osed,
                                 parthno3sed)], g_q, g_h, it, 1 if fuse_conserv else 0))
 
@@ -161,11 +165,11 @@ class ChemistryStep:
     def run_day_device(self, grid: Grid, nsteps: int, day0: float, dt: float, trajlon, trajlat,
                        T, p, alt, sza, qj1, hc, surfarea, vsed3d, parth2osed, parthno3sed,
                        use_graph: bool = True, converged: bool = False, tol: float = 1e-10,
-                       maxiter: int = 2000, iters=None) -> None:
+                       maxiter: int = 2000, iters=None, accel: int = 0) -> None:
         """nsteps sub-steps (device zenith + chemistry, conserv fused), optionally as one
         captured HIP graph."""
         dc = DayConfig(int(nsteps), 1 if use_graph else 0, 1 if converged else 0, int(maxiter),
-                       float(day0), float(dt), float(tol))
+                       float(day0), float(dt), float(tol), int(accel), 0)
         self._check(self._L.rb_run_day_dev(
             self._ctx, C.byref(grid), C.byref(dc),
             *[_ptr(a) for a in (trajlon, trajlat, T, p, alt, sza, qj1, hc, surfarea, vsed3d,

@@ -51,14 +51,25 @@ struct ConvArgs {
     int32_t *iters;                    // per-box sweep count (nbox) or null; -n: the box
                                        // did not converge in n sweeps and holds the
                                        // fixed-sweep (niter) answer instead
+    int32_t accel;                     // Anderson depth: 0 (plain sweeps) or 2
+    double *aa_ws;                     // accel > 0: 2 (accel + 1) * 57 doubles per lane
+    int32_t refill;                    // idle lanes of a wave before they take new boxes
+    const double *rec;                 // per-box set-up records (launch_conv_setup)
+    int32_t aa_start;                  // accel > 0: plain sweeps before this sweep number
 };
+// doubles per box of the converged mode's set-up record
+constexpr int CONV_RECORD = 166;
+hipError_t launch_conv_setup(const StepArgs &a, double *rec, int block, hipStream_t s);
+// Anderson depths the converged kernel is built for
+constexpr int AA_DEPTH = 2;
+constexpr int AA_NX = 57;
 
 Troe troe_constants();
 hipError_t launch_columns(const StepArgs &a, double *col, hipStream_t s);
 hipError_t launch_chem(const StepArgs &a, int block, hipStream_t s);
 hipError_t launch_chem_conv(const StepArgs &a, const ConvArgs &cv, int block, int nblocks,
                             hipStream_t s);
-int conv_blocks_per_cu(int block);
+int conv_blocks_per_cu(int block, int accel);
 hipError_t launch_zero_counter(unsigned long long *p, hipStream_t s);
 // conserv on boxes below plane*nivbas and the floor on every box, for boxes [start, field)
 // day-dependent terms of zenith (F:5334-5380), computed once per call on the host

@@ -641,15 +641,21 @@ struct BoxEnv {
 // every long- and short-lived species changed by less than tol relative to its new value
 // (values under the floor fl compared at fl): |v - old| < tol max(|v|, fl); `fin` stays true
 // while every new value is finite.  A NaN fails both.
+// With g set (Anderson acceleration), the change v - old of species index i (long-lived
+// 0..41, then short-lived 42..56) is also stored at g[i * gstride]: the residual of the
+// sweep map at the previous iterate.
 struct Track {
     bool conv, fin;
     double tol, fl;
+    double *g;
+    int64_t gstride;
 };
 
-__device__ __forceinline__ void track(Track &t, double v, double old)
+__device__ __forceinline__ void track(Track &t, double v, double old, int i)
 {
     t.conv = t.conv && (fabs(v - old) < t.tol * fmax(fabs(v), t.fl));
     t.fin = t.fin && (fabs(v) <= 1.7976931348623157e308);
+    if (t.g) t.g[i * t.gstride] = v - old;
 }
 
 // ck0 is the start-of-step state in LDS, one column per lane: ck0[s * CK0_STRIDE].
@@ -662,7 +668,7 @@ __device__ void sweep(const Rates &R, const double *tj, const BoxEnv &e,
 #define CCSET(sl, expr)                                                              \
     do {                                                                             \
         const double v_ = (expr);                                                    \
-        if (TRACK) track(tr, v_, cc[sl]);                                            \
+        if (TRACK) track(tr, v_, cc[sl], NLONG - 1 + (sl));                          \
         cc[sl] = v_;                                                                 \
     } while (0)
     const double hnm = e.hnm, n2 = e.n2, o2 = e.o2, co2 = e.co2, h2 = e.h2;
@@ -907,7 +913,7 @@ __device__ void sweep(const Rates &R, const double *tj, const BoxEnv &e,
 #undef UPDATE
 #pragma unroll
     for (int s = 0; s < NLONG - 1; ++s) {
-        if (TRACK) track(tr, nk[s], ck[s]);
+        if (TRACK) track(tr, nk[s], ck[s], s);
         ck[s] = nk[s];
     }
 #undef CCSET
@@ -1059,6 +1065,394 @@ __device__ __forceinline__ void box_writeback(const StepArgs &a, int64_t b, cons
     a.pno3[b] = x.h.pno3sl;
 }
 
+
+// Per-box set-up record of the converged mode: everything box_setup computes that the
+// sweeps read (densities, heterogeneous and rain-out rates, thermal rates, J-values, the
+// aerosol outputs), stored field-major (rec[f * nbox + b]) so that a wave's loads and
+// stores of one field coalesce.  166 doubles per box.
+constexpr int NREC = 166;
+static_assert(NJ == 41, "record layout assumes 41 J-values");
+static_assert(NREC == CONV_RECORD, "record size");
+
+__device__ __forceinline__ void rec_store(const Box &x, double *__restrict__ rec, int64_t nbox,
+                                          int64_t b)
+{
+    rec[(int64_t)0 * nbox + b] = x.e.hnm;
+    rec[(int64_t)1 * nbox + b] = x.e.n2;
+    rec[(int64_t)2 * nbox + b] = x.e.o2;
+    rec[(int64_t)3 * nbox + b] = x.e.co2;
+    rec[(int64_t)4 * nbox + b] = x.e.h2;
+    rec[(int64_t)5 * nbox + b] = x.e.dt;
+    rec[(int64_t)6 * nbox + b] = x.e.het1;
+    rec[(int64_t)7 * nbox + b] = x.e.het2;
+    rec[(int64_t)8 * nbox + b] = x.e.het3;
+    rec[(int64_t)9 * nbox + b] = x.e.het4;
+    rec[(int64_t)10 * nbox + b] = x.e.het5;
+    rec[(int64_t)11 * nbox + b] = x.e.het6;
+    rec[(int64_t)12 * nbox + b] = x.e.het7;
+    rec[(int64_t)13 * nbox + b] = x.e.het8;
+    rec[(int64_t)14 * nbox + b] = x.e.het9;
+    rec[(int64_t)15 * nbox + b] = x.e.pno3;
+    rec[(int64_t)16 * nbox + b] = x.e.ph2o;
+    rec[(int64_t)17 * nbox + b] = x.e.phcl;
+    rec[(int64_t)18 * nbox + b] = x.e.pbr;
+    rec[(int64_t)19 * nbox + b] = x.e.rain_hno3;
+    rec[(int64_t)20 * nbox + b] = x.e.rain_hno4;
+    rec[(int64_t)21 * nbox + b] = x.e.rain_hcl;
+    rec[(int64_t)22 * nbox + b] = x.e.rain_hbr;
+    rec[(int64_t)23 * nbox + b] = x.e.rain_h2o2;
+    rec[(int64_t)24 * nbox + b] = x.e.rain_ch2o;
+    rec[(int64_t)25 * nbox + b] = x.e.rain_ch3o2h;
+    rec[(int64_t)26 * nbox + b] = x.R.a1;
+    rec[(int64_t)27 * nbox + b] = x.R.a1et;
+    rec[(int64_t)28 * nbox + b] = x.R.a2;
+    rec[(int64_t)29 * nbox + b] = x.R.a5;
+    rec[(int64_t)30 * nbox + b] = x.R.a6;
+    rec[(int64_t)31 * nbox + b] = x.R.a6b;
+    rec[(int64_t)32 * nbox + b] = x.R.a7;
+    rec[(int64_t)33 * nbox + b] = x.R.a17;
+    rec[(int64_t)34 * nbox + b] = x.R.a19;
+    rec[(int64_t)35 * nbox + b] = x.R.a26;
+    rec[(int64_t)36 * nbox + b] = x.R.a27;
+    rec[(int64_t)37 * nbox + b] = x.R.a36;
+    rec[(int64_t)38 * nbox + b] = x.R.b3;
+    rec[(int64_t)39 * nbox + b] = x.R.b4;
+    rec[(int64_t)40 * nbox + b] = x.R.b6;
+    rec[(int64_t)41 * nbox + b] = x.R.b7;
+    rec[(int64_t)42 * nbox + b] = x.R.b9;
+    rec[(int64_t)43 * nbox + b] = x.R.b12;
+    rec[(int64_t)44 * nbox + b] = x.R.b22;
+    rec[(int64_t)45 * nbox + b] = x.R.b23;
+    rec[(int64_t)46 * nbox + b] = x.R.b24;
+    rec[(int64_t)47 * nbox + b] = x.R.b27;
+    rec[(int64_t)48 * nbox + b] = x.R.b28;
+    rec[(int64_t)49 * nbox + b] = x.R.b32;
+    rec[(int64_t)50 * nbox + b] = x.R.b38;
+    rec[(int64_t)51 * nbox + b] = x.R.b39;
+    rec[(int64_t)52 * nbox + b] = x.R.c2;
+    rec[(int64_t)53 * nbox + b] = x.R.c4;
+    rec[(int64_t)54 * nbox + b] = x.R.c5;
+    rec[(int64_t)55 * nbox + b] = x.R.c7;
+    rec[(int64_t)56 * nbox + b] = x.R.c8;
+    rec[(int64_t)57 * nbox + b] = x.R.c9;
+    rec[(int64_t)58 * nbox + b] = x.R.c10;
+    rec[(int64_t)59 * nbox + b] = x.R.c14;
+    rec[(int64_t)60 * nbox + b] = x.R.c15;
+    rec[(int64_t)61 * nbox + b] = x.R.c17a;
+    rec[(int64_t)62 * nbox + b] = x.R.c17b;
+    rec[(int64_t)63 * nbox + b] = x.R.c18;
+    rec[(int64_t)64 * nbox + b] = x.R.c19;
+    rec[(int64_t)65 * nbox + b] = x.R.d2;
+    rec[(int64_t)66 * nbox + b] = x.R.d3;
+    rec[(int64_t)67 * nbox + b] = x.R.d4;
+    rec[(int64_t)68 * nbox + b] = x.R.d5;
+    rec[(int64_t)69 * nbox + b] = x.R.d6;
+    rec[(int64_t)70 * nbox + b] = x.R.d7;
+    rec[(int64_t)71 * nbox + b] = x.R.d8;
+    rec[(int64_t)72 * nbox + b] = x.R.d9;
+    rec[(int64_t)73 * nbox + b] = x.R.d10;
+    rec[(int64_t)74 * nbox + b] = x.R.d11;
+    rec[(int64_t)75 * nbox + b] = x.R.d31;
+    rec[(int64_t)76 * nbox + b] = x.R.d32;
+    rec[(int64_t)77 * nbox + b] = x.R.d33;
+    rec[(int64_t)78 * nbox + b] = x.R.d34;
+    rec[(int64_t)79 * nbox + b] = x.R.d36;
+    rec[(int64_t)80 * nbox + b] = x.R.d37;
+    rec[(int64_t)81 * nbox + b] = x.R.d60;
+    rec[(int64_t)82 * nbox + b] = x.R.d61;
+    rec[(int64_t)83 * nbox + b] = x.R.d62;
+    rec[(int64_t)84 * nbox + b] = x.R.d94;
+    rec[(int64_t)85 * nbox + b] = x.R.d95;
+    rec[(int64_t)86 * nbox + b] = x.R.d97;
+    rec[(int64_t)87 * nbox + b] = x.R.e2;
+    rec[(int64_t)88 * nbox + b] = x.R.e3;
+    rec[(int64_t)89 * nbox + b] = x.R.e4;
+    rec[(int64_t)90 * nbox + b] = x.R.e5a;
+    rec[(int64_t)91 * nbox + b] = x.R.e5b;
+    rec[(int64_t)92 * nbox + b] = x.R.e5c;
+    rec[(int64_t)93 * nbox + b] = x.R.e6;
+    rec[(int64_t)94 * nbox + b] = x.R.e7;
+    rec[(int64_t)95 * nbox + b] = x.R.e8;
+    rec[(int64_t)96 * nbox + b] = x.R.e10;
+    rec[(int64_t)97 * nbox + b] = x.R.e11;
+    rec[(int64_t)98 * nbox + b] = x.R.e13;
+    rec[(int64_t)99 * nbox + b] = x.R.e15;
+    rec[(int64_t)100 * nbox + b] = x.R.e16;
+    rec[(int64_t)101 * nbox + b] = x.R.e51;
+    rec[(int64_t)102 * nbox + b] = x.R.e54;
+    rec[(int64_t)103 * nbox + b] = x.R.hk1;
+    rec[(int64_t)104 * nbox + b] = x.R.hk2;
+    rec[(int64_t)105 * nbox + b] = x.R.hk3;
+    rec[(int64_t)106 * nbox + b] = x.R.hk4;
+    rec[(int64_t)107 * nbox + b] = x.R.hk5;
+    rec[(int64_t)108 * nbox + b] = x.tj[0];
+    rec[(int64_t)109 * nbox + b] = x.tj[1];
+    rec[(int64_t)110 * nbox + b] = x.tj[2];
+    rec[(int64_t)111 * nbox + b] = x.tj[3];
+    rec[(int64_t)112 * nbox + b] = x.tj[4];
+    rec[(int64_t)113 * nbox + b] = x.tj[5];
+    rec[(int64_t)114 * nbox + b] = x.tj[6];
+    rec[(int64_t)115 * nbox + b] = x.tj[7];
+    rec[(int64_t)116 * nbox + b] = x.tj[8];
+    rec[(int64_t)117 * nbox + b] = x.tj[9];
+    rec[(int64_t)118 * nbox + b] = x.tj[10];
+    rec[(int64_t)119 * nbox + b] = x.tj[11];
+    rec[(int64_t)120 * nbox + b] = x.tj[12];
+    rec[(int64_t)121 * nbox + b] = x.tj[13];
+    rec[(int64_t)122 * nbox + b] = x.tj[14];
+    rec[(int64_t)123 * nbox + b] = x.tj[15];
+    rec[(int64_t)124 * nbox + b] = x.tj[16];
+    rec[(int64_t)125 * nbox + b] = x.tj[17];
+    rec[(int64_t)126 * nbox + b] = x.tj[18];
+    rec[(int64_t)127 * nbox + b] = x.tj[19];
+    rec[(int64_t)128 * nbox + b] = x.tj[20];
+    rec[(int64_t)129 * nbox + b] = x.tj[21];
+    rec[(int64_t)130 * nbox + b] = x.tj[22];
+    rec[(int64_t)131 * nbox + b] = x.tj[23];
+    rec[(int64_t)132 * nbox + b] = x.tj[24];
+    rec[(int64_t)133 * nbox + b] = x.tj[25];
+    rec[(int64_t)134 * nbox + b] = x.tj[26];
+    rec[(int64_t)135 * nbox + b] = x.tj[27];
+    rec[(int64_t)136 * nbox + b] = x.tj[28];
+    rec[(int64_t)137 * nbox + b] = x.tj[29];
+    rec[(int64_t)138 * nbox + b] = x.tj[30];
+    rec[(int64_t)139 * nbox + b] = x.tj[31];
+    rec[(int64_t)140 * nbox + b] = x.tj[32];
+    rec[(int64_t)141 * nbox + b] = x.tj[33];
+    rec[(int64_t)142 * nbox + b] = x.tj[34];
+    rec[(int64_t)143 * nbox + b] = x.tj[35];
+    rec[(int64_t)144 * nbox + b] = x.tj[36];
+    rec[(int64_t)145 * nbox + b] = x.tj[37];
+    rec[(int64_t)146 * nbox + b] = x.tj[38];
+    rec[(int64_t)147 * nbox + b] = x.tj[39];
+    rec[(int64_t)148 * nbox + b] = x.tj[40];
+    rec[(int64_t)149 * nbox + b] = x.h.k[0];
+    rec[(int64_t)150 * nbox + b] = x.h.k[1];
+    rec[(int64_t)151 * nbox + b] = x.h.k[2];
+    rec[(int64_t)152 * nbox + b] = x.h.k[3];
+    rec[(int64_t)153 * nbox + b] = x.h.k[4];
+    rec[(int64_t)154 * nbox + b] = x.h.k[5];
+    rec[(int64_t)155 * nbox + b] = x.h.k[6];
+    rec[(int64_t)156 * nbox + b] = x.h.k[7];
+    rec[(int64_t)157 * nbox + b] = x.h.k[8];
+    rec[(int64_t)158 * nbox + b] = x.h.pno3;
+    rec[(int64_t)159 * nbox + b] = x.h.pno3sl;
+    rec[(int64_t)160 * nbox + b] = x.h.ph2o;
+    rec[(int64_t)161 * nbox + b] = x.h.phcl;
+    rec[(int64_t)162 * nbox + b] = x.h.surf;
+    rec[(int64_t)163 * nbox + b] = x.h.vsed;
+    rec[(int64_t)164 * nbox + b] = x.e.day ? 1. : 0.;
+    rec[(int64_t)165 * nbox + b] = x.het_level ? 1. : 0.;
+}
+
+__device__ __forceinline__ void rec_load(Box &x, const double *__restrict__ rec, int64_t nbox,
+                                         int64_t b)
+{
+    x.e.hnm = rec[(int64_t)0 * nbox + b];
+    x.e.n2 = rec[(int64_t)1 * nbox + b];
+    x.e.o2 = rec[(int64_t)2 * nbox + b];
+    x.e.co2 = rec[(int64_t)3 * nbox + b];
+    x.e.h2 = rec[(int64_t)4 * nbox + b];
+    x.e.dt = rec[(int64_t)5 * nbox + b];
+    x.e.het1 = rec[(int64_t)6 * nbox + b];
+    x.e.het2 = rec[(int64_t)7 * nbox + b];
+    x.e.het3 = rec[(int64_t)8 * nbox + b];
+    x.e.het4 = rec[(int64_t)9 * nbox + b];
+    x.e.het5 = rec[(int64_t)10 * nbox + b];
+    x.e.het6 = rec[(int64_t)11 * nbox + b];
+    x.e.het7 = rec[(int64_t)12 * nbox + b];
+    x.e.het8 = rec[(int64_t)13 * nbox + b];
+    x.e.het9 = rec[(int64_t)14 * nbox + b];
+    x.e.pno3 = rec[(int64_t)15 * nbox + b];
+    x.e.ph2o = rec[(int64_t)16 * nbox + b];
+    x.e.phcl = rec[(int64_t)17 * nbox + b];
+    x.e.pbr = rec[(int64_t)18 * nbox + b];
+    x.e.rain_hno3 = rec[(int64_t)19 * nbox + b];
+    x.e.rain_hno4 = rec[(int64_t)20 * nbox + b];
+    x.e.rain_hcl = rec[(int64_t)21 * nbox + b];
+    x.e.rain_hbr = rec[(int64_t)22 * nbox + b];
+    x.e.rain_h2o2 = rec[(int64_t)23 * nbox + b];
+    x.e.rain_ch2o = rec[(int64_t)24 * nbox + b];
+    x.e.rain_ch3o2h = rec[(int64_t)25 * nbox + b];
+    x.R.a1 = rec[(int64_t)26 * nbox + b];
+    x.R.a1et = rec[(int64_t)27 * nbox + b];
+    x.R.a2 = rec[(int64_t)28 * nbox + b];
+    x.R.a5 = rec[(int64_t)29 * nbox + b];
+    x.R.a6 = rec[(int64_t)30 * nbox + b];
+    x.R.a6b = rec[(int64_t)31 * nbox + b];
+    x.R.a7 = rec[(int64_t)32 * nbox + b];
+    x.R.a17 = rec[(int64_t)33 * nbox + b];
+    x.R.a19 = rec[(int64_t)34 * nbox + b];
+    x.R.a26 = rec[(int64_t)35 * nbox + b];
+    x.R.a27 = rec[(int64_t)36 * nbox + b];
+    x.R.a36 = rec[(int64_t)37 * nbox + b];
+    x.R.b3 = rec[(int64_t)38 * nbox + b];
+    x.R.b4 = rec[(int64_t)39 * nbox + b];
+    x.R.b6 = rec[(int64_t)40 * nbox + b];
+    x.R.b7 = rec[(int64_t)41 * nbox + b];
+    x.R.b9 = rec[(int64_t)42 * nbox + b];
+    x.R.b12 = rec[(int64_t)43 * nbox + b];
+    x.R.b22 = rec[(int64_t)44 * nbox + b];
+    x.R.b23 = rec[(int64_t)45 * nbox + b];
+    x.R.b24 = rec[(int64_t)46 * nbox + b];
+    x.R.b27 = rec[(int64_t)47 * nbox + b];
+    x.R.b28 = rec[(int64_t)48 * nbox + b];
+    x.R.b32 = rec[(int64_t)49 * nbox + b];
+    x.R.b38 = rec[(int64_t)50 * nbox + b];
+    x.R.b39 = rec[(int64_t)51 * nbox + b];
+    x.R.c2 = rec[(int64_t)52 * nbox + b];
+    x.R.c4 = rec[(int64_t)53 * nbox + b];
+    x.R.c5 = rec[(int64_t)54 * nbox + b];
+    x.R.c7 = rec[(int64_t)55 * nbox + b];
+    x.R.c8 = rec[(int64_t)56 * nbox + b];
+    x.R.c9 = rec[(int64_t)57 * nbox + b];
+    x.R.c10 = rec[(int64_t)58 * nbox + b];
+    x.R.c14 = rec[(int64_t)59 * nbox + b];
+    x.R.c15 = rec[(int64_t)60 * nbox + b];
+    x.R.c17a = rec[(int64_t)61 * nbox + b];
+    x.R.c17b = rec[(int64_t)62 * nbox + b];
+    x.R.c18 = rec[(int64_t)63 * nbox + b];
+    x.R.c19 = rec[(int64_t)64 * nbox + b];
+    x.R.d2 = rec[(int64_t)65 * nbox + b];
+    x.R.d3 = rec[(int64_t)66 * nbox + b];
+    x.R.d4 = rec[(int64_t)67 * nbox + b];
+    x.R.d5 = rec[(int64_t)68 * nbox + b];
+    x.R.d6 = rec[(int64_t)69 * nbox + b];
+    x.R.d7 = rec[(int64_t)70 * nbox + b];
+    x.R.d8 = rec[(int64_t)71 * nbox + b];
+    x.R.d9 = rec[(int64_t)72 * nbox + b];
+    x.R.d10 = rec[(int64_t)73 * nbox + b];
+    x.R.d11 = rec[(int64_t)74 * nbox + b];
+    x.R.d31 = rec[(int64_t)75 * nbox + b];
+    x.R.d32 = rec[(int64_t)76 * nbox + b];
+    x.R.d33 = rec[(int64_t)77 * nbox + b];
+    x.R.d34 = rec[(int64_t)78 * nbox + b];
+    x.R.d36 = rec[(int64_t)79 * nbox + b];
+    x.R.d37 = rec[(int64_t)80 * nbox + b];
+    x.R.d60 = rec[(int64_t)81 * nbox + b];
+    x.R.d61 = rec[(int64_t)82 * nbox + b];
+    x.R.d62 = rec[(int64_t)83 * nbox + b];
+    x.R.d94 = rec[(int64_t)84 * nbox + b];
+    x.R.d95 = rec[(int64_t)85 * nbox + b];
+    x.R.d97 = rec[(int64_t)86 * nbox + b];
+    x.R.e2 = rec[(int64_t)87 * nbox + b];
+    x.R.e3 = rec[(int64_t)88 * nbox + b];
+    x.R.e4 = rec[(int64_t)89 * nbox + b];
+    x.R.e5a = rec[(int64_t)90 * nbox + b];
+    x.R.e5b = rec[(int64_t)91 * nbox + b];
+    x.R.e5c = rec[(int64_t)92 * nbox + b];
+    x.R.e6 = rec[(int64_t)93 * nbox + b];
+    x.R.e7 = rec[(int64_t)94 * nbox + b];
+    x.R.e8 = rec[(int64_t)95 * nbox + b];
+    x.R.e10 = rec[(int64_t)96 * nbox + b];
+    x.R.e11 = rec[(int64_t)97 * nbox + b];
+    x.R.e13 = rec[(int64_t)98 * nbox + b];
+    x.R.e15 = rec[(int64_t)99 * nbox + b];
+    x.R.e16 = rec[(int64_t)100 * nbox + b];
+    x.R.e51 = rec[(int64_t)101 * nbox + b];
+    x.R.e54 = rec[(int64_t)102 * nbox + b];
+    x.R.hk1 = rec[(int64_t)103 * nbox + b];
+    x.R.hk2 = rec[(int64_t)104 * nbox + b];
+    x.R.hk3 = rec[(int64_t)105 * nbox + b];
+    x.R.hk4 = rec[(int64_t)106 * nbox + b];
+    x.R.hk5 = rec[(int64_t)107 * nbox + b];
+    x.tj[0] = rec[(int64_t)108 * nbox + b];
+    x.tj[1] = rec[(int64_t)109 * nbox + b];
+    x.tj[2] = rec[(int64_t)110 * nbox + b];
+    x.tj[3] = rec[(int64_t)111 * nbox + b];
+    x.tj[4] = rec[(int64_t)112 * nbox + b];
+    x.tj[5] = rec[(int64_t)113 * nbox + b];
+    x.tj[6] = rec[(int64_t)114 * nbox + b];
+    x.tj[7] = rec[(int64_t)115 * nbox + b];
+    x.tj[8] = rec[(int64_t)116 * nbox + b];
+    x.tj[9] = rec[(int64_t)117 * nbox + b];
+    x.tj[10] = rec[(int64_t)118 * nbox + b];
+    x.tj[11] = rec[(int64_t)119 * nbox + b];
+    x.tj[12] = rec[(int64_t)120 * nbox + b];
+    x.tj[13] = rec[(int64_t)121 * nbox + b];
+    x.tj[14] = rec[(int64_t)122 * nbox + b];
+    x.tj[15] = rec[(int64_t)123 * nbox + b];
+    x.tj[16] = rec[(int64_t)124 * nbox + b];
+    x.tj[17] = rec[(int64_t)125 * nbox + b];
+    x.tj[18] = rec[(int64_t)126 * nbox + b];
+    x.tj[19] = rec[(int64_t)127 * nbox + b];
+    x.tj[20] = rec[(int64_t)128 * nbox + b];
+    x.tj[21] = rec[(int64_t)129 * nbox + b];
+    x.tj[22] = rec[(int64_t)130 * nbox + b];
+    x.tj[23] = rec[(int64_t)131 * nbox + b];
+    x.tj[24] = rec[(int64_t)132 * nbox + b];
+    x.tj[25] = rec[(int64_t)133 * nbox + b];
+    x.tj[26] = rec[(int64_t)134 * nbox + b];
+    x.tj[27] = rec[(int64_t)135 * nbox + b];
+    x.tj[28] = rec[(int64_t)136 * nbox + b];
+    x.tj[29] = rec[(int64_t)137 * nbox + b];
+    x.tj[30] = rec[(int64_t)138 * nbox + b];
+    x.tj[31] = rec[(int64_t)139 * nbox + b];
+    x.tj[32] = rec[(int64_t)140 * nbox + b];
+    x.tj[33] = rec[(int64_t)141 * nbox + b];
+    x.tj[34] = rec[(int64_t)142 * nbox + b];
+    x.tj[35] = rec[(int64_t)143 * nbox + b];
+    x.tj[36] = rec[(int64_t)144 * nbox + b];
+    x.tj[37] = rec[(int64_t)145 * nbox + b];
+    x.tj[38] = rec[(int64_t)146 * nbox + b];
+    x.tj[39] = rec[(int64_t)147 * nbox + b];
+    x.tj[40] = rec[(int64_t)148 * nbox + b];
+    x.h.k[0] = rec[(int64_t)149 * nbox + b];
+    x.h.k[1] = rec[(int64_t)150 * nbox + b];
+    x.h.k[2] = rec[(int64_t)151 * nbox + b];
+    x.h.k[3] = rec[(int64_t)152 * nbox + b];
+    x.h.k[4] = rec[(int64_t)153 * nbox + b];
+    x.h.k[5] = rec[(int64_t)154 * nbox + b];
+    x.h.k[6] = rec[(int64_t)155 * nbox + b];
+    x.h.k[7] = rec[(int64_t)156 * nbox + b];
+    x.h.k[8] = rec[(int64_t)157 * nbox + b];
+    x.h.pno3 = rec[(int64_t)158 * nbox + b];
+    x.h.pno3sl = rec[(int64_t)159 * nbox + b];
+    x.h.ph2o = rec[(int64_t)160 * nbox + b];
+    x.h.phcl = rec[(int64_t)161 * nbox + b];
+    x.h.surf = rec[(int64_t)162 * nbox + b];
+    x.h.vsed = rec[(int64_t)163 * nbox + b];
+    x.e.day = rec[(int64_t)164 * nbox + b] != 0.;
+    x.het_level = rec[(int64_t)165 * nbox + b] != 0.;
+}
+
+// Converged mode, pass 1: the set-up of every box (box_setup, F:800-1338) into the record.
+template <int BLK>
+__global__ void __launch_bounds__(BLK) k_conv_setup(StepArgs a, double *__restrict__ rec)
+{
+    __shared__ double s_ck0[(NLONG - 1) * BLK];      // box_setup's start-of-step copy (unused)
+    const int64_t b = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    if (b >= a.nbox) return;
+    Box x;
+    box_setup<BLK>(a, b, x, s_ck0 + threadIdx.x, false);
+    rec_store(x, rec, a.nbox, b);
+}
+
+// The state of box b from its set-up record: the iterate from the incoming state (or the
+// guess), the start-of-step state to the LDS column ck0 -- the values box_setup gives.
+template <int BLK>
+__device__ __forceinline__ void box_load(const StepArgs &a, const double *__restrict__ rec,
+                                         int64_t b, Box &x, double *ck0, bool use_guess)
+{
+    rec_load(x, rec, a.nbox, b);
+    const double hnm = x.e.hnm;
+#pragma unroll
+    for (int s = 0; s < NLONG; ++s) {
+        x.ck[s] = a.qj1[(int64_t)s * a.field + b] * hnm;
+        if (s < NLONG - 1) ck0[s * BLK] = x.ck[s];
+    }
+#pragma unroll
+    for (int s = 0; s < NSHORT; ++s) x.cc[s] = a.hc[(int64_t)s * a.field + b] * hnm;
+    x.ch3o = 0.;
+    x.hco = 0.;
+    if (use_guess && a.guess_q) {
+#pragma unroll
+        for (int s = 0; s < NLONG - 1; ++s) x.ck[s] = a.guess_q[(int64_t)s * a.field + b] * hnm;
+#pragma unroll
+        for (int s = 0; s < NSHORT; ++s) x.cc[s] = a.guess_h[(int64_t)s * a.field + b] * hnm;
+    }
+}
+
 // Fixed-sweep step (the reference scheme, F:1348 niter = 5): one lane per box.  Kept
 // as one body: factoring it into the shared set-up and write-back adds ~1 KB of scratch
 // per lane and costs about half again in time.
@@ -1125,7 +1519,7 @@ __global__ void __launch_bounds__(BLK) k_chem(StepArgs a)
     thermal_rates(a.K, t, e.hnm, pm, R);
 
     double ch3o = 0., hco = 0.;
-    Track notrack{true, true, 0., 0.};
+    Track notrack{true, true, 0., 0., nullptr, 0};
     for (int it = 0; it < a.niter; ++it) {
         // keeps the LDS reads of ck0 inside the loop (not hoisted into registers)
         asm volatile("" ::: "memory");
@@ -1176,25 +1570,156 @@ __global__ void __launch_bounds__(BLK) k_chem(StepArgs a)
 // set up again and given the fixed-sweep answer (a.niter sweeps from the incoming state),
 // its count stored negated.  Every lane leaves once the queue is exhausted, so every wave
 // reaches the exit.
-template <int BLK>
+// Anderson acceleration (Walker & Ni 2011, type II) of depth M on x = (ck, cc), the same
+// arithmetic in the same order as the restatement's aa_update.  Per lane, a ring of M + 1
+// (f, g) pairs lives in device memory: F(slot, i) = ws[((2 slot) * NX + i) * stride],
+// G(slot, i) = ws[((2 slot + 1) * NX + i) * stride] (stride = lanes of the grid, so a
+// wave's accesses coalesce).  On entry x holds f = F(x_k) and G(head) the residual
+// g_k = f - x_k (stored by the sweep); on exit x holds x_{k+1} = f - dF gamma, gamma
+// minimising || w (g_k - dG gamma) ||, w_i = 1 / max(|f_i|, fl)^2; components that would
+// turn non-positive keep f_i.
+constexpr int NX = NLONG - 1 + NSHORT;
+
+template <int M>
+__device__ void aa_update(double *ck, double *cc, double *ws, int64_t stride, int &head, int &n,
+                          double fl)
+{
+    auto F = [&](int slot, int i) -> double & { return ws[((int64_t)(2 * slot) * NX + i) * stride]; };
+    auto G = [&](int slot, int i) -> double & { return ws[((int64_t)(2 * slot + 1) * NX + i) * stride]; };
+    auto X = [&](int i) -> double & { return i < NLONG - 1 ? ck[i] : cc[i - (NLONG - 1)]; };
+    const int slot = head;
+#pragma unroll
+    for (int i = 0; i < NX; ++i) F(slot, i) = X(i);
+    head = (head + 1) % (M + 1);
+    if (n < M + 1) ++n;
+    const int mk = n - 1;
+    if (mk < 1) return;
+    int ia[M], ib[M];
+#pragma unroll
+    for (int j = 0; j < M; ++j) {
+        ia[j] = (slot - j + (M + 1)) % (M + 1);
+        ib[j] = (slot - j - 1 + 2 * (M + 1)) % (M + 1);
+    }
+    double Mx[M][M + 1];
+#pragma unroll
+    for (int r = 0; r < M; ++r)
+#pragma unroll
+        for (int c = 0; c <= M; ++c) Mx[r][c] = 0.;
+#pragma unroll
+    for (int i = 0; i < NX; ++i) {
+        const double sc = fmax(fabs(X(i)), fl);
+        const double w = 1. / (sc * sc);
+        const double gk = G(slot, i);
+        double dg[M];
+#pragma unroll
+        for (int j = 0; j < M; ++j) dg[j] = (j < mk) ? G(ia[j], i) - G(ib[j], i) : 0.;
+#pragma unroll
+        for (int r = 0; r < M; ++r) {
+#pragma unroll
+            for (int c = 0; c < M; ++c) Mx[r][c] += dg[r] * w * dg[c];
+            Mx[r][M] += dg[r] * w * gk;
+        }
+    }
+    // restrict to the mk x mk system; the right-hand side moves to column mk
+    double A[M][M + 1];
+#pragma unroll
+    for (int r = 0; r < M; ++r) {
+#pragma unroll
+        for (int c = 0; c < M; ++c) A[r][c] = Mx[r][c];
+        A[r][M] = Mx[r][M];
+    }
+#pragma unroll
+    for (int r = 0; r < M; ++r)
+        if (r < mk) A[r][r] *= (1. + 1.e-10);
+    bool ok = true;
+#pragma unroll
+    for (int c = 0; c < M; ++c) {
+        if (c >= mk || !ok) continue;
+        int piv = c;
+#pragma unroll
+        for (int r = c + 1; r < M; ++r)
+            if (r < mk && fabs(A[r][c]) > fabs(A[piv][c])) piv = r;
+        if (!(fabs(A[piv][c]) > 0.)) { ok = false; continue; }
+        if (piv != c) {
+#pragma unroll
+            for (int r = c + 1; r < M; ++r)
+                if (r == piv) {
+#pragma unroll
+                    for (int k = 0; k <= M; ++k) {
+                        const double t_ = A[c][k]; A[c][k] = A[r][k]; A[r][k] = t_;
+                    }
+                }
+        }
+#pragma unroll
+        for (int r = c + 1; r < M; ++r) {
+            if (r >= mk) continue;
+            const double f_ = A[r][c] / A[c][c];
+#pragma unroll
+            for (int k = 0; k <= M; ++k) {
+                if (k < c) continue;
+                const int kk = (k == M) ? M : k;
+                if (k < mk || k == M) A[r][kk] -= f_ * A[c][kk];
+            }
+        }
+    }
+    if (!ok) return;
+    double gam[M];
+#pragma unroll
+    for (int r = M - 1; r >= 0; --r) {
+        gam[r] = 0.;
+        if (r >= mk) continue;
+        double acc = A[r][M];
+#pragma unroll
+        for (int k = r + 1; k < M; ++k)
+            if (k < mk) acc -= A[r][k] * gam[k];
+        gam[r] = acc / A[r][r];
+        if (!(fabs(gam[r]) <= 1.e6)) ok = false;
+    }
+    if (!ok) return;
+#pragma unroll
+    for (int i = 0; i < NX; ++i) {
+        double v = X(i);
+#pragma unroll
+        for (int j = 0; j < M; ++j)
+            if (j < mk) v -= gam[j] * (F(ia[j], i) - F(ib[j], i));
+        if (v > 0. && v <= 1.7976931348623157e308) X(i) = v;
+    }
+}
+
+// Converged mode: sweeps to the fixed point of the sweep map, optionally Anderson-
+// accelerated (depth AAM).  Persistent: each lane takes boxes from a device work queue
+// (one atomic per wave for all lanes that need work), sweeps its box until every species
+// changed by less than tol relative, writes it back and takes the next box, so a wave
+// never waits for its slowest box.  A box whose iteration goes non-finite or reaches
+// maxiter sweeps without converging is set up again and given the fixed-sweep answer
+// (a.niter sweeps from the incoming state), its count stored negated.  Every lane leaves
+// once the queue is exhausted, so every wave reaches the exit.
+template <int BLK, int AAM>
 __global__ void __launch_bounds__(BLK) k_chem_conv(StepArgs a, ConvArgs cv)
 {
     __shared__ double s_ck0[(NLONG - 1) * BLK];
     double *ck0 = s_ck0 + threadIdx.x;
     const int lane = threadIdx.x & (warpSize - 1);
+    const int64_t lanes = (int64_t)gridDim.x * BLK;
+    double *ws = AAM > 0 ? cv.aa_ws + (int64_t)blockIdx.x * BLK + threadIdx.x : nullptr;
     bool active = false, finished = false, fallback = false, restart = false;
     int64_t b = 0;
-    int it = 0, tried = 0;
+    int it = 0, tried = 0, head = 0, nstored = 0;
     Box x;
     for (;;) {
-        const unsigned long long need = __ballot(!active && !finished && !restart);
+        // queue chunking: idle lanes take boxes together, once at least cv.refill lanes of
+        // the wave are idle (or none is busy), so one set-up pass serves many lanes
+        const unsigned long long idle = __ballot(!active && !finished && !restart);
+        const unsigned long long busy = __ballot(active || restart);
+        const unsigned long long need =
+            (__popcll(idle) >= (unsigned)cv.refill || busy == 0ull) ? idle : 0ull;
         bool setup = restart;
         if (need) {
             const int leader = __ffsll((long long)need) - 1;
             unsigned long long base = 0;
             if (lane == leader) base = atomicAdd(cv.counter, (unsigned long long)__popcll(need));
             base = __shfl(base, leader);
-            if (!active && !finished && !restart) {
+            if ((need >> lane) & 1ull) {
                 const unsigned long long below = need & ((1ull << lane) - 1ull);
                 b = (int64_t)(base + (unsigned long long)__popcll(below));
                 if (b < a.nbox) {
@@ -1206,15 +1731,19 @@ __global__ void __launch_bounds__(BLK) k_chem_conv(StepArgs a, ConvArgs cv)
             }
         }
         if (setup) {
-            box_setup<BLK>(a, b, x, ck0, !fallback);
+            box_load<BLK>(a, cv.rec, b, x, ck0, !fallback);
             active = true;
             restart = false;
             it = 0;
+            head = 0;
+            nstored = 0;
         }
         if (__ballot(active) == 0ull) break;
         if (active) {
             asm volatile("" ::: "memory");
-            Track tr{true, true, cv.tol, 1.e-30 * x.e.hnm};
+            const bool acc = AAM > 0 && !fallback && it + 1 >= cv.aa_start;
+            Track tr{true, true, cv.tol, 1.e-30 * x.e.hnm,
+                     acc ? ws + (int64_t)(2 * head + 1) * NX * lanes : nullptr, lanes};
             sweep<BLK, true>(x.R, x.tj, x.e, ck0, x.ck, x.cc, x.ch3o, x.hco, tr);
             ++it;
             if (fallback) {
@@ -1233,6 +1762,8 @@ __global__ void __launch_bounds__(BLK) k_chem_conv(StepArgs a, ConvArgs cv)
                 fallback = true;
                 restart = true;
                 active = false;
+            } else if (AAM > 0 && it >= cv.aa_start) {
+                aa_update<(AAM > 0 ? AAM : 1)>(x.ck, x.cc, ws, lanes, head, nstored, 1.e-30 * x.e.hnm);
             }
         }
     }
@@ -1374,26 +1905,54 @@ hipError_t launch_chem(const StepArgs &a, int block, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launch_chem_conv(const StepArgs &a, const ConvArgs &cv, int block, int nblocks,
-                            hipStream_t s)
+hipError_t launch_conv_setup(const StepArgs &a, double *rec, int block, hipStream_t s)
 {
     const int blk = block > 0 ? block : 256;
+    const unsigned grid = (unsigned)((a.nbox + blk - 1) / blk);
     switch (blk) {
-    case 64: hipLaunchKernelGGL(k_chem_conv<64>, dim3(nblocks), dim3(64), 0, s, a, cv); break;
-    case 128: hipLaunchKernelGGL(k_chem_conv<128>, dim3(nblocks), dim3(128), 0, s, a, cv); break;
-    case 256: hipLaunchKernelGGL(k_chem_conv<256>, dim3(nblocks), dim3(256), 0, s, a, cv); break;
+    case 64: hipLaunchKernelGGL(k_conv_setup<64>, dim3(grid), dim3(64), 0, s, a, rec); break;
+    case 128: hipLaunchKernelGGL(k_conv_setup<128>, dim3(grid), dim3(128), 0, s, a, rec); break;
+    case 256: hipLaunchKernelGGL(k_conv_setup<256>, dim3(grid), dim3(256), 0, s, a, rec); break;
     default: return hipErrorInvalidConfiguration;
     }
     return hipGetLastError();
 }
 
-int conv_blocks_per_cu(int block)
+template <int AAM>
+static hipError_t launch_conv_depth(const StepArgs &a, const ConvArgs &cv, int blk, int nblocks,
+                                    hipStream_t s)
+{
+    switch (blk) {
+    case 64: hipLaunchKernelGGL((k_chem_conv<64, AAM>), dim3(nblocks), dim3(64), 0, s, a, cv); break;
+    case 128: hipLaunchKernelGGL((k_chem_conv<128, AAM>), dim3(nblocks), dim3(128), 0, s, a, cv); break;
+    case 256: hipLaunchKernelGGL((k_chem_conv<256, AAM>), dim3(nblocks), dim3(256), 0, s, a, cv); break;
+    default: return hipErrorInvalidConfiguration;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_chem_conv(const StepArgs &a, const ConvArgs &cv, int block, int nblocks,
+                            hipStream_t s)
+{
+    const int blk = block > 0 ? block : 256;
+    if (cv.accel == 0) return launch_conv_depth<0>(a, cv, blk, nblocks, s);
+    if (cv.accel == 2) return launch_conv_depth<2>(a, cv, blk, nblocks, s);
+    return hipErrorInvalidValue;
+}
+
+int conv_blocks_per_cu(int block, int accel)
 {
     int n = 0;
     hipError_t e = hipErrorInvalidValue;
-    if (block == 64) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_chem_conv<64>, 64, 0);
-    if (block == 128) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_chem_conv<128>, 128, 0);
-    if (block == 256) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_chem_conv<256>, 256, 0);
+    if (accel == 0) {
+        if (block == 64) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_chem_conv<64, 0>, 64, 0);
+        if (block == 128) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_chem_conv<128, 0>, 128, 0);
+        if (block == 256) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_chem_conv<256, 0>, 256, 0);
+    } else {
+        if (block == 64) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_chem_conv<64, 2>, 64, 0);
+        if (block == 128) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_chem_conv<128, 2>, 128, 0);
+        if (block == 256) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_chem_conv<256, 2>, 256, 0);
+    }
     return (e == hipSuccess && n > 0) ? n : 1;
 }
 

@@ -17,7 +17,7 @@ struct rb_ctx {
     int device = 0;
     hipStream_t own = nullptr;
     hipStream_t stream = nullptr;
-    rb_config cfg{5, 0, 256, 0};
+    rb_config cfg{5, 0, 256, 0, 16, 60};
     rb::Troe K{};
     double *d_ajl = nullptr, *d_o3up = nullptr;
     bool table = false;
@@ -29,6 +29,10 @@ struct rb_ctx {
     unsigned long long *d_queue = nullptr;   // converged-mode work-queue head
     double *d_work = nullptr;                // altitude: level pressure, Tv, geopotential
     size_t work_cap = 0;
+    double *d_aa = nullptr;                  // converged mode: Anderson history per lane
+    size_t aa_cap = 0;
+    double *d_rec = nullptr;                 // converged mode: per-box set-up records
+    size_t rec_cap = 0;
     int n_cu = 0;
     bool capturing = false;                  // inside a graph capture: no event records
     hipGraph_t day_graph = nullptr;          // the captured day and what it was built for
@@ -116,6 +120,9 @@ int rb_init(rb_ctx **out, int device, const rb_config *cfg)
     if (cfg) ctx->cfg = *cfg;
     if (ctx->cfg.niter < 1) ctx->cfg.niter = 5;
     if (ctx->cfg.block <= 0) ctx->cfg.block = 256;
+    if (ctx->cfg.conv_refill <= 0) ctx->cfg.conv_refill = 16;
+    if (ctx->cfg.conv_refill > 64) ctx->cfg.conv_refill = 64;
+    if (ctx->cfg.conv_aa_start < 0) ctx->cfg.conv_aa_start = 60;
     if (ctx->cfg.block != 64 && ctx->cfg.block != 128 && ctx->cfg.block != 256) {
         delete ctx;
         return RB_EINVAL;   // the per-box kernel is instantiated for 64, 128 and 256 threads
@@ -226,10 +233,11 @@ int rb_chem_step_dev(rb_ctx *ctx, const rb_grid *g, double dt, const double *T, 
 }
 
 int rb_chem_converge_dev(rb_ctx *ctx, const rb_grid *g, double dt, double tol, int32_t maxiter,
-                         const double *T, const double *p, const double *sza, const double *alt,
-                         double *qj1, double *hc, double *surfarea, double *vsed3d,
-                         double *parth2osed, double *parthno3sed, const double *guess_qj1,
-                         const double *guess_hc, int32_t *iters, int fuse_conserv)
+                         int32_t accel, const double *T, const double *p, const double *sza,
+                         const double *alt, double *qj1, double *hc, double *surfarea,
+                         double *vsed3d, double *parth2osed, double *parthno3sed,
+                         const double *guess_qj1, const double *guess_hc, int32_t *iters,
+                         int fuse_conserv)
 {
     if (!ctx) return RB_EINVAL;
     int rc = check_grid(ctx, g);
@@ -241,6 +249,8 @@ int rb_chem_converge_dev(rb_ctx *ctx, const rb_grid *g, double dt, double tol, i
         return fail(ctx, RB_EINVAL, "guess_qj1 and guess_hc must be both set or both NULL");
     if (!(dt > 0.) || !std::isfinite(dt)) return fail(ctx, RB_EINVAL, "dt must be positive");
     if (!(tol > 0.) || maxiter < 1) return fail(ctx, RB_EINVAL, "tol must be > 0, maxiter >= 1");
+    if (accel != 0 && accel != rb::AA_DEPTH)
+        return fail(ctx, RB_EINVAL, "accel must be 0 (plain sweeps) or 2 (Anderson depth 2)");
     HIPCHK(hipSetDevice(ctx->device));
     const int64_t plane = (int64_t)g->nlon * g->nlat;
     const int64_t field = plane * g->niv;
@@ -259,12 +269,20 @@ int rb_chem_converge_dev(rb_ctx *ctx, const rb_grid *g, double dt, double tol, i
     a.K = ctx->K;
     a.guess_q = guess_qj1;
     a.guess_h = guess_hc;
-    rb::ConvArgs cv{tol, maxiter, ctx->d_queue, iters};
     // one resident wave set: blocks per CU from the occupancy calculator
-    const int nblocks = ctx->n_cu * rb::conv_blocks_per_cu(ctx->cfg.block);
+    const int nblocks = ctx->n_cu * rb::conv_blocks_per_cu(ctx->cfg.block, accel);
+    if (accel) {
+        const size_t lanes = (size_t)nblocks * (ctx->cfg.block > 0 ? ctx->cfg.block : 256);
+        if ((rc = ensure(ctx, &ctx->d_aa, &ctx->aa_cap, 2 * (accel + 1) * rb::AA_NX * lanes)))
+            return rc;
+    }
+    if ((rc = ensure(ctx, &ctx->d_rec, &ctx->rec_cap, (size_t)rb::CONV_RECORD * nbox))) return rc;
+    rb::ConvArgs cv{tol, maxiter, ctx->d_queue, iters, accel, accel ? ctx->d_aa : nullptr,
+                    ctx->cfg.conv_refill, ctx->d_rec, ctx->cfg.conv_aa_start};
 
     if (!ctx->capturing) HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
     HIPCHK(rb::launch_columns(a, ctx->d_col, ctx->stream));
+    HIPCHK(rb::launch_conv_setup(a, ctx->d_rec, ctx->cfg.block, ctx->stream));
     HIPCHK(rb::launch_zero_counter(ctx->d_queue, ctx->stream));
     if (ctx->cfg.profile && !ctx->capturing) {
         if (ctx->prof_used == ctx->prof.size()) {
@@ -527,8 +545,8 @@ static int day_step(rb_ctx *ctx, const rb_grid *g, const rb_day_config *dc, int 
     int rc = rb_zenith_dev(ctx, g, dc->day0 + k * dc->dt / 86400., trajlon, trajlat, sza);
     if (rc) return rc;
     if (dc->converged)
-        return rb_chem_converge_dev(ctx, g, dc->dt, dc->tol, dc->maxiter, T, p, sza, alt, qj1, hc,
-                                    surfarea, vsed3d, p2o, pno3, nullptr, nullptr, iters, 1);
+        return rb_chem_converge_dev(ctx, g, dc->dt, dc->tol, dc->maxiter, dc->accel, T, p, sza, alt,
+                                    qj1, hc, surfarea, vsed3d, p2o, pno3, nullptr, nullptr, iters, 1);
     return rb_chem_step_dev(ctx, g, dc->dt, T, p, sza, alt, qj1, hc, surfarea, vsed3d, p2o, pno3, 1);
 }
 
@@ -562,6 +580,14 @@ int rb_run_day_dev(rb_ctx *ctx, const rb_grid *g, const rb_day_config *dc, const
     // scratch the steps use must exist before capture (no allocation inside it)
     const size_t nbox = (size_t)g->nlon * g->nlat * g->nivbas;
     if ((rc = ensure(ctx, &ctx->d_col, &ctx->col_cap, 3 * nbox))) return rc;
+    if (dc->converged && (rc = ensure(ctx, &ctx->d_rec, &ctx->rec_cap, (size_t)rb::CONV_RECORD * nbox)))
+        return rc;
+    if (dc->converged && dc->accel) {
+        const int blk = ctx->cfg.block > 0 ? ctx->cfg.block : 256;
+        const size_t lanes = (size_t)ctx->n_cu * rb::conv_blocks_per_cu(blk, dc->accel) * blk;
+        if ((rc = ensure(ctx, &ctx->d_aa, &ctx->aa_cap, 2 * (dc->accel + 1) * rb::AA_NX * lanes)))
+            return rc;
+    }
     // the graph is rebuilt when anything it was captured with changes
     std::vector<char> key;
     auto add = [&](const void *v, size_t n) {
@@ -569,7 +595,8 @@ int rb_run_day_dev(rb_ctx *ctx, const rb_grid *g, const rb_day_config *dc, const
         key.insert(key.end(), c, c + n);
     };
     const void *ptrs[] = {trajlon, trajlat, T, p, alt, sza, qj1, hc, surfarea, vsed3d,
-                          parth2osed, parthno3sed, iters, ctx->stream, ctx->d_col};
+                          parth2osed, parthno3sed, iters, ctx->stream, ctx->d_col, ctx->d_aa,
+                          ctx->d_rec};
     add(g, sizeof *g);
     add(dc, sizeof *dc);
     add(ptrs, sizeof ptrs);
@@ -640,6 +667,8 @@ int rb_finalize(rb_ctx *ctx)
     if (ctx->d_count) (void)hipFree(ctx->d_count);
     if (ctx->d_queue) (void)hipFree(ctx->d_queue);
     if (ctx->d_work) (void)hipFree(ctx->d_work);
+    if (ctx->d_aa) (void)hipFree(ctx->d_aa);
+    if (ctx->d_rec) (void)hipFree(ctx->d_rec);
     if (ctx->day_exec) (void)hipGraphExecDestroy(ctx->day_exec);
     if (ctx->day_graph) (void)hipGraphDestroy(ctx->day_graph);
     for (hipEvent_t e : {ctx->ev0, ctx->ev1, ctx->ev_h2d, ctx->ev_d2h})

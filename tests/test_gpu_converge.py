@@ -18,11 +18,14 @@ TOL = 1e-10
 MAXITER = 5000
 
 
+AA_START = 40          # the library's default: plain sweeps before, Anderson mixing after
+
+
 @pytest.fixture(scope="module")
 def step(table):
     from reprobus_amd.chem import ChemistryStep
 
-    s = ChemistryStep(*table, device=0)
+    s = ChemistryStep(*table, device=0, conv_aa_start=AA_START)
     yield s
     s.close()
 
@@ -42,7 +45,7 @@ def _dev(inp):
     return d
 
 
-def converge_gpu(step, inp, tol=TOL, maxiter=MAXITER, guess=None, fuse=True):
+def converge_gpu(step, inp, tol=TOL, maxiter=MAXITER, guess=None, fuse=True, accel=0):
     import torch
     from reprobus_amd.chem import make_grid
 
@@ -56,7 +59,8 @@ def converge_gpu(step, inp, tol=TOL, maxiter=MAXITER, guess=None, fuse=True):
     step.set_stream(torch.cuda.current_stream().cuda_stream)
     step.converge_device(grid, inp.dt, tol, maxiter, d["T"], d["p"], d["sza"], d["alt"], d["qj1"],
                          d["hc"], d["surfarea"], d["vsed3d"], d["parth2osed"], d["parthno3sed"],
-                         guess_qj1=gq, guess_hc=gh, iters=d["iters"], fuse_conserv=fuse)
+                         guess_qj1=gq, guess_hc=gh, iters=d["iters"], fuse_conserv=fuse,
+                         accel=accel)
     torch.cuda.synchronize()
     step.set_stream(None)
     return {k: v.cpu().numpy() for k, v in d.items()}
@@ -90,6 +94,33 @@ def test_converged_matches_restatement(step, table, cr_psc):
                        rel_err(out["hc"], ref["hc"]).max(axis=0))
         over = d > PARITY_TOL
         assert over.sum() <= 1e-3 * d.size and np.all(inp.T[over] < 215.0)
+
+
+@pytest.mark.parametrize("seed", [1, 0])
+def test_accelerated_matches_restatement(step, table, seed):
+    """Anderson depth 2 on the GPU against the restatement's depth-2 iteration (the same
+    arithmetic): the same sweep counts on almost every box and agreement at the 1e-9 bar;
+    and both reach the plain iteration's fixed point."""
+    import cport
+    from reprobus_amd import synth
+
+    inp = synth.make_c2(seed, nlon=24, nlat=13)
+    out = converge_gpu(step, inp, maxiter=2000, accel=2)
+    ref = cport.chem_converge(inp, *table, tol=TOL, maxiter=2000, cr_psc=True, accel=2,
+                              aa_start=AA_START)
+    it, rit = out["iters"], ref["iters"]
+    same = float((it == rit).mean())
+    w = _worst(out, ref)
+    plain = cport.chem_converge(inp, *table, tol=1e-13, maxiter=20000, cr_psc=True)
+    live = (plain["iters"] > 0) & (it > 0)
+    d = np.maximum(rel_err(out["qj1"], plain["qj1"]).max(axis=0),
+                   rel_err(out["hc"], plain["hc"]).max(axis=0))
+    print(f"[converge] AA(2) from sweep {AA_START}, seed {seed}: sweeps mean {np.abs(it).mean():.1f} max {np.abs(it).max()} "
+          f"(plain {np.abs(plain['iters']).mean():.1f}), same count {same:.4%}, vs restatement "
+          f"{w:.2e}, vs tight plain fixed point p99 {np.percentile(d[live], 99):.2e}", flush=True)
+    assert same > 0.99
+    assert w < PARITY_TOL
+    assert np.percentile(d[live], 99) < PARITY_TOL
 
 
 def test_converged_c2_with_fallback(step, table):

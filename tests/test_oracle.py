@@ -463,3 +463,21 @@ def test_altitude_isothermal_dry_closed_form():
     x = np.radians(90.0)
     expect = (1 + 0.002644 * np.cos(2 * x)) * geo + (1 + 0.0089 * np.cos(2 * x)) * geo ** 2 / 6245.0
     assert np.max(np.abs(alt[1:] - expect[1:])) < 1e-12
+
+
+def test_anderson_acceleration_reaches_the_same_fixed_point(table):
+    """Depth-2 Anderson mixing of the sweep map: the same fixed point as the plain sweeps
+    (both converged tightly), in fewer sweeps, with no box left unconverged."""
+    import cport
+    from reprobus_amd import synth
+
+    inp = synth.make_c2(3, nlon=8, nlat=7)
+    plain = cport.chem_converge(inp, *table, tol=1e-13, maxiter=20000, conserv=False)
+    aa = cport.chem_converge(inp, *table, tol=1e-13, maxiter=20000, conserv=False, accel=2)
+    live = (plain["iters"] > 0) & (aa["iters"] > 0)
+    assert live.mean() > 0.99
+    for k in ("qj1_chem", "hc_chem"):
+        d = rel_err(aa[k][:42] if k == "qj1_chem" else aa[k],
+                    plain[k][:42] if k == "qj1_chem" else plain[k])
+        assert np.percentile(d[:, live].max(axis=0), 99) < 1e-9, k
+    assert np.abs(aa["iters"]).mean() < 0.6 * np.abs(plain["iters"]).mean()
