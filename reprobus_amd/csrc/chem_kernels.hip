@@ -317,13 +317,37 @@ __device__ double ternary_density(double ws, double wn, double t)
     return 1.0 / vm * 0.001;
 }
 
+// Elementary functions of the PSC block up to the liquid composition.  MathCR: the
+// correctly rounded ones of crmath.h, for boxes that may take the ternary branch
+// (T < 215 K, F:2994), whose solution is ill-conditioned near its binary limit and
+// amplifies last-bit differences (see crmath.h).  MathLib: the device library, for the
+// warmer boxes, where NAT, ice and the binary solution are well conditioned.
+struct MathCR {
+    static __device__ __forceinline__ double pow10(double x) { return crm::pow10_cr(x); }
+    static __device__ __forceinline__ double log10(double x) { return crm::log10_fdlibm(x); }
+    static __device__ __forceinline__ double log(double x) { return crm::log_cr(x); }
+    static __device__ __forceinline__ double exp(double x) { return crm::exp_cr(x); }
+    static __device__ __forceinline__ double pow(double x, double y) { return crm::pow_cr(x, y); }
+    static __device__ __forceinline__ double atan(double x) { return crm::atan_cr(x); }
+    static __device__ __forceinline__ double cos(double x) { return crm::cos_cr(x); }
+};
+struct MathLib {
+    static __device__ __forceinline__ double pow10(double x) { return ::pow(10., x); }
+    static __device__ __forceinline__ double log10(double x) { return ::log10(x); }
+    static __device__ __forceinline__ double log(double x) { return ::log(x); }
+    static __device__ __forceinline__ double exp(double x) { return ::exp(x); }
+    static __device__ __forceinline__ double pow(double x, double y) { return ::pow(x, y); }
+    static __device__ __forceinline__ double atan(double x) { return ::atan(x); }
+    static __device__ __forceinline__ double cos(double x) { return ::cos(x); }
+};
+
 // analytic (NAT F:2795-2857, ice F:2861-2913, liquid F:2921-3228, area F:3240, HNO3
-// partition F:3252, settling F:3264-3348) then hetero (F:3436-4023) for one box.
-// Up to the liquid composition (ws, wn, pn) the elementary functions are the correctly
-// rounded ones of crmath.h: the ternary solution is ill-conditioned near its binary
-// limit and amplifies last-bit differences there (see crmath.h).  Area, solubilities,
-// settling and the uptake rates downstream are well conditioned and use the library.
-__device__ void psc_and_het(double temp, double ptot, double hnm, const double *ck, Het &o)
+// partition F:3252, settling F:3264-3348) then hetero (F:3436-4023) for one box, with
+// the elementary functions of M up to the liquid composition (ws, wn, pn).  Area,
+// solubilities, settling and the uptake rates downstream are well conditioned and use
+// the library.
+template <class M>
+__device__ void psc_block(double temp, double ptot, double hnm, const double *ck, Het &o)
 {
     constexpr double R = 8.205e-5, NDROP = 10., NNAT_SMALL = 1., RNATS = 0.5e-4,
                      RNATL = 6.5e-4, RICE = 10.e-4, AMU = 1.66e-24;
@@ -346,7 +370,7 @@ __device__ void psc_and_het(double temp, double ptot, double hnm, const double *
     {
         const double zmt = -2.7836 - 0.00088 * tm;
         const double zbt = 38.9855 - 11397.0 / tm + 0.009179 * tm;
-        const double eq = crm::pow10_cr(zmt * crm::log10_fdlibm(pw * 1013. * 0.75) + zbt);
+        const double eq = M::pow10(zmt * M::log10(pw * 1013. * 0.75) + zbt);
         if (pn0 * 1013. * 0.75 / eq >= 1.) {
             lnat = 1;
             pn = eq / (0.75 * 1013.);
@@ -368,12 +392,12 @@ __device__ void psc_and_het(double temp, double ptot, double hnm, const double *
         }
     }
     // ice
-    const double tice = 2668.70 / (10.4310 - (crm::log_cr(pw) + LOG760) / LOG10);
+    const double tice = 2668.70 / (10.4310 - (M::log(pw) + LOG760) / LOG10);
     if (tm <= tice) {
         lice = 1;
         lnat = 0;
         anat = 0.;
-        const double ph2o = crm::pow10_cr(-2668.7 / tm + 10.4310) / 760.0;
+        const double ph2o = M::pow10(-2668.7 / tm + 10.4310) / 760.0;
         const double h2os = fmax(pw - ph2o, 0.) * 1013. * hnm / ptot;
         const double vice = h2os * AMU * 18. / 0.928;
         nice = vice / (4. / 3. * pi * RICE3);
@@ -381,7 +405,7 @@ __device__ void psc_and_het(double temp, double ptot, double hnm, const double *
         o.ph2o = fmin(fmax(1.0 - fmax(pw - ph2o, 0.) / pw, 0.), 1.);
         const double zmt = -2.7836 - 0.00088 * tm;
         const double zbt = 38.9855 - 11397.0 / tm + 0.009179 * tm;
-        const double eq = crm::pow10_cr(zmt * crm::log10_fdlibm(pw * o.ph2o * 1013. * 0.75) + zbt);
+        const double eq = M::pow10(zmt * M::log10(pw * o.ph2o * 1013. * 0.75) + zbt);
         pn = eq / (0.75 * 1013.);
         pno3s = fmin(fmax(1.0 - (pn0 - pn) / pn0, 0.), 1.);
         o.pno3sl = pno3s;
@@ -389,7 +413,7 @@ __device__ void psc_and_het(double temp, double ptot, double hnm, const double *
     // liquid binary H2SO4/H2O then ternary (Carslaw et al. 1995)
     const double t = fmax(temp, fmax(tice - 3.0, 185.));
     pw = fmax(fmin(ck[H2O] * o.ph2o / hnm * ptot / 1013.0, 2.e-3 / 1013.), 2.e-5 / 1013.);
-    const double lt = crm::log_cr(t), lpw = crm::log_cr(pw);   // shared by binary and ternary
+    const double lt = M::log(t), lpw = M::log(pw);   // shared by binary and ternary
     const double qh2so4 = fmax(ck[H2SO4] / hnm, 1.e-18);
     const double ns = ptot * 100.0 * qh2so4 / 8.314 / t;
     pn0 = ptot / 1013.0 * (ck[HNO3] * pno3s / hnm);
@@ -414,8 +438,8 @@ __device__ void psc_and_het(double temp, double ptot, double hnm, const double *
         // H2SO4 saturation pressure (Ayers et al. 1980, Giauque et al. 1960)
         const double wts = fmax(41., ws * 100.);
         const double mu = 4.184 * (1.514e4 - 286. * (wts - 40.) + 1.08 * sq(wts - 40.) -
-                                   3941. / crm::pow_cr(wts - 40., 0.1));
-        const double ph2so4 = crm::exp_cr(-10156. / t + 16.2590 - mu / (8.314 * t));
+                                   3941. / M::pow(wts - 40., 0.1));
+        const double ph2so4 = M::exp(-10156. / t + 16.2590 - mu / (8.314 * t));
         if (qh2so4 * ptot / 1013. <= ph2so4) {
             ms = 0.;
             ws = 0.;
@@ -430,7 +454,7 @@ __device__ void psc_and_het(double temp, double ptot, double hnm, const double *
         constexpr double kn[7] = {-39.136, 6358.4, 83.29, -17650.0, 198.53, -11948, -28.469};
         const double tt = R * t * ns, tr = 1.0e4 / t - 43.4782608, pr = lpw + 18.4;
         const double tr2 = tr * tr, tr3 = tr * tr * tr, pr2 = pr * pr, pr3 = pr * pr * pr;
-        const double hsb = crm::exp_cr(qs[0] + qs[1] * tr2 +
+        const double hsb = M::exp(qs[0] + qs[1] * tr2 +
                                (qs[2] + qs[3] * tr + qs[4] * tr2 + qs[5] * tr3) * pr +
                                (qs[6] + qs[7] * tr + qs[8] * tr2) * pr2 + qs[9] * tr * pr3);
         const double kk = kn[0] + kn[1] / t;
@@ -439,7 +463,7 @@ __device__ void psc_and_het(double temp, double ptot, double hnm, const double *
                             sqrt_pow(kk * kk - 4.0 * (kn[2] + kn[3] / t) *
                                                    (kn[4] + kn[5] / t + kn[6] * lt - lpw)));
         const double mnb = 55.51 * xnb / (1.0 - xnb);
-        const double hnb = crm::exp_cr(qn[0] + qn[1] * tr2 +
+        const double hnb = M::exp(qn[0] + qn[1] * tr2 +
                                (qn[2] + qn[3] * tr + qn[4] * tr2 + qn[5] * tr3) * pr +
                                (qn[6] + qn[7] * tr + qn[8] * tr2) * pr2 + qn[9] * tr * pr3);
         const double A = (tt * hnb * (mnb * mnb) - tt * hsb * mnb * msb - 2.0 * (mnb * mnb) * msb +
@@ -451,9 +475,9 @@ __device__ void psc_and_het(double temp, double ptot, double hnm, const double *
         const double C = (tt * hnb * mnb * (msb * msb)) / (mnb - msb);
         const double u = (A * A) - 3.0 * B;
         const double v = -2.0 * (A * A * A) + 9.0 * A * B - 27.0 * C;
-        double phi = crm::atan_cr(sqrt(fmax(4.0 * (u * u * u) - (v * v), 0.)) / v);
+        double phi = M::atan(sqrt(fmax(4.0 * (u * u * u) - (v * v), 0.)) / v);
         if (phi < 0.) phi = phi + pi;
-        const double msf = fmax(-1.0 / 3.0 * (A + 2.0 * sqrt((A * A) - 3.0 * B) * crm::cos_cr((pi + phi) / 3.0)), 0.);
+        const double msf = fmax(-1.0 / 3.0 * (A + 2.0 * sqrt((A * A) - 3.0 * B) * M::cos((pi + phi) / 3.0)), 0.);
         const double mnf = fmax(mnb * (1.0 - msf / msb), 0.);
         pn = mnf / (hnb * mnf / (mnf + msf) + hsb * msf / (mnf + msf));
         ms = msf;
@@ -624,6 +648,16 @@ __device__ void psc_and_het(double temp, double ptot, double hnm, const double *
 #pragma unroll
     for (int i = 0; i < 8; ++i) o.k[i] = kl[i] + nsol * ks[i];
     o.k[8] = 0.;
+}
+
+// The PSC block with correctly rounded math where the ternary branch can be taken.
+__device__ __forceinline__ void psc_and_het(double temp, double ptot, double hnm, const double *ck,
+                                            Het &o)
+{
+    if (temp < 215.)
+        psc_block<MathCR>(temp, ptot, hnm, ck, o);
+    else
+        psc_block<MathLib>(temp, ptot, hnm, ck, o);
 }
 
 // ------------------------------------------------------------------ the sweeps
