@@ -47,7 +47,9 @@ struct StepArgs {
 struct ConvArgs {
     double tol;                        // fixed-point tolerance (relative change per sweep)
     int32_t maxiter;                   // sweep cap per box
-    unsigned long long *counter;       // work-queue head, zero at launch
+    unsigned long long *counter;       // 3 counters, zero at launch: work-queue head of the
+                                       // sweep phase, spill count, queue head of the
+                                       // Anderson phase
     int32_t *iters;                    // per-box sweep count (nbox) or null; -n: the box
                                        // did not converge in n sweeps and holds the
                                        // fixed-sweep (niter) answer instead
@@ -56,6 +58,9 @@ struct ConvArgs {
     int32_t refill;                    // idle lanes of a wave before they take new boxes
     const double *rec;                 // per-box set-up records (launch_conv_setup)
     int32_t aa_start;                  // accel > 0: plain sweeps before this sweep number
+    double *spill;                     // accel > 0: boxes handed from the sweep phase to
+                                       // the Anderson phase, CONV_SPILL doubles per box
+                                       // (iterate, box index, sweep count), field-major
 };
 // doubles per box of the converged mode's set-up record
 constexpr int CONV_RECORD = 166;
@@ -63,14 +68,20 @@ hipError_t launch_conv_setup(const StepArgs &a, double *rec, int block, hipStrea
 // Anderson depths the converged kernel is built for
 constexpr int AA_DEPTH = 2;
 constexpr int AA_NX = 57;
+// doubles per box of the spill list between the two phases of the accelerated mode
+constexpr int CONV_SPILL = AA_NX + 2;
 
 Troe troe_constants();
 hipError_t launch_columns(const StepArgs &a, double *col, hipStream_t s);
 hipError_t launch_chem(const StepArgs &a, int block, hipStream_t s);
-hipError_t launch_chem_conv(const StepArgs &a, const ConvArgs &cv, int block, int nblocks,
-                            hipStream_t s);
-int conv_blocks_per_cu(int block, int accel);
-hipError_t launch_zero_counter(unsigned long long *p, hipStream_t s);
+// plain: one launch of nblocks_sweep blocks; accelerated: the sweep phase on nblocks_sweep
+// blocks, then the Anderson phase on nblocks_aa (the Anderson workspace holds its lanes)
+hipError_t launch_chem_conv(const StepArgs &a, const ConvArgs &cv, int block, int nblocks_sweep,
+                            int nblocks_aa, hipStream_t s);
+// resident blocks per CU of the converged kernel (the Anderson phase's with accel and
+// anderson_phase)
+int conv_blocks_per_cu(int block, int accel, bool anderson_phase = false);
+hipError_t launch_zero_counter(unsigned long long *p, hipStream_t s, int n = 1);
 // conserv on boxes below plane*nivbas and the floor on every box, for boxes [start, field)
 // day-dependent terms of zenith (F:5334-5380), computed once per call on the host
 struct ZenithTerms {

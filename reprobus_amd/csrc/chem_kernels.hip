@@ -674,22 +674,18 @@ struct BoxEnv {
 // The converged mode's fixed-point test, without divisions: a sweep has converged when
 // every long- and short-lived species changed by less than tol relative to its new value
 // (values under the floor fl compared at fl): |v - old| < tol max(|v|, fl); `fin` stays true
-// while every new value is finite.  A NaN fails both.
-// With g set (Anderson acceleration), the change v - old of species index i (long-lived
-// 0..41, then short-lived 42..56) is also stored at g[i * gstride]: the residual of the
-// sweep map at the previous iterate.
+// while every new value is finite.  A NaN fails both.  Each species is set once per sweep,
+// so `old` is its value at the start of the sweep.
 struct Track {
     bool conv, fin;
     double tol, fl;
-    double *g;
-    int64_t gstride;
 };
 
-__device__ __forceinline__ void track(Track &t, double v, double old, int i)
+__device__ __forceinline__ void track(Track &t, double v, double old)
 {
-    t.conv = t.conv && (fabs(v - old) < t.tol * fmax(fabs(v), t.fl));
-    t.fin = t.fin && (fabs(v) <= 1.7976931348623157e308);
-    if (t.g) t.g[i * t.gstride] = v - old;
+    // non-short-circuit: a lane mask AND per test, no branch
+    t.conv &= fabs(v - old) < t.tol * fmax(fabs(v), t.fl);
+    t.fin &= fabs(v) <= 1.7976931348623157e308;
 }
 
 // ck0 is the start-of-step state in LDS, one column per lane: ck0[s * CK0_STRIDE].
@@ -702,7 +698,7 @@ __device__ void sweep(const Rates &R, const double *tj, const BoxEnv &e,
 #define CCSET(sl, expr)                                                              \
     do {                                                                             \
         const double v_ = (expr);                                                    \
-        if (TRACK) track(tr, v_, cc[sl], NLONG - 1 + (sl));                          \
+        if (TRACK) track(tr, v_, cc[sl]);                                            \
         cc[sl] = v_;                                                                 \
     } while (0)
     const double hnm = e.hnm, n2 = e.n2, o2 = e.o2, co2 = e.co2, h2 = e.h2;
@@ -947,7 +943,7 @@ __device__ void sweep(const Rates &R, const double *tj, const BoxEnv &e,
 #undef UPDATE
 #pragma unroll
     for (int s = 0; s < NLONG - 1; ++s) {
-        if (TRACK) track(tr, nk[s], ck[s], s);
+        if (TRACK) track(tr, nk[s], ck[s]);
         ck[s] = nk[s];
     }
 #undef CCSET
@@ -1553,7 +1549,7 @@ __global__ void __launch_bounds__(BLK) k_chem(StepArgs a)
     thermal_rates(a.K, t, e.hnm, pm, R);
 
     double ch3o = 0., hco = 0.;
-    Track notrack{true, true, 0., 0., nullptr, 0};
+    Track notrack{true, true, 0., 0.};
     for (int it = 0; it < a.niter; ++it) {
         // keeps the LDS reads of ck0 inside the loop (not hoisted into registers)
         asm volatile("" ::: "memory");
@@ -1596,23 +1592,28 @@ __global__ void __launch_bounds__(BLK) k_chem(StepArgs a)
     a.pno3[b] = h.pno3sl;
 }
 
-// Converged mode: sweeps to the fixed point of the sweep map.  Persistent: each lane
-// takes boxes from a device work queue (one atomic per wave for all lanes that need
-// work), sweeps its box until the largest relative change of any species falls under
-// tol, writes it back and takes the next box, so a wave never waits for its slowest box.
-// A box whose iteration goes non-finite or reaches maxiter sweeps without converging is
-// set up again and given the fixed-sweep answer (a.niter sweeps from the incoming state),
-// its count stored negated.  Every lane leaves once the queue is exhausted, so every wave
-// reaches the exit.
 // Anderson acceleration (Walker & Ni 2011, type II) of depth M on x = (ck, cc), the same
 // arithmetic in the same order as the restatement's aa_update.  Per lane, a ring of M + 1
 // (f, g) pairs lives in device memory: F(slot, i) = ws[((2 slot) * NX + i) * stride],
 // G(slot, i) = ws[((2 slot + 1) * NX + i) * stride] (stride = lanes of the grid, so a
-// wave's accesses coalesce).  On entry x holds f = F(x_k) and G(head) the residual
-// g_k = f - x_k (stored by the sweep); on exit x holds x_{k+1} = f - dF gamma, gamma
+// wave's accesses coalesce).  On entry x holds f = F(x_k) and G(head) the iterate x_k
+// (aa_save before the sweep), which becomes the residual g_k = f - x_k; on exit x holds
+// x_{k+1} = f - dF gamma, gamma
 // minimising || w (g_k - dG gamma) ||, w_i = 1 / max(|f_i|, fl)^2; components that would
 // turn non-positive keep f_i.
 constexpr int NX = NLONG - 1 + NSHORT;
+
+// x_k = (ck, cc) into G(head) before an accelerated sweep (one block of coalesced stores
+// instead of a conditional store per species inside the sweep).
+__device__ __forceinline__ void aa_save(const double *ck, const double *cc, double *ws,
+                                        int64_t stride, int head)
+{
+    double *g = ws + (int64_t)(2 * head + 1) * NX * stride;
+#pragma unroll
+    for (int i = 0; i < NLONG - 1; ++i) g[i * stride] = ck[i];
+#pragma unroll
+    for (int i = 0; i < NSHORT; ++i) g[(NLONG - 1 + i) * stride] = cc[i];
+}
 
 template <int M>
 __device__ void aa_update(double *ck, double *cc, double *ws, int64_t stride, int &head, int &n,
@@ -1623,7 +1624,10 @@ __device__ void aa_update(double *ck, double *cc, double *ws, int64_t stride, in
     auto X = [&](int i) -> double & { return i < NLONG - 1 ? ck[i] : cc[i - (NLONG - 1)]; };
     const int slot = head;
 #pragma unroll
-    for (int i = 0; i < NX; ++i) F(slot, i) = X(i);
+    for (int i = 0; i < NX; ++i) {
+        F(slot, i) = X(i);
+        G(slot, i) = X(i) - G(slot, i);
+    }
     head = (head + 1) % (M + 1);
     if (n < M + 1) ++n;
     const int mk = n - 1;
@@ -1720,44 +1724,75 @@ __device__ void aa_update(double *ck, double *cc, double *ws, int64_t stride, in
     }
 }
 
-// Converged mode: sweeps to the fixed point of the sweep map, optionally Anderson-
-// accelerated (depth AAM).  Persistent: each lane takes boxes from a device work queue
-// (one atomic per wave for all lanes that need work), sweeps its box until every species
-// changed by less than tol relative, writes it back and takes the next box, so a wave
-// never waits for its slowest box.  A box whose iteration goes non-finite or reaches
-// maxiter sweeps without converging is set up again and given the fixed-sweep answer
-// (a.niter sweeps from the incoming state), its count stored negated.  Every lane leaves
-// once the queue is exhausted, so every wave reaches the exit.
-template <int BLK, int AAM>
+// Converged mode: sweeps to the fixed point of the sweep map.  Persistent: each lane
+// takes boxes from a device work queue (one atomic per wave for all lanes that need work)
+// and sweeps its box until every species changed by less than tol relative; a converged
+// lane holds its result until the wave is served again, so a wave never waits for its
+// slowest box.  A box whose iteration goes non-finite or reaches maxiter sweeps without
+// converging is set up again and given the fixed-sweep answer (a.niter sweeps from the
+// incoming state), its count stored negated.  Every lane leaves once its queue is
+// exhausted, so every wave reaches the exit.
+//
+// MODE CONV_PLAIN: plain sweeps.  With Anderson acceleration (from sweep cv.aa_start) the
+// solve is two launches, so that the wide history code never runs masked in a wave whose
+// other lanes do plain sweeps: MODE CONV_SWEEP sweeps like CONV_PLAIN but hands a box that
+// reaches sweep aa_start unconverged, with its iterate and count, to the spill list;
+// MODE CONV_ANDERSON takes its boxes from that list and continues them with accelerated
+// sweeps.  The sequence of operations per box is the one-launch sequence.
+//
+// Two loops: the inner one holds only the sweep, the fixed-point test and the Anderson
+// update, and runs until cv.refill lanes of the wave wait for work (or none is busy, or a
+// lane needs its fallback set-up); the outer one serves the wave -- write-back of the
+// finished boxes, one queue claim for every waiting lane, set-up from the records.  Keeping
+// the set-up and write-back out of the sweep loop keeps the box state in registers there.
+constexpr int CONV_PLAIN = 0, CONV_SWEEP = 1, CONV_ANDERSON = 2;
+
+template <int BLK, int MODE>
 __global__ void __launch_bounds__(BLK) k_chem_conv(StepArgs a, ConvArgs cv)
 {
+    constexpr int AAM = MODE == CONV_ANDERSON ? AA_DEPTH : 0;
     __shared__ double s_ck0[(NLONG - 1) * BLK];
     double *ck0 = s_ck0 + threadIdx.x;
     const int lane = threadIdx.x & (warpSize - 1);
     const int64_t lanes = (int64_t)gridDim.x * BLK;
     double *ws = AAM > 0 ? cv.aa_ws + (int64_t)blockIdx.x * BLK + threadIdx.x : nullptr;
-    bool active = false, finished = false, fallback = false, restart = false;
+    // the spill list: the iterate (NX fields), box index and sweep count per entry
+    const int64_t cap = a.nbox;
+    int64_t *sp_box = MODE != CONV_PLAIN ? (int64_t *)(cv.spill + NX * cap) : nullptr;
+    int32_t *sp_it = MODE != CONV_PLAIN ? (int32_t *)(cv.spill + (NX + 1) * cap) : nullptr;
+    unsigned long long *queue = cv.counter + (MODE == CONV_ANDERSON ? 2 : 0);
+    // the Anderson phase starts after the sweep phase has ended: the list is complete
+    const int64_t nwork = MODE == CONV_ANDERSON
+                              ? (int64_t)__atomic_load_n(cv.counter + 1, __ATOMIC_RELAXED)
+                              : a.nbox;
+    // lane states: active (sweeping), done (holding a result not yet written), restart
+    // (to be set up again for the fallback), finished (queue exhausted); none = waiting
+    bool active = false, done = false, finished = false, fallback = false, restart = false;
     int64_t b = 0;
-    int it = 0, tried = 0, head = 0, nstored = 0;
+    int it = 0, tried = 0, head = 0, nstored = 0, result = 0;
     Box x;
     for (;;) {
-        // queue chunking: idle lanes take boxes together, once at least cv.refill lanes of
-        // the wave are idle (or none is busy), so one set-up pass serves many lanes
-        const unsigned long long idle = __ballot(!active && !finished && !restart);
-        const unsigned long long busy = __ballot(active || restart);
-        const unsigned long long need =
-            (__popcll(idle) >= (unsigned)cv.refill || busy == 0ull) ? idle : 0ull;
-        bool setup = restart;
-        if (need) {
-            const int leader = __ffsll((long long)need) - 1;
+        // serve the wave
+        if (done) {
+            box_writeback(a, b, x);
+            if (cv.iters) cv.iters[b] = result;
+            done = false;
+        }
+        const unsigned long long waiting = __ballot(!active && !finished && !restart);
+        bool setup = restart, resume = false;
+        int64_t w = 0;
+        if (waiting) {
+            const int leader = __ffsll((long long)waiting) - 1;
             unsigned long long base = 0;
-            if (lane == leader) base = atomicAdd(cv.counter, (unsigned long long)__popcll(need));
+            if (lane == leader) base = atomicAdd(queue, (unsigned long long)__popcll(waiting));
             base = __shfl(base, leader);
-            if ((need >> lane) & 1ull) {
-                const unsigned long long below = need & ((1ull << lane) - 1ull);
-                b = (int64_t)(base + (unsigned long long)__popcll(below));
-                if (b < a.nbox) {
+            if ((waiting >> lane) & 1ull) {
+                const unsigned long long below = waiting & ((1ull << lane) - 1ull);
+                w = (int64_t)(base + (unsigned long long)__popcll(below));
+                if (w < nwork) {
+                    b = MODE == CONV_ANDERSON ? sp_box[w] : w;
                     setup = true;
+                    resume = MODE == CONV_ANDERSON;
                     fallback = false;
                 } else {
                     finished = true;
@@ -1765,40 +1800,76 @@ __global__ void __launch_bounds__(BLK) k_chem_conv(StepArgs a, ConvArgs cv)
             }
         }
         if (setup) {
-            box_load<BLK>(a, cv.rec, b, x, ck0, !fallback);
+            box_load<BLK>(a, cv.rec, b, x, ck0, !fallback && !resume);
             active = true;
             restart = false;
             it = 0;
             head = 0;
             nstored = 0;
+            if (resume) {
+                // the iterate and sweep count the sweep phase handed over
+#pragma unroll
+                for (int i = 0; i < NLONG - 1; ++i) x.ck[i] = cv.spill[i * cap + w];
+#pragma unroll
+                for (int i = 0; i < NSHORT; ++i) x.cc[i] = cv.spill[(NLONG - 1 + i) * cap + w];
+                it = sp_it[w];
+            }
         }
         if (__ballot(active) == 0ull) break;
-        if (active) {
-            asm volatile("" ::: "memory");
-            const bool acc = AAM > 0 && !fallback && it + 1 >= cv.aa_start;
-            Track tr{true, true, cv.tol, 1.e-30 * x.e.hnm,
-                     acc ? ws + (int64_t)(2 * head + 1) * NX * lanes : nullptr, lanes};
-            sweep<BLK, true>(x.R, x.tj, x.e, ck0, x.ck, x.cc, x.ch3o, x.hco, tr);
-            ++it;
-            if (fallback) {
-                if (it >= a.niter) {
-                    box_writeback(a, b, x);
-                    if (cv.iters) cv.iters[b] = -tried;
-                    active = false;
+        // sweep until the wave needs serving
+        for (;;) {
+            if (MODE == CONV_SWEEP) {
+                // a box about to take its first accelerated sweep goes to the spill list
+                const bool give = active && !fallback && it + 1 >= cv.aa_start;
+                const unsigned long long gv = __ballot(give);
+                if (gv) {
+                    const int leader = __ffsll((long long)gv) - 1;
+                    unsigned long long base = 0;
+                    if (lane == leader) base = atomicAdd(cv.counter + 1, (unsigned long long)__popcll(gv));
+                    base = __shfl(base, leader);
+                    if (give) {
+                        const int64_t s = (int64_t)(base + (unsigned long long)__popcll(gv & ((1ull << lane) - 1ull)));
+#pragma unroll
+                        for (int i = 0; i < NLONG - 1; ++i) cv.spill[i * cap + s] = x.ck[i];
+#pragma unroll
+                        for (int i = 0; i < NSHORT; ++i) cv.spill[(NLONG - 1 + i) * cap + s] = x.cc[i];
+                        sp_box[s] = b;
+                        sp_it[s] = it;
+                        active = false;
+                    }
                 }
-            } else if (tr.conv && tr.fin) {
-                box_writeback(a, b, x);
-                if (cv.iters) cv.iters[b] = it;
-                active = false;
-            } else if (!tr.fin || it >= cv.maxiter) {
-                // diverged (a non-finite value) or out of sweeps
-                tried = it;
-                fallback = true;
-                restart = true;
-                active = false;
-            } else if (AAM > 0 && it >= cv.aa_start) {
-                aa_update<(AAM > 0 ? AAM : 1)>(x.ck, x.cc, ws, lanes, head, nstored, 1.e-30 * x.e.hnm);
             }
+            if (active) {
+                const bool acc = AAM > 0 && !fallback && it + 1 >= cv.aa_start;
+                if (AAM > 0 && acc) aa_save(x.ck, x.cc, ws, lanes, head);
+                Track tr{true, true, cv.tol, 1.e-30 * x.e.hnm};
+                sweep<BLK, true>(x.R, x.tj, x.e, ck0, x.ck, x.cc, x.ch3o, x.hco, tr);
+                ++it;
+                if (fallback) {
+                    if (it >= a.niter) {
+                        result = -tried;
+                        done = true;
+                        active = false;
+                    }
+                } else if (tr.conv && tr.fin) {
+                    result = it;
+                    done = true;
+                    active = false;
+                } else if (!tr.fin || it >= cv.maxiter) {
+                    // diverged (a non-finite value) or out of sweeps
+                    tried = it;
+                    fallback = true;
+                    restart = true;
+                    active = false;
+                } else if (AAM > 0 && acc) {
+                    aa_update<(AAM > 0 ? AAM : 1)>(x.ck, x.cc, ws, lanes, head, nstored,
+                                                   1.e-30 * x.e.hnm);
+                }
+            }
+            const unsigned long long waiting = __ballot(!active && !finished && !restart);
+            const unsigned long long busy = __ballot(active);
+            if (__popcll(waiting) >= (unsigned)cv.refill || busy == 0ull || __ballot(restart))
+                break;
         }
     }
 }
@@ -1952,49 +2023,55 @@ hipError_t launch_conv_setup(const StepArgs &a, double *rec, int block, hipStrea
     return hipGetLastError();
 }
 
-template <int AAM>
-static hipError_t launch_conv_depth(const StepArgs &a, const ConvArgs &cv, int blk, int nblocks,
-                                    hipStream_t s)
+template <int MODE>
+static hipError_t launch_conv_mode(const StepArgs &a, const ConvArgs &cv, int blk, int nblocks,
+                                   hipStream_t s)
 {
     switch (blk) {
-    case 64: hipLaunchKernelGGL((k_chem_conv<64, AAM>), dim3(nblocks), dim3(64), 0, s, a, cv); break;
-    case 128: hipLaunchKernelGGL((k_chem_conv<128, AAM>), dim3(nblocks), dim3(128), 0, s, a, cv); break;
-    case 256: hipLaunchKernelGGL((k_chem_conv<256, AAM>), dim3(nblocks), dim3(256), 0, s, a, cv); break;
+    case 64: hipLaunchKernelGGL((k_chem_conv<64, MODE>), dim3(nblocks), dim3(64), 0, s, a, cv); break;
+    case 128: hipLaunchKernelGGL((k_chem_conv<128, MODE>), dim3(nblocks), dim3(128), 0, s, a, cv); break;
+    case 256: hipLaunchKernelGGL((k_chem_conv<256, MODE>), dim3(nblocks), dim3(256), 0, s, a, cv); break;
     default: return hipErrorInvalidConfiguration;
     }
     return hipGetLastError();
 }
 
-hipError_t launch_chem_conv(const StepArgs &a, const ConvArgs &cv, int block, int nblocks,
-                            hipStream_t s)
+hipError_t launch_chem_conv(const StepArgs &a, const ConvArgs &cv, int block, int nblocks_sweep,
+                            int nblocks_aa, hipStream_t s)
 {
     const int blk = block > 0 ? block : 256;
-    if (cv.accel == 0) return launch_conv_depth<0>(a, cv, blk, nblocks, s);
-    if (cv.accel == 2) return launch_conv_depth<2>(a, cv, blk, nblocks, s);
-    return hipErrorInvalidValue;
+    if (cv.accel == 0) return launch_conv_mode<CONV_PLAIN>(a, cv, blk, nblocks_sweep, s);
+    if (cv.accel != AA_DEPTH || !cv.spill || !cv.aa_ws) return hipErrorInvalidValue;
+    const hipError_t e = launch_conv_mode<CONV_SWEEP>(a, cv, blk, nblocks_sweep, s);
+    if (e != hipSuccess) return e;
+    return launch_conv_mode<CONV_ANDERSON>(a, cv, blk, nblocks_aa, s);
 }
 
-int conv_blocks_per_cu(int block, int accel)
+template <int MODE>
+static int conv_occupancy(int block)
 {
     int n = 0;
     hipError_t e = hipErrorInvalidValue;
-    if (accel == 0) {
-        if (block == 64) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_chem_conv<64, 0>, 64, 0);
-        if (block == 128) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_chem_conv<128, 0>, 128, 0);
-        if (block == 256) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_chem_conv<256, 0>, 256, 0);
-    } else {
-        if (block == 64) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_chem_conv<64, 2>, 64, 0);
-        if (block == 128) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_chem_conv<128, 2>, 128, 0);
-        if (block == 256) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_chem_conv<256, 2>, 256, 0);
-    }
+    if (block == 64) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_chem_conv<64, MODE>, 64, 0);
+    if (block == 128) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_chem_conv<128, MODE>, 128, 0);
+    if (block == 256) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_chem_conv<256, MODE>, 256, 0);
     return (e == hipSuccess && n > 0) ? n : 1;
 }
 
-__global__ void k_zero_u64(unsigned long long *p) { *p = 0ull; }
-
-hipError_t launch_zero_counter(unsigned long long *p, hipStream_t s)
+int conv_blocks_per_cu(int block, int accel, bool anderson_phase)
 {
-    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, s, p);
+    if (accel == 0) return conv_occupancy<CONV_PLAIN>(block);
+    return anderson_phase ? conv_occupancy<CONV_ANDERSON>(block) : conv_occupancy<CONV_SWEEP>(block);
+}
+
+__global__ void k_zero_u64(unsigned long long *p, int n)
+{
+    if ((int)threadIdx.x < n) p[threadIdx.x] = 0ull;
+}
+
+hipError_t launch_zero_counter(unsigned long long *p, hipStream_t s, int n)
+{
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, p, n);
     return hipGetLastError();
 }
 

@@ -26,13 +26,15 @@ struct rb_ctx {
     double *d_stage = nullptr;        // staging for the host-array entry points
     size_t stage_cap = 0;
     unsigned long long *d_count = nullptr;
-    unsigned long long *d_queue = nullptr;   // converged-mode work-queue head
+    unsigned long long *d_queue = nullptr;   // converged mode: queue heads, spill count (3)
     double *d_work = nullptr;                // altitude: level pressure, Tv, geopotential
     size_t work_cap = 0;
     double *d_aa = nullptr;                  // converged mode: Anderson history per lane
     size_t aa_cap = 0;
     double *d_rec = nullptr;                 // converged mode: per-box set-up records
     size_t rec_cap = 0;
+    double *d_spill = nullptr;               // accelerated mode: boxes between the phases
+    size_t spill_cap = 0;
     int n_cu = 0;
     bool capturing = false;                  // inside a graph capture: no event records
     hipGraph_t day_graph = nullptr;          // the captured day and what it was built for
@@ -93,6 +95,19 @@ int ensure(rb_ctx *ctx, double **buf, size_t *cap, size_t n)
     return RB_OK;
 }
 
+// scratch of the converged mode for nbox boxes: set-up records; with Anderson acceleration
+// also the per-lane history of the Anderson phase's grid and the spill list
+int conv_buffers(rb_ctx *ctx, size_t nbox, int accel)
+{
+    int rc = ensure(ctx, &ctx->d_rec, &ctx->rec_cap, (size_t)rb::CONV_RECORD * nbox);
+    if (rc || !accel) return rc;
+    const int blk = ctx->cfg.block > 0 ? ctx->cfg.block : 256;
+    const size_t lanes = (size_t)ctx->n_cu * rb::conv_blocks_per_cu(blk, accel, true) * blk;
+    if ((rc = ensure(ctx, &ctx->d_aa, &ctx->aa_cap, 2 * (accel + 1) * rb::AA_NX * lanes)))
+        return rc;
+    return ensure(ctx, &ctx->d_spill, &ctx->spill_cap, (size_t)rb::CONV_SPILL * nbox);
+}
+
 bool any_null(std::initializer_list<const void *> ps)
 {
     for (const void *p : ps)
@@ -130,7 +145,7 @@ int rb_init(rb_ctx **out, int device, const rb_config *cfg)
     ctx->K = rb::troe_constants();
     if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&ctx->own) != hipSuccess ||
         hipMalloc((void **)&ctx->d_count, sizeof(unsigned long long)) != hipSuccess ||
-        hipMalloc((void **)&ctx->d_queue, sizeof(unsigned long long)) != hipSuccess ||
+        hipMalloc((void **)&ctx->d_queue, 3 * sizeof(unsigned long long)) != hipSuccess ||
         hipDeviceGetAttribute(&ctx->n_cu, hipDeviceAttributeMultiprocessorCount, device) !=
             hipSuccess ||
         hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess ||
@@ -269,21 +284,18 @@ int rb_chem_converge_dev(rb_ctx *ctx, const rb_grid *g, double dt, double tol, i
     a.K = ctx->K;
     a.guess_q = guess_qj1;
     a.guess_h = guess_hc;
-    // one resident wave set: blocks per CU from the occupancy calculator
+    // one resident wave set per launch: blocks per CU from the occupancy calculator
     const int nblocks = ctx->n_cu * rb::conv_blocks_per_cu(ctx->cfg.block, accel);
-    if (accel) {
-        const size_t lanes = (size_t)nblocks * (ctx->cfg.block > 0 ? ctx->cfg.block : 256);
-        if ((rc = ensure(ctx, &ctx->d_aa, &ctx->aa_cap, 2 * (accel + 1) * rb::AA_NX * lanes)))
-            return rc;
-    }
-    if ((rc = ensure(ctx, &ctx->d_rec, &ctx->rec_cap, (size_t)rb::CONV_RECORD * nbox))) return rc;
+    const int nblocks_aa = ctx->n_cu * rb::conv_blocks_per_cu(ctx->cfg.block, accel, true);
+    if ((rc = conv_buffers(ctx, (size_t)nbox, accel))) return rc;
     rb::ConvArgs cv{tol, maxiter, ctx->d_queue, iters, accel, accel ? ctx->d_aa : nullptr,
-                    ctx->cfg.conv_refill, ctx->d_rec, ctx->cfg.conv_aa_start};
+                    ctx->cfg.conv_refill, ctx->d_rec, ctx->cfg.conv_aa_start,
+                    accel ? ctx->d_spill : nullptr};
 
     if (!ctx->capturing) HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
     HIPCHK(rb::launch_columns(a, ctx->d_col, ctx->stream));
     HIPCHK(rb::launch_conv_setup(a, ctx->d_rec, ctx->cfg.block, ctx->stream));
-    HIPCHK(rb::launch_zero_counter(ctx->d_queue, ctx->stream));
+    HIPCHK(rb::launch_zero_counter(ctx->d_queue, ctx->stream, 3));
     if (ctx->cfg.profile && !ctx->capturing) {
         if (ctx->prof_used == ctx->prof.size()) {
             hipEvent_t e0, e1;
@@ -293,10 +305,10 @@ int rb_chem_converge_dev(rb_ctx *ctx, const rb_grid *g, double dt, double tol, i
         }
         auto &pe = ctx->prof[ctx->prof_used++];
         HIPCHK(hipEventRecord(pe.first, ctx->stream));
-        HIPCHK(rb::launch_chem_conv(a, cv, ctx->cfg.block, nblocks, ctx->stream));
+        HIPCHK(rb::launch_chem_conv(a, cv, ctx->cfg.block, nblocks, nblocks_aa, ctx->stream));
         HIPCHK(hipEventRecord(pe.second, ctx->stream));
     } else {
-        HIPCHK(rb::launch_chem_conv(a, cv, ctx->cfg.block, nblocks, ctx->stream));
+        HIPCHK(rb::launch_chem_conv(a, cv, ctx->cfg.block, nblocks, nblocks_aa, ctx->stream));
     }
     if (fuse_conserv && field > nbox)
         HIPCHK(rb::launch_conserv_floor(qj1, hc, plane, field, g->nivbas, nbox, ctx->stream));
@@ -580,14 +592,7 @@ int rb_run_day_dev(rb_ctx *ctx, const rb_grid *g, const rb_day_config *dc, const
     // scratch the steps use must exist before capture (no allocation inside it)
     const size_t nbox = (size_t)g->nlon * g->nlat * g->nivbas;
     if ((rc = ensure(ctx, &ctx->d_col, &ctx->col_cap, 3 * nbox))) return rc;
-    if (dc->converged && (rc = ensure(ctx, &ctx->d_rec, &ctx->rec_cap, (size_t)rb::CONV_RECORD * nbox)))
-        return rc;
-    if (dc->converged && dc->accel) {
-        const int blk = ctx->cfg.block > 0 ? ctx->cfg.block : 256;
-        const size_t lanes = (size_t)ctx->n_cu * rb::conv_blocks_per_cu(blk, dc->accel) * blk;
-        if ((rc = ensure(ctx, &ctx->d_aa, &ctx->aa_cap, 2 * (dc->accel + 1) * rb::AA_NX * lanes)))
-            return rc;
-    }
+    if (dc->converged && (rc = conv_buffers(ctx, nbox, dc->accel))) return rc;
     // the graph is rebuilt when anything it was captured with changes
     std::vector<char> key;
     auto add = [&](const void *v, size_t n) {
@@ -596,7 +601,7 @@ int rb_run_day_dev(rb_ctx *ctx, const rb_grid *g, const rb_day_config *dc, const
     };
     const void *ptrs[] = {trajlon, trajlat, T, p, alt, sza, qj1, hc, surfarea, vsed3d,
                           parth2osed, parthno3sed, iters, ctx->stream, ctx->d_col, ctx->d_aa,
-                          ctx->d_rec};
+                          ctx->d_rec, ctx->d_spill};
     add(g, sizeof *g);
     add(dc, sizeof *dc);
     add(ptrs, sizeof ptrs);
@@ -669,6 +674,7 @@ int rb_finalize(rb_ctx *ctx)
     if (ctx->d_work) (void)hipFree(ctx->d_work);
     if (ctx->d_aa) (void)hipFree(ctx->d_aa);
     if (ctx->d_rec) (void)hipFree(ctx->d_rec);
+    if (ctx->d_spill) (void)hipFree(ctx->d_spill);
     if (ctx->day_exec) (void)hipGraphExecDestroy(ctx->day_exec);
     if (ctx->day_graph) (void)hipGraphDestroy(ctx->day_graph);
     for (hipEvent_t e : {ctx->ev0, ctx->ev1, ctx->ev_h2d, ctx->ev_d2h})
