@@ -37,6 +37,19 @@ constexpr int64_t BOFF_W3 = BOFF_W2 + (int64_t)MLP_W * MLP_W;
 constexpr int64_t BOFF_W4 = BOFF_W3 + (int64_t)MLP_W * MLP_W;
 constexpr int64_t NWEIGHT = BOFF_W4 + (int64_t)MLP_W * MLP_PAD;
 
+// per-lane weight fragments of the fused guess kernel (k_pack_frags), BF16 count
+constexpr int64_t NFRAG = NWEIGHT;
+
+struct GuessArgs {
+    const double *T, *p, *sza, *qj1, *hc;   // state and inputs (model layout)
+    int64_t field, nbox;
+    double dt;
+    const __bf16 *frag;                      // packed weights (launch_pack_frags)
+    const float *bias;                       // the FP32 parameter block (biases at OFF_B*)
+    const float *norm;                       // in_mu, in_rsd, out_mu, out_sd (64 each)
+    double *gq, *gh;                         // the guess (model layout)
+};
+
 enum Epilogue { EPI_F32 = 0, EPI_HIDDEN_FWD = 1, EPI_OUT_FWD = 2, EPI_HIDDEN_BWD = 3 };
 
 struct GemmArgs {
@@ -77,5 +90,10 @@ hipError_t launch_features(const double *T, const double *p, const double *sza, 
                            hipStream_t s);
 hipError_t launch_apply_guess(const double *qj1, const double *hc, const float *Y, int64_t field,
                               int64_t nbox, double *gq, double *gh, hipStream_t s);
+
+hipError_t launch_pack_frags(const __bf16 *pb, __bf16 *frag, hipStream_t s);
+// features, the network and the guess formula in one launch (as launch_features,
+// launch_pack_inputs, the four GEMMs, launch_unpack_outputs and launch_apply_guess)
+hipError_t launch_guess_fused(const GuessArgs &a, hipStream_t s);
 
 }  // namespace rbm

@@ -17,18 +17,26 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 // -------------------------------------------------------------------- GELU (tanh form)
-__device__ __forceinline__ float gelu(float z)
+// gelu(z) = 0.5 z (1 + tanh u), u = sqrt(2/pi) (z + 0.044715 z^3), evaluated as
+// z sigma(2u) = z / (1 + exp(-2u)) (the same function: 1 + tanh u = 2 sigma(2u)), which
+// has no cancellation and no branch (the library tanh branches on |u|, which splits an
+// unrolled epilogue into one divergent region per value).  exp(-2u) = 2^(-2u log2 e) by
+// the hardware exp2; it underflows to 0 for large u and overflows to inf for very negative
+// u, where the quotient is z and -0 as it should be.
+__device__ __forceinline__ float gelu_sigma(float z)
 {
     const float u = 0.7978845608028654f * (z + 0.044715f * z * z * z);
-    return 0.5f * z * (1.f + tanhf(u));
+    return 1.f / (1.f + __builtin_amdgcn_exp2f(-2.8853900817779268f * u));
 }
 
+__device__ __forceinline__ float gelu(float z) { return z * gelu_sigma(z); }
+
+// d/dz [z sigma(2u)] = sigma + 2 z sigma (1 - sigma) u'
 __device__ __forceinline__ float gelu_grad(float z)
 {
-    const float u = 0.7978845608028654f * (z + 0.044715f * z * z * z);
-    const float t = tanhf(u);
-    return 0.5f * (1.f + t) +
-           0.5f * z * (1.f - t * t) * 0.7978845608028654f * (1.f + 3.f * 0.044715f * z * z);
+    const float sg = gelu_sigma(z);
+    const float du = 0.7978845608028654f * (1.f + 3.f * 0.044715f * z * z);
+    return sg + 2.f * z * sg * (1.f - sg) * du;
 }
 
 // -------------------------------------------------------------------- tiled GEMM
@@ -327,6 +335,201 @@ __global__ void k_apply_guess(const double *__restrict__ qj1, const double *__re
     }
 }
 
+// -------------------------------------------------------------------- fused guess
+// The guess of rb_mlp_guess_dev in one launch: features, the four layers and the guess
+// formula for 32 boxes per wave (8 waves per workgroup), activations in registers from
+// the features to the output.
+//
+// Each layer is computed transposed, C^T = W^T act^T, so that boxes are the MFMA columns:
+// lane l then holds outputs 4 (l / 16) + e (e = 0..3) of box l % 16 in each 16 x 16 tile.
+// Up to an order of k inside each 32-wide slab these are the values the next layer's B
+// operand needs from lane l (8 values of box l % 16 at k = 8 (l / 16) .. + 7).  The order
+// is absorbed into the weights: slot j of lane group g of slab s stands for input feature
+// 32 s + kperm(g, j) on both operands, so no activation passes through LDS.  The current
+// layer's weights are in LDS, prepacked as per-lane 16-byte fragments (k_pack_frags): the
+// copy in and every read are straight vector accesses, conflict-free.
+__host__ __device__ constexpr int kperm(int g, int j) { return j < 4 ? 4 * g + j : 16 + 4 * g + (j - 4); }
+
+// fragment block of layer l (KS slabs of 32 inputs, FT tiles of 16 outputs), BF16 offsets
+constexpr int64_t FOFF_L1 = 0;
+constexpr int64_t FOFF_L2 = FOFF_L1 + (int64_t)MLP_PAD * MLP_W;
+constexpr int64_t FOFF_L3 = FOFF_L2 + (int64_t)MLP_W * MLP_W;
+constexpr int64_t FOFF_L4 = FOFF_L3 + (int64_t)MLP_W * MLP_W;
+static_assert(FOFF_L4 + (int64_t)MLP_W * MLP_PAD == NFRAG, "fragment block size");
+
+// frag[((ft * KS + ks) * 64 + lane) * 8 + j] = W[32 ks + kperm(lane / 16, j)][16 ft + lane % 16]
+// for the four weight matrices (W row-major [in][out], the BF16 copies).
+__global__ void k_pack_frags(const __bf16 *__restrict__ pb, __bf16 *__restrict__ frag)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= NFRAG) return;
+    int64_t base, wbase;
+    int nin, nout;
+    if (i < FOFF_L2) { base = FOFF_L1; wbase = BOFF_W1; nin = MLP_PAD; nout = MLP_W; }
+    else if (i < FOFF_L3) { base = FOFF_L2; wbase = BOFF_W2; nin = MLP_W; nout = MLP_W; }
+    else if (i < FOFF_L4) { base = FOFF_L3; wbase = BOFF_W3; nin = MLP_W; nout = MLP_W; }
+    else { base = FOFF_L4; wbase = BOFF_W4; nin = MLP_W; nout = MLP_PAD; }
+    const int64_t li = i - base;
+    const int j = (int)(li & 7), lane = (int)((li >> 3) & 63);
+    const int tile = (int)(li >> 9), ks_n = nin / 32;
+    const int ks = tile % ks_n, ft = tile / ks_n;
+    const int k = 32 * ks + kperm(lane >> 4, j), n = 16 * ft + (lane & 15);
+    frag[i] = pb[wbase + (int64_t)k * nout + n];
+}
+
+constexpr int GW = 8;    // waves per workgroup
+constexpr int GB = 32;   // boxes per wave
+
+// raw feature f of box b, as k_features computes it
+__device__ __forceinline__ float guess_feature(const GuessArgs &a, int f, int64_t b)
+{
+    if (f < MLP_NLONG) return (float)log(fmax(a.qj1[(int64_t)long_slot(f) * a.field + b], 1.e-30));
+    if (f < MLP_NOUT) return (float)log(fmax(a.hc[(int64_t)(f - MLP_NLONG) * a.field + b], 1.e-30));
+    if (f == 55) return (float)a.T[b];
+    if (f == 56) return (float)log(a.p[b]);
+    if (f == 57) return (float)cos(a.sza[b] * 0.017453292519943295);
+    if (f == 58) return (float)log(fmax(a.qj1[(int64_t)SLOT_H2SO4 * a.field + b], 1.e-30));
+    return (float)log(a.dt);
+}
+
+// copies a layer's fragments to LDS; the fence keeps the compiler from starting these
+// loads while the previous layer's accumulators are still live (register pressure)
+__device__ __forceinline__ void stage_frags(bf16x8 *sW, const bf16x8 *src, int nfrag)
+{
+    asm volatile("" ::: "memory");
+#pragma unroll 4
+    for (int i = threadIdx.x; i < nfrag; i += GW * 64) sW[i] = src[i];
+}
+
+// bias + GELU of the FT = 16 output tiles into the next layer's operand (8 slabs)
+__device__ __forceinline__ void hidden_epilogue(const f32x4 (&acc)[16][2], const float *bias,
+                                                int g, bf16x8 (&act)[8][2])
+{
+#pragma unroll
+    for (int ks = 0; ks < 8; ++ks) {
+        // one slab at a time: its accumulator tiles die as its operand fills (the
+        // scheduling barrier keeps the GELUs of different slabs from interleaving)
+        __builtin_amdgcn_sched_barrier(0);
+        const float4 b0 = *reinterpret_cast<const float4 *>(bias + 32 * ks + 4 * g);
+        const float4 b1 = *reinterpret_cast<const float4 *>(bias + 32 * ks + 16 + 4 * g);
+        const float bb[8] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w};
+#pragma unroll
+        for (int bt = 0; bt < 2; ++bt)
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const int ft = 2 * ks + (j >> 2), e = j & 3;
+                act[ks][bt][j] = (__bf16)gelu(acc[ft][bt][e] + bb[j]);
+            }
+    }
+}
+
+template <int KS>
+__device__ __forceinline__ void hidden_layer(const bf16x8 *sW, const bf16x8 (&act)[8][2], int lane,
+                                             f32x4 (&acc)[16][2])
+{
+#pragma unroll
+    for (int ft = 0; ft < 16; ++ft)
+#pragma unroll
+        for (int bt = 0; bt < 2; ++bt) acc[ft][bt] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks)
+#pragma unroll
+        for (int ft = 0; ft < 16; ++ft) {
+            const bf16x8 w = sW[(ft * KS + ks) * 64 + lane];
+#pragma unroll
+            for (int bt = 0; bt < 2; ++bt)
+                acc[ft][bt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w, act[ks][bt], acc[ft][bt], 0, 0, 0);
+        }
+}
+
+__global__ void __launch_bounds__(GW * 64) k_guess_fused(GuessArgs a)
+{
+    __shared__ bf16x8 sW[(MLP_W / 32) * (MLP_W / 16) * 64];   // one layer's fragments, 128 KB
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, g = lane >> 4, r = lane & 15;
+    const int64_t box0 = ((int64_t)blockIdx.x * GW + wave) * GB;
+    const bf16x8 *frag = reinterpret_cast<const bf16x8 *>(a.frag);
+    const float *in_mu = a.norm, *in_rsd = a.norm + 64, *out_mu = a.norm + 128, *out_sd = a.norm + 192;
+
+    // features, normalised, as the layer-1 operand (slabs 0 and 1; inputs >= 60 are 0)
+    bf16x8 act[8][2];
+#pragma unroll
+    for (int bt = 0; bt < 2; ++bt) {
+        const int64_t b = box0 + 16 * bt + r;
+#pragma unroll
+        for (int ks = 0; ks < 2; ++ks)
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                asm volatile("" ::: "memory");
+                const int f = 32 * ks + kperm(g, j);
+                float v = 0.f;
+                if (b < a.nbox && f < MLP_NIN) v = (guess_feature(a, f, b) - in_mu[f]) * in_rsd[f];
+                act[ks][bt][j] = (__bf16)v;
+            }
+    }
+    f32x4 acc[16][2];
+    // layers 1-3 (every wave takes part in the staging and barriers, boxes or not)
+    stage_frags(sW, frag + FOFF_L1 / 8, 2 * 16 * 64);
+    __syncthreads();
+    hidden_layer<2>(sW, act, lane, acc);
+    __syncthreads();
+    hidden_epilogue(acc, a.bias + OFF_B1, g, act);
+    stage_frags(sW, frag + FOFF_L2 / 8, 8 * 16 * 64);
+    __syncthreads();
+    hidden_layer<8>(sW, act, lane, acc);
+    __syncthreads();
+    hidden_epilogue(acc, a.bias + OFF_B2, g, act);
+    stage_frags(sW, frag + FOFF_L3 / 8, 8 * 16 * 64);
+    __syncthreads();
+    hidden_layer<8>(sW, act, lane, acc);
+    __syncthreads();
+    hidden_epilogue(acc, a.bias + OFF_B3, g, act);
+    // layer 4: 4 output tiles
+    stage_frags(sW, frag + FOFF_L4 / 8, 8 * 4 * 64);
+    __syncthreads();
+    f32x4 out[4][2];
+#pragma unroll
+    for (int ft = 0; ft < 4; ++ft)
+#pragma unroll
+        for (int bt = 0; bt < 2; ++bt) out[ft][bt] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int ks = 0; ks < 8; ++ks)
+#pragma unroll
+        for (int ft = 0; ft < 4; ++ft) {
+            const bf16x8 w = sW[(ft * 8 + ks) * 64 + lane];
+#pragma unroll
+            for (int bt = 0; bt < 2; ++bt)
+                out[ft][bt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w, act[ks][bt], out[ft][bt], 0, 0, 0);
+        }
+    // the guess: max(x, 1e-30) exp(y) for output y = f, as k_unpack_outputs and k_apply_guess
+    const float *b4 = a.bias + OFF_B4;
+#pragma unroll
+    for (int bt = 0; bt < 2; ++bt) {
+        const int64_t b = box0 + 16 * bt + r;
+        if (b >= a.nbox) continue;
+#pragma unroll
+        for (int ft = 0; ft < 4; ++ft)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int f = 16 * ft + 4 * g + e;
+                if (f >= MLP_NOUT) continue;
+                const float yn = out[ft][bt][e] + b4[f];
+                const float y = yn * out_sd[f] + out_mu[f];
+                if (f < MLP_NLONG) {
+                    const int64_t o = (int64_t)long_slot(f) * a.field + b;
+                    a.gq[o] = fmax(a.qj1[o], 1.e-30) * exp((double)y);
+                } else {
+                    const int64_t o = (int64_t)(f - MLP_NLONG) * a.field + b;
+                    a.gh[o] = fmax(a.hc[o], 1.e-30) * exp((double)y);
+                }
+            }
+        if (g == 0) {
+            a.gq[(int64_t)SLOT_OX_PASSIVE * a.field + b] = a.qj1[(int64_t)SLOT_OX_PASSIVE * a.field + b];
+            a.gq[(int64_t)SLOT_H2SO4 * a.field + b] = a.qj1[(int64_t)SLOT_H2SO4 * a.field + b];
+            a.gq[(int64_t)SLOT_HNO3G * a.field + b] = a.qj1[(int64_t)SLOT_HNO3G * a.field + b];
+        }
+    }
+}
+
 // -------------------------------------------------------------------- launchers
 static inline unsigned blocks(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
 
@@ -391,6 +594,20 @@ hipError_t launch_apply_guess(const double *qj1, const double *hc, const float *
 {
     hipLaunchKernelGGL(k_apply_guess, dim3(blocks(nbox, 256)), dim3(256), 0, s, qj1, hc, Y, field,
                        nbox, gq, gh);
+    return hipGetLastError();
+}
+
+hipError_t launch_pack_frags(const __bf16 *pb, __bf16 *frag, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_pack_frags, dim3(blocks(NFRAG, 256)), dim3(256), 0, s, pb, frag);
+    return hipGetLastError();
+}
+
+hipError_t launch_guess_fused(const GuessArgs &a, hipStream_t s)
+{
+    if (a.nbox <= 0) return hipSuccess;
+    const int64_t per = (int64_t)GW * GB;
+    hipLaunchKernelGGL(k_guess_fused, dim3((unsigned)((a.nbox + per - 1) / per)), dim3(GW * 64), 0, s, a);
     return hipGetLastError();
 }
 

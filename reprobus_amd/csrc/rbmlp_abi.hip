@@ -31,6 +31,7 @@ struct rb_mlp {
     __bf16 *dyb = nullptr;
     float *dyf = nullptr, *partial = nullptr, *loss = nullptr, *yraw = nullptr, *xf = nullptr;
     float *ws = nullptr;         // split-K and column-sum partials
+    __bf16 *frag = nullptr;      // the BF16 weights as per-lane fragments of the fused guess
     bool train_bufs = false;
 };
 
@@ -200,6 +201,7 @@ int rb_mlp_create(rb_mlp **out, int device)
     if ((e = hipMalloc((void **)&m->m1, NPARAM * sizeof(float))) != hipSuccess) return bad(e);
     if ((e = hipMalloc((void **)&m->m2, NPARAM * sizeof(float))) != hipSuccess) return bad(e);
     if ((e = hipMalloc((void **)&m->pb, NWEIGHT * sizeof(__bf16))) != hipSuccess) return bad(e);
+    if ((e = hipMalloc((void **)&m->frag, NFRAG * sizeof(__bf16))) != hipSuccess) return bad(e);
     if ((e = hipMalloc((void **)&m->wbf, NPARAM * sizeof(int32_t))) != hipSuccess) return bad(e);
     if ((e = hipMalloc((void **)&m->norm, 5 * 64 * sizeof(float))) != hipSuccess) return bad(e);
     if ((e = hipMalloc((void **)&m->loss, sizeof(float))) != hipSuccess) return bad(e);
@@ -232,7 +234,7 @@ void rb_mlp_destroy(rb_mlp *m)
     void *bufs[] = {m->p, m->g, m->m1, m->m2, m->pb, m->wbf, m->norm, m->a0, m->act[0],
                     m->act[1], m->act[2], m->db[0], m->db[1], m->z[0], m->z[1], m->z[2],
                     m->yn, m->df[0], m->df[1], m->dyb, m->dyf, m->partial, m->loss, m->yraw,
-                    m->xf, m->ws};
+                    m->xf, m->ws, m->frag};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     delete m;
@@ -378,12 +380,11 @@ int rb_mlp_guess_dev(rb_mlp *m, const rb_grid *g, double dt, const double *T, co
     if (rc) return rc;
     if ((rc = check_n(m, nbox))) return rc;
     HIPCHK(hipSetDevice(m->device));
-    if ((rc = ensure_rows(m, nbox, false))) return rc;
-    HIPCHK(launch_features(T, p, sza, qj1, hc, field, nbox, dt, m->xf, m->stream));
-    HIPCHK(launch_pack_inputs(m->xf, nbox, m->norm, m->norm + 64, m->a0, m->stream));
-    if ((rc = forward(m, nbox, false))) return rc;
-    HIPCHK(launch_unpack_outputs(m->yn, nbox, m->norm + 128, m->norm + 192, m->yraw, m->stream));
-    HIPCHK(launch_apply_guess(qj1, hc, m->yraw, field, nbox, guess_qj1, guess_hc, m->stream));
+    // one launch for features, network and guess (the weights repacked first: they may
+    // have changed since the last call); no per-box work buffers
+    HIPCHK(launch_pack_frags(m->pb, m->frag, m->stream));
+    GuessArgs a{T, p, sza, qj1, hc, field, nbox, dt, m->frag, m->p, m->norm, guess_qj1, guess_hc};
+    HIPCHK(launch_guess_fused(a, m->stream));
     return RB_OK;
 }
 

@@ -146,6 +146,10 @@ def test_features_and_guess(net):
                           np.cos(np.radians(inp.sza)).reshape(1, -1).T,
                           lg(inp.qj1[[11]]), np.full((nb, 1), np.log(inp.dt))], axis=1)
     assert np.max(np.abs(Xh - ref.astype(np.float32)) / np.maximum(np.abs(ref), 1e-3)) < 1e-6
+    # the guess (one fused launch: features, network, formula): its implied log
+    # increments against the GEMM forward on the same features and against the reference;
+    # the fused network sums each MFMA slab in another order, so BF16 roundings of
+    # activations may differ: the forward tolerances apply
     flat, _, _, norm = _case(8, 7)
     net.set_params(flat)
     net.set_norm(*norm)
@@ -155,11 +159,21 @@ def test_features_and_guess(net):
     net.guess(grid, inp.dt, T, p, sza, q, h, gq, gh)
     torch.cuda.synchronize()
     y = Y.cpu().numpy().astype(np.float64)
-    qs = np.maximum(inp.qj1[LONG_SLOTS].reshape(40, -1), 1e-30) * np.exp(y[:, :40].T)
-    hs = np.maximum(inp.hc.reshape(15, -1), 1e-30) * np.exp(y[:, 40:].T)
     gqh, ghh = gq.cpu().numpy(), gh.cpu().numpy()
-    assert np.max(np.abs(gqh[LONG_SLOTS].reshape(40, -1) / qs - 1)) < 1e-12
-    assert np.max(np.abs(ghh.reshape(15, -1) / hs - 1)) < 1e-12
+    x_long = np.maximum(inp.qj1[LONG_SLOTS].reshape(40, -1), 1e-30)
+    x_short = np.maximum(inp.hc.reshape(15, -1), 1e-30)
+    y_fused = np.concatenate([np.log(gqh[LONG_SLOTS].reshape(40, -1) / x_long),
+                              np.log(ghh.reshape(15, -1) / x_short)]).T
+    ref = R.forward(flat, Xh, norm)
+    scale = np.abs(ref).max()
+    for name, other in (("gemm forward", y), ("reference", ref)):
+        err = np.abs(y_fused - other)
+        print(f"[mlp] fused guess vs {name}: max {err.max():.3e} median {np.median(err):.3e} "
+              f"(scale {scale:.2f})", flush=True)
+        assert err.max() < 2e-2 * scale
+        assert np.median(err) < 1e-4 * scale
+    # finite everywhere (boxes of the ragged last workgroup included); copied slots exact
+    assert np.all(np.isfinite(gqh)) and np.all(np.isfinite(ghh))
     for s in (10, 11, 42):
         assert np.array_equal(gqh[s], inp.qj1[s])
 
