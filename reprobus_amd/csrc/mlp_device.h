@@ -43,7 +43,7 @@ constexpr int64_t NFRAG = NWEIGHT;
 struct GuessArgs {
     const double *T, *p, *sza, *qj1, *hc;   // state and inputs (model layout)
     int64_t field, nbox;
-    double dt;
+    float log_dt;                            // log of the step (a feature)
     const __bf16 *frag;                      // packed weights (launch_pack_frags)
     const float *bias;                       // the FP32 parameter block (biases at OFF_B*)
     const float *norm;                       // in_mu, in_rsd, out_mu, out_sd (64 each)

@@ -21,12 +21,15 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 // z sigma(2u) = z / (1 + exp(-2u)) (the same function: 1 + tanh u = 2 sigma(2u)), which
 // has no cancellation and no branch (the library tanh branches on |u|, which splits an
 // unrolled epilogue into one divergent region per value).  exp(-2u) = 2^(-2u log2 e) by
-// the hardware exp2; it underflows to 0 for large u and overflows to inf for very negative
-// u, where the quotient is z and -0 as it should be.
+// the hardware exp2 and the reciprocal by the hardware rcp (1 ulp each, far below the
+// BF16 rounding of the activations); exp2 overflows to inf for very negative z, where
+// rcp gives 0 and the product -0, and underflows to 0 for large z, where it gives z.
 __device__ __forceinline__ float gelu_sigma(float z)
 {
-    const float u = 0.7978845608028654f * (z + 0.044715f * z * z * z);
-    return 1.f / (1.f + __builtin_amdgcn_exp2f(-2.8853900817779268f * u));
+    // -2 u log2(e) = z (c1 + c2 z^2), c1 = -2 sqrt(2/pi) log2(e), c2 = 0.044715 c1
+    const float c1 = -2.302208198144325f, c2 = -0.10294324f;
+    const float t = z * fmaf(c2, z * z, c1);
+    return __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(t));
 }
 
 __device__ __forceinline__ float gelu(float z) { return z * gelu_sigma(z); }
@@ -337,7 +340,7 @@ __global__ void k_apply_guess(const double *__restrict__ qj1, const double *__re
 
 // -------------------------------------------------------------------- fused guess
 // The guess of rb_mlp_guess_dev in one launch: features, the four layers and the guess
-// formula for 32 boxes per wave (8 waves per workgroup), activations in registers from
+// formula for 32 boxes per wave (4 waves per workgroup), activations in registers from
 // the features to the output.
 //
 // Each layer is computed transposed, C^T = W^T act^T, so that boxes are the MFMA columns:
@@ -345,9 +348,10 @@ __global__ void k_apply_guess(const double *__restrict__ qj1, const double *__re
 // Up to an order of k inside each 32-wide slab these are the values the next layer's B
 // operand needs from lane l (8 values of box l % 16 at k = 8 (l / 16) .. + 7).  The order
 // is absorbed into the weights: slot j of lane group g of slab s stands for input feature
-// 32 s + kperm(g, j) on both operands, so no activation passes through LDS.  The current
-// layer's weights are in LDS, prepacked as per-lane 16-byte fragments (k_pack_frags): the
-// copy in and every read are straight vector accesses, conflict-free.
+// 32 s + kperm(g, j) on both operands, so no activation passes through LDS.  The weights
+// are in LDS, prepacked as per-lane 16-byte fragments (k_pack_frags), 64 KB at a time (a
+// small layer, or half a 256 x 256 one): the copy in and every read are straight vector
+// accesses, conflict-free.
 __host__ __device__ constexpr int kperm(int g, int j) { return j < 4 ? 4 * g + j : 16 + 4 * g + (j - 4); }
 
 // fragment block of layer l (KS slabs of 32 inputs, FT tiles of 16 outputs), BF16 offsets
@@ -377,28 +381,42 @@ __global__ void k_pack_frags(const __bf16 *__restrict__ pb, __bf16 *__restrict__
     frag[i] = pb[wbase + (int64_t)k * nout + n];
 }
 
-constexpr int GW = 8;    // waves per workgroup
+constexpr int GW = 4;    // waves per workgroup (two workgroups per CU: one stages or waits
+                         // at a barrier while the other computes)
 constexpr int GB = 32;   // boxes per wave
+constexpr int GTILES = 64;   // LDS holds 64 fragment tiles (64 KB): half of a 256 x 256 layer
 
-// raw feature f of box b, as k_features computes it
-__device__ __forceinline__ float guess_feature(const GuessArgs &a, int f, int64_t b)
+// raw feature f of box b as k_features computes it, except that logarithms and the cosine
+// are taken in FP32 (hardware log2 times ln 2 and the hardware cosine, a few ulp of FP32;
+// the feature is rounded to BF16 right after normalisation); species are floored at 1e-30,
+// a normal FP32 number
+__device__ __forceinline__ float log_floor(double x)
 {
-    if (f < MLP_NLONG) return (float)log(fmax(a.qj1[(int64_t)long_slot(f) * a.field + b], 1.e-30));
-    if (f < MLP_NOUT) return (float)log(fmax(a.hc[(int64_t)(f - MLP_NLONG) * a.field + b], 1.e-30));
-    if (f == 55) return (float)a.T[b];
-    if (f == 56) return (float)log(a.p[b]);
-    if (f == 57) return (float)cos(a.sza[b] * 0.017453292519943295);
-    if (f == 58) return (float)log(fmax(a.qj1[(int64_t)SLOT_H2SO4 * a.field + b], 1.e-30));
-    return (float)log(a.dt);
+    return __builtin_amdgcn_logf((float)fmax(x, 1.e-30)) * 0.6931471805599453f;
 }
 
-// copies a layer's fragments to LDS; the fence keeps the compiler from starting these
-// loads while the previous layer's accumulators are still live (register pressure)
+__device__ __forceinline__ float guess_feature(const GuessArgs &a, int f, int64_t b)
+{
+    if (f < MLP_NLONG) return log_floor(a.qj1[(int64_t)long_slot(f) * a.field + b]);
+    if (f < MLP_NOUT) return log_floor(a.hc[(int64_t)(f - MLP_NLONG) * a.field + b]);
+    if (f == 55) return (float)a.T[b];
+    if (f == 56) return __builtin_amdgcn_logf((float)a.p[b]) * 0.6931471805599453f;
+    if (f == 57) return __cosf((float)a.sza[b] * 0.017453292519943295f);
+    if (f == 58) return log_floor(a.qj1[(int64_t)SLOT_H2SO4 * a.field + b]);
+    return a.log_dt;
+}
+
+// copies a layer's fragments to LDS with direct global-to-LDS loads (16 bytes per lane,
+// landing at the wave-uniform LDS base + 16 * lane, which is fragment i = base + lane):
+// no registers held, every copy in flight at once; waits for them before returning
 __device__ __forceinline__ void stage_frags(bf16x8 *sW, const bf16x8 *src, int nfrag)
 {
-    asm volatile("" ::: "memory");
-#pragma unroll 4
-    for (int i = threadIdx.x; i < nfrag; i += GW * 64) sW[i] = src[i];
+    const int lane = threadIdx.x & 63;
+#pragma unroll 16
+    for (int i = threadIdx.x; i < nfrag; i += GW * 64)
+        __builtin_amdgcn_global_load_lds(src + i, (__attribute__((address_space(3))) void *)(sW + (i - lane)),
+                                         16, 0, 0);
+    __builtin_amdgcn_s_waitcnt(0);
 }
 
 // bias + GELU of the FT = 16 output tiles into the next layer's operand (8 slabs)
@@ -407,9 +425,7 @@ __device__ __forceinline__ void hidden_epilogue(const f32x4 (&acc)[16][2], const
 {
 #pragma unroll
     for (int ks = 0; ks < 8; ++ks) {
-        // one slab at a time: its accumulator tiles die as its operand fills (the
-        // scheduling barrier keeps the GELUs of different slabs from interleaving)
-        __builtin_amdgcn_sched_barrier(0);
+        // one slab at a time: its accumulator tiles die as its operand fills
         const float4 b0 = *reinterpret_cast<const float4 *>(bias + 32 * ks + 4 * g);
         const float4 b1 = *reinterpret_cast<const float4 *>(bias + 32 * ks + 16 + 4 * g);
         const float bb[8] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w};
@@ -423,28 +439,49 @@ __device__ __forceinline__ void hidden_epilogue(const f32x4 (&acc)[16][2], const
     }
 }
 
-template <int KS>
-__device__ __forceinline__ void hidden_layer(const bf16x8 *sW, const bf16x8 (&act)[8][2], int lane,
-                                             f32x4 (&acc)[16][2])
+// MFMAs of output tiles [ft0, ft0 + NFT) over KS slabs, the tiles' fragments in LDS
+// (tile (ft - ft0) * KS + ks)
+template <int KS, int NFT>
+__device__ __forceinline__ void layer_tiles(const bf16x8 *sW, const bf16x8 (&act)[8][2], int lane,
+                                            int ft0, f32x4 (&acc)[16][2])
+{
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks)
+#pragma unroll
+        for (int f = 0; f < NFT; ++f) {
+            const bf16x8 w = sW[(f * KS + ks) * 64 + lane];
+#pragma unroll
+            for (int bt = 0; bt < 2; ++bt)
+                acc[ft0 + f][bt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w, act[ks][bt], acc[ft0 + f][bt], 0, 0, 0);
+        }
+}
+
+__device__ __forceinline__ void zero_acc(f32x4 (&acc)[16][2])
 {
 #pragma unroll
     for (int ft = 0; ft < 16; ++ft)
 #pragma unroll
         for (int bt = 0; bt < 2; ++bt) acc[ft][bt] = f32x4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int ks = 0; ks < KS; ++ks)
-#pragma unroll
-        for (int ft = 0; ft < 16; ++ft) {
-            const bf16x8 w = sW[(ft * KS + ks) * 64 + lane];
-#pragma unroll
-            for (int bt = 0; bt < 2; ++bt)
-                acc[ft][bt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w, act[ks][bt], acc[ft][bt], 0, 0, 0);
-        }
 }
 
-__global__ void __launch_bounds__(GW * 64) k_guess_fused(GuessArgs a)
+// a 256 x 256 layer in two halves of 8 output tiles (64 KB of fragments each)
+__device__ __forceinline__ void wide_layer(bf16x8 *sW, const bf16x8 *frag, const bf16x8 (&act)[8][2],
+                                           int lane, f32x4 (&acc)[16][2])
 {
-    __shared__ bf16x8 sW[(MLP_W / 32) * (MLP_W / 16) * 64];   // one layer's fragments, 128 KB
+    zero_acc(acc);
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        stage_frags(sW, frag + h * GTILES * 64, GTILES * 64);
+        __syncthreads();
+        if (h == 0) layer_tiles<8, 8>(sW, act, lane, 0, acc);
+        else layer_tiles<8, 8>(sW, act, lane, 8, acc);
+        __syncthreads();
+    }
+}
+
+__global__ void __launch_bounds__(GW * 64, 2) k_guess_fused(GuessArgs a)
+{
+    __shared__ bf16x8 sW[GTILES * 64];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, g = lane >> 4, r = lane & 15;
     const int64_t box0 = ((int64_t)blockIdx.x * GW + wave) * GB;
     const bf16x8 *frag = reinterpret_cast<const bf16x8 *>(a.frag);
@@ -459,7 +496,6 @@ __global__ void __launch_bounds__(GW * 64) k_guess_fused(GuessArgs a)
         for (int ks = 0; ks < 2; ++ks)
 #pragma unroll
             for (int j = 0; j < 8; ++j) {
-                asm volatile("" ::: "memory");
                 const int f = 32 * ks + kperm(g, j);
                 float v = 0.f;
                 if (b < a.nbox && f < MLP_NIN) v = (guess_feature(a, f, b) - in_mu[f]) * in_rsd[f];
@@ -467,39 +503,23 @@ __global__ void __launch_bounds__(GW * 64) k_guess_fused(GuessArgs a)
             }
     }
     f32x4 acc[16][2];
-    // layers 1-3 (every wave takes part in the staging and barriers, boxes or not)
-    stage_frags(sW, frag + FOFF_L1 / 8, 2 * 16 * 64);
+    // every wave takes part in the staging and barriers, boxes or not
+    // layer 1: 16 output tiles x 2 slabs (32 KB)
+    zero_acc(acc);
+    stage_frags(sW, frag + FOFF_L1 / 8, 32 * 64);
     __syncthreads();
-    hidden_layer<2>(sW, act, lane, acc);
+    layer_tiles<2, 16>(sW, act, lane, 0, acc);
     __syncthreads();
     hidden_epilogue(acc, a.bias + OFF_B1, g, act);
-    stage_frags(sW, frag + FOFF_L2 / 8, 8 * 16 * 64);
-    __syncthreads();
-    hidden_layer<8>(sW, act, lane, acc);
-    __syncthreads();
+    wide_layer(sW, frag + FOFF_L2 / 8, act, lane, acc);
     hidden_epilogue(acc, a.bias + OFF_B2, g, act);
-    stage_frags(sW, frag + FOFF_L3 / 8, 8 * 16 * 64);
-    __syncthreads();
-    hidden_layer<8>(sW, act, lane, acc);
-    __syncthreads();
+    wide_layer(sW, frag + FOFF_L3 / 8, act, lane, acc);
     hidden_epilogue(acc, a.bias + OFF_B3, g, act);
-    // layer 4: 4 output tiles
-    stage_frags(sW, frag + FOFF_L4 / 8, 8 * 4 * 64);
+    // layer 4: 4 output tiles x 8 slabs (32 KB)
+    zero_acc(acc);
+    stage_frags(sW, frag + FOFF_L4 / 8, 32 * 64);
     __syncthreads();
-    f32x4 out[4][2];
-#pragma unroll
-    for (int ft = 0; ft < 4; ++ft)
-#pragma unroll
-        for (int bt = 0; bt < 2; ++bt) out[ft][bt] = f32x4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int ks = 0; ks < 8; ++ks)
-#pragma unroll
-        for (int ft = 0; ft < 4; ++ft) {
-            const bf16x8 w = sW[(ft * 8 + ks) * 64 + lane];
-#pragma unroll
-            for (int bt = 0; bt < 2; ++bt)
-                out[ft][bt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w, act[ks][bt], out[ft][bt], 0, 0, 0);
-        }
+    layer_tiles<8, 4>(sW, act, lane, 0, acc);
     // the guess: max(x, 1e-30) exp(y) for output y = f, as k_unpack_outputs and k_apply_guess
     const float *b4 = a.bias + OFF_B4;
 #pragma unroll
@@ -512,7 +532,7 @@ __global__ void __launch_bounds__(GW * 64) k_guess_fused(GuessArgs a)
             for (int e = 0; e < 4; ++e) {
                 const int f = 16 * ft + 4 * g + e;
                 if (f >= MLP_NOUT) continue;
-                const float yn = out[ft][bt][e] + b4[f];
+                const float yn = acc[ft][bt][e] + b4[f];
                 const float y = yn * out_sd[f] + out_mu[f];
                 if (f < MLP_NLONG) {
                     const int64_t o = (int64_t)long_slot(f) * a.field + b;

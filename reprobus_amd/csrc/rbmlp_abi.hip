@@ -383,7 +383,8 @@ int rb_mlp_guess_dev(rb_mlp *m, const rb_grid *g, double dt, const double *T, co
     // one launch for features, network and guess (the weights repacked first: they may
     // have changed since the last call); no per-box work buffers
     HIPCHK(launch_pack_frags(m->pb, m->frag, m->stream));
-    GuessArgs a{T, p, sza, qj1, hc, field, nbox, dt, m->frag, m->p, m->norm, guess_qj1, guess_hc};
+    GuessArgs a{T, p, sza, qj1, hc, field, nbox, (float)std::log(dt), m->frag, m->p, m->norm,
+                guess_qj1, guess_hc};
     HIPCHK(launch_guess_fused(a, m->stream));
     return RB_OK;
 }
