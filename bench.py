@@ -4,9 +4,9 @@ Headline (`value`): the full 144 x 91 x 60 grid (786,240 boxes), one 900 s chemi
 step per timed step -- the reference scheme, 5 fixed sweeps in FP64, conserv + floor
 fused -- state resident in HBM; successive steps integrate the evolving state.
 
-Beside it on the same line: the converged mode (sweeps to the fixed point) from the
-incoming state ("plain") and from the learned initial guess ("mlp_guess") on the
-786,240-box c2 grid (polar vortex), with their agreement; a 96-step day of
+Beside it on the same line: the converged mode (sweeps to the fixed point; plain and
+Anderson-accelerated, each from the incoming state and from the learned initial guess)
+on the 786,240-box c2 grid (polar vortex), with agreement and sweep histograms; a 96-step day of
 (device zenith + step) as one HIP graph against the same steps launched one by one;
 the autotuned launch shapes; the roofline of the per-box kernel; and the reference
 Fortran timed on the host cores.  Prints one JSON line.
@@ -37,12 +37,13 @@ BYTES_PER_BOX = (65 + 62) * 8
 # restatement with a counting number type (oracle/opcount.cpp): 2,201 add, 3,544 mul,
 # 595 div, 13 sqrt; plus 166 calls of exp/log/pow/trig, counted as one operation each.
 FLOP_PER_BOX = 2201 + 3544 + 595 + 13 + 166
-# Measured HBM bytes per k_chem<256> launch on this workload (FETCH_SIZE and WRITE_SIZE
-# scaled by a 1 GiB copy of known traffic; profiles/k_chem_c1_pmc.md).
-TRAFFIC_BYTES = {256: (1101882 + 1358277) * 1024, 128: (1097924 + 1334544) * 1024}
-# FP64 VALU instructions per k_chem launch (SQ_INSTS_VALU_*_F64, same profile): add + mul
-# + transcendental + 2 x fma, per lane
-FP64_INSTR_FLOP = (68.80e6 + 55.70e6 + 7.10e6 + 2 * 72.58e6) * 64
+# Measured HBM bytes per k_chem launch on this workload, per block size (FETCH_SIZE x 2
+# + WRITE_SIZE, KiB, the factors from a 1 GiB copy of known traffic;
+# profiles/k_chem_c1_pmc.md).
+TRAFFIC_BYTES = {128: (2 * 501848 + 980881) * 1024, 256: (2 * 495735 + 955507) * 1024}
+# FP64 VALU instructions per k_chem launch (SQ_INSTS_VALU_*_F64, same profile, equal for
+# both block sizes): add + mul + transcendental + 2 x fma, per lane
+FP64_INSTR_FLOP = (58.116e6 + 53.457e6 + 7.444e6 + 2 * 75.381e6) * 64
 
 
 def parse():
@@ -120,10 +121,12 @@ def _timed(fn, reps, stream):
     return sorted(ms)[len(ms) // 2]
 
 
-def converged_section(table, block_conv, dev, tol=1e-10, maxiter=2000):
-    """Converged mode on the c2 grid (seed 1, the guess network's evaluation seed) from
-    the incoming state and from the learned guess; agreement over boxes that converged
-    on both paths."""
+def converged_section(table, block_conv, refill, dev, tol=1e-11, maxiter=2000):
+    """Converged mode on the c2 grid (seed 1, the guess network's evaluation seed): plain
+    sweeps and Anderson-accelerated sweeps, each from the incoming state and from the
+    learned guess (the guess launch inside the timed solve); agreement with the plain
+    solve over boxes that converged on both paths.  tol 1e-11 on the relative change per
+    sweep keeps the solves within 1e-9 of each other on all but ~1e-4 of the boxes."""
     import torch
 
     from reprobus_amd import guess, synth
@@ -134,7 +137,7 @@ def converged_section(table, block_conv, dev, tol=1e-10, maxiter=2000):
     g = inp.grid
     grid = make_grid(g.nlon, g.nlat, g.niv, g.nivbas, g.nivhau)
     st = _device_state(inp, dev)
-    step = ChemistryStep(*table, device=0, block=block_conv)
+    step = ChemistryStep(*table, device=0, block=block_conv, conv_refill=refill)
     net = GuessMLP(0)
     if not guess.load(net):
         return {"error": "no trained guess weights (reprobus_amd/data/guess_mlp.npz)"}
@@ -143,8 +146,7 @@ def converged_section(table, block_conv, dev, tol=1e-10, maxiter=2000):
     net.set_stream(stream.cuda_stream)
     q0, h0 = st["qj1"].clone(), st["hc"].clone()
     shb = (g.nivbas, g.nlat, g.nlon)
-    it_p = torch.zeros(shb, dtype=torch.int32, device=dev)
-    it_g = torch.zeros_like(it_p)
+    it = torch.zeros(shb, dtype=torch.int32, device=dev)
     gq, gh = torch.zeros_like(q0), torch.zeros_like(h0)
     args = [st[k] for k in ("T", "p", "sza", "alt")]
     aux = [st[k] for k in ("surfarea", "vsed3d", "parth2osed", "parthno3sed")]
@@ -152,17 +154,17 @@ def converged_section(table, block_conv, dev, tol=1e-10, maxiter=2000):
     nb = g.nivbas
     rel = lambda a, b: ((a - b).abs() / torch.maximum(a.abs(), b.abs()).clamp(min=1e-30)).amax(0)
     nbox = g.nbox
+    guess_fn = lambda: net.guess(grid, inp.dt, st["T"], st["p"], st["sza"], st["qj1"], st["hc"], gq, gh)
+    guess_ms = _timed(guess_fn, 5, stream)
     res, ref = {}, None
     # (name, Anderson depth, learned guess)
     for name, accel, use_guess in (("plain", 0, False), ("mlp_guess", 0, True),
                                    ("accelerated", 2, False), ("mlp_guess_accelerated", 2, True)):
-        it = it_g if use_guess else it_p
-
         def solve():
             st["qj1"].copy_(q0)
             st["hc"].copy_(h0)
             if use_guess:
-                net.guess(grid, inp.dt, st["T"], st["p"], st["sza"], st["qj1"], st["hc"], gq, gh)
+                guess_fn()
             step.converge_device(grid, inp.dt, tol, maxiter, *args, st["qj1"], st["hc"], *aux,
                                  guess_qj1=gq if use_guess else None,
                                  guess_hc=gh if use_guess else None, iters=it, accel=accel)
@@ -172,7 +174,8 @@ def converged_section(table, block_conv, dev, tol=1e-10, maxiter=2000):
         ms = _timed(solve, 3, stream)
         r = {"ms": ms, "box_steps_per_s": nbox / (ms / 1e3),
              "sweeps_mean": float(it.abs().float().mean()), "sweeps_max": int(it.abs().max()),
-             "fallback_boxes": int((it < 0).sum())}
+             "fallback_boxes": int((it < 0).sum()),
+             "sweep_histogram": _histogram(it.abs())}
         if ref is None:
             ref = (st["qj1"].clone(), st["hc"].clone(), it.clone())
         else:
@@ -186,11 +189,24 @@ def converged_section(table, block_conv, dev, tol=1e-10, maxiter=2000):
             r["speedup_over_plain"] = res["plain"]["ms"] / ms
         res[name] = r
     out = {"workload": "c2 seed 1 (786,240 boxes, polar vortex), incoming state, dt 900 s",
-           "tol": tol, "maxiter": maxiter, "block": block_conv,
-           "anderson": {"depth": 2, "from_sweep": 60}, **res,
+           "tol": tol, "maxiter": maxiter, "block": block_conv, "refill": refill,
+           "anderson": {"depth": 2, "from_sweep": step.conv_aa_start}, "guess_ms": guess_ms, **res,
            "speedup_mlp_over_plain": res["plain"]["ms"] / res["mlp_guess"]["ms"]}
     step.close()
     net.close()
+    return out
+
+
+def _histogram(iters):
+    """Boxes per sweep-count bin (edges 1, 5, 10, 20, 40, 60, 80, 120, 200, 500, 1000,
+    2000, above)."""
+    import torch
+
+    edges = [1, 5, 10, 20, 40, 60, 80, 120, 200, 500, 1000, 2000]
+    x = iters.reshape(-1).long()
+    out = {}
+    for lo, hi in zip(edges, edges[1:] + [1 << 30]):
+        out[f"{lo}-{hi - 1}" if hi < (1 << 30) else f">={lo}"] = int(((x >= lo) & (x < hi)).sum())
     return out
 
 
@@ -305,11 +321,10 @@ def main():
                 "kernel": f"k_chem<{block}>", "kernel_ms": chem_ms, "bytes_per_box": BYTES_PER_BOX,
                 "traffic_gbs": (traffic / (chem_ms / 1e3) / 1e9) if traffic else None,
                 "fp64": {"algorithmic_tflops": nbox * FLOP_PER_BOX / (chem_ms / 1e3) / 1e12,
-                         "instruction_tflops": (FP64_INSTR_FLOP / (chem_ms / 1e3) / 1e12
-                                                if block == 256 else None),
+                         "instruction_tflops": FP64_INSTR_FLOP / (chem_ms / 1e3) / 1e12,
                          "peak_tflops": FP64_PEAK_TFLOPS, "flop_per_box": FLOP_PER_BOX,
-                         "note": "instruction_tflops from the SQ_INSTS_VALU_*_F64 counters of "
-                                 "the block-256 kernel (profiles/k_chem_c1_pmc.md)"}}
+                         "note": "instruction_tflops from the SQ_INSTS_VALU_*_F64 counters "
+                                 "(profiles/k_chem_c1_pmc.md)"}}
 
     extras = {}
     if not args.no_extras:
@@ -317,9 +332,9 @@ def main():
             cache = os.path.join(tempfile.gettempdir(), "reprobus_amd_tune.json")
             c2 = _device_state(synth.make_c2(1), dev)
             r = autotune.tune(table, grid, c2, mode="converged", reps=2, cache_path=cache)
-            tuned["converged"] = {"block": r.block, "ms": r.ms}
+            tuned["converged"] = {"block": r.block, "refill": r.refill, "ms": r.ms}
             del c2
-            extras["converged"] = converged_section(table, r.block, dev)
+            extras["converged"] = converged_section(table, r.block, r.refill, dev)
         except Exception as exc:  # reported beside the headline, never instead of it
             extras["converged"] = {"error": repr(exc)}
         try:

@@ -3,13 +3,16 @@
 The per-box kernels run one box per lane; what is left to choose per device and grid is
 the workgroup size (64, 128 or 256 lanes: how many waves share one LDS block of
 start-of-step state, and so how the blocks pack onto the CUs and XCDs) for the
-fixed-sweep step and for the persistent converged-mode kernel, whose grid is always the
-resident wave set the occupancy calculator gives for that size.
+fixed-sweep step and for the persistent converged-mode kernels, whose grid is always the
+resident wave set the occupancy calculator gives for that size; and, for the converged
+mode, the queue chunk (how many lanes of a wave must be idle before the wave takes new
+boxes: a small chunk keeps lanes busy, a large one runs the set-up and write-back code
+less often).  The converged mode is tuned on the accelerated solve.
 
 `tune` times each candidate on the caller's own device state (restored between trials,
 so the state is unchanged on return), keeps the fastest, and caches the choice in a JSON
 file keyed by device name, grid and mode.  Every candidate computes the same numbers:
-the launch shape changes no result (checked by the tests).
+the launch shape and the chunk change no result (checked by the tests).
 """
 from __future__ import annotations
 
@@ -19,13 +22,15 @@ import time
 from dataclasses import dataclass, field
 
 BLOCKS = (64, 128, 256)
+REFILLS = (8, 16, 32)
 
 
 @dataclass
 class TuneResult:
     mode: str
     block: int
-    ms: dict = field(default_factory=dict)       # block -> median ms
+    refill: int | None = None                    # converged mode: the queue chunk
+    ms: dict = field(default_factory=dict)       # "block" or "block/refill" -> median ms
     cached: bool = False
 
 
@@ -49,18 +54,27 @@ def save_cache(path: str | None, cache: dict) -> None:
     os.replace(tmp, path)
 
 
-def choose(timings: dict) -> int:
-    """The fastest candidate; ties go to the larger block (fewer workgroups)."""
-    return min(timings, key=lambda b: (timings[b], -b))
+def _label(shape: tuple) -> str:
+    return "/".join(str(v) for v in shape)
+
+
+def choose(timings: dict):
+    """The fastest candidate (a block, or a (block, refill) pair); ties go to the larger
+    block (fewer workgroups), then the larger chunk."""
+    def key(c):
+        t = c if isinstance(c, tuple) else (c,)
+        return (timings[c],) + tuple(-v for v in t)
+    return min(timings, key=key)
 
 
 def tune(table, grid, state: dict, mode: str = "plain", candidates=BLOCKS, reps: int = 3,
-         cache_path: str | None = None, device: int = 0, tol: float = 1e-10,
-         maxiter: int = 2000, timer=None) -> TuneResult:
-    """Pick the block size for `mode` ("plain": the fixed-sweep step, "converged": the
-    persistent converged-mode kernel) on `state` (device tensors T, p, sza, alt, qj1, hc,
-    surfarea, vsed3d, parth2osed, parthno3sed, dt).  `timer(block) -> ms` replaces the
-    GPU timing (tests)."""
+         cache_path: str | None = None, device: int = 0, tol: float = 1e-11,
+         maxiter: int = 2000, accel: int = 2, refills=REFILLS, timer=None) -> TuneResult:
+    """Pick the launch shape for `mode` ("plain": the fixed-sweep step's block;
+    "converged": block and queue chunk of the converged mode with Anderson depth `accel`)
+    on `state` (device tensors T, p, sza, alt, qj1, hc, surfarea, vsed3d, parth2osed,
+    parthno3sed, dt).  `timer(block)` (plain) or `timer(block, refill)` (converged) -> ms
+    replaces the GPU timing (tests)."""
     import torch
 
     from .chem import ChemistryStep
@@ -68,11 +82,13 @@ def tune(table, grid, state: dict, mode: str = "plain", candidates=BLOCKS, reps:
     name = torch.cuda.get_device_name(device) if timer is None else "test-device"
     cache = load_cache(cache_path)
     key = _key(name, grid, mode)
+    conv = mode != "plain"
     if key in cache and cache[key]["block"] in candidates:
-        return TuneResult(mode, cache[key]["block"], cache[key]["ms"], cached=True)
+        c = cache[key]
+        return TuneResult(mode, c["block"], c.get("refill"), c["ms"], cached=True)
 
-    def measure(block: int) -> float:
-        step = ChemistryStep(*table, device=device, block=block)
+    def measure(block: int, refill: int = 16) -> float:
+        step = ChemistryStep(*table, device=device, block=block, conv_refill=refill)
         try:
             s = torch.cuda.current_stream()
             step.set_stream(s.cuda_stream)
@@ -81,12 +97,12 @@ def tune(table, grid, state: dict, mode: str = "plain", candidates=BLOCKS, reps:
             outs = [state[k] for k in ("surfarea", "vsed3d", "parth2osed", "parthno3sed")]
 
             def once():
-                if mode == "plain":
+                if not conv:
                     step.step_device(grid, state["dt"], *args, state["qj1"], state["hc"], *outs,
                                      fuse_conserv=True)
                 else:
                     step.converge_device(grid, state["dt"], tol, maxiter, *args, state["qj1"],
-                                         state["hc"], *outs)
+                                         state["hc"], *outs, accel=accel)
 
             times = []
             for r in range(reps + 1):                   # first run: warm-up
@@ -107,9 +123,15 @@ def tune(table, grid, state: dict, mode: str = "plain", candidates=BLOCKS, reps:
             step.close()
 
     timer = timer or measure
-    ms = {int(b): float(timer(int(b))) for b in candidates}
-    best = choose(ms)
-    cache[key] = {"block": best, "ms": {str(b): v for b, v in ms.items()},
+    if conv:
+        ms = {(int(b), int(f)): float(timer(int(b), int(f))) for b in candidates for f in refills}
+        best = choose(ms)
+        block, refill = best
+    else:
+        ms = {int(b): float(timer(int(b))) for b in candidates}
+        block, refill = choose(ms), None
+    labels = {_label(k if isinstance(k, tuple) else (k,)): v for k, v in ms.items()}
+    cache[key] = {"block": block, "refill": refill, "ms": labels,
                   "when": time.strftime("%Y-%m-%dT%H:%M:%S")}
     save_cache(cache_path, cache)
-    return TuneResult(mode, best, ms)
+    return TuneResult(mode, block, refill, labels)

@@ -44,6 +44,7 @@ class ChemistryStep:
                  conv_refill: int = 16, conv_aa_start: int = 60):
         self._L = _lib.load()
         self._ctx = C.c_void_p()
+        self.conv_refill, self.conv_aa_start = conv_refill, conv_aa_start
         cfg = Config(niter, 1 if check_finite else 0, block, 1 if profile else 0, conv_refill,
                      conv_aa_start)
         rc = self._L.rb_init(C.byref(self._ctx), device, C.byref(cfg))

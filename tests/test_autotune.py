@@ -1,4 +1,4 @@
-"""Autotuner host logic: choice, tie rule and the JSON cache (timings injected)."""
+"""Autotuner host logic: choice, tie rules and the JSON cache (timings injected)."""
 from reprobus_amd import autotune
 from reprobus_amd.chem import make_grid
 
@@ -9,20 +9,26 @@ def test_choice_and_cache(tmp_path):
     fake = {64: 2.0, 128: 1.5, 256: 1.7}
     calls = []
 
-    def timer(b):
-        calls.append(b)
-        return fake[b]
+    def timer(b, refill=None):
+        calls.append((b, refill))
+        return fake[b] + (0.0 if refill in (None, 16) else 0.25)
 
     r = autotune.tune(None, grid, {}, mode="plain", cache_path=path, timer=timer)
-    assert r.block == 128 and not r.cached and sorted(calls) == [64, 128, 256]
+    assert r.block == 128 and r.refill is None and not r.cached and len(calls) == 3
+    assert r.ms == {"64": 2.0, "128": 1.5, "256": 1.7}
     r2 = autotune.tune(None, grid, {}, mode="plain", cache_path=path, timer=timer)
     assert r2.cached and r2.block == 128 and len(calls) == 3        # served from the cache
+    # the converged mode: every (block, chunk) pair timed, another cache key
     r3 = autotune.tune(None, grid, {}, mode="converged", cache_path=path, timer=timer)
-    assert not r3.cached and len(calls) == 6                          # another key
+    assert not r3.cached and len(calls) == 3 + 9
+    assert (r3.block, r3.refill) == (128, 16) and r3.ms["128/16"] == 1.5
+    r4 = autotune.tune(None, grid, {}, mode="converged", cache_path=path, timer=timer)
+    assert r4.cached and (r4.block, r4.refill) == (128, 16)
     other = make_grid(72, 46, 60, 60, 14)
     assert not autotune.tune(None, other, {}, cache_path=path, timer=timer).cached
 
 
-def test_ties_go_to_the_larger_block():
+def test_ties_go_to_the_larger_block_then_chunk():
     assert autotune.choose({64: 1.0, 128: 1.0, 256: 1.2}) == 128
     assert autotune.choose({64: 1.0, 128: 1.0, 256: 1.0}) == 256
+    assert autotune.choose({(128, 8): 1.0, (128, 32): 1.0, (64, 32): 1.0}) == (128, 32)

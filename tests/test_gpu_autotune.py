@@ -33,12 +33,16 @@ def test_tune_and_shape_independence(table, mode, tmp_path):
     st = _state(inp)
     q_before = st["qj1"].cpu().numpy().copy()
     r = autotune.tune(table, grid, st, mode=mode, reps=2, cache_path=str(tmp_path / "t.json"))
-    print(f"[autotune] {mode}: block {r.block}, ms {r.ms}", flush=True)
+    print(f"[autotune] {mode}: block {r.block}, refill {r.refill}, ms {r.ms}", flush=True)
     assert r.block in autotune.BLOCKS
+    assert (r.refill is None) == (mode == "plain")
     assert np.array_equal(st["qj1"].cpu().numpy(), q_before)
     outs = []
-    for b in autotune.BLOCKS:
-        s = ChemistryStep(*table, device=0, block=b)
+    shapes = [(b, 16) for b in autotune.BLOCKS]
+    if mode == "converged":
+        shapes += [(128, f) for f in autotune.REFILLS if f != 16]
+    for b, f in shapes:
+        s = ChemistryStep(*table, device=0, block=b, conv_refill=f)
         s.set_stream(torch.cuda.current_stream().cuda_stream)
         q, h = st["qj1"].clone(), st["hc"].clone()
         args = [st[k] for k in ("T", "p", "sza", "alt")]
@@ -46,7 +50,7 @@ def test_tune_and_shape_independence(table, mode, tmp_path):
         if mode == "plain":
             s.step_device(grid, inp.dt, *args, q, h, *aux, fuse_conserv=True)
         else:
-            s.converge_device(grid, inp.dt, 1e-10, 500, *args, q, h, *aux)
+            s.converge_device(grid, inp.dt, 1e-10, 500, *args, q, h, *aux, accel=2)
         torch.cuda.synchronize()
         outs.append((q.cpu().numpy(), h.cpu().numpy()))
         s.close()
