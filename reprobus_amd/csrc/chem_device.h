@@ -54,7 +54,7 @@ struct ConvArgs {
                                        // did not converge in n sweeps and holds the
                                        // fixed-sweep (niter) answer instead
     int32_t accel;                     // Anderson depth: 0 (plain sweeps) or 2
-    double *aa_ws;                     // accel > 0: 2 (accel + 1) * 57 doubles per lane
+    double *aa_ws;                     // accel > 0: (2 (accel + 1) + 1) * 57 doubles per lane
     int32_t refill;                    // idle lanes of a wave before they take new boxes
     const double *rec;                 // per-box set-up records (launch_conv_setup)
     int32_t aa_start;                  // accel > 0: plain sweeps before this sweep number

@@ -1596,7 +1596,7 @@ __global__ void __launch_bounds__(BLK) k_chem(StepArgs a)
 // arithmetic in the same order as the restatement's aa_update.  Per lane, a ring of M + 1
 // (f, g) pairs lives in device memory: F(slot, i) = ws[((2 slot) * NX + i) * stride],
 // G(slot, i) = ws[((2 slot + 1) * NX + i) * stride] (stride = lanes of the grid, so a
-// wave's accesses coalesce).  On entry x holds f = F(x_k) and G(head) the iterate x_k
+// wave's accesses coalesce), and one more row for the new iterate: 2 (M + 1) + 1 rows.  On entry x holds f = F(x_k) and G(head) the iterate x_k
 // (aa_save before the sweep), which becomes the residual g_k = f - x_k; on exit x holds
 // x_{k+1} = f - dF gamma, gamma
 // minimising || w (g_k - dG gamma) ||, w_i = 1 / max(|f_i|, fl)^2; components that would
@@ -1621,8 +1621,10 @@ __device__ void aa_update(double *ck, double *cc, double *ws, int64_t stride, in
 {
     auto F = [&](int slot, int i) -> double & { return ws[((int64_t)(2 * slot) * NX + i) * stride]; };
     auto G = [&](int slot, int i) -> double & { return ws[((int64_t)(2 * slot + 1) * NX + i) * stride]; };
+    auto XN = [&](int i) -> double & { return ws[((int64_t)(2 * (M + 1)) * NX + i) * stride]; };
     auto X = [&](int i) -> double & { return i < NLONG - 1 ? ck[i] : cc[i - (NLONG - 1)]; };
     const int slot = head;
+    // f and g = f - x_k into the ring (x_k was saved into G(slot) before the sweep)
 #pragma unroll
     for (int i = 0; i < NX; ++i) {
         F(slot, i) = X(i);
@@ -1638,14 +1640,16 @@ __device__ void aa_update(double *ck, double *cc, double *ws, int64_t stride, in
         ia[j] = (slot - j + (M + 1)) % (M + 1);
         ib[j] = (slot - j - 1 + 2 * (M + 1)) % (M + 1);
     }
+    // the normal equations, streaming over the species from the ring (a runtime loop: the
+    // box state stays in registers, the history is read as it is needed)
     double Mx[M][M + 1];
 #pragma unroll
     for (int r = 0; r < M; ++r)
 #pragma unroll
         for (int c = 0; c <= M; ++c) Mx[r][c] = 0.;
-#pragma unroll
+#pragma unroll 1
     for (int i = 0; i < NX; ++i) {
-        const double sc = fmax(fabs(X(i)), fl);
+        const double sc = fmax(fabs(F(slot, i)), fl);
         const double w = 1. / (sc * sc);
         const double gk = G(slot, i);
         double dg[M];
@@ -1714,14 +1718,19 @@ __device__ void aa_update(double *ck, double *cc, double *ws, int64_t stride, in
         if (!(fabs(gam[r]) <= 1.e6)) ok = false;
     }
     if (!ok) return;
-#pragma unroll
+    // x_{k+1} = f - dF gamma into the XN row (components that would turn non-positive
+    // keep f), then back into the registers
+#pragma unroll 1
     for (int i = 0; i < NX; ++i) {
-        double v = X(i);
+        const double fi = F(slot, i);
+        double v = fi;
 #pragma unroll
         for (int j = 0; j < M; ++j)
             if (j < mk) v -= gam[j] * (F(ia[j], i) - F(ib[j], i));
-        if (v > 0. && v <= 1.7976931348623157e308) X(i) = v;
+        XN(i) = (v > 0. && v <= 1.7976931348623157e308) ? v : fi;
     }
+#pragma unroll
+    for (int i = 0; i < NX; ++i) X(i) = XN(i);
 }
 
 // Converged mode: sweeps to the fixed point of the sweep map.  Persistent: each lane

@@ -103,7 +103,7 @@ int conv_buffers(rb_ctx *ctx, size_t nbox, int accel)
     if (rc || !accel) return rc;
     const int blk = ctx->cfg.block > 0 ? ctx->cfg.block : 256;
     const size_t lanes = (size_t)ctx->n_cu * rb::conv_blocks_per_cu(blk, accel, true) * blk;
-    if ((rc = ensure(ctx, &ctx->d_aa, &ctx->aa_cap, 2 * (accel + 1) * rb::AA_NX * lanes)))
+    if ((rc = ensure(ctx, &ctx->d_aa, &ctx->aa_cap, (2 * (accel + 1) + 1) * rb::AA_NX * lanes)))
         return rc;
     return ensure(ctx, &ctx->d_spill, &ctx->spill_cap, (size_t)rb::CONV_SPILL * nbox);
 }
