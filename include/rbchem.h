@@ -125,6 +125,31 @@ int rb_altitude_dev(rb_ctx *ctx, const rb_grid *g, double dlon, double dlat,
                     const double *trajlon, const double *trajlat, const double *trajpre,
                     double *alt);
 
+/* Semi-Lagrangian transport interpolation: what semilag does after the trajectories
+ * (F:6121-6153), on the transport grid nlon x nlat x niv (g->nivbas, g->nivhau unused; the
+ * model: 180 x 91 x 137, lon fastest as the model's (nlon, nlat, niv) arrays).  Departure
+ * points: x0, y0 longitude and latitude (rad, x0 in [0, 2 pi), y0 inside the poles), z0
+ * pressure (Pa) and p0 the surface pressure of the departure column (hPa), as backtra's
+ * COMMON /depo/ (F:6215-6217); aaa, bbb the niv hybrid coefficients (COMMON /verniv/).
+ * Each entry synchronises the stream (it reads back the count of departure points whose
+ * cell falls outside the grid, RB_EINVAL when there is any). */
+/* rb_transport_weights_dev replaces posx/posy/posz and poidx/poidy/poidz as backtra calls
+ * them (F:6444-6451, F:7371-7647): idx (3 n int32: ig0, kg0, lg0, 1-based, COMMON
+ * /indices/) and w (12 n doubles: alfxm1 .. alfzp2, COMMON /poids/ order), device. */
+int rb_transport_weights_dev(rb_ctx *ctx, const rb_grid *g, const double *aaa, const double *bbb,
+                             const double *x0, const double *y0, const double *z0,
+                             const double *p0, int32_t *idx, double *w);
+/* rb_transport_interp_dev replaces `do jc = 1, nbcon; call internew(qj1, jc)` (F:6148-6150,
+ * F:8339-8618): ntr fields q_in (tracer stride n) interpolated at the departure points
+ * described by idx and w into q_out (same layout, must not overlap q_in), device. */
+int rb_transport_interp_dev(rb_ctx *ctx, const rb_grid *g, const int32_t *idx, const double *w,
+                            int32_t ntr, const double *q_in, double *q_out);
+/* rb_transport: both on host arrays, the ntr fields of qj1 updated in place (copies
+ * through the context's device buffers). */
+int rb_transport(rb_ctx *ctx, const rb_grid *g, const double *aaa, const double *bbb,
+                 const double *x0, const double *y0, const double *z0, const double *p0,
+                 int32_t ntr, double *qj1);
+
 /* A day of sub-steps on device-resident state (the model's loop F:300-440 over one day of
  * chemistry, without transport): for k = 0..nsteps-1, rb_zenith_dev at day number
  * day0 + k dt / 86400 into sza (device, nlon*nlat*nivbas), then rb_chem_step_dev (or, with

@@ -58,6 +58,11 @@ def lib(cr_psc: bool = False):
             [dp] * 12 + [np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS"), C.c_void_p,
                          C.c_void_p, C.c_int]
         L.rbo_chem_converge_aa.restype = C.c_int
+        ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+        L.rbo_transport_weights.argtypes = [C.c_int, C.c_int, C.c_int] + [dp] * 6 + [ip, dp]
+        L.rbo_transport_weights.restype = None
+        L.rbo_internew.argtypes = [C.c_int, C.c_int, C.c_int, ip, dp, dp, dp]
+        L.rbo_internew.restype = C.c_int64
         L.rbo_troe_constants.argtypes = [dp]
         L.rbo_troe.argtypes = [C.c_double] * 6
         L.rbo_troe.restype = C.c_double
@@ -171,3 +176,23 @@ def altitude(relief, psol, aaa, bbb, t, h2o, tlon, tlat, tpre, dlon=2.0, dlat=2.
     lib().rbo_altitude(mlon, mlat, mniv, dlon, dlat, c(relief), c(psol), c(aaa), c(bbb), c(t),
                        c(h2o), c(tlon), c(tlat), c(tpre), out)
     return out
+
+
+def transport_weights(tr) -> tuple:
+    """Cell indices (3, n) int32 and Lagrange weights (12, n) of the departure points of
+    `tr` (synth.TransportInputs): posx/posy/posz and poidx/poidy/poidz (F:7371-7647)."""
+    n = tr.nlon * tr.nlat * tr.niv
+    idx = np.empty((3, n), dtype=np.int32)
+    w = np.empty((12, n))
+    lib().rbo_transport_weights(tr.nlon, tr.nlat, tr.niv, tr.aaa, tr.bbb, tr.x0.reshape(-1),
+                                tr.y0.reshape(-1), tr.z0.reshape(-1), tr.p0.reshape(-1), idx, w)
+    return idx, w
+
+
+def internew(tr, idx, w, q) -> tuple:
+    """One tracer field q (niv, nlat, nlon) interpolated at the departure points (internew,
+    F:8339-8618); returns (field, number of points outside the grid)."""
+    out = np.zeros_like(np.ascontiguousarray(q, dtype=np.float64))
+    bad = lib().rbo_internew(tr.nlon, tr.nlat, tr.niv, np.ascontiguousarray(idx, dtype=np.int32),
+                             np.ascontiguousarray(w), np.ascontiguousarray(q, dtype=np.float64), out)
+    return out, int(bad)

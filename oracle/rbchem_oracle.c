@@ -1654,4 +1654,149 @@ void rbo_altitude(int mlon, int mlat, int mniv, double dlon, double dlat, const 
     free(tv);
     free(geo);
 }
+
+/* ---------------------------------------------------------------- transport interpolation */
+/* The grid constants of initransp (F:6028-6034, 6041, 6051, 6075-6085). */
+typedef struct {
+    double ddlon, ddlat, xpih, xcoef[4], ycoef[4];
+} tgrid_t;
+
+static tgrid_t transport_grid(int nlon, int nlat)
+{
+    tgrid_t t;
+    const double xpi = 2. * asin(1.);
+    const double xpi2 = 2. * xpi;
+    t.xpih = xpi / 2.;
+    t.ddlat = xpi / (double)(nlat - 1);
+    t.ddlon = xpi2 / (double)nlon;
+    const double coefx = 1. / (t.ddlon * t.ddlon * t.ddlon);
+    t.xcoef[0] = -coefx / 6.;
+    t.xcoef[1] = coefx / 2.;
+    t.xcoef[2] = -coefx / 2.;
+    t.xcoef[3] = coefx / 6.;
+    const double coefy = 1. / (t.ddlat * t.ddlat * t.ddlat);
+    t.ycoef[0] = -coefy / 6.;
+    t.ycoef[1] = coefy / 2.;
+    t.ycoef[2] = -coefy / 2.;
+    t.ycoef[3] = coefy / 6.;
+    return t;
+}
+
+void rbo_transport_weights(int nlon, int nlat, int niv, const double *aaa, const double *bbb,
+                           const double *x0, const double *y0, const double *z0,
+                           const double *p0, int32_t *idx, double *w)
+{
+    const tgrid_t t = transport_grid(nlon, nlat);
+    const int64_t n = (int64_t)nlon * nlat * niv;
+#define A(l) aaa[(l) - 1]
+#define B(l) bbb[(l) - 1]
+    for (int64_t i = 0; i < n; ++i) {
+        /* posx F:7402-7406 */
+        const int ix = (int)(x0[i] / t.ddlon);
+        const double dx = x0[i] - (double)ix * t.ddlon;
+        /* posy F:7442-7446 */
+        const int iy = (int)((t.xpih - y0[i]) / t.ddlat);
+        const double dy = t.xpih - (double)iy * t.ddlat - y0[i];
+        /* posz F:7485-7494: the last level l in 2..niv-1 whose pressure is <= z0 */
+        const double zz = z0[i], pp = p0[i];
+        int ll = 1;
+        for (int l = 2; l <= niv - 1; ++l) {
+            const double pres = (A(l) + B(l) * pp) * 100.;
+            if (zz - pres >= 0.) ll = l;
+        }
+        idx[i] = ix + 1;
+        idx[n + i] = iy + 1;
+        idx[2 * n + i] = ll;
+        /* poidx F:7529-7536 */
+        {
+            const double dxm1 = dx + t.ddlon, dxp0 = dx, dxp1 = dx - t.ddlon, dxp2 = dx - t.ddlon * 2.;
+            w[i] = t.xcoef[0] * dxp0 * dxp1 * dxp2;
+            w[n + i] = t.xcoef[1] * dxm1 * dxp1 * dxp2;
+            w[2 * n + i] = t.xcoef[2] * dxm1 * dxp0 * dxp2;
+            w[3 * n + i] = t.xcoef[3] * dxm1 * dxp0 * dxp1;
+        }
+        /* poidy F:7571-7578 */
+        {
+            const double dym1 = dy + t.ddlat, dyp0 = dy, dyp1 = dy - t.ddlat, dyp2 = dy - t.ddlat * 2.;
+            w[4 * n + i] = t.ycoef[0] * dyp0 * dyp1 * dyp2;
+            w[5 * n + i] = t.ycoef[1] * dym1 * dyp1 * dyp2;
+            w[6 * n + i] = t.ycoef[2] * dym1 * dyp0 * dyp2;
+            w[7 * n + i] = t.ycoef[3] * dym1 * dyp0 * dyp1;
+        }
+        /* poidz F:7614-7644 */
+        {
+            const double prem1 = ll > 1 ? (A(ll - 1) + B(ll - 1) * pp) * 100. : 5.e-3 * 100.;
+            const double prep0 = (A(ll) + B(ll) * pp) * 100.;
+            const double prep1 = (A(ll + 1) + B(ll + 1) * pp) * 100.;
+            const double prep2 = ll < niv - 1 ? (A(ll + 2) + B(ll + 2) * pp) * 100. : 1200. * 100.;
+            const double dzm1 = zz - prem1, dzp0 = zz - prep0, dzp1 = zz - prep1, dzp2 = zz - prep2;
+            const double dzp2m1 = prep2 - prem1, dzp2p0 = prep2 - prep0, dzp2p1 = prep2 - prep1;
+            const double dzp1m1 = prep1 - prem1, dzp1p0 = prep1 - prep0, dzp0m1 = prep0 - prem1;
+            w[8 * n + i] = -(dzp0 * dzp1 * dzp2) / (dzp0m1 * dzp1m1 * dzp2m1);
+            w[9 * n + i] = (dzm1 * dzp1 * dzp2) / (dzp0m1 * dzp1p0 * dzp2p0);
+            w[10 * n + i] = -(dzm1 * dzp0 * dzp2) / (dzp1m1 * dzp1p0 * dzp2p1);
+            w[11 * n + i] = (dzm1 * dzp0 * dzp1) / (dzp2m1 * dzp2p0 * dzp2p1);
+        }
+    }
+#undef A
+#undef B
+}
+
+int64_t rbo_internew(int nlon, int nlat, int niv, const int32_t *idx, const double *w,
+                     const double *q, double *out)
+{
+    const int64_t n = (int64_t)nlon * nlat * niv, plane = (int64_t)nlon * nlat;
+    /* the extended grid x(0:nlon+2, 0:nlat+1, 0:niv+1) of F:8374-8413, as a map to the
+     * field: levels clamped (F:8408-8411), longitudes periodic (F:8399-8403), the polar
+     * rows mirrored through the pole (F:8388-8393, x(i,0) = x(ip(i),2),
+     * x(i,nlat+1) = x(ip(i),nlat-1), ip(i) = mod(i+nlon/2-1,nlon)+1, F:6089-6091) */
+#define IP(i) (((i) + nlon / 2 - 1) % nlon + 1)
+    int64_t bad = 0;
+    for (int64_t p = 0; p < n; ++p) {
+        const int i = idx[p], k = idx[n + p], l = idx[2 * n + p];
+        if (i < 1 || i > nlon || k < 1 || k > nlat - 1 || l < 1 || l > niv) {
+            ++bad;
+            continue;
+        }
+        double xv[4][4][4];   /* [level][lat][lon] offsets -1..2 */
+        for (int c = 0; c < 4; ++c) {
+            int lc = l + c - 1;
+            if (c == 0 && lc < 0) lc = 0;               /* lm1 = max(l-1, 0) */
+            if (c == 3 && lc > niv + 1) lc = niv + 1;   /* lp2 = min(l+2, niv+1) */
+            const int lv = lc < 1 ? 1 : (lc > niv ? niv : lc);
+            for (int b = 0; b < 4; ++b) {
+                const int kk = k + b - 1;
+                for (int a = 0; a < 4; ++a) {
+                    int ii = i + a - 1;
+                    if (ii == 0) ii = nlon;
+                    else if (ii > nlon) ii -= nlon;
+                    int is = ii, ks = kk;
+                    if (kk == 0) { is = IP(ii); ks = 2; }
+                    else if (kk == nlat + 1) { is = IP(ii); ks = nlat - 1; }
+                    xv[c][b][a] = q[(int64_t)(lv - 1) * plane + (int64_t)(ks - 1) * nlon + (is - 1)];
+                }
+            }
+        }
+        /* F:8515-8555: along x; F:8589-8599: along y; F:8603-8606: along z */
+        double z[4];
+        for (int c = 0; c < 4; ++c) {
+            double y[4];
+            for (int b = 0; b < 4; ++b)
+                y[b] = w[p] * xv[c][b][0] + w[n + p] * xv[c][b][1] + w[2 * n + p] * xv[c][b][2] +
+                       w[3 * n + p] * xv[c][b][3];
+            z[c] = w[4 * n + p] * y[0] + w[5 * n + p] * y[1] + w[6 * n + p] * y[2] + w[7 * n + p] * y[3];
+        }
+        double v = w[8 * n + p] * z[0] + w[9 * n + p] * z[1] + w[10 * n + p] * z[2] + w[11 * n + p] * z[3];
+        /* the limiter, F:8560-8585 and F:8611-8613: the range of the 8 nearest values */
+        const double p00mx = fmax(xv[1][1][1], xv[1][2][1]), p00mn = fmin(xv[1][1][1], xv[1][2][1]);
+        const double p10mx = fmax(xv[1][1][2], xv[1][2][2]), p10mn = fmin(xv[1][1][2], xv[1][2][2]);
+        const double p01mx = fmax(xv[2][1][1], xv[2][2][1]), p01mn = fmin(xv[2][1][1], xv[2][2][1]);
+        const double p11mx = fmax(xv[2][1][2], xv[2][2][2]), p11mn = fmin(xv[2][1][2], xv[2][2][2]);
+        const double rmx = fmax(fmax(p00mx, p10mx), fmax(p01mx, p11mx));
+        const double rmn = fmin(fmin(p00mn, p10mn), fmin(p01mn, p11mn));
+        out[p] = fmax(fmin(v, rmx), rmn);
+    }
+#undef IP
+    return bad;
+}
 #endif /* RBO_OPCOUNT */

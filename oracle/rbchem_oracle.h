@@ -78,6 +78,22 @@ void rbo_altitude(int mlon, int mlat, int mniv, double dlon, double dlat, const 
                   const double *h2o, const double *tlon, const double *tlat, const double *tpre,
                   double *alt);
 
+/* Semi-Lagrangian interpolation of the transport on an nlon x nlat x niv grid (the model:
+ * 180 x 91 x 137), arrays lon-fastest as the model's (nlon, nlat, niv).
+ * rbo_transport_weights: departure points (x0 longitude and y0 latitude in radians, z0 and
+ * p0 pressure and surface pressure of the departure column in Pa and hPa) to the cell
+ * indices idx[3][n] (ig0, kg0, lg0, 1-based) and Lagrange weights w[12][n] (alfxm1 alfxp0
+ * alfxp1 alfxp2 alfym1 .. alfzp2), as posx/posy/posz and poidx/poidy/poidz (F:7371-7647)
+ * with the grid of initransp (F:6028-6090); aaa, bbb are the niv hybrid coefficients.
+ * rbo_internew: one tracer field q interpolated at the departure points into out
+ * (internew, F:8339-8618).  Returns the number of points whose indices fall outside the
+ * grid (0 for valid departure points; such points are left untouched). */
+void rbo_transport_weights(int nlon, int nlat, int niv, const double *aaa, const double *bbb,
+                           const double *x0, const double *y0, const double *z0,
+                           const double *p0, int32_t *idx, double *w);
+int64_t rbo_internew(int nlon, int nlat, int niv, const int32_t *idx, const double *w,
+                     const double *q, double *out);
+
 /* ---- pieces exposed for unit tests ---- */
 /* Troe prefactors of const (F:5927-5948), 18 values in COMMON order. */
 void rbo_troe_constants(double out18[18]);

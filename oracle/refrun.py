@@ -227,3 +227,36 @@ def run_ecmwf_reference(forcing_file: str, ian: int, day: float, idx, ntime: int
         return out
     finally:
         shutil.rmtree(run, ignore_errors=True)
+
+
+REF_TRANSPORT_BIN = os.path.join(HERE, "_ref", "ref_transport")
+
+
+def run_transport_reference(tr) -> tuple:
+    """The reference's initransp, posx..poidz and internew (flang build) on the 180 x 91 x
+    137 grid of `tr` (synth.TransportInputs): returns (idx (3, n) int32, w (12, n),
+    q (ntr, 137, 91, 180) after interpolation)."""
+    if not os.path.exists(REF_TRANSPORT_BIN):
+        raise RuntimeError(f"{REF_TRANSPORT_BIN} not built (run `make -C oracle ref`)")
+    assert (tr.nlon, tr.nlat, tr.niv) == (180, 91, 137), "the reference's grid is fixed"
+    n = 180 * 91 * 137
+    ntr = len(tr.jc)
+    base = os.path.join(HERE, "_ref")
+    run = tempfile.mkdtemp(prefix="tr_", dir=base)
+    try:
+        fin, fout = os.path.join(run, "in.bin"), os.path.join(run, "out.bin")
+        with open(fin, "wb") as f:
+            np.array([ntr], dtype=np.int32).tofile(f)
+            np.asarray(tr.jc, dtype=np.int32).tofile(f)
+            for a in (tr.aaa, tr.bbb, tr.x0, tr.y0, tr.z0, tr.p0, tr.q):
+                np.ascontiguousarray(a, dtype=np.float64).tofile(f)
+        r = subprocess.run([REF_TRANSPORT_BIN, fin, fout], cwd=run, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"ref_transport failed ({r.returncode}): {r.stderr[-2000:]}")
+        with open(fout, "rb") as f:
+            idx = np.fromfile(f, dtype=np.int32, count=3 * n).reshape(3, n)
+            w = np.fromfile(f, dtype=np.float64, count=12 * n).reshape(12, n)
+            q = np.fromfile(f, dtype=np.float64, count=ntr * n).reshape(ntr, 137, 91, 180)
+        return idx, w, q
+    finally:
+        shutil.rmtree(run, ignore_errors=True)

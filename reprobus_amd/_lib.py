@@ -20,7 +20,8 @@ ERROR_NAMES = {RB_EINVAL: "RB_EINVAL", RB_EHIP: "RB_EHIP", RB_ENOTABLE: "RB_ENOT
 EXPORTS = ["rb_init", "rb_load_jtable", "rb_set_stream", "rb_chem_step", "rb_conserv_floor",
            "rb_chem_step_dev", "rb_conserv_floor_dev", "rb_chem_converge_dev", "rb_sediment",
            "rb_sediment_dev", "rb_zenith", "rb_zenith_dev", "rb_altitude", "rb_altitude_dev",
-           "rb_run_day_dev", "rb_sync", "rb_get_stats", "rb_last_error", "rb_finalize"]
+           "rb_run_day_dev", "rb_transport_weights_dev", "rb_transport_interp_dev", "rb_transport",
+           "rb_sync", "rb_get_stats", "rb_last_error", "rb_finalize"]
 # and include/rbmlp.h (the learned initial guess)
 MLP_EXPORTS = ["rb_mlp_create", "rb_mlp_destroy", "rb_mlp_set_stream", "rb_mlp_last_error",
                "rb_mlp_param_count", "rb_mlp_set_params", "rb_mlp_get_params", "rb_mlp_set_norm",
@@ -98,6 +99,9 @@ def load() -> C.CDLL:
     L.rb_altitude.argtypes = [P, C.POINTER(Grid), d, d] + [P] * 10
     L.rb_altitude_dev.argtypes = [P, C.POINTER(Grid), d, d] + [P] * 10
     L.rb_run_day_dev.argtypes = [P, C.POINTER(Grid), C.POINTER(DayConfig)] + [P] * 13
+    L.rb_transport_weights_dev.argtypes = [P, C.POINTER(Grid)] + [P] * 8
+    L.rb_transport_interp_dev.argtypes = [P, C.POINTER(Grid), P, P, C.c_int32, P, P]
+    L.rb_transport.argtypes = [P, C.POINTER(Grid)] + [P] * 6 + [C.c_int32, P]
     L.rb_sync.argtypes = [P]
     L.rb_get_stats.argtypes = [P, C.POINTER(Stats)]
     L.rb_last_error.argtypes = [P]

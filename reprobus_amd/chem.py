@@ -163,6 +163,32 @@ class ChemistryStep:
         self._check(self._L.rb_altitude_dev(self._ctx, C.byref(grid), float(dlon), float(dlat),
                                             *[_ptr(a) for a in arrs]))
 
+    # -- semi-Lagrangian transport interpolation -------------------------------------
+    def transport_weights_device(self, grid: Grid, aaa, bbb, x0, y0, z0, p0, idx, w) -> None:
+        """Cell indices idx (3, n) int32 and Lagrange weights w (12, n) of the departure
+        points (device tensors; posx..poidz)."""
+        self._check(self._L.rb_transport_weights_dev(
+            self._ctx, C.byref(grid), *[_ptr(a) for a in (aaa, bbb, x0, y0, z0, p0)],
+            idx.data_ptr(), _ptr(w)))
+
+    def transport_interp_device(self, grid: Grid, idx, w, q_in, q_out) -> None:
+        """The ntr fields of q_in (ntr, niv, nlat, nlon) interpolated at the departure points
+        into q_out (internew for each tracer)."""
+        ntr = int(q_in.shape[0])
+        assert tuple(q_out.shape) == tuple(q_in.shape)
+        self._check(self._L.rb_transport_interp_dev(self._ctx, C.byref(grid), idx.data_ptr(),
+                                                    _ptr(w), ntr, _ptr(q_in), _ptr(q_out)))
+
+    def transport(self, grid: Grid, aaa, bbb, x0, y0, z0, p0, q) -> None:
+        """Host arrays: weights and interpolation of the ntr fields of q (ntr, niv, nlat,
+        nlon), in place."""
+        arrs = (aaa, bbb, x0, y0, z0, p0, q)
+        for a in arrs:
+            assert a.dtype == np.float64 and a.flags.c_contiguous
+        self._check(self._L.rb_transport(self._ctx, C.byref(grid),
+                                         *[a.ctypes.data for a in arrs[:6]], int(q.shape[0]),
+                                         q.ctypes.data))
+
     def run_day_device(self, grid: Grid, nsteps: int, day0: float, dt: float, trajlon, trajlat,
                        T, p, alt, sza, qj1, hc, surfarea, vsed3d, parth2osed, parthno3sed,
                        use_graph: bool = True, converged: bool = False, tol: float = 1e-10,

@@ -71,6 +71,21 @@ constexpr int AA_NX = 57;
 // doubles per box of the spill list between the two phases of the accelerated mode
 constexpr int CONV_SPILL = AA_NX + 2;
 
+// semi-Lagrangian transport interpolation (transport_kernels.hip): the grid constants of
+// initransp and the launches of the departure-point weights and the tracer interpolation
+struct TransportGrid {
+    double ddlon, ddlat, xpih, xcoef[4], ycoef[4];
+    int nlon, nlat, niv;
+};
+TransportGrid transport_grid(int nlon, int nlat, int niv);
+hipError_t launch_transport_weights(const TransportGrid &t, const double *aaa, const double *bbb,
+                                    const double *x0, const double *y0, const double *z0,
+                                    const double *p0, int32_t *idx, double *w,
+                                    unsigned long long *bad, hipStream_t s);
+hipError_t launch_transport_interp(const TransportGrid &t, const int32_t *idx, const double *w,
+                                   const double *q_in, int64_t sin_, double *q_out, int64_t sout,
+                                   int ntr, unsigned long long *bad, hipStream_t s);
+
 Troe troe_constants();
 hipError_t launch_columns(const StepArgs &a, double *col, hipStream_t s);
 hipError_t launch_chem(const StepArgs &a, int block, hipStream_t s);

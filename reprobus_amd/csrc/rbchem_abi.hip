@@ -35,6 +35,10 @@ struct rb_ctx {
     size_t rec_cap = 0;
     double *d_spill = nullptr;               // accelerated mode: boxes between the phases
     size_t spill_cap = 0;
+    double *d_tw = nullptr;                  // transport (host entry): indices, weights,
+    size_t tw_cap = 0;                       // positions, levels
+    double *d_tq = nullptr;                  // transport (host entry): fields in and out
+    size_t tq_cap = 0;
     int n_cu = 0;
     bool capturing = false;                  // inside a graph capture: no event records
     hipGraph_t day_graph = nullptr;          // the captured day and what it was built for
@@ -489,6 +493,95 @@ int rb_altitude(rb_ctx *ctx, const rb_grid *g, double dlon, double dlat, const d
     return RB_OK;
 }
 
+static int check_transport(rb_ctx *ctx, const rb_grid *g)
+{
+    if (!g) return fail(ctx, RB_EINVAL, "grid is NULL");
+    if (g->nlon < 4 || g->nlat < 4 || g->niv < 3)
+        return fail(ctx, RB_EINVAL, "transport needs nlon >= 4, nlat >= 4 and niv >= 3");
+    if ((int64_t)g->nlon * g->nlat * g->niv >= (int64_t)1 << 31)
+        return fail(ctx, RB_EINVAL, "transport grid above 2^31 points");
+    return RB_OK;
+}
+
+// the transport kernels count departure points outside the grid; the count is read back
+static int transport_outside(rb_ctx *ctx, const char *what)
+{
+    unsigned long long bad = 0;
+    HIPCHK(hipMemcpyAsync(&bad, ctx->d_count, sizeof bad, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (bad)
+        return fail(ctx, RB_EINVAL, std::string(what) + ": " + std::to_string(bad) +
+                                        " departure points outside the grid");
+    return RB_OK;
+}
+
+int rb_transport_weights_dev(rb_ctx *ctx, const rb_grid *g, const double *aaa, const double *bbb,
+                             const double *x0, const double *y0, const double *z0,
+                             const double *p0, int32_t *idx, double *w)
+{
+    if (!ctx) return RB_EINVAL;
+    int rc = check_transport(ctx, g);
+    if (rc) return rc;
+    if (any_null({aaa, bbb, x0, y0, z0, p0, w}) || !idx)
+        return fail(ctx, RB_EINVAL, "a device array is NULL");
+    HIPCHK(hipSetDevice(ctx->device));
+    const rb::TransportGrid t = rb::transport_grid(g->nlon, g->nlat, g->niv);
+    HIPCHK(rb::launch_zero_counter(ctx->d_count, ctx->stream));
+    HIPCHK(rb::launch_transport_weights(t, aaa, bbb, x0, y0, z0, p0, idx, w, ctx->d_count,
+                                        ctx->stream));
+    return transport_outside(ctx, "rb_transport_weights");
+}
+
+int rb_transport_interp_dev(rb_ctx *ctx, const rb_grid *g, const int32_t *idx, const double *w,
+                            int32_t ntr, const double *q_in, double *q_out)
+{
+    if (!ctx) return RB_EINVAL;
+    int rc = check_transport(ctx, g);
+    if (rc) return rc;
+    if (ntr < 1) return fail(ctx, RB_EINVAL, "ntr must be >= 1");
+    if (any_null({w, q_in, q_out}) || !idx) return fail(ctx, RB_EINVAL, "a device array is NULL");
+    const int64_t field = (int64_t)g->nlon * g->nlat * g->niv;
+    if (q_out < q_in + ntr * field && q_in < q_out + ntr * field)
+        return fail(ctx, RB_EINVAL, "q_out must not overlap q_in");
+    HIPCHK(hipSetDevice(ctx->device));
+    const rb::TransportGrid t = rb::transport_grid(g->nlon, g->nlat, g->niv);
+    HIPCHK(rb::launch_zero_counter(ctx->d_count, ctx->stream));
+    HIPCHK(rb::launch_transport_interp(t, idx, w, q_in, field, q_out, field, ntr, ctx->d_count,
+                                       ctx->stream));
+    return transport_outside(ctx, "rb_transport_interp");
+}
+
+int rb_transport(rb_ctx *ctx, const rb_grid *g, const double *aaa, const double *bbb,
+                 const double *x0, const double *y0, const double *z0, const double *p0,
+                 int32_t ntr, double *qj1)
+{
+    if (!ctx) return RB_EINVAL;
+    int rc = check_transport(ctx, g);
+    if (rc) return rc;
+    if (ntr < 1) return fail(ctx, RB_EINVAL, "ntr must be >= 1");
+    if (any_null({aaa, bbb, x0, y0, z0, p0, qj1})) return fail(ctx, RB_EINVAL, "a host array is NULL");
+    HIPCHK(hipSetDevice(ctx->device));
+    const size_t n = (size_t)g->nlon * g->nlat * g->niv, B = sizeof(double);
+    // indices (3 n int32), weights 12 n, positions 4 n, levels 2 niv
+    if ((rc = ensure(ctx, &ctx->d_tw, &ctx->tw_cap, (3 * n + 1) / 2 + 16 * n + 2 * g->niv))) return rc;
+    if ((rc = ensure(ctx, &ctx->d_tq, &ctx->tq_cap, 2 * (size_t)ntr * n))) return rc;
+    int32_t *didx = (int32_t *)ctx->d_tw;
+    double *dw = ctx->d_tw + (3 * n + 1) / 2, *dx = dw + 12 * n, *dy = dx + n, *dz = dy + n,
+           *dp = dz + n, *da = dp + n, *db = da + g->niv;
+    double *qin = ctx->d_tq, *qout = ctx->d_tq + (size_t)ntr * n;
+    hipStream_t s = ctx->stream;
+    const std::pair<double *, const double *> in[] = {{dx, x0}, {dy, y0}, {dz, z0}, {dp, p0}};
+    for (auto &io : in) HIPCHK(hipMemcpyAsync(io.first, io.second, n * B, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(da, aaa, g->niv * B, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(db, bbb, g->niv * B, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(qin, qj1, (size_t)ntr * n * B, hipMemcpyHostToDevice, s));
+    if ((rc = rb_transport_weights_dev(ctx, g, da, db, dx, dy, dz, dp, didx, dw))) return rc;
+    if ((rc = rb_transport_interp_dev(ctx, g, didx, dw, ntr, qin, qout))) return rc;
+    HIPCHK(hipMemcpyAsync(qj1, qout, (size_t)ntr * n * B, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return RB_OK;
+}
+
 static int check_sediment(rb_ctx *ctx, const rb_grid *g, double dt, int32_t nivh2o)
 {
     int rc = check_grid(ctx, g);
@@ -675,6 +768,8 @@ int rb_finalize(rb_ctx *ctx)
     if (ctx->d_aa) (void)hipFree(ctx->d_aa);
     if (ctx->d_rec) (void)hipFree(ctx->d_rec);
     if (ctx->d_spill) (void)hipFree(ctx->d_spill);
+    if (ctx->d_tw) (void)hipFree(ctx->d_tw);
+    if (ctx->d_tq) (void)hipFree(ctx->d_tq);
     if (ctx->day_exec) (void)hipGraphExecDestroy(ctx->day_exec);
     if (ctx->day_graph) (void)hipGraphDestroy(ctx->day_graph);
     for (hipEvent_t e : {ctx->ev0, ctx->ev1, ctx->ev_h2d, ctx->ev_d2h})

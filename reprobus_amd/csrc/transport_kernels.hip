@@ -1,0 +1,210 @@
+// Semi-Lagrangian transport interpolation on an nlon x nlat x niv grid (the model:
+// 180 x 91 x 137): the cell indices and Lagrange weights of the departure points
+// (posx/posy/posz, poidx/poidy/poidz, F:7371-7647, as backtra calls them at F:6444-6451)
+// and the tricubic interpolation with its limiter (internew, F:8339-8618) of every tracer.
+//
+// One lane per grid point.  The weights kernel reproduces the reference's arithmetic
+// operation for operation (FP64, built with -ffp-contract=off).  The interpolation kernel
+// reads a point's 3 indices and 12 weights once and loops over the tracers: per tracer 64
+// gathers (the 4 x 4 x 4 stencil, neighbouring lanes share most of it through the caches)
+// and one coalesced store.  The reference's extended grid x(0:nlon+2, 0:nlat+1, 0:niv+1)
+// (F:8374-8413) is not built: each stencil index is mapped to the field instead.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "chem_device.h"
+
+namespace rb {
+
+// Cell of departure point p and its 12 weights (field-major: w[m * n + p], m = alfxm1
+// alfxp0 alfxp1 alfxp2 alfym1 .. alfzp2; idx[m * n + p], m = ig0 kg0 lg0, 1-based).
+__global__ void k_transport_weights(TransportGrid t, const double *__restrict__ aaa,
+                                    const double *__restrict__ bbb, const double *__restrict__ x0,
+                                    const double *__restrict__ y0, const double *__restrict__ z0,
+                                    const double *__restrict__ p0, int64_t n,
+                                    int32_t *__restrict__ idx, double *__restrict__ w,
+                                    unsigned long long *__restrict__ bad)
+{
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    // posx F:7402-7406, posy F:7442-7446
+    const double xx = x0[p], yy = y0[p];
+    const int ix = (int)(xx / t.ddlon);
+    const double dx = xx - (double)ix * t.ddlon;
+    const int iy = (int)((t.xpih - yy) / t.ddlat);
+    const double dy = t.xpih - (double)iy * t.ddlat - yy;
+    // posz F:7485-7494: the last level l in 2..niv-1 whose pressure is <= zz
+    const double zz = z0[p], pp = p0[p];
+    int ll = 1;
+    for (int l = 2; l <= t.niv - 1; ++l) {
+        const double pres = (aaa[l - 1] + bbb[l - 1] * pp) * 100.;
+        if (zz - pres >= 0.) ll = l;
+    }
+    idx[p] = ix + 1;
+    idx[n + p] = iy + 1;
+    idx[2 * n + p] = ll;
+    if (!(ix + 1 >= 1 && ix + 1 <= t.nlon && iy + 1 >= 1 && iy + 1 <= t.nlat - 1)) atomicAdd(bad, 1ull);
+    // poidx F:7529-7536
+    {
+        const double dxm1 = dx + t.ddlon, dxp0 = dx, dxp1 = dx - t.ddlon, dxp2 = dx - t.ddlon * 2.;
+        w[p] = t.xcoef[0] * dxp0 * dxp1 * dxp2;
+        w[n + p] = t.xcoef[1] * dxm1 * dxp1 * dxp2;
+        w[2 * n + p] = t.xcoef[2] * dxm1 * dxp0 * dxp2;
+        w[3 * n + p] = t.xcoef[3] * dxm1 * dxp0 * dxp1;
+    }
+    // poidy F:7571-7578
+    {
+        const double dym1 = dy + t.ddlat, dyp0 = dy, dyp1 = dy - t.ddlat, dyp2 = dy - t.ddlat * 2.;
+        w[4 * n + p] = t.ycoef[0] * dyp0 * dyp1 * dyp2;
+        w[5 * n + p] = t.ycoef[1] * dym1 * dyp1 * dyp2;
+        w[6 * n + p] = t.ycoef[2] * dym1 * dyp0 * dyp2;
+        w[7 * n + p] = t.ycoef[3] * dym1 * dyp0 * dyp1;
+    }
+    // poidz F:7614-7644
+    {
+        const double prem1 = ll > 1 ? (aaa[ll - 2] + bbb[ll - 2] * pp) * 100. : 5.e-3 * 100.;
+        const double prep0 = (aaa[ll - 1] + bbb[ll - 1] * pp) * 100.;
+        const double prep1 = (aaa[ll] + bbb[ll] * pp) * 100.;
+        const double prep2 = ll < t.niv - 1 ? (aaa[ll + 1] + bbb[ll + 1] * pp) * 100. : 1200. * 100.;
+        const double dzm1 = zz - prem1, dzp0 = zz - prep0, dzp1 = zz - prep1, dzp2 = zz - prep2;
+        const double dzp2m1 = prep2 - prem1, dzp2p0 = prep2 - prep0, dzp2p1 = prep2 - prep1;
+        const double dzp1m1 = prep1 - prem1, dzp1p0 = prep1 - prep0, dzp0m1 = prep0 - prem1;
+        w[8 * n + p] = -(dzp0 * dzp1 * dzp2) / (dzp0m1 * dzp1m1 * dzp2m1);
+        w[9 * n + p] = (dzm1 * dzp1 * dzp2) / (dzp0m1 * dzp1p0 * dzp2p0);
+        w[10 * n + p] = -(dzm1 * dzp0 * dzp2) / (dzp1m1 * dzp1p0 * dzp2p1);
+        w[11 * n + p] = (dzm1 * dzp0 * dzp1) / (dzp2m1 * dzp2p0 * dzp2p1);
+    }
+}
+
+// ntr fields (q_in + t * sin, t < ntr) interpolated at the departure points into
+// q_out + t * sout.  A point whose indices fall outside the grid is counted in *bad and
+// not written.
+__global__ void __launch_bounds__(256) k_transport_interp(TransportGrid t, const int32_t *__restrict__ idx,
+                                                          const double *__restrict__ w,
+                                                          const double *__restrict__ q_in, int64_t sin_,
+                                                          double *__restrict__ q_out, int64_t sout,
+                                                          int ntr, int64_t n,
+                                                          unsigned long long *__restrict__ bad)
+{
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const int i = idx[p], k = idx[n + p], l = idx[2 * n + p];
+    if (!(i >= 1 && i <= t.nlon && k >= 1 && k <= t.nlat - 1 && l >= 1 && l <= t.niv)) {
+        atomicAdd(bad, 1ull);
+        return;
+    }
+    double wx[4], wy[4], wz[4];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        wx[m] = w[m * n + p];
+        wy[m] = w[(4 + m) * n + p];
+        wz[m] = w[(8 + m) * n + p];
+    }
+    // the extended grid as a map (F:8374-8413): longitudes periodic, the polar rows
+    // mirrored through the pole, x(i, 0) = x(ip(i), 2) and x(i, nlat+1) = x(ip(i), nlat-1)
+    // with ip(i) = mod(i + nlon/2 - 1, nlon) + 1 (F:6089-6091), levels clamped to 1..niv
+    int32_t po[4][4];   // [lat offset][lon offset] -> offset in a level plane
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+        const int kk = k + b - 1;
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            int ii = i + a - 1;
+            if (ii == 0) ii = t.nlon;
+            else if (ii > t.nlon) ii -= t.nlon;
+            int is = ii, ks = kk;
+            if (kk == 0 || kk == t.nlat + 1) {
+                is = (ii + t.nlon / 2 - 1) % t.nlon + 1;
+                ks = kk == 0 ? 2 : t.nlat - 1;
+            }
+            po[b][a] = (ks - 1) * t.nlon + (is - 1);
+        }
+    }
+    int32_t lo[4];
+    const int32_t plane = t.nlon * t.nlat;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const int lc = l + c - 1;
+        lo[c] = ((lc < 1 ? 1 : (lc > t.niv ? t.niv : lc)) - 1) * plane;
+    }
+    for (int tr = 0; tr < ntr; ++tr) {
+        const double *q = q_in + (int64_t)tr * sin_;
+        double z[4], in[2][2][2];   // in: [level lp0/lp1][lat kp0/kp1][lon ip0/ip1]
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            double xv[4][4];
+#pragma unroll
+            for (int b = 0; b < 4; ++b)
+#pragma unroll
+                for (int a = 0; a < 4; ++a) xv[b][a] = q[lo[c] + po[b][a]];
+            if (c == 1 || c == 2)
+#pragma unroll
+                for (int b = 0; b < 2; ++b)
+#pragma unroll
+                    for (int a = 0; a < 2; ++a) in[c - 1][b][a] = xv[b + 1][a + 1];
+            // F:8515-8555 along x, F:8589-8599 along y
+            double y[4];
+#pragma unroll
+            for (int b = 0; b < 4; ++b)
+                y[b] = wx[0] * xv[b][0] + wx[1] * xv[b][1] + wx[2] * xv[b][2] + wx[3] * xv[b][3];
+            z[c] = wy[0] * y[0] + wy[1] * y[1] + wy[2] * y[2] + wy[3] * y[3];
+        }
+        // F:8603-8606 along z; the limiter (F:8560-8585, F:8611-8613): the range of the 8
+        // nearest values
+        const double v = wz[0] * z[0] + wz[1] * z[1] + wz[2] * z[2] + wz[3] * z[3];
+        const double rmx = fmax(fmax(fmax(in[0][0][0], in[0][1][0]), fmax(in[0][0][1], in[0][1][1])),
+                                fmax(fmax(in[1][0][0], in[1][1][0]), fmax(in[1][0][1], in[1][1][1])));
+        const double rmn = fmin(fmin(fmin(in[0][0][0], in[0][1][0]), fmin(in[0][0][1], in[0][1][1])),
+                                fmin(fmin(in[1][0][0], in[1][1][0]), fmin(in[1][0][1], in[1][1][1])));
+        q_out[(int64_t)tr * sout + p] = fmax(fmin(v, rmx), rmn);
+    }
+}
+
+TransportGrid transport_grid(int nlon, int nlat, int niv)
+{
+    // initransp F:6028-6034, 6041, 6051, 6075-6085 (host libm, as the model's)
+    TransportGrid t{};
+    const double xpi = 2. * asin(1.);
+    const double xpi2 = 2. * xpi;
+    t.xpih = xpi / 2.;
+    t.ddlat = xpi / (double)(nlat - 1);
+    t.ddlon = xpi2 / (double)nlon;
+    const double coefx = 1. / (t.ddlon * t.ddlon * t.ddlon);
+    t.xcoef[0] = -coefx / 6.;
+    t.xcoef[1] = coefx / 2.;
+    t.xcoef[2] = -coefx / 2.;
+    t.xcoef[3] = coefx / 6.;
+    const double coefy = 1. / (t.ddlat * t.ddlat * t.ddlat);
+    t.ycoef[0] = -coefy / 6.;
+    t.ycoef[1] = coefy / 2.;
+    t.ycoef[2] = -coefy / 2.;
+    t.ycoef[3] = coefy / 6.;
+    t.nlon = nlon;
+    t.nlat = nlat;
+    t.niv = niv;
+    return t;
+}
+
+hipError_t launch_transport_weights(const TransportGrid &t, const double *aaa, const double *bbb,
+                                    const double *x0, const double *y0, const double *z0,
+                                    const double *p0, int32_t *idx, double *w,
+                                    unsigned long long *bad, hipStream_t s)
+{
+    const int64_t n = (int64_t)t.nlon * t.nlat * t.niv;
+    hipLaunchKernelGGL(k_transport_weights, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, t,
+                       aaa, bbb, x0, y0, z0, p0, n, idx, w, bad);
+    return hipGetLastError();
+}
+
+hipError_t launch_transport_interp(const TransportGrid &t, const int32_t *idx, const double *w,
+                                   const double *q_in, int64_t sin_, double *q_out, int64_t sout,
+                                   int ntr, unsigned long long *bad, hipStream_t s)
+{
+    const int64_t n = (int64_t)t.nlon * t.nlat * t.niv;
+    hipLaunchKernelGGL(k_transport_interp, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, t,
+                       idx, w, q_in, sin_, q_out, sout, ntr, n, bad);
+    return hipGetLastError();
+}
+
+}  // namespace rb

@@ -314,3 +314,62 @@ def make_c2(seed: int = 0, nlon: int = 144, nlat: int = 91, decl_deg: float = 0.
     inp.sza = c2_sza(inp.meta, 0)
     inp.meta.update({"name": "c2", "vortex": vortex})
     return inp
+
+
+# ------------------------------------------------------------------ transport inputs
+@dataclass
+class TransportInputs:
+    """Departure points and tracer fields of one semi-Lagrangian step on an nlon x nlat x niv
+    grid (arrays numpy order (niv, nlat, nlon) = the model's (nlon, nlat, niv) memory)."""
+    nlon: int
+    nlat: int
+    niv: int
+    aaa: np.ndarray      # (niv,) hybrid coefficients, p = aaa + bbb psol (hPa)
+    bbb: np.ndarray
+    x0: np.ndarray       # departure longitude (rad, [0, 2 pi))
+    y0: np.ndarray       # departure latitude (rad, inside (-pi/2, pi/2))
+    z0: np.ndarray       # departure pressure (Pa)
+    p0: np.ndarray       # surface pressure of the departure column (hPa)
+    q: np.ndarray        # (ntr, niv, nlat, nlon) tracer mixing ratios
+    jc: np.ndarray       # (ntr,) 1-based tracer slots of the fields
+
+
+def transport_levels(niv: int) -> tuple:
+    """Hybrid coefficients (aaa hPa, bbb) of niv levels, top first: p rises monotonically
+    from ~0.01 hPa to the surface for any surface pressure above 500 hPa."""
+    eta = (np.arange(1, niv + 1) / (niv + 0.5)) ** 1.5
+    return 0.01 + 200.0 * eta * (1 - eta) ** 2, eta ** 3
+
+
+def make_transport(seed: int = 0, nlon: int = 180, nlat: int = 91, niv: int = 137,
+                   jc=(1, 5, 20), dt: float = 3600.0) -> TransportInputs:
+    """Departure points of a 1 h step under a smooth synthetic wind (zonal jet with waves,
+    meridional and vertical wobble, a little noise), kept strictly inside the grid as the
+    model's trajectory code keeps them; tracers are positive smooth fields with noise."""
+    rng = np.random.default_rng(seed)
+    xpi = 2.0 * np.arcsin(1.0)
+    ddlon, ddlat = 2 * xpi / nlon, xpi / (nlat - 1)
+    lon = ddlon * np.arange(nlon)
+    lat = xpi / 2 - ddlat * np.arange(nlat)
+    aaa, bbb = transport_levels(niv)
+    L, LA, LO = np.meshgrid(np.arange(niv), lat, lon, indexing="ij")
+    psol = 1000.0 + 30.0 * np.cos(2 * LA[0]) * np.sin(LO[0]) + rng.normal(0, 2.0, LO[0].shape)
+    pres = (aaa[:, None, None] + bbb[:, None, None] * psol[None]) * 100.0
+    R = 6.371e6
+    u = (30.0 + 20.0 * np.cos(3 * LO)) * np.cos(LA) ** 2 * (1 + 0.5 * np.sin(L / 9.0)) \
+        + rng.normal(0, 2.0, LO.shape)
+    v = 8.0 * np.sin(2 * LA) * np.cos(2 * LO) + rng.normal(0, 1.0, LO.shape)
+    w = 0.02 * pres / 1e4 * np.sin(LO + LA) + rng.normal(0, 0.01, LO.shape)
+    x0 = np.mod(LO - u * dt / (R * np.maximum(np.cos(LA), 0.05)), 2 * xpi)
+    x0 = np.where(x0 >= 2 * xpi, x0 - 2 * xpi, x0)
+    y0 = np.clip(LA - v * dt / R, -xpi / 2 + 1e-7, xpi / 2 - 1e-7)
+    z0 = np.maximum(pres - w * dt, 1.0)
+    p0 = psol[None].repeat(niv, 0) + rng.normal(0, 1.0, LO.shape)
+    q = np.empty((len(jc), niv, nlat, nlon))
+    for t in range(len(jc)):
+        a, b = rng.uniform(-3, 3), rng.uniform(0.5, 2.0)
+        q[t] = np.exp(-18.0 + a * np.sin(LO + t) * np.cos(LA) + b * np.log(pres / 1e4 + 1.0)) \
+            * (1.0 + 0.05 * rng.standard_normal(LO.shape))
+    c = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+    return TransportInputs(nlon, nlat, niv, c(aaa), c(bbb), c(x0), c(y0), c(z0), c(p0), c(q),
+                           np.asarray(jc, dtype=np.int32))

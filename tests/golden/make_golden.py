@@ -5,9 +5,10 @@ on small synthetic grids.
 Run from the repository root where /root/reference is present:
     make -C oracle ref && python tests/golden/make_golden.py
 
-Each fixture stores the step inputs (with the SZA the reference realised,
+Each chemistry fixture stores the step inputs (with the SZA the reference realised,
 ``sza3d``) and the reference outputs after chem3d and after conserv + floor; the
-sedimentation fixture also holds qj1 after the reference's sedimentation.
+sedimentation fixture also holds qj1 after the reference's sedimentation.  The input
+producers, ECMWF reader and transport interpolation have fixtures of their own.
 """
 from __future__ import annotations
 
@@ -137,6 +138,29 @@ def main():
     np.savez_compressed(os.path.join(OUT, "inputs_seed11.npz"), sza=sza, alt=alt, **d)
     print("wrote inputs_seed11", sza.size, "SZA values,", alt.size, "altitudes")
     ecmwf_fixture()
+    transport_fixture()
+
+
+def transport_fixture(seed: int = 0, nsamp: int = 4000):
+    """The reference's transport weights and tricubic interpolation (initransp, posx..poidz,
+    internew; oracle/ref_transport.f90) on synth.make_transport(seed), 180 x 91 x 137 x 3
+    tracers.  The full arrays are hundreds of MB: the fixture keeps the values at sampled
+    points (random ones plus points whose stencil crosses a pole or the date line), the sums
+    of every output array, and sums of the inputs (the generator must still give them)."""
+    tr = synth.make_transport(seed)
+    idx, w, q = refrun.run_transport_reference(tr)
+    n = idx.shape[1]
+    rng = np.random.default_rng(seed)
+    polar = np.flatnonzero((idx[1] == 1) | (idx[1] == 90))
+    wrap = np.flatnonzero((idx[0] == 1) | (idx[0] >= 179))
+    pts = np.unique(np.concatenate([rng.integers(0, n, nsamp), rng.choice(polar, 1000),
+                                    rng.choice(wrap, 1000)]))
+    qf = q.reshape(q.shape[0], -1)
+    np.savez_compressed(os.path.join(OUT, f"transport_seed{seed}.npz"), seed=seed, jc=tr.jc,
+                        pts=pts, idx=idx[:, pts], w=w[:, pts], q=qf[:, pts],
+                        q_sum=qf.sum(axis=1), w_sum=w.sum(axis=1), idx_sum=idx.astype(np.int64).sum(axis=1),
+                        in_sums=np.array([tr.x0.sum(), tr.y0.sum(), tr.z0.sum(), tr.p0.sum(), tr.q.sum()]))
+    print(f"wrote transport_seed{seed}: {pts.size} sampled points of {n}")
 
 
 def ecmwf_fixture(seed: int = 13, nsamp: int = 3000):
@@ -160,4 +184,7 @@ def ecmwf_fixture(seed: int = 13, nsamp: int = 3000):
 
 
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["transport"]:
+        transport_fixture()
+    else:
+        main()
