@@ -6,7 +6,9 @@ fused -- state resident in HBM; successive steps integrate the evolving state.
 
 Beside it on the same line: the converged mode (sweeps to the fixed point; plain and
 Anderson-accelerated, each from the incoming state and from the learned initial guess)
-on the 786,240-box c2 grid (polar vortex), with agreement and sweep histograms; a 96-step day of
+on the 786,240-box c2 grid (polar vortex), with agreement and sweep histograms; the
+semi-Lagrangian transport interpolation of 43 tracers on the model's 180 x 91 x 137 grid
+against the HBM roofline; a 96-step day of
 (device zenith + step) as one HIP graph against the same steps launched one by one;
 the autotuned launch shapes; the roofline of the per-box kernel; and the reference
 Fortran timed on the host cores.  Prints one JSON line.
@@ -250,6 +252,48 @@ def day_section(table, block, dev, nsteps=96):
             "box_steps_per_s_graph": g.nbox * nsteps / (res["graph"] / 1e3)}
 
 
+def transport_section(table, dev, ntr=43, reps=5):
+    """Semi-Lagrangian transport interpolation on the model's 180 x 91 x 137 grid
+    (2,244,060 points): departure-point cells and weights, then all ntr tracers (the
+    model's 43) interpolated out of place; inputs resident in HBM.  Fields: 3 synthetic
+    tracers, scaled into 43 (the interpolation's cost does not depend on the values)."""
+    import torch
+
+    from reprobus_amd import synth
+    from reprobus_amd.chem import ChemistryStep, make_grid
+
+    tr = synth.make_transport(0)
+    grid = make_grid(tr.nlon, tr.nlat, tr.niv, tr.niv, 1)
+    n = tr.nlon * tr.nlat * tr.niv
+    step = ChemistryStep(*table, device=0)
+    stream = torch.cuda.current_stream()
+    step.set_stream(stream.cuda_stream)
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).to(dev)
+    pos = [t(a) for a in (tr.aaa, tr.bbb, tr.x0, tr.y0, tr.z0, tr.p0)]
+    idx = torch.zeros((3, n), dtype=torch.int32, device=dev)
+    w = torch.zeros((12, n), dtype=torch.float64, device=dev)
+    q3 = t(tr.q)
+    q_in = torch.stack([q3[k % 3] * (1.0 + 1e-3 * k) for k in range(ntr)])
+    q_out = torch.empty_like(q_in)
+    w_ms = _timed(lambda: step.transport_weights_device(grid, *pos, idx, w), reps, stream)
+    i_ms = _timed(lambda: step.transport_interp_device(grid, idx, w, q_in, q_out), reps, stream)
+    step.close()
+    b_interp = n * (3 * 4 + 12 * 8 + ntr * 2 * 8)
+    b_weights = n * (4 * 8 + 3 * 4 + 12 * 8)
+    gbs = lambda b, ms: b / (ms / 1e3) / 1e9
+    return {"workload": f"180 x 91 x 137 grid ({n:,} points), {ntr} tracers, departure points of "
+                        "a synthetic 1 h wind; inputs resident in HBM",
+            "weights_ms": w_ms, "interp_ms": i_ms, "ms": w_ms + i_ms,
+            "point_tracers_per_s": n * ntr / ((w_ms + i_ms) / 1e3),
+            "roofline": {"bound": "hbm", "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "interp_achieved": gbs(b_interp, i_ms),
+                         "interp_frac": gbs(b_interp, i_ms) / HBM_PEAK_GBS,
+                         "interp_bytes": b_interp,
+                         "weights_achieved": gbs(b_weights, w_ms),
+                         "weights_frac": gbs(b_weights, w_ms) / HBM_PEAK_GBS,
+                         "weights_bytes": b_weights}}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -341,6 +385,10 @@ def main():
             extras["day"] = day_section(table, block, dev)
         except Exception as exc:
             extras["day"] = {"error": repr(exc)}
+        try:
+            extras["transport"] = transport_section(table, dev)
+        except Exception as exc:
+            extras["transport"] = {"error": repr(exc)}
 
     cpu = None
     if not args.no_cpu_baseline:

@@ -84,16 +84,19 @@ __global__ void __launch_bounds__(256) k_transport_interp(TransportGrid t, const
                                                           const double *__restrict__ w,
                                                           const double *__restrict__ q_in, int64_t sin_,
                                                           double *__restrict__ q_out, int64_t sout,
-                                                          int ntr, int64_t n,
+                                                          int ntr, int64_t n, int group,
                                                           unsigned long long *__restrict__ bad)
 {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
     const int i = idx[p], k = idx[n + p], l = idx[2 * n + p];
     if (!(i >= 1 && i <= t.nlon && k >= 1 && k <= t.nlat - 1 && l >= 1 && l <= t.niv)) {
-        atomicAdd(bad, 1ull);
+        if (blockIdx.y == 0) atomicAdd(bad, 1ull);
         return;
     }
+    // this block's tracers: [blockIdx.y * group, min(ntr, (blockIdx.y + 1) * group))
+    const int tr0 = (int)blockIdx.y * group;
+    const int tr1 = tr0 + group < ntr ? tr0 + group : ntr;
     double wx[4], wy[4], wz[4];
 #pragma unroll
     for (int m = 0; m < 4; ++m) {
@@ -128,7 +131,7 @@ __global__ void __launch_bounds__(256) k_transport_interp(TransportGrid t, const
         const int lc = l + c - 1;
         lo[c] = ((lc < 1 ? 1 : (lc > t.niv ? t.niv : lc)) - 1) * plane;
     }
-    for (int tr = 0; tr < ntr; ++tr) {
+    for (int tr = tr0; tr < tr1; ++tr) {
         const double *q = q_in + (int64_t)tr * sin_;
         double z[4], in[2][2][2];   // in: [level lp0/lp1][lat kp0/kp1][lon ip0/ip1]
 #pragma unroll
@@ -199,11 +202,13 @@ hipError_t launch_transport_weights(const TransportGrid &t, const double *aaa, c
 
 hipError_t launch_transport_interp(const TransportGrid &t, const int32_t *idx, const double *w,
                                    const double *q_in, int64_t sin_, double *q_out, int64_t sout,
-                                   int ntr, unsigned long long *bad, hipStream_t s)
+                                   int ntr, unsigned long long *bad, hipStream_t s, int group)
 {
     const int64_t n = (int64_t)t.nlon * t.nlat * t.niv;
-    hipLaunchKernelGGL(k_transport_interp, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, t,
-                       idx, w, q_in, sin_, q_out, sout, ntr, n, bad);
+    if (group < 1) group = TRANSPORT_GROUP;
+    const unsigned ng = (unsigned)((ntr + group - 1) / group);
+    hipLaunchKernelGGL(k_transport_interp, dim3((unsigned)((n + 255) / 256), ng), dim3(256), 0, s, t,
+                       idx, w, q_in, sin_, q_out, sout, ntr, n, group, bad);
     return hipGetLastError();
 }
 

@@ -344,8 +344,10 @@ def transport_levels(niv: int) -> tuple:
 def make_transport(seed: int = 0, nlon: int = 180, nlat: int = 91, niv: int = 137,
                    jc=(1, 5, 20), dt: float = 3600.0) -> TransportInputs:
     """Departure points of a 1 h step under a smooth synthetic wind (zonal jet with waves,
-    meridional and vertical wobble, a little noise), kept strictly inside the grid as the
-    model's trajectory code keeps them; tracers are positive smooth fields with noise."""
+    meridional wobble, a little noise; vertical velocity in pressure coordinates proportional
+    to pressure, ~0.7 % of p per hour with 10 % noise, as stratospheric motion scales), kept
+    strictly inside the grid as the model's trajectory code keeps them; tracers are positive
+    smooth fields with noise."""
     rng = np.random.default_rng(seed)
     xpi = 2.0 * np.arcsin(1.0)
     ddlon, ddlat = 2 * xpi / nlon, xpi / (nlat - 1)
@@ -359,7 +361,7 @@ def make_transport(seed: int = 0, nlon: int = 180, nlat: int = 91, niv: int = 13
     u = (30.0 + 20.0 * np.cos(3 * LO)) * np.cos(LA) ** 2 * (1 + 0.5 * np.sin(L / 9.0)) \
         + rng.normal(0, 2.0, LO.shape)
     v = 8.0 * np.sin(2 * LA) * np.cos(2 * LO) + rng.normal(0, 1.0, LO.shape)
-    w = 0.02 * pres / 1e4 * np.sin(LO + LA) + rng.normal(0, 0.01, LO.shape)
+    w = pres * 2e-6 * (np.sin(LO + LA) + rng.normal(0, 0.1, LO.shape))
     x0 = np.mod(LO - u * dt / (R * np.maximum(np.cos(LA), 0.05)), 2 * xpi)
     x0 = np.where(x0 >= 2 * xpi, x0 - 2 * xpi, x0)
     y0 = np.clip(LA - v * dt / R, -xpi / 2 + 1e-7, xpi / 2 - 1e-7)
