@@ -7,6 +7,8 @@ fused -- state resident in HBM; successive steps integrate the evolving state.
 Beside it on the same line: the converged mode (sweeps to the fixed point; plain and
 Anderson-accelerated, each from the incoming state and from the learned initial guess)
 on the 786,240-box c2 grid (polar vortex), with agreement and sweep histograms; the
+surrogate step (the learned network's answer kept where its FP64 residual is under 1e-8,
+the FP64 solve elsewhere) over a day of sub-steps beside the FP64 solve; the
 semi-Lagrangian transport interpolation of 43 tracers on the model's 180 x 91 x 137 grid
 against the HBM roofline; a 96-step day of
 (device zenith + step) as one HIP graph against the same steps launched one by one;
@@ -14,6 +16,7 @@ the autotuned launch shapes; the roofline of the per-box kernel; and the referen
 Fortran timed on the host cores.  Prints one JSON line.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--no-extras]
+                  [--extras converged,surrogate,day,transport] [--surrogate-steps 96]
 """
 from __future__ import annotations
 
@@ -59,6 +62,10 @@ def parse():
     ap.add_argument("--cpu-nrep", type=int, default=8)
     ap.add_argument("--no-extras", action="store_true",
                     help="only the headline step (for profiler runs)")
+    ap.add_argument("--extras", default="converged,surrogate,day,transport",
+                    help="comma-separated sections reported beside the headline")
+    ap.add_argument("--surrogate-steps", type=int, default=96,
+                    help="sub-steps of the surrogate run (96 = one day, 960 = ten days)")
     ap.add_argument("--block", type=int, default=0, help="0: autotuned")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="after the timed region, one device copy of this many MB (a kernel of "
@@ -252,6 +259,144 @@ def day_section(table, block, dev, nsteps=96):
             "box_steps_per_s_graph": g.nbox * nsteps / (res["graph"] / 1e3)}
 
 
+def surrogate_section(table, block_conv, refill, dev, nsteps=96, accept_tol=1e-8, tol=1e-11,
+                      maxiter=2000):
+    """Surrogate step with fallback on the c2 grid (seed 0, vortex and liquid PSC), nsteps
+    sub-steps of 15 min with diurnal SZA (96 = one day).  Two states advance side by side
+    from the same start: the surrogate path (the learned network's answer for the step,
+    kept where one FP64 sweep from it changes no species by more than accept_tol, the
+    plain sweeps to tol from it elsewhere) and the FP64 path (plain sweeps to tol from the
+    incoming state); conserv + floor after each step on both.  Reported: accept rate by
+    latitude band and by day/night, the network's one-step error per species against the
+    FP64 step, the end-of-run difference between the two states, the drift of total
+    chlorine and bromine (Cly + its source gases, Bry + its source gases: what the FP64
+    solve conserves at its fixed point), and box-steps/s of both paths."""
+    import torch
+
+    from reprobus_amd import guess, synth
+    from reprobus_amd.chem import ChemistryStep, make_grid
+    from reprobus_amd.mlp import GuessMLP
+
+    inp = synth.make_c2(0)
+    g = inp.grid
+    nb = g.nivbas
+    grid = make_grid(g.nlon, g.nlat, g.niv, g.nivbas, g.nivhau)
+    net = GuessMLP(0)
+    if not guess.load(net):
+        return {"error": "no trained guess weights (reprobus_amd/data/guess_mlp.npz)"}
+    sp = _device_state(inp, dev)                    # surrogate path
+    fp = _device_state(inp, dev)                    # FP64 path
+    step = ChemistryStep(*table, device=0, block=block_conv, conv_refill=refill)
+    stream = torch.cuda.current_stream()
+    step.set_stream(stream.cuda_stream)
+    net.set_stream(stream.cuda_stream)
+    shb = (nb, g.nlat, g.nlon)
+    it_s = torch.zeros(shb, dtype=torch.int32, device=dev)
+    it_f = torch.zeros(shb, dtype=torch.int32, device=dev)
+    gq, gh = torch.zeros_like(sp["qj1"]), torch.zeros_like(sp["hc"])
+    # the diurnal SZA of the day's 96 steps (one surface field each, the same on every level)
+    s2 = torch.from_numpy(np.stack([synth.c2_sza(inp.meta, k)[0] for k in range(96)])).to(dev)
+    sza = sp["sza"]                                  # shared by both paths
+    fp["sza"] = sza
+    lat = torch.from_numpy(np.asarray(inp.meta["lat"], dtype=np.float64)).to(dev)
+    bands = {"vortex, lat <= 60S": lat <= -60.0, "60S-30S": (lat > -60.0) & (lat < -30.0),
+             "30S-30N": (lat >= -30.0) & (lat <= 30.0), "30N-60N": (lat > 30.0) & (lat < 60.0),
+             "60N-90N": lat >= 60.0}
+    band_acc = {k: torch.zeros((), dtype=torch.float64, device=dev) for k in bands}
+    band_n = {k: 0 for k in bands}
+    dn_acc = {"day": torch.zeros((), dtype=torch.float64, device=dev),
+              "night": torch.zeros((), dtype=torch.float64, device=dev)}
+    dn_n = {"day": torch.zeros((), dtype=torch.float64, device=dev),
+            "night": torch.zeros((), dtype=torch.float64, device=dev)}
+    L, S = synth.L, synth.S
+    cl_w = {"cly": 1, "cfc11": 3, "cfc12": 2, "cfc113": 3, "ccl4": 4, "ch3ccl3": 3, "ch3cl": 1,
+            "hcfc22": 1, "h1211": 1}
+    br_w = {"bry": 1, "ch3br": 1, "h1211": 1, "h1301": 1, "ch2br2": 2}
+    total = lambda q, w: sum(c * q[L[n], :nb] for n, c in w.items())
+    cl0, br0 = total(sp["qj1"], cl_w), total(sp["qj1"], br_w)
+    args = lambda s: [s[k] for k in ("T", "p", "sza", "alt")]
+    aux = lambda s: [s[k] for k in ("surfarea", "vsed3d", "parth2osed", "parthno3sed")]
+
+    def rel_stats(a, b):
+        """per species (max, p99) of |a - b| / |b| over boxes where either is above 1e-30"""
+        a, b = a.reshape(a.shape[0], -1), b.reshape(b.shape[0], -1)
+        m = torch.maximum(a.abs(), b.abs()) > 1e-30
+        d = ((a - b).abs() / b.abs().clamp(min=1e-30)).where(m, torch.zeros_like(a))
+        q = torch.quantile(d[:, ::7].float(), 0.99, dim=1)   # p99 on a 1-in-7 subsample
+        return d.amax(1).tolist(), q.tolist()
+
+    def named(mx, p99, names):
+        return {n: [float(f"{x:.3g}"), float(f"{y:.3g}")] for n, x, y in zip(names, mx, p99)}
+
+    names_long = [n for n in synth.LONG[:42] if n not in ("ox_passive", "h2so4")]
+    idx_long = [L[n] for n in names_long]
+    ev = []
+    one_step = None
+    t_wall = time.time()
+    for k in range(nsteps):
+        sza.copy_(s2[k % 96].expand(shb))
+        day = sza[0] <= 95.0
+        e = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+        e[0].record(stream)
+        net.guess(grid, inp.dt, sp["T"], sp["p"], sza, sp["qj1"], sp["hc"], gq, gh)
+        step.surrogate_device(grid, inp.dt, accept_tol, tol, maxiter, *args(sp), sp["qj1"],
+                              sp["hc"], *aux(sp), gq, gh, iters=it_s, fuse_conserv=True)
+        e[1].record(stream)
+        e[2].record(stream)
+        step.converge_device(grid, inp.dt, tol, maxiter, *args(fp), fp["qj1"], fp["hc"],
+                             *aux(fp), iters=it_f, fuse_conserv=True)
+        e[3].record(stream)
+        ev.append(e)
+        acc = (it_s == 0).double()
+        for name, m in bands.items():
+            band_acc[name] += acc[:, m, :].sum()
+            band_n[name] += nb * int(m.sum()) * g.nlon
+        dmask = day.double().expand(shb)
+        dn_acc["day"] += (acc * dmask).sum()
+        dn_acc["night"] += (acc * (1.0 - dmask)).sum()
+        dn_n["day"] += dmask.sum()
+        dn_n["night"] += (1.0 - dmask).sum()
+        if k == 0:
+            # the network's answer for the first step against the FP64 step from the same state
+            mq, pq = rel_stats(gq[idx_long, :nb], fp["qj1"][idx_long, :nb])
+            mh, ph = rel_stats(gh[:, :nb], fp["hc"][:, :nb])
+            one_step = {"long": named(mq, pq, names_long), "short": named(mh, ph, synth.SHORT)}
+        if k % 96 == 95:
+            print(f"[surrogate] step {k + 1}/{nsteps}, {time.time() - t_wall:.0f} s",
+                  file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    ms_sur = sum(e[0].elapsed_time(e[1]) for e in ev)
+    ms_fp = sum(e[2].elapsed_time(e[3]) for e in ev)
+    mq, pq = rel_stats(sp["qj1"][idx_long, :nb], fp["qj1"][idx_long, :nb])
+    mh, ph = rel_stats(sp["hc"][:, :nb], fp["hc"][:, :nb])
+    drift = {}
+    for name, w, x0 in (("chlorine", cl_w, cl0), ("bromine", br_w, br0)):
+        for path, st in (("surrogate", sp), ("fp64", fp)):
+            r = (total(st["qj1"], w) / x0 - 1.0).abs()
+            drift[f"{name}_{path}"] = {"max": float(r.max()), "mean": float(r.mean())}
+    nbox = g.nbox
+    accepted = sum(float(v) for v in band_acc.values())
+    out = {"workload": f"c2 seed 0 (786,240 boxes, vortex and liquid PSC), {nsteps} x 15 min "
+                       "sub-steps with diurnal SZA; surrogate path and FP64 path from the same "
+                       "start, conserv + floor after each step",
+           "steps": nsteps, "accept_tol": accept_tol, "tol": tol, "maxiter": maxiter,
+           "solver": "plain sweeps", "block": block_conv, "refill": refill,
+           "accept_rate": accepted / (nbox * nsteps),
+           "accept_rate_by_band": {k: float(band_acc[k]) / max(band_n[k], 1) for k in bands},
+           "accept_rate_day_night": {k: float(dn_acc[k]) / max(float(dn_n[k]), 1.0) for k in dn_acc},
+           "ms_surrogate_path": ms_sur, "ms_fp64_path": ms_fp,
+           "box_steps_per_s_surrogate": nbox * nsteps / (ms_sur / 1e3),
+           "box_steps_per_s_fp64": nbox * nsteps / (ms_fp / 1e3),
+           "speedup_surrogate_over_fp64": ms_fp / ms_sur,
+           "one_step_error_max_p99": one_step,
+           "end_state_difference_max_p99": {"long": named(mq, pq, names_long),
+                                             "short": named(mh, ph, synth.SHORT)},
+           "element_drift": drift}
+    step.close()
+    net.close()
+    return out
+
+
 def transport_section(table, dev, ntr=43, reps=5):
     """Semi-Lagrangian transport interpolation on the model's 180 x 91 x 137 grid
     (2,244,060 points): departure-point cells and weights, then all ntr tracers (the
@@ -371,20 +516,35 @@ def main():
                                  "(profiles/k_chem_c1_pmc.md)"}}
 
     extras = {}
-    if not args.no_extras:
+    want = set() if args.no_extras else set(s for s in args.extras.split(",") if s)
+    conv_shape = None
+    if want & {"converged", "surrogate"}:
         try:
             cache = os.path.join(tempfile.gettempdir(), "reprobus_amd_tune.json")
             c2 = _device_state(synth.make_c2(1), dev)
             r = autotune.tune(table, grid, c2, mode="converged", reps=2, cache_path=cache)
             tuned["converged"] = {"block": r.block, "refill": r.refill, "ms": r.ms}
             del c2
-            extras["converged"] = converged_section(table, r.block, r.refill, dev)
+            conv_shape = (r.block, r.refill)
         except Exception as exc:  # reported beside the headline, never instead of it
             extras["converged"] = {"error": repr(exc)}
+    if "converged" in want and conv_shape:
+        try:
+            extras["converged"] = converged_section(table, *conv_shape, dev)
+        except Exception as exc:
+            extras["converged"] = {"error": repr(exc)}
+    if "surrogate" in want and conv_shape:
+        try:
+            extras["surrogate"] = surrogate_section(table, *conv_shape, dev,
+                                                    nsteps=args.surrogate_steps)
+        except Exception as exc:
+            extras["surrogate"] = {"error": repr(exc)}
+    if "day" in want:
         try:
             extras["day"] = day_section(table, block, dev)
         except Exception as exc:
             extras["day"] = {"error": repr(exc)}
+    if "transport" in want:
         try:
             extras["transport"] = transport_section(table, dev)
         except Exception as exc:

@@ -210,6 +210,21 @@ int rb_chem_converge_dev(rb_ctx *ctx, const rb_grid *g, double dt, double tol, i
                          const double *guess_qj1, const double *guess_hc, int32_t *iters,
                          int fuse_conserv);
 
+/* Surrogate step (no counterpart in the reference): sur_qj1 / sur_hc (device, mixing
+ * ratios, state layout) are a surrogate model's answer for the step -- the learned
+ * network's, rb_mlp_guess_dev.  One FP64 sweep from it is its implicit-step residual
+ * test: where every long- and short-lived species changes by less than accept_tol
+ * relative (the converged mode's test), the surrogate's answer is kept and iters[box] = 0;
+ * every other box goes on sweeping to tol from there, exactly as rb_chem_converge_dev with
+ * that guess (the plain FP64 solver is the fallback and the judge of the surrogate).
+ * accept_tol must be positive; the other arguments are rb_chem_converge_dev's. */
+int rb_chem_surrogate_dev(rb_ctx *ctx, const rb_grid *g, double dt, double accept_tol,
+                          double tol, int32_t maxiter, int32_t accel, const double *T,
+                          const double *p, const double *sza, const double *alt, double *qj1,
+                          double *hc, double *surfarea, double *vsed3d, double *parth2osed,
+                          double *parthno3sed, const double *sur_qj1, const double *sur_hc,
+                          int32_t *iters, int fuse_conserv);
+
 int rb_sync(rb_ctx *ctx);
 /* Waits for the timed launches, fills *out and resets the profile accumulators. */
 int rb_get_stats(rb_ctx *ctx, rb_stats_t *out);

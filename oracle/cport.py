@@ -58,6 +58,11 @@ def lib(cr_psc: bool = False):
             [dp] * 12 + [np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS"), C.c_void_p,
                          C.c_void_p, C.c_int]
         L.rbo_chem_converge_aa.restype = C.c_int
+        L.rbo_chem_surrogate.argtypes = [C.POINTER(_Grid), C.c_double, C.c_double, C.c_double,
+                                         C.c_int, C.c_int] + \
+            [dp] * 12 + [np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS"), dp, dp,
+                         C.c_int]
+        L.rbo_chem_surrogate.restype = C.c_int
         ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
         L.rbo_transport_weights.argtypes = [C.c_int, C.c_int, C.c_int] + [dp] * 6 + [ip, dp]
         L.rbo_transport_weights.restype = None
@@ -140,6 +145,38 @@ def chem_converge(inp, ajl, o3up, tol: float = 1e-10, maxiter: int = 2000, nthre
                                 None if gh is None else gh.ctypes.data, nthreads)
     if rc != 0:
         raise RuntimeError(f"rbo_chem_converge failed: {rc}")
+    out = {"qj1_chem": q.copy(), "hc_chem": h.copy(), "surfarea": surf, "vsed3d": vsed,
+           "parth2osed": p2o, "parthno3sed": pno3, "iters": iters}
+    if conserv:
+        L.rbo_conserv_floor(C.byref(G), q, h, nthreads)
+        out["qj1"], out["hc"] = q, h
+    return out
+
+
+def chem_surrogate(inp, ajl, o3up, sur, accept_tol: float = 1e-8, tol: float = 1e-11,
+                   maxiter: int = 2000, nthreads: int = 0, conserv: bool = True,
+                   cr_psc: bool = False, accel: int = 0, aa_start: int = 0) -> dict:
+    """Surrogate step of the restatement: sur = (qj1, hc) mixing ratios in state layout, a
+    surrogate's answer for the step.  Boxes whose first sweep from it changes every species
+    by less than accept_tol keep it (iters 0); the rest sweep on to tol ('iters')."""
+    g = inp.grid
+    G = _Grid(g.nlon, g.nlat, g.niv, g.nivbas, g.nivhau, maxiter)
+    q = np.ascontiguousarray(inp.qj1, dtype=np.float64).copy()
+    h = np.ascontiguousarray(inp.hc, dtype=np.float64).copy()
+    surf = np.zeros(g.shape3)
+    shb = (g.nivbas, g.nlat, g.nlon)
+    vsed, p2o, pno3 = np.zeros(shb), np.zeros(shb), np.zeros(shb)
+    iters = np.zeros(shb, dtype=np.int32)
+    c = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+    L = lib(cr_psc)
+    sq, sh = c(sur[0]), c(sur[1])
+    assert sq.shape == q.shape and sh.shape == h.shape
+    rc = L.rbo_chem_surrogate(C.byref(G), float(inp.dt), float(accept_tol), float(tol),
+                              int(accel), int(aa_start), c(inp.T), c(inp.p), c(inp.sza),
+                              c(inp.alt), fortran_ajl(ajl), c(o3up), q, h, surf, vsed, p2o, pno3,
+                              iters, sq, sh, nthreads)
+    if rc != 0:
+        raise RuntimeError(f"rbo_chem_surrogate failed: {rc}")
     out = {"qj1_chem": q.copy(), "hc_chem": h.copy(), "surfarea": surf, "vsed3d": vsed,
            "parth2osed": p2o, "parthno3sed": pno3, "iters": iters}
     if conserv:

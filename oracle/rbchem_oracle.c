@@ -989,7 +989,8 @@ static void aa_update(aa_t *A, double *x, const double *xprev, double fl)
 static void box_step(const troe_t *K, const double *ajl, const double *o3up, const boxenv_t *e,
                      double dt, int niter, double *q, double *h, double *surf, double *vsed,
                      double *p2osed, double *pno3sed, int *surf_written, double tol, int *iters,
-                     const double *gq, const double *gh, int accel, int aa_start)
+                     const double *gq, const double *gh, int accel, int aa_start,
+                     double accept_tol)
 {
     double ck0[NBCON + 1], ck[NBCON + 1], cc[NCM + 1], tj[NPHOT + 1], pr[NBCON + 1],
         pe[NBCON + 1];
@@ -1340,14 +1341,25 @@ static void box_step(const troe_t *K, const double *ajl, const double *o3up, con
          * mole-fraction floor compared at the floor): |v - old| < tol max(|v|, floor) */
         if (tol > 0.) {
             const double fl = 1.e-30 * hnm;
+            /* surrogate step: the first sweep from the surrogate's answer measures its
+             * residual as a fixed point of the sweep map; under accept_tol the answer
+             * itself is kept (count 0), otherwise the sweeps go on at tol */
+            const int accept = iter == 1 && accept_tol > 0. && gq;
+            const double tt = accept ? accept_tol : tol;
             int conv = 1, fin = 1;
             for (n = 1; n <= NBCON - 1; ++n) {
-                conv = conv && (fabs(ck[n] - prev[n]) < tol * fmax(fabs(ck[n]), fl));
+                conv = conv && (fabs(ck[n] - prev[n]) < tt * fmax(fabs(ck[n]), fl));
                 fin = fin && (fabs(ck[n]) <= 1.7976931348623157e308);
             }
             for (n = 1; n <= NCM; ++n) {
-                conv = conv && (fabs(cc[n] - prev[NBCON - 1 + n]) < tol * fmax(fabs(cc[n]), fl));
+                conv = conv && (fabs(cc[n] - prev[NBCON - 1 + n]) < tt * fmax(fabs(cc[n]), fl));
                 fin = fin && (fabs(cc[n]) <= 1.7976931348623157e308);
+            }
+            if (conv && fin && accept) {
+                for (n = 1; n <= NBCON - 1; ++n) ck[n] = gq[n] * hnm;
+                for (n = 1; n <= NCM; ++n) cc[n] = gh[n] * hnm;
+                *iters = 0;
+                break;
             }
             if (conv && fin) {
                 *iters = iter;
@@ -1380,7 +1392,7 @@ static int chem_grid(const rbo_grid *g, double dt, const double *T, const double
                      double *qj1, double *hc, double *surfarea, double *vsed3d,
                      double *parth2osed, double *parthno3sed, int nthreads, double tol,
                      int *iters, const double *guess_qj1, const double *guess_hc, int accel,
-                     int aa_start);
+                     int aa_start, double accept_tol);
 
 int rbo_chem_step(const rbo_grid *g, double dt, const double *T, const double *p,
                   const double *sza, const double *alt, const double *ajl, const double *o3up,
@@ -1388,7 +1400,7 @@ int rbo_chem_step(const rbo_grid *g, double dt, const double *T, const double *p
                   double *parth2osed, double *parthno3sed, int nthreads)
 {
     return chem_grid(g, dt, T, p, sza, alt, ajl, o3up, qj1, hc, surfarea, vsed3d, parth2osed,
-                     parthno3sed, nthreads, 0., NULL, NULL, NULL, 0, 0);
+                     parthno3sed, nthreads, 0., NULL, NULL, NULL, 0, 0, 0.);
 }
 
 int rbo_chem_converge(const rbo_grid *g, double dt, double tol, const double *T, const double *p,
@@ -1411,7 +1423,20 @@ int rbo_chem_converge_aa(const rbo_grid *g, double dt, double tol, int accel, in
 {
     if ((guess_qj1 == NULL) != (guess_hc == NULL) || accel < 0 || accel > AA_MAX) return -1;
     return chem_grid(g, dt, T, p, sza, alt, ajl, o3up, qj1, hc, surfarea, vsed3d, parth2osed,
-                     parthno3sed, nthreads, tol, iters, guess_qj1, guess_hc, accel, aa_start);
+                     parthno3sed, nthreads, tol, iters, guess_qj1, guess_hc, accel, aa_start, 0.);
+}
+
+int rbo_chem_surrogate(const rbo_grid *g, double dt, double accept_tol, double tol, int accel,
+                       int aa_start, const double *T, const double *p, const double *sza,
+                       const double *alt, const double *ajl, const double *o3up, double *qj1,
+                       double *hc, double *surfarea, double *vsed3d, double *parth2osed,
+                       double *parthno3sed, int *iters, const double *sur_qj1,
+                       const double *sur_hc, int nthreads)
+{
+    if (!sur_qj1 || !sur_hc || !(accept_tol > 0.) || accel < 0 || accel > AA_MAX) return -1;
+    return chem_grid(g, dt, T, p, sza, alt, ajl, o3up, qj1, hc, surfarea, vsed3d, parth2osed,
+                     parthno3sed, nthreads, tol, iters, sur_qj1, sur_hc, accel, aa_start,
+                     accept_tol);
 }
 
 static int chem_grid(const rbo_grid *g, double dt, const double *T, const double *p,
@@ -1419,7 +1444,7 @@ static int chem_grid(const rbo_grid *g, double dt, const double *T, const double
                      double *qj1, double *hc, double *surfarea, double *vsed3d,
                      double *parth2osed, double *parthno3sed, int nthreads, double tol,
                      int *iters, const double *guess_qj1, const double *guess_hc, int accel,
-                     int aa_start)
+                     int aa_start, double accept_tol)
 {
     const size_t plane = (size_t)g->nlon * g->nlat;          /* one level of a field */
     const size_t field = plane * g->niv;                       /* one species field */
@@ -1458,7 +1483,8 @@ static int chem_grid(const rbo_grid *g, double dt, const double *T, const double
                 for (int n = 1; n <= NCM; ++n) gh[n] = guess_hc[(size_t)(n - 1) * field + b];
             }
             box_step(&K, ajl, o3up, &e, dt, g->niter, q, h, &surf, &vsed, &p2o, &pno3, &sw, tol,
-                     &it, guess_qj1 ? gq : NULL, guess_qj1 ? gh : NULL, accel, aa_start);
+                     &it, guess_qj1 ? gq : NULL, guess_qj1 ? gh : NULL, accel, aa_start,
+                     accept_tol);
             if (it < 0) {
                 /* converged mode without convergence: the reference answer, RBO_NITER_REF
                  * sweeps from the incoming state */
@@ -1466,7 +1492,7 @@ static int chem_grid(const rbo_grid *g, double dt, const double *T, const double
                 for (int n = 1; n <= NCM; ++n) h[n] = hc[(size_t)(n - 1) * field + b];
                 int it5;
                 box_step(&K, ajl, o3up, &e, dt, RBO_NITER_REF, q, h, &surf, &vsed, &p2o, &pno3,
-                         &sw, 0., &it5, NULL, NULL, 0, 0);
+                         &sw, 0., &it5, NULL, NULL, 0, 0, 0.);
             }
             if (iters) iters[b] = it;
             for (int n = 1; n <= NBCON; ++n) qj1[(size_t)(n - 1) * field + b] = q[n];

@@ -55,6 +55,18 @@ int rbo_chem_converge_aa(const rbo_grid *g, double dt, double tol, int accel, in
                          double *parthno3sed, int *iters, const double *guess_qj1,
                          const double *guess_hc, int nthreads);
 
+/* Surrogate step: sur_qj1/sur_hc (state layout) are a surrogate model's answer for the
+ * step.  One sweep from it measures its residual as a fixed point of the sweep map (the
+ * converged mode's test at accept_tol); a box under accept_tol keeps the surrogate's
+ * answer and gets iters[box] = 0, every other box goes on sweeping to tol exactly as
+ * rbo_chem_converge_aa does from that guess. */
+int rbo_chem_surrogate(const rbo_grid *g, double dt, double accept_tol, double tol, int accel,
+                       int aa_start, const double *T, const double *p, const double *sza,
+                       const double *alt, const double *ajl, const double *o3up, double *qj1,
+                       double *hc, double *surfarea, double *vsed3d, double *parth2osed,
+                       double *parthno3sed, int *iters, const double *sur_qj1,
+                       const double *sur_hc, int nthreads);
+
 /* conserv (F:2526-2616) on levels 1..nivbas, then the 1e-30 floor of
  * species 1..42 on all niv levels (F:431-440), then HNO3 gas = HNO3 on levels
  * nivbas+1..niv (F:444-450). */

@@ -121,6 +121,20 @@ class ChemistryStep:
             *[_ptr(a) for a in (T, p, sza, alt, qj1, hc, surfarea, vsed3d, parth2osed,
                                 parthno3sed)], g_q, g_h, it, 1 if fuse_conserv else 0))
 
+    def surrogate_device(self, grid: Grid, dt: float, accept_tol: float, tol: float,
+                         maxiter: int, T, p, sza, alt, qj1, hc, surfarea, vsed3d, parth2osed,
+                         parthno3sed, sur_qj1, sur_hc, iters=None, fuse_conserv: bool = True,
+                         accel: int = 0) -> None:
+        """Surrogate step: keep the surrogate's answer (sur_qj1, sur_hc) where one FP64
+        sweep from it changes every species by less than accept_tol (iters 0), sweep the
+        other boxes on to tol from it."""
+        it = iters.data_ptr() if iters is not None else None
+        self._check(self._L.rb_chem_surrogate_dev(
+            self._ctx, C.byref(grid), float(dt), float(accept_tol), float(tol), int(maxiter),
+            int(accel), *[_ptr(a) for a in (T, p, sza, alt, qj1, hc, surfarea, vsed3d,
+                                            parth2osed, parthno3sed, sur_qj1, sur_hc)], it,
+            1 if fuse_conserv else 0))
+
     def sediment(self, grid: Grid, dt: float, nivh2o: int, T, p, qj1, vsed3d, parth2osed,
                  parthno3sed) -> None:
         """Sedimentation on host arrays (qj1 updated in place), F:2416-2522."""

@@ -61,6 +61,9 @@ struct ConvArgs {
     double *spill;                     // accel > 0: boxes handed from the sweep phase to
                                        // the Anderson phase, CONV_SPILL doubles per box
                                        // (iterate, box index, sweep count), field-major
+    double accept_tol;                 // > 0 (surrogate step, with a guess): the first
+                                       // sweep from the guess tests it at accept_tol; a
+                                       // box under it keeps the guess, count 0
 };
 // doubles per box of the converged mode's set-up record
 constexpr int CONV_RECORD = 166;

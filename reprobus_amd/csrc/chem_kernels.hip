@@ -1783,6 +1783,14 @@ __global__ void __launch_bounds__(BLK) k_chem_conv(StepArgs a, ConvArgs cv)
     for (;;) {
         // serve the wave
         if (done) {
+            if (result == 0) {
+                // the surrogate's answer passed its residual test: it is the result
+                const double hnm = x.e.hnm;
+#pragma unroll
+                for (int s = 0; s < NLONG - 1; ++s) x.ck[s] = a.guess_q[(int64_t)s * a.field + b] * hnm;
+#pragma unroll
+                for (int s = 0; s < NSHORT; ++s) x.cc[s] = a.guess_h[(int64_t)s * a.field + b] * hnm;
+            }
             box_writeback(a, b, x);
             if (cv.iters) cv.iters[b] = result;
             done = false;
@@ -1851,7 +1859,9 @@ __global__ void __launch_bounds__(BLK) k_chem_conv(StepArgs a, ConvArgs cv)
             if (active) {
                 const bool acc = AAM > 0 && !fallback && it + 1 >= cv.aa_start;
                 if (AAM > 0 && acc) aa_save(x.ck, x.cc, ws, lanes, head);
-                Track tr{true, true, cv.tol, 1.e-30 * x.e.hnm};
+                // the first sweep from a surrogate's answer is its acceptance test
+                const bool accept = cv.accept_tol > 0. && it == 0 && !fallback;
+                Track tr{true, true, accept ? cv.accept_tol : cv.tol, 1.e-30 * x.e.hnm};
                 sweep<BLK, true>(x.R, x.tj, x.e, ck0, x.ck, x.cc, x.ch3o, x.hco, tr);
                 ++it;
                 if (fallback) {
@@ -1861,7 +1871,7 @@ __global__ void __launch_bounds__(BLK) k_chem_conv(StepArgs a, ConvArgs cv)
                         active = false;
                     }
                 } else if (tr.conv && tr.fin) {
-                    result = it;
+                    result = accept ? 0 : it;
                     done = true;
                     active = false;
                 } else if (!tr.fin || it >= cv.maxiter) {

@@ -251,12 +251,13 @@ int rb_chem_step_dev(rb_ctx *ctx, const rb_grid *g, double dt, const double *T, 
     return RB_OK;
 }
 
-int rb_chem_converge_dev(rb_ctx *ctx, const rb_grid *g, double dt, double tol, int32_t maxiter,
+// the converged mode; accept_tol > 0 makes it the surrogate step (guess = the surrogate)
+static int converge_impl(rb_ctx *ctx, const rb_grid *g, double dt, double tol, int32_t maxiter,
                          int32_t accel, const double *T, const double *p, const double *sza,
                          const double *alt, double *qj1, double *hc, double *surfarea,
                          double *vsed3d, double *parth2osed, double *parthno3sed,
                          const double *guess_qj1, const double *guess_hc, int32_t *iters,
-                         int fuse_conserv)
+                         int fuse_conserv, double accept_tol)
 {
     if (!ctx) return RB_EINVAL;
     int rc = check_grid(ctx, g);
@@ -294,7 +295,7 @@ int rb_chem_converge_dev(rb_ctx *ctx, const rb_grid *g, double dt, double tol, i
     if ((rc = conv_buffers(ctx, (size_t)nbox, accel))) return rc;
     rb::ConvArgs cv{tol, maxiter, ctx->d_queue, iters, accel, accel ? ctx->d_aa : nullptr,
                     ctx->cfg.conv_refill, ctx->d_rec, ctx->cfg.conv_aa_start,
-                    accel ? ctx->d_spill : nullptr};
+                    accel ? ctx->d_spill : nullptr, accept_tol};
 
     if (!ctx->capturing) HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
     HIPCHK(rb::launch_columns(a, ctx->d_col, ctx->stream));
@@ -320,6 +321,35 @@ int rb_chem_converge_dev(rb_ctx *ctx, const rb_grid *g, double dt, double tol, i
     ctx->timed = !ctx->capturing;
     ctx->stats.boxes = nbox;
     return RB_OK;
+}
+
+int rb_chem_converge_dev(rb_ctx *ctx, const rb_grid *g, double dt, double tol, int32_t maxiter,
+                         int32_t accel, const double *T, const double *p, const double *sza,
+                         const double *alt, double *qj1, double *hc, double *surfarea,
+                         double *vsed3d, double *parth2osed, double *parthno3sed,
+                         const double *guess_qj1, const double *guess_hc, int32_t *iters,
+                         int fuse_conserv)
+{
+    return converge_impl(ctx, g, dt, tol, maxiter, accel, T, p, sza, alt, qj1, hc, surfarea,
+                         vsed3d, parth2osed, parthno3sed, guess_qj1, guess_hc, iters,
+                         fuse_conserv, 0.);
+}
+
+int rb_chem_surrogate_dev(rb_ctx *ctx, const rb_grid *g, double dt, double accept_tol,
+                          double tol, int32_t maxiter, int32_t accel, const double *T,
+                          const double *p, const double *sza, const double *alt, double *qj1,
+                          double *hc, double *surfarea, double *vsed3d, double *parth2osed,
+                          double *parthno3sed, const double *sur_qj1, const double *sur_hc,
+                          int32_t *iters, int fuse_conserv)
+{
+    if (!ctx) return RB_EINVAL;
+    if (!sur_qj1 || !sur_hc)
+        return fail(ctx, RB_EINVAL, "the surrogate's answer (sur_qj1, sur_hc) is NULL");
+    if (!(accept_tol > 0.) || !std::isfinite(accept_tol))
+        return fail(ctx, RB_EINVAL, "accept_tol must be positive and finite");
+    return converge_impl(ctx, g, dt, tol, maxiter, accel, T, p, sza, alt, qj1, hc, surfarea,
+                         vsed3d, parth2osed, parthno3sed, sur_qj1, sur_hc, iters, fuse_conserv,
+                         accept_tol);
 }
 
 int rb_conserv_floor_dev(rb_ctx *ctx, const rb_grid *g, double *qj1, double *hc)

@@ -397,6 +397,38 @@ def test_converged_state_is_a_fixed_point(table):
         assert d[:, live].max() < 1e-9, k
 
 
+def test_surrogate_keeps_exactly_the_fixed_points(table):
+    """Surrogate step of the restatement: an answer that is the converged state (a fixed
+    point of the sweep map to 1e-12) passes the 1e-8 residual test and is kept as it is
+    (count 0); an answer 1e-4 off is rejected, and the box then holds exactly what the
+    converged mode gives from that answer as its initial guess."""
+    from reprobus_amd import synth
+
+    inp = synth.make_c1(seed=5, nlon=6, nlat=5)
+    base = cport.chem_converge(inp, *table, tol=1e-12, maxiter=20000, conserv=False)
+    live = base["iters"] > 0
+    rng = np.random.default_rng(2)
+    keep = (rng.random(live.shape) < 0.5) & live
+    gq, gh = base["qj1_chem"].copy(), base["hc_chem"].copy()
+    gq[:42, ~keep] *= np.exp(rng.normal(0, 1e-4, gq[:42, ~keep].shape))
+    gh[:, ~keep] *= np.exp(rng.normal(0, 1e-4, gh[:, ~keep].shape))
+    sur = cport.chem_surrogate(inp, *table, (gq, gh), accept_tol=1e-8, tol=1e-12,
+                               maxiter=20000, conserv=False)
+    it = sur["iters"]
+    assert np.array_equal(it == 0, keep)
+    # kept: the surrogate's own answer (through the number-density round trip)
+    assert rel_err(sur["qj1_chem"][:42], gq[:42])[:, keep].max() < 1e-15
+    assert rel_err(sur["hc_chem"], gh)[:, keep].max() < 1e-15
+    # rejected: the converged mode from the same guess, bit for bit
+    conv = cport.chem_converge(inp, *table, tol=1e-12, maxiter=20000, conserv=False,
+                               guess=(gq, gh))
+    rej = ~keep
+    assert np.array_equal(it[rej], conv["iters"][rej])
+    assert np.all(it[rej] != 0)
+    for k in ("qj1_chem", "hc_chem"):
+        assert np.array_equal(sur[k][:, rej], conv[k][:, rej]), k
+
+
 def _inputs_fixture():
     import os
 
