@@ -88,7 +88,7 @@ hipError_t launch_transport_weights(const TransportGrid &t, const double *aaa, c
 // tracers per workgroup of the interpolation (a grid dimension: a smaller group keeps the
 // cached neighbourhoods of fewer fields live at once, at the cost of reading each point's
 // indices and weights once per group)
-constexpr int TRANSPORT_GROUP = 4;
+constexpr int TRANSPORT_GROUP = 11;
 hipError_t launch_transport_interp(const TransportGrid &t, const int32_t *idx, const double *w,
                                    const double *q_in, int64_t sin_, double *q_out, int64_t sout,
                                    int ntr, unsigned long long *bad, hipStream_t s, int group = 0);

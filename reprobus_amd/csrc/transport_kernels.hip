@@ -5,9 +5,9 @@
 //
 // One lane per grid point.  The weights kernel reproduces the reference's arithmetic
 // operation for operation (FP64, built with -ffp-contract=off).  The interpolation kernel
-// reads a point's 3 indices and 12 weights once and loops over the tracers: per tracer 64
-// gathers (the 4 x 4 x 4 stencil, neighbouring lanes share most of it through the caches)
-// and one coalesced store.  The reference's extended grid x(0:nlon+2, 0:nlat+1, 0:niv+1)
+// reads a point's 3 indices and 12 weights once per group of tracers and loops over the
+// group: per tracer the 4 x 4 x 4 stencil (16 rows of four longitudes, neighbouring lanes
+// share most of it through the caches) and one coalesced store.  The reference's extended grid x(0:nlon+2, 0:nlat+1, 0:niv+1)
 // (F:8374-8413) is not built: each stencil index is mapped to the field instead.
 #include <hip/hip_runtime.h>
 
@@ -77,17 +77,31 @@ __global__ void k_transport_weights(TransportGrid t, const double *__restrict__ 
     }
 }
 
+// Two doubles at an 8-byte aligned address: one global_load_dwordx4 (multi-dword global
+// accesses need dword alignment only).
+typedef double double2_a8 __attribute__((ext_vector_type(2), aligned(8)));
+
 // ntr fields (q_in + t * sin, t < ntr) interpolated at the departure points into
 // q_out + t * sout.  A point whose indices fall outside the grid is counted in *bad and
 // not written.
+//
+// Launch shape: workgroups go round-robin to the 8 XCDs, so workgroup j computes the
+// points of tile (j mod 8) * per + j / 8 (per = tiles per XCD, the grid rounded up to
+// 8 * per workgroups): each XCD takes one contiguous eighth of the grid, and the
+// neighbouring rows and levels its stencils share stay in that XCD's L2.  The level loop
+// is not unrolled (126 VGPRs, four waves per SIMD to overlap the gathers), and a stencil
+// row of four longitudes that needs neither the periodic wrap nor the polar mirror is read
+// as two 16-byte loads.  The arithmetic is the same in every path.
 __global__ void __launch_bounds__(256) k_transport_interp(TransportGrid t, const int32_t *__restrict__ idx,
                                                           const double *__restrict__ w,
                                                           const double *__restrict__ q_in, int64_t sin_,
                                                           double *__restrict__ q_out, int64_t sout,
-                                                          int ntr, int64_t n, int group,
+                                                          int ntr, int64_t n, int group, int ntiles,
                                                           unsigned long long *__restrict__ bad)
 {
-    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int per = (ntiles + 7) / 8;
+    const int64_t tile = (int64_t)(blockIdx.x % 8) * per + blockIdx.x / 8;
+    const int64_t p = tile * blockDim.x + threadIdx.x;
     if (p >= n) return;
     const int i = idx[p], k = idx[n + p], l = idx[2 * n + p];
     if (!(i >= 1 && i <= t.nlon && k >= 1 && k <= t.nlat - 1 && l >= 1 && l <= t.niv)) {
@@ -124,6 +138,8 @@ __global__ void __launch_bounds__(256) k_transport_interp(TransportGrid t, const
             po[b][a] = (ks - 1) * t.nlon + (is - 1);
         }
     }
+    // rows i-1 .. i+2 of lat rows k-1 .. k+2 all inside the field: po[b][a] = po[b][0] + a
+    const bool rows = i >= 2 && i + 2 <= t.nlon && k >= 2 && k + 2 <= t.nlat;
     int32_t lo[4];
     const int32_t plane = t.nlon * t.nlat;
 #pragma unroll
@@ -134,13 +150,26 @@ __global__ void __launch_bounds__(256) k_transport_interp(TransportGrid t, const
     for (int tr = tr0; tr < tr1; ++tr) {
         const double *q = q_in + (int64_t)tr * sin_;
         double z[4], in[2][2][2];   // in: [level lp0/lp1][lat kp0/kp1][lon ip0/ip1]
-#pragma unroll
+#pragma unroll 1
         for (int c = 0; c < 4; ++c) {
             double xv[4][4];
+            if (rows) {
 #pragma unroll
-            for (int b = 0; b < 4; ++b)
+                for (int b = 0; b < 4; ++b) {
+                    const double *r = q + lo[c] + po[b][0];
+                    const double2_a8 u0 = *(const double2_a8 *)r;
+                    const double2_a8 u1 = *(const double2_a8 *)(r + 2);
+                    xv[b][0] = u0.x;
+                    xv[b][1] = u0.y;
+                    xv[b][2] = u1.x;
+                    xv[b][3] = u1.y;
+                }
+            } else {
 #pragma unroll
-                for (int a = 0; a < 4; ++a) xv[b][a] = q[lo[c] + po[b][a]];
+                for (int b = 0; b < 4; ++b)
+#pragma unroll
+                    for (int a = 0; a < 4; ++a) xv[b][a] = q[lo[c] + po[b][a]];
+            }
             if (c == 1 || c == 2)
 #pragma unroll
                 for (int b = 0; b < 2; ++b)
@@ -207,8 +236,11 @@ hipError_t launch_transport_interp(const TransportGrid &t, const int32_t *idx, c
     const int64_t n = (int64_t)t.nlon * t.nlat * t.niv;
     if (group < 1) group = TRANSPORT_GROUP;
     const unsigned ng = (unsigned)((ntr + group - 1) / group);
-    hipLaunchKernelGGL(k_transport_interp, dim3((unsigned)((n + 255) / 256), ng), dim3(256), 0, s, t,
-                       idx, w, q_in, sin_, q_out, sout, ntr, n, group, bad);
+    // 256-point tiles, the workgroups rounded up to a multiple of the 8 XCDs
+    const int ntiles = (int)((n + 255) / 256);
+    const unsigned nwg = (unsigned)(((ntiles + 7) / 8) * 8);
+    hipLaunchKernelGGL(k_transport_interp, dim3(nwg, ng), dim3(256), 0, s, t, idx, w, q_in, sin_,
+                       q_out, sout, ntr, n, group, ntiles, bad);
     return hipGetLastError();
 }
 
