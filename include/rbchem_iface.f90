@@ -92,6 +92,18 @@ module rbchem_iface
        real(c_double), intent(out) :: alt(*)
      end function rb_altitude
 
+     ! semilag after the trajectories (F:6121-6153): posx..poidz at the departure points
+     ! x0, y0, z0, p0 (COMMON /depo/), then internew for tracers 1..ntr of qj1, in place
+     integer(c_int) function rb_transport(ctx, g, aaa, bbb, x0, y0, z0, p0, ntr, qj1) &
+                                          bind(C, name="rb_transport")
+       import :: c_int, c_int32_t, c_ptr, c_double, rb_grid
+       type(c_ptr), value :: ctx
+       type(rb_grid), intent(in) :: g
+       real(c_double), intent(in) :: aaa(*), bbb(*), x0(*), y0(*), z0(*), p0(*)
+       integer(c_int32_t), value :: ntr
+       real(c_double), intent(inout) :: qj1(*)
+     end function rb_transport
+
      ! rbio.h: the history/restart record (F:587-588, F:231-233)
      integer(c_int) function rb_history_write(path, nlon, nlat, niv, h, pj1, uj1, vj1, alt, tj1, &
                                               qj1, surfarea, hc) bind(C, name="rb_history_write")
