@@ -52,6 +52,9 @@ typedef struct {
                                together (1..64; 0: 16) -- one set-up pass serves them all */
     int32_t conv_aa_start;  /* converged mode with accel: plain sweeps before this sweep
                                number, Anderson mixing from it on (0: from the first) */
+    int32_t conv_aa_budget; /* converged mode with accel: accelerated sweeps a box gets
+                               before it falls back, i.e. the sweep cap is
+                               min(maxiter, conv_aa_start + conv_aa_budget) (0: 240) */
 } rb_config;
 
 typedef struct {

@@ -54,12 +54,12 @@ def lib(cr_psc: bool = False):
             C.c_int]
         L.rbo_chem_converge.restype = C.c_int
         L.rbo_chem_converge_aa.argtypes = [C.POINTER(_Grid), C.c_double, C.c_double, C.c_int,
-                                           C.c_int] + \
+                                           C.c_int, C.c_int] + \
             [dp] * 12 + [np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS"), C.c_void_p,
                          C.c_void_p, C.c_int]
         L.rbo_chem_converge_aa.restype = C.c_int
         L.rbo_chem_surrogate.argtypes = [C.POINTER(_Grid), C.c_double, C.c_double, C.c_double,
-                                         C.c_int, C.c_int] + \
+                                         C.c_int, C.c_int, C.c_int] + \
             [dp] * 12 + [np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS"), dp, dp,
                          C.c_int]
         L.rbo_chem_surrogate.restype = C.c_int
@@ -120,9 +120,10 @@ def fortran_ajl(ajl):
 
 def chem_converge(inp, ajl, o3up, tol: float = 1e-10, maxiter: int = 2000, nthreads: int = 0,
                   conserv: bool = True, cr_psc: bool = False, guess=None, accel: int = 0,
-                  aa_start: int = 0) -> dict:
+                  aa_start: int = 0, aa_budget: int = 240) -> dict:
     """Converged mode of the restatement: sweeps to the fixed point (per-box counts in 'iters').
-    guess = (qj1, hc) mixing ratios in state layout replaces the initial iterate."""
+    guess = (qj1, hc) mixing ratios in state layout replaces the initial iterate; with accel
+    the sweep cap is min(maxiter, aa_start + aa_budget), as the library's."""
     g = inp.grid
     G = _Grid(g.nlon, g.nlat, g.niv, g.nivbas, g.nivhau, maxiter)
     q = np.ascontiguousarray(inp.qj1, dtype=np.float64).copy()
@@ -138,7 +139,7 @@ def chem_converge(inp, ajl, o3up, tol: float = 1e-10, maxiter: int = 2000, nthre
         gq, gh = c(guess[0]), c(guess[1])
         assert gq.shape == q.shape and gh.shape == h.shape
     rc = L.rbo_chem_converge_aa(C.byref(G), float(inp.dt), float(tol), int(accel), int(aa_start),
-                                c(inp.T),
+                                int(aa_budget), c(inp.T),
                                 c(inp.p), c(inp.sza), c(inp.alt), fortran_ajl(ajl), c(o3up), q, h,
                                 surf, vsed, p2o, pno3, iters,
                                 None if gq is None else gq.ctypes.data,
@@ -155,7 +156,8 @@ def chem_converge(inp, ajl, o3up, tol: float = 1e-10, maxiter: int = 2000, nthre
 
 def chem_surrogate(inp, ajl, o3up, sur, accept_tol: float = 1e-8, tol: float = 1e-11,
                    maxiter: int = 2000, nthreads: int = 0, conserv: bool = True,
-                   cr_psc: bool = False, accel: int = 0, aa_start: int = 0) -> dict:
+                   cr_psc: bool = False, accel: int = 0, aa_start: int = 0,
+                   aa_budget: int = 240) -> dict:
     """Surrogate step of the restatement: sur = (qj1, hc) mixing ratios in state layout, a
     surrogate's answer for the step.  Boxes whose first sweep from it changes every species
     by less than accept_tol keep it (iters 0); the rest sweep on to tol ('iters')."""
@@ -172,7 +174,8 @@ def chem_surrogate(inp, ajl, o3up, sur, accept_tol: float = 1e-8, tol: float = 1
     sq, sh = c(sur[0]), c(sur[1])
     assert sq.shape == q.shape and sh.shape == h.shape
     rc = L.rbo_chem_surrogate(C.byref(G), float(inp.dt), float(accept_tol), float(tol),
-                              int(accel), int(aa_start), c(inp.T), c(inp.p), c(inp.sza),
+                              int(accel), int(aa_start), int(aa_budget), c(inp.T), c(inp.p),
+                              c(inp.sza),
                               c(inp.alt), fortran_ajl(ajl), c(o3up), q, h, surf, vsed, p2o, pno3,
                               iters, sq, sh, nthreads)
     if rc != 0:

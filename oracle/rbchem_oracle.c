@@ -1409,32 +1409,44 @@ int rbo_chem_converge(const rbo_grid *g, double dt, double tol, const double *T,
                       double *parth2osed, double *parthno3sed, int *iters,
                       const double *guess_qj1, const double *guess_hc, int nthreads)
 {
-    return rbo_chem_converge_aa(g, dt, tol, 0, 0, T, p, sza, alt, ajl, o3up, qj1, hc, surfarea,
-                                vsed3d, parth2osed, parthno3sed, iters, guess_qj1, guess_hc,
-                                nthreads);
+    return rbo_chem_converge_aa(g, dt, tol, 0, 0, 0, T, p, sza, alt, ajl, o3up, qj1, hc,
+                                surfarea, vsed3d, parth2osed, parthno3sed, iters, guess_qj1,
+                                guess_hc, nthreads);
+}
+
+/* accelerated solves: the sweep cap is min(maxiter, aa_start + aa_budget) */
+static rbo_grid capped(const rbo_grid *g, int accel, int aa_start, int aa_budget)
+{
+    rbo_grid c = *g;
+    if (accel > 0 && aa_budget > 0 && (long)aa_start + aa_budget < c.niter)
+        c.niter = aa_start + aa_budget;
+    return c;
 }
 
 int rbo_chem_converge_aa(const rbo_grid *g, double dt, double tol, int accel, int aa_start,
-                         const double *T, const double *p, const double *sza, const double *alt,
-                         const double *ajl, const double *o3up, double *qj1, double *hc,
-                         double *surfarea, double *vsed3d, double *parth2osed,
+                         int aa_budget, const double *T, const double *p, const double *sza,
+                         const double *alt, const double *ajl, const double *o3up, double *qj1,
+                         double *hc, double *surfarea, double *vsed3d, double *parth2osed,
                          double *parthno3sed, int *iters, const double *guess_qj1,
                          const double *guess_hc, int nthreads)
 {
     if ((guess_qj1 == NULL) != (guess_hc == NULL) || accel < 0 || accel > AA_MAX) return -1;
-    return chem_grid(g, dt, T, p, sza, alt, ajl, o3up, qj1, hc, surfarea, vsed3d, parth2osed,
+    const rbo_grid c = capped(g, accel, aa_start, aa_budget);
+    return chem_grid(&c, dt, T, p, sza, alt, ajl, o3up, qj1, hc, surfarea, vsed3d, parth2osed,
                      parthno3sed, nthreads, tol, iters, guess_qj1, guess_hc, accel, aa_start, 0.);
 }
 
 int rbo_chem_surrogate(const rbo_grid *g, double dt, double accept_tol, double tol, int accel,
-                       int aa_start, const double *T, const double *p, const double *sza,
+                       int aa_start, int aa_budget, const double *T, const double *p,
+                       const double *sza,
                        const double *alt, const double *ajl, const double *o3up, double *qj1,
                        double *hc, double *surfarea, double *vsed3d, double *parth2osed,
                        double *parthno3sed, int *iters, const double *sur_qj1,
                        const double *sur_hc, int nthreads)
 {
     if (!sur_qj1 || !sur_hc || !(accept_tol > 0.) || accel < 0 || accel > AA_MAX) return -1;
-    return chem_grid(g, dt, T, p, sza, alt, ajl, o3up, qj1, hc, surfarea, vsed3d, parth2osed,
+    const rbo_grid c = capped(g, accel, aa_start, aa_budget);
+    return chem_grid(&c, dt, T, p, sza, alt, ajl, o3up, qj1, hc, surfarea, vsed3d, parth2osed,
                      parthno3sed, nthreads, tol, iters, sur_qj1, sur_hc, accel, aa_start,
                      accept_tol);
 }

@@ -47,11 +47,12 @@ int rbo_chem_converge(const rbo_grid *g, double dt, double tol, const double *T,
 
 /* Converged mode with Anderson acceleration of depth accel (0 = plain sweeps, <= 4) on the
  * sweep map from sweep aa_start on (plain sweeps before it): the same fixed point and
- * stopping test (the plain map's residual), a different sequence of iterates. */
+ * stopping test (the plain map's residual), a different sequence of iterates.  With
+ * accel > 0 and aa_budget > 0 the sweep cap is min(g->niter, aa_start + aa_budget). */
 int rbo_chem_converge_aa(const rbo_grid *g, double dt, double tol, int accel, int aa_start,
-                         const double *T, const double *p, const double *sza, const double *alt,
-                         const double *ajl, const double *o3up, double *qj1, double *hc,
-                         double *surfarea, double *vsed3d, double *parth2osed,
+                         int aa_budget, const double *T, const double *p, const double *sza,
+                         const double *alt, const double *ajl, const double *o3up, double *qj1,
+                         double *hc, double *surfarea, double *vsed3d, double *parth2osed,
                          double *parthno3sed, int *iters, const double *guess_qj1,
                          const double *guess_hc, int nthreads);
 
@@ -61,7 +62,8 @@ int rbo_chem_converge_aa(const rbo_grid *g, double dt, double tol, int accel, in
  * answer and gets iters[box] = 0, every other box goes on sweeping to tol exactly as
  * rbo_chem_converge_aa does from that guess. */
 int rbo_chem_surrogate(const rbo_grid *g, double dt, double accept_tol, double tol, int accel,
-                       int aa_start, const double *T, const double *p, const double *sza,
+                       int aa_start, int aa_budget, const double *T, const double *p,
+                       const double *sza,
                        const double *alt, const double *ajl, const double *o3up, double *qj1,
                        double *hc, double *surfarea, double *vsed3d, double *parth2osed,
                        double *parthno3sed, int *iters, const double *sur_qj1,

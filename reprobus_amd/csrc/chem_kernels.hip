@@ -1595,8 +1595,13 @@ __global__ void __launch_bounds__(BLK) k_chem(StepArgs a)
 // Anderson acceleration (Walker & Ni 2011, type II) of depth M on x = (ck, cc), the same
 // arithmetic in the same order as the restatement's aa_update.  Per lane, a ring of M + 1
 // (f, g) pairs lives in device memory: F(slot, i) = ws[((2 slot) * NX + i) * stride],
-// G(slot, i) = ws[((2 slot + 1) * NX + i) * stride] (stride = lanes of the grid, so a
-// wave's accesses coalesce), and one more row for the new iterate: 2 (M + 1) + 1 rows.  On entry x holds f = F(x_k) and G(head) the iterate x_k
+// G(slot, i) = ws[((2 slot + 1) * NX + i) * stride], and one more row for the new iterate:
+// 2 (M + 1) + 1 rows.  The kernel passes stride 1 and a contiguous block per lane: every
+// address is then the lane's base plus a constant, an immediate offset of the load or
+// store.  (Interleaving the lanes, stride = lanes, coalesces a wave's accesses but needs a
+// 64-bit offset per species; the compiler hoisted the 57 offsets out of the sweep loop and
+// spilled them, 1.2 KB of scratch per lane against 0.9 KB, and the Anderson phase's tail,
+// where it matters, runs one or two lanes per wave.)  On entry x holds f = F(x_k) and G(head) the iterate x_k
 // (aa_save before the sweep), which becomes the residual g_k = f - x_k; on exit x holds
 // x_{k+1} = f - dF gamma, gamma
 // minimising || w (g_k - dG gamma) ||, w_i = 1 / max(|f_i|, fl)^2; components that would
@@ -1640,14 +1645,15 @@ __device__ void aa_update(double *ck, double *cc, double *ws, int64_t stride, in
         ia[j] = (slot - j + (M + 1)) % (M + 1);
         ib[j] = (slot - j - 1 + 2 * (M + 1)) % (M + 1);
     }
-    // the normal equations, streaming over the species from the ring (a runtime loop: the
-    // box state stays in registers, the history is read as it is needed)
+    // the normal equations, streaming over the species from the ring (a runtime loop, so
+    // the box state stays in registers; unrolled by 19 = 57 / 3, so that the loads of 19
+    // species are in flight together: fully unrolled, the loop spills the box state)
     double Mx[M][M + 1];
 #pragma unroll
     for (int r = 0; r < M; ++r)
 #pragma unroll
         for (int c = 0; c <= M; ++c) Mx[r][c] = 0.;
-#pragma unroll 1
+#pragma unroll 19
     for (int i = 0; i < NX; ++i) {
         const double sc = fmax(fabs(F(slot, i)), fl);
         const double w = 1. / (sc * sc);
@@ -1720,7 +1726,7 @@ __device__ void aa_update(double *ck, double *cc, double *ws, int64_t stride, in
     if (!ok) return;
     // x_{k+1} = f - dF gamma into the XN row (components that would turn non-positive
     // keep f), then back into the registers
-#pragma unroll 1
+#pragma unroll 19
     for (int i = 0; i < NX; ++i) {
         const double fi = F(slot, i);
         double v = fi;
@@ -1763,8 +1769,10 @@ __global__ void __launch_bounds__(BLK) k_chem_conv(StepArgs a, ConvArgs cv)
     __shared__ double s_ck0[(NLONG - 1) * BLK];
     double *ck0 = s_ck0 + threadIdx.x;
     const int lane = threadIdx.x & (warpSize - 1);
-    const int64_t lanes = (int64_t)gridDim.x * BLK;
-    double *ws = AAM > 0 ? cv.aa_ws + (int64_t)blockIdx.x * BLK + threadIdx.x : nullptr;
+    // the lane's Anderson history: one contiguous block of 2 (AAM + 1) + 1 rows of NX
+    double *ws = AAM > 0 ? cv.aa_ws + ((int64_t)blockIdx.x * BLK + threadIdx.x) *
+                                          (2 * (AAM + 1) + 1) * NX
+                         : nullptr;
     // the spill list: the iterate (NX fields), box index and sweep count per entry
     const int64_t cap = a.nbox;
     int64_t *sp_box = MODE != CONV_PLAIN ? (int64_t *)(cv.spill + NX * cap) : nullptr;
@@ -1858,7 +1866,7 @@ __global__ void __launch_bounds__(BLK) k_chem_conv(StepArgs a, ConvArgs cv)
             }
             if (active) {
                 const bool acc = AAM > 0 && !fallback && it + 1 >= cv.aa_start;
-                if (AAM > 0 && acc) aa_save(x.ck, x.cc, ws, lanes, head);
+                if (AAM > 0 && acc) aa_save(x.ck, x.cc, ws, 1, head);
                 // the first sweep from a surrogate's answer is its acceptance test
                 const bool accept = cv.accept_tol > 0. && it == 0 && !fallback;
                 Track tr{true, true, accept ? cv.accept_tol : cv.tol, 1.e-30 * x.e.hnm};
@@ -1881,7 +1889,7 @@ __global__ void __launch_bounds__(BLK) k_chem_conv(StepArgs a, ConvArgs cv)
                     restart = true;
                     active = false;
                 } else if (AAM > 0 && acc) {
-                    aa_update<(AAM > 0 ? AAM : 1)>(x.ck, x.cc, ws, lanes, head, nstored,
+                    aa_update<(AAM > 0 ? AAM : 1)>(x.ck, x.cc, ws, 1, head, nstored,
                                                    1.e-30 * x.e.hnm);
                 }
             }

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -142,6 +143,7 @@ int rb_init(rb_ctx **out, int device, const rb_config *cfg)
     if (ctx->cfg.conv_refill <= 0) ctx->cfg.conv_refill = 16;
     if (ctx->cfg.conv_refill > 64) ctx->cfg.conv_refill = 64;
     if (ctx->cfg.conv_aa_start < 0) ctx->cfg.conv_aa_start = 60;
+    if (ctx->cfg.conv_aa_budget <= 0) ctx->cfg.conv_aa_budget = 240;
     if (ctx->cfg.block != 64 && ctx->cfg.block != 128 && ctx->cfg.block != 256) {
         delete ctx;
         return RB_EINVAL;   // the per-box kernel is instantiated for 64, 128 and 256 threads
@@ -293,7 +295,12 @@ static int converge_impl(rb_ctx *ctx, const rb_grid *g, double dt, double tol, i
     const int nblocks = ctx->n_cu * rb::conv_blocks_per_cu(ctx->cfg.block, accel);
     const int nblocks_aa = ctx->n_cu * rb::conv_blocks_per_cu(ctx->cfg.block, accel, true);
     if ((rc = conv_buffers(ctx, (size_t)nbox, accel))) return rc;
-    rb::ConvArgs cv{tol, maxiter, ctx->d_queue, iters, accel, accel ? ctx->d_aa : nullptr,
+    // accelerated solves cap the sweeps at conv_aa_start + conv_aa_budget: a box that the
+    // mixing has not brought to the fixed point by then only costs more accelerated sweeps
+    const int32_t cap = accel ? (int32_t)std::min<int64_t>(maxiter, (int64_t)ctx->cfg.conv_aa_start +
+                                                                          ctx->cfg.conv_aa_budget)
+                              : maxiter;
+    rb::ConvArgs cv{tol, cap, ctx->d_queue, iters, accel, accel ? ctx->d_aa : nullptr,
                     ctx->cfg.conv_refill, ctx->d_rec, ctx->cfg.conv_aa_start,
                     accel ? ctx->d_spill : nullptr, accept_tol};
 

@@ -429,6 +429,27 @@ def test_surrogate_keeps_exactly_the_fixed_points(table):
         assert np.array_equal(sur[k][:, rej], conv[k][:, rej]), k
 
 
+def test_anderson_budget_caps_the_sweeps(table):
+    """With Anderson mixing the sweep cap is min(maxiter, aa_start + aa_budget): boxes that
+    converge by then are untouched, the others fall back to the 5-sweep answer with the
+    capped count, exactly as a plain maxiter does."""
+    from reprobus_amd import synth
+
+    inp = synth.make_c1(seed=5, nlon=6, nlat=5)
+    kw = dict(tol=1e-12, maxiter=20000, conserv=False, accel=2, aa_start=10)
+    full = cport.chem_converge(inp, *table, aa_budget=0, **kw)
+    cap = cport.chem_converge(inp, *table, aa_budget=5, **kw)
+    f, c = full["iters"], cap["iters"]
+    within = np.abs(f) <= 15
+    assert 0 < within.mean() < 1
+    assert np.array_equal(c[within], f[within])
+    assert np.all(c[~within] == -15)
+    ref5 = cport.chem_step(inp, *table, conserv=False)
+    for k in ("qj1_chem", "hc_chem"):
+        assert np.array_equal(cap[k][:, within], full[k][:, within]), k
+        assert np.array_equal(cap[k][:, ~within], ref5[k][:, ~within]), k
+
+
 def _inputs_fixture():
     import os
 
