@@ -42,13 +42,15 @@ BYTES_PER_BOX = (65 + 62) * 8
 # restatement with a counting number type (oracle/opcount.cpp): 2,201 add, 3,544 mul,
 # 595 div, 13 sqrt; plus 166 calls of exp/log/pow/trig, counted as one operation each.
 FLOP_PER_BOX = 2201 + 3544 + 595 + 13 + 166
-# Measured HBM bytes per k_chem launch on this workload, per block size (FETCH_SIZE x 2
-# + WRITE_SIZE, KiB, the factors from a 1 GiB copy of known traffic;
-# profiles/k_chem_c1_pmc.md).
-TRAFFIC_BYTES = {128: (2 * 501848 + 980881) * 1024, 256: (2 * 495735 + 955507) * 1024}
-# FP64 VALU instructions per k_chem launch (SQ_INSTS_VALU_*_F64, same profile, equal for
-# both block sizes): add + mul + transcendental + 2 x fma, per lane
-FP64_INSTR_FLOP = (58.116e6 + 53.457e6 + 7.444e6 + 2 * 75.381e6) * 64
+# Measured HBM bytes per step of the per-box kernels on this workload (k_setup + k_chem),
+# per block size: FETCH_SIZE x 2 + WRITE_SIZE in KiB, the factors from a 1 GiB copy of
+# known traffic (profiles/k_chem_c1_pmc.md).  Block 128 was not measured (null).
+TRAFFIC_BYTES = {256: (2 * (50254 + 834572) + 1022726 + 778767) * 1024,
+                 64: (2 * (48339 + 919144) + 1022726 + 781838) * 1024}
+# FP64 VALU instructions per step of the two kernels (SQ_INSTS_VALU_*_F64, same profile,
+# equal for every block size): add + mul + transcendental + 2 x fma, per lane
+FP64_INSTR_FLOP = ((34.387e6 + 23.730e6) + (19.991e6 + 33.606e6) + (2.318e6 + 5.126e6)
+                   + 2 * (44.572e6 + 30.810e6)) * 64
 
 
 def parse():
@@ -529,7 +531,8 @@ def main():
     traffic = TRAFFIC_BYTES.get(block)
     roofline = {"bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": traffic,
-                "kernel": f"k_chem<{block}>", "kernel_ms": chem_ms, "bytes_per_box": BYTES_PER_BOX,
+                "kernel": f"k_setup<{block}> + k_chem<{block}> (the step's per-box kernels)",
+                "kernel_ms": chem_ms, "bytes_per_box": BYTES_PER_BOX,
                 "traffic_gbs": (traffic / (chem_ms / 1e3) / 1e9) if traffic else None,
                 "fp64": {"algorithmic_tflops": nbox * FLOP_PER_BOX / (chem_ms / 1e3) / 1e12,
                          "instruction_tflops": FP64_INSTR_FLOP / (chem_ms / 1e3) / 1e12,

@@ -47,7 +47,8 @@ typedef struct {
     int32_t niter;          /* fixed sweeps per step; the reference uses 5 (F:630) */
     int32_t check_finite;   /* 1: count non-finite outputs and return RB_ENONFINITE */
     int32_t block;          /* threads per block of the per-box kernel: 64, 128, 256 (0: 256) */
-    int32_t profile;        /* 1: time every launch of the per-box kernel with HIP events */
+    int32_t profile;        /* 1: time every step's per-box kernels (set-up and sweeps) with
+                               HIP events */
     int32_t conv_refill;    /* converged mode: idle lanes of a wave that take new boxes
                                together (1..64; 0: 16) -- one set-up pass serves them all */
     int32_t conv_aa_start;  /* converged mode with accel: plain sweeps before this sweep
@@ -62,7 +63,7 @@ typedef struct {
     int64_t nonfinite;      /* non-finite outputs of the last checked step */
     double last_kernel_ms;  /* device time of the last step's kernels */
     double h2d_ms, d2h_ms;  /* copy times of the last host-array call */
-    double chem_ms_total;   /* profile mode: summed device time of the per-box kernel ... */
+    double chem_ms_total;   /* profile mode: summed device time of the per-box kernels ... */
     int64_t chem_launches;  /* ... over this many launches since the previous rb_get_stats */
 } rb_stats_t;
 

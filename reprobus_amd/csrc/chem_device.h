@@ -56,7 +56,7 @@ struct ConvArgs {
     int32_t accel;                     // Anderson depth: 0 (plain sweeps) or 2
     double *aa_ws;                     // accel > 0: (2 (accel + 1) + 1) * 57 doubles per lane
     int32_t refill;                    // idle lanes of a wave before they take new boxes
-    const double *rec;                 // per-box set-up records (launch_conv_setup)
+    const double *rec;                 // per-box set-up records (launch_setup)
     int32_t aa_start;                  // accel > 0: plain sweeps before this sweep number
     double *spill;                     // accel > 0: boxes handed from the sweep phase to
                                        // the Anderson phase, CONV_SPILL doubles per box
@@ -65,9 +65,9 @@ struct ConvArgs {
                                        // sweep from the guess tests it at accept_tol; a
                                        // box under it keeps the guess, count 0
 };
-// doubles per box of the converged mode's set-up record
+// doubles per box of the set-up record
 constexpr int CONV_RECORD = 166;
-hipError_t launch_conv_setup(const StepArgs &a, double *rec, int block, hipStream_t s);
+hipError_t launch_setup(const StepArgs &a, double *rec, int block, hipStream_t s);
 // Anderson depths the converged kernel is built for
 constexpr int AA_DEPTH = 2;
 constexpr int AA_NX = 57;
@@ -95,7 +95,7 @@ hipError_t launch_transport_interp(const TransportGrid &t, const int32_t *idx, c
 
 Troe troe_constants();
 hipError_t launch_columns(const StepArgs &a, double *col, hipStream_t s);
-hipError_t launch_chem(const StepArgs &a, int block, hipStream_t s);
+hipError_t launch_chem(const StepArgs &a, const double *rec, int block, hipStream_t s);
 // plain: one launch of nblocks_sweep blocks; accelerated: the sweep phase on nblocks_sweep
 // blocks, then the Anderson phase on nblocks_aa (the Anderson workspace holds its lanes)
 hipError_t launch_chem_conv(const StepArgs &a, const ConvArgs &cv, int block, int nblocks_sweep,

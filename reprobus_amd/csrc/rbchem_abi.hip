@@ -204,6 +204,7 @@ int rb_chem_step_dev(rb_ctx *ctx, const rb_grid *g, double dt, const double *T, 
     const int64_t field = plane * g->niv;
     const int64_t nbox = plane * g->nivbas;
     if ((rc = ensure(ctx, &ctx->d_col, &ctx->col_cap, (size_t)(3 * nbox)))) return rc;
+    if ((rc = conv_buffers(ctx, (size_t)nbox, 0))) return rc;        // the set-up record
 
     rb::StepArgs a{};
     a.T = T; a.p = p; a.sza = sza; a.alt = alt; a.col = ctx->d_col;
@@ -227,10 +228,12 @@ int rb_chem_step_dev(rb_ctx *ctx, const rb_grid *g, double dt, const double *T, 
         }
         auto &pe = ctx->prof[ctx->prof_used++];
         HIPCHK(hipEventRecord(pe.first, ctx->stream));
-        HIPCHK(rb::launch_chem(a, ctx->cfg.block, ctx->stream));
+        HIPCHK(rb::launch_setup(a, ctx->d_rec, ctx->cfg.block, ctx->stream));
+        HIPCHK(rb::launch_chem(a, ctx->d_rec, ctx->cfg.block, ctx->stream));
         HIPCHK(hipEventRecord(pe.second, ctx->stream));
     } else {
-        HIPCHK(rb::launch_chem(a, ctx->cfg.block, ctx->stream));
+        HIPCHK(rb::launch_setup(a, ctx->d_rec, ctx->cfg.block, ctx->stream));
+        HIPCHK(rb::launch_chem(a, ctx->d_rec, ctx->cfg.block, ctx->stream));
     }
     if (fuse_conserv && field > nbox)   // floor of the levels below the chemistry
         HIPCHK(rb::launch_conserv_floor(qj1, hc, plane, field, g->nivbas, nbox, ctx->stream));
@@ -306,7 +309,7 @@ static int converge_impl(rb_ctx *ctx, const rb_grid *g, double dt, double tol, i
 
     if (!ctx->capturing) HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
     HIPCHK(rb::launch_columns(a, ctx->d_col, ctx->stream));
-    HIPCHK(rb::launch_conv_setup(a, ctx->d_rec, ctx->cfg.block, ctx->stream));
+    HIPCHK(rb::launch_setup(a, ctx->d_rec, ctx->cfg.block, ctx->stream));
     HIPCHK(rb::launch_zero_counter(ctx->d_queue, ctx->stream, 3));
     if (ctx->cfg.profile && !ctx->capturing) {
         if (ctx->prof_used == ctx->prof.size()) {
@@ -722,7 +725,7 @@ int rb_run_day_dev(rb_ctx *ctx, const rb_grid *g, const rb_day_config *dc, const
     // scratch the steps use must exist before capture (no allocation inside it)
     const size_t nbox = (size_t)g->nlon * g->nlat * g->nivbas;
     if ((rc = ensure(ctx, &ctx->d_col, &ctx->col_cap, 3 * nbox))) return rc;
-    if (dc->converged && (rc = conv_buffers(ctx, nbox, dc->accel))) return rc;
+    if ((rc = conv_buffers(ctx, nbox, dc->converged ? dc->accel : 0))) return rc;
     // the graph is rebuilt when anything it was captured with changes
     std::vector<char> key;
     auto add = [&](const void *v, size_t n) {
