@@ -66,7 +66,7 @@ struct ConvArgs {
                                        // box under it keeps the guess, count 0
 };
 // doubles per box of the set-up record
-constexpr int CONV_RECORD = 166;
+constexpr int CONV_RECORD = 141;
 hipError_t launch_setup(const StepArgs &a, double *rec, int block, hipStream_t s);
 // Anderson depths the converged kernel is built for
 constexpr int AA_DEPTH = 2;

@@ -1032,358 +1032,257 @@ __device__ __forceinline__ void box_writeback(const StepArgs &a, int64_t b, cons
 }
 
 
-// Per-box set-up record: everything the set-up computes that the sweeps read (densities,
-// heterogeneous and rain-out rates, thermal rates, J-values, the aerosol outputs, the day
-// and level flags), stored field-major (rec[f * nbox + b]) so that a wave's loads and
-// stores of one field coalesce.  166 doubles per box.  Every field is written and read
-// unconditionally: with the J-values and aerosol terms stored only where they are not
-// zero, the loads sat behind branches and the sweep kernel took 0.2 ms longer.
-constexpr int NREC = 166;
+// Per-box set-up record: everything the set-up computes that the sweeps read, once each,
+// stored field-major (rec[f * nbox + b]) so that a wave's loads and stores of one field
+// coalesce -- the air density (the other densities are fixed fractions of it), the
+// rain-out rate, the day and level flags, the 82 thermal rates, the 41 J-values and the 15
+// aerosol terms: 141 doubles per box.  Every field is written and read unconditionally:
+// with the J-values and aerosol terms stored only where they are not zero, the loads sat
+// behind branches and the sweep kernel took 0.2 ms longer.
+constexpr int REC_HNM = 0, REC_RAIN = 1, REC_FLAGS = 2, REC_R = 3, REC_J = REC_R + 82,
+              REC_H = REC_J + NJ, NREC = REC_H + 15;
 static_assert(NJ == 41, "record layout assumes 41 J-values");
 static_assert(NREC == CONV_RECORD, "record size");
+static_assert(sizeof(Rates) == 82 * sizeof(double), "record layout assumes 82 rates");
 
-// Fields lo <= f < hi of the record (one stage of the set-up at a time).
+// Fields LO <= f < HI of the record (one stage of the set-up at a time).
 template <int LO, int HI>
 __device__ __forceinline__ void rec_store(const Box &x, double *__restrict__ rec, int64_t nbox,
                                           int64_t b)
 {
-    if (0 >= LO && 0 < HI) rec[(int64_t)0 * nbox + b] = x.e.hnm;
-    if (1 >= LO && 1 < HI) rec[(int64_t)1 * nbox + b] = x.e.n2;
-    if (2 >= LO && 2 < HI) rec[(int64_t)2 * nbox + b] = x.e.o2;
-    if (3 >= LO && 3 < HI) rec[(int64_t)3 * nbox + b] = x.e.co2;
-    if (4 >= LO && 4 < HI) rec[(int64_t)4 * nbox + b] = x.e.h2;
-    if (5 >= LO && 5 < HI) rec[(int64_t)5 * nbox + b] = x.e.dt;
-    if (6 >= LO && 6 < HI) rec[(int64_t)6 * nbox + b] = x.e.het1;
-    if (7 >= LO && 7 < HI) rec[(int64_t)7 * nbox + b] = x.e.het2;
-    if (8 >= LO && 8 < HI) rec[(int64_t)8 * nbox + b] = x.e.het3;
-    if (9 >= LO && 9 < HI) rec[(int64_t)9 * nbox + b] = x.e.het4;
-    if (10 >= LO && 10 < HI) rec[(int64_t)10 * nbox + b] = x.e.het5;
-    if (11 >= LO && 11 < HI) rec[(int64_t)11 * nbox + b] = x.e.het6;
-    if (12 >= LO && 12 < HI) rec[(int64_t)12 * nbox + b] = x.e.het7;
-    if (13 >= LO && 13 < HI) rec[(int64_t)13 * nbox + b] = x.e.het8;
-    if (14 >= LO && 14 < HI) rec[(int64_t)14 * nbox + b] = x.e.het9;
-    if (15 >= LO && 15 < HI) rec[(int64_t)15 * nbox + b] = x.e.pno3;
-    if (16 >= LO && 16 < HI) rec[(int64_t)16 * nbox + b] = x.e.ph2o;
-    if (17 >= LO && 17 < HI) rec[(int64_t)17 * nbox + b] = x.e.phcl;
-    if (18 >= LO && 18 < HI) rec[(int64_t)18 * nbox + b] = x.e.pbr;
-    if (19 >= LO && 19 < HI) rec[(int64_t)19 * nbox + b] = x.e.rain_hno3;
-    if (20 >= LO && 20 < HI) rec[(int64_t)20 * nbox + b] = x.e.rain_hno4;
-    if (21 >= LO && 21 < HI) rec[(int64_t)21 * nbox + b] = x.e.rain_hcl;
-    if (22 >= LO && 22 < HI) rec[(int64_t)22 * nbox + b] = x.e.rain_hbr;
-    if (23 >= LO && 23 < HI) rec[(int64_t)23 * nbox + b] = x.e.rain_h2o2;
-    if (24 >= LO && 24 < HI) rec[(int64_t)24 * nbox + b] = x.e.rain_ch2o;
-    if (25 >= LO && 25 < HI) rec[(int64_t)25 * nbox + b] = x.e.rain_ch3o2h;
-    if (26 >= LO && 26 < HI) rec[(int64_t)26 * nbox + b] = x.R.a1;
-    if (27 >= LO && 27 < HI) rec[(int64_t)27 * nbox + b] = x.R.a1et;
-    if (28 >= LO && 28 < HI) rec[(int64_t)28 * nbox + b] = x.R.a2;
-    if (29 >= LO && 29 < HI) rec[(int64_t)29 * nbox + b] = x.R.a5;
-    if (30 >= LO && 30 < HI) rec[(int64_t)30 * nbox + b] = x.R.a6;
-    if (31 >= LO && 31 < HI) rec[(int64_t)31 * nbox + b] = x.R.a6b;
-    if (32 >= LO && 32 < HI) rec[(int64_t)32 * nbox + b] = x.R.a7;
-    if (33 >= LO && 33 < HI) rec[(int64_t)33 * nbox + b] = x.R.a17;
-    if (34 >= LO && 34 < HI) rec[(int64_t)34 * nbox + b] = x.R.a19;
-    if (35 >= LO && 35 < HI) rec[(int64_t)35 * nbox + b] = x.R.a26;
-    if (36 >= LO && 36 < HI) rec[(int64_t)36 * nbox + b] = x.R.a27;
-    if (37 >= LO && 37 < HI) rec[(int64_t)37 * nbox + b] = x.R.a36;
-    if (38 >= LO && 38 < HI) rec[(int64_t)38 * nbox + b] = x.R.b3;
-    if (39 >= LO && 39 < HI) rec[(int64_t)39 * nbox + b] = x.R.b4;
-    if (40 >= LO && 40 < HI) rec[(int64_t)40 * nbox + b] = x.R.b6;
-    if (41 >= LO && 41 < HI) rec[(int64_t)41 * nbox + b] = x.R.b7;
-    if (42 >= LO && 42 < HI) rec[(int64_t)42 * nbox + b] = x.R.b9;
-    if (43 >= LO && 43 < HI) rec[(int64_t)43 * nbox + b] = x.R.b12;
-    if (44 >= LO && 44 < HI) rec[(int64_t)44 * nbox + b] = x.R.b22;
-    if (45 >= LO && 45 < HI) rec[(int64_t)45 * nbox + b] = x.R.b23;
-    if (46 >= LO && 46 < HI) rec[(int64_t)46 * nbox + b] = x.R.b24;
-    if (47 >= LO && 47 < HI) rec[(int64_t)47 * nbox + b] = x.R.b27;
-    if (48 >= LO && 48 < HI) rec[(int64_t)48 * nbox + b] = x.R.b28;
-    if (49 >= LO && 49 < HI) rec[(int64_t)49 * nbox + b] = x.R.b32;
-    if (50 >= LO && 50 < HI) rec[(int64_t)50 * nbox + b] = x.R.b38;
-    if (51 >= LO && 51 < HI) rec[(int64_t)51 * nbox + b] = x.R.b39;
-    if (52 >= LO && 52 < HI) rec[(int64_t)52 * nbox + b] = x.R.c2;
-    if (53 >= LO && 53 < HI) rec[(int64_t)53 * nbox + b] = x.R.c4;
-    if (54 >= LO && 54 < HI) rec[(int64_t)54 * nbox + b] = x.R.c5;
-    if (55 >= LO && 55 < HI) rec[(int64_t)55 * nbox + b] = x.R.c7;
-    if (56 >= LO && 56 < HI) rec[(int64_t)56 * nbox + b] = x.R.c8;
-    if (57 >= LO && 57 < HI) rec[(int64_t)57 * nbox + b] = x.R.c9;
-    if (58 >= LO && 58 < HI) rec[(int64_t)58 * nbox + b] = x.R.c10;
-    if (59 >= LO && 59 < HI) rec[(int64_t)59 * nbox + b] = x.R.c14;
-    if (60 >= LO && 60 < HI) rec[(int64_t)60 * nbox + b] = x.R.c15;
-    if (61 >= LO && 61 < HI) rec[(int64_t)61 * nbox + b] = x.R.c17a;
-    if (62 >= LO && 62 < HI) rec[(int64_t)62 * nbox + b] = x.R.c17b;
-    if (63 >= LO && 63 < HI) rec[(int64_t)63 * nbox + b] = x.R.c18;
-    if (64 >= LO && 64 < HI) rec[(int64_t)64 * nbox + b] = x.R.c19;
-    if (65 >= LO && 65 < HI) rec[(int64_t)65 * nbox + b] = x.R.d2;
-    if (66 >= LO && 66 < HI) rec[(int64_t)66 * nbox + b] = x.R.d3;
-    if (67 >= LO && 67 < HI) rec[(int64_t)67 * nbox + b] = x.R.d4;
-    if (68 >= LO && 68 < HI) rec[(int64_t)68 * nbox + b] = x.R.d5;
-    if (69 >= LO && 69 < HI) rec[(int64_t)69 * nbox + b] = x.R.d6;
-    if (70 >= LO && 70 < HI) rec[(int64_t)70 * nbox + b] = x.R.d7;
-    if (71 >= LO && 71 < HI) rec[(int64_t)71 * nbox + b] = x.R.d8;
-    if (72 >= LO && 72 < HI) rec[(int64_t)72 * nbox + b] = x.R.d9;
-    if (73 >= LO && 73 < HI) rec[(int64_t)73 * nbox + b] = x.R.d10;
-    if (74 >= LO && 74 < HI) rec[(int64_t)74 * nbox + b] = x.R.d11;
-    if (75 >= LO && 75 < HI) rec[(int64_t)75 * nbox + b] = x.R.d31;
-    if (76 >= LO && 76 < HI) rec[(int64_t)76 * nbox + b] = x.R.d32;
-    if (77 >= LO && 77 < HI) rec[(int64_t)77 * nbox + b] = x.R.d33;
-    if (78 >= LO && 78 < HI) rec[(int64_t)78 * nbox + b] = x.R.d34;
-    if (79 >= LO && 79 < HI) rec[(int64_t)79 * nbox + b] = x.R.d36;
-    if (80 >= LO && 80 < HI) rec[(int64_t)80 * nbox + b] = x.R.d37;
-    if (81 >= LO && 81 < HI) rec[(int64_t)81 * nbox + b] = x.R.d60;
-    if (82 >= LO && 82 < HI) rec[(int64_t)82 * nbox + b] = x.R.d61;
-    if (83 >= LO && 83 < HI) rec[(int64_t)83 * nbox + b] = x.R.d62;
-    if (84 >= LO && 84 < HI) rec[(int64_t)84 * nbox + b] = x.R.d94;
-    if (85 >= LO && 85 < HI) rec[(int64_t)85 * nbox + b] = x.R.d95;
-    if (86 >= LO && 86 < HI) rec[(int64_t)86 * nbox + b] = x.R.d97;
-    if (87 >= LO && 87 < HI) rec[(int64_t)87 * nbox + b] = x.R.e2;
-    if (88 >= LO && 88 < HI) rec[(int64_t)88 * nbox + b] = x.R.e3;
-    if (89 >= LO && 89 < HI) rec[(int64_t)89 * nbox + b] = x.R.e4;
-    if (90 >= LO && 90 < HI) rec[(int64_t)90 * nbox + b] = x.R.e5a;
-    if (91 >= LO && 91 < HI) rec[(int64_t)91 * nbox + b] = x.R.e5b;
-    if (92 >= LO && 92 < HI) rec[(int64_t)92 * nbox + b] = x.R.e5c;
-    if (93 >= LO && 93 < HI) rec[(int64_t)93 * nbox + b] = x.R.e6;
-    if (94 >= LO && 94 < HI) rec[(int64_t)94 * nbox + b] = x.R.e7;
-    if (95 >= LO && 95 < HI) rec[(int64_t)95 * nbox + b] = x.R.e8;
-    if (96 >= LO && 96 < HI) rec[(int64_t)96 * nbox + b] = x.R.e10;
-    if (97 >= LO && 97 < HI) rec[(int64_t)97 * nbox + b] = x.R.e11;
-    if (98 >= LO && 98 < HI) rec[(int64_t)98 * nbox + b] = x.R.e13;
-    if (99 >= LO && 99 < HI) rec[(int64_t)99 * nbox + b] = x.R.e15;
-    if (100 >= LO && 100 < HI) rec[(int64_t)100 * nbox + b] = x.R.e16;
-    if (101 >= LO && 101 < HI) rec[(int64_t)101 * nbox + b] = x.R.e51;
-    if (102 >= LO && 102 < HI) rec[(int64_t)102 * nbox + b] = x.R.e54;
-    if (103 >= LO && 103 < HI) rec[(int64_t)103 * nbox + b] = x.R.hk1;
-    if (104 >= LO && 104 < HI) rec[(int64_t)104 * nbox + b] = x.R.hk2;
-    if (105 >= LO && 105 < HI) rec[(int64_t)105 * nbox + b] = x.R.hk3;
-    if (106 >= LO && 106 < HI) rec[(int64_t)106 * nbox + b] = x.R.hk4;
-    if (107 >= LO && 107 < HI) rec[(int64_t)107 * nbox + b] = x.R.hk5;
-    if (108 >= LO && 108 < HI) rec[(int64_t)108 * nbox + b] = x.tj[0];
-    if (109 >= LO && 109 < HI) rec[(int64_t)109 * nbox + b] = x.tj[1];
-    if (110 >= LO && 110 < HI) rec[(int64_t)110 * nbox + b] = x.tj[2];
-    if (111 >= LO && 111 < HI) rec[(int64_t)111 * nbox + b] = x.tj[3];
-    if (112 >= LO && 112 < HI) rec[(int64_t)112 * nbox + b] = x.tj[4];
-    if (113 >= LO && 113 < HI) rec[(int64_t)113 * nbox + b] = x.tj[5];
-    if (114 >= LO && 114 < HI) rec[(int64_t)114 * nbox + b] = x.tj[6];
-    if (115 >= LO && 115 < HI) rec[(int64_t)115 * nbox + b] = x.tj[7];
-    if (116 >= LO && 116 < HI) rec[(int64_t)116 * nbox + b] = x.tj[8];
-    if (117 >= LO && 117 < HI) rec[(int64_t)117 * nbox + b] = x.tj[9];
-    if (118 >= LO && 118 < HI) rec[(int64_t)118 * nbox + b] = x.tj[10];
-    if (119 >= LO && 119 < HI) rec[(int64_t)119 * nbox + b] = x.tj[11];
-    if (120 >= LO && 120 < HI) rec[(int64_t)120 * nbox + b] = x.tj[12];
-    if (121 >= LO && 121 < HI) rec[(int64_t)121 * nbox + b] = x.tj[13];
-    if (122 >= LO && 122 < HI) rec[(int64_t)122 * nbox + b] = x.tj[14];
-    if (123 >= LO && 123 < HI) rec[(int64_t)123 * nbox + b] = x.tj[15];
-    if (124 >= LO && 124 < HI) rec[(int64_t)124 * nbox + b] = x.tj[16];
-    if (125 >= LO && 125 < HI) rec[(int64_t)125 * nbox + b] = x.tj[17];
-    if (126 >= LO && 126 < HI) rec[(int64_t)126 * nbox + b] = x.tj[18];
-    if (127 >= LO && 127 < HI) rec[(int64_t)127 * nbox + b] = x.tj[19];
-    if (128 >= LO && 128 < HI) rec[(int64_t)128 * nbox + b] = x.tj[20];
-    if (129 >= LO && 129 < HI) rec[(int64_t)129 * nbox + b] = x.tj[21];
-    if (130 >= LO && 130 < HI) rec[(int64_t)130 * nbox + b] = x.tj[22];
-    if (131 >= LO && 131 < HI) rec[(int64_t)131 * nbox + b] = x.tj[23];
-    if (132 >= LO && 132 < HI) rec[(int64_t)132 * nbox + b] = x.tj[24];
-    if (133 >= LO && 133 < HI) rec[(int64_t)133 * nbox + b] = x.tj[25];
-    if (134 >= LO && 134 < HI) rec[(int64_t)134 * nbox + b] = x.tj[26];
-    if (135 >= LO && 135 < HI) rec[(int64_t)135 * nbox + b] = x.tj[27];
-    if (136 >= LO && 136 < HI) rec[(int64_t)136 * nbox + b] = x.tj[28];
-    if (137 >= LO && 137 < HI) rec[(int64_t)137 * nbox + b] = x.tj[29];
-    if (138 >= LO && 138 < HI) rec[(int64_t)138 * nbox + b] = x.tj[30];
-    if (139 >= LO && 139 < HI) rec[(int64_t)139 * nbox + b] = x.tj[31];
-    if (140 >= LO && 140 < HI) rec[(int64_t)140 * nbox + b] = x.tj[32];
-    if (141 >= LO && 141 < HI) rec[(int64_t)141 * nbox + b] = x.tj[33];
-    if (142 >= LO && 142 < HI) rec[(int64_t)142 * nbox + b] = x.tj[34];
-    if (143 >= LO && 143 < HI) rec[(int64_t)143 * nbox + b] = x.tj[35];
-    if (144 >= LO && 144 < HI) rec[(int64_t)144 * nbox + b] = x.tj[36];
-    if (145 >= LO && 145 < HI) rec[(int64_t)145 * nbox + b] = x.tj[37];
-    if (146 >= LO && 146 < HI) rec[(int64_t)146 * nbox + b] = x.tj[38];
-    if (147 >= LO && 147 < HI) rec[(int64_t)147 * nbox + b] = x.tj[39];
-    if (148 >= LO && 148 < HI) rec[(int64_t)148 * nbox + b] = x.tj[40];
-    if (149 >= LO && 149 < HI) rec[(int64_t)149 * nbox + b] = x.h.k[0];
-    if (150 >= LO && 150 < HI) rec[(int64_t)150 * nbox + b] = x.h.k[1];
-    if (151 >= LO && 151 < HI) rec[(int64_t)151 * nbox + b] = x.h.k[2];
-    if (152 >= LO && 152 < HI) rec[(int64_t)152 * nbox + b] = x.h.k[3];
-    if (153 >= LO && 153 < HI) rec[(int64_t)153 * nbox + b] = x.h.k[4];
-    if (154 >= LO && 154 < HI) rec[(int64_t)154 * nbox + b] = x.h.k[5];
-    if (155 >= LO && 155 < HI) rec[(int64_t)155 * nbox + b] = x.h.k[6];
-    if (156 >= LO && 156 < HI) rec[(int64_t)156 * nbox + b] = x.h.k[7];
-    if (157 >= LO && 157 < HI) rec[(int64_t)157 * nbox + b] = x.h.k[8];
-    if (158 >= LO && 158 < HI) rec[(int64_t)158 * nbox + b] = x.h.pno3;
-    if (159 >= LO && 159 < HI) rec[(int64_t)159 * nbox + b] = x.h.pno3sl;
-    if (160 >= LO && 160 < HI) rec[(int64_t)160 * nbox + b] = x.h.ph2o;
-    if (161 >= LO && 161 < HI) rec[(int64_t)161 * nbox + b] = x.h.phcl;
-    if (162 >= LO && 162 < HI) rec[(int64_t)162 * nbox + b] = x.h.surf;
-    if (163 >= LO && 163 < HI) rec[(int64_t)163 * nbox + b] = x.h.vsed;
-    if (164 >= LO && 164 < HI) rec[(int64_t)164 * nbox + b] = x.e.day ? 1. : 0.;
-    if (165 >= LO && 165 < HI) rec[(int64_t)165 * nbox + b] = x.het_level ? 1. : 0.;
+#define REC_PUT(f, v) \
+    if ((f) >= LO && (f) < HI) rec[(int64_t)(f) * nbox + b] = (v)
+    REC_PUT(REC_HNM, x.e.hnm);
+    REC_PUT(REC_RAIN, x.e.rain_hno3);
+    REC_PUT(REC_FLAGS, (x.e.day ? 1. : 0.) + (x.het_level ? 2. : 0.));
+    REC_PUT(REC_R + 0, x.R.a1);
+    REC_PUT(REC_R + 1, x.R.a1et);
+    REC_PUT(REC_R + 2, x.R.a2);
+    REC_PUT(REC_R + 3, x.R.a5);
+    REC_PUT(REC_R + 4, x.R.a6);
+    REC_PUT(REC_R + 5, x.R.a6b);
+    REC_PUT(REC_R + 6, x.R.a7);
+    REC_PUT(REC_R + 7, x.R.a17);
+    REC_PUT(REC_R + 8, x.R.a19);
+    REC_PUT(REC_R + 9, x.R.a26);
+    REC_PUT(REC_R + 10, x.R.a27);
+    REC_PUT(REC_R + 11, x.R.a36);
+    REC_PUT(REC_R + 12, x.R.b3);
+    REC_PUT(REC_R + 13, x.R.b4);
+    REC_PUT(REC_R + 14, x.R.b6);
+    REC_PUT(REC_R + 15, x.R.b7);
+    REC_PUT(REC_R + 16, x.R.b9);
+    REC_PUT(REC_R + 17, x.R.b12);
+    REC_PUT(REC_R + 18, x.R.b22);
+    REC_PUT(REC_R + 19, x.R.b23);
+    REC_PUT(REC_R + 20, x.R.b24);
+    REC_PUT(REC_R + 21, x.R.b27);
+    REC_PUT(REC_R + 22, x.R.b28);
+    REC_PUT(REC_R + 23, x.R.b32);
+    REC_PUT(REC_R + 24, x.R.b38);
+    REC_PUT(REC_R + 25, x.R.b39);
+    REC_PUT(REC_R + 26, x.R.c2);
+    REC_PUT(REC_R + 27, x.R.c4);
+    REC_PUT(REC_R + 28, x.R.c5);
+    REC_PUT(REC_R + 29, x.R.c7);
+    REC_PUT(REC_R + 30, x.R.c8);
+    REC_PUT(REC_R + 31, x.R.c9);
+    REC_PUT(REC_R + 32, x.R.c10);
+    REC_PUT(REC_R + 33, x.R.c14);
+    REC_PUT(REC_R + 34, x.R.c15);
+    REC_PUT(REC_R + 35, x.R.c17a);
+    REC_PUT(REC_R + 36, x.R.c17b);
+    REC_PUT(REC_R + 37, x.R.c18);
+    REC_PUT(REC_R + 38, x.R.c19);
+    REC_PUT(REC_R + 39, x.R.d2);
+    REC_PUT(REC_R + 40, x.R.d3);
+    REC_PUT(REC_R + 41, x.R.d4);
+    REC_PUT(REC_R + 42, x.R.d5);
+    REC_PUT(REC_R + 43, x.R.d6);
+    REC_PUT(REC_R + 44, x.R.d7);
+    REC_PUT(REC_R + 45, x.R.d8);
+    REC_PUT(REC_R + 46, x.R.d9);
+    REC_PUT(REC_R + 47, x.R.d10);
+    REC_PUT(REC_R + 48, x.R.d11);
+    REC_PUT(REC_R + 49, x.R.d31);
+    REC_PUT(REC_R + 50, x.R.d32);
+    REC_PUT(REC_R + 51, x.R.d33);
+    REC_PUT(REC_R + 52, x.R.d34);
+    REC_PUT(REC_R + 53, x.R.d36);
+    REC_PUT(REC_R + 54, x.R.d37);
+    REC_PUT(REC_R + 55, x.R.d60);
+    REC_PUT(REC_R + 56, x.R.d61);
+    REC_PUT(REC_R + 57, x.R.d62);
+    REC_PUT(REC_R + 58, x.R.d94);
+    REC_PUT(REC_R + 59, x.R.d95);
+    REC_PUT(REC_R + 60, x.R.d97);
+    REC_PUT(REC_R + 61, x.R.e2);
+    REC_PUT(REC_R + 62, x.R.e3);
+    REC_PUT(REC_R + 63, x.R.e4);
+    REC_PUT(REC_R + 64, x.R.e5a);
+    REC_PUT(REC_R + 65, x.R.e5b);
+    REC_PUT(REC_R + 66, x.R.e5c);
+    REC_PUT(REC_R + 67, x.R.e6);
+    REC_PUT(REC_R + 68, x.R.e7);
+    REC_PUT(REC_R + 69, x.R.e8);
+    REC_PUT(REC_R + 70, x.R.e10);
+    REC_PUT(REC_R + 71, x.R.e11);
+    REC_PUT(REC_R + 72, x.R.e13);
+    REC_PUT(REC_R + 73, x.R.e15);
+    REC_PUT(REC_R + 74, x.R.e16);
+    REC_PUT(REC_R + 75, x.R.e51);
+    REC_PUT(REC_R + 76, x.R.e54);
+    REC_PUT(REC_R + 77, x.R.hk1);
+    REC_PUT(REC_R + 78, x.R.hk2);
+    REC_PUT(REC_R + 79, x.R.hk3);
+    REC_PUT(REC_R + 80, x.R.hk4);
+    REC_PUT(REC_R + 81, x.R.hk5);
+#pragma unroll
+    for (int n = 0; n < NJ; ++n) REC_PUT(REC_J + n, x.tj[n]);
+    REC_PUT(REC_H + 0, x.h.k[0]);
+    REC_PUT(REC_H + 1, x.h.k[1]);
+    REC_PUT(REC_H + 2, x.h.k[2]);
+    REC_PUT(REC_H + 3, x.h.k[3]);
+    REC_PUT(REC_H + 4, x.h.k[4]);
+    REC_PUT(REC_H + 5, x.h.k[5]);
+    REC_PUT(REC_H + 6, x.h.k[6]);
+    REC_PUT(REC_H + 7, x.h.k[7]);
+    REC_PUT(REC_H + 8, x.h.k[8]);
+    REC_PUT(REC_H + 9, x.h.pno3);
+    REC_PUT(REC_H + 10, x.h.pno3sl);
+    REC_PUT(REC_H + 11, x.h.ph2o);
+    REC_PUT(REC_H + 12, x.h.phcl);
+    REC_PUT(REC_H + 13, x.h.surf);
+    REC_PUT(REC_H + 14, x.h.vsed);
+#undef REC_PUT
 }
 
+// The set-up of box b from its record, as the set-up left it (rec_load is exact: the
+// densities are the same products of the air density, the flags small integers).
 __device__ __forceinline__ void rec_load(Box &x, const double *__restrict__ rec, int64_t nbox,
-                                         int64_t b)
+                                         int64_t b, double dt)
 {
-    x.e.hnm = rec[(int64_t)0 * nbox + b];
-    x.e.n2 = rec[(int64_t)1 * nbox + b];
-    x.e.o2 = rec[(int64_t)2 * nbox + b];
-    x.e.co2 = rec[(int64_t)3 * nbox + b];
-    x.e.h2 = rec[(int64_t)4 * nbox + b];
-    x.e.dt = rec[(int64_t)5 * nbox + b];
-    x.e.het1 = rec[(int64_t)6 * nbox + b];
-    x.e.het2 = rec[(int64_t)7 * nbox + b];
-    x.e.het3 = rec[(int64_t)8 * nbox + b];
-    x.e.het4 = rec[(int64_t)9 * nbox + b];
-    x.e.het5 = rec[(int64_t)10 * nbox + b];
-    x.e.het6 = rec[(int64_t)11 * nbox + b];
-    x.e.het7 = rec[(int64_t)12 * nbox + b];
-    x.e.het8 = rec[(int64_t)13 * nbox + b];
-    x.e.het9 = rec[(int64_t)14 * nbox + b];
-    x.e.pno3 = rec[(int64_t)15 * nbox + b];
-    x.e.ph2o = rec[(int64_t)16 * nbox + b];
-    x.e.phcl = rec[(int64_t)17 * nbox + b];
-    x.e.pbr = rec[(int64_t)18 * nbox + b];
-    x.e.rain_hno3 = rec[(int64_t)19 * nbox + b];
-    x.e.rain_hno4 = rec[(int64_t)20 * nbox + b];
-    x.e.rain_hcl = rec[(int64_t)21 * nbox + b];
-    x.e.rain_hbr = rec[(int64_t)22 * nbox + b];
-    x.e.rain_h2o2 = rec[(int64_t)23 * nbox + b];
-    x.e.rain_ch2o = rec[(int64_t)24 * nbox + b];
-    x.e.rain_ch3o2h = rec[(int64_t)25 * nbox + b];
-    x.R.a1 = rec[(int64_t)26 * nbox + b];
-    x.R.a1et = rec[(int64_t)27 * nbox + b];
-    x.R.a2 = rec[(int64_t)28 * nbox + b];
-    x.R.a5 = rec[(int64_t)29 * nbox + b];
-    x.R.a6 = rec[(int64_t)30 * nbox + b];
-    x.R.a6b = rec[(int64_t)31 * nbox + b];
-    x.R.a7 = rec[(int64_t)32 * nbox + b];
-    x.R.a17 = rec[(int64_t)33 * nbox + b];
-    x.R.a19 = rec[(int64_t)34 * nbox + b];
-    x.R.a26 = rec[(int64_t)35 * nbox + b];
-    x.R.a27 = rec[(int64_t)36 * nbox + b];
-    x.R.a36 = rec[(int64_t)37 * nbox + b];
-    x.R.b3 = rec[(int64_t)38 * nbox + b];
-    x.R.b4 = rec[(int64_t)39 * nbox + b];
-    x.R.b6 = rec[(int64_t)40 * nbox + b];
-    x.R.b7 = rec[(int64_t)41 * nbox + b];
-    x.R.b9 = rec[(int64_t)42 * nbox + b];
-    x.R.b12 = rec[(int64_t)43 * nbox + b];
-    x.R.b22 = rec[(int64_t)44 * nbox + b];
-    x.R.b23 = rec[(int64_t)45 * nbox + b];
-    x.R.b24 = rec[(int64_t)46 * nbox + b];
-    x.R.b27 = rec[(int64_t)47 * nbox + b];
-    x.R.b28 = rec[(int64_t)48 * nbox + b];
-    x.R.b32 = rec[(int64_t)49 * nbox + b];
-    x.R.b38 = rec[(int64_t)50 * nbox + b];
-    x.R.b39 = rec[(int64_t)51 * nbox + b];
-    x.R.c2 = rec[(int64_t)52 * nbox + b];
-    x.R.c4 = rec[(int64_t)53 * nbox + b];
-    x.R.c5 = rec[(int64_t)54 * nbox + b];
-    x.R.c7 = rec[(int64_t)55 * nbox + b];
-    x.R.c8 = rec[(int64_t)56 * nbox + b];
-    x.R.c9 = rec[(int64_t)57 * nbox + b];
-    x.R.c10 = rec[(int64_t)58 * nbox + b];
-    x.R.c14 = rec[(int64_t)59 * nbox + b];
-    x.R.c15 = rec[(int64_t)60 * nbox + b];
-    x.R.c17a = rec[(int64_t)61 * nbox + b];
-    x.R.c17b = rec[(int64_t)62 * nbox + b];
-    x.R.c18 = rec[(int64_t)63 * nbox + b];
-    x.R.c19 = rec[(int64_t)64 * nbox + b];
-    x.R.d2 = rec[(int64_t)65 * nbox + b];
-    x.R.d3 = rec[(int64_t)66 * nbox + b];
-    x.R.d4 = rec[(int64_t)67 * nbox + b];
-    x.R.d5 = rec[(int64_t)68 * nbox + b];
-    x.R.d6 = rec[(int64_t)69 * nbox + b];
-    x.R.d7 = rec[(int64_t)70 * nbox + b];
-    x.R.d8 = rec[(int64_t)71 * nbox + b];
-    x.R.d9 = rec[(int64_t)72 * nbox + b];
-    x.R.d10 = rec[(int64_t)73 * nbox + b];
-    x.R.d11 = rec[(int64_t)74 * nbox + b];
-    x.R.d31 = rec[(int64_t)75 * nbox + b];
-    x.R.d32 = rec[(int64_t)76 * nbox + b];
-    x.R.d33 = rec[(int64_t)77 * nbox + b];
-    x.R.d34 = rec[(int64_t)78 * nbox + b];
-    x.R.d36 = rec[(int64_t)79 * nbox + b];
-    x.R.d37 = rec[(int64_t)80 * nbox + b];
-    x.R.d60 = rec[(int64_t)81 * nbox + b];
-    x.R.d61 = rec[(int64_t)82 * nbox + b];
-    x.R.d62 = rec[(int64_t)83 * nbox + b];
-    x.R.d94 = rec[(int64_t)84 * nbox + b];
-    x.R.d95 = rec[(int64_t)85 * nbox + b];
-    x.R.d97 = rec[(int64_t)86 * nbox + b];
-    x.R.e2 = rec[(int64_t)87 * nbox + b];
-    x.R.e3 = rec[(int64_t)88 * nbox + b];
-    x.R.e4 = rec[(int64_t)89 * nbox + b];
-    x.R.e5a = rec[(int64_t)90 * nbox + b];
-    x.R.e5b = rec[(int64_t)91 * nbox + b];
-    x.R.e5c = rec[(int64_t)92 * nbox + b];
-    x.R.e6 = rec[(int64_t)93 * nbox + b];
-    x.R.e7 = rec[(int64_t)94 * nbox + b];
-    x.R.e8 = rec[(int64_t)95 * nbox + b];
-    x.R.e10 = rec[(int64_t)96 * nbox + b];
-    x.R.e11 = rec[(int64_t)97 * nbox + b];
-    x.R.e13 = rec[(int64_t)98 * nbox + b];
-    x.R.e15 = rec[(int64_t)99 * nbox + b];
-    x.R.e16 = rec[(int64_t)100 * nbox + b];
-    x.R.e51 = rec[(int64_t)101 * nbox + b];
-    x.R.e54 = rec[(int64_t)102 * nbox + b];
-    x.R.hk1 = rec[(int64_t)103 * nbox + b];
-    x.R.hk2 = rec[(int64_t)104 * nbox + b];
-    x.R.hk3 = rec[(int64_t)105 * nbox + b];
-    x.R.hk4 = rec[(int64_t)106 * nbox + b];
-    x.R.hk5 = rec[(int64_t)107 * nbox + b];
-    x.tj[0] = rec[(int64_t)108 * nbox + b];
-    x.tj[1] = rec[(int64_t)109 * nbox + b];
-    x.tj[2] = rec[(int64_t)110 * nbox + b];
-    x.tj[3] = rec[(int64_t)111 * nbox + b];
-    x.tj[4] = rec[(int64_t)112 * nbox + b];
-    x.tj[5] = rec[(int64_t)113 * nbox + b];
-    x.tj[6] = rec[(int64_t)114 * nbox + b];
-    x.tj[7] = rec[(int64_t)115 * nbox + b];
-    x.tj[8] = rec[(int64_t)116 * nbox + b];
-    x.tj[9] = rec[(int64_t)117 * nbox + b];
-    x.tj[10] = rec[(int64_t)118 * nbox + b];
-    x.tj[11] = rec[(int64_t)119 * nbox + b];
-    x.tj[12] = rec[(int64_t)120 * nbox + b];
-    x.tj[13] = rec[(int64_t)121 * nbox + b];
-    x.tj[14] = rec[(int64_t)122 * nbox + b];
-    x.tj[15] = rec[(int64_t)123 * nbox + b];
-    x.tj[16] = rec[(int64_t)124 * nbox + b];
-    x.tj[17] = rec[(int64_t)125 * nbox + b];
-    x.tj[18] = rec[(int64_t)126 * nbox + b];
-    x.tj[19] = rec[(int64_t)127 * nbox + b];
-    x.tj[20] = rec[(int64_t)128 * nbox + b];
-    x.tj[21] = rec[(int64_t)129 * nbox + b];
-    x.tj[22] = rec[(int64_t)130 * nbox + b];
-    x.tj[23] = rec[(int64_t)131 * nbox + b];
-    x.tj[24] = rec[(int64_t)132 * nbox + b];
-    x.tj[25] = rec[(int64_t)133 * nbox + b];
-    x.tj[26] = rec[(int64_t)134 * nbox + b];
-    x.tj[27] = rec[(int64_t)135 * nbox + b];
-    x.tj[28] = rec[(int64_t)136 * nbox + b];
-    x.tj[29] = rec[(int64_t)137 * nbox + b];
-    x.tj[30] = rec[(int64_t)138 * nbox + b];
-    x.tj[31] = rec[(int64_t)139 * nbox + b];
-    x.tj[32] = rec[(int64_t)140 * nbox + b];
-    x.tj[33] = rec[(int64_t)141 * nbox + b];
-    x.tj[34] = rec[(int64_t)142 * nbox + b];
-    x.tj[35] = rec[(int64_t)143 * nbox + b];
-    x.tj[36] = rec[(int64_t)144 * nbox + b];
-    x.tj[37] = rec[(int64_t)145 * nbox + b];
-    x.tj[38] = rec[(int64_t)146 * nbox + b];
-    x.tj[39] = rec[(int64_t)147 * nbox + b];
-    x.tj[40] = rec[(int64_t)148 * nbox + b];
-    x.h.k[0] = rec[(int64_t)149 * nbox + b];
-    x.h.k[1] = rec[(int64_t)150 * nbox + b];
-    x.h.k[2] = rec[(int64_t)151 * nbox + b];
-    x.h.k[3] = rec[(int64_t)152 * nbox + b];
-    x.h.k[4] = rec[(int64_t)153 * nbox + b];
-    x.h.k[5] = rec[(int64_t)154 * nbox + b];
-    x.h.k[6] = rec[(int64_t)155 * nbox + b];
-    x.h.k[7] = rec[(int64_t)156 * nbox + b];
-    x.h.k[8] = rec[(int64_t)157 * nbox + b];
-    x.h.pno3 = rec[(int64_t)158 * nbox + b];
-    x.h.pno3sl = rec[(int64_t)159 * nbox + b];
-    x.h.ph2o = rec[(int64_t)160 * nbox + b];
-    x.h.phcl = rec[(int64_t)161 * nbox + b];
-    x.h.surf = rec[(int64_t)162 * nbox + b];
-    x.h.vsed = rec[(int64_t)163 * nbox + b];
-    x.e.day = rec[(int64_t)164 * nbox + b] != 0.;
-    x.het_level = rec[(int64_t)165 * nbox + b] != 0.;
+#define REC_GET(f) rec[(int64_t)(f) * nbox + b]
+    BoxEnv &e = x.e;
+    e.hnm = REC_GET(REC_HNM);
+    e.n2 = 0.78 * e.hnm;
+    e.o2 = 0.21 * e.hnm;
+    e.co2 = 360.e-6 * e.hnm;
+    e.h2 = 5.e-7 * e.hnm;
+    e.dt = dt;
+    const double rr = REC_GET(REC_RAIN);
+    e.rain_hno3 = rr; e.rain_hcl = rr; e.rain_hbr = rr; e.rain_h2o2 = rr;
+    e.rain_hno4 = 0.; e.rain_ch2o = 0.; e.rain_ch3o2h = 0.;
+    const double fl = REC_GET(REC_FLAGS);
+    e.day = fl == 1. || fl == 3.;
+    x.het_level = fl >= 2.;
+    x.R.a1 = REC_GET(REC_R + 0);
+    x.R.a1et = REC_GET(REC_R + 1);
+    x.R.a2 = REC_GET(REC_R + 2);
+    x.R.a5 = REC_GET(REC_R + 3);
+    x.R.a6 = REC_GET(REC_R + 4);
+    x.R.a6b = REC_GET(REC_R + 5);
+    x.R.a7 = REC_GET(REC_R + 6);
+    x.R.a17 = REC_GET(REC_R + 7);
+    x.R.a19 = REC_GET(REC_R + 8);
+    x.R.a26 = REC_GET(REC_R + 9);
+    x.R.a27 = REC_GET(REC_R + 10);
+    x.R.a36 = REC_GET(REC_R + 11);
+    x.R.b3 = REC_GET(REC_R + 12);
+    x.R.b4 = REC_GET(REC_R + 13);
+    x.R.b6 = REC_GET(REC_R + 14);
+    x.R.b7 = REC_GET(REC_R + 15);
+    x.R.b9 = REC_GET(REC_R + 16);
+    x.R.b12 = REC_GET(REC_R + 17);
+    x.R.b22 = REC_GET(REC_R + 18);
+    x.R.b23 = REC_GET(REC_R + 19);
+    x.R.b24 = REC_GET(REC_R + 20);
+    x.R.b27 = REC_GET(REC_R + 21);
+    x.R.b28 = REC_GET(REC_R + 22);
+    x.R.b32 = REC_GET(REC_R + 23);
+    x.R.b38 = REC_GET(REC_R + 24);
+    x.R.b39 = REC_GET(REC_R + 25);
+    x.R.c2 = REC_GET(REC_R + 26);
+    x.R.c4 = REC_GET(REC_R + 27);
+    x.R.c5 = REC_GET(REC_R + 28);
+    x.R.c7 = REC_GET(REC_R + 29);
+    x.R.c8 = REC_GET(REC_R + 30);
+    x.R.c9 = REC_GET(REC_R + 31);
+    x.R.c10 = REC_GET(REC_R + 32);
+    x.R.c14 = REC_GET(REC_R + 33);
+    x.R.c15 = REC_GET(REC_R + 34);
+    x.R.c17a = REC_GET(REC_R + 35);
+    x.R.c17b = REC_GET(REC_R + 36);
+    x.R.c18 = REC_GET(REC_R + 37);
+    x.R.c19 = REC_GET(REC_R + 38);
+    x.R.d2 = REC_GET(REC_R + 39);
+    x.R.d3 = REC_GET(REC_R + 40);
+    x.R.d4 = REC_GET(REC_R + 41);
+    x.R.d5 = REC_GET(REC_R + 42);
+    x.R.d6 = REC_GET(REC_R + 43);
+    x.R.d7 = REC_GET(REC_R + 44);
+    x.R.d8 = REC_GET(REC_R + 45);
+    x.R.d9 = REC_GET(REC_R + 46);
+    x.R.d10 = REC_GET(REC_R + 47);
+    x.R.d11 = REC_GET(REC_R + 48);
+    x.R.d31 = REC_GET(REC_R + 49);
+    x.R.d32 = REC_GET(REC_R + 50);
+    x.R.d33 = REC_GET(REC_R + 51);
+    x.R.d34 = REC_GET(REC_R + 52);
+    x.R.d36 = REC_GET(REC_R + 53);
+    x.R.d37 = REC_GET(REC_R + 54);
+    x.R.d60 = REC_GET(REC_R + 55);
+    x.R.d61 = REC_GET(REC_R + 56);
+    x.R.d62 = REC_GET(REC_R + 57);
+    x.R.d94 = REC_GET(REC_R + 58);
+    x.R.d95 = REC_GET(REC_R + 59);
+    x.R.d97 = REC_GET(REC_R + 60);
+    x.R.e2 = REC_GET(REC_R + 61);
+    x.R.e3 = REC_GET(REC_R + 62);
+    x.R.e4 = REC_GET(REC_R + 63);
+    x.R.e5a = REC_GET(REC_R + 64);
+    x.R.e5b = REC_GET(REC_R + 65);
+    x.R.e5c = REC_GET(REC_R + 66);
+    x.R.e6 = REC_GET(REC_R + 67);
+    x.R.e7 = REC_GET(REC_R + 68);
+    x.R.e8 = REC_GET(REC_R + 69);
+    x.R.e10 = REC_GET(REC_R + 70);
+    x.R.e11 = REC_GET(REC_R + 71);
+    x.R.e13 = REC_GET(REC_R + 72);
+    x.R.e15 = REC_GET(REC_R + 73);
+    x.R.e16 = REC_GET(REC_R + 74);
+    x.R.e51 = REC_GET(REC_R + 75);
+    x.R.e54 = REC_GET(REC_R + 76);
+    x.R.hk1 = REC_GET(REC_R + 77);
+    x.R.hk2 = REC_GET(REC_R + 78);
+    x.R.hk3 = REC_GET(REC_R + 79);
+    x.R.hk4 = REC_GET(REC_R + 80);
+    x.R.hk5 = REC_GET(REC_R + 81);
+#pragma unroll
+    for (int n = 0; n < NJ; ++n) x.tj[n] = REC_GET(REC_J + n);
+    x.h.k[0] = REC_GET(REC_H + 0);
+    x.h.k[1] = REC_GET(REC_H + 1);
+    x.h.k[2] = REC_GET(REC_H + 2);
+    x.h.k[3] = REC_GET(REC_H + 3);
+    x.h.k[4] = REC_GET(REC_H + 4);
+    x.h.k[5] = REC_GET(REC_H + 5);
+    x.h.k[6] = REC_GET(REC_H + 6);
+    x.h.k[7] = REC_GET(REC_H + 7);
+    x.h.k[8] = REC_GET(REC_H + 8);
+    x.h.pno3 = REC_GET(REC_H + 9);
+    x.h.pno3sl = REC_GET(REC_H + 10);
+    x.h.ph2o = REC_GET(REC_H + 11);
+    x.h.phcl = REC_GET(REC_H + 12);
+    x.h.surf = REC_GET(REC_H + 13);
+    x.h.vsed = REC_GET(REC_H + 14);
+    e.het1 = x.h.k[0]; e.het2 = x.h.k[1]; e.het3 = x.h.k[2]; e.het4 = x.h.k[3];
+    e.het5 = x.h.k[4]; e.het6 = x.h.k[5]; e.het7 = x.h.k[6]; e.het8 = x.h.k[7];
+    e.het9 = x.h.k[8];
+    e.pno3 = x.h.pno3;
+    e.ph2o = x.h.ph2o;
+    e.phcl = x.h.phcl;
+    e.pbr = 1.;
+#undef REC_GET
 }
 
 // Set-up of every box (F:800-1338 without the sweeps): thermal rates, then J-values, then
@@ -1408,10 +1307,10 @@ __global__ void __launch_bounds__(BLK, 2) k_setup(StepArgs a, double *__restrict
     e.dt = a.dt;
     e.day = sza <= 95.;
     thermal_rates(a.K, t, e.hnm, pm, x.R);
-    rec_store<26, 108>(x, rec, a.nbox, b);
+    rec_store<REC_R, REC_J>(x, rec, a.nbox, b);
     table_j(a.ajl, a.o3up, a.alt[b], sza, a.col[b], x.tj);
     x.tj[JNO] = j_no(a.col[a.nbox + b], a.col[2 * a.nbox + b], a.col[b], e.hnm, a.sza[c]);
-    rec_store<108, 149>(x, rec, a.nbox, b);
+    rec_store<REC_J, REC_H>(x, rec, a.nbox, b);
 #pragma unroll
     for (int s = 0; s < NLONG; ++s) x.ck[s] = a.qj1[(int64_t)s * a.field + b] * e.hnm;
     Het &h = x.h;
@@ -1438,8 +1337,8 @@ __global__ void __launch_bounds__(BLK, 2) k_setup(StepArgs a, double *__restrict
         e.rain_hno3 = rr; e.rain_hcl = rr; e.rain_hbr = rr; e.rain_h2o2 = rr;
         e.rain_hno4 = 0.; e.rain_ch2o = 0.; e.rain_ch3o2h = 0.;
     }
-    rec_store<0, 26>(x, rec, a.nbox, b);
-    rec_store<149, NREC>(x, rec, a.nbox, b);
+    rec_store<REC_HNM, REC_R>(x, rec, a.nbox, b);
+    rec_store<REC_H, NREC>(x, rec, a.nbox, b);
 }
 
 // The state of box b from its set-up record (chem3d before its sweep loop, F:800-1338):
@@ -1449,7 +1348,7 @@ template <int BLK>
 __device__ __forceinline__ void box_load(const StepArgs &a, const double *__restrict__ rec,
                                          int64_t b, Box &x, double *ck0, bool use_guess)
 {
-    rec_load(x, rec, a.nbox, b);
+    rec_load(x, rec, a.nbox, b, a.dt);
     const double hnm = x.e.hnm;
 #pragma unroll
     for (int s = 0; s < NLONG; ++s) {
