@@ -45,11 +45,11 @@ FLOP_PER_BOX = 2201 + 3544 + 595 + 13 + 166
 # Measured HBM bytes per step of the per-box kernels on this workload (k_setup + k_chem),
 # per block size: FETCH_SIZE x 2 + WRITE_SIZE in KiB, the factors from a 1 GiB copy of
 # known traffic (profiles/k_chem_c1_pmc.md).  Block 128 was not measured (null).
-TRAFFIC_BYTES = {256: (2 * (50254 + 834572) + 1022726 + 778767) * 1024,
-                 64: (2 * (48339 + 919144) + 1022726 + 781838) * 1024}
+TRAFFIC_BYTES = {256: (2 * (49413 + 759423) + 869164 + 717342) * 1024,
+                 64: (2 * (47929 + 788125) + 869164 + 701985) * 1024}
 # FP64 VALU instructions per step of the two kernels (SQ_INSTS_VALU_*_F64, same profile,
 # equal for every block size): add + mul + transcendental + 2 x fma, per lane
-FP64_INSTR_FLOP = ((34.387e6 + 23.730e6) + (19.991e6 + 33.606e6) + (2.318e6 + 5.126e6)
+FP64_INSTR_FLOP = ((34.400e6 + 23.730e6) + (19.942e6 + 33.480e6) + (2.318e6 + 5.126e6)
                    + 2 * (44.572e6 + 30.810e6)) * 64
 
 
