@@ -3,7 +3,8 @@
 The per-box kernels run one box per lane; what is left to choose per device and grid is
 the workgroup size (64, 128 or 256 lanes: how many waves share one LDS block of
 start-of-step state, and so how the blocks pack onto the CUs and XCDs) for the
-fixed-sweep step and for the persistent converged-mode kernels, whose grid is always the
+fixed-sweep step (its set-up and sweep kernels) and for the persistent converged-mode
+kernels, whose grid is always the
 resident wave set the occupancy calculator gives for that size; and, for the converged
 mode, the queue chunk (how many lanes of a wave must be idle before the wave takes new
 boxes: a small chunk keeps lanes busy, a large one runs the set-up and write-back code

@@ -4,12 +4,15 @@
 //   k_columns       : O3 column (total, F:5153-5186) and the O2 / NO column sums of
 //                     jno (J:103-126) -- the only vertical coupling of the step; one
 //                     lane per column walks its levels top-down.
-//   k_chem          : everything else per box: J-values (phot F:4996-5149, jno
+//   k_setup         : the per-box set-up into a record: J-values (phot F:4996-5149, jno
 //                     J:128-246), PSC/aerosol and heterogeneous rates (analytic
 //                     F:2620-3361, hetero F:3365-4029, density F:4033-4073), thermal
-//                     rates (F:920-1306), rain-out (F:1315-1338), the fixed-sweep
-//                     QSSA + linearised backward-Euler integration (F:1348-2378) and
-//                     the write-back (F:2380-2410); optionally conserv + floor.
+//                     rates (F:920-1306), rain-out (F:1315-1338).
+//   k_chem          : from the record, the fixed-sweep QSSA + linearised backward-Euler
+//                     integration (F:1348-2378) and the write-back (F:2380-2410);
+//                     optionally conserv + floor.
+//   k_chem_conv     : the converged mode (sweeps to the fixed point, persistent, with a
+//                     device work queue; optionally Anderson-accelerated).
 //   k_conserv_floor : conserv (F:2526-2616) and the 1e-30 floor (F:431-440).
 // Expressions keep the reference's evaluation order (the library is built with
 // -ffp-contract=off) so the step reproduces the reference to a few ulps per box.
