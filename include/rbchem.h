@@ -220,7 +220,7 @@ int rb_chem_converge_dev(rb_ctx *ctx, const rb_grid *g, double dt, double tol, i
  * test: where every long- and short-lived species changes by less than accept_tol
  * relative (the converged mode's test), the surrogate's answer is kept and iters[box] = 0;
  * every other box goes on sweeping to tol from there, exactly as rb_chem_converge_dev with
- * that guess (the plain FP64 solver is the fallback and the judge of the surrogate).
+ * that guess (the plain FP64 solver is the fallback and decides what is kept).
  * accept_tol must be positive; the other arguments are rb_chem_converge_dev's. */
 int rb_chem_surrogate_dev(rb_ctx *ctx, const rb_grid *g, double dt, double accept_tol,
                           double tol, int32_t maxiter, int32_t accel, const double *T,

@@ -210,7 +210,7 @@ def run_ecmwf_reference(forcing_file: str, ian: int, day: float, idx, ntime: int
     try:
         os.symlink(os.path.abspath(forcing_file), os.path.join(run, os.path.basename(forcing_file)))
         fin, fout = os.path.join(run, "in.bin"), os.path.join(run, "out.bin")
-        # the driver runs with big-endian unformatted I/O throughout, its own files too
+        # the harness program runs with big-endian unformatted I/O throughout, its own files too
         with open(fin, "wb") as f:
             np.array([ian, ntime, nintecmwf, idx.size], dtype=">i4").tofile(f)
             np.array([day], dtype=">f8").tofile(f)

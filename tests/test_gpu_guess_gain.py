@@ -3,8 +3,9 @@ sweeps to tol) from the incoming state and from the trained network's guess, on 
 subgrid of the evaluation seed (seed 1, polar vortex).  Both must reach the same fixed point
 (1e-9 per species above the 1e-30 floor on boxes that converge on both paths, as the
 converged-mode parity bar), and the guess must take fewer sweeps on average.  The gain is
-printed: it is small, because the sweep iteration converges linearly (see the design notes
-on the learned guess).
+printed: it is small, because the sweep iteration converges linearly (each sweep divides the
+remaining error by a roughly constant factor), so a start that is k times closer saves only
+about log(k) / log(that factor) sweeps.
 """
 import numpy as np
 import pytest
