@@ -44,13 +44,14 @@ BYTES_PER_BOX = (65 + 62) * 8
 FLOP_PER_BOX = 2201 + 3544 + 595 + 13 + 166
 # Measured HBM bytes per step of the per-box kernels on this workload (k_setup + k_chem),
 # per block size: FETCH_SIZE x 2 + WRITE_SIZE in KiB, the factors from a 1 GiB copy of
-# known traffic (profiles/k_chem_c1_pmc.md).  Block 128 was not measured (null).
-TRAFFIC_BYTES = {256: (2 * (49413 + 759423) + 869164 + 717342) * 1024,
-                 64: (2 * (47929 + 788125) + 869164 + 701985) * 1024}
+# known traffic (profiles/k_chem_c1_pmc.md).
+TRAFFIC_BYTES = {64: (2 * (47862 + 678354) + 869164 + 520782) * 1024,
+                 128: (2 * (47746 + 684205) + 869164 + 533067) * 1024,
+                 256: (2 * (49534 + 707407) + 869164 + 582207) * 1024}
 # FP64 VALU instructions per step of the two kernels (SQ_INSTS_VALU_*_F64, same profile,
 # equal for every block size): add + mul + transcendental + 2 x fma, per lane
-FP64_INSTR_FLOP = ((34.400e6 + 23.730e6) + (19.942e6 + 33.480e6) + (2.318e6 + 5.126e6)
-                   + 2 * (44.572e6 + 30.810e6)) * 64
+FP64_INSTR_FLOP = ((34.400e6 + 24.511e6) + (19.942e6 + 34.126e6) + (2.318e6 + 5.265e6)
+                   + 2 * (44.572e6 + 31.644e6)) * 64
 
 
 def parse():

@@ -691,13 +691,48 @@ __device__ __forceinline__ void track(Track &t, double v, double old)
     t.fin &= fabs(v) <= 1.7976931348623157e308;
 }
 
+// J-values that the sweep kernels keep in LDS beside ck0 (lane-private columns, stride
+// BLK): 42 + 38 doubles per lane fill the 160 KB of LDS at one wave per SIMD.  The three
+// the sweep reads most often stay in registers with the box state and the 82 rates.  At
+// block 64 this took the fixed-sweep kernel's spill from 106 to 46 registers and the plain
+// converged kernel's from 163 to 95; an LDS read also replaces an AGPR read, which is a
+// VALU instruction.
+// JL_SET orders the J-values by how often the sweep reads them; the first NJL go to LDS.
+constexpr int NJL = 38;
+constexpr int JL_SET[NJ] = {
+    JBRO, JCO2, JN2O, JNO2, JNO, JO2, JO3_O3P,                                 // 1 use
+    JCH2BR2, JCH2O_CO, JCH3BR, JCL2, JH1301, JH2O2, JH2O, JHCL, JN2O5, JO3_O1D,
+    JOCLO,                                                                     // 2 uses
+    JCCL4, JCFC113, JCFC11, JCFC12, JCH2O_HCO, JCH3CCL3, JCH3CL, JCH3O2H, JCLNO2,
+    JH1211, JHBR, JHCFC22, JHNO3, JHNO4, JNO3_NO2,                             // 3 uses
+    JCLONO2_CLO, JBRONO2, JCL2O2, JCLONO2_CL, JHOBR, JHOCL, JNO3_NO, JBRCL};
+__host__ __device__ constexpr int jl_slot(int j)
+{
+    for (int i = 0; i < NJL; ++i)
+        if (JL_SET[i] == j) return i;
+    return -1;
+}
+constexpr bool jl_set_is_permutation()
+{
+    for (int j = 0; j < NJ; ++j) {
+        int n = 0;
+        for (int i = 0; i < NJ; ++i) n += JL_SET[i] == j;
+        if (n != 1) return false;
+    }
+    return true;
+}
+static_assert(jl_set_is_permutation(), "JL_SET lists every J-value once");
+static_assert(NJL >= 0 && NJL <= NJ, "J-values in LDS");
+
 // ck0 is the start-of-step state in LDS, one column per lane: ck0[s * CK0_STRIDE].
 // With TRACK, tr accumulates the fixed-point test over this sweep.
-template <int CK0_STRIDE, bool TRACK>
+template <int CK0_STRIDE, bool TRACK, bool JLDS = false>
 __device__ void sweep(const Rates &R, const double *tj, const BoxEnv &e,
                                       const double *ck0, double *ck, double *cc, double &ch3o,
-                                      double &hco, Track &tr)
+                                      double &hco, Track &tr, const double *tjl = nullptr)
 {
+// J-value X: from LDS (tjl, stride CK0_STRIDE) if JLDS and X is one of JL_SET, else tj[X]
+#define TJ(X) ((JLDS && jl_slot(X) >= 0) ? tjl[jl_slot(X) * CK0_STRIDE] : tj[X])
 #define CCSET(sl, expr)                                                              \
     do {                                                                             \
         const double v_ = (expr);                                                    \
@@ -712,22 +747,22 @@ __device__ void sweep(const Rates &R, const double *tj, const BoxEnv &e,
                   (R.c4 * o2 + C3 * cc[O3P]));
         ch3o = (C1B * cc[O1D] * ck[CH4] +
                 (R.c5 * cc[NO] + 2. * R.c14 * ck[CH3O2] + R.c18 * cc[CLO]) * ck[CH3O2] +
-                tj[JCH3O2H] * ck[CH3O2H]) /
+                TJ(JCH3O2H) * ck[CH3O2H]) /
                (R.c15 * o2);
-        hco = (R.c8 * cc[OH] + R.c9 * cc[O3P] + R.c10 * cc[CL] + R.c19 * cc[BR] + tj[JCH2O_HCO]) *
+        hco = (R.c8 * cc[OH] + R.c9 * cc[O3P] + R.c10 * cc[CL] + R.c19 * cc[BR] + TJ(JCH2O_HCO)) *
               ck[CH2O] / (C12 * o2);
-        const double r1 = (tj[JO3_O1D] + tj[JO3_O3P] + R.a2 * cc[H] + R.a6 * cc[OH] +
+        const double r1 = (TJ(JO3_O1D) + TJ(JO3_O3P) + R.a2 * cc[H] + R.a6 * cc[OH] +
                            R.a6b * cc[HO2] + R.b4 * cc[NO] + R.d2 * cc[CL]) /
                           (R.hk2 * o2);
         CCSET(O3, 1. / (1. + r1) * ck[OX]);
         CCSET(O3P, r1 * cc[O3]);
-        CCSET(O1D, tj[JO3_O1D] / (R.hk4 * n2 + R.hk5 * o2) * cc[O3]);
+        CCSET(O1D, TJ(JO3_O1D) / (R.hk4 * n2 + R.hk5 * o2) * cc[O3]);
         const double r4 = (R.a5 * cc[O3P] + R.a36 * ck[CO] + R.a19 * h2) /
                           (R.a1 * o2 + R.a2 * cc[O3] + (A23A + A23B + A23C) * cc[HO2]);
         CCSET(H, r4 * cc[OH]);
-        const double r7 = tj[JNO] / (R.b7 * o2 + R.b6 * cc[NO]);
-        const double r8 = (R.b9 * cc[O3]) / (tj[JNO3_NO] + tj[JNO3_NO2] + R.b12 * cc[NO2]);
-        const double r5 = (tj[JNO2] + R.b3 * cc[O3P] + tj[JNO3_NO] * r8) /
+        const double r7 = TJ(JNO) / (R.b7 * o2 + R.b6 * cc[NO]);
+        const double r8 = (R.b9 * cc[O3]) / (TJ(JNO3_NO) + TJ(JNO3_NO2) + R.b12 * cc[NO2]);
+        const double r5 = (TJ(JNO2) + R.b3 * cc[O3P] + TJ(JNO3_NO) * r8) /
                           (R.b4 * cc[O3] + R.a26 * cc[HO2] + R.d4 * cc[CLO] + R.e4 * cc[BRO] +
                            R.c5 * ck[CH3O2]);
         CCSET(NO2, ck[NOX] / (1. + r5 + r7 * r5 + r8));
@@ -742,25 +777,25 @@ __device__ void sweep(const Rates &R, const double *tj, const BoxEnv &e,
                            R.e15 * cc[BRO]);
         CCSET(HO2, .5 * cc[HO2] + .5 * cc[OH] * r6);        // 0.5 under-relaxation
         const double r13 = R.d31 * cc[NO2] /
-                           (tj[JCLONO2_CL] + tj[JCLONO2_CLO] + R.d32 * cc[O3P] +
+                           (TJ(JCLONO2_CL) + TJ(JCLONO2_CLO) + R.d32 * cc[O3P] +
                             e.het1 * ck[H2O] + e.het2 * ck[HCL]);
         const double r9 = (R.d33 * cc[HO2] + e.het1 * ck[H2O] * r13) /
-                          (e.het5 * ck[HCL] + D35 * cc[O3P] + tj[JHOCL]);
-        const double r10 = (R.d60 * cc[CLO]) / (R.d61 * hnm + tj[JCL2O2]);
-        const double r12 = R.e5c * cc[BRO] / tj[JBRCL];
+                          (e.het5 * ck[HCL] + D35 * cc[O3P] + TJ(JHOCL));
+        const double r10 = (R.d60 * cc[CLO]) / (R.d61 * hnm + TJ(JCL2O2));
+        const double r12 = R.e5c * cc[BRO] / TJ(JBRCL);
         const double r11 = (R.d3 * cc[O3P] + R.d4 * cc[NO] + R.d8 * cc[OH] + R.c18 * ck[CH3O2] +
-                            R.e5b * cc[BRO] + tj[JCLONO2_CL] * r13 + tj[JHOCL] * r9 +
-                            2. * tj[JCL2O2] * r10 + tj[JBRCL] * r12) /
+                            R.e5b * cc[BRO] + TJ(JCLONO2_CL) * r13 + TJ(JHOCL) * r9 +
+                            2. * TJ(JCL2O2) * r10 + TJ(JBRCL) * r12) /
                            (R.d2 * cc[O3] + R.d5 * ck[CH4] + R.d7 * cc[HO2] + R.d10 * cc[HO2] +
                             R.d37 * ck[HOCL]);
         CCSET(CLO, .5 * cc[CLO] + .5 * ck[CLOX] / (1. + r11));
         CCSET(CL, r11 * cc[CLO]);
-        const double r27 = R.e13 * cc[NO2] / (tj[JBRONO2] + R.e8 * cc[O3P]);
-        const double r28 = R.e15 * cc[HO2] / (tj[JHOBR] + R.e16 * cc[O3P]);
-        const double r29 = R.e5c * cc[CLO] / tj[JBRCL];
+        const double r27 = R.e13 * cc[NO2] / (TJ(JBRONO2) + R.e8 * cc[O3P]);
+        const double r28 = R.e15 * cc[HO2] / (TJ(JHOBR) + R.e16 * cc[O3P]);
+        const double r29 = R.e5c * cc[CLO] / TJ(JBRCL);
         const double r30 = (R.e3 * cc[O3P] + R.e4 * cc[NO] + R.e5a * cc[CLO] + R.e5b * cc[CLO] +
-                            2. * R.e6 * cc[BRO] + R.e10 * cc[OH] + tj[JBRO] +
-                            tj[JBRONO2] * 0.85 * r27 + tj[JHOBR] * r28 + tj[JBRCL] * r29) /
+                            2. * R.e6 * cc[BRO] + R.e10 * cc[OH] + TJ(JBRO) +
+                            TJ(JBRONO2) * 0.85 * r27 + TJ(JHOBR) * r28 + TJ(JBRCL) * r29) /
                            (R.e2 * cc[O3] + R.e7 * cc[HO2] + R.c19 * ck[CH2O]);
         CCSET(BRO, .5 * cc[BRO] + .5 * ck[BROX] / (1. + r30));
         CCSET(BR, r30 * cc[BRO]);
@@ -773,14 +808,14 @@ __device__ void sweep(const Rates &R, const double *tj, const BoxEnv &e,
                             R.d34 * ck[HOCL] + R.e11 * ck[HBR] * e.pbr + R.e7 * cc[BR] * r6 +
                             R.e15 * cc[BRO] * r6;
         const double coec =
-            2. * (R.a1et * cc[O1D] + tj[JH2O]) * ck[H2O] * e.ph2o +
+            2. * (R.a1et * cc[O1D] + TJ(JH2O)) * ck[H2O] * e.ph2o +
             ((C1A + C1B) * ck[CH4] + 2. * A3ET * h2) * cc[O1D] + C3 * cc[CH3] * cc[O3P] +
-            tj[JCH3O2H] * ck[CH3O2H] + R.c15 * ch3o * o2 +
-            (tj[JCH2O_HCO] + R.c9 * cc[O3P]) * ck[CH2O] + C12 * hco * o2 +
-            tj[JHNO3] * ck[HNO3] * e.pno3 + (R.b24 * hnm + tj[JHNO4]) * ck[HO2NO2] +
-            (D35 * cc[O3P] + R.d37 * cc[CL] + tj[JHOCL]) * ck[HOCL] + 2. * tj[JH2O2] * ck[H2O2] +
-            R.d6 * h2 * cc[CL] + tj[JHOBR] * ck[HOBR] + R.e16 * cc[O3P] * ck[HOBR] +
-            tj[JHBR] * ck[HBR];
+            TJ(JCH3O2H) * ck[CH3O2H] + R.c15 * ch3o * o2 +
+            (TJ(JCH2O_HCO) + R.c9 * cc[O3P]) * ck[CH2O] + C12 * hco * o2 +
+            TJ(JHNO3) * ck[HNO3] * e.pno3 + (R.b24 * hnm + TJ(JHNO4)) * ck[HO2NO2] +
+            (D35 * cc[O3P] + R.d37 * cc[CL] + TJ(JHOCL)) * ck[HOCL] + 2. * TJ(JH2O2) * ck[H2O2] +
+            R.d6 * h2 * cc[CL] + TJ(JHOBR) * ck[HOBR] + R.e16 * cc[O3P] * ck[HOBR] +
+            TJ(JHBR) * ck[HBR];
         const double argum = fmax(coeb * coeb + 4. * coea * coec, 0.);
         CCSET(OH, .5 * cc[OH] + .5 * (sqrt(argum) - coeb) / (2. * coea));
     } else {
@@ -810,139 +845,139 @@ __device__ void sweep(const Rates &R, const double *tj, const BoxEnv &e,
     const double dt = e.dt;
     double nk[NLONG];
 #define UPDATE(s, P, Lr) nk[s] = (ck0[(s) * CK0_STRIDE] + dt * (P)) / (1. + dt * (Lr))
-    UPDATE(N2O, 0.e0, tj[JN2O] + (R.b38 + R.b39) * cc[O1D]);
+    UPDATE(N2O, 0.e0, TJ(JN2O) + (R.b38 + R.b39) * cc[O1D]);
     UPDATE(CH4, 0.e0, R.c2 * cc[OH] + C1A * cc[O1D] + C1B * cc[O1D] + C1C * cc[O1D] + R.d5 * cc[CL]);
     UPDATE(H2O,
            R.c2 * cc[OH] * ck[CH4] + R.c8 * cc[OH] * ck[CH2O] + R.c17a * ck[CH3O2H] * cc[OH] +
                R.c17b * ck[CH3O2H] * cc[OH] + (R.a17 * cc[OH] + A23C * cc[H]) * cc[HO2] +
                R.a19 * cc[OH] * h2 + A30 * cc[OH] * ck[H2O2] + A31 * cc[OH] * cc[OH] +
                R.b27 * cc[OH] * ck[HNO3] * e.pno3 + R.d11 * cc[OH] * ck[HCL] * e.phcl,
-           (tj[JH2O] + R.a1et * cc[O1D]) * e.ph2o);
+           (TJ(JH2O) + R.a1et * cc[O1D]) * e.ph2o);
     UPDATE(NOY, 2. * R.b39 * cc[O1D] * ck[N2O],
            (2. * R.b6 * cc[N] * cc[NO] + e.rain_hno3 * ck[HNO3] + e.rain_hno4 * ck[HO2NO2]) / ck[NOY]);
     UPDATE(HNO3,
            R.b22 * cc[OH] * cc[NO2] + e.het1 * ck[CLONO2] * ck[H2O] + e.het2 * ck[CLONO2] * ck[HCL] +
                2. * e.het3 * ck[N2O5] * ck[H2O] + e.het4 * ck[N2O5] * ck[HCL] +
                e.het6 * ck[BRONO2] * ck[H2O],
-           (tj[JHNO3] + R.b27 * cc[OH]) * e.pno3 + e.rain_hno3);
+           (TJ(JHNO3) + R.b27 * cc[OH]) * e.pno3 + e.rain_hno3);
     UPDATE(N2O5, R.b12 * cc[NO3] * cc[NO2],
-           tj[JN2O5] + R.b32 * hnm + e.het3 * ck[H2O] + e.het4 * ck[HCL]);
+           TJ(JN2O5) + R.b32 * hnm + e.het3 * ck[H2O] + e.het4 * ck[HCL]);
     UPDATE(CLY,
-           3. * (tj[JCFC11] + D90 * cc[O1D]) * ck[CFC11] + 2. * (tj[JCFC12] + D91 * cc[O1D]) * ck[CFC12] +
-               3. * (tj[JCFC113] + D92 * cc[O1D]) * ck[CFC113] +
-               4. * (tj[JCCL4] + D93 * cc[O1D]) * ck[CCL4] +
-               3. * (tj[JCH3CCL3] + R.d94 * cc[OH]) * ck[CH3CCL3] +
-               1. * (tj[JCH3CL] + R.d95 * cc[OH]) * ck[CH3CL] +
-               1. * (tj[JHCFC22] + D96 * cc[O1D] + R.d97 * cc[OH]) * ck[HCFC22] +
-               1. * (tj[JH1211] + E52 * cc[O1D]) * ck[H1211],
+           3. * (TJ(JCFC11) + D90 * cc[O1D]) * ck[CFC11] + 2. * (TJ(JCFC12) + D91 * cc[O1D]) * ck[CFC12] +
+               3. * (TJ(JCFC113) + D92 * cc[O1D]) * ck[CFC113] +
+               4. * (TJ(JCCL4) + D93 * cc[O1D]) * ck[CCL4] +
+               3. * (TJ(JCH3CCL3) + R.d94 * cc[OH]) * ck[CH3CCL3] +
+               1. * (TJ(JCH3CL) + R.d95 * cc[OH]) * ck[CH3CL] +
+               1. * (TJ(JHCFC22) + D96 * cc[O1D] + R.d97 * cc[OH]) * ck[HCFC22] +
+               1. * (TJ(JH1211) + E52 * cc[O1D]) * ck[H1211],
            e.rain_hcl * ck[HCL] / ck[CLY]);
     {
         // odd oxygen (F:1864-1950)
-        const double ratio = tj[JCL2O2] / (tj[JCL2O2] + R.d61 * hnm);
-        const double fno3 = tj[JNO3_NO] / fmax(tj[JNO3_NO] + tj[JNO3_NO2], 1.e-30);
+        const double ratio = TJ(JCL2O2) / (TJ(JCL2O2) + R.d61 * hnm);
+        const double fno3 = TJ(JNO3_NO) / fmax(TJ(JNO3_NO) + TJ(JNO3_NO2), 1.e-30);
         const double loss =
             2. * R.hk1 * cc[O3P] * cc[O3P] + 2. * R.hk3 * cc[O3] * cc[O3P] +
             2. * R.b3 * cc[NO2] * cc[O3P] + 2. * R.d3 * cc[CLO] * cc[O3P] +
             2. * R.d60 * ratio * cc[CLO] * cc[CLO] +
-            fmin(tj[JHOCL] * ck[HOCL], R.d33 * cc[HO2] * cc[CLO]) +
-            2. * fmin(R.e5c * cc[BRO] * cc[CLO], tj[JBRCL] * ck[BRCL]) +
-            2. * R.e5b * cc[BRO] * cc[CLO] * day + fmin(R.e15 * cc[BRO] * cc[HO2], tj[JHOBR] * ck[HOBR]) +
-            2. * R.e6 * cc[BRO] * cc[BRO] * day + 2. * fno3 * tj[JCLONO2_CL] * ck[CLONO2] +
+            fmin(TJ(JHOCL) * ck[HOCL], R.d33 * cc[HO2] * cc[CLO]) +
+            2. * fmin(R.e5c * cc[BRO] * cc[CLO], TJ(JBRCL) * ck[BRCL]) +
+            2. * R.e5b * cc[BRO] * cc[CLO] * day + fmin(R.e15 * cc[BRO] * cc[HO2], TJ(JHOBR) * ck[HOBR]) +
+            2. * R.e6 * cc[BRO] * cc[BRO] * day + 2. * fno3 * TJ(JCLONO2_CL) * ck[CLONO2] +
             ((R.a2 * cc[H] + R.a6 * cc[OH] + R.a6b * cc[HO2]) * cc[O3] +
              (R.a5 * cc[OH] + R.a7 * cc[HO2]) * cc[O3P]) +
-            2. * R.b9 * cc[NO2] * cc[O3] * tj[JNO3_NO] / fmax(tj[JNO3_NO] + tj[JNO3_NO2], 1.e-30);
-        UPDATE(OX, 2. * tj[JO2] * o2 + R.a26 * cc[HO2] * cc[NO] + R.c5 * cc[NO] * ck[CH3O2],
+            2. * R.b9 * cc[NO2] * cc[O3] * TJ(JNO3_NO) / fmax(TJ(JNO3_NO) + TJ(JNO3_NO2), 1.e-30);
+        UPDATE(OX, 2. * TJ(JO2) * o2 + R.a26 * cc[HO2] * cc[NO] + R.c5 * cc[NO] * ck[CH3O2],
                loss / ck[OX]);
     }
-    UPDATE(CO, tj[JCO2] * co2 + tj[JCH2O_CO] * ck[CH2O] + C12 * hco * o2, R.a36 * cc[OH]);
-    UPDATE(OCLO, R.e5a * cc[BRO] * cc[CLO], tj[JOCLO]);
+    UPDATE(CO, TJ(JCO2) * co2 + TJ(JCH2O_CO) * ck[CH2O] + C12 * hco * o2, R.a36 * cc[OH]);
+    UPDATE(OCLO, R.e5a * cc[BRO] * cc[CLO], TJ(JOCLO));
     UPDATE(OXP, 0., 0.);
     UPDATE(H2SO4, 0., 0.);
     UPDATE(HCL,
            (R.c10 * ck[CH2O] + R.d5 * ck[CH4] + R.d7 * cc[HO2] + R.d6 * h2) * cc[CL] +
                R.d9 * cc[OH] * cc[CLO],
-           (tj[JHCL] + R.d11 * cc[OH]) * e.phcl + e.het2 * ck[CLONO2] + e.het4 * ck[N2O5] +
+           (TJ(JHCL) + R.d11 * cc[OH]) * e.phcl + e.het2 * ck[CLONO2] + e.het4 * ck[N2O5] +
                e.het5 * ck[HOCL] + e.het7 * ck[HOBR] + e.rain_hcl);
     UPDATE(CLONO2, R.d31 * cc[NO2] * cc[CLO],
-           tj[JCLONO2_CL] + tj[JCLONO2_CLO] + R.d32 * cc[O3P] + e.het1 * ck[H2O] + e.het2 * ck[HCL]);
+           TJ(JCLONO2_CL) + TJ(JCLONO2_CLO) + R.d32 * cc[O3P] + e.het1 * ck[H2O] + e.het2 * ck[HCL]);
     UPDATE(HOCL, R.d33 * cc[HO2] * cc[CLO] + e.het1 * ck[CLONO2] * ck[H2O],
-           tj[JHOCL] + R.d34 * cc[OH] + D35 * cc[O3P] + R.d37 * cc[CL] + e.het5 * ck[HCL] +
+           TJ(JHOCL) + R.d34 * cc[OH] + D35 * cc[O3P] + R.d37 * cc[CL] + e.het5 * ck[HCL] +
                e.het9 * ck[HBR]);
     UPDATE(CL2,
            R.d37 * ck[HOCL] * cc[CL] + R.d62 * cc[CLO] * cc[CLO] + e.het2 * ck[CLONO2] * ck[HCL] +
                e.het5 * ck[HOCL] * ck[HCL],
-           tj[JCL2]);
-    UPDATE(H2O2, R.a27 * cc[HO2] * cc[HO2], tj[JH2O2] + A30 * cc[OH] + e.rain_h2o2);
-    UPDATE(CLNO2, R.d36 * cc[CL] * cc[NO2] + e.het4 * ck[N2O5] * ck[HCL], tj[JCLNO2]);
+           TJ(JCL2));
+    UPDATE(H2O2, R.a27 * cc[HO2] * cc[HO2], TJ(JH2O2) + A30 * cc[OH] + e.rain_h2o2);
+    UPDATE(CLNO2, R.d36 * cc[CL] * cc[NO2] + e.het4 * ck[N2O5] * ck[HCL], TJ(JCLNO2));
     UPDATE(HBR, R.e7 * cc[HO2] * cc[BR] + R.c19 * ck[CH2O] * cc[BR],
-           (R.e11 * cc[OH] + E9 * cc[O1D] + tj[JHBR]) * e.pbr + e.het8 * ck[HOBR] +
+           (R.e11 * cc[OH] + E9 * cc[O1D] + TJ(JHBR)) * e.pbr + e.het8 * ck[HOBR] +
                e.het9 * ck[HOCL] + e.rain_hbr);
-    UPDATE(BRONO2, R.e13 * cc[BRO] * cc[NO2], tj[JBRONO2] + R.e8 * cc[O3P] + e.het6 * ck[H2O]);
+    UPDATE(BRONO2, R.e13 * cc[BRO] * cc[NO2], TJ(JBRONO2) + R.e8 * cc[O3P] + e.het6 * ck[H2O]);
     UPDATE(NOX,
-           (tj[JHNO4] + R.b28 * cc[OH] + R.b24 * hnm) * ck[HO2NO2] + tj[JHNO3] * ck[HNO3] * e.pno3 +
-               2. * (tj[JN2O5] + R.b32 * hnm) * ck[N2O5] + 2. * R.b39 * cc[O1D] * ck[N2O] +
-               tj[JCLNO2] * ck[CLNO2] + R.b27 * ck[HNO3] * e.pno3 * cc[OH] +
-               R.d32 * ck[CLONO2] * cc[O3P] + (tj[JCLONO2_CL] + tj[JCLONO2_CLO]) * ck[CLONO2] +
-               tj[JBRONO2] * ck[BRONO2] + R.e8 * ck[BRONO2] * cc[O3P],
+           (TJ(JHNO4) + R.b28 * cc[OH] + R.b24 * hnm) * ck[HO2NO2] + TJ(JHNO3) * ck[HNO3] * e.pno3 +
+               2. * (TJ(JN2O5) + R.b32 * hnm) * ck[N2O5] + 2. * R.b39 * cc[O1D] * ck[N2O] +
+               TJ(JCLNO2) * ck[CLNO2] + R.b27 * ck[HNO3] * e.pno3 * cc[OH] +
+               R.d32 * ck[CLONO2] * cc[O3P] + (TJ(JCLONO2_CL) + TJ(JCLONO2_CLO)) * ck[CLONO2] +
+               TJ(JBRONO2) * ck[BRONO2] + R.e8 * ck[BRONO2] * cc[O3P],
            ((R.d31 * cc[CLO] + R.b22 * cc[OH] + R.b23 * cc[HO2] + 2. * R.b12 * cc[NO3] +
              R.d36 * cc[CL] + R.e13 * cc[BRO]) * cc[NO2] +
             2. * R.b6 * cc[N] * cc[NO]) / ck[NOX]);
-    UPDATE(HO2NO2, R.b23 * cc[HO2] * cc[NO2], tj[JHNO4] + R.b28 * cc[OH] + R.b24 * hnm + e.rain_hno4);
+    UPDATE(HO2NO2, R.b23 * cc[HO2] * cc[NO2], TJ(JHNO4) + R.b28 * cc[OH] + R.b24 * hnm + e.rain_hno4);
     UPDATE(CLOX,
-           2. * (R.d61 * hnm + tj[JCL2O2]) * ck[CL2O2] +
-               (tj[JCLONO2_CL] + tj[JCLONO2_CLO] + R.d32 * cc[O3P]) * ck[CLONO2] +
-               (tj[JHOCL] + R.d34 * cc[OH] + D35 * cc[O3P]) * ck[HOCL] +
-               (tj[JOCLO] * ck[OCLO] + (tj[JHCL] + R.d11 * cc[OH]) * ck[HCL] * e.phcl) +
-               2. * tj[JCL2] * ck[CL2] + tj[JCLNO2] * ck[CLNO2] + tj[JBRCL] * ck[BRCL] +
-               3. * (tj[JCFC11] + D90 * cc[O1D]) * ck[CFC11] +
-               2. * (tj[JCFC12] + D91 * cc[O1D]) * ck[CFC12] +
-               3. * (tj[JCFC113] + D92 * cc[O1D]) * ck[CFC113] +
-               4. * (tj[JCCL4] + D93 * cc[O1D]) * ck[CCL4] +
-               3. * (tj[JCH3CCL3] + R.d94 * cc[OH]) * ck[CH3CCL3] +
-               1. * (tj[JCH3CL] + R.d95 * cc[OH]) * ck[CH3CL] +
-               1. * (tj[JHCFC22] + D96 * cc[O1D] + R.d97 * cc[OH]) * ck[HCFC22],
+           2. * (R.d61 * hnm + TJ(JCL2O2)) * ck[CL2O2] +
+               (TJ(JCLONO2_CL) + TJ(JCLONO2_CLO) + R.d32 * cc[O3P]) * ck[CLONO2] +
+               (TJ(JHOCL) + R.d34 * cc[OH] + D35 * cc[O3P]) * ck[HOCL] +
+               (TJ(JOCLO) * ck[OCLO] + (TJ(JHCL) + R.d11 * cc[OH]) * ck[HCL] * e.phcl) +
+               2. * TJ(JCL2) * ck[CL2] + TJ(JCLNO2) * ck[CLNO2] + TJ(JBRCL) * ck[BRCL] +
+               3. * (TJ(JCFC11) + D90 * cc[O1D]) * ck[CFC11] +
+               2. * (TJ(JCFC12) + D91 * cc[O1D]) * ck[CFC12] +
+               3. * (TJ(JCFC113) + D92 * cc[O1D]) * ck[CFC113] +
+               4. * (TJ(JCCL4) + D93 * cc[O1D]) * ck[CCL4] +
+               3. * (TJ(JCH3CCL3) + R.d94 * cc[OH]) * ck[CH3CCL3] +
+               1. * (TJ(JCH3CL) + R.d95 * cc[OH]) * ck[CH3CL] +
+               1. * (TJ(JHCFC22) + D96 * cc[O1D] + R.d97 * cc[OH]) * ck[HCFC22],
            ((R.d31 * cc[NO2] + 2. * R.d60 * cc[CLO] + 2. * R.d62 * cc[CLO] + R.e5a * cc[BRO] +
              R.e5c * cc[BRO] + R.d9 * cc[OH] + R.d33 * cc[HO2]) * cc[CLO] +
             (R.c10 * ck[CH2O] + R.d5 * ck[CH4] + R.d7 * cc[HO2] + R.d6 * h2 + R.d36 * cc[NO2] +
              R.d37 * ck[HOCL]) * cc[CL]) / ck[CLOX]);
     UPDATE(BROX,
-           tj[JHOBR] * ck[HOBR] + tj[JBRONO2] * ck[BRONO2] + R.e8 * ck[BRONO2] * cc[O3P] +
-               R.e16 * ck[HOBR] * cc[O3P] + tj[JBRCL] * ck[BRCL] +
-               (R.e11 * cc[OH] + E9 * cc[O1D] + tj[JHBR]) * ck[HBR] * e.pbr +
+           TJ(JHOBR) * ck[HOBR] + TJ(JBRONO2) * ck[BRONO2] + R.e8 * ck[BRONO2] * cc[O3P] +
+               R.e16 * ck[HOBR] * cc[O3P] + TJ(JBRCL) * ck[BRCL] +
+               (R.e11 * cc[OH] + E9 * cc[O1D] + TJ(JHBR)) * ck[HBR] * e.pbr +
                2. * e.het8 * ck[HBR] * ck[HOBR],
            ((R.e13 * cc[NO2] + R.e5c * cc[CLO] + R.e15 * cc[HO2]) * cc[BRO] +
             R.c19 * cc[BR] * ck[CH2O] + R.e7 * cc[BR] * cc[HO2]) / ck[BROX]);
-    UPDATE(CL2O2, R.d60 * cc[CLO] * cc[CLO], tj[JCL2O2] + R.d61 * hnm);
+    UPDATE(CL2O2, R.d60 * cc[CLO] * cc[CLO], TJ(JCL2O2) + R.d61 * hnm);
     UPDATE(HOBR, R.e15 * cc[BRO] * cc[HO2] + e.het6 * ck[BRONO2] * ck[H2O],
-           tj[JHOBR] + R.e16 * cc[O3P] + e.het7 * ck[HCL] + e.het8 * ck[HBR]);
+           TJ(JHOBR) + R.e16 * cc[O3P] + e.het7 * ck[HCL] + e.het8 * ck[HBR]);
     UPDATE(BRCL,
            R.e5c * cc[CLO] * cc[BRO] + e.het7 * ck[HOBR] * ck[HCL] + e.het9 * ck[HBR] * ck[HOCL],
-           tj[JBRCL]);
+           TJ(JBRCL));
     UPDATE(CH2O,
            (C1C * cc[O1D] * ck[CH4] + C3 * cc[O3P] * cc[CH3] + R.c15 * ch3o * o2 +
             R.c17b * cc[OH] * ck[CH3O2H]),
-           R.c8 * cc[OH] + R.c9 * cc[O3P] + R.c10 * cc[CL] + R.c19 * cc[BR] + tj[JCH2O_HCO] +
-               tj[JCH2O_CO] + e.rain_ch2o);
+           R.c8 * cc[OH] + R.c9 * cc[O3P] + R.c10 * cc[CL] + R.c19 * cc[BR] + TJ(JCH2O_HCO) +
+               TJ(JCH2O_CO) + e.rain_ch2o);
     UPDATE(CH3O2, R.c4 * cc[CH3] * o2 + R.c17a * cc[OH] * ck[CH3O2H],
            2. * R.c14 * ck[CH3O2] + R.c7 * cc[HO2] + R.c5 * cc[NO] + R.c18 * cc[CLO]);
     UPDATE(CH3O2H, R.c7 * ck[CH3O2] * cc[HO2],
-           R.c17a * cc[OH] + R.c17b * cc[OH] + tj[JCH3O2H] + e.rain_ch3o2h);
-    UPDATE(CFC11, 0., tj[JCFC11] + D90 * cc[O1D]);
-    UPDATE(CFC12, 0., tj[JCFC12] + D91 * cc[O1D]);
-    UPDATE(CFC113, 0., tj[JCFC113] + D92 * cc[O1D]);
-    UPDATE(CCL4, 0., tj[JCCL4] + D93 * cc[O1D]);
-    UPDATE(CH3CCL3, 0., tj[JCH3CCL3] + R.d94 * cc[OH]);
-    UPDATE(CH3CL, 0., tj[JCH3CL] + R.d95 * cc[OH]);
-    UPDATE(HCFC22, 0., tj[JHCFC22] + D96 * cc[O1D] + R.d97 * cc[OH]);
-    UPDATE(CH3BR, 0., tj[JCH3BR] + E50 * cc[O1D] + R.e51 * cc[OH]);
-    UPDATE(H1211, 0., tj[JH1211] + E52 * cc[O1D]);
-    UPDATE(H1301, 0., tj[JH1301] + E53 * cc[O1D]);
+           R.c17a * cc[OH] + R.c17b * cc[OH] + TJ(JCH3O2H) + e.rain_ch3o2h);
+    UPDATE(CFC11, 0., TJ(JCFC11) + D90 * cc[O1D]);
+    UPDATE(CFC12, 0., TJ(JCFC12) + D91 * cc[O1D]);
+    UPDATE(CFC113, 0., TJ(JCFC113) + D92 * cc[O1D]);
+    UPDATE(CCL4, 0., TJ(JCCL4) + D93 * cc[O1D]);
+    UPDATE(CH3CCL3, 0., TJ(JCH3CCL3) + R.d94 * cc[OH]);
+    UPDATE(CH3CL, 0., TJ(JCH3CL) + R.d95 * cc[OH]);
+    UPDATE(HCFC22, 0., TJ(JHCFC22) + D96 * cc[O1D] + R.d97 * cc[OH]);
+    UPDATE(CH3BR, 0., TJ(JCH3BR) + E50 * cc[O1D] + R.e51 * cc[OH]);
+    UPDATE(H1211, 0., TJ(JH1211) + E52 * cc[O1D]);
+    UPDATE(H1301, 0., TJ(JH1301) + E53 * cc[O1D]);
     UPDATE(BRY,
-           (tj[JCH3BR] + E50 * cc[O1D] + R.e51 * cc[OH]) * ck[CH3BR] +
-               (tj[JH1211] + E52 * cc[O1D]) * ck[H1211] + (tj[JH1301] + E53 * cc[O1D]) * ck[H1301] +
-               2. * (R.e54 * cc[OH] + tj[JCH2BR2]) * ck[CH2BR2],
+           (TJ(JCH3BR) + E50 * cc[O1D] + R.e51 * cc[OH]) * ck[CH3BR] +
+               (TJ(JH1211) + E52 * cc[O1D]) * ck[H1211] + (TJ(JH1301) + E53 * cc[O1D]) * ck[H1301] +
+               2. * (R.e54 * cc[OH] + TJ(JCH2BR2)) * ck[CH2BR2],
            e.rain_hbr * ck[HBR] / ck[BRY]);
-    UPDATE(CH2BR2, 0., tj[JCH2BR2] + R.e54 * cc[OH]);
+    UPDATE(CH2BR2, 0., TJ(JCH2BR2) + R.e54 * cc[OH]);
 #undef UPDATE
 #pragma unroll
     for (int s = 0; s < NLONG - 1; ++s) {
@@ -950,6 +985,7 @@ __device__ void sweep(const Rates &R, const double *tj, const BoxEnv &e,
         ck[s] = nk[s];
     }
 #undef CCSET
+#undef TJ
 }
 
 // ------------------------------------------------------------------ kernels
@@ -1380,17 +1416,23 @@ __global__ void __launch_bounds__(BLK) k_chem(StepArgs a, const double *__restri
 {
     // start-of-step state of the block's boxes, species-major: lane-private columns,
     // conflict-free, and off the register file that the sweep needs for rates and J
-    __shared__ double s_ck0[(NLONG - 1) * BLK];
+    // (and NJL of its J-values, the same way)
+    static_assert((NLONG - 1 + NJL) * BLK * sizeof(double) <= 160 * 1024, "LDS per workgroup");
+    __shared__ double s_ck0[(NLONG - 1 + NJL) * BLK];
     double *ck0 = s_ck0 + threadIdx.x;
+    double *tjl = s_ck0 + (NLONG - 1) * BLK + threadIdx.x;
     const int64_t b = (int64_t)blockIdx.x * BLK + threadIdx.x;
     if (b >= a.nbox) return;
     Box x;
     box_load<BLK>(a, rec, b, x, ck0, false);
+#pragma unroll
+    for (int i = 0; i < NJL; ++i) tjl[i * BLK] = x.tj[JL_SET[i]];
     Track notrack{true, true, 0., 0.};
     for (int it = 0; it < a.niter; ++it) {
-        // keeps the LDS reads of ck0 inside the loop (not hoisted into registers)
+        // keeps the LDS reads of ck0 and the J-values inside the loop (not hoisted into
+        // registers)
         asm volatile("" ::: "memory");
-        sweep<BLK, false>(x.R, x.tj, x.e, ck0, x.ck, x.cc, x.ch3o, x.hco, notrack);
+        sweep<BLK, false, true>(x.R, x.tj, x.e, ck0, x.ck, x.cc, x.ch3o, x.hco, notrack, tjl);
     }
     box_writeback(a, b, x);
 }
@@ -1569,8 +1611,11 @@ template <int BLK, int MODE>
 __global__ void __launch_bounds__(BLK) k_chem_conv(StepArgs a, ConvArgs cv)
 {
     constexpr int AAM = MODE == CONV_ANDERSON ? AA_DEPTH : 0;
-    __shared__ double s_ck0[(NLONG - 1) * BLK];
+    // ck0 and NJL J-values in LDS, as in k_chem
+    static_assert((NLONG - 1 + NJL) * BLK * sizeof(double) <= 160 * 1024, "LDS per workgroup");
+    __shared__ double s_ck0[(NLONG - 1 + NJL) * BLK];
     double *ck0 = s_ck0 + threadIdx.x;
+    double *tjl = s_ck0 + (NLONG - 1) * BLK + threadIdx.x;
     const int lane = threadIdx.x & (warpSize - 1);
     // the lane's Anderson history: one contiguous block of 2 (AAM + 1) + 1 rows of NX
     double *ws = AAM > 0 ? cv.aa_ws + ((int64_t)blockIdx.x * BLK + threadIdx.x) *
@@ -1629,6 +1674,8 @@ __global__ void __launch_bounds__(BLK) k_chem_conv(StepArgs a, ConvArgs cv)
         }
         if (setup) {
             box_load<BLK>(a, cv.rec, b, x, ck0, !fallback && !resume);
+#pragma unroll
+            for (int i = 0; i < NJL; ++i) tjl[i * BLK] = x.tj[JL_SET[i]];
             active = true;
             restart = false;
             it = 0;
@@ -1673,7 +1720,9 @@ __global__ void __launch_bounds__(BLK) k_chem_conv(StepArgs a, ConvArgs cv)
                 // the first sweep from a surrogate's answer is its acceptance test
                 const bool accept = cv.accept_tol > 0. && it == 0 && !fallback;
                 Track tr{true, true, accept ? cv.accept_tol : cv.tol, 1.e-30 * x.e.hnm};
-                sweep<BLK, true>(x.R, x.tj, x.e, ck0, x.ck, x.cc, x.ch3o, x.hco, tr);
+                // keeps the LDS reads inside the loop (not hoisted into registers)
+                asm volatile("" ::: "memory");
+                sweep<BLK, true, true>(x.R, x.tj, x.e, ck0, x.ck, x.cc, x.ch3o, x.hco, tr, tjl);
                 ++it;
                 if (fallback) {
                     if (it >= a.niter) {
