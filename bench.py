@@ -61,7 +61,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-procs", type=int, default=16)
+    ap.add_argument("--cpu-procs", type=int, default=0,
+                    help="CPU baseline processes (0: the cores this process may use)")
     ap.add_argument("--cpu-nrep", type=int, default=8)
     ap.add_argument("--no-extras", action="store_true",
                     help="only the headline step (for profiler runs)")
@@ -76,12 +77,54 @@ def parse():
     return ap.parse_args()
 
 
+def default_cpu_procs() -> int:
+    """Host cores this process may use: its CPU affinity set, capped at the job's CPU
+    share where the environment states one (OMP_NUM_THREADS)."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    share = os.environ.get("OMP_NUM_THREADS", "")
+    return max(1, min(n, int(share))) if share.isdigit() and int(share) > 0 else n
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _port_rate(sub, table, nthreads, min_s=2.0):
+    """Box-steps/s of the C restatement on `sub` with nthreads OpenMP threads: successive
+    steps for at least min_s seconds."""
+    import cport
+
+    g = sub.grid
+    nb = g.nlon * g.nlat * g.nivbas
+    x = sub.copy()
+    n, t0 = 0, time.perf_counter()
+    while True:
+        out = cport.chem_step(x, *table, nthreads=nthreads, conserv=False)
+        x.qj1, x.hc = out["qj1_chem"], out["hc_chem"]
+        n += 1
+        dt = time.perf_counter() - t0
+        if dt >= min_s:
+            return n * nb / dt
+
+
 def cpu_baseline(inp, nproc, nrep):
     """Reference Fortran (flang build) on a bounded sample: 180 columns spread over the
-    grid (one reference latitude row, 94 levels), nrep successive steps per process."""
+    grid (one reference latitude row), nrep successive steps per process in nproc
+    concurrent single-threaded processes.  The reference always computes its 94 levels: the
+    sample's 60 levels sit at reference levels 1-60 and level 60 is repeated on 61-94, so
+    81 of 94 levels are heterogeneous against 47 of 60 in the c1 grid.  Beside it, the C
+    restatement on nproc threads on the same 94-level sample and on the sample's own 60
+    levels gives the level-mix factor, and with it the reference's rate on c1's levels."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import refrun
-    from reprobus_amd import synth
+    from reprobus_amd import jtable, synth
 
     if not refrun.available():
         return None
@@ -93,16 +136,38 @@ def cpu_baseline(inp, nproc, nrep):
     flat = lambda a: np.ascontiguousarray(a.reshape(a.shape[:-2] + (ncol,))[..., cols][..., None, :])
     sub.T, sub.p, sub.sza, sub.alt = flat(inp.T), flat(inp.p), flat(inp.sza), flat(inp.alt)
     sub.qj1, sub.hc = flat(inp.qj1), flat(inp.hc)
-    nproc = max(1, min(nproc, os.cpu_count() or 1))
+    affinity = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None
     r = refrun.time_reference(sub, nrep=nrep, nproc=nproc)
+    # the 94-level embedding the reference runs, as the restatement's input
+    ext = 94 - g.nivbas
+    rep = lambda a, ax: np.ascontiguousarray(np.concatenate(
+        [a, np.repeat(np.take(a, [g.nivbas - 1], axis=ax), ext, axis=ax)], axis=ax))
+    s94 = sub.copy()
+    s94.grid = synth.Grid(180, 1, 94, 94, g.nivhau)
+    s94.T, s94.p, s94.sza, s94.alt = (rep(a, 0) for a in (sub.T, sub.p, sub.sza, sub.alt))
+    s94.qj1, s94.hc = rep(sub.qj1, 1), rep(sub.hc, 1)
+    table = jtable.default_table()
+    port60 = _port_rate(sub, table, nproc)
+    port94 = _port_rate(s94, table, nproc)
+    factor = port60 / port94
     return {"value": r["box_steps_per_s"], "unit": "box-steps/s", "cores": nproc,
             "kind": "reference",
+            "cpu_model": cpu_model(), "host_affinity_cores": affinity,
+            "levels": (f"the reference's 94 levels: the sample's {g.nivbas} at reference levels "
+                       f"1-{g.nivbas}, level {g.nivbas} repeated on {g.nivbas + 1}-94 "
+                       f"(heterogeneous chemistry on {94 - g.nivhau + 1} of 94 levels; c1: "
+                       f"{g.nivbas - g.nivhau + 1} of {g.nivbas})"),
             "sample": (f"flang build of the reference chem3d (src/reprobus_l137_20230723.f + "
                        f"jno.f90), {nproc} concurrent single-threaded processes, each "
                        f"{nrep} successive steps of 180 columns sampled across the 144x91 grid "
-                       f"x 94 reference levels ({r['calls']} row calls, {r['chem3d_s']:.1f} "
-                       f"CPU-s); rate summed over processes, timed around chem3d"),
-            "per_core": r["per_core_box_steps_per_s"]}
+                       f"({r['calls']} row calls, {r['chem3d_s']:.1f} CPU-s); rate summed over "
+                       f"processes, timed around chem3d"),
+            "per_core": r["per_core_box_steps_per_s"],
+            "c1_levels": {"value": r["box_steps_per_s"] * factor,
+                          "how": (f"the reference's rate x the C restatement's ratio on {nproc} "
+                                  f"threads between the sample's own {g.nivbas} levels "
+                                  f"({port60:.3g} box-steps/s) and the 94-level embedding "
+                                  f"({port94:.3g}): {factor:.3f}")}}
 
 
 def _device_state(inp, dev):
@@ -579,7 +644,7 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         try:
-            cpu = cpu_baseline(inp, args.cpu_procs, args.cpu_nrep)
+            cpu = cpu_baseline(inp, args.cpu_procs or default_cpu_procs(), args.cpu_nrep)
         except Exception as exc:  # the baseline is reported, never the target
             cpu = {"error": str(exc)}
 

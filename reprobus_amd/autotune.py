@@ -35,8 +35,14 @@ class TuneResult:
     cached: bool = False
 
 
-def _key(device_name: str, grid, mode: str) -> str:
-    return f"{device_name}|{grid.nlon}x{grid.nlat}x{grid.niv}/{grid.nivbas}|{mode}"
+def _key(device_name: str, grid, mode: str, accel: int | None = None,
+         tol: float | None = None) -> str:
+    """Cache key: device, grid and mode; the converged mode's choice also depends on the
+    Anderson depth and the tolerance it was timed with."""
+    k = f"{device_name}|{grid.nlon}x{grid.nlat}x{grid.niv}/{grid.nivbas}|{mode}"
+    if accel is not None:
+        k += f"|accel={int(accel)}|tol={float(tol):g}"
+    return k
 
 
 def load_cache(path: str | None) -> dict:
@@ -82,9 +88,10 @@ def tune(table, grid, state: dict, mode: str = "plain", candidates=BLOCKS, reps:
 
     name = torch.cuda.get_device_name(device) if timer is None else "test-device"
     cache = load_cache(cache_path)
-    key = _key(name, grid, mode)
     conv = mode != "plain"
-    if key in cache and cache[key]["block"] in candidates:
+    key = _key(name, grid, mode, accel, tol) if conv else _key(name, grid, mode)
+    if key in cache and cache[key]["block"] in candidates and (
+            not conv or cache[key].get("refill") in refills):
         c = cache[key]
         return TuneResult(mode, c["block"], c.get("refill"), c["ms"], cached=True)
 

@@ -21,15 +21,50 @@ from . import _lib
 from ._lib import Config, DayConfig, Grid, RbError, Stats
 
 
-def _ptr(a) -> int:
+def _numel(a) -> int:
+    return int(a.size) if isinstance(a, np.ndarray) else int(a.numel())
+
+
+def _ptr(a, n: int | None = None, what: str = "array") -> int:
+    """Address of a C-contiguous float64 numpy array or GPU tensor holding at least n
+    values (the extent the C ABI reads or writes)."""
     if isinstance(a, np.ndarray):
         if a.dtype != np.float64 or not a.flags.c_contiguous:
-            raise TypeError("arrays must be C-contiguous float64")
-        return a.ctypes.data
-    # torch tensor
-    if a.dtype.__repr__() != "torch.float64" or not a.is_contiguous() or a.device.type != "cuda":
-        raise TypeError("tensors must be contiguous float64 on a GPU")
+            raise TypeError(f"{what}: arrays must be C-contiguous float64")
+        p = a.ctypes.data
+    else:   # torch tensor
+        if a.dtype.__repr__() != "torch.float64" or not a.is_contiguous() or a.device.type != "cuda":
+            raise TypeError(f"{what}: tensors must be contiguous float64 on a GPU")
+        p = a.data_ptr()
+    if n is not None and _numel(a) < n:
+        raise ValueError(f"{what}: {_numel(a)} values, the grid needs {n}")
+    return p
+
+
+def _iptr(a, n: int, what: str) -> int:
+    """Address of a contiguous int32 GPU tensor of at least n values."""
+    if a.dtype.__repr__() != "torch.int32" or not a.is_contiguous() or a.device.type != "cuda":
+        raise TypeError(f"{what}: must be a contiguous int32 tensor on a GPU")
+    if _numel(a) < n:
+        raise ValueError(f"{what}: {_numel(a)} values, the grid needs {n}")
     return a.data_ptr()
+
+
+def _extents(grid) -> tuple[int, int, int]:
+    """(field, nbox, plane): values per (nlon, nlat, niv) field, chemistry boxes, columns."""
+    plane = int(grid.nlon) * int(grid.nlat)
+    return plane * int(grid.niv), plane * int(grid.nivbas), plane
+
+
+def _step_ptrs(grid, T, p, sza, alt, qj1, hc, surfarea, vsed3d, parth2osed, parthno3sed):
+    """Checked addresses of a chemistry step's ten arrays (inputs read on the chemistry
+    levels, the state on every level)."""
+    field, nbox, _ = _extents(grid)
+    sizes = (nbox, nbox, nbox, nbox, 43 * field, 15 * field, field, nbox, nbox, nbox)
+    names = ("T", "p", "sza", "alt", "qj1", "hc", "surfarea", "vsed3d", "parth2osed",
+             "parthno3sed")
+    return [_ptr(a, n, w) for a, n, w in zip((T, p, sza, alt, qj1, hc, surfarea, vsed3d,
+                                              parth2osed, parthno3sed), sizes, names)]
 
 
 def make_grid(nlon: int, nlat: int, niv: int, nivbas: int, nivhau: int) -> Grid:
@@ -92,20 +127,22 @@ class ChemistryStep:
                parth2osed, parthno3sed) -> None:
         """All rows of chem3d (F:398-415) on host arrays, in place."""
         self._check(self._L.rb_chem_step(
-            self._ctx, C.byref(grid), float(dt), *[_ptr(a) for a in (
-                T, p, sza, alt, qj1, hc, surfarea, vsed3d, parth2osed, parthno3sed)]))
+            self._ctx, C.byref(grid), float(dt), *_step_ptrs(
+                grid, T, p, sza, alt, qj1, hc, surfarea, vsed3d, parth2osed, parthno3sed)))
 
     def conserv(self, grid: Grid, qj1, hc) -> None:
         """conserv (F:2526-2616) + floor (F:431-440) on host arrays, in place."""
-        self._check(self._L.rb_conserv_floor(self._ctx, C.byref(grid), _ptr(qj1), _ptr(hc)))
+        field = _extents(grid)[0]
+        self._check(self._L.rb_conserv_floor(self._ctx, C.byref(grid), _ptr(qj1, 43 * field, "qj1"),
+                                             _ptr(hc, 15 * field, "hc")))
 
     # -- device tensors -----------------------------------------------------------
     def step_device(self, grid: Grid, dt: float, T, p, sza, alt, qj1, hc, surfarea, vsed3d,
                     parth2osed, parthno3sed, fuse_conserv: bool = True) -> None:
         """Enqueue one step on device tensors (conserv + floor fused when asked)."""
         self._check(self._L.rb_chem_step_dev(
-            self._ctx, C.byref(grid), float(dt), *[_ptr(a) for a in (
-                T, p, sza, alt, qj1, hc, surfarea, vsed3d, parth2osed, parthno3sed)],
+            self._ctx, C.byref(grid), float(dt), *_step_ptrs(
+                grid, T, p, sza, alt, qj1, hc, surfarea, vsed3d, parth2osed, parthno3sed),
             1 if fuse_conserv else 0))
 
     def converge_device(self, grid: Grid, dt: float, tol: float, maxiter: int, T, p, sza, alt,
@@ -114,13 +151,14 @@ class ChemistryStep:
                         accel: int = 0) -> None:
         """Converged mode: sweep every box to the fixed point (optionally from a guess;
         accel = 2: Anderson-accelerated sweeps)."""
-        g_q = _ptr(guess_qj1) if guess_qj1 is not None else None
-        g_h = _ptr(guess_hc) if guess_hc is not None else None
-        it = iters.data_ptr() if iters is not None else None
+        field, nbox, _ = _extents(grid)
+        g_q = _ptr(guess_qj1, 42 * field, "guess_qj1") if guess_qj1 is not None else None
+        g_h = _ptr(guess_hc, 15 * field, "guess_hc") if guess_hc is not None else None
+        it = _iptr(iters, nbox, "iters") if iters is not None else None
         self._check(self._L.rb_chem_converge_dev(
             self._ctx, C.byref(grid), float(dt), float(tol), int(maxiter), int(accel),
-            *[_ptr(a) for a in (T, p, sza, alt, qj1, hc, surfarea, vsed3d, parth2osed,
-                                parthno3sed)], g_q, g_h, it, 1 if fuse_conserv else 0))
+            *_step_ptrs(grid, T, p, sza, alt, qj1, hc, surfarea, vsed3d, parth2osed,
+                        parthno3sed), g_q, g_h, it, 1 if fuse_conserv else 0))
 
     def surrogate_device(self, grid: Grid, dt: float, accept_tol: float, tol: float,
                          maxiter: int, T, p, sza, alt, qj1, hc, surfarea, vsed3d, parth2osed,
@@ -129,11 +167,13 @@ class ChemistryStep:
         """Surrogate step: keep the surrogate's answer (sur_qj1, sur_hc) where one FP64
         sweep from it changes every species by less than accept_tol (iters 0), sweep the
         other boxes on to tol from it."""
-        it = iters.data_ptr() if iters is not None else None
+        field, nbox, _ = _extents(grid)
+        it = _iptr(iters, nbox, "iters") if iters is not None else None
         self._check(self._L.rb_chem_surrogate_dev(
             self._ctx, C.byref(grid), float(dt), float(accept_tol), float(tol), int(maxiter),
-            int(accel), *[_ptr(a) for a in (T, p, sza, alt, qj1, hc, surfarea, vsed3d,
-                                            parth2osed, parthno3sed, sur_qj1, sur_hc)], it,
+            int(accel), *_step_ptrs(grid, T, p, sza, alt, qj1, hc, surfarea, vsed3d,
+                                    parth2osed, parthno3sed),
+            _ptr(sur_qj1, 42 * field, "sur_qj1"), _ptr(sur_hc, 15 * field, "sur_hc"), it,
             1 if fuse_conserv else 0))
 
     def sediment(self, grid: Grid, dt: float, nivh2o: int, T, p, qj1, vsed3d, parth2osed,
@@ -182,17 +222,23 @@ class ChemistryStep:
     def transport_weights_device(self, grid: Grid, aaa, bbb, x0, y0, z0, p0, idx, w) -> None:
         """Cell indices idx (3, n) int32 and Lagrange weights w (12, n) of the departure
         points (device tensors; posx..poidz)."""
+        n = _extents(grid)[0]
+        niv = int(grid.niv)
         self._check(self._L.rb_transport_weights_dev(
-            self._ctx, C.byref(grid), *[_ptr(a) for a in (aaa, bbb, x0, y0, z0, p0)],
-            idx.data_ptr(), _ptr(w)))
+            self._ctx, C.byref(grid), _ptr(aaa, niv, "aaa"), _ptr(bbb, niv, "bbb"),
+            *[_ptr(a, n, w_) for a, w_ in ((x0, "x0"), (y0, "y0"), (z0, "z0"), (p0, "p0"))],
+            _iptr(idx, 3 * n, "idx"), _ptr(w, 12 * n, "w")))
 
     def transport_interp_device(self, grid: Grid, idx, w, q_in, q_out) -> None:
         """The ntr fields of q_in (ntr, niv, nlat, nlon) interpolated at the departure points
         into q_out (internew for each tracer)."""
         ntr = int(q_in.shape[0])
-        assert tuple(q_out.shape) == tuple(q_in.shape)
-        self._check(self._L.rb_transport_interp_dev(self._ctx, C.byref(grid), idx.data_ptr(),
-                                                    _ptr(w), ntr, _ptr(q_in), _ptr(q_out)))
+        n = _extents(grid)[0]
+        if tuple(q_out.shape) != tuple(q_in.shape):
+            raise ValueError("q_out must have the shape of q_in")
+        self._check(self._L.rb_transport_interp_dev(
+            self._ctx, C.byref(grid), _iptr(idx, 3 * n, "idx"), _ptr(w, 12 * n, "w"), ntr,
+            _ptr(q_in, ntr * n, "q_in"), _ptr(q_out, ntr * n, "q_out")))
 
     def transport(self, grid: Grid, aaa, bbb, x0, y0, z0, p0, q) -> None:
         """Host arrays: weights and interpolation of the ntr fields of q (ntr, niv, nlat,
@@ -210,16 +256,24 @@ class ChemistryStep:
                        maxiter: int = 2000, iters=None, accel: int = 0) -> None:
         """nsteps sub-steps (device zenith + chemistry, conserv fused), optionally as one
         captured HIP graph."""
+        field, nbox, _ = _extents(grid)
         dc = DayConfig(int(nsteps), 1 if use_graph else 0, 1 if converged else 0, int(maxiter),
                        float(day0), float(dt), float(tol), int(accel), 0)
         self._check(self._L.rb_run_day_dev(
             self._ctx, C.byref(grid), C.byref(dc),
-            *[_ptr(a) for a in (trajlon, trajlat, T, p, alt, sza, qj1, hc, surfarea, vsed3d,
-                                parth2osed, parthno3sed)],
-            iters.data_ptr() if iters is not None else None))
+            *[_ptr(a, n, w) for a, n, w in zip(
+                (trajlon, trajlat, T, p, alt, sza, qj1, hc, surfarea, vsed3d, parth2osed,
+                 parthno3sed),
+                (nbox,) * 6 + (43 * field, 15 * field, field, nbox, nbox, nbox),
+                ("trajlon", "trajlat", "T", "p", "alt", "sza", "qj1", "hc", "surfarea",
+                 "vsed3d", "parth2osed", "parthno3sed"))],
+            _iptr(iters, nbox, "iters") if iters is not None else None))
 
     def conserv_device(self, grid: Grid, qj1, hc) -> None:
-        self._check(self._L.rb_conserv_floor_dev(self._ctx, C.byref(grid), _ptr(qj1), _ptr(hc)))
+        field = _extents(grid)[0]
+        self._check(self._L.rb_conserv_floor_dev(self._ctx, C.byref(grid),
+                                                 _ptr(qj1, 43 * field, "qj1"),
+                                                 _ptr(hc, 15 * field, "hc")))
 
     # -- convenience on synth.StepInputs -------------------------------------------
     def run_inputs(self, inp, conserv: bool = True) -> dict:

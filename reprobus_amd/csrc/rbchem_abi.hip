@@ -120,6 +120,23 @@ bool any_null(std::initializer_list<const void *> ps)
     return false;
 }
 
+// rb_config.check_finite: count the non-finite values of the state after a step (the
+// whole qj1 and hc fields), report the count in rb_stats and return RB_ENONFINITE when
+// there is any.  It synchronises the stream, so it is skipped inside a graph capture.
+int finite_check(rb_ctx *ctx, const double *qj1, const double *hc, int64_t field)
+{
+    if (!ctx->cfg.check_finite || ctx->capturing) return RB_OK;
+    HIPCHK(rb::launch_zero_counter(ctx->d_count, ctx->stream));
+    HIPCHK(rb::launch_count_nonfinite(qj1, 43 * field, ctx->d_count, ctx->stream));
+    HIPCHK(rb::launch_count_nonfinite(hc, 15 * field, ctx->d_count, ctx->stream));
+    unsigned long long cnt = 0;
+    HIPCHK(hipMemcpyAsync(&cnt, ctx->d_count, sizeof cnt, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    ctx->stats.nonfinite = (int64_t)cnt;
+    if (cnt) return fail(ctx, RB_ENONFINITE, std::to_string(cnt) + " non-finite outputs");
+    return RB_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -242,18 +259,7 @@ int rb_chem_step_dev(rb_ctx *ctx, const rb_grid *g, double dt, const double *T, 
         ctx->timed = true;
     }
     ctx->stats.boxes = nbox;
-
-    if (ctx->cfg.check_finite && !ctx->capturing) {
-        HIPCHK(rb::launch_zero_counter(ctx->d_count, ctx->stream));
-        HIPCHK(rb::launch_count_nonfinite(qj1, 43 * field, ctx->d_count, ctx->stream));
-        HIPCHK(rb::launch_count_nonfinite(hc, 15 * field, ctx->d_count, ctx->stream));
-        unsigned long long cnt = 0;
-        HIPCHK(hipMemcpyAsync(&cnt, ctx->d_count, sizeof cnt, hipMemcpyDeviceToHost, ctx->stream));
-        HIPCHK(hipStreamSynchronize(ctx->stream));
-        ctx->stats.nonfinite = (int64_t)cnt;
-        if (cnt) return fail(ctx, RB_ENONFINITE, std::to_string(cnt) + " non-finite outputs");
-    }
-    return RB_OK;
+    return finite_check(ctx, qj1, hc, field);
 }
 
 // the converged mode; accept_tol > 0 makes it the surrogate step (guess = the surrogate)
@@ -330,7 +336,7 @@ static int converge_impl(rb_ctx *ctx, const rb_grid *g, double dt, double tol, i
     if (!ctx->capturing) HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
     ctx->timed = !ctx->capturing;
     ctx->stats.boxes = nbox;
-    return RB_OK;
+    return finite_check(ctx, qj1, hc, field);
 }
 
 int rb_chem_converge_dev(rb_ctx *ctx, const rb_grid *g, double dt, double tol, int32_t maxiter,
@@ -403,8 +409,11 @@ int rb_chem_step(rb_ctx *ctx, const rb_grid *g, double dt, const double *T, cons
     HIPCHK(hipMemcpyAsync(dq, qj1, 43 * field * B, hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(dh, hc, 15 * field * B, hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(dsurf, surfarea, field * B, hipMemcpyHostToDevice, s));
-    rc = rb_chem_step_dev(ctx, g, dt, dT, dp, dsza, dalt, dq, dh, dsurf, dv, dp2o, dpno3, 0);
-    if (rc) return rc;
+    // a step with non-finite outputs still hands them back (then returns RB_ENONFINITE)
+    const int step_rc = rb_chem_step_dev(ctx, g, dt, dT, dp, dsza, dalt, dq, dh, dsurf, dv, dp2o,
+                                         dpno3, 0);
+    if (step_rc && step_rc != RB_ENONFINITE) return step_rc;
+    const std::string step_err = ctx->err;
     HIPCHK(hipMemcpyAsync(qj1, dq, 43 * field * B, hipMemcpyDeviceToHost, s));
     HIPCHK(hipMemcpyAsync(hc, dh, 15 * field * B, hipMemcpyDeviceToHost, s));
     HIPCHK(hipMemcpyAsync(surfarea, dsurf, field * B, hipMemcpyDeviceToHost, s));
@@ -416,7 +425,8 @@ int rb_chem_step(rb_ctx *ctx, const rb_grid *g, double dt, const double *T, cons
     float ms = 0.f;
     if (hipEventElapsedTime(&ms, ctx->ev_h2d, ctx->ev0) == hipSuccess) ctx->stats.h2d_ms = ms;
     if (hipEventElapsedTime(&ms, ctx->ev1, ctx->ev_d2h) == hipSuccess) ctx->stats.d2h_ms = ms;
-    return RB_OK;
+    if (step_rc) ctx->err = step_err;
+    return step_rc;
 }
 
 int rb_conserv_floor(rb_ctx *ctx, const rb_grid *g, double *qj1, double *hc)

@@ -4,6 +4,7 @@ on small synthetic grids.
 
 Run from the repository root where /root/reference is present:
     make -C oracle ref && python tests/golden/make_golden.py
+(``python tests/golden/make_golden.py wet`` or ``transport`` writes that fixture only.)
 
 Each chemistry fixture stores the step inputs (with the SZA the reference realised,
 ``sza3d``) and the reference outputs after chem3d and after conserv + floor; the
@@ -77,6 +78,27 @@ def sedi_grid(seed: int = 9, ncol: int = 8, nlev: int = 94):
                             {"name": "sedi", "seed": seed})
 
 
+def wet_grid(seed: int = 15, ncol: int = 8, nlev: int = 94):
+    """Rain-out (F:1315-1338): the reference's 94 levels with tropospheric water on the
+    lower ones -- H2O log-uniform 30 ppmv to 3000 ppmv on levels 55..94, so most of those
+    boxes sit above the 100 ppmv threshold and some below it -- at 220-260 K there (liquid
+    aerosol only: no NAT, ice or ternary solution)."""
+    rng = np.random.default_rng(seed)
+    g = synth.Grid(nlon=ncol, nlat=1, niv=nlev, nivbas=nlev)
+    shape = g.shape3
+    p = np.broadcast_to(synth.level_pressures(nlev)[:, None, None], shape).copy()
+    T = rng.uniform(200.0, 240.0, shape)
+    T[54:] = rng.uniform(220.0, 260.0, (nlev - 54,) + shape[1:])
+    sza = np.degrees(np.arccos(rng.uniform(-0.3, 1.0, shape)))
+    q, h = synth.draw_species(rng, shape)
+    q[synth.L["h2o"], 54:] = np.exp(rng.uniform(np.log(30e-6), np.log(3e-3),
+                                                (nlev - 54,) + shape[1:]))
+    sad = np.exp(rng.uniform(np.log(0.1), np.log(10.0), shape)) * 1e-8
+    q[synth.L["h2so4"]] = synth.h2so4_for_sad(T, p, q[synth.L["h2o"]], sad)
+    return synth.StepInputs(g, 900.0, T, p, sza, synth.altitude_km(p), q, h, sad,
+                            {"name": "wet", "seed": seed})
+
+
 DTSEDI = 10800.0       # 12 sub-steps of 900 s
 NIVH2O = 59            # nivh2oecmwf, F:18
 
@@ -131,6 +153,7 @@ def main():
     save("psc_seed7_8col", psc, refrun.run_reference(psc))
     sedi = sedi_grid()
     save("sedi_seed9_8col", sedi, refrun.run_reference(sedi, dtsedi=DTSEDI))
+    wet_fixture()
     d = input_producers()
     sza, alt = refrun.run_inputs_reference(d["days"], d["zlon"], d["zlat"], d["relief"],
                                            d["psol"], d["aaa"], d["bbb"], d["t"], d["h2o"],
@@ -183,8 +206,15 @@ def ecmwf_fixture(seed: int = 13, nsamp: int = 3000):
     print(f"wrote ecmwf_seed{seed}: days {ref['daym1']} {ref['dayp1']}")
 
 
+def wet_fixture():
+    wet = wet_grid()
+    save("wet_seed15_8col", wet, refrun.run_reference(wet))
+
+
 if __name__ == "__main__":
     if sys.argv[1:] == ["transport"]:
         transport_fixture()
+    elif sys.argv[1:] == ["wet"]:
+        wet_fixture()
     else:
         main()

@@ -26,6 +26,17 @@ def test_choice_and_cache(tmp_path):
     assert r4.cached and (r4.block, r4.refill) == (128, 16)
     other = make_grid(72, 46, 60, 60, 14)
     assert not autotune.tune(None, other, {}, cache_path=path, timer=timer).cached
+    # a converged-mode choice is cached per Anderson depth and tolerance
+    n = len(calls)
+    r5 = autotune.tune(None, grid, {}, mode="converged", cache_path=path, timer=timer, accel=0)
+    assert not r5.cached and len(calls) == n + 9
+    r6 = autotune.tune(None, grid, {}, mode="converged", cache_path=path, timer=timer,
+                       tol=1e-10)
+    assert not r6.cached
+    # a cached chunk outside the requested ones is not served
+    r7 = autotune.tune(None, grid, {}, mode="converged", cache_path=path, timer=timer,
+                       refills=(8, 32))
+    assert not r7.cached and r7.refill in (8, 32)
 
 
 def test_ties_go_to_the_larger_block_then_chunk():

@@ -93,7 +93,9 @@ def test_converged_matches_restatement(step, table, cr_psc):
         d = np.maximum(rel_err(out["qj1"], ref["qj1"]).max(axis=0),
                        rel_err(out["hc"], ref["hc"]).max(axis=0))
         over = d > PARITY_TOL
-        assert over.sum() <= 1e-3 * d.size and np.all(inp.T[over] < 215.0)
+        print(f"[converge] C-library variant: {int(over.sum())} of {d.size} boxes above "
+              f"{PARITY_TOL:g}, max {d.max():.2e}", flush=True)
+        assert over.sum() <= 5 and d.max() < 1e-8 and np.all(inp.T[over] < 215.0)
 
 
 @pytest.mark.parametrize("seed", [1, 0])

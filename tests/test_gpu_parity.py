@@ -13,9 +13,21 @@ from conftest import FLOOR, PARITY_TOL, load_fixture, max_rel_by_species, rel_er
 
 pytestmark = pytest.mark.gpu
 
-FIXTURES = ["c0_seed0_16col", "psc_seed7_8col"]   # (sedi_seed9_8col: its own test)
+FIXTURES = ["c0_seed0_16col", "psc_seed7_8col", "wet_seed15_8col"]   # (sedi: its own test)
 KEYS = ("qj1_chem", "hc_chem", "qj1", "hc")
 AUX = ("surfarea", "vsed3d", "parth2osed", "parthno3sed")
+
+
+def _species_maxima(tag, out, ref):
+    """Print the largest relative difference of every species (final state)."""
+    from reprobus_amd import synth
+
+    mq = max_rel_by_species(out["qj1"], ref["qj1"])
+    mh = max_rel_by_species(out["hc"], ref["hc"])
+    top = sorted([(float(v), n) for n, v in zip(synth.LONG, mq)] +
+                 [(float(v), n) for n, v in zip(synth.SHORT, mh)], reverse=True)[:8]
+    print(f"[parity] {tag}, largest per species: " +
+          ", ".join(f"{n} {v:.1e}" for v, n in top), flush=True)
 
 
 def _report(tag, out, ref):
@@ -156,7 +168,9 @@ def test_gpu_full_grid_parity(step, table):
     over = d > PARITY_TOL
     print(f"[parity] c1 full vs C-library restatement: max {d.max():.2e}, "
           f"{int(over.sum())} of {d.size} boxes above {PARITY_TOL:g}", flush=True)
-    assert over.sum() <= 1e-4 * d.size
+    _species_maxima("c1 full vs C-library restatement", out, lib)
+    # measured on MI355X: 10 boxes, max 7.3e-9, all ternary
+    assert over.sum() <= 12 and d.max() < 1e-8
     assert np.all(inp.T[over] < 215.0)
 
 
@@ -182,7 +196,9 @@ def test_gpu_full_grid_vs_reference_fortran(step):
     print(f"[parity] c1 full vs reference Fortran: max {d.max():.2e}, median {np.median(d):.2e}, "
           f"{int(over.sum())} of {d.size} boxes above {PARITY_TOL:g} (all ternary: "
           f"{bool(np.all(inp.T[over] < 215.0))})", flush=True)
-    assert over.sum() <= 1e-4 * d.size
+    _species_maxima("c1 full vs reference Fortran", out, ref)
+    # measured on MI355X: 3 boxes, max 2.8e-9, all ternary
+    assert over.sum() <= 5 and d.max() < 5e-9
     assert np.all(inp.T[over] < 215.0)
     for key in ("surfarea", "vsed3d", "parth2osed", "parthno3sed"):
         assert np.isfinite(out[key]).all()

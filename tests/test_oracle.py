@@ -13,7 +13,7 @@ from conftest import FLOOR, load_fixture, max_rel_by_species, rel_err
 
 cport = pytest.importorskip("cport")
 
-FIXTURES = ["c0_seed0_16col", "psc_seed7_8col", "sedi_seed9_8col"]
+FIXTURES = ["c0_seed0_16col", "psc_seed7_8col", "sedi_seed9_8col", "wet_seed15_8col"]
 
 
 @pytest.fixture(scope="module")
@@ -111,6 +111,33 @@ def test_restatement_matches_reference_golden(name, table):
         assert worst < 1e-11, (key, worst)
     for key in ("surfarea", "vsed3d", "parth2osed", "parthno3sed"):
         assert rel_err(out[key], ref[key], floor=1e-300).max() < 1e-11, key
+
+
+def test_wet_fixture_exercises_rainout(table):
+    """The wet fixture pins the rain-out branch (F:1315-1338): it holds boxes on both sides
+    of the 100 ppmv threshold, and moving its wet boxes across the threshold (H2O 1e-9
+    relative above or below it: the same chemistry otherwise) changes HNO3, HCl, HBr and
+    H2O2 by about dt x 1e-6 s^-1 = 9e-4, and HNO4, CH2O and CH3OOH (rate 0 in the branch)
+    only through the chemistry, 100x less.  A wrong rate or flag therefore moves the step by
+    far more than the 1e-11 the restatement agrees with the reference on this fixture."""
+    from reprobus_amd import synth
+
+    inp, _ = load_fixture("wet_seed15_8col")
+    h2o = inp.qj1[synth.L["h2o"]]
+    wet = h2o >= 100e-6
+    assert wet.sum() >= 100 and ((h2o < 100e-6) & (h2o > 30e-6)).sum() >= 50
+    a, b = inp.copy(), inp.copy()
+    a.qj1[synth.L["h2o"]][wet] = 100e-6 * (1 + 1e-9)
+    b.qj1[synth.L["h2o"]][wet] = 100e-6 * (1 - 1e-9)
+    oa = cport.chem_step(a, *table, conserv=False)
+    ob = cport.chem_step(b, *table, conserv=False)
+    d = np.abs(oa["qj1_chem"] - ob["qj1_chem"]) / np.abs(ob["qj1_chem"])
+    for n in ("hno3", "hcl", "h2o2"):
+        assert np.all((d[synth.L[n]][wet] > 7e-4) & (d[synth.L[n]][wet] < 1e-3)), n
+    assert np.all((d[synth.L["hbr"]][wet] > 4e-4) & (d[synth.L["hbr"]][wet] < 1e-3))
+    for n in ("ho2no2", "ch2o", "ch3o2h"):
+        assert d[synth.L[n]][wet].max() < 1e-4, n
+    assert np.all(d[:, ~wet] == 0)
 
 
 def test_psc_fixture_exercises_all_aerosol_branches(table):
