@@ -910,6 +910,11 @@ typedef struct {
  * Components that would turn non-positive keep f_i. */
 #define AA_MAX 4
 #define NX (NBCON - 1 + NCM)
+/* converged mode: successive plain sweeps before the distance-to-fixed-point estimate,
+ * and the cap of its contraction rate (changes at rounding level have ratios near 1 at
+ * random: the cap accepts a change below ~1e-3 tol whatever its ratios) */
+#define RBO_CONV_NPLAIN 3
+#define RBO_CONV_RHOMAX 0.999
 typedef struct {
     int m, n;                       /* depth; stored pairs (<= m + 1, ring) */
     int head;
@@ -1058,6 +1063,10 @@ static void box_step(const troe_t *K, const double *ajl, const double *o3up, con
     aa.n = 0;
     aa.head = 0;
     *iters = niter;
+    /* converged mode: largest relative change of the previous sweep, and whether that
+     * sweep was a plain one (its iterate is the sweep map's image of the one before) */
+    double dprev = 0., dprev2 = 0.;
+    int nplain = 0;
     for (int iter = 1; iter <= niter; ++iter) {
         double prev[NBCON + NCM + 1];
         if (tol > 0.) {
@@ -1336,9 +1345,17 @@ static void box_step(const troe_t *K, const double *ajl, const double *o3up, con
         /* F:2364-2372 */
         for (n = 1; n <= NBCON - 1; ++n) ck[n] = (ck0[n] + dt * pr[n]) / (1. + dt * pe[n]);
 
-        /* converged mode: stop at the fixed point of the sweep map -- every species
-         * changed by less than tol relative to its new value (values under the 1e-30
-         * mole-fraction floor compared at the floor): |v - old| < tol max(|v|, floor) */
+        /* converged mode: stop near the fixed point of the sweep map.  Every species
+         * must have changed by less than tol relative to its new value (values under the
+         * 1e-30 mole-fraction floor compared at the floor): |v - old| < tol max(|v|, fl);
+         * and, since the sweeps converge linearly (after a sweep whose largest relative
+         * change is d the distance to the fixed point is about d rho / (1 - rho), rho the
+         * contraction per sweep), the estimate of that distance must be below tol too.
+         * rho is estimated as the larger of the last two ratios of successive changes
+         * (d_k / d_(k-1), d_(k-1) / d_(k-2)), so the estimate needs RBO_CONV_NPLAIN = 3
+         * successive plain sweeps: the larger ratio keeps an oscillating sequence (the
+         * damped radicals) from showing a small rate at a low point.  With Anderson mixing,
+         * a box whose change is below tol takes unmixed steps until the estimate decides. */
         if (tol > 0.) {
             const double fl = 1.e-30 * hnm;
             /* surrogate step: the first sweep from the surrogate's answer measures its
@@ -1347,13 +1364,18 @@ static void box_step(const troe_t *K, const double *ajl, const double *o3up, con
             const int accept = iter == 1 && accept_tol > 0. && gq;
             const double tt = accept ? accept_tol : tol;
             int conv = 1, fin = 1;
+            double dk = 0.;
             for (n = 1; n <= NBCON - 1; ++n) {
-                conv = conv && (fabs(ck[n] - prev[n]) < tt * fmax(fabs(ck[n]), fl));
+                const double sc = fmax(fabs(ck[n]), fl), dv = fabs(ck[n] - prev[n]);
+                conv = conv && (dv < tt * sc);
                 fin = fin && (fabs(ck[n]) <= 1.7976931348623157e308);
+                dk = fmax(dk, dv / sc);
             }
             for (n = 1; n <= NCM; ++n) {
-                conv = conv && (fabs(cc[n] - prev[NBCON - 1 + n]) < tt * fmax(fabs(cc[n]), fl));
+                const double sc = fmax(fabs(cc[n]), fl), dv = fabs(cc[n] - prev[NBCON - 1 + n]);
+                conv = conv && (dv < tt * sc);
                 fin = fin && (fabs(cc[n]) <= 1.7976931348623157e308);
+                dk = fmax(dk, dv / sc);
             }
             if (conv && fin && accept) {
                 for (n = 1; n <= NBCON - 1; ++n) ck[n] = gq[n] * hnm;
@@ -1361,7 +1383,12 @@ static void box_step(const troe_t *K, const double *ajl, const double *o3up, con
                 *iters = 0;
                 break;
             }
-            if (conv && fin) {
+            int near = 0;
+            if (nplain >= RBO_CONV_NPLAIN) {
+                const double rho = fmin(fmax(dk / dprev, dprev / dprev2), RBO_CONV_RHOMAX);
+                near = dk == 0. || dk * rho < tol * (1. - rho);
+            }
+            if (conv && fin && near) {
                 *iters = iter;
                 break;
             }
@@ -1369,7 +1396,8 @@ static void box_step(const troe_t *K, const double *ajl, const double *o3up, con
                 *iters = -iter;       /* diverged or out of sweeps */
                 break;
             }
-            if (accel > 0 && iter >= aa_start) {
+            const int mix = accel > 0 && iter >= aa_start && !conv;
+            if (mix) {
                 double xv[NX];
                 for (n = 1; n <= NBCON - 1; ++n) xv[n - 1] = ck[n];
                 for (n = 1; n <= NCM; ++n) xv[NBCON - 2 + n] = cc[n];
@@ -1377,6 +1405,9 @@ static void box_step(const troe_t *K, const double *ajl, const double *o3up, con
                 for (n = 1; n <= NBCON - 1; ++n) ck[n] = xv[n - 1];
                 for (n = 1; n <= NCM; ++n) cc[n] = xv[NBCON - 2 + n];
             }
+            nplain = mix ? 0 : nplain + 1;
+            dprev2 = dprev;
+            dprev = dk;
         }
     }
 

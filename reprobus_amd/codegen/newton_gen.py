@@ -1,0 +1,714 @@
+"""Code generator of the Newton solve of the chemistry step's fixed point.
+
+The converged mode solves for the fixed point of the reference's sweep map (chem3d's
+QSSA + linearised backward-Euler sweep, src/reprobus_l137_20230723.f:1350-2372, swept
+until nothing changes).  At a fixed point every quantity the sweep sets equals the value
+its own formula gives from the current state, so the fixed point is the root of the
+local (Jacobi) residual
+
+    short-lived species  x_i - g_i(x) = 0   (the QSSA ratio of F:1353-1746, damping
+                                             removed: at a fixed point x = .5 x + .5 g)
+    long-lived species   ck (1 + dt L) - ck0 - dt P = 0   (F:2364-2372 with the family
+                                             loss rates L = loss / ck of F:1810-2342
+                                             multiplied out)
+
+over 55 unknowns: 40 long-lived species (OXP and H2SO4 have P = L = 0, F:1970-1976, and
+stay at ck0; HNO3 gas, slot 43, is set after the sweeps) and the 15 short-lived ones.
+Unlike the Gauss-Seidel sweep map, whose Jacobian is dense, each residual reads a
+handful of species, so the Newton matrix is sparse.  The day branch (SZA <= 95 deg) and
+the night branch (F:1671-1746) are separate systems; the non-smooth min/max terms
+(F:1900-1917, F:1660, F:1712) take the derivative of their active side.
+
+This module traces those residuals once through a small forward-mode differentiator
+whose values are C expressions (or, in numeric mode, floats), which yields
+  * straight-line code for the residual and its exact Jacobian on the sparsity pattern;
+  * the pattern, a minimum-degree pivot order on it, the symbolic fill-in of the LU
+    factors and straight-line code for the factorisation (no pivoting: the order is fixed
+    at generation time) and for the two triangular solves.
+The same trace in numeric mode evaluates residual and Jacobian in Python, which the tests
+use to check the transcription against the restatement's fixed point and the Jacobian
+against finite differences.
+
+Run ``python -m reprobus_amd.codegen.newton_gen`` to rewrite
+``reprobus_amd/csrc/newton_gen.h`` (and print the pattern statistics).
+"""
+from __future__ import annotations
+
+import os
+import sys
+
+# ------------------------------------------------------------------ species and unknowns
+LONG = ["N2O", "CH4", "H2O", "NOY", "HNO3", "N2O5", "CLY", "OX", "CO", "OCLO", "OXP",
+        "H2SO4", "HCL", "CLONO2", "HOCL", "CL2", "H2O2", "CLNO2", "HBR", "BRONO2", "NOX",
+        "HO2NO2", "CLOX", "BROX", "CL2O2", "HOBR", "BRCL", "CH2O", "CH3O2", "CH3O2H", "CFC11",
+        "CFC12", "CFC113", "CCL4", "CH3CCL3", "CH3CL", "HCFC22", "CH3BR", "H1211", "H1301",
+        "BRY", "CH2BR2", "HNO3G"]
+SHORT = ["O1D", "OH", "CL", "O3P", "O3", "HO2", "NO2", "NO", "BR", "N", "CLO", "BRO", "NO3",
+         "H", "CH3"]
+FIXED = ("OXP", "H2SO4", "HNO3G")
+UNKNOWNS = [("ck", s) for s in LONG if s not in FIXED] + [("cc", s) for s in SHORT]
+NU = len(UNKNOWNS)
+INDEX = {u: i for i, u in enumerate(UNKNOWNS)}
+# the Rates fields of the kernel (thermal rates that depend on T, M or p)
+RATES = ("a1 a1et a2 a5 a6 a6b a7 a17 a19 a26 a27 a36 b3 b4 b6 b7 b9 b12 b22 b23 b24 b27 "
+         "b28 b32 b38 b39 c2 c4 c5 c7 c8 c9 c10 c14 c15 c17a c17b c18 c19 d2 d3 d4 d5 d6 d7 "
+         "d8 d9 d10 d11 d31 d32 d33 d34 d36 d37 d60 d61 d62 d94 d95 d97 e2 e3 e4 e5a e5b "
+         "e5c e6 e7 e8 e10 e11 e13 e15 e16 e51 e54 hk1 hk2 hk3 hk4 hk5").split()
+# the temperature-independent rates, compile-time constants of the kernel
+CONSTS = ("A3ET A23A A23B A23C A30 A31 C1A C1B C1C C3 C12 D35 D90 D91 D92 D93 D96 E9 E50 "
+          "E52 E53").split()
+
+
+# ------------------------------------------------------------------ the differentiator
+class Emit:
+    """Straight-line code under construction (C mode) or nothing (numeric mode)."""
+
+    def __init__(self, numeric: bool, params: dict | None = None):
+        self.numeric = numeric
+        self.params = params or {}
+        self.lines: list[str] = []
+        self.n = 0
+
+    def tmp(self, expr: str, kind: str = "double") -> str:
+        name = f"t{self.n}"
+        self.n += 1
+        self.lines.append(f"const {kind} {name} = {expr};")
+        return name
+
+
+G: Emit = Emit(True)
+
+
+def _isnum(a) -> bool:
+    return isinstance(a, float)
+
+
+def _fmt(a) -> str:
+    return repr(a) if _isnum(a) else a
+
+
+def _add(a, b):
+    if _isnum(a) and a == 0.0:
+        return b
+    if _isnum(b) and b == 0.0:
+        return a
+    if _isnum(a) and _isnum(b):
+        return a + b
+    return G.tmp(f"{_fmt(a)} + {_fmt(b)}")
+
+
+def _neg(a):
+    if _isnum(a):
+        return -a
+    return G.tmp(f"-{a}")
+
+
+def _sub(a, b):
+    if _isnum(b) and b == 0.0:
+        return a
+    if _isnum(a) and a == 0.0:
+        return _neg(b)
+    if _isnum(a) and _isnum(b):
+        return a - b
+    return G.tmp(f"{_fmt(a)} - {_fmt(b)}")
+
+
+def _mul(a, b):
+    if (_isnum(a) and a == 0.0) or (_isnum(b) and b == 0.0):
+        return 0.0
+    if _isnum(a) and a == 1.0:
+        return b
+    if _isnum(b) and b == 1.0:
+        return a
+    if _isnum(a) and _isnum(b):
+        return a * b
+    return G.tmp(f"{_fmt(a)} * {_fmt(b)}")
+
+
+def _div(a, b):
+    if _isnum(a) and a == 0.0:
+        return 0.0
+    if _isnum(b) and b == 1.0:
+        return a
+    if _isnum(a) and _isnum(b):
+        return a / b
+    return G.tmp(f"{_fmt(a)} / {_fmt(b)}")
+
+
+class E:
+    """A value and its sparse gradient with respect to the unknowns."""
+    __slots__ = ("v", "d")
+
+    def __init__(self, v, d=None):
+        self.v = v
+        self.d = d or {}
+
+    @staticmethod
+    def lift(x) -> "E":
+        return x if isinstance(x, E) else E(float(x))
+
+    def __add__(self, o):
+        o = E.lift(o)
+        d = dict(self.d)
+        for k, v in o.d.items():
+            d[k] = _add(d[k], v) if k in d else v
+        return E(_add(self.v, o.v), d)
+
+    __radd__ = __add__
+
+    def __sub__(self, o):
+        o = E.lift(o)
+        d = dict(self.d)
+        for k, v in o.d.items():
+            d[k] = _sub(d[k], v) if k in d else _neg(v)
+        return E(_sub(self.v, o.v), d)
+
+    def __rsub__(self, o):
+        return E.lift(o) - self
+
+    def __neg__(self):
+        return E(_neg(self.v), {k: _neg(v) for k, v in self.d.items()})
+
+    def __mul__(self, o):
+        o = E.lift(o)
+        d = {}
+        for k in set(self.d) | set(o.d):
+            a = _mul(self.d[k], o.v) if k in self.d else 0.0
+            b = _mul(self.v, o.d[k]) if k in o.d else 0.0
+            d[k] = _add(a, b)
+        return E(_mul(self.v, o.v), d)
+
+    __rmul__ = __mul__
+
+    def __truediv__(self, o):
+        o = E.lift(o)
+        if not o.d:
+            r = _div(1.0, o.v) if not _isnum(o.v) or o.v != 1.0 else 1.0
+            return E(_mul(self.v, r) if o.d == {} and not _isnum(o.v) else _div(self.v, o.v),
+                     {k: _mul(v, r) for k, v in self.d.items()})
+        q = _div(self.v, o.v)
+        r = _div(1.0, o.v)
+        d = {}
+        for k in set(self.d) | set(o.d):
+            a = self.d.get(k, 0.0)
+            b = _mul(q, o.d[k]) if k in o.d else 0.0
+            d[k] = _mul(_sub(a, b), r)
+        return E(q, d)
+
+    def __rtruediv__(self, o):
+        return E.lift(o) / self
+
+
+def par(name: str) -> E:
+    """A per-box parameter (rate, J-value, density, aerosol term): no gradient."""
+    if G.numeric:
+        return E(float(G.params[name]))
+    return E(name)
+
+
+def unk(kind: str, s: str) -> E:
+    i = INDEX.get((kind, s))
+    name = f"{kind}[{s}]"
+    v = float(G.params[name]) if G.numeric else name
+    return E(v, {i: 1.0} if i is not None else {})
+
+
+def _select(cond, a, b):
+    if G.numeric:
+        return a if cond else b
+    if _isnum(a) and _isnum(b) and a == b:
+        return a
+    return G.tmp(f"{cond} ? {_fmt(a)} : {_fmt(b)}")
+
+
+def emin(a: E, b: E) -> E:
+    """fmin with the derivative of the smaller side (the reference's min, F:1900-1917)."""
+    a, b = E.lift(a), E.lift(b)
+    if G.numeric:
+        c = a.v <= b.v
+    else:
+        c = G.tmp(f"{_fmt(a.v)} <= {_fmt(b.v)}", "bool")
+    keys = set(a.d) | set(b.d)
+    return E(_select(c, a.v, b.v),
+             {k: _select(c, a.d.get(k, 0.0), b.d.get(k, 0.0)) for k in keys})
+
+
+def emax(a: E, b: E) -> E:
+    a, b = E.lift(a), E.lift(b)
+    if G.numeric:
+        c = a.v >= b.v
+    else:
+        c = G.tmp(f"{_fmt(a.v)} >= {_fmt(b.v)}", "bool")
+    keys = set(a.d) | set(b.d)
+    return E(_select(c, a.v, b.v),
+             {k: _select(c, a.d.get(k, 0.0), b.d.get(k, 0.0)) for k in keys})
+
+
+# ------------------------------------------------------------------ the residuals
+def residuals(day: bool) -> list[E]:
+    """Residuals of the fixed point, in UNKNOWNS order, traced on the current G."""
+    ck = {s: unk("ck", s) for s in LONG}
+    cc = {s: unk("cc", s) for s in SHORT}
+    R = {r: par(f"R.{r}") for r in RATES}
+    K = {c: par(c) for c in CONSTS}
+    J = lambda j: par(f"TJ(J{j})")
+    ck0 = lambda s: par(f"CK0({s})")
+    e = {n: par(f"e.{n}") for n in (
+        "hnm n2 o2 co2 h2 dt het1 het2 het3 het4 het5 het6 het7 het8 het9 pno3 ph2o phcl pbr "
+        "rain_hno3 rain_hno4 rain_hcl rain_hbr rain_h2o2 rain_ch2o rain_ch3o2h").split()}
+    hnm, n2, o2, co2, h2, dt = (e[k] for k in ("hnm", "n2", "o2", "co2", "h2", "dt"))
+    c, k = cc, ck
+    res: dict = {}
+    A23 = K["A23A"] + K["A23B"] + K["A23C"]
+    if day:
+        # F:1353-1669 (the kernel's sweep, day branch)
+        res["CH3"] = c["CH3"] * (R["c4"] * o2 + K["C3"] * c["O3P"]) - \
+            (R["c2"] * c["OH"] + K["C1A"] * c["O1D"] + R["d5"] * c["CL"]) * k["CH4"]
+        ch3o = (K["C1B"] * c["O1D"] * k["CH4"] +
+                (R["c5"] * c["NO"] + 2. * R["c14"] * k["CH3O2"] + R["c18"] * c["CLO"]) * k["CH3O2"] +
+                J("CH3O2H") * k["CH3O2H"]) / (R["c15"] * o2)
+        hco = (R["c8"] * c["OH"] + R["c9"] * c["O3P"] + R["c10"] * c["CL"] + R["c19"] * c["BR"] +
+               J("CH2O_HCO")) * k["CH2O"] / (K["C12"] * o2)
+        r1 = (J("O3_O1D") + J("O3_O3P") + R["a2"] * c["H"] + R["a6"] * c["OH"] +
+              R["a6b"] * c["HO2"] + R["b4"] * c["NO"] + R["d2"] * c["CL"]) / (R["hk2"] * o2)
+        res["O3"] = c["O3"] * (1. + r1) - k["OX"]
+        res["O3P"] = c["O3P"] - r1 * c["O3"]
+        res["O1D"] = c["O1D"] * (R["hk4"] * n2 + R["hk5"] * o2) - J("O3_O1D") * c["O3"]
+        r4 = (R["a5"] * c["O3P"] + R["a36"] * k["CO"] + R["a19"] * h2) / \
+            (R["a1"] * o2 + R["a2"] * c["O3"] + A23 * c["HO2"])
+        res["H"] = c["H"] - r4 * c["OH"]
+        r7 = J("NO") / (R["b7"] * o2 + R["b6"] * c["NO"])
+        r8 = (R["b9"] * c["O3"]) / (J("NO3_NO") + J("NO3_NO2") + R["b12"] * c["NO2"])
+        r5 = (J("NO2") + R["b3"] * c["O3P"] + J("NO3_NO") * r8) / \
+            (R["b4"] * c["O3"] + R["a26"] * c["HO2"] + R["d4"] * c["CLO"] + R["e4"] * c["BRO"] +
+             R["c5"] * k["CH3O2"])
+        res["NO2"] = c["NO2"] * (1. + r5 + r7 * r5 + r8) - k["NOX"]
+        res["NO"] = c["NO"] - r5 * c["NO2"]
+        res["N"] = c["N"] - r7 * c["NO"]
+        res["NO3"] = c["NO3"] - r8 * c["NO2"]
+        r6 = (R["a6"] * c["O3"] + K["A30"] * k["H2O2"] + R["c8"] * k["CH2O"] + R["d8"] * c["CLO"] +
+              R["e10"] * c["BRO"] + R["a1"] * o2 * r4) / \
+            (A23 * c["H"] + R["a6b"] * c["O3"] + R["a7"] * c["O3P"] + R["a26"] * c["NO"] +
+             2. * R["a27"] * c["HO2"] + R["a17"] * c["OH"] + R["d7"] * c["CL"] +
+             R["d10"] * c["CL"] + R["d33"] * c["CLO"] + R["e7"] * c["BR"] + R["e15"] * c["BRO"])
+        res["HO2"] = c["HO2"] - c["OH"] * r6                       # damped: F:1495-1498
+        r13 = R["d31"] * c["NO2"] / (J("CLONO2_CL") + J("CLONO2_CLO") + R["d32"] * c["O3P"] +
+                                     e["het1"] * k["H2O"] + e["het2"] * k["HCL"])
+        r9 = (R["d33"] * c["HO2"] + e["het1"] * k["H2O"] * r13) / \
+            (e["het5"] * k["HCL"] + K["D35"] * c["O3P"] + J("HOCL"))
+        r10 = (R["d60"] * c["CLO"]) / (R["d61"] * hnm + J("CL2O2"))
+        r12 = R["e5c"] * c["BRO"] / J("BRCL")
+        r11 = (R["d3"] * c["O3P"] + R["d4"] * c["NO"] + R["d8"] * c["OH"] + R["c18"] * k["CH3O2"] +
+               R["e5b"] * c["BRO"] + J("CLONO2_CL") * r13 + J("HOCL") * r9 +
+               2. * J("CL2O2") * r10 + J("BRCL") * r12) / \
+            (R["d2"] * c["O3"] + R["d5"] * k["CH4"] + R["d7"] * c["HO2"] + R["d10"] * c["HO2"] +
+             R["d37"] * k["HOCL"])
+        res["CLO"] = c["CLO"] * (1. + r11) - k["CLOX"]            # damped: F:1547-1550
+        res["CL"] = c["CL"] - r11 * c["CLO"]
+        r27 = R["e13"] * c["NO2"] / (J("BRONO2") + R["e8"] * c["O3P"])
+        r28 = R["e15"] * c["HO2"] / (J("HOBR") + R["e16"] * c["O3P"])
+        r29 = R["e5c"] * c["CLO"] / J("BRCL")
+        r30 = (R["e3"] * c["O3P"] + R["e4"] * c["NO"] + R["e5a"] * c["CLO"] + R["e5b"] * c["CLO"] +
+               2. * R["e6"] * c["BRO"] + R["e10"] * c["OH"] + J("BRO") +
+               J("BRONO2") * 0.85 * r27 + J("HOBR") * r28 + J("BRCL") * r29) / \
+            (R["e2"] * c["O3"] + R["e7"] * c["HO2"] + R["c19"] * k["CH2O"])
+        res["BRO"] = c["BRO"] * (1. + r30) - k["BROX"]            # damped: F:1591-1594
+        res["BR"] = c["BR"] - r30 * c["BRO"]
+        # OH: the positive root of coea OH^2 + coeb OH - coec (F:1600-1665)
+        coea = 2. * K["A31"] + 2. * (R["a17"] + R["a27"] * r6 + (K["A23B"] + K["A23C"]) * r4) * r6
+        coeb = (R["b27"] * k["HNO3"] * e["pno3"] + R["b22"] * c["NO2"] + R["b28"] * k["HO2NO2"] +
+                R["b23"] * c["NO2"] * r6 + R["c2"] * k["CH4"] + R["c7"] * k["CH3O2"] * r6 +
+                R["c8"] * k["CH2O"] + R["c17a"] * k["CH3O2H"] + R["d7"] * c["CL"] * r6 +
+                R["d9"] * c["CLO"] + R["d11"] * k["HCL"] * e["phcl"] + R["d33"] * c["CLO"] * r6 +
+                R["d34"] * k["HOCL"] + R["e11"] * k["HBR"] * e["pbr"] + R["e7"] * c["BR"] * r6 +
+                R["e15"] * c["BRO"] * r6)
+        coec = (2. * (R["a1et"] * c["O1D"] + J("H2O")) * k["H2O"] * e["ph2o"] +
+                ((K["C1A"] + K["C1B"]) * k["CH4"] + 2. * K["A3ET"] * h2) * c["O1D"] +
+                K["C3"] * c["CH3"] * c["O3P"] + J("CH3O2H") * k["CH3O2H"] + R["c15"] * ch3o * o2 +
+                (J("CH2O_HCO") + R["c9"] * c["O3P"]) * k["CH2O"] + K["C12"] * hco * o2 +
+                J("HNO3") * k["HNO3"] * e["pno3"] + (R["b24"] * hnm + J("HNO4")) * k["HO2NO2"] +
+                (K["D35"] * c["O3P"] + R["d37"] * c["CL"] + J("HOCL")) * k["HOCL"] +
+                2. * J("H2O2") * k["H2O2"] + R["d6"] * h2 * c["CL"] + J("HOBR") * k["HOBR"] +
+                R["e16"] * c["O3P"] * k["HOBR"] + J("HBR") * k["HBR"])
+        res["OH"] = (coea * c["OH"] + coeb) * c["OH"] - coec      # damped: F:1662-1665
+        dayf = 1.
+    else:
+        # F:1671-1746 (night)
+        for s in ("H", "CH3", "O1D", "O3P", "OH", "CL", "HO2", "NO", "N", "BR"):
+            res[s] = c[s] * 1.
+        ch3o = hco = E(0.0)
+        res["O3"] = c["O3"] - k["OX"]
+        r8 = R["b9"] * c["O3"] / (R["b12"] * emax(c["NO2"], E(1.e-30)))
+        res["NO2"] = c["NO2"] * (1. + r8) - k["NOX"]
+        res["NO3"] = c["NO3"] - r8 * c["NO2"]
+        res["CLO"] = c["CLO"] - emax(k["CLOX"], E(0.0))
+        res["BRO"] = c["BRO"] - emax(k["BROX"], E(0.0))
+        dayf = 0.
+
+    # production P and loss rate L (F:1758-2358): residual ck (1 + dt L) - ck0 - dt P; the
+    # family losses L = loss / ck (F:1810, 1860, 1950, 2100, 2166, 2187, 2342) as dt loss
+    def be(s, P, L=None, loss=None):
+        r = k[s] - ck0(s)
+        if L is not None:
+            r = r + dt * L * k[s]
+        if loss is not None:
+            r = r + dt * loss
+        if not (isinstance(P, float) and P == 0.0):
+            r = r - dt * P
+        res[s] = r
+
+    be("N2O", 0., J("N2O") + (R["b38"] + R["b39"]) * c["O1D"])
+    be("CH4", 0., R["c2"] * c["OH"] + K["C1A"] * c["O1D"] + K["C1B"] * c["O1D"] +
+       K["C1C"] * c["O1D"] + R["d5"] * c["CL"])
+    be("H2O", R["c2"] * c["OH"] * k["CH4"] + R["c8"] * c["OH"] * k["CH2O"] +
+       R["c17a"] * k["CH3O2H"] * c["OH"] + R["c17b"] * k["CH3O2H"] * c["OH"] +
+       (R["a17"] * c["OH"] + K["A23C"] * c["H"]) * c["HO2"] + R["a19"] * c["OH"] * h2 +
+       K["A30"] * c["OH"] * k["H2O2"] + K["A31"] * c["OH"] * c["OH"] +
+       R["b27"] * c["OH"] * k["HNO3"] * e["pno3"] + R["d11"] * c["OH"] * k["HCL"] * e["phcl"],
+       (J("H2O") + R["a1et"] * c["O1D"]) * e["ph2o"])
+    be("NOY", 2. * R["b39"] * c["O1D"] * k["N2O"],
+       loss=2. * R["b6"] * c["N"] * c["NO"] + e["rain_hno3"] * k["HNO3"] +
+       e["rain_hno4"] * k["HO2NO2"])
+    be("HNO3", R["b22"] * c["OH"] * c["NO2"] + e["het1"] * k["CLONO2"] * k["H2O"] +
+       e["het2"] * k["CLONO2"] * k["HCL"] + 2. * e["het3"] * k["N2O5"] * k["H2O"] +
+       e["het4"] * k["N2O5"] * k["HCL"] + e["het6"] * k["BRONO2"] * k["H2O"],
+       (J("HNO3") + R["b27"] * c["OH"]) * e["pno3"] + e["rain_hno3"])
+    be("N2O5", R["b12"] * c["NO3"] * c["NO2"],
+       J("N2O5") + R["b32"] * hnm + e["het3"] * k["H2O"] + e["het4"] * k["HCL"])
+    src_cl = (3. * (J("CFC11") + K["D90"] * c["O1D"]) * k["CFC11"] +
+              2. * (J("CFC12") + K["D91"] * c["O1D"]) * k["CFC12"] +
+              3. * (J("CFC113") + K["D92"] * c["O1D"]) * k["CFC113"] +
+              4. * (J("CCL4") + K["D93"] * c["O1D"]) * k["CCL4"] +
+              3. * (J("CH3CCL3") + R["d94"] * c["OH"]) * k["CH3CCL3"] +
+              1. * (J("CH3CL") + R["d95"] * c["OH"]) * k["CH3CL"] +
+              1. * (J("HCFC22") + K["D96"] * c["O1D"] + R["d97"] * c["OH"]) * k["HCFC22"])
+    be("CLY", src_cl + (J("H1211") + K["E52"] * c["O1D"]) * k["H1211"],
+       loss=e["rain_hcl"] * k["HCL"])
+    # odd oxygen, F:1864-1950
+    ratio = J("CL2O2") / (J("CL2O2") + R["d61"] * hnm)
+    jno3 = J("NO3_NO") + J("NO3_NO2")
+    fno3 = J("NO3_NO") / emax(jno3, E(1.e-30))
+    lox = (2. * R["hk1"] * c["O3P"] * c["O3P"] + 2. * R["hk3"] * c["O3"] * c["O3P"] +
+           2. * R["b3"] * c["NO2"] * c["O3P"] + 2. * R["d3"] * c["CLO"] * c["O3P"] +
+           2. * R["d60"] * ratio * c["CLO"] * c["CLO"] +
+           emin(J("HOCL") * k["HOCL"], R["d33"] * c["HO2"] * c["CLO"]) +
+           2. * emin(R["e5c"] * c["BRO"] * c["CLO"], J("BRCL") * k["BRCL"]) +
+           2. * R["e5b"] * c["BRO"] * c["CLO"] * dayf +
+           emin(R["e15"] * c["BRO"] * c["HO2"], J("HOBR") * k["HOBR"]) +
+           2. * R["e6"] * c["BRO"] * c["BRO"] * dayf + 2. * fno3 * J("CLONO2_CL") * k["CLONO2"] +
+           ((R["a2"] * c["H"] + R["a6"] * c["OH"] + R["a6b"] * c["HO2"]) * c["O3"] +
+            (R["a5"] * c["OH"] + R["a7"] * c["HO2"]) * c["O3P"]) +
+           2. * R["b9"] * c["NO2"] * c["O3"] * J("NO3_NO") / emax(jno3, E(1.e-30)))
+    be("OX", 2. * J("O2") * o2 + R["a26"] * c["HO2"] * c["NO"] + R["c5"] * c["NO"] * k["CH3O2"],
+       loss=lox)
+    be("CO", J("CO2") * co2 + J("CH2O_CO") * k["CH2O"] + K["C12"] * hco * o2, R["a36"] * c["OH"])
+    be("OCLO", R["e5a"] * c["BRO"] * c["CLO"], J("OCLO"))
+    be("HCL", (R["c10"] * k["CH2O"] + R["d5"] * k["CH4"] + R["d7"] * c["HO2"] + R["d6"] * h2) *
+       c["CL"] + R["d9"] * c["OH"] * c["CLO"],
+       (J("HCL") + R["d11"] * c["OH"]) * e["phcl"] + e["het2"] * k["CLONO2"] +
+       e["het4"] * k["N2O5"] + e["het5"] * k["HOCL"] + e["het7"] * k["HOBR"] + e["rain_hcl"])
+    be("CLONO2", R["d31"] * c["NO2"] * c["CLO"],
+       J("CLONO2_CL") + J("CLONO2_CLO") + R["d32"] * c["O3P"] + e["het1"] * k["H2O"] +
+       e["het2"] * k["HCL"])
+    be("HOCL", R["d33"] * c["HO2"] * c["CLO"] + e["het1"] * k["CLONO2"] * k["H2O"],
+       J("HOCL") + R["d34"] * c["OH"] + K["D35"] * c["O3P"] + R["d37"] * c["CL"] +
+       e["het5"] * k["HCL"] + e["het9"] * k["HBR"])
+    be("CL2", R["d37"] * k["HOCL"] * c["CL"] + R["d62"] * c["CLO"] * c["CLO"] +
+       e["het2"] * k["CLONO2"] * k["HCL"] + e["het5"] * k["HOCL"] * k["HCL"], J("CL2"))
+    be("H2O2", R["a27"] * c["HO2"] * c["HO2"], J("H2O2") + K["A30"] * c["OH"] + e["rain_h2o2"])
+    be("CLNO2", R["d36"] * c["CL"] * c["NO2"] + e["het4"] * k["N2O5"] * k["HCL"], J("CLNO2"))
+    be("HBR", R["e7"] * c["HO2"] * c["BR"] + R["c19"] * k["CH2O"] * c["BR"],
+       (R["e11"] * c["OH"] + K["E9"] * c["O1D"] + J("HBR")) * e["pbr"] + e["het8"] * k["HOBR"] +
+       e["het9"] * k["HOCL"] + e["rain_hbr"])
+    be("BRONO2", R["e13"] * c["BRO"] * c["NO2"], J("BRONO2") + R["e8"] * c["O3P"] +
+       e["het6"] * k["H2O"])
+    be("NOX", (J("HNO4") + R["b28"] * c["OH"] + R["b24"] * hnm) * k["HO2NO2"] +
+       J("HNO3") * k["HNO3"] * e["pno3"] + 2. * (J("N2O5") + R["b32"] * hnm) * k["N2O5"] +
+       2. * R["b39"] * c["O1D"] * k["N2O"] + J("CLNO2") * k["CLNO2"] +
+       R["b27"] * k["HNO3"] * e["pno3"] * c["OH"] + R["d32"] * k["CLONO2"] * c["O3P"] +
+       (J("CLONO2_CL") + J("CLONO2_CLO")) * k["CLONO2"] + J("BRONO2") * k["BRONO2"] +
+       R["e8"] * k["BRONO2"] * c["O3P"],
+       loss=(R["d31"] * c["CLO"] + R["b22"] * c["OH"] + R["b23"] * c["HO2"] +
+             2. * R["b12"] * c["NO3"] + R["d36"] * c["CL"] + R["e13"] * c["BRO"]) * c["NO2"] +
+       2. * R["b6"] * c["N"] * c["NO"])
+    be("HO2NO2", R["b23"] * c["HO2"] * c["NO2"],
+       J("HNO4") + R["b28"] * c["OH"] + R["b24"] * hnm + e["rain_hno4"])
+    be("CLOX", 2. * (R["d61"] * hnm + J("CL2O2")) * k["CL2O2"] +
+       (J("CLONO2_CL") + J("CLONO2_CLO") + R["d32"] * c["O3P"]) * k["CLONO2"] +
+       (J("HOCL") + R["d34"] * c["OH"] + K["D35"] * c["O3P"]) * k["HOCL"] +
+       (J("OCLO") * k["OCLO"] + (J("HCL") + R["d11"] * c["OH"]) * k["HCL"] * e["phcl"]) +
+       2. * J("CL2") * k["CL2"] + J("CLNO2") * k["CLNO2"] + J("BRCL") * k["BRCL"] + src_cl,
+       loss=(R["d31"] * c["NO2"] + 2. * R["d60"] * c["CLO"] + 2. * R["d62"] * c["CLO"] +
+             R["e5a"] * c["BRO"] + R["e5c"] * c["BRO"] + R["d9"] * c["OH"] +
+             R["d33"] * c["HO2"]) * c["CLO"] +
+       (R["c10"] * k["CH2O"] + R["d5"] * k["CH4"] + R["d7"] * c["HO2"] + R["d6"] * h2 +
+        R["d36"] * c["NO2"] + R["d37"] * k["HOCL"]) * c["CL"])
+    be("BROX", J("HOBR") * k["HOBR"] + J("BRONO2") * k["BRONO2"] +
+       R["e8"] * k["BRONO2"] * c["O3P"] + R["e16"] * k["HOBR"] * c["O3P"] + J("BRCL") * k["BRCL"] +
+       (R["e11"] * c["OH"] + K["E9"] * c["O1D"] + J("HBR")) * k["HBR"] * e["pbr"] +
+       2. * e["het8"] * k["HBR"] * k["HOBR"],
+       loss=(R["e13"] * c["NO2"] + R["e5c"] * c["CLO"] + R["e15"] * c["HO2"]) * c["BRO"] +
+       R["c19"] * c["BR"] * k["CH2O"] + R["e7"] * c["BR"] * c["HO2"])
+    be("CL2O2", R["d60"] * c["CLO"] * c["CLO"], J("CL2O2") + R["d61"] * hnm)
+    be("HOBR", R["e15"] * c["BRO"] * c["HO2"] + e["het6"] * k["BRONO2"] * k["H2O"],
+       J("HOBR") + R["e16"] * c["O3P"] + e["het7"] * k["HCL"] + e["het8"] * k["HBR"])
+    be("BRCL", R["e5c"] * c["CLO"] * c["BRO"] + e["het7"] * k["HOBR"] * k["HCL"] +
+       e["het9"] * k["HBR"] * k["HOCL"], J("BRCL"))
+    be("CH2O", K["C1C"] * c["O1D"] * k["CH4"] + K["C3"] * c["O3P"] * c["CH3"] +
+       R["c15"] * ch3o * o2 + R["c17b"] * c["OH"] * k["CH3O2H"],
+       R["c8"] * c["OH"] + R["c9"] * c["O3P"] + R["c10"] * c["CL"] + R["c19"] * c["BR"] +
+       J("CH2O_HCO") + J("CH2O_CO") + e["rain_ch2o"])
+    be("CH3O2", R["c4"] * c["CH3"] * o2 + R["c17a"] * c["OH"] * k["CH3O2H"],
+       2. * R["c14"] * k["CH3O2"] + R["c7"] * c["HO2"] + R["c5"] * c["NO"] + R["c18"] * c["CLO"])
+    be("CH3O2H", R["c7"] * k["CH3O2"] * c["HO2"],
+       R["c17a"] * c["OH"] + R["c17b"] * c["OH"] + J("CH3O2H") + e["rain_ch3o2h"])
+    be("CFC11", 0., J("CFC11") + K["D90"] * c["O1D"])
+    be("CFC12", 0., J("CFC12") + K["D91"] * c["O1D"])
+    be("CFC113", 0., J("CFC113") + K["D92"] * c["O1D"])
+    be("CCL4", 0., J("CCL4") + K["D93"] * c["O1D"])
+    be("CH3CCL3", 0., J("CH3CCL3") + R["d94"] * c["OH"])
+    be("CH3CL", 0., J("CH3CL") + R["d95"] * c["OH"])
+    be("HCFC22", 0., J("HCFC22") + K["D96"] * c["O1D"] + R["d97"] * c["OH"])
+    be("CH3BR", 0., J("CH3BR") + K["E50"] * c["O1D"] + R["e51"] * c["OH"])
+    be("H1211", 0., J("H1211") + K["E52"] * c["O1D"])
+    be("H1301", 0., J("H1301") + K["E53"] * c["O1D"])
+    be("BRY", (J("CH3BR") + K["E50"] * c["O1D"] + R["e51"] * c["OH"]) * k["CH3BR"] +
+       (J("H1211") + K["E52"] * c["O1D"]) * k["H1211"] +
+       (J("H1301") + K["E53"] * c["O1D"]) * k["H1301"] +
+       2. * (R["e54"] * c["OH"] + J("CH2BR2")) * k["CH2BR2"], loss=e["rain_hbr"] * k["HBR"])
+    be("CH2BR2", 0., J("CH2BR2") + R["e54"] * c["OH"])
+    out = []
+    for kind, s in UNKNOWNS:
+        out.append(E.lift(res[s]))
+    return out
+
+
+def trace(day: bool, numeric: bool = False, params: dict | None = None):
+    """Trace the residuals; returns (emitter, residual list)."""
+    global G
+    G = Emit(numeric, params)
+    return G, residuals(day)
+
+
+def pattern(res: list[E]) -> set:
+    """Structural nonzeros (i, j) of the Jacobian, diagonal always included."""
+    p = {(i, i) for i in range(NU)}
+    for i, r in enumerate(res):
+        for j, v in r.d.items():
+            if not (_isnum(v) and v == 0.0):
+                p.add((i, j))
+    return p
+
+
+# ------------------------------------------------------------------ ordering and fill
+def symbolic_lu(pat: set, order: list[int]):
+    """Fill pattern of the LU factors of the matrix permuted symmetrically by `order`
+    (pivot k is unknown order[k]).  Returns (filled pattern in original indices, flops of
+    the factorisation: one division per pivot, one multiplication per multiplier, one
+    fused multiply-add per update)."""
+    pos = {u: k for k, u in enumerate(order)}
+    n = len(order)
+    rows = [set() for _ in range(n)]
+    for i, j in pat:
+        rows[pos[i]].add(pos[j])
+    flops = 0
+    for k in range(n):
+        upper = sorted(j for j in rows[k] if j > k)
+        flops += 1
+        for i in range(k + 1, n):
+            if k in rows[i]:
+                flops += 1 + len(upper)
+                rows[i].update(upper)
+    filled = {(order[i], order[j]) for i in range(n) for j in rows[i]}
+    return filled, flops
+
+
+def min_degree(pat: set, last: list[int] | None = None) -> list[int]:
+    """Minimum-degree order on the symmetrised pattern (ties by index); the unknowns in
+    `last` are held back and eliminated after all others, in the same rule."""
+    adj = {i: set() for i in range(NU)}
+    for i, j in pat:
+        if i != j:
+            adj[i].add(j)
+            adj[j].add(i)
+    held = set(last or [])
+    order = []
+    alive = set(range(NU))
+    while alive:
+        pool = (alive - held) or alive
+        u = min(pool, key=lambda v: (len(adj[v] & alive), v))
+        nb = adj[u] & alive
+        for a in nb:
+            adj[a] |= nb - {a}
+        alive.remove(u)
+        order.append(u)
+    return order
+
+
+def best_order(pat: set):
+    """The cheaper of minimum degree and minimum degree with the short-lived species last."""
+    radicals = [INDEX[("cc", s)] for s in SHORT]
+    cands = []
+    for name, last in (("min-degree", None), ("min-degree, short-lived last", radicals)):
+        o = min_degree(pat, last)
+        filled, fl = symbolic_lu(pat, o)
+        cands.append((fl, len(filled), name, o, filled))
+    cands.sort(key=lambda c: (c[0], c[1]))
+    return cands
+
+
+# ------------------------------------------------------------------ C emission
+def _slot_map(filled: set, order: list[int]):
+    """Storage slot of every entry of the factors: row by row in pivot order."""
+    pos = {u: k for k, u in enumerate(order)}
+    ent = sorted(filled, key=lambda ij: (pos[ij[0]], pos[ij[1]]))
+    return {ij: s for s, ij in enumerate(ent)}
+
+
+def emit_system(day: bool):
+    """C code of one system: residual and Jacobian into b[] and a[], the LU and the solve.
+    Returns a dict of code strings and statistics."""
+    g, res = trace(day)
+    pat = pattern(res)
+    cands = best_order(pat)
+    flops, nnz, oname, order, filled = cands[0]
+    slot = _slot_map(filled, order)
+    pos = {u: k for k, u in enumerate(order)}
+    lines = list(g.lines)
+    body = []
+    for i, r in enumerate(res):
+        body.append(f"b[{i}] = {_fmt(r.v)};")
+    for (i, j), s in sorted(slot.items(), key=lambda kv: kv[1]):
+        v = res[i].d.get(j, 0.0)
+        body.append(f"a[{s}] = {_fmt(v)};")
+    jac = "\n".join("    " + l for l in lines + body)
+    # factorisation: for pivot k, l_ik = a_ik / a_kk for the rows below, then the updates
+    n = NU
+    rows = [[] for _ in range(n)]            # per pivot position: column positions present
+    for (i, j) in filled:
+        rows[pos[i]].append(pos[j])
+    for r in rows:
+        r.sort()
+    lu, fwd, bwd = [], [], []
+    col_below = [[] for _ in range(n)]
+    for (i, j) in filled:
+        if pos[i] > pos[j]:
+            col_below[pos[j]].append(pos[i])
+    for c in col_below:
+        c.sort()
+    S = lambda pi, pj: slot[(order[pi], order[pj])]
+    for k in range(n):
+        lu.append(f"    const double d{k} = 1. / a[{S(k, k)}];")
+        upper = [j for j in rows[k] if j > k]
+        for i in col_below[k]:
+            lu.append(f"    {{ const double l = a[{S(i, k)}] * d{k}; a[{S(i, k)}] = l;")
+            for j in upper:
+                lu.append(f"      a[{S(i, j)}] -= l * a[{S(k, j)}];")
+            lu.append("    }")
+        lu.append(f"    a[{S(k, k)}] = d{k};")
+    # solve L y = b (unit lower), U x = y; b in unknown order, solution overwrites b
+    for i in range(n):
+        for j in rows[i]:
+            if j < i:
+                fwd.append(f"    b[{order[i]}] -= a[{S(i, j)}] * b[{order[j]}];")
+    for i in reversed(range(n)):
+        for j in rows[i]:
+            if j > i:
+                bwd.append(f"    b[{order[i]}] -= a[{S(i, j)}] * b[{order[j]}];")
+        bwd.append(f"    b[{order[i]}] *= a[{S(i, i)}];")
+    nnz_j = len(pat)
+    solve_flops = sum(1 for l in fwd + bwd)
+    return {"jac": jac, "lu": "\n".join(lu), "solve": "\n".join(fwd + bwd), "nnz_j": nnz_j,
+            "nnz_lu": len(filled), "flops_lu": flops, "flops_solve": solve_flops,
+            "order_name": oname, "order": order, "cands": [(c[0], c[1], c[2]) for c in cands],
+            "jac_ops": len(lines) + len(body), "pattern": pat, "filled": filled}
+
+
+HEADER = """// GENERATED by reprobus_amd/codegen/newton_gen.py -- do not edit.
+//
+// Newton solve of the chemistry step's fixed point (the converged mode's answer): the
+// residual of the sweep map in its local form (short-lived: x - g(x) of the QSSA ratios
+// of chem3d, src/reprobus_l137_20230723.f:1353-1746; long-lived: ck (1 + dt L) - ck0 -
+// dt P of F:1758-2372), its exact Jacobian on the sparsity pattern, and a sparse LU with a
+// fixed pivot order chosen at generation time, as straight-line FP64 code.
+//
+// The including file defines TJ(X) (J-value X), CK0(S) (start-of-step value of species S)
+// and provides R (Rates), e (BoxEnv), ck[] and cc[] (the iterate, number densities).
+//
+{stats}
+#pragma once
+
+namespace rb {{
+
+constexpr int NEWTON_N = {n};
+constexpr int NEWTON_NNZ_DAY = {nnz_day}, NEWTON_NNZ_NIGHT = {nnz_night};
+constexpr int NEWTON_NNZ = NEWTON_NNZ_DAY > NEWTON_NNZ_NIGHT ? NEWTON_NNZ_DAY : NEWTON_NNZ_NIGHT;
+// unknown i: long-lived slot NEWTON_SLOT[i] (i < NEWTON_NLONG) or short-lived slot
+constexpr int NEWTON_NLONG = {nlong};
+constexpr int NEWTON_SLOT[NEWTON_N] = {{{slots}}};
+
+"""
+
+FUNC = """// {what}: b <- residual, a <- Jacobian (factor slots, fill-in zero)
+#define RB_NEWTON_{tag}_JAC(a, b) do {{ \\
+{jac}
+}} while (0)
+
+// {what}: a <- LU factors (inverted diagonal), in place
+#define RB_NEWTON_{tag}_LU(a) do {{ \\
+{lu}
+}} while (0)
+
+// {what}: b <- A^-1 b from the factors
+#define RB_NEWTON_{tag}_SOLVE(a, b) do {{ \\
+{solve}
+}} while (0)
+
+"""
+
+
+def _macro(body: str) -> str:
+    return " \\\n".join(l.rstrip() for l in body.split("\n"))
+
+
+def generate() -> tuple[str, dict]:
+    day, night = emit_system(True), emit_system(False)
+    lines = ["// statistics (unknowns 55):"]
+    for name, s in (("day", day), ("night", night)):
+        lines.append(f"//   {name}: Jacobian {s['nnz_j']} nonzeros ({s['jac_ops']} operations to "
+                     f"form it with the residual); LU {s['nnz_lu']} nonzeros, "
+                     f"{s['flops_lu']} operations ({s['order_name']}); solve "
+                     f"{s['flops_solve']} operations")
+    slots = ", ".join(s for _, s in UNKNOWNS)
+    text = HEADER.format(stats="\n".join(lines), n=NU, nnz_day=day["nnz_lu"],
+                         nnz_night=night["nnz_lu"], nlong=sum(1 for k, _ in UNKNOWNS if k == "ck"),
+                         slots=slots)
+    for tag, what, s in (("DAY", "day (SZA <= 95 deg)", day), ("NIGHT", "night", night)):
+        text += FUNC.format(tag=tag, what=what, jac=_macro(s["jac"]), lu=_macro(s["lu"]),
+                            solve=_macro(s["solve"]))
+    text += "}  // namespace rb\n"
+    return text, {"day": day, "night": night}
+
+
+OUT = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "csrc",
+                   "newton_gen.h")
+
+
+def main(argv=None) -> int:
+    text, st = generate()
+    for name in ("day", "night"):
+        s = st[name]
+        print(f"{name}: J nnz {s['nnz_j']}, LU nnz {s['nnz_lu']}, LU ops {s['flops_lu']}, "
+              f"solve ops {s['flops_solve']}, residual+Jacobian ops {s['jac_ops']}; "
+              f"orders tried: {s['cands']}")
+    if "--check" in (argv or []):
+        with open(OUT) as f:
+            return 0 if f.read() == text else 1
+    with open(OUT, "w") as f:
+        f.write(text)
+    print("wrote", OUT)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main(sys.argv[1:]))

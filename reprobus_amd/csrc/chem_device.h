@@ -71,8 +71,9 @@ hipError_t launch_setup(const StepArgs &a, double *rec, int block, hipStream_t s
 // Anderson depths the converged kernel is built for
 constexpr int AA_DEPTH = 2;
 constexpr int AA_NX = 57;
-// doubles per box of the spill list between the two phases of the accelerated mode
-constexpr int CONV_SPILL = AA_NX + 2;
+// doubles per box of the spill list between the two phases of the accelerated mode: the
+// iterate, box index, sweep count and the last two sweeps' largest relative changes
+constexpr int CONV_SPILL = AA_NX + 4;
 
 // semi-Lagrangian transport interpolation (transport_kernels.hip): the grid constants of
 // initransp and the launches of the departure-point weights and the tracer interpolation

@@ -679,16 +679,37 @@ struct BoxEnv {
 // (values under the floor fl compared at fl): |v - old| < tol max(|v|, fl); `fin` stays true
 // while every new value is finite.  A NaN fails both.  Each species is set once per sweep,
 // so `old` is its value at the start of the sweep.
+// dmax accumulates the largest relative change of the sweep, |v - old| / max(|v|, fl), for
+// the distance-to-fixed-point estimate; it only steers when to stop, so the hardware
+// reciprocal (not the IEEE division) forms it.
 struct Track {
     bool conv, fin;
     double tol, fl;
+    double dmax;
 };
 
 __device__ __forceinline__ void track(Track &t, double v, double old)
 {
     // non-short-circuit: a lane mask AND per test, no branch
-    t.conv &= fabs(v - old) < t.tol * fmax(fabs(v), t.fl);
+    const double dv = fabs(v - old), sc = fmax(fabs(v), t.fl);
+    t.conv &= dv < t.tol * sc;
     t.fin &= fabs(v) <= 1.7976931348623157e308;
+    t.dmax = fmax(t.dmax, dv * __builtin_amdgcn_rcp(sc));
+}
+
+// Successive plain sweeps the converged mode needs before it may stop: its estimate of
+// the distance to the fixed point, d rho / (1 - rho) after a sweep of largest relative
+// change d, takes rho as the larger of the last two ratios of successive changes (the
+// larger, so that an oscillating sequence -- the damped radicals -- cannot show a small
+// rate at a low point), capped at CONV_RHOMAX: changes at rounding level have ratios near 1
+// at random, and the cap accepts a change below ~1e-3 tol whatever its ratios.
+constexpr int CONV_NPLAIN = 3;
+constexpr double CONV_RHOMAX = 0.999;
+
+__device__ __forceinline__ bool near_fixed_point(double d, double d1, double d2, double tol)
+{
+    const double rho = fmin(fmax(d / d1, d1 / d2), CONV_RHOMAX);
+    return d == 0. || d * rho < tol * (1. - rho);
 }
 
 // J-values that the sweep kernels keep in LDS beside ck0 (lane-private columns, stride
@@ -1429,7 +1450,7 @@ __global__ void __launch_bounds__(BLK) k_chem(StepArgs a, const double *__restri
     box_load<BLK>(a, rec, b, x, ck0, false);
 #pragma unroll
     for (int i = 0; i < NJL; ++i) tjl[i * BLK] = x.tj[JL_SET[i]];
-    Track notrack{true, true, 0., 0.};
+    Track notrack{true, true, 0., 0., 0.};
     for (int it = 0; it < a.niter; ++it) {
         // keeps the LDS reads of ck0 and the J-values inside the loop (not hoisted into
         // registers)
@@ -1623,10 +1644,12 @@ __global__ void __launch_bounds__(BLK) k_chem_conv(StepArgs a, ConvArgs cv)
     double *ws = AAM > 0 ? cv.aa_ws + ((int64_t)blockIdx.x * BLK + threadIdx.x) *
                                           (2 * (AAM + 1) + 1) * NX
                          : nullptr;
-    // the spill list: the iterate (NX fields), box index and sweep count per entry
+    // the spill list: the iterate (NX fields), box index, sweep count and the last two
+    // sweeps' largest relative changes per entry
     const int64_t cap = a.nbox;
     int64_t *sp_box = MODE != CONV_PLAIN ? (int64_t *)(cv.spill + NX * cap) : nullptr;
     int32_t *sp_it = MODE != CONV_PLAIN ? (int32_t *)(cv.spill + (NX + 1) * cap) : nullptr;
+    double *sp_d = MODE != CONV_PLAIN ? cv.spill + (NX + 2) * cap : nullptr;
     unsigned long long *queue = cv.counter + (MODE == CONV_ANDERSON ? 2 : 0);
     // the Anderson phase starts after the sweep phase has ended: the list is complete
     const int64_t nwork = MODE == CONV_ANDERSON
@@ -1637,6 +1660,10 @@ __global__ void __launch_bounds__(BLK) k_chem_conv(StepArgs a, ConvArgs cv)
     bool active = false, done = false, finished = false, fallback = false, restart = false;
     int64_t b = 0;
     int it = 0, tried = 0, head = 0, nstored = 0, result = 0;
+    // the last two sweeps' largest relative changes and the number of successive plain
+    // (unmixed) sweeps: the stopping estimate's inputs
+    double d1 = 0., d2 = 0.;
+    int nplain = 0;
     Box x;
     for (;;) {
         // serve the wave
@@ -1683,13 +1710,19 @@ __global__ void __launch_bounds__(BLK) k_chem_conv(StepArgs a, ConvArgs cv)
             it = 0;
             head = 0;
             nstored = 0;
+            d1 = d2 = 0.;
+            nplain = 0;
             if (resume) {
-                // the iterate and sweep count the sweep phase handed over
+                // the iterate, sweep count and last changes the sweep phase handed over
+                // (every sweep there was plain)
 #pragma unroll
                 for (int i = 0; i < NLONG - 1; ++i) x.ck[i] = cv.spill[i * cap + w];
 #pragma unroll
                 for (int i = 0; i < NSHORT; ++i) x.cc[i] = cv.spill[(NLONG - 1 + i) * cap + w];
                 it = sp_it[w];
+                d1 = sp_d[w];
+                d2 = sp_d[cap + w];
+                nplain = it;
             }
         }
         if (__ballot(active) == 0ull) break;
@@ -1712,6 +1745,8 @@ __global__ void __launch_bounds__(BLK) k_chem_conv(StepArgs a, ConvArgs cv)
                         for (int i = 0; i < NSHORT; ++i) cv.spill[(NLONG - 1 + i) * cap + s] = x.cc[i];
                         sp_box[s] = b;
                         sp_it[s] = it;
+                        sp_d[s] = d1;
+                        sp_d[cap + s] = d2;
                         active = false;
                     }
                 }
@@ -1721,7 +1756,7 @@ __global__ void __launch_bounds__(BLK) k_chem_conv(StepArgs a, ConvArgs cv)
                 if (AAM > 0 && acc) aa_save(x.ck, x.cc, ws, 1, head);
                 // the first sweep from a surrogate's answer is its acceptance test
                 const bool accept = cv.accept_tol > 0. && it == 0 && !fallback;
-                Track tr{true, true, accept ? cv.accept_tol : cv.tol, 1.e-30 * x.e.hnm};
+                Track tr{true, true, accept ? cv.accept_tol : cv.tol, 1.e-30 * x.e.hnm, 0.};
                 // keeps the LDS reads inside the loop (not hoisted into registers)
                 asm volatile("" ::: "memory");
                 sweep<BLK, true, true>(x.R, x.tj, x.e, ck0, x.ck, x.cc, x.ch3o, x.hco, tr, tjl);
@@ -1732,8 +1767,13 @@ __global__ void __launch_bounds__(BLK) k_chem_conv(StepArgs a, ConvArgs cv)
                         done = true;
                         active = false;
                     }
-                } else if (tr.conv && tr.fin) {
-                    result = accept ? 0 : it;
+                } else if (accept && tr.conv && tr.fin) {
+                    result = 0;
+                    done = true;
+                    active = false;
+                } else if (tr.conv && tr.fin && nplain >= CONV_NPLAIN &&
+                           near_fixed_point(tr.dmax, d1, d2, cv.tol)) {
+                    result = it;
                     done = true;
                     active = false;
                 } else if (!tr.fin || it >= cv.maxiter) {
@@ -1742,9 +1782,16 @@ __global__ void __launch_bounds__(BLK) k_chem_conv(StepArgs a, ConvArgs cv)
                     fallback = true;
                     restart = true;
                     active = false;
-                } else if (AAM > 0 && acc) {
-                    aa_update<(AAM > 0 ? AAM : 1)>(x.ck, x.cc, ws, 1, head, nstored,
-                                                   1.e-30 * x.e.hnm);
+                } else {
+                    // with Anderson mixing, a box whose change is already below tol takes
+                    // plain steps until the estimate decides
+                    const bool mix = AAM > 0 && acc && !tr.conv;
+                    if (AAM > 0 && mix)
+                        aa_update<(AAM > 0 ? AAM : 1)>(x.ck, x.cc, ws, 1, head, nstored,
+                                                       1.e-30 * x.e.hnm);
+                    nplain = mix ? 0 : nplain + 1;
+                    d2 = d1;
+                    d1 = tr.dmax;
                 }
             }
             const unsigned long long waiting = __ballot(!active && !finished && !restart);

@@ -63,6 +63,7 @@ int fail(rb_ctx *c, int code, const std::string &msg)
 
 int hip_fail(rb_ctx *c, hipError_t e, const char *what)
 {
+    (void)hipGetLastError();   // reported here: not again by the next call's launch check
     return fail(c, RB_EHIP, std::string(what) + ": " + hipGetErrorString(e));
 }
 
@@ -91,6 +92,9 @@ int ensure(rb_ctx *ctx, double **buf, size_t *cap, size_t n)
     *cap = 0;
     hipError_t e = hipMalloc((void **)buf, n * sizeof(double));
     if (e != hipSuccess) {
+        // the runtime keeps the failure as the thread's last error, which the next launch
+        // check (hipGetLastError) would report: take it here, where it belongs
+        (void)hipGetLastError();
         *buf = nullptr;
         return fail(ctx, RB_ENOMEM, std::string("device allocation of ") +
                                         std::to_string(n * sizeof(double)) + " bytes failed: " +

@@ -45,6 +45,7 @@ int fail(rb_mlp *m, int code, const std::string &msg)
 
 int hip_fail(rb_mlp *m, hipError_t e, const char *what)
 {
+    (void)hipGetLastError();   // reported here: not again by the next call's launch check
     return fail(m, e == hipErrorOutOfMemory ? RB_ENOMEM : RB_EHIP,
                 std::string(what) + ": " + hipGetErrorString(e));
 }
@@ -60,7 +61,9 @@ hipError_t realloc_dev(T **ptr, size_t n)
 {
     if (*ptr) (void)hipFree(*ptr);
     *ptr = nullptr;
-    return hipMalloc((void **)ptr, n * sizeof(T));
+    const hipError_t e = hipMalloc((void **)ptr, n * sizeof(T));
+    if (e != hipSuccess) (void)hipGetLastError();   // not reported again by the next launch
+    return e;
 }
 
 // grows the activation buffers to `rows` rows (inference needs no pre-activations or
@@ -191,6 +194,7 @@ int rb_mlp_create(rb_mlp **out, int device)
     rb_mlp *m = new rb_mlp();
     m->device = device;
     auto bad = [&](hipError_t e) {
+        (void)hipGetLastError();
         rb_mlp_destroy(m);
         return e == hipErrorOutOfMemory ? RB_ENOMEM : RB_EHIP;
     };

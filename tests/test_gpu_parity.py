@@ -285,12 +285,21 @@ def test_error_paths(step):
     from reprobus_amd._lib import RbError
     from reprobus_amd.chem import make_grid
 
-    bad = make_grid(4, 1, 10, 20, 14)            # nivbas > niv
-    z = np.zeros(10 * 4)
-    with pytest.raises(RbError) as e:
-        step.chem3d(bad, 900.0, z, z, z, z, np.zeros(43 * 40), np.zeros(15 * 40), z, z, z, z)
-    assert e.value.code == _lib.RB_EINVAL
     L = _lib.load()
+    bad = make_grid(4, 1, 10, 20, 14)            # nivbas > niv: the C ABI refuses the grid
+    z = np.zeros(10 * 4)
+    rc = L.rb_chem_step(step._ctx, C.byref(bad), 900.0, *([z.ctypes.data] * 4),
+                        np.zeros(43 * 40).ctypes.data, np.zeros(15 * 40).ctypes.data,
+                        *([z.ctypes.data] * 4))
+    assert rc == _lib.RB_EINVAL
+    assert b"nivbas" in L.rb_last_error(step._ctx)
+    # the host mirror checks every array against the grid before calling in
+    good = make_grid(4, 1, 10, 10, 14)
+    with pytest.raises(ValueError, match="qj1"):
+        step.chem3d(good, 900.0, z, z, z, z, np.zeros(43 * 40 - 1), np.zeros(15 * 40), z, z, z, z)
+    with pytest.raises(RbError) as e:
+        step.chem3d(good, -1.0, z, z, z, z, np.zeros(43 * 40), np.zeros(15 * 40), z, z, z, z)
+    assert e.value.code == _lib.RB_EINVAL
     ctx = C.c_void_p()
     assert L.rb_init(C.byref(ctx), 0, None) == 0
     try:

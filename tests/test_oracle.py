@@ -397,7 +397,8 @@ def test_diverging_box_falls_back_to_the_reference_step(table):
 
 
 def test_converged_state_is_a_fixed_point(table):
-    """Started from its own converged state, the sweep stops after one sweep and moves no
+    """Started from its own converged state, the sweep stops as soon as its stopping
+    estimate can be formed (three plain sweeps, stopping on the fourth) and moves no
     species by more than the tolerance; started from a perturbed state it returns."""
     import cport
     from reprobus_amd import synth
@@ -409,7 +410,8 @@ def test_converged_state_is_a_fixed_point(table):
     assert live.mean() > 0.98
     again = cport.chem_converge(inp, *table, tol=tol, maxiter=20000, conserv=False,
                                 guess=(base["qj1_chem"], base["hc_chem"]))
-    assert np.all(again["iters"][live] == 1)
+    it = again["iters"][live]
+    assert np.all((it >= 4) & (it <= 8)) and (it == 4).mean() > 0.99
     for k in ("qj1_chem", "hc_chem"):
         d = rel_err(again[k][:42] if k == "qj1_chem" else again[k],
                     base[k][:42] if k == "qj1_chem" else base[k], floor=1e-30)
