@@ -399,7 +399,8 @@ def test_diverging_box_falls_back_to_the_reference_step(table):
 def test_converged_state_is_a_fixed_point(table):
     """Started from its own converged state, the sweep stops as soon as its stopping
     estimate can be formed (three plain sweeps, stopping on the fourth) and moves no
-    species by more than the tolerance; started from a perturbed state it returns."""
+    species by more than a few tolerances (each state lies within ~tol of the fixed point);
+    started from a perturbed state it returns."""
     import cport
     from reprobus_amd import synth
 
@@ -415,7 +416,7 @@ def test_converged_state_is_a_fixed_point(table):
     for k in ("qj1_chem", "hc_chem"):
         d = rel_err(again[k][:42] if k == "qj1_chem" else again[k],
                     base[k][:42] if k == "qj1_chem" else base[k], floor=1e-30)
-        assert d[:, live].max() < tol
+        assert d[:, live].max() < 5 * tol        # both within ~tol of the fixed point
     rng = np.random.default_rng(1)
     gq = base["qj1_chem"] * np.exp(rng.normal(0, 1e-2, base["qj1_chem"].shape))
     gh = base["hc_chem"] * np.exp(rng.normal(0, 1e-2, base["hc_chem"].shape))
