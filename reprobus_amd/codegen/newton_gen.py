@@ -65,7 +65,7 @@ class Emit:
 
     def __init__(self, numeric: bool, params: dict | None = None):
         self.numeric = numeric
-        self.params = params or {}
+        self.params = params if params is not None else {}
         self.lines: list[str] = []
         self.n = 0
 
@@ -338,9 +338,14 @@ def residuals(day: bool) -> list[E]:
             res[s] = c[s] * 1.
         ch3o = hco = E(0.0)
         res["O3"] = c["O3"] - k["OX"]
-        r8 = R["b9"] * c["O3"] / (R["b12"] * emax(c["NO2"], E(1.e-30)))
-        res["NO2"] = c["NO2"] * (1. + r8) - k["NOX"]
-        res["NO3"] = c["NO3"] - r8 * c["NO2"]
+        # r8 = b9 O3 / (b12 max(NO2, 1e-30)) (F:1712); r8 NO2 is written as
+        # (b9 O3 / b12) q, q = NO2 / max(NO2, 1e-30) (1 above the clamp), so that
+        # d(NO2 (1 + r8))/dNO2 = 1 does not come out of 1 + r8 - r8, which cancels to 0
+        # when NO2 is tiny
+        q8 = c["NO2"] / emax(c["NO2"], E(1.e-30))
+        cq = R["b9"] * c["O3"] / R["b12"]
+        res["NO2"] = c["NO2"] + cq * q8 - k["NOX"]
+        res["NO3"] = c["NO3"] - cq * q8
         res["CLO"] = c["CLO"] - emax(k["CLOX"], E(0.0))
         res["BRO"] = c["BRO"] - emax(k["BROX"], E(0.0))
         dayf = 0.
@@ -565,15 +570,110 @@ def _slot_map(filled: set, order: list[int]):
     return {ij: s for s, ij in enumerate(ent)}
 
 
-def emit_system(day: bool):
-    """C code of one system: residual and Jacobian into b[] and a[], the LU and the solve.
-    Returns a dict of code strings and statistics."""
-    g, res = trace(day)
+def lu_ops(filled: set, order: list[int], slot: dict):
+    """The factorisation and the two triangular solves as op lists (slots of the factor
+    storage, unknown indices of the right-hand side):
+      ('recip', k, s)      d[k] = 1 / a[s]
+      ('scale', s, k)      a[s] *= d[k]            (a multiplier of column k)
+      ('upd', s, si, sj)   a[s] -= a[si] * a[sj]   (an update of the trailing rows)
+      ('setinv', s, k)     a[s] = d[k]             (the inverted pivot, for the solve)
+      ('fsub', i, s, j)    b[i] -= a[s] * b[j]     (L y = b, unit diagonal)
+      ('bsub', i, s, j)    b[i] -= a[s] * b[j]     (U x = y)
+      ('bmul', i, s)       b[i] *= a[s]
+    For pivot k every multiplier of column k is scaled before the updates that read it."""
+    pos = {u: k for k, u in enumerate(order)}
+    n = len(order)
+    rows = [[] for _ in range(n)]
+    col_below = [[] for _ in range(n)]
+    for (i, j) in filled:
+        rows[pos[i]].append(pos[j])
+        if pos[i] > pos[j]:
+            col_below[pos[j]].append(pos[i])
+    for r in rows:
+        r.sort()
+    for c in col_below:
+        c.sort()
+    S = lambda pi, pj: slot[(order[pi], order[pj])]
+    lu, sv = [], []
+    for k in range(n):
+        lu.append(("recip", k, S(k, k)))
+        upper = [j for j in rows[k] if j > k]
+        for i in col_below[k]:
+            lu.append(("scale", S(i, k), k))
+            for j in upper:
+                lu.append(("upd", S(i, j), S(i, k), S(k, j)))
+        lu.append(("setinv", S(k, k), k))
+    for i in range(n):
+        for j in rows[i]:
+            if j < i:
+                sv.append(("fsub", order[i], S(i, j), order[j]))
+    for i in reversed(range(n)):
+        for j in rows[i]:
+            if j > i:
+                sv.append(("bsub", order[i], S(i, j), order[j]))
+        sv.append(("bmul", order[i], S(i, i)))
+    return lu, sv
+
+
+def render(ops) -> list[str]:
+    out = []
+    for op in ops:
+        t = op[0]
+        if t == "recip":
+            out.append(f"const double d{op[1]} = 1. / a[{op[2]}];")
+        elif t == "scale":
+            out.append(f"a[{op[1]}] *= d{op[2]};")
+        elif t == "upd":
+            out.append(f"a[{op[1]}] -= a[{op[2]}] * a[{op[3]}];")
+        elif t == "setinv":
+            out.append(f"a[{op[1]}] = d{op[2]};")
+        elif t in ("fsub", "bsub"):
+            out.append(f"b[{op[1]}] -= a[{op[2]}] * b[{op[3]}];")
+        elif t == "bmul":
+            out.append(f"b[{op[1]}] *= a[{op[2]}];")
+    return out
+
+
+def interpret(ops, a, b) -> None:
+    """Run an op list on numpy arrays a (factor slots) and b (right-hand side), in place:
+    the same sequence the generated code performs."""
+    d = {}
+    for op in ops:
+        t = op[0]
+        if t == "recip":
+            d[op[1]] = 1.0 / a[op[2]]
+        elif t == "scale":
+            a[op[1]] *= d[op[2]]
+        elif t == "upd":
+            a[op[1]] -= a[op[2]] * a[op[3]]
+        elif t == "setinv":
+            a[op[1]] = d[op[2]]
+        elif t in ("fsub", "bsub"):
+            b[op[1]] -= a[op[2]] * b[op[3]]
+        elif t == "bmul":
+            b[op[1]] *= a[op[2]]
+
+
+def system(day: bool) -> dict:
+    """Pattern, pivot order, factor slots and LU/solve ops of one system."""
+    # the C trace keeps both sides of every min/max: its pattern covers either branch
+    _, res = trace(day)
     pat = pattern(res)
     cands = best_order(pat)
     flops, nnz, oname, order, filled = cands[0]
     slot = _slot_map(filled, order)
-    pos = {u: k for k, u in enumerate(order)}
+    lu, sv = lu_ops(filled, order, slot)
+    return {"pattern": pat, "order": order, "filled": filled, "slot": slot, "lu": lu,
+            "solve": sv, "flops_lu": flops, "order_name": oname,
+            "cands": [(c[0], c[1], c[2]) for c in cands]}
+
+
+def emit_system(day: bool):
+    """C code of one system: residual and Jacobian into b[] and a[], the LU and the solve.
+    Returns a dict of code strings and statistics."""
+    sy = system(day)
+    g, res = trace(day)
+    slot = sy["slot"]
     lines = list(g.lines)
     body = []
     for i, r in enumerate(res):
@@ -582,46 +682,12 @@ def emit_system(day: bool):
         v = res[i].d.get(j, 0.0)
         body.append(f"a[{s}] = {_fmt(v)};")
     jac = "\n".join("    " + l for l in lines + body)
-    # factorisation: for pivot k, l_ik = a_ik / a_kk for the rows below, then the updates
-    n = NU
-    rows = [[] for _ in range(n)]            # per pivot position: column positions present
-    for (i, j) in filled:
-        rows[pos[i]].append(pos[j])
-    for r in rows:
-        r.sort()
-    lu, fwd, bwd = [], [], []
-    col_below = [[] for _ in range(n)]
-    for (i, j) in filled:
-        if pos[i] > pos[j]:
-            col_below[pos[j]].append(pos[i])
-    for c in col_below:
-        c.sort()
-    S = lambda pi, pj: slot[(order[pi], order[pj])]
-    for k in range(n):
-        lu.append(f"    const double d{k} = 1. / a[{S(k, k)}];")
-        upper = [j for j in rows[k] if j > k]
-        for i in col_below[k]:
-            lu.append(f"    {{ const double l = a[{S(i, k)}] * d{k}; a[{S(i, k)}] = l;")
-            for j in upper:
-                lu.append(f"      a[{S(i, j)}] -= l * a[{S(k, j)}];")
-            lu.append("    }")
-        lu.append(f"    a[{S(k, k)}] = d{k};")
-    # solve L y = b (unit lower), U x = y; b in unknown order, solution overwrites b
-    for i in range(n):
-        for j in rows[i]:
-            if j < i:
-                fwd.append(f"    b[{order[i]}] -= a[{S(i, j)}] * b[{order[j]}];")
-    for i in reversed(range(n)):
-        for j in rows[i]:
-            if j > i:
-                bwd.append(f"    b[{order[i]}] -= a[{S(i, j)}] * b[{order[j]}];")
-        bwd.append(f"    b[{order[i]}] *= a[{S(i, i)}];")
-    nnz_j = len(pat)
-    solve_flops = sum(1 for l in fwd + bwd)
-    return {"jac": jac, "lu": "\n".join(lu), "solve": "\n".join(fwd + bwd), "nnz_j": nnz_j,
-            "nnz_lu": len(filled), "flops_lu": flops, "flops_solve": solve_flops,
-            "order_name": oname, "order": order, "cands": [(c[0], c[1], c[2]) for c in cands],
-            "jac_ops": len(lines) + len(body), "pattern": pat, "filled": filled}
+    return {"jac": jac, "lu": "\n".join("    " + l for l in render(sy["lu"])),
+            "solve": "\n".join("    " + l for l in render(sy["solve"])),
+            "nnz_j": len(sy["pattern"]), "nnz_lu": len(sy["filled"]),
+            "flops_lu": sy["flops_lu"], "flops_solve": len(sy["solve"]),
+            "order_name": sy["order_name"], "cands": sy["cands"],
+            "jac_ops": len(lines) + len(body)}
 
 
 HEADER = """// GENERATED by reprobus_amd/codegen/newton_gen.py -- do not edit.

@@ -1,0 +1,229 @@
+"""CPU checks of the generated Newton system (reprobus_amd/codegen/newton_gen.py).
+
+The generator transcribes the sweep map's fixed-point conditions (src/reprobus_l137_
+20230723.f:1353-2372) into a residual; the same trace evaluated numerically is checked here
+(the C code the kernel includes is rendered from that trace and from the same op lists):
+  * the residual vanishes at the restatement's fixed point (the converged mode swept to
+    1e-14), day and night, with every box's set-up taken from the restatement's own
+    per-box routines -- a dropped term, a wrong species or rate shows as an O(1) residual;
+  * the Jacobian equals central finite differences of the residual;
+  * the generated LU (fixed minimum-degree pivot order, no pivoting) with its two
+    triangular solves has a backward error at rounding level and agrees with a dense
+    pivoted solve;
+  * the committed header is what the generator writes now.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+cport = pytest.importorskip("cport")
+from reprobus_amd.codegen import newton_gen as ng  # noqa: E402
+
+# photolysis slots in the kernel's order (F:754-794)
+JNAMES = ["O2", "NO2", "O3_O1D", "O3_O3P", "NO3_NO", "NO3_NO2", "N2O5", "HNO3", "HNO4", "H2O2",
+          "H2O", "N2O", "HOCL", "CLONO2_CL", "CLONO2_CLO", "CL2O2", "HCL", "CL2", "CLNO2", "BRO",
+          "BRONO2", "HOBR", "OCLO", "BRCL", "CH3O2H", "CH2O_HCO", "CH2O_CO", "CFC11", "CFC12",
+          "CFC113", "CCL4", "CH3CCL3", "CH3CL", "HCFC22", "CH3BR", "H1211", "H1301", "CO2",
+          "CH2BR2", "HBR", "NO"]
+# chem3d's scalar constants not in the restatement's rate table (F:1083-1086)
+SCALARS = {"A3ET": 1.2e-10, "A23A": 7.2e-11, "A23B": 6.9e-12, "A23C": 1.6e-12}
+
+
+def column_setup(inp, table, j, i):
+    """Per-level set-up of column (j, i) from the restatement's routines: a list of dicts
+    of the generator's parameter names (rates, J-values, densities, aerosol terms)."""
+    L = cport.lib(cr_psc=True)
+    g = inp.grid
+    nl = g.nivbas
+    p, T = np.ascontiguousarray(inp.p[:nl, j, i]), np.ascontiguousarray(inp.T[:nl, j, i])
+    o3 = np.ascontiguousarray(inp.hc[4, :nl, j, i])
+    no = np.ascontiguousarray(inp.hc[7, :nl, j, i])
+    o3t, co2, cno = np.zeros(nl), np.zeros(nl), np.zeros(nl)
+    L.rbo_column(nl, p, T, o3, no, 1, o3t, co2, cno)
+    ajl = cport.fortran_ajl(table[0])
+    o3up = np.ascontiguousarray(table[1])
+    nrates = L.rbo_rates(200.0, 1e17, 10.0, np.zeros(128))
+    names = [L.rbo_rate_name(n).decode() for n in range(nrates)]
+    out = []
+    for k in range(nl):
+        t, pm = float(T[k]), float(p[k])
+        hnm = pm / (t * 1.38e-19)
+        r = np.zeros(128)
+        L.rbo_rates(t, hnm, pm, r)
+        rates = dict(zip(names, r))
+        prm = {f"R.{n}": rates[n] for n in ng.RATES}
+        for c in ng.CONSTS:
+            prm[c] = SCALARS.get(c, rates.get(c.lower()))
+        tj = np.zeros(40)
+        L.rbo_phot_box(ajl, o3up, float(inp.alt[k, j, i]), float(inp.sza[k, j, i]), o3t[k], tj)
+        tj = list(tj) + [L.rbo_jno_box(co2[k], cno[k], o3t[k], hnm, float(inp.sza[0, j, i]))]
+        prm.update({f"TJ(J{n})": v for n, v in zip(JNAMES, tj)})
+        ck_in = np.concatenate([[0.0], inp.qj1[:, k, j, i] * hnm])
+        het, part = np.zeros(9), np.ones(5)
+        if k + 1 >= g.nivhau:
+            surf, vsed, flags = np.zeros(1), np.zeros(1), np.zeros(3, dtype=np.int32)
+            L.rbo_aerosol_box(t, pm, hnm, ck_in, het, part, surf, vsed, flags)
+        e = {"hnm": hnm, "n2": 0.78 * hnm, "o2": 0.21 * hnm, "co2": 360.e-6 * hnm,
+             "h2": 5.e-7 * hnm, "dt": inp.dt, "pno3": part[0], "ph2o": part[2],
+             "phcl": part[3], "pbr": 1.0}
+        e.update({f"het{n + 1}": het[n] for n in range(9)})
+        rr = 1.e-6 if inp.qj1[2, k, j, i] >= 100.e-6 else 0.0
+        e.update(rain_hno3=rr, rain_hcl=rr, rain_hbr=rr, rain_h2o2=rr, rain_hno4=0.0,
+                 rain_ch2o=0.0, rain_ch3o2h=0.0)
+        prm.update({f"e.{n}": v for n, v in e.items()})
+        prm.update({f"CK0({s})": ck_in[1 + n] for n, s in enumerate(ng.LONG)})
+        prm["_day"] = float(inp.sza[k, j, i]) <= 95.0
+        out.append(prm)
+    return out
+
+
+def with_state(prm, q, h):
+    """prm plus the iterate (mixing ratios q (43,), h (15,) to number densities)."""
+    d = dict(prm)
+    hnm = prm["e.hnm"]
+    d.update({f"ck[{s}]": q[n] * hnm for n, s in enumerate(ng.LONG)})
+    d.update({f"cc[{s}]": h[n] * hnm for n, s in enumerate(ng.SHORT)})
+    return d
+
+
+def numeric(prm):
+    """Residual (55,) and dense Jacobian (55, 55) at the state in prm."""
+    _, res = ng.trace(prm["_day"], numeric=True, params=prm)
+    r = np.array([float(x.v) for x in res])
+    J = np.zeros((ng.NU, ng.NU))
+    for i, x in enumerate(res):
+        for j, v in x.d.items():
+            J[i, j] = float(v)
+    return r, J
+
+
+def state_vector(prm):
+    return np.array([prm[f"{k}[{s}]"] for k, s in ng.UNKNOWNS])
+
+
+@pytest.fixture(scope="module")
+def fixed_points(table):
+    """A c2 subgrid (polar vortex, terminator, night) swept to its fixed point at 1e-14."""
+    from reprobus_amd import synth
+
+    inp = synth.make_c2(1, nlon=6, nlat=13)
+    fp = cport.chem_converge(inp, *table, tol=1e-14, maxiter=50000, conserv=False,
+                             cr_psc=True)
+    return inp, fp
+
+
+def _sample(inp, fp, n, rng):
+    g = inp.grid
+    live = np.argwhere(fp["iters"] > 0)
+    return live[rng.choice(len(live), size=n, replace=False)]
+
+
+def test_residual_vanishes_at_the_fixed_point(table, fixed_points):
+    inp, fp = fixed_points
+    rng = np.random.default_rng(0)
+    boxes = _sample(inp, fp, 60, rng)
+    cols = {}
+    worst = {True: 0.0, False: 0.0}
+    for k, j, i in boxes:
+        prm = cols.setdefault((j, i), column_setup(inp, table, j, i))[k]
+        st = with_state(prm, fp["qj1_chem"][:, k, j, i], fp["hc_chem"][:, k, j, i])
+        r, J = numeric(st)
+        x = state_vector(st)
+        # the size of each equation's terms: its linear terms and the start-of-step value
+        scale = np.abs(J) @ np.abs(x) + np.array(
+            [abs(st[f"CK0({s})"]) if kd == "ck" else 0.0 for kd, s in ng.UNKNOWNS])
+        rel = np.abs(r) / np.maximum(scale, 1e-300)
+        # at night with NOx below b9 O3 / b12 the sweep drives NO2 down geometrically
+        # toward its clamp (max(NO2, 1e-30), F:1712); the converged test stops it below the
+        # 1e-30 mole-fraction floor, where changes count at the floor, so the NO2 and NO3
+        # equations are checked only above it
+        fl = 1e-30 * prm["e.hnm"]
+        if x[ng.INDEX[("cc", "NO2")]] < fl:
+            rel[[ng.INDEX[("cc", "NO2")], ng.INDEX[("cc", "NO3")]]] = 0.0
+        worst[prm["_day"]] = max(worst[prm["_day"]], float(rel.max()))
+    print("[newton] largest scaled residual at the fixed point: day %.2e, night %.2e"
+          % (worst[True], worst[False]))
+    assert worst[True] < 1e-9 and worst[False] < 1e-9
+    assert worst[True] > 0 or worst[False] > 0        # both systems were evaluated
+
+
+def test_residual_detects_a_moved_state(table, fixed_points):
+    """Moving one species 1e-6 off the fixed point raises the residual of the equations
+    that read it far above the fixed-point level."""
+    inp, fp = fixed_points
+    k, j, i = 20, 6, 2
+    prm = column_setup(inp, table, j, i)[k]
+    q, h = fp["qj1_chem"][:, k, j, i].copy(), fp["hc_chem"][:, k, j, i].copy()
+    r0, J = numeric(with_state(prm, q, h))
+    q[ng.LONG.index("HNO3")] *= 1 + 1e-6
+    r1, _ = numeric(with_state(prm, q, h))
+    x = state_vector(with_state(prm, q, h))
+    scale = np.abs(J) @ np.abs(x)
+    assert (np.abs(r1) / scale).max() > 1e-8 > (np.abs(r0) / scale).max()
+
+
+def test_jacobian_matches_finite_differences(table, fixed_points):
+    inp, fp = fixed_points
+    rng = np.random.default_rng(1)
+    for k, j, i in _sample(inp, fp, 6, rng):
+        prm = column_setup(inp, table, j, i)[k]
+        # a state 1 % off the fixed point, so that no term sits at a kink
+        q = fp["qj1_chem"][:, k, j, i] * np.exp(rng.normal(0, 0.01, 43))
+        h = fp["hc_chem"][:, k, j, i] * np.exp(rng.normal(0, 0.01, 15))
+        st = with_state(prm, q, h)
+        _, J = numeric(st)
+        x = state_vector(st)
+        Jfd = np.zeros_like(J)
+        for c, (kind, s) in enumerate(ng.UNKNOWNS):
+            dx = 1e-6 * (abs(x[c]) or 1e-30 * prm["e.hnm"])
+            up, dn = dict(st), dict(st)
+            up[f"{kind}[{s}]"] += dx
+            dn[f"{kind}[{s}]"] -= dx
+            Jfd[:, c] = (numeric(up)[0] - numeric(dn)[0]) / (2 * dx)
+        # columns weighted by the species' size (at least the floor), rows by their terms
+        w = np.maximum(np.abs(x), 1e-30 * prm["e.hnm"])
+        scale = np.abs(J) @ w
+        err = np.abs(J - Jfd) * w[None, :] / scale[:, None]
+        assert err.max() < 1e-5, (k, j, i, float(err.max()))
+        # the generated pattern holds every nonzero
+        sy = ng.system(prm["_day"])
+        nz = {(a, b) for a, b in zip(*np.nonzero(J))}
+        assert nz <= sy["pattern"]
+
+
+@pytest.mark.parametrize("day", [True, False])
+def test_generated_lu_solves_like_a_dense_solve(table, fixed_points, day):
+    inp, fp = fixed_points
+    sy = ng.system(day)
+    rng = np.random.default_rng(2)
+    done = 0
+    for k, j, i in _sample(inp, fp, 80, rng):
+        prm = column_setup(inp, table, j, i)[k]
+        if prm["_day"] != day:
+            continue
+        q = fp["qj1_chem"][:, k, j, i] * np.exp(rng.normal(0, 0.1, 43))
+        h = fp["hc_chem"][:, k, j, i] * np.exp(rng.normal(0, 0.1, 15))
+        r, J = numeric(with_state(prm, q, h))
+        a = np.zeros(len(sy["slot"]))
+        for (ii, jj), s in sy["slot"].items():
+            a[s] = J[ii, jj]
+        b = r.copy()
+        ng.interpret(sy["lu"], a, b)
+        ng.interpret(sy["solve"], a, b)
+        ref = np.linalg.solve(J, r)
+        # backward error of the generated solve (what a Newton step needs) and agreement
+        # with the pivoted dense solve (within the conditioning of the matrix)
+        bw = np.abs(J @ b - r) / np.maximum(np.abs(J) @ np.abs(b) + np.abs(r), 1e-300)
+        assert bw.max() < 1e-12, (k, j, i, float(bw.max()))
+        rel = np.abs(b - ref) / np.maximum(np.abs(ref), np.abs(ref).max() * 1e-12)
+        cond = np.linalg.cond(J)
+        assert rel.max() < max(1e-9, 1e-14 * cond), (k, j, i, float(rel.max()), cond)
+        done += 1
+        if done == 8:
+            break
+    assert done >= 3
+
+
+def test_committed_header_is_current():
+    assert ng.main(["--check"]) == 0, "run python -m reprobus_amd.codegen.newton_gen"
