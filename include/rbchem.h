@@ -214,6 +214,26 @@ int rb_chem_converge_dev(rb_ctx *ctx, const rb_grid *g, double dt, double tol, i
                          const double *guess_qj1, const double *guess_hc, int32_t *iters,
                          int fuse_conserv);
 
+/* Newton solve of the converged mode's fixed point (no counterpart in the reference): the
+ * same answer as rb_chem_converge_dev -- the fixed point of the per-box sweep map -- from
+ * Newton iterations on the fixed-point conditions in their local form (the QSSA ratios of
+ * the short-lived species, the backward-Euler balance of the long-lived ones), whose
+ * Jacobian is sparse (55 unknowns, about 400 nonzeros) and factorised by a generated
+ * sparse LU with a fixed pivot order.  Per box: one plain sweep from the initial iterate
+ * (guess_qj1 / guess_hc, or the incoming state), at most max_newton Newton iterations until
+ * a full step moves every species by less than tol relative, then one plain sweep as the
+ * check (every species must change by less than tol; the swept state is the result).
+ * iters receives the Newton iterations of such a box.  A box Newton does not solve is
+ * finished by plain sweeps under rb_chem_converge_dev's stopping rule (iters = 1000 + its
+ * sweeps), or gets the fixed-sweep answer with a negative count like there.  The other
+ * arguments are rb_chem_converge_dev's. */
+int rb_chem_newton_dev(rb_ctx *ctx, const rb_grid *g, double dt, double tol, int32_t max_newton,
+                       int32_t maxiter, const double *T, const double *p, const double *sza,
+                       const double *alt, double *qj1, double *hc, double *surfarea,
+                       double *vsed3d, double *parth2osed, double *parthno3sed,
+                       const double *guess_qj1, const double *guess_hc, int32_t *iters,
+                       int fuse_conserv);
+
 /* Surrogate step (no counterpart in the reference): sur_qj1 / sur_hc (device, mixing
  * ratios, state layout) are a surrogate model's answer for the step -- the learned
  * network's, rb_mlp_guess_dev.  One FP64 sweep from it is its implicit-step residual

@@ -160,6 +160,21 @@ class ChemistryStep:
             *_step_ptrs(grid, T, p, sza, alt, qj1, hc, surfarea, vsed3d, parth2osed,
                         parthno3sed), g_q, g_h, it, 1 if fuse_conserv else 0))
 
+    def newton_device(self, grid: Grid, dt: float, tol: float, max_newton: int, maxiter: int,
+                      T, p, sza, alt, qj1, hc, surfarea, vsed3d, parth2osed, parthno3sed,
+                      guess_qj1=None, guess_hc=None, iters=None,
+                      fuse_conserv: bool = True) -> None:
+        """Converged mode by Newton iterations with the generated sparse LU (optionally from
+        a guess); boxes it does not solve are finished by plain sweeps (iters 1000 + sweeps)."""
+        field, nbox, _ = _extents(grid)
+        g_q = _ptr(guess_qj1, 42 * field, "guess_qj1") if guess_qj1 is not None else None
+        g_h = _ptr(guess_hc, 15 * field, "guess_hc") if guess_hc is not None else None
+        it = _iptr(iters, nbox, "iters") if iters is not None else None
+        self._check(self._L.rb_chem_newton_dev(
+            self._ctx, C.byref(grid), float(dt), float(tol), int(max_newton), int(maxiter),
+            *_step_ptrs(grid, T, p, sza, alt, qj1, hc, surfarea, vsed3d, parth2osed,
+                        parthno3sed), g_q, g_h, it, 1 if fuse_conserv else 0))
+
     def surrogate_device(self, grid: Grid, dt: float, accept_tol: float, tol: float,
                          maxiter: int, T, p, sza, alt, qj1, hc, surfarea, vsed3d, parth2osed,
                          parthno3sed, sur_qj1, sur_hc, iters=None, fuse_conserv: bool = True,

@@ -35,6 +35,7 @@ Run ``python -m reprobus_amd.codegen.newton_gen`` to rewrite
 from __future__ import annotations
 
 import os
+import re
 import sys
 
 # ------------------------------------------------------------------ species and unknowns
@@ -668,19 +669,36 @@ def system(day: bool) -> dict:
             "cands": [(c[0], c[1], c[2]) for c in cands]}
 
 
+_TMP = re.compile(r"\bt\d+\b")
+
+
+def _live(lines: list[str], uses: list[str]) -> list[str]:
+    """The temporaries the outputs need, in order (the others are dropped)."""
+    need = set()
+    for u in uses:
+        need.update(_TMP.findall(u))
+    keep = []
+    for line in reversed(lines):
+        name = line.split()[2]
+        if name in need:
+            keep.append(line)
+            need.update(_TMP.findall(line.split("=", 1)[1]))
+    return keep[::-1]
+
+
 def emit_system(day: bool):
     """C code of one system: residual and Jacobian into b[] and a[], the LU and the solve.
     Returns a dict of code strings and statistics."""
     sy = system(day)
     g, res = trace(day)
     slot = sy["slot"]
-    lines = list(g.lines)
     body = []
     for i, r in enumerate(res):
         body.append(f"b[{i}] = {_fmt(r.v)};")
     for (i, j), s in sorted(slot.items(), key=lambda kv: kv[1]):
         v = res[i].d.get(j, 0.0)
         body.append(f"a[{s}] = {_fmt(v)};")
+    lines = _live(g.lines, body)
     jac = "\n".join("    " + l for l in lines + body)
     return {"jac": jac, "lu": "\n".join("    " + l for l in render(sy["lu"])),
             "solve": "\n".join("    " + l for l in render(sy["solve"])),
@@ -717,24 +735,22 @@ constexpr int NEWTON_SLOT[NEWTON_N] = {{{slots}}};
 
 FUNC = """// {what}: b <- residual, a <- Jacobian (factor slots, fill-in zero)
 #define RB_NEWTON_{tag}_JAC(a, b) do {{ \\
-{jac}
-}} while (0)
+{jac}}} while (0)
 
 // {what}: a <- LU factors (inverted diagonal), in place
 #define RB_NEWTON_{tag}_LU(a) do {{ \\
-{lu}
-}} while (0)
+{lu}}} while (0)
 
 // {what}: b <- A^-1 b from the factors
 #define RB_NEWTON_{tag}_SOLVE(a, b) do {{ \\
-{solve}
-}} while (0)
+{solve}}} while (0)
 
 """
 
 
 def _macro(body: str) -> str:
-    return " \\\n".join(l.rstrip() for l in body.split("\n"))
+    """Lines of a macro body, each continued onto the next."""
+    return "".join(l.rstrip() + " \\\n" for l in body.split("\n"))
 
 
 def generate() -> tuple[str, dict]:

@@ -64,6 +64,10 @@ struct ConvArgs {
     double accept_tol;                 // > 0 (surrogate step, with a guess): the first
                                        // sweep from the guess tests it at accept_tol; a
                                        // box under it keeps the guess, count 0
+    int32_t iters_offset;              // added to the sweep count of a converged box (the
+                                       // Newton solve's sweep continuation: 1000)
+    int32_t max_newton;                // Newton solve: iterations before a box is handed
+                                       // to the sweeps
 };
 // doubles per box of the set-up record
 constexpr int CONV_RECORD = 141;
@@ -101,6 +105,14 @@ hipError_t launch_chem(const StepArgs &a, const double *rec, int block, hipStrea
 // blocks, then the Anderson phase on nblocks_aa (the Anderson workspace holds its lanes)
 hipError_t launch_chem_conv(const StepArgs &a, const ConvArgs &cv, int block, int nblocks_sweep,
                             int nblocks_aa, hipStream_t s);
+// Newton solve of the converged mode's fixed point: the Newton launch on nblocks blocks,
+// then plain sweeps (the resume phase, nblocks_resume blocks) for the boxes it hands over;
+// cv.spill holds CONV_SPILL doubles per box and cv.max_newton bounds the iterations.  The
+// resume phase adds NEWTON_SWEEP_OFFSET to the sweep counts it reports.
+constexpr int32_t NEWTON_SWEEP_OFFSET = 1000;
+hipError_t launch_chem_newton(const StepArgs &a, const ConvArgs &cv, int block, int nblocks,
+                              int nblocks_resume, hipStream_t s);
+int newton_blocks_per_cu(int block);
 // resident blocks per CU of the converged kernel (the Anderson phase's with accel and
 // anderson_phase)
 int conv_blocks_per_cu(int block, int accel, bool anderson_phase = false);

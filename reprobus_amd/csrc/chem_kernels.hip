@@ -18,6 +18,7 @@
 // -ffp-contract=off) so the step reproduces the reference to a few ulps per box.
 #include "chem_device.h"
 #include "crmath.h"
+#include "newton_gen.h"
 
 #include <math.h>
 
@@ -1632,12 +1633,17 @@ __device__ void aa_update(double *ck, double *cc, double *ws, int64_t stride, in
 // lane needs its fallback set-up); the outer one serves the wave -- write-back of the
 // finished boxes, one queue claim for every waiting lane, set-up from the records.  Keeping
 // the set-up and write-back out of the sweep loop keeps the box state in registers there.
-constexpr int CONV_PLAIN = 0, CONV_SWEEP = 1, CONV_ANDERSON = 2;
+// MODE CONV_RESUME: plain sweeps that continue the boxes the Newton solve hands over on the
+// spill list (boxes it did not bring to the fixed point), like the Anderson phase without
+// the mixing.
+constexpr int CONV_PLAIN = 0, CONV_SWEEP = 1, CONV_ANDERSON = 2, CONV_RESUME = 3;
 
 template <int BLK, int MODE>
 __global__ void __launch_bounds__(BLK) k_chem_conv(StepArgs a, ConvArgs cv)
 {
     constexpr int AAM = MODE == CONV_ANDERSON ? AA_DEPTH : 0;
+    // the second launch of a two-phase solve: its boxes come from the spill list
+    constexpr bool FROM_SPILL = MODE == CONV_ANDERSON || MODE == CONV_RESUME;
     // ck0 and NJL J-values in LDS, as in k_chem
     static_assert((NLONG - 1 + NJL) * BLK * sizeof(double) <= 160 * 1024, "LDS per workgroup");
     __shared__ double s_ck0[(NLONG - 1 + NJL) * BLK];
@@ -1654,9 +1660,9 @@ __global__ void __launch_bounds__(BLK) k_chem_conv(StepArgs a, ConvArgs cv)
     int64_t *sp_box = MODE != CONV_PLAIN ? (int64_t *)(cv.spill + NX * cap) : nullptr;
     int32_t *sp_it = MODE != CONV_PLAIN ? (int32_t *)(cv.spill + (NX + 1) * cap) : nullptr;
     double *sp_d = MODE != CONV_PLAIN ? cv.spill + (NX + 2) * cap : nullptr;
-    unsigned long long *queue = cv.counter + (MODE == CONV_ANDERSON ? 2 : 0);
+    unsigned long long *queue = cv.counter + (FROM_SPILL ? 2 : 0);
     // the Anderson phase starts after the sweep phase has ended: the list is complete
-    const int64_t nwork = MODE == CONV_ANDERSON
+    const int64_t nwork = FROM_SPILL
                               ? (int64_t)__atomic_load_n(cv.counter + 1, __ATOMIC_RELAXED)
                               : a.nbox;
     // lane states: active (sweeping), done (holding a result not yet written), restart
@@ -1698,9 +1704,9 @@ __global__ void __launch_bounds__(BLK) k_chem_conv(StepArgs a, ConvArgs cv)
                 const unsigned long long below = waiting & ((1ull << lane) - 1ull);
                 w = (int64_t)(base + (unsigned long long)__popcll(below));
                 if (w < nwork) {
-                    b = MODE == CONV_ANDERSON ? sp_box[w] : w;
+                    b = FROM_SPILL ? sp_box[w] : w;
                     setup = true;
-                    resume = MODE == CONV_ANDERSON;
+                    resume = FROM_SPILL;
                     fallback = false;
                 } else {
                     finished = true;
@@ -1730,7 +1736,9 @@ __global__ void __launch_bounds__(BLK) k_chem_conv(StepArgs a, ConvArgs cv)
                 it = sp_it[w];
                 d1 = sp_d[w];
                 d2 = sp_d[cap + w];
-                nplain = it;
+                // the sweep phase's boxes arrive after plain sweeps; a box from the Newton
+                // solve (no changes recorded) starts its estimate afresh
+                nplain = d2 > 0. ? it : 0;
             }
         }
         if (__ballot(active) == 0ull) break;
@@ -1782,7 +1790,7 @@ __global__ void __launch_bounds__(BLK) k_chem_conv(StepArgs a, ConvArgs cv)
                 } else if (tr.conv && tr.fin &&
                            ((nplain >= CONV_NPLAIN && near_fixed_point(tr.dmax, d1, d2, cv.tol)) ||
                             (crun + 1 >= CONV_STALL && (rose || tr.dmax >= d1) && nplain >= 1))) {
-                    result = it;
+                    result = it + cv.iters_offset;
                     done = true;
                     active = false;
                 } else if (!tr.fin || it >= cv.maxiter) {
@@ -1814,6 +1822,133 @@ __global__ void __launch_bounds__(BLK) k_chem_conv(StepArgs a, ConvArgs cv)
             const unsigned long long busy = __ballot(active);
             if (__popcll(waiting) >= (unsigned)cv.refill || busy == 0ull || __ballot(restart))
                 break;
+        }
+    }
+}
+
+// Newton solve of the converged mode's fixed point (the same answer as the sweeps to the
+// fixed point, reached in a few iterations whatever the sweep map's contraction): per box
+// and lane,
+//   1. the set-up from the record and one plain sweep from the initial iterate (the guess
+//      or the incoming state), which makes the short-lived species consistent with the
+//      long-lived ones and sets the night zeros exactly;
+//   2. Newton iterations on the local form of the fixed-point conditions, with the
+//      generated residual, Jacobian and sparse LU (newton_gen.h: 55 unknowns, the day or
+//      the night system), the step shortened where it would take a species below a tenth
+//      of its value, until every species moves by less than tol relative in a full step;
+//   3. one plain sweep from the Newton point as its check: every species must change by
+//      less than tol, and the swept state is the result, as for the sweep solver.
+// A box that fails (no convergence in max_newton iterations, a non-finite value, or the
+// check) goes to the spill list with its iterate, and the CONV_RESUME launch finishes it
+// with plain sweeps under the converged mode's stopping rule.  The persistent waves take 64
+// boxes per queue claim; every lane leaves once the queue is exhausted.
+template <int BLK>
+__global__ void __launch_bounds__(BLK) k_chem_newton(StepArgs a, ConvArgs cv)
+{
+    static_assert((NLONG - 1 + NJL) * BLK * sizeof(double) <= 160 * 1024, "LDS per workgroup");
+    __shared__ double s_ck0[(NLONG - 1 + NJL) * BLK];
+    double *ck0 = s_ck0 + threadIdx.x;
+    double *tjl = s_ck0 + (NLONG - 1) * BLK + threadIdx.x;
+    const int lane = threadIdx.x & (warpSize - 1);
+    const int64_t cap = a.nbox;
+    int64_t *sp_box = (int64_t *)(cv.spill + NX * cap);
+    int32_t *sp_it = (int32_t *)(cv.spill + (NX + 1) * cap);
+    double *sp_d = cv.spill + (NX + 2) * cap;
+    for (;;) {
+        unsigned long long base = 0;
+        if (lane == 0) base = atomicAdd(cv.counter, (unsigned long long)warpSize);
+        base = __shfl(base, 0);
+        if ((int64_t)base >= a.nbox) break;
+        const int64_t b = (int64_t)base + lane;
+        if (b < a.nbox) {
+            Box x;
+            box_load<BLK>(a, cv.rec, b, x, ck0, true);
+#pragma unroll
+            for (int i = 0; i < NJL; ++i) tjl[i * BLK] = x.tj[JL_SET[i]];
+            const double fl = 1.e-30 * x.e.hnm;
+            Track tr{true, true, cv.tol, fl, 0.};
+            asm volatile("" ::: "memory");
+            sweep<BLK, true, true>(x.R, x.tj, x.e, ck0, x.ck, x.cc, x.ch3o, x.hco, tr, tjl);
+            int nit = 0;
+            bool ok = false, fin = tr.fin;
+            {
+                const Rates &R = x.R;
+                const BoxEnv &e = x.e;
+                const double *ck = x.ck, *cc = x.cc;
+#define TJ(X) (jl_slot(X) >= 0 ? tjl[jl_slot(X) * BLK] : x.tj[X])
+#define CK0(S) ck0[(S) * BLK]
+                for (nit = 1; fin && nit <= cv.max_newton; ++nit) {
+                    double A[NEWTON_NNZ], B[NEWTON_N];
+                    asm volatile("" ::: "memory");
+                    if (x.e.day) {
+                        RB_NEWTON_DAY_JAC(A, B);
+                        RB_NEWTON_DAY_LU(A);
+                        RB_NEWTON_DAY_SOLVE(A, B);
+                    } else {
+                        RB_NEWTON_NIGHT_JAC(A, B);
+                        RB_NEWTON_NIGHT_LU(A);
+                        RB_NEWTON_NIGHT_SOLVE(A, B);
+                    }
+                    // step length: no species below a tenth of its value in one step
+                    // (a non-finite step -- a zero pivot -- fails the box: fmax below would
+                    // turn its NaN into 0)
+                    double lam = 1.;
+#pragma unroll
+                    for (int i = 0; i < NEWTON_N; ++i) {
+                        const double v = i < NEWTON_NLONG ? x.ck[NEWTON_SLOT[i]] : x.cc[NEWTON_SLOT[i]];
+                        if (B[i] > 0.9 * v && v > 0.) lam = fmin(lam, 0.9 * v / B[i]);
+                        fin &= fabs(B[i]) <= 1.7976931348623157e308;
+                    }
+                    bool conv = lam == 1.;
+#pragma unroll
+                    for (int i = 0; i < NEWTON_N; ++i) {
+                        double &v = i < NEWTON_NLONG ? x.ck[NEWTON_SLOT[i]] : x.cc[NEWTON_SLOT[i]];
+                        const double nv = fmax(v - lam * B[i], 0.);
+                        conv &= fabs(nv - v) < cv.tol * fmax(nv, fl);
+                        fin &= fabs(nv) <= 1.7976931348623157e308;
+                        v = nv;
+                    }
+                    if (conv && fin) {
+                        ok = true;
+                        break;
+                    }
+                }
+#undef TJ
+#undef CK0
+                (void)R; (void)e; (void)ck; (void)cc;
+            }
+            if (ok) {
+                // the check: one plain sweep from the Newton point
+                Track tc{true, true, cv.tol, fl, 0.};
+                asm volatile("" ::: "memory");
+                sweep<BLK, true, true>(x.R, x.tj, x.e, ck0, x.ck, x.cc, x.ch3o, x.hco, tc, tjl);
+                ok = tc.conv && tc.fin;
+                fin = tc.fin;
+            }
+            if (ok) {
+                box_writeback(a, b, x);
+                if (cv.iters) cv.iters[b] = nit;
+            } else {
+                // hand the box to the sweeps: its iterate if finite, else its initial one
+                if (!fin) {
+                    const double hnm = x.e.hnm;
+                    const double *gq = a.guess_q ? a.guess_q : a.qj1;
+                    const double *gh = a.guess_q ? a.guess_h : a.hc;
+#pragma unroll
+                    for (int s = 0; s < NLONG - 1; ++s) x.ck[s] = gq[(int64_t)s * a.field + b] * hnm;
+#pragma unroll
+                    for (int s = 0; s < NSHORT; ++s) x.cc[s] = gh[(int64_t)s * a.field + b] * hnm;
+                }
+                const int64_t s = (int64_t)atomicAdd(cv.counter + 1, 1ull);
+#pragma unroll
+                for (int i = 0; i < NLONG - 1; ++i) cv.spill[i * cap + s] = x.ck[i];
+#pragma unroll
+                for (int i = 0; i < NSHORT; ++i) cv.spill[(NLONG - 1 + i) * cap + s] = x.cc[i];
+                sp_box[s] = b;
+                sp_it[s] = 0;
+                sp_d[s] = 0.;
+                sp_d[cap + s] = 0.;
+            }
         }
     }
 }
@@ -1989,6 +2124,36 @@ hipError_t launch_chem_conv(const StepArgs &a, const ConvArgs &cv, int block, in
     const hipError_t e = launch_conv_mode<CONV_SWEEP>(a, cv, blk, nblocks_sweep, s);
     if (e != hipSuccess) return e;
     return launch_conv_mode<CONV_ANDERSON>(a, cv, blk, nblocks_aa, s);
+}
+
+hipError_t launch_chem_newton(const StepArgs &a, const ConvArgs &cv, int block, int nblocks,
+                              int nblocks_resume, hipStream_t s)
+{
+    const int blk = block > 0 ? block : 256;
+    if (!cv.spill || !cv.rec) return hipErrorInvalidValue;
+    switch (blk) {
+    case 64: hipLaunchKernelGGL(k_chem_newton<64>, dim3(nblocks), dim3(64), 0, s, a, cv); break;
+    case 128: hipLaunchKernelGGL(k_chem_newton<128>, dim3(nblocks), dim3(128), 0, s, a, cv); break;
+    case 256: hipLaunchKernelGGL(k_chem_newton<256>, dim3(nblocks), dim3(256), 0, s, a, cv); break;
+    default: return hipErrorInvalidConfiguration;
+    }
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    ConvArgs rc = cv;
+    rc.accel = 0;
+    rc.aa_ws = nullptr;
+    rc.iters_offset = NEWTON_SWEEP_OFFSET;
+    return launch_conv_mode<CONV_RESUME>(a, rc, blk, nblocks_resume, s);
+}
+
+int newton_blocks_per_cu(int block)
+{
+    int n = 0;
+    hipError_t e = hipErrorInvalidValue;
+    if (block == 64) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_chem_newton<64>, 64, 0);
+    if (block == 128) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_chem_newton<128>, 128, 0);
+    if (block == 256) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_chem_newton<256>, 256, 0);
+    return (e == hipSuccess && n > 0) ? n : 1;
 }
 
 template <int MODE>

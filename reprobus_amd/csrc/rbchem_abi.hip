@@ -355,6 +355,78 @@ int rb_chem_converge_dev(rb_ctx *ctx, const rb_grid *g, double dt, double tol, i
                          fuse_conserv, 0.);
 }
 
+int rb_chem_newton_dev(rb_ctx *ctx, const rb_grid *g, double dt, double tol, int32_t max_newton,
+                       int32_t maxiter, const double *T, const double *p, const double *sza,
+                       const double *alt, double *qj1, double *hc, double *surfarea,
+                       double *vsed3d, double *parth2osed, double *parthno3sed,
+                       const double *guess_qj1, const double *guess_hc, int32_t *iters,
+                       int fuse_conserv)
+{
+    if (!ctx) return RB_EINVAL;
+    int rc = check_grid(ctx, g);
+    if (rc) return rc;
+    if (!ctx->table) return fail(ctx, RB_ENOTABLE, "rb_load_jtable has not been called");
+    if (any_null({T, p, sza, alt, qj1, hc, surfarea, vsed3d, parth2osed, parthno3sed}))
+        return fail(ctx, RB_EINVAL, "a device array is NULL");
+    if ((guess_qj1 == nullptr) != (guess_hc == nullptr))
+        return fail(ctx, RB_EINVAL, "guess_qj1 and guess_hc must be both set or both NULL");
+    if (!(dt > 0.) || !std::isfinite(dt)) return fail(ctx, RB_EINVAL, "dt must be positive");
+    if (!(tol > 0.) || maxiter < 1 || max_newton < 1)
+        return fail(ctx, RB_EINVAL, "tol must be > 0, maxiter >= 1, max_newton >= 1");
+    HIPCHK(hipSetDevice(ctx->device));
+    const int64_t plane = (int64_t)g->nlon * g->nlat;
+    const int64_t field = plane * g->niv;
+    const int64_t nbox = plane * g->nivbas;
+    if ((rc = ensure(ctx, &ctx->d_col, &ctx->col_cap, (size_t)(3 * nbox)))) return rc;
+    if ((rc = conv_buffers(ctx, (size_t)nbox, 0))) return rc;
+    if ((rc = ensure(ctx, &ctx->d_spill, &ctx->spill_cap, (size_t)rb::CONV_SPILL * nbox)))
+        return rc;
+
+    rb::StepArgs a{};
+    a.T = T; a.p = p; a.sza = sza; a.alt = alt; a.col = ctx->d_col;
+    a.qj1 = qj1; a.hc = hc; a.surfarea = surfarea;
+    a.vsed = vsed3d; a.p2o = parth2osed; a.pno3 = parthno3sed;
+    a.ajl = ctx->d_ajl; a.o3up = ctx->d_o3up;
+    a.plane = plane; a.field = field; a.nbox = nbox;
+    a.nivbas = g->nivbas; a.nivhau = g->nivhau; a.niter = ctx->cfg.niter;   // fallback
+    a.fuse_conserv = fuse_conserv ? 1 : 0;
+    a.dt = dt;
+    a.K = ctx->K;
+    a.guess_q = guess_qj1;
+    a.guess_h = guess_hc;
+    const int blk = ctx->cfg.block;
+    const int nblocks = ctx->n_cu * rb::newton_blocks_per_cu(blk);
+    const int nblocks_resume = ctx->n_cu * rb::conv_blocks_per_cu(blk, 0);
+    rb::ConvArgs cv{tol, maxiter, ctx->d_queue, iters, 0, nullptr, ctx->cfg.conv_refill,
+                    ctx->d_rec, 0, ctx->d_spill, 0.};
+    cv.max_newton = max_newton;
+
+    if (!ctx->capturing) HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
+    HIPCHK(rb::launch_columns(a, ctx->d_col, ctx->stream));
+    HIPCHK(rb::launch_setup(a, ctx->d_rec, blk, ctx->stream));
+    HIPCHK(rb::launch_zero_counter(ctx->d_queue, ctx->stream, 3));
+    if (ctx->cfg.profile && !ctx->capturing) {
+        if (ctx->prof_used == ctx->prof.size()) {
+            hipEvent_t e0, e1;
+            HIPCHK(hipEventCreate(&e0));
+            HIPCHK(hipEventCreate(&e1));
+            ctx->prof.emplace_back(e0, e1);
+        }
+        auto &pe = ctx->prof[ctx->prof_used++];
+        HIPCHK(hipEventRecord(pe.first, ctx->stream));
+        HIPCHK(rb::launch_chem_newton(a, cv, blk, nblocks, nblocks_resume, ctx->stream));
+        HIPCHK(hipEventRecord(pe.second, ctx->stream));
+    } else {
+        HIPCHK(rb::launch_chem_newton(a, cv, blk, nblocks, nblocks_resume, ctx->stream));
+    }
+    if (fuse_conserv && field > nbox)
+        HIPCHK(rb::launch_conserv_floor(qj1, hc, plane, field, g->nivbas, nbox, ctx->stream));
+    if (!ctx->capturing) HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
+    ctx->timed = !ctx->capturing;
+    ctx->stats.boxes = nbox;
+    return finite_check(ctx, qj1, hc, field);
+}
+
 int rb_chem_surrogate_dev(rb_ctx *ctx, const rb_grid *g, double dt, double accept_tol,
                           double tol, int32_t maxiter, int32_t accel, const double *T,
                           const double *p, const double *sza, const double *alt, double *qj1,
