@@ -56,6 +56,9 @@ typedef struct {
     int32_t conv_aa_budget; /* converged mode with accel: accelerated sweeps a box gets
                                before it falls back, i.e. the sweep cap is
                                min(maxiter, conv_aa_start + conv_aa_budget) (0: 240) */
+    int32_t newton_waves;   /* Newton solve: resident waves per SIMD of its kernel, 1, 2 or 4
+                               (0: 2) -- more waves hide the latency of its factor storage,
+                               fewer leave each more registers */
 } rb_config;
 
 typedef struct {
@@ -218,21 +221,27 @@ int rb_chem_converge_dev(rb_ctx *ctx, const rb_grid *g, double dt, double tol, i
  * same answer as rb_chem_converge_dev -- the fixed point of the per-box sweep map -- from
  * Newton iterations on the fixed-point conditions in their local form (the QSSA ratios of
  * the short-lived species, the backward-Euler balance of the long-lived ones), whose
- * Jacobian is sparse (55 unknowns, about 400 nonzeros) and factorised by a generated
- * sparse LU with a fixed pivot order.  Per box: one plain sweep from the initial iterate
- * (guess_qj1 / guess_hc, or the incoming state), at most max_newton Newton iterations until
- * a full step moves every species by less than tol relative, then one plain sweep as the
- * check (every species must change by less than tol; the swept state is the result).
- * iters receives the Newton iterations of such a box.  A box Newton does not solve is
- * finished by plain sweeps under rb_chem_converge_dev's stopping rule (iters = 1000 + its
- * sweeps), or gets the fixed-sweep answer with a negative count like there.  The other
- * arguments are rb_chem_converge_dev's. */
-int rb_chem_newton_dev(rb_ctx *ctx, const rb_grid *g, double dt, double tol, int32_t max_newton,
-                       int32_t maxiter, const double *T, const double *p, const double *sza,
-                       const double *alt, double *qj1, double *hc, double *surfarea,
-                       double *vsed3d, double *parth2osed, double *parthno3sed,
-                       const double *guess_qj1, const double *guess_hc, int32_t *iters,
-                       int fuse_conserv);
+ * Jacobian is sparse (55 unknowns, at most 399 nonzeros) and is factorised by a generated
+ * sparse LU with a fixed pivot order, in FP64.
+ *   - With sweeps_first > 0, plain sweeps from the initial iterate (guess_qj1 / guess_hc, or
+ *     the incoming state) come first, under rb_chem_converge_dev's stopping rule; the boxes
+ *     still unconverged at sweep sweeps_first go on to Newton from their iterate.  With
+ *     sweeps_first = 0 every box goes to Newton after one plain sweep.
+ *   - Newton: at most max_newton iterations, each step projected onto non-negative species,
+ *     until a step moves every species by less than tol relative; then one plain sweep as
+ *     the check (every species must change by less than tol; the swept state is the result).
+ *   - A box Newton does not solve gets plain sweeps again, at most 300 more (and maxiter in
+ *     all), then the fixed-sweep answer, as in rb_chem_converge_dev.
+ * iters (optional): k (1 <= k < 10000) for a box the first sweeps converged in k sweeps,
+ * 10000 + n for a box Newton solved in n iterations, 20000 + k for a box the last sweeps
+ * finished after k sweeps in all, negative for the fixed-sweep fallback.  The other
+ * arguments are rb_chem_converge_dev's; 0 <= sweeps_first < maxiter. */
+int rb_chem_newton_dev(rb_ctx *ctx, const rb_grid *g, double dt, double tol,
+                       int32_t sweeps_first, int32_t max_newton, int32_t maxiter,
+                       const double *T, const double *p, const double *sza, const double *alt,
+                       double *qj1, double *hc, double *surfarea, double *vsed3d,
+                       double *parth2osed, double *parthno3sed, const double *guess_qj1,
+                       const double *guess_hc, int32_t *iters, int fuse_conserv);
 
 /* Surrogate step (no counterpart in the reference): sur_qj1 / sur_hc (device, mixing
  * ratios, state layout) are a surrogate model's answer for the step -- the learned

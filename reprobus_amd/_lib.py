@@ -41,7 +41,7 @@ class Grid(C.Structure):
 class Config(C.Structure):
     _fields_ = [("niter", C.c_int32), ("check_finite", C.c_int32), ("block", C.c_int32),
                 ("profile", C.c_int32), ("conv_refill", C.c_int32), ("conv_aa_start", C.c_int32),
-                ("conv_aa_budget", C.c_int32)]
+                ("conv_aa_budget", C.c_int32), ("newton_waves", C.c_int32)]
 
 
 class DayConfig(C.Structure):
@@ -94,7 +94,8 @@ def load() -> C.CDLL:
     L.rb_chem_step_dev.argtypes = [P, C.POINTER(Grid), d] + [P] * 10 + [C.c_int]
     L.rb_conserv_floor_dev.argtypes = [P, C.POINTER(Grid), P, P]
     L.rb_chem_converge_dev.argtypes = [P, C.POINTER(Grid), d, d, C.c_int32, C.c_int32] + [P] * 13 + [C.c_int]
-    L.rb_chem_newton_dev.argtypes = [P, C.POINTER(Grid), d, d, C.c_int32, C.c_int32] + [P] * 13 + [C.c_int]
+    L.rb_chem_newton_dev.argtypes = [P, C.POINTER(Grid), d, d, C.c_int32, C.c_int32, C.c_int32] + \
+        [P] * 13 + [C.c_int]
     L.rb_chem_surrogate_dev.argtypes = [P, C.POINTER(Grid), d, d, d, C.c_int32, C.c_int32] + \
         [P] * 13 + [C.c_int]
     L.rb_sediment.argtypes = [P, C.POINTER(Grid), d, C.c_int32] + [P] * 6

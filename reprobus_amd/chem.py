@@ -76,13 +76,14 @@ class ChemistryStep:
 
     def __init__(self, ajl: np.ndarray, o3up: np.ndarray, device: int = 0, niter: int = 5,
                  block: int = 256, check_finite: bool = False, profile: bool = False,
-                 conv_refill: int = 16, conv_aa_start: int = 60, conv_aa_budget: int = 240):
+                 conv_refill: int = 16, conv_aa_start: int = 60, conv_aa_budget: int = 240,
+                 newton_waves: int = 2):
         self._L = _lib.load()
         self._ctx = C.c_void_p()
         self.conv_refill, self.conv_aa_start = conv_refill, conv_aa_start
         self.conv_aa_budget = conv_aa_budget
         cfg = Config(niter, 1 if check_finite else 0, block, 1 if profile else 0, conv_refill,
-                     conv_aa_start, conv_aa_budget)
+                     conv_aa_start, conv_aa_budget, newton_waves)
         rc = self._L.rb_init(C.byref(self._ctx), device, C.byref(cfg))
         if rc != 0:
             raise RbError(rc, f"rb_init(device={device}) failed")
@@ -162,16 +163,19 @@ class ChemistryStep:
 
     def newton_device(self, grid: Grid, dt: float, tol: float, max_newton: int, maxiter: int,
                       T, p, sza, alt, qj1, hc, surfarea, vsed3d, parth2osed, parthno3sed,
-                      guess_qj1=None, guess_hc=None, iters=None,
-                      fuse_conserv: bool = True) -> None:
+                      guess_qj1=None, guess_hc=None, iters=None, fuse_conserv: bool = True,
+                      sweeps_first: int = 0) -> None:
         """Converged mode by Newton iterations with the generated sparse LU (optionally from
-        a guess); boxes it does not solve are finished by plain sweeps (iters 1000 + sweeps)."""
+        a guess; with sweeps_first > 0 after that many plain sweeps, for the boxes still
+        unconverged).  iters: k sweeps, 10000 + Newton iterations, 20000 + sweeps (boxes
+        Newton handed back to the sweeps), negative: fallback."""
         field, nbox, _ = _extents(grid)
         g_q = _ptr(guess_qj1, 42 * field, "guess_qj1") if guess_qj1 is not None else None
         g_h = _ptr(guess_hc, 15 * field, "guess_hc") if guess_hc is not None else None
         it = _iptr(iters, nbox, "iters") if iters is not None else None
         self._check(self._L.rb_chem_newton_dev(
-            self._ctx, C.byref(grid), float(dt), float(tol), int(max_newton), int(maxiter),
+            self._ctx, C.byref(grid), float(dt), float(tol), int(sweeps_first), int(max_newton),
+            int(maxiter),
             *_step_ptrs(grid, T, p, sza, alt, qj1, hc, surfarea, vsed3d, parth2osed,
                         parthno3sed), g_q, g_h, it, 1 if fuse_conserv else 0))
 

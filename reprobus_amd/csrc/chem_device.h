@@ -65,7 +65,7 @@ struct ConvArgs {
                                        // sweep from the guess tests it at accept_tol; a
                                        // box under it keeps the guess, count 0
     int32_t iters_offset;              // added to the sweep count of a converged box (the
-                                       // Newton solve's sweep continuation: 1000)
+                                       // Newton solve's resume phase: NEWTON_RESUMED)
     int32_t max_newton;                // Newton solve: iterations before a box is handed
                                        // to the sweeps
 };
@@ -105,14 +105,29 @@ hipError_t launch_chem(const StepArgs &a, const double *rec, int block, hipStrea
 // blocks, then the Anderson phase on nblocks_aa (the Anderson workspace holds its lanes)
 hipError_t launch_chem_conv(const StepArgs &a, const ConvArgs &cv, int block, int nblocks_sweep,
                             int nblocks_aa, hipStream_t s);
-// Newton solve of the converged mode's fixed point: the Newton launch on nblocks blocks,
-// then plain sweeps (the resume phase, nblocks_resume blocks) for the boxes it hands over;
-// cv.spill holds CONV_SPILL doubles per box and cv.max_newton bounds the iterations.  The
-// resume phase adds NEWTON_SWEEP_OFFSET to the sweep counts it reports.
-constexpr int32_t NEWTON_SWEEP_OFFSET = 1000;
-hipError_t launch_chem_newton(const StepArgs &a, const ConvArgs &cv, int block, int nblocks,
-                              int nblocks_resume, hipStream_t s);
-int newton_blocks_per_cu(int block);
+// Newton solve of the converged mode's fixed point (launch_chem_newton): optionally plain
+// sweeps first (sweeps_first > 0: boxes unconverged at that sweep go on), Newton on those
+// boxes (or on every box), plain sweeps (the resume phase, at most resume_maxiter) for the
+// boxes Newton hands over.  cv.counter holds 5 counters (zero at launch), cv.spill and
+// spill2 CONV_SPILL doubles per box each.  Per-box counts: a box converged by the first
+// sweeps has its sweep count, a box Newton solved NEWTON_SOLVED + its Newton iterations,
+// a box finished by the resume phase NEWTON_RESUMED + its sweeps; negative: fallback.
+constexpr int32_t NEWTON_SOLVED = 10000, NEWTON_RESUMED = 20000;
+struct NewtonArgs {
+    const double *in;                  // input spill list (boxes from the sweep phase) or null
+    double *out;                       // output spill list (boxes for the resume phase)
+    unsigned long long *queue;         // work-queue head of the Newton launch
+    const unsigned long long *n_in;    // entries of the input list
+    unsigned long long *n_out;         // entries of the output list
+};
+struct NewtonLaunch {
+    int32_t sweeps_first, waves, block, resume_maxiter;
+    int32_t nblocks_sweep, nblocks_newton, nblocks_resume;
+    double *spill2;
+};
+hipError_t launch_chem_newton(const StepArgs &a, const ConvArgs &cv, const NewtonLaunch &nl,
+                              hipStream_t s);
+int newton_blocks_per_cu(int waves);
 // resident blocks per CU of the converged kernel (the Anderson phase's with accel and
 // anderson_phase)
 int conv_blocks_per_cu(int block, int accel, bool anderson_phase = false);

@@ -1827,59 +1827,80 @@ __global__ void __launch_bounds__(BLK) k_chem_conv(StepArgs a, ConvArgs cv)
 }
 
 // Newton solve of the converged mode's fixed point (the same answer as the sweeps to the
-// fixed point, reached in a few iterations whatever the sweep map's contraction): per box
-// and lane,
-//   1. the set-up from the record and one plain sweep from the initial iterate (the guess
-//      or the incoming state), which makes the short-lived species consistent with the
-//      long-lived ones and sets the night zeros exactly;
-//   2. Newton iterations on the local form of the fixed-point conditions, with the
+// fixed point, reached in a few iterations whatever the sweep map's contraction), per box
+// and lane.  Its boxes come either from the whole grid (FROM_SPILL false: set-up from the
+// record and one plain sweep from the initial iterate -- the guess or the incoming state --
+// which makes the short-lived species consistent with the long-lived ones and sets the
+// night zeros exactly) or from a spill list (FROM_SPILL true: the boxes a plain sweep phase
+// left unconverged after its first sweeps, with their iterates).  Then
+//   1. Newton iterations on the local form of the fixed-point conditions, with the
 //      generated residual, Jacobian and sparse LU (newton_gen.h: 55 unknowns, the day or
-//      the night system), the step shortened where it would take a species below a tenth
-//      of its value, until every species moves by less than tol relative in a full step;
-//   3. one plain sweep from the Newton point as its check: every species must change by
+//      the night system), each step projected onto non-negative species, until a step
+//      moves every species by less than tol relative (or the steps stop shrinking below
+//      1e-8: the conditioning's floor, where the check decides);
+//   2. one plain sweep from the Newton point as its check: every species must change by
 //      less than tol, and the swept state is the result, as for the sweep solver.
 // A box that fails (no convergence in max_newton iterations, a non-finite value, or the
-// check) goes to the spill list with its iterate, and the CONV_RESUME launch finishes it
-// with plain sweeps under the converged mode's stopping rule.  The persistent waves take 64
-// boxes per queue claim; every lane leaves once the queue is exhausted.
-template <int BLK>
-__global__ void __launch_bounds__(BLK) k_chem_newton(StepArgs a, ConvArgs cv)
+// check) goes to the output spill list with its iterate, for the plain-sweep resume phase.
+//
+// The factors (514 doubles by day) do not fit the register file, so they live in the
+// lane's scratch; WAVES resident waves per SIMD (1, 2, 4: a launch shape the autotuner
+// chooses) overlap one wave's scratch traffic with another's arithmetic.  The start-of-step
+// state and the J-values stay with the lane too (no LDS, which would cap the waves).  The
+// persistent waves take 64 boxes per queue claim; every lane leaves once the queue is
+// exhausted.
+constexpr int NEWTON_BLK = 64;
+
+template <int WAVES, bool FROM_SPILL>
+__global__ void __launch_bounds__(NEWTON_BLK, WAVES) k_chem_newton(StepArgs a, ConvArgs cv,
+                                                                   NewtonArgs na)
 {
-    static_assert((NLONG - 1 + NJL) * BLK * sizeof(double) <= 160 * 1024, "LDS per workgroup");
-    __shared__ double s_ck0[(NLONG - 1 + NJL) * BLK];
-    double *ck0 = s_ck0 + threadIdx.x;
-    double *tjl = s_ck0 + (NLONG - 1) * BLK + threadIdx.x;
     const int lane = threadIdx.x & (warpSize - 1);
     const int64_t cap = a.nbox;
-    int64_t *sp_box = (int64_t *)(cv.spill + NX * cap);
-    int32_t *sp_it = (int32_t *)(cv.spill + (NX + 1) * cap);
-    double *sp_d = cv.spill + (NX + 2) * cap;
+    const int64_t *in_box = FROM_SPILL ? (const int64_t *)(na.in + NX * cap) : nullptr;
+    const int32_t *in_it = FROM_SPILL ? (const int32_t *)(na.in + (NX + 1) * cap) : nullptr;
+    int64_t *out_box = (int64_t *)(na.out + NX * cap);
+    int32_t *out_it = (int32_t *)(na.out + (NX + 1) * cap);
+    double *out_d = na.out + (NX + 2) * cap;
+    // the input list is complete when this launch starts (the sweep phase has ended)
+    const int64_t nwork = FROM_SPILL ? (int64_t)__atomic_load_n(na.n_in, __ATOMIC_RELAXED) : a.nbox;
     for (;;) {
         unsigned long long base = 0;
-        if (lane == 0) base = atomicAdd(cv.counter, (unsigned long long)warpSize);
+        if (lane == 0) base = atomicAdd(na.queue, (unsigned long long)warpSize);
         base = __shfl(base, 0);
-        if ((int64_t)base >= a.nbox) break;
-        const int64_t b = (int64_t)base + lane;
-        if (b < a.nbox) {
+        if ((int64_t)base >= nwork) break;
+        const int64_t w = (int64_t)base + lane;
+        if (w < nwork) {
+            const int64_t b = FROM_SPILL ? in_box[w] : w;
             Box x;
-            box_load<BLK>(a, cv.rec, b, x, ck0, true);
-#pragma unroll
-            for (int i = 0; i < NJL; ++i) tjl[i * BLK] = x.tj[JL_SET[i]];
+            double ck0[NLONG - 1];
+            box_load<1>(a, cv.rec, b, x, ck0, !FROM_SPILL);
             const double fl = 1.e-30 * x.e.hnm;
-            Track tr{true, true, cv.tol, fl, 0.};
-            asm volatile("" ::: "memory");
-            sweep<BLK, true, true>(x.R, x.tj, x.e, ck0, x.ck, x.cc, x.ch3o, x.hco, tr, tjl);
+            int sweeps = 0;
+            bool fin = true;
+            if (FROM_SPILL) {
+#pragma unroll
+                for (int i = 0; i < NLONG - 1; ++i) x.ck[i] = na.in[i * cap + w];
+#pragma unroll
+                for (int i = 0; i < NSHORT; ++i) x.cc[i] = na.in[(NLONG - 1 + i) * cap + w];
+                sweeps = in_it[w];
+            } else {
+                Track tr{true, true, cv.tol, fl, 0.};
+                sweep<1, true, false>(x.R, x.tj, x.e, ck0, x.ck, x.cc, x.ch3o, x.hco, tr);
+                sweeps = 1;
+                fin = tr.fin;
+            }
             int nit = 0;
-            bool ok = false, fin = tr.fin;
+            bool ok = false;
+            double sprev = 1.e300;
             {
                 const Rates &R = x.R;
                 const BoxEnv &e = x.e;
                 const double *ck = x.ck, *cc = x.cc;
-#define TJ(X) (jl_slot(X) >= 0 ? tjl[jl_slot(X) * BLK] : x.tj[X])
-#define CK0(S) ck0[(S) * BLK]
+#define TJ(X) x.tj[X]
+#define CK0(S) ck0[S]
                 for (nit = 1; fin && nit <= cv.max_newton; ++nit) {
                     double A[NEWTON_NNZ], B[NEWTON_N];
-                    asm volatile("" ::: "memory");
                     if (x.e.day) {
                         RB_NEWTON_DAY_JAC(A, B);
                         RB_NEWTON_DAY_LU(A);
@@ -1889,26 +1910,25 @@ __global__ void __launch_bounds__(BLK) k_chem_newton(StepArgs a, ConvArgs cv)
                         RB_NEWTON_NIGHT_LU(A);
                         RB_NEWTON_NIGHT_SOLVE(A, B);
                     }
-                    // step length: no species below a tenth of its value in one step
-                    // (a non-finite step -- a zero pivot -- fails the box: fmax below would
-                    // turn its NaN into 0)
-                    double lam = 1.;
-#pragma unroll
-                    for (int i = 0; i < NEWTON_N; ++i) {
-                        const double v = i < NEWTON_NLONG ? x.ck[NEWTON_SLOT[i]] : x.cc[NEWTON_SLOT[i]];
-                        if (B[i] > 0.9 * v && v > 0.) lam = fmin(lam, 0.9 * v / B[i]);
-                        fin &= fabs(B[i]) <= 1.7976931348623157e308;
-                    }
-                    bool conv = lam == 1.;
+                    // the full step, each species projected onto x >= 0: a species the step
+                    // would take below zero (one heading for a tiny root: night NO2 below its
+                    // clamp, HO2 or CH3OOH in weak sunlight) goes to 0 and the next step
+                    // lifts it to its root.  (Shortening the whole step instead let one such
+                    // species freeze every other one for dozens of iterations.)  A
+                    // non-finite step -- a zero pivot -- fails the box: fmax would turn its
+                    // NaN into 0.
+                    double smax = 0.;
 #pragma unroll
                     for (int i = 0; i < NEWTON_N; ++i) {
                         double &v = i < NEWTON_NLONG ? x.ck[NEWTON_SLOT[i]] : x.cc[NEWTON_SLOT[i]];
-                        const double nv = fmax(v - lam * B[i], 0.);
-                        conv &= fabs(nv - v) < cv.tol * fmax(nv, fl);
-                        fin &= fabs(nv) <= 1.7976931348623157e308;
+                        fin &= fabs(B[i]) <= 1.7976931348623157e308;
+                        const double nv = fmax(v - B[i], 0.);
+                        smax = fmax(smax, fabs(nv - v) * __builtin_amdgcn_rcp(fmax(nv, fl)));
                         v = nv;
                     }
-                    if (conv && fin) {
+                    const bool done = smax < cv.tol || (smax < 1.e-8 && smax >= 0.5 * sprev);
+                    sprev = smax;
+                    if (done && fin) {
                         ok = true;
                         break;
                     }
@@ -1920,14 +1940,14 @@ __global__ void __launch_bounds__(BLK) k_chem_newton(StepArgs a, ConvArgs cv)
             if (ok) {
                 // the check: one plain sweep from the Newton point
                 Track tc{true, true, cv.tol, fl, 0.};
-                asm volatile("" ::: "memory");
-                sweep<BLK, true, true>(x.R, x.tj, x.e, ck0, x.ck, x.cc, x.ch3o, x.hco, tc, tjl);
+                sweep<1, true, false>(x.R, x.tj, x.e, ck0, x.ck, x.cc, x.ch3o, x.hco, tc);
+                ++sweeps;
                 ok = tc.conv && tc.fin;
                 fin = tc.fin;
             }
             if (ok) {
                 box_writeback(a, b, x);
-                if (cv.iters) cv.iters[b] = nit;
+                if (cv.iters) cv.iters[b] = NEWTON_SOLVED + nit;
             } else {
                 // hand the box to the sweeps: its iterate if finite, else its initial one
                 if (!fin) {
@@ -1939,15 +1959,15 @@ __global__ void __launch_bounds__(BLK) k_chem_newton(StepArgs a, ConvArgs cv)
 #pragma unroll
                     for (int s = 0; s < NSHORT; ++s) x.cc[s] = gh[(int64_t)s * a.field + b] * hnm;
                 }
-                const int64_t s = (int64_t)atomicAdd(cv.counter + 1, 1ull);
+                const int64_t s = (int64_t)atomicAdd(na.n_out, 1ull);
 #pragma unroll
-                for (int i = 0; i < NLONG - 1; ++i) cv.spill[i * cap + s] = x.ck[i];
+                for (int i = 0; i < NLONG - 1; ++i) na.out[i * cap + s] = x.ck[i];
 #pragma unroll
-                for (int i = 0; i < NSHORT; ++i) cv.spill[(NLONG - 1 + i) * cap + s] = x.cc[i];
-                sp_box[s] = b;
-                sp_it[s] = 0;
-                sp_d[s] = 0.;
-                sp_d[cap + s] = 0.;
+                for (int i = 0; i < NSHORT; ++i) na.out[(NLONG - 1 + i) * cap + s] = x.cc[i];
+                out_box[s] = b;
+                out_it[s] = sweeps;
+                out_d[s] = 0.;
+                out_d[cap + s] = 0.;
             }
         }
     }
@@ -2126,33 +2146,65 @@ hipError_t launch_chem_conv(const StepArgs &a, const ConvArgs &cv, int block, in
     return launch_conv_mode<CONV_ANDERSON>(a, cv, blk, nblocks_aa, s);
 }
 
-hipError_t launch_chem_newton(const StepArgs &a, const ConvArgs &cv, int block, int nblocks,
-                              int nblocks_resume, hipStream_t s)
+template <bool FS>
+static hipError_t launch_newton_kernel(const StepArgs &a, const ConvArgs &cv, const NewtonArgs &na,
+                                       int waves, int nblocks, hipStream_t s)
 {
-    const int blk = block > 0 ? block : 256;
-    if (!cv.spill || !cv.rec) return hipErrorInvalidValue;
-    switch (blk) {
-    case 64: hipLaunchKernelGGL(k_chem_newton<64>, dim3(nblocks), dim3(64), 0, s, a, cv); break;
-    case 128: hipLaunchKernelGGL(k_chem_newton<128>, dim3(nblocks), dim3(128), 0, s, a, cv); break;
-    case 256: hipLaunchKernelGGL(k_chem_newton<256>, dim3(nblocks), dim3(256), 0, s, a, cv); break;
+    switch (waves) {
+    case 1: hipLaunchKernelGGL((k_chem_newton<1, FS>), dim3(nblocks), dim3(NEWTON_BLK), 0, s, a, cv, na); break;
+    case 2: hipLaunchKernelGGL((k_chem_newton<2, FS>), dim3(nblocks), dim3(NEWTON_BLK), 0, s, a, cv, na); break;
+    case 4: hipLaunchKernelGGL((k_chem_newton<4, FS>), dim3(nblocks), dim3(NEWTON_BLK), 0, s, a, cv, na); break;
     default: return hipErrorInvalidConfiguration;
     }
-    const hipError_t e = hipGetLastError();
+    return hipGetLastError();
+}
+
+hipError_t launch_chem_newton(const StepArgs &a, const ConvArgs &cv, const NewtonLaunch &nl,
+                              hipStream_t s)
+{
+    if (!cv.spill || !cv.rec || !nl.spill2) return hipErrorInvalidValue;
+    const int blk = nl.block > 0 ? nl.block : 256;
+    unsigned long long *q = cv.counter;   // 5 counters, zero at launch
+    NewtonArgs na{};
+    na.out = nl.spill2;
+    na.n_out = q + 3;
+    hipError_t e;
+    if (nl.sweeps_first > 0) {
+        // plain sweeps first; the boxes still unconverged at sweep sweeps_first go to the
+        // spill list (counters 0, 1), Newton takes them from there (counter 2)
+        ConvArgs sc = cv;
+        sc.accel = 0;
+        sc.aa_ws = nullptr;
+        sc.aa_start = nl.sweeps_first;
+        if ((e = launch_conv_mode<CONV_SWEEP>(a, sc, blk, nl.nblocks_sweep, s)) != hipSuccess)
+            return e;
+        na.in = cv.spill;
+        na.n_in = q + 1;
+        na.queue = q + 2;
+        e = launch_newton_kernel<true>(a, cv, na, nl.waves, nl.nblocks_newton, s);
+    } else {
+        na.queue = q;
+        e = launch_newton_kernel<false>(a, cv, na, nl.waves, nl.nblocks_newton, s);
+    }
     if (e != hipSuccess) return e;
+    // plain sweeps for the boxes Newton handed over (list counters 3, 4)
     ConvArgs rc = cv;
     rc.accel = 0;
     rc.aa_ws = nullptr;
-    rc.iters_offset = NEWTON_SWEEP_OFFSET;
-    return launch_conv_mode<CONV_RESUME>(a, rc, blk, nblocks_resume, s);
+    rc.counter = q + 2;
+    rc.spill = nl.spill2;
+    rc.maxiter = nl.resume_maxiter;
+    rc.iters_offset = NEWTON_RESUMED;
+    return launch_conv_mode<CONV_RESUME>(a, rc, blk, nl.nblocks_resume, s);
 }
 
-int newton_blocks_per_cu(int block)
+int newton_blocks_per_cu(int waves)
 {
     int n = 0;
     hipError_t e = hipErrorInvalidValue;
-    if (block == 64) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_chem_newton<64>, 64, 0);
-    if (block == 128) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_chem_newton<128>, 128, 0);
-    if (block == 256) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_chem_newton<256>, 256, 0);
+    if (waves == 1) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_chem_newton<1, true>, NEWTON_BLK, 0);
+    if (waves == 2) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_chem_newton<2, true>, NEWTON_BLK, 0);
+    if (waves == 4) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_chem_newton<4, true>, NEWTON_BLK, 0);
     return (e == hipSuccess && n > 0) ? n : 1;
 }
 

@@ -18,7 +18,7 @@ struct rb_ctx {
     int device = 0;
     hipStream_t own = nullptr;
     hipStream_t stream = nullptr;
-    rb_config cfg{5, 0, 256, 0, 16, 60};
+    rb_config cfg{5, 0, 256, 0, 16, 60, 240, 2};
     rb::Troe K{};
     double *d_ajl = nullptr, *d_o3up = nullptr;
     bool table = false;
@@ -27,7 +27,7 @@ struct rb_ctx {
     double *d_stage = nullptr;        // staging for the host-array entry points
     size_t stage_cap = 0;
     unsigned long long *d_count = nullptr;
-    unsigned long long *d_queue = nullptr;   // converged mode: queue heads, spill count (3)
+    unsigned long long *d_queue = nullptr;   // converged mode: queue heads, spill counts (5)
     double *d_work = nullptr;                // altitude: level pressure, Tv, geopotential
     size_t work_cap = 0;
     double *d_aa = nullptr;                  // converged mode: Anderson history per lane
@@ -127,6 +127,9 @@ bool any_null(std::initializer_list<const void *> ps)
 // rb_config.check_finite: count the non-finite values of the state after a step (the
 // whole qj1 and hc fields), report the count in rb_stats and return RB_ENONFINITE when
 // there is any.  It synchronises the stream, so it is skipped inside a graph capture.
+// Newton solve: plain sweeps a box that Newton hands over still gets
+constexpr int32_t NEWTON_RESUME_SWEEPS = 300;
+
 int finite_check(rb_ctx *ctx, const double *qj1, const double *hc, int64_t field)
 {
     if (!ctx->cfg.check_finite || ctx->capturing) return RB_OK;
@@ -165,6 +168,11 @@ int rb_init(rb_ctx **out, int device, const rb_config *cfg)
     if (ctx->cfg.conv_refill > 64) ctx->cfg.conv_refill = 64;
     if (ctx->cfg.conv_aa_start < 0) ctx->cfg.conv_aa_start = 60;
     if (ctx->cfg.conv_aa_budget <= 0) ctx->cfg.conv_aa_budget = 240;
+    if (ctx->cfg.newton_waves <= 0) ctx->cfg.newton_waves = 2;
+    if (ctx->cfg.newton_waves != 1 && ctx->cfg.newton_waves != 2 && ctx->cfg.newton_waves != 4) {
+        delete ctx;
+        return RB_EINVAL;   // the Newton kernel is instantiated for 1, 2 and 4 waves per SIMD
+    }
     if (ctx->cfg.block != 64 && ctx->cfg.block != 128 && ctx->cfg.block != 256) {
         delete ctx;
         return RB_EINVAL;   // the per-box kernel is instantiated for 64, 128 and 256 threads
@@ -172,7 +180,7 @@ int rb_init(rb_ctx **out, int device, const rb_config *cfg)
     ctx->K = rb::troe_constants();
     if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&ctx->own) != hipSuccess ||
         hipMalloc((void **)&ctx->d_count, sizeof(unsigned long long)) != hipSuccess ||
-        hipMalloc((void **)&ctx->d_queue, 3 * sizeof(unsigned long long)) != hipSuccess ||
+        hipMalloc((void **)&ctx->d_queue, 5 * sizeof(unsigned long long)) != hipSuccess ||
         hipDeviceGetAttribute(&ctx->n_cu, hipDeviceAttributeMultiprocessorCount, device) !=
             hipSuccess ||
         hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess ||
@@ -355,8 +363,9 @@ int rb_chem_converge_dev(rb_ctx *ctx, const rb_grid *g, double dt, double tol, i
                          fuse_conserv, 0.);
 }
 
-int rb_chem_newton_dev(rb_ctx *ctx, const rb_grid *g, double dt, double tol, int32_t max_newton,
-                       int32_t maxiter, const double *T, const double *p, const double *sza,
+int rb_chem_newton_dev(rb_ctx *ctx, const rb_grid *g, double dt, double tol, int32_t sweeps_first,
+                       int32_t max_newton, int32_t maxiter, const double *T, const double *p,
+                       const double *sza,
                        const double *alt, double *qj1, double *hc, double *surfarea,
                        double *vsed3d, double *parth2osed, double *parthno3sed,
                        const double *guess_qj1, const double *guess_hc, int32_t *iters,
@@ -371,15 +380,18 @@ int rb_chem_newton_dev(rb_ctx *ctx, const rb_grid *g, double dt, double tol, int
     if ((guess_qj1 == nullptr) != (guess_hc == nullptr))
         return fail(ctx, RB_EINVAL, "guess_qj1 and guess_hc must be both set or both NULL");
     if (!(dt > 0.) || !std::isfinite(dt)) return fail(ctx, RB_EINVAL, "dt must be positive");
-    if (!(tol > 0.) || maxiter < 1 || max_newton < 1)
-        return fail(ctx, RB_EINVAL, "tol must be > 0, maxiter >= 1, max_newton >= 1");
+    if (!(tol > 0.) || maxiter < 1 || max_newton < 1 || sweeps_first < 0 ||
+        sweeps_first >= maxiter)
+        return fail(ctx, RB_EINVAL,
+                    "tol must be > 0, maxiter >= 1, max_newton >= 1, 0 <= sweeps_first < maxiter");
     HIPCHK(hipSetDevice(ctx->device));
     const int64_t plane = (int64_t)g->nlon * g->nlat;
     const int64_t field = plane * g->niv;
     const int64_t nbox = plane * g->nivbas;
     if ((rc = ensure(ctx, &ctx->d_col, &ctx->col_cap, (size_t)(3 * nbox)))) return rc;
     if ((rc = conv_buffers(ctx, (size_t)nbox, 0))) return rc;
-    if ((rc = ensure(ctx, &ctx->d_spill, &ctx->spill_cap, (size_t)rb::CONV_SPILL * nbox)))
+    // two spill lists: sweep phase -> Newton, Newton -> resume phase
+    if ((rc = ensure(ctx, &ctx->d_spill, &ctx->spill_cap, 2 * (size_t)rb::CONV_SPILL * nbox)))
         return rc;
 
     rb::StepArgs a{};
@@ -395,16 +407,27 @@ int rb_chem_newton_dev(rb_ctx *ctx, const rb_grid *g, double dt, double tol, int
     a.guess_q = guess_qj1;
     a.guess_h = guess_hc;
     const int blk = ctx->cfg.block;
-    const int nblocks = ctx->n_cu * rb::newton_blocks_per_cu(blk);
-    const int nblocks_resume = ctx->n_cu * rb::conv_blocks_per_cu(blk, 0);
     rb::ConvArgs cv{tol, maxiter, ctx->d_queue, iters, 0, nullptr, ctx->cfg.conv_refill,
                     ctx->d_rec, 0, ctx->d_spill, 0.};
     cv.max_newton = max_newton;
+    rb::NewtonLaunch nl{};
+    nl.sweeps_first = sweeps_first;
+    nl.waves = ctx->cfg.newton_waves;
+    nl.block = blk;
+    // a box Newton hands over gets at most NEWTON_RESUME_SWEEPS more plain sweeps: such
+    // boxes are the ones plain sweeps converge slowest or not at all, and one lane sweeping
+    // one of them to maxiter would set the time of the whole solve
+    nl.resume_maxiter = (int32_t)std::min<int64_t>(maxiter, (int64_t)sweeps_first + 2 +
+                                                                NEWTON_RESUME_SWEEPS);
+    nl.nblocks_sweep = ctx->n_cu * rb::conv_blocks_per_cu(blk, 2);
+    nl.nblocks_newton = ctx->n_cu * rb::newton_blocks_per_cu(nl.waves);
+    nl.nblocks_resume = ctx->n_cu * rb::conv_blocks_per_cu(blk, 0);
+    nl.spill2 = ctx->d_spill + (size_t)rb::CONV_SPILL * nbox;
 
     if (!ctx->capturing) HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
     HIPCHK(rb::launch_columns(a, ctx->d_col, ctx->stream));
     HIPCHK(rb::launch_setup(a, ctx->d_rec, blk, ctx->stream));
-    HIPCHK(rb::launch_zero_counter(ctx->d_queue, ctx->stream, 3));
+    HIPCHK(rb::launch_zero_counter(ctx->d_queue, ctx->stream, 5));
     if (ctx->cfg.profile && !ctx->capturing) {
         if (ctx->prof_used == ctx->prof.size()) {
             hipEvent_t e0, e1;
@@ -414,10 +437,10 @@ int rb_chem_newton_dev(rb_ctx *ctx, const rb_grid *g, double dt, double tol, int
         }
         auto &pe = ctx->prof[ctx->prof_used++];
         HIPCHK(hipEventRecord(pe.first, ctx->stream));
-        HIPCHK(rb::launch_chem_newton(a, cv, blk, nblocks, nblocks_resume, ctx->stream));
+        HIPCHK(rb::launch_chem_newton(a, cv, nl, ctx->stream));
         HIPCHK(hipEventRecord(pe.second, ctx->stream));
     } else {
-        HIPCHK(rb::launch_chem_newton(a, cv, blk, nblocks, nblocks_resume, ctx->stream));
+        HIPCHK(rb::launch_chem_newton(a, cv, nl, ctx->stream));
     }
     if (fuse_conserv && field > nbox)
         HIPCHK(rb::launch_conserv_floor(qj1, hc, plane, field, g->nivbas, nbox, ctx->stream));

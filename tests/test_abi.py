@@ -40,6 +40,6 @@ def test_struct_layouts_match_header():
     from reprobus_amd import _lib
 
     assert C.sizeof(_lib.Grid) == 5 * 4
-    assert C.sizeof(_lib.Config) == 7 * 4
+    assert C.sizeof(_lib.Config) == 8 * 4
     assert C.sizeof(_lib.Stats) == 8 * 7
     assert C.sizeof(_lib.DayConfig) == 4 * 4 + 3 * 8 + 2 * 4

@@ -42,7 +42,8 @@ def _dev(inp):
     return d
 
 
-def solve_gpu(step, inp, how, tol=TOL, guess=None, max_newton=20, maxiter=2000, fuse=False):
+def solve_gpu(step, inp, how, tol=TOL, guess=None, max_newton=20, maxiter=2000, fuse=False,
+              sweeps_first=0):
     import torch
     from reprobus_amd.chem import make_grid
 
@@ -59,7 +60,8 @@ def solve_gpu(step, inp, how, tol=TOL, guess=None, max_newton=20, maxiter=2000, 
     step.set_stream(torch.cuda.current_stream().cuda_stream)
     if how == "newton":
         step.newton_device(grid, inp.dt, tol, max_newton, maxiter, *args, guess_qj1=gq,
-                           guess_hc=gh, iters=d["iters"], fuse_conserv=fuse)
+                           guess_hc=gh, iters=d["iters"], fuse_conserv=fuse,
+                           sweeps_first=sweeps_first)
     else:
         step.converge_device(grid, inp.dt, tol, maxiter, *args, guess_qj1=gq, guess_hc=gh,
                              iters=d["iters"], fuse_conserv=fuse)
@@ -68,14 +70,21 @@ def solve_gpu(step, inp, how, tol=TOL, guess=None, max_newton=20, maxiter=2000, 
     return {k: v.cpu().numpy() for k, v in d.items()}
 
 
+SOLVED, RESUMED = 10000, 20000
+
+
 def _summary(tag, it):
-    nw = (it > 0) & (it < 1000)
-    sw = it >= 1000
-    h = np.bincount(it[nw].ravel(), minlength=8)[1:8]
-    print(f"[newton] {tag}: Newton {int(nw.sum())} boxes (iterations 1..7: {h.tolist()}, max "
-          f"{int(it[nw].max()) if nw.any() else 0}), handed to sweeps {int(sw.sum())}, "
-          f"fixed-sweep fallback {int((it < 0).sum())} of {it.size}", flush=True)
-    return nw, sw
+    """Boxes the first sweeps converged, Newton solved, the last sweeps finished."""
+    sw = (it > 0) & (it < SOLVED)
+    nw = (it >= SOLVED) & (it < RESUMED)
+    rs = it >= RESUMED
+    n = it[nw] - SOLVED
+    h = np.bincount(n.ravel(), minlength=8)[1:8] if nw.any() else np.zeros(7, int)
+    print(f"[newton] {tag}: sweeps {int(sw.sum())}, Newton {int(nw.sum())} (iterations 1..7: "
+          f"{h.tolist()}, max {int(n.max()) if nw.any() else 0}), handed back to sweeps "
+          f"{int(rs.sum())}, fixed-sweep fallback {int((it < 0).sum())} of {it.size}",
+          flush=True)
+    return nw, rs
 
 
 @pytest.fixture(scope="module")
@@ -89,25 +98,27 @@ def c2(table):
     return inp, tight
 
 
-def test_newton_reaches_the_fixed_point(step, c2):
+@pytest.mark.parametrize("sweeps_first", [0, 30])
+def test_newton_reaches_the_fixed_point(step, c2, sweeps_first):
     inp, tight = c2
-    out = solve_gpu(step, inp, "newton")
+    out = solve_gpu(step, inp, "newton", sweeps_first=sweeps_first)
     it = out["iters"]
-    nw, sw = _summary("c2 24x13 from the incoming state", it)
+    nw, sw = _summary(f"c2 24x13 from the incoming state, {sweeps_first} sweeps first", it)
     both = (it > 0) & (tight["iters"] > 0)
     d = box_rel(out["qj1"], out["hc"], tight["qj1_chem"], tight["hc_chem"])
     print(f"[newton] vs the restatement at 1e-14: max {d[both].max():.2e}, "
           f"{int((d[both] > PARITY_TOL).sum())} boxes above {PARITY_TOL:g}; Newton boxes max "
           f"{d[nw & both].max():.2e}", flush=True)
-    assert nw.mean() > 0.95
+    assert (it < 0).mean() < 1e-3 and sw.mean() < 1e-2
     assert d[nw & both].max() < PARITY_TOL
     assert d[both].max() < PARITY_TOL
     assert np.isfinite(out["qj1"]).all() and np.isfinite(out["hc"]).all()
 
 
-def test_newton_agrees_with_the_plain_converged_solve(step, c2):
+@pytest.mark.parametrize("sweeps_first", [0, 30])
+def test_newton_agrees_with_the_plain_converged_solve(step, c2, sweeps_first):
     inp, _ = c2
-    new = solve_gpu(step, inp, "newton")
+    new = solve_gpu(step, inp, "newton", sweeps_first=sweeps_first)
     plain = solve_gpu(step, inp, "plain")
     both = (new["iters"] > 0) & (plain["iters"] > 0)
     d = box_rel(new["qj1"], new["hc"], plain["qj1"], plain["hc"])
@@ -130,10 +141,10 @@ def test_newton_from_a_guess(step, c2):
     nw, _ = _summary("c2 24x13 from a guess 5 % off", out["iters"])
     both = (out["iters"] > 0) & (tight["iters"] > 0)
     d = box_rel(out["qj1"], out["hc"], tight["qj1_chem"], tight["hc_chem"])
-    bn = (base["iters"] > 0) & (base["iters"] < 1000) & nw
+    bn = (base["iters"] >= SOLVED) & (base["iters"] < RESUMED) & nw
     print(f"[newton] from the guess: vs the restatement max {d[both].max():.2e}; mean Newton "
-          f"iterations {base['iters'][bn].mean():.2f} -> {out['iters'][bn].mean():.2f}",
-          flush=True)
+          f"iterations {base['iters'][bn].mean() - SOLVED:.2f} -> "
+          f"{out['iters'][bn].mean() - SOLVED:.2f}", flush=True)
     assert d[both].max() < PARITY_TOL
     assert out["iters"][bn].mean() < base["iters"][bn].mean()
 
@@ -159,6 +170,7 @@ def test_newton_rejects_bad_arguments(step):
     from reprobus_amd._lib import RbError
 
     inp = synth.make_c1(seed=3, nlon=4, nlat=3)
-    for kw in ({"tol": 0.0}, {"max_newton": 0}, {"maxiter": 0}):
+    for kw in ({"tol": 0.0}, {"max_newton": 0}, {"maxiter": 0}, {"sweeps_first": -1},
+               {"sweeps_first": 2000}):
         with pytest.raises(RbError):
             solve_gpu(step, inp, "newton", **kw)
