@@ -128,7 +128,10 @@ bool any_null(std::initializer_list<const void *> ps)
 // whole qj1 and hc fields), report the count in rb_stats and return RB_ENONFINITE when
 // there is any.  It synchronises the stream, so it is skipped inside a graph capture.
 // Newton solve: plain sweeps a box that Newton hands over still gets
-constexpr int32_t NEWTON_RESUME_SWEEPS = 300;
+#ifndef RB_NEWTON_RESUME_SWEEPS
+#define RB_NEWTON_RESUME_SWEEPS 100
+#endif
+constexpr int32_t NEWTON_RESUME_SWEEPS = RB_NEWTON_RESUME_SWEEPS;
 
 int finite_check(rb_ctx *ctx, const double *qj1, const double *hc, int64_t field)
 {
