@@ -198,12 +198,13 @@ def _timed(fn, reps, stream):
     return sorted(ms)[len(ms) // 2]
 
 
-def converged_section(table, block_conv, refill, dev, tol=1e-11, maxiter=2000):
+def converged_section(table, block_conv, refill, dev, newton_shape=(60, 1), tol=1e-11,
+                      maxiter=2000, max_newton=20):
     """Converged mode on the c2 grid (seed 1, the guess network's evaluation seed): plain
-    sweeps and Anderson-accelerated sweeps, each from the incoming state and from the
-    learned guess (the guess launch inside the timed solve); agreement with the plain
-    solve over boxes that converged on both paths.  tol 1e-11 on the relative change per
-    sweep keeps the solves within 1e-9 of each other on all but ~1e-4 of the boxes."""
+    sweeps, Anderson-accelerated sweeps and the Newton solve (generated sparse LU, after
+    newton_shape[0] plain sweeps, newton_shape[1] waves per SIMD), each from the incoming
+    state and from the learned guess (the guess launch inside the timed solve); agreement
+    with the plain solve over boxes that converged on both paths."""
     import torch
 
     from reprobus_amd import guess, synth
@@ -214,7 +215,8 @@ def converged_section(table, block_conv, refill, dev, tol=1e-11, maxiter=2000):
     g = inp.grid
     grid = make_grid(g.nlon, g.nlat, g.niv, g.nivbas, g.nivhau)
     st = _device_state(inp, dev)
-    step = ChemistryStep(*table, device=0, block=block_conv, conv_refill=refill)
+    step = ChemistryStep(*table, device=0, block=block_conv, conv_refill=refill,
+                         newton_waves=newton_shape[1])
     net = GuessMLP(0)
     if not guess.load(net):
         return {"error": "no trained guess weights (reprobus_amd/data/guess_mlp.npz)"}
@@ -234,25 +236,43 @@ def converged_section(table, block_conv, refill, dev, tol=1e-11, maxiter=2000):
     guess_fn = lambda: net.guess(grid, inp.dt, st["T"], st["p"], st["sza"], st["qj1"], st["hc"], gq, gh)
     guess_ms = _timed(guess_fn, 5, stream)
     res, ref = {}, None
-    # (name, Anderson depth, learned guess)
+    # (name, solver: Anderson depth or "newton", learned guess)
     for name, accel, use_guess in (("plain", 0, False), ("mlp_guess", 0, True),
-                                   ("accelerated", 2, False), ("mlp_guess_accelerated", 2, True)):
+                                   ("accelerated", 2, False), ("mlp_guess_accelerated", 2, True),
+                                   ("newton", "newton", False), ("mlp_guess_newton", "newton", True)):
         def solve():
             st["qj1"].copy_(q0)
             st["hc"].copy_(h0)
             if use_guess:
                 guess_fn()
-            step.converge_device(grid, inp.dt, tol, maxiter, *args, st["qj1"], st["hc"], *aux,
-                                 guess_qj1=gq if use_guess else None,
-                                 guess_hc=gh if use_guess else None, iters=it, accel=accel)
+            g_ = {"guess_qj1": gq if use_guess else None, "guess_hc": gh if use_guess else None}
+            if accel == "newton":
+                step.newton_device(grid, inp.dt, tol, max_newton, maxiter, *args, st["qj1"],
+                                   st["hc"], *aux, iters=it, sweeps_first=newton_shape[0], **g_)
+            else:
+                step.converge_device(grid, inp.dt, tol, maxiter, *args, st["qj1"], st["hc"], *aux,
+                                     iters=it, accel=accel, **g_)
 
         solve()
         torch.cuda.synchronize()
         ms = _timed(solve, 3, stream)
         r = {"ms": ms, "box_steps_per_s": nbox / (ms / 1e3),
-             "sweeps_mean": float(it.abs().float().mean()), "sweeps_max": int(it.abs().max()),
-             "fallback_boxes": int((it < 0).sum()),
-             "sweep_histogram": _histogram(it.abs())}
+             "fallback_boxes": int((it < 0).sum())}
+        if accel == "newton":
+            sw = (it > 0) & (it < 10000)
+            nw = (it >= 10000) & (it < 20000)
+            r.update({"boxes_converged_by_first_sweeps": int(sw.sum()),
+                      "boxes_solved_by_newton": int(nw.sum()),
+                      "newton_iterations_mean": float((it[nw] - 10000).float().mean()),
+                      "newton_iterations_max": int((it[nw] - 10000).max()),
+                      "newton_iteration_histogram": {
+                          str(k): int(v) for k, v in
+                          enumerate(torch.bincount((it[nw] - 10000).reshape(-1).long()).tolist())
+                          if v},
+                      "boxes_handed_back_to_sweeps": int((it >= 20000).sum())})
+        else:
+            r.update({"sweeps_mean": float(it.abs().float().mean()),
+                      "sweeps_max": int(it.abs().max()), "sweep_histogram": _histogram(it.abs())})
         if ref is None:
             ref = (st["qj1"].clone(), st["hc"].clone(), it.clone())
         else:
@@ -273,23 +293,38 @@ def converged_section(table, block_conv, refill, dev, tol=1e-11, maxiter=2000):
     q0.copy_(st["qj1"])
     h0.copy_(st["hc"])
     nxt = {}
-    for name, accel in (("plain", 0), ("accelerated", 2)):
+    for name, accel in (("plain", 0), ("accelerated", 2), ("newton", "newton")):
         def solve():
             st["qj1"].copy_(q0)
             st["hc"].copy_(h0)
-            step.converge_device(grid, inp.dt, tol, maxiter, *args, st["qj1"], st["hc"], *aux,
-                                 iters=it, accel=accel)
+            if accel == "newton":
+                step.newton_device(grid, inp.dt, tol, max_newton, maxiter, *args, st["qj1"],
+                                   st["hc"], *aux, iters=it, sweeps_first=newton_shape[0])
+            else:
+                step.converge_device(grid, inp.dt, tol, maxiter, *args, st["qj1"], st["hc"],
+                                     *aux, iters=it, accel=accel)
 
         ms = _timed(solve, 3, stream)
         nxt[name] = {"ms": ms, "box_steps_per_s": nbox / (ms / 1e3),
-                     "sweeps_mean": float(it.abs().float().mean()),
-                     "sweeps_max": int(it.abs().max()), "fallback_boxes": int((it < 0).sum())}
+                     "fallback_boxes": int((it < 0).sum())}
+        if accel != "newton":
+            nxt[name].update({"sweeps_mean": float(it.abs().float().mean()),
+                              "sweeps_max": int(it.abs().max())})
     nxt["speedup_accelerated_over_plain"] = nxt["plain"]["ms"] / nxt["accelerated"]["ms"]
+    nxt["speedup_newton_over_plain"] = nxt["plain"]["ms"] / nxt["newton"]["ms"]
     out = {"workload": "c2 seed 1 (786,240 boxes, polar vortex), incoming state, dt 900 s",
            "tol": tol, "maxiter": maxiter, "block": block_conv, "refill": refill,
            "anderson": {"depth": 2, "from_sweep": step.conv_aa_start,
-                        "budget": step.conv_aa_budget}, "guess_ms": guess_ms, **res,
+                        "budget": step.conv_aa_budget},
+           "newton": {"sweeps_first": newton_shape[0], "waves_per_simd": newton_shape[1],
+                      "max_newton": max_newton, "unknowns": 55,
+                      "jacobian_nonzeros_day_night": [399, 278],
+                      "lu_nonzeros_day_night": [514, 365]},
+           "guess_ms": guess_ms, **res,
            "speedup_mlp_over_plain": res["plain"]["ms"] / res["mlp_guess"]["ms"],
+           "speedup_mlp_newton_over_plain": res["plain"]["ms"] / res["mlp_guess_newton"]["ms"],
+           "speedup_newton_over_plain": res["plain"]["ms"] / res["newton"]["ms"],
+           "mlp_gain_in_newton": res["newton"]["ms"] / res["mlp_guess_newton"]["ms"],
            "next_step": nxt}
     step.close()
     net.close()
@@ -376,7 +411,8 @@ def surrogate_section(table, block_conv, refill, dev, nsteps=96, accept_tol=1e-8
         return {"error": "no trained guess weights (reprobus_amd/data/guess_mlp.npz)"}
     sp = _device_state(inp, dev)                    # surrogate path
     fp = _device_state(inp, dev)                    # FP64 path
-    step = ChemistryStep(*table, device=0, block=block_conv, conv_refill=refill)
+    step = ChemistryStep(*table, device=0, block=block_conv, conv_refill=refill,
+                         newton_waves=newton_shape[1])
     stream = torch.cuda.current_stream()
     step.set_stream(stream.cuda_stream)
     net.set_stream(stream.cuda_stream)
@@ -609,19 +645,24 @@ def main():
     extras = {}
     want = set() if args.no_extras else set(s for s in args.extras.split(",") if s)
     conv_shape = None
+    newton_shape = (60, 1)
     if want & {"converged", "surrogate"}:
         try:
             cache = os.path.join(tempfile.gettempdir(), "reprobus_amd_tune.json")
             c2 = _device_state(synth.make_c2(1), dev)
             r = autotune.tune(table, grid, c2, mode="converged", reps=2, cache_path=cache)
             tuned["converged"] = {"block": r.block, "refill": r.refill, "ms": r.ms}
-            del c2
             conv_shape = (r.block, r.refill)
+            rn = autotune.tune_newton(table, grid, c2, block=r.block, refill=r.refill, reps=2,
+                                      cache_path=cache)
+            tuned["newton"] = {"sweeps_first": rn.sweeps_first, "waves": rn.waves, "ms": rn.ms}
+            newton_shape = (rn.sweeps_first, rn.waves)
+            del c2
         except Exception as exc:  # reported beside the headline, never instead of it
             extras["converged"] = {"error": repr(exc)}
     if "converged" in want and conv_shape:
         try:
-            extras["converged"] = converged_section(table, *conv_shape, dev)
+            extras["converged"] = converged_section(table, *conv_shape, dev, newton_shape)
         except Exception as exc:
             extras["converged"] = {"error": repr(exc)}
     if "surrogate" in want and conv_shape:

@@ -12,7 +12,10 @@ less often).  The converged mode is tuned on the accelerated solve.
 
 `tune` times each candidate on the caller's own device state (restored between trials,
 so the state is unchanged on return), keeps the fastest, and caches the choice in a JSON
-file keyed by device name, grid and mode.  Every candidate computes the same numbers:
+file keyed by device name, grid and mode.  `tune_newton` chooses the Newton solve's launch
+shape the same way: how many plain sweeps run before Newton takes the unconverged boxes,
+and the Newton kernel's resident waves per SIMD (its occupancy against the register file
+its factors need).  Every candidate computes the same numbers:
 the launch shape and the chunk change no result (checked by the tests).
 """
 from __future__ import annotations
@@ -24,6 +27,10 @@ from dataclasses import dataclass, field
 
 BLOCKS = (64, 128, 256)
 REFILLS = (8, 16, 32)
+# Newton solve: plain sweeps before Newton takes the still unconverged boxes, and resident
+# waves per SIMD of the Newton kernel (the occupancy it runs at)
+SWEEPS_FIRST = (40, 60, 80, 100, 130, 160)
+NEWTON_WAVES = (1, 2)
 
 
 @dataclass
@@ -33,6 +40,8 @@ class TuneResult:
     refill: int | None = None                    # converged mode: the queue chunk
     ms: dict = field(default_factory=dict)       # "block" or "block/refill" -> median ms
     cached: bool = False
+    sweeps_first: int | None = None              # Newton solve: sweeps before Newton
+    waves: int | None = None                     # Newton solve: waves per SIMD
 
 
 def _key(device_name: str, grid, mode: str, accel: int | None = None,
@@ -143,3 +152,63 @@ def tune(table, grid, state: dict, mode: str = "plain", candidates=BLOCKS, reps:
                   "when": time.strftime("%Y-%m-%dT%H:%M:%S")}
     save_cache(cache_path, cache)
     return TuneResult(mode, block, refill, labels)
+
+
+def tune_newton(table, grid, state: dict, block: int = 64, refill: int = 32, reps: int = 3,
+                cache_path: str | None = None, device: int = 0, tol: float = 1e-11,
+                maxiter: int = 2000, max_newton: int = 20, sweeps=SWEEPS_FIRST,
+                waves=NEWTON_WAVES, timer=None) -> TuneResult:
+    """Pick the Newton solve's sweeps before Newton and waves per SIMD on `state` (the
+    device tensors of tune()), timing every pair (restoring the state between trials), and
+    cache the choice keyed by device, grid and tolerance.  Every pair reaches the same
+    fixed point (the tests check the answers against the restatement).  `timer(sweeps,
+    waves)` -> ms replaces the GPU timing (tests)."""
+    import torch
+
+    name = torch.cuda.get_device_name(device) if timer is None else "test-device"
+    cache = load_cache(cache_path)
+    key = _key(name, grid, "newton") + f"|tol={float(tol):g}|block={block}"
+    c = cache.get(key)
+    if c and c.get("sweeps_first") in sweeps and c.get("waves") in waves:
+        return TuneResult("newton", block, refill, c["ms"], cached=True,
+                          sweeps_first=c["sweeps_first"], waves=c["waves"])
+
+    def measure(sf: int, w: int) -> float:
+        from .chem import ChemistryStep
+
+        step = ChemistryStep(*table, device=device, block=block, conv_refill=refill,
+                             newton_waves=w)
+        try:
+            s = torch.cuda.current_stream()
+            step.set_stream(s.cuda_stream)
+            q0, h0 = state["qj1"].clone(), state["hc"].clone()
+            args = [state[k] for k in ("T", "p", "sza", "alt")]
+            outs = [state[k] for k in ("surfarea", "vsed3d", "parth2osed", "parthno3sed")]
+            times = []
+            for r in range(reps + 1):                   # first run: warm-up
+                state["qj1"].copy_(q0)
+                state["hc"].copy_(h0)
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record(s)
+                step.newton_device(grid, state["dt"], tol, max_newton, maxiter, *args,
+                                   state["qj1"], state["hc"], *outs, sweeps_first=sf)
+                e1.record(s)
+                torch.cuda.synchronize()
+                if r:
+                    times.append(e0.elapsed_time(e1))
+            state["qj1"].copy_(q0)
+            state["hc"].copy_(h0)
+            torch.cuda.synchronize()
+            return sorted(times)[len(times) // 2]
+        finally:
+            step.close()
+
+    timer = timer or measure
+    ms = {(int(a), int(b)): float(timer(int(a), int(b))) for a in sweeps for b in waves}
+    # ties: fewer sweeps first (more boxes to Newton), then fewer waves
+    sf, w = min(ms, key=lambda k: (ms[k], k[0], k[1]))
+    labels = {_label(k): v for k, v in ms.items()}
+    cache[key] = {"sweeps_first": sf, "waves": w, "ms": labels,
+                  "when": time.strftime("%Y-%m-%dT%H:%M:%S")}
+    save_cache(cache_path, cache)
+    return TuneResult("newton", block, refill, labels, sweeps_first=sf, waves=w)

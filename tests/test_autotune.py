@@ -43,3 +43,23 @@ def test_ties_go_to_the_larger_block_then_chunk():
     assert autotune.choose({64: 1.0, 128: 1.0, 256: 1.2}) == 128
     assert autotune.choose({64: 1.0, 128: 1.0, 256: 1.0}) == 256
     assert autotune.choose({(128, 8): 1.0, (128, 32): 1.0, (64, 32): 1.0}) == (128, 32)
+
+
+def test_newton_choice_and_cache(tmp_path):
+    grid = make_grid(144, 91, 60, 60, 14)
+    path = str(tmp_path / "tune.json")
+    calls = []
+
+    def timer(sf, w):
+        calls.append((sf, w))
+        return {40: 20.0, 60: 16.0, 80: 16.0, 100: 17.0, 130: 18.0, 160: 19.0}[sf] + \
+            (0.5 if w == 2 else 0.0)
+
+    r = autotune.tune_newton(None, grid, {}, cache_path=path, timer=timer)
+    assert (r.sweeps_first, r.waves) == (60, 1) and not r.cached   # tie 60/80 -> fewer sweeps
+    assert len(calls) == len(autotune.SWEEPS_FIRST) * len(autotune.NEWTON_WAVES)
+    r2 = autotune.tune_newton(None, grid, {}, cache_path=path, timer=timer)
+    assert r2.cached and (r2.sweeps_first, r2.waves) == (60, 1)
+    assert not autotune.tune_newton(None, grid, {}, cache_path=path, timer=timer, tol=1e-10).cached
+    assert not autotune.tune_newton(None, grid, {}, cache_path=path, timer=timer,
+                                    sweeps=(40, 80)).cached
