@@ -105,6 +105,21 @@ int rb_chem_step_dev(rb_ctx *ctx, const rb_grid *g, double dt, const double *T, 
                      int fuse_conserv);
 int rb_conserv_floor_dev(rb_ctx *ctx, const rb_grid *g, double *qj1, double *hc);
 
+/* The device-resident step (rb_chem_step_dev's contract and results) without any device
+ * allocation -- no counterpart in the reference: for callers that cannot allocate (code
+ * under HIP graph capture, serving loops, harnesses that forbid hipMalloc).  Its work
+ * buffers (the column pre-pass and the set-up record, 144 doubles per chemistry box) live in
+ * a fixed device array that every GPU the library runs on holds until the process ends
+ * (37.7 MB), so a grid may hold at most RB_NOALLOC_MAX_BOXES chemistry boxes
+ * (nlon*nlat*nivbas; larger: RB_EINVAL, nothing launched).  Every call on one GPU shares
+ * that array: calls that overlap on one GPU (from several contexts or streams) race.  The
+ * per-launch timing of rb_config.profile is not taken (creating its events allocates). */
+#define RB_NOALLOC_MAX_BOXES 32768
+int rb_chem_step_noalloc_dev(rb_ctx *ctx, const rb_grid *g, double dt, const double *T,
+                             const double *p, const double *sza, const double *alt, double *qj1,
+                             double *hc, double *surfarea, double *vsed3d, double *parth2osed,
+                             double *parthno3sed, int fuse_conserv);
+
 /* Input producers (replace zenith(trajlon, trajlat, irapp, ilat, daynum, sza), F:5308-5406,
  * called from chem3d at F:845, and altitude(nlon, nlat, niv, relief, psol, aaa, bbb, t,
  * h2o, trajlon, trajlat, trajpre, alt), src/altitude.f90:1-148, called at F:391), so the

@@ -13,12 +13,14 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "librbchem.so")
 
 RB_OK, RB_EINVAL, RB_EHIP, RB_ENOTABLE, RB_ENONFINITE, RB_ENOMEM, RB_EIO = 0, -1, -2, -3, -4, -5, -6
+RB_NOALLOC_MAX_BOXES = 32768     # include/rbchem.h
 ERROR_NAMES = {RB_EINVAL: "RB_EINVAL", RB_EHIP: "RB_EHIP", RB_ENOTABLE: "RB_ENOTABLE",
                RB_ENONFINITE: "RB_ENONFINITE", RB_ENOMEM: "RB_ENOMEM", RB_EIO: "RB_EIO"}
 
 # every symbol include/rbchem.h declares
 EXPORTS = ["rb_init", "rb_load_jtable", "rb_set_stream", "rb_chem_step", "rb_conserv_floor",
-           "rb_chem_step_dev", "rb_conserv_floor_dev", "rb_chem_converge_dev",
+           "rb_chem_step_dev", "rb_chem_step_noalloc_dev", "rb_conserv_floor_dev",
+           "rb_chem_converge_dev",
            "rb_chem_newton_dev", "rb_chem_surrogate_dev", "rb_sediment",
            "rb_sediment_dev", "rb_zenith", "rb_zenith_dev", "rb_altitude", "rb_altitude_dev",
            "rb_run_day_dev", "rb_transport_weights_dev", "rb_transport_interp_dev", "rb_transport",
@@ -92,6 +94,7 @@ def load() -> C.CDLL:
     L.rb_chem_step.argtypes = [P, C.POINTER(Grid), d] + [P] * 10
     L.rb_conserv_floor.argtypes = [P, C.POINTER(Grid), P, P]
     L.rb_chem_step_dev.argtypes = [P, C.POINTER(Grid), d] + [P] * 10 + [C.c_int]
+    L.rb_chem_step_noalloc_dev.argtypes = [P, C.POINTER(Grid), d] + [P] * 10 + [C.c_int]
     L.rb_conserv_floor_dev.argtypes = [P, C.POINTER(Grid), P, P]
     L.rb_chem_converge_dev.argtypes = [P, C.POINTER(Grid), d, d, C.c_int32, C.c_int32] + [P] * 13 + [C.c_int]
     L.rb_chem_newton_dev.argtypes = [P, C.POINTER(Grid), d, d, C.c_int32, C.c_int32, C.c_int32] + \

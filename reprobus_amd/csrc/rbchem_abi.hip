@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <utility>
@@ -41,6 +42,7 @@ struct rb_ctx {
     double *d_tq = nullptr;                  // transport (host entry): fields in and out
     size_t tq_cap = 0;
     int n_cu = 0;
+    double *noalloc = nullptr;               // this device's copy of `scratch` (rb_init)
     bool capturing = false;                  // inside a graph capture: no event records
     hipGraph_t day_graph = nullptr;          // the captured day and what it was built for
     hipGraphExec_t day_exec = nullptr;
@@ -147,6 +149,99 @@ int finite_check(rb_ctx *ctx, const double *qj1, const double *hc, int64_t field
     return RB_OK;
 }
 
+// The allocation-free step (rb_chem_step_noalloc_dev) keeps its work buffers -- the column
+// pre-pass (3 doubles per box) and the set-up record (CONV_RECORD doubles per box) of at
+// most RB_NOALLOC_MAX_BOXES boxes -- in this array, which every GPU the library runs on
+// holds until the process ends.  Host code never passes `scratch` itself to a kernel (on the
+// host the name is a stand-in, not a device address): rb_init looks up each device's copy
+// once with hipGetSymbolAddress and the entry point passes that address.
+constexpr size_t NOALLOC_DOUBLES = (size_t)(3 + rb::CONV_RECORD) * RB_NOALLOC_MAX_BOXES;
+static __device__ __align__(256) double scratch[NOALLOC_DOUBLES];
+
+std::mutex g_noalloc_mu;
+std::vector<double *> g_noalloc;             // per device: its copy of scratch, or null
+
+// the address of `device`'s copy of scratch, looked up once per device; the device must be
+// current
+hipError_t noalloc_scratch(int device, double **out)
+{
+    std::lock_guard<std::mutex> lock(g_noalloc_mu);
+    if ((size_t)device >= g_noalloc.size()) g_noalloc.resize((size_t)device + 1, nullptr);
+    if (!g_noalloc[(size_t)device]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) {
+            (void)hipGetLastError();
+            return e != hipSuccess ? e : hipErrorInvalidSymbol;
+        }
+        g_noalloc[(size_t)device] = (double *)ptr;
+    }
+    *out = g_noalloc[(size_t)device];
+    return hipSuccess;
+}
+
+// The launches of one chemistry step (column pre-pass, set-up, sweeps, conserv below the
+// chemistry levels, the non-finite check) on caller-supplied work buffers: col (3 nbox
+// doubles) and rec (CONV_RECORD nbox doubles).  profile = false skips the per-launch events
+// (their creation allocates).
+int step_launch(rb_ctx *ctx, const rb_grid *g, double dt, const double *T, const double *p,
+                const double *sza, const double *alt, double *qj1, double *hc, double *surfarea,
+                double *vsed3d, double *parth2osed, double *parthno3sed, int fuse_conserv,
+                double *col, double *rec, bool profile)
+{
+    const int64_t plane = (int64_t)g->nlon * g->nlat;
+    const int64_t field = plane * g->niv;
+    const int64_t nbox = plane * g->nivbas;
+    rb::StepArgs a{};
+    a.T = T; a.p = p; a.sza = sza; a.alt = alt; a.col = col;
+    a.qj1 = qj1; a.hc = hc; a.surfarea = surfarea;
+    a.vsed = vsed3d; a.p2o = parth2osed; a.pno3 = parthno3sed;
+    a.ajl = ctx->d_ajl; a.o3up = ctx->d_o3up;
+    a.plane = plane; a.field = field; a.nbox = nbox;
+    a.nivbas = g->nivbas; a.nivhau = g->nivhau; a.niter = ctx->cfg.niter;
+    a.fuse_conserv = fuse_conserv ? 1 : 0;
+    a.dt = dt;
+    a.K = ctx->K;
+
+    if (!ctx->capturing) HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
+    HIPCHK(rb::launch_columns(a, col, ctx->stream));
+    if (profile && ctx->cfg.profile && !ctx->capturing) {
+        if (ctx->prof_used == ctx->prof.size()) {
+            hipEvent_t e0, e1;
+            HIPCHK(hipEventCreate(&e0));
+            HIPCHK(hipEventCreate(&e1));
+            ctx->prof.emplace_back(e0, e1);
+        }
+        auto &pe = ctx->prof[ctx->prof_used++];
+        HIPCHK(hipEventRecord(pe.first, ctx->stream));
+        HIPCHK(rb::launch_setup(a, rec, ctx->cfg.block, ctx->stream));
+        HIPCHK(rb::launch_chem(a, rec, ctx->cfg.block, ctx->stream));
+        HIPCHK(hipEventRecord(pe.second, ctx->stream));
+    } else {
+        HIPCHK(rb::launch_setup(a, rec, ctx->cfg.block, ctx->stream));
+        HIPCHK(rb::launch_chem(a, rec, ctx->cfg.block, ctx->stream));
+    }
+    if (fuse_conserv && field > nbox)   // floor of the levels below the chemistry
+        HIPCHK(rb::launch_conserv_floor(qj1, hc, plane, field, g->nivbas, nbox, ctx->stream));
+    if (!ctx->capturing) {
+        HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
+        ctx->timed = true;
+    }
+    ctx->stats.boxes = nbox;
+    return finite_check(ctx, qj1, hc, field);
+}
+
+int step_checks(rb_ctx *ctx, const rb_grid *g, double dt,
+                std::initializer_list<const void *> arrays)
+{
+    int rc = check_grid(ctx, g);
+    if (rc) return rc;
+    if (!ctx->table) return fail(ctx, RB_ENOTABLE, "rb_load_jtable has not been called");
+    if (any_null(arrays)) return fail(ctx, RB_EINVAL, "a device array is NULL");
+    if (!(dt > 0.) || !std::isfinite(dt)) return fail(ctx, RB_EINVAL, "dt must be positive");
+    return RB_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -192,6 +287,11 @@ int rb_init(rb_ctx **out, int device, const rb_config *cfg)
         return RB_EHIP;
     }
     ctx->stream = ctx->own;
+    // this device's copy of the allocation-free step's scratch (the device is current)
+    if (noalloc_scratch(device, &ctx->noalloc) != hipSuccess) {
+        rb_finalize(ctx);
+        return RB_EHIP;
+    }
     *out = ctx;
     return RB_OK;
 }
@@ -225,56 +325,37 @@ int rb_chem_step_dev(rb_ctx *ctx, const rb_grid *g, double dt, const double *T, 
                      int fuse_conserv)
 {
     if (!ctx) return RB_EINVAL;
-    int rc = check_grid(ctx, g);
+    int rc = step_checks(ctx, g, dt, {T, p, sza, alt, qj1, hc, surfarea, vsed3d, parth2osed,
+                                      parthno3sed});
     if (rc) return rc;
-    if (!ctx->table) return fail(ctx, RB_ENOTABLE, "rb_load_jtable has not been called");
-    if (any_null({T, p, sza, alt, qj1, hc, surfarea, vsed3d, parth2osed, parthno3sed}))
-        return fail(ctx, RB_EINVAL, "a device array is NULL");
-    if (!(dt > 0.) || !std::isfinite(dt)) return fail(ctx, RB_EINVAL, "dt must be positive");
     HIPCHK(hipSetDevice(ctx->device));
-    const int64_t plane = (int64_t)g->nlon * g->nlat;
-    const int64_t field = plane * g->niv;
-    const int64_t nbox = plane * g->nivbas;
+    const int64_t nbox = (int64_t)g->nlon * g->nlat * g->nivbas;
     if ((rc = ensure(ctx, &ctx->d_col, &ctx->col_cap, (size_t)(3 * nbox)))) return rc;
     if ((rc = conv_buffers(ctx, (size_t)nbox, 0))) return rc;        // the set-up record
+    return step_launch(ctx, g, dt, T, p, sza, alt, qj1, hc, surfarea, vsed3d, parth2osed,
+                       parthno3sed, fuse_conserv, ctx->d_col, ctx->d_rec, true);
+}
 
-    rb::StepArgs a{};
-    a.T = T; a.p = p; a.sza = sza; a.alt = alt; a.col = ctx->d_col;
-    a.qj1 = qj1; a.hc = hc; a.surfarea = surfarea;
-    a.vsed = vsed3d; a.p2o = parth2osed; a.pno3 = parthno3sed;
-    a.ajl = ctx->d_ajl; a.o3up = ctx->d_o3up;
-    a.plane = plane; a.field = field; a.nbox = nbox;
-    a.nivbas = g->nivbas; a.nivhau = g->nivhau; a.niter = ctx->cfg.niter;
-    a.fuse_conserv = fuse_conserv ? 1 : 0;
-    a.dt = dt;
-    a.K = ctx->K;
-
-    if (!ctx->capturing) HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
-    HIPCHK(rb::launch_columns(a, ctx->d_col, ctx->stream));
-    if (ctx->cfg.profile && !ctx->capturing) {
-        if (ctx->prof_used == ctx->prof.size()) {
-            hipEvent_t e0, e1;
-            HIPCHK(hipEventCreate(&e0));
-            HIPCHK(hipEventCreate(&e1));
-            ctx->prof.emplace_back(e0, e1);
-        }
-        auto &pe = ctx->prof[ctx->prof_used++];
-        HIPCHK(hipEventRecord(pe.first, ctx->stream));
-        HIPCHK(rb::launch_setup(a, ctx->d_rec, ctx->cfg.block, ctx->stream));
-        HIPCHK(rb::launch_chem(a, ctx->d_rec, ctx->cfg.block, ctx->stream));
-        HIPCHK(hipEventRecord(pe.second, ctx->stream));
-    } else {
-        HIPCHK(rb::launch_setup(a, ctx->d_rec, ctx->cfg.block, ctx->stream));
-        HIPCHK(rb::launch_chem(a, ctx->d_rec, ctx->cfg.block, ctx->stream));
-    }
-    if (fuse_conserv && field > nbox)   // floor of the levels below the chemistry
-        HIPCHK(rb::launch_conserv_floor(qj1, hc, plane, field, g->nivbas, nbox, ctx->stream));
-    if (!ctx->capturing) {
-        HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
-        ctx->timed = true;
-    }
-    ctx->stats.boxes = nbox;
-    return finite_check(ctx, qj1, hc, field);
+int rb_chem_step_noalloc_dev(rb_ctx *ctx, const rb_grid *g, double dt, const double *T,
+                             const double *p, const double *sza, const double *alt, double *qj1,
+                             double *hc, double *surfarea, double *vsed3d, double *parth2osed,
+                             double *parthno3sed, int fuse_conserv)
+{
+    if (!ctx) return RB_EINVAL;
+    int rc = step_checks(ctx, g, dt, {T, p, sza, alt, qj1, hc, surfarea, vsed3d, parth2osed,
+                                      parthno3sed});
+    if (rc) return rc;
+    const int64_t nbox = (int64_t)g->nlon * g->nlat * g->nivbas;
+    if (nbox > RB_NOALLOC_MAX_BOXES)
+        return fail(ctx, RB_EINVAL, "the grid has " + std::to_string(nbox) +
+                                        " chemistry boxes; the allocation-free step takes at most " +
+                                        std::to_string((long long)RB_NOALLOC_MAX_BOXES));
+    if (!ctx->noalloc) return fail(ctx, RB_EHIP, "the allocation-free scratch is not available");
+    HIPCHK(hipSetDevice(ctx->device));
+    // both buffers are written in full before they are read: nothing to clear
+    double *col = ctx->noalloc, *rec = ctx->noalloc + 3 * nbox;
+    return step_launch(ctx, g, dt, T, p, sza, alt, qj1, hc, surfarea, vsed3d, parth2osed,
+                       parthno3sed, fuse_conserv, col, rec, false);
 }
 
 // the converged mode; accept_tol > 0 makes it the surrogate step (guess = the surrogate)
