@@ -4,9 +4,10 @@ Headline (`value`): the full 144 x 91 x 60 grid (786,240 boxes), one 900 s chemi
 step per timed step -- the reference scheme, 5 fixed sweeps in FP64, conserv + floor
 fused -- state resident in HBM; successive steps integrate the evolving state.
 
-Beside it on the same line: the converged mode (sweeps to the fixed point; plain and
-Anderson-accelerated, each from the incoming state and from the learned initial guess)
-on the 786,240-box c2 grid (polar vortex), with agreement and sweep histograms; the
+Beside it on the same line: the converged mode (sweeps to the fixed point: plain sweeps
+and the Newton solve, each from the incoming state and from the learned initial guess,
+and Anderson-accelerated sweeps from the incoming state) on the 786,240-box c2 grid
+(polar vortex), with agreement and sweep histograms; the
 surrogate step (the learned network's answer kept where its FP64 residual is under 1e-8,
 the FP64 solve elsewhere) over a day of sub-steps beside the FP64 solve; the
 semi-Lagrangian transport interpolation of 43 tracers on the model's 180 x 91 x 137 grid
@@ -238,7 +239,7 @@ def converged_section(table, block_conv, refill, dev, newton_shape=(60, 1), tol=
     res, ref = {}, None
     # (name, solver: Anderson depth or "newton", learned guess)
     for name, accel, use_guess in (("plain", 0, False), ("mlp_guess", 0, True),
-                                   ("accelerated", 2, False), ("mlp_guess_accelerated", 2, True),
+                                   ("accelerated", 2, False),
                                    ("newton", "newton", False), ("mlp_guess_newton", "newton", True)):
         def solve():
             st["qj1"].copy_(q0)

@@ -45,7 +45,12 @@ typedef struct {
 
 typedef struct {
     int32_t niter;          /* fixed sweeps per step; the reference uses 5 (F:630) */
-    int32_t check_finite;   /* 1: count non-finite outputs and return RB_ENONFINITE */
+    int32_t check_finite;   /* 1: after rb_chem_step(_dev), rb_chem_step_noalloc_dev,
+                               rb_chem_converge_dev, rb_chem_newton_dev and
+                               rb_chem_surrogate_dev, count the non-finite values of qj1
+                               and hc (rb_stats.nonfinite; synchronises the stream) and
+                               return RB_ENONFINITE if there are any; skipped while a graph
+                               is captured (rb_run_day_dev with graph = 1) */
     int32_t block;          /* threads per block of the per-box kernel: 64, 128, 256 (0: 256) */
     int32_t profile;        /* 1: time every step's per-box kernels (set-up and sweeps) with
                                HIP events */
@@ -212,19 +217,35 @@ int rb_sediment_dev(rb_ctx *ctx, const rb_grid *g, double dt, int32_t nivh2o, co
                     const double *parth2osed, const double *parthno3sed);
 
 /* Converged mode (no counterpart in the reference, which always stops after niter = 5
- * sweeps, F:630): the same per-box algorithm swept to its fixed point -- until the largest
- * relative change of any long- or short-lived species over a sweep is below tol (values
- * under the 1e-30 mole-fraction floor compared at the floor) or maxiter sweeps.  A
- * persistent kernel takes boxes from a device work queue, so converged boxes free their
- * lanes at once.  guess_qj1 / guess_hc (device, mixing ratios, state layout, or both
- * NULL) give the initial iterate instead of the incoming state -- the learned initial
- * guess; the start-of-step state ck0 is always qj1.  iters (device, nlon*nlat*nivbas
- * int32, or NULL) receives the sweep count of each box.  accel = 2 accelerates the
- * iteration (Anderson mixing of depth 2 over the last three sweeps, Walker & Ni 2011,
- * from sweep rb_config.conv_aa_start on): the same fixed point and stopping test, fewer
- * sweeps; 0 sweeps plainly.  A box that
- * goes non-finite or does not converge in maxiter sweeps gets the fixed-sweep answer
- * (niter sweeps from the incoming state) and a negative count. */
+ * sweeps, F:630): the same per-box algorithm swept to its fixed point.  d_k is the largest
+ * relative change of any long- or short-lived species over sweep k (values under the 1e-30
+ * mole-fraction floor compared at the floor).  A box stops when every species changed by
+ * less than tol and either
+ *   - after at least 3 unmixed sweeps, the estimated distance to the fixed point is below
+ *     tol: d_k rho / (1 - rho) < tol, rho = min(max(d_k / d_(k-1), d_(k-1) / d_(k-2)), 0.999);
+ *   - or 8 sweeps in a row changed by less than tol and the change rose during them (the
+ *     rounding-noise floor, or a 2-cycle across a min/max kink of the map);
+ * or after maxiter sweeps.  A persistent kernel takes boxes from a device work queue, so
+ * converged boxes free their lanes at once.
+ *   - guess_qj1 / guess_hc (device, mixing ratios, state layout, or both NULL) give the
+ *     initial iterate instead of the incoming state -- the learned initial guess; the
+ *     start-of-step state ck0 is always qj1.  A guess that puts a species whose fixed point
+ *     lies above the floor far under it can stop a box short of the fixed point, because
+ *     changes under the floor are not seen; on the c2 grid the learned guess does not (its
+ *     answer and the incoming state's differ by more than 1e-9 at 1 box of 786,240, by
+ *     6.8e-9).
+ *   - iters (device, nlon*nlat*nivbas int32, or NULL) receives the sweep count of each box.
+ *   - accel = 2 accelerates the iteration (Anderson mixing of depth 2 over the last three
+ *     sweeps, Walker & Ni 2011, from sweep rb_config.conv_aa_start on; a box whose change
+ *     is below tol sweeps unmixed): the same fixed point and stopping test, fewer sweeps;
+ *     0 sweeps plainly.  accel with a guess returns RB_EINVAL: from the learned guess the
+ *     mixing carries night NO2 of ~150 c2 boxes far under the floor, as above.
+ * At a few dozen boxes of 786,240 the fixed point is not defined to 1e-9: the map has
+ * branches that switch within rounding of it, and its answer moves by up to 1.6e-8 when T
+ * or p moves by one part in 1e15.  Different starts or mixing may end on either side there.
+ * A box that goes non-finite or does not converge in maxiter sweeps (in min(maxiter,
+ * conv_aa_start + conv_aa_budget) with accel) gets the fixed-sweep answer (niter sweeps
+ * from the incoming state) and a negative count. */
 int rb_chem_converge_dev(rb_ctx *ctx, const rb_grid *g, double dt, double tol, int32_t maxiter,
                          int32_t accel, const double *T, const double *p, const double *sza,
                          const double *alt, double *qj1, double *hc, double *surfarea,
@@ -265,7 +286,8 @@ int rb_chem_newton_dev(rb_ctx *ctx, const rb_grid *g, double dt, double tol,
  * relative (the converged mode's test), the surrogate's answer is kept and iters[box] = 0;
  * every other box goes on sweeping to tol from there, exactly as rb_chem_converge_dev with
  * that guess (the plain FP64 solver is the fallback and decides what is kept).
- * accept_tol must be positive; the other arguments are rb_chem_converge_dev's. */
+ * accept_tol must be positive; accel must be 0 (the surrogate's answer is a guess); the
+ * other arguments are rb_chem_converge_dev's. */
 int rb_chem_surrogate_dev(rb_ctx *ctx, const rb_grid *g, double dt, double accept_tol,
                           double tol, int32_t maxiter, int32_t accel, const double *T,
                           const double *p, const double *sza, const double *alt, double *qj1,

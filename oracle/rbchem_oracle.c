@@ -1474,6 +1474,7 @@ int rbo_chem_converge_aa(const rbo_grid *g, double dt, double tol, int accel, in
                          const double *guess_hc, int nthreads)
 {
     if ((guess_qj1 == NULL) != (guess_hc == NULL) || accel < 0 || accel > AA_MAX) return -1;
+    if (accel > 0 && guess_qj1) return -2;      /* refused, as the library refuses it */
     const rbo_grid c = capped(g, accel, aa_start, aa_budget);
     return chem_grid(&c, dt, T, p, sza, alt, ajl, o3up, qj1, hc, surfarea, vsed3d, parth2osed,
                      parthno3sed, nthreads, tol, iters, guess_qj1, guess_hc, accel, aa_start, 0.);
@@ -1488,6 +1489,7 @@ int rbo_chem_surrogate(const rbo_grid *g, double dt, double accept_tol, double t
                        const double *sur_hc, int nthreads)
 {
     if (!sur_qj1 || !sur_hc || !(accept_tol > 0.) || accel < 0 || accel > AA_MAX) return -1;
+    if (accel > 0) return -2;                   /* refused, as the library refuses it */
     const rbo_grid c = capped(g, accel, aa_start, aa_budget);
     return chem_grid(&c, dt, T, p, sza, alt, ajl, o3up, qj1, hc, surfarea, vsed3d, parth2osed,
                      parthno3sed, nthreads, tol, iters, sur_qj1, sur_hc, accel, aa_start,

@@ -48,7 +48,10 @@ int rbo_chem_converge(const rbo_grid *g, double dt, double tol, const double *T,
 /* Converged mode with Anderson acceleration of depth accel (0 = plain sweeps, <= 4) on the
  * sweep map from sweep aa_start on (plain sweeps before it): the same fixed point and
  * stopping test (the plain map's residual), a different sequence of iterates.  With
- * accel > 0 and aa_budget > 0 the sweep cap is min(g->niter, aa_start + aa_budget). */
+ * accel > 0 and aa_budget > 0 the sweep cap is min(g->niter, aa_start + aa_budget).
+ * accel > 0 with a guess returns -2, as the library refuses it: from a guess the mixing
+ * can carry a species whose fixed point is above the 1e-30 floor far under it, where the
+ * stopping test no longer sees it move. */
 int rbo_chem_converge_aa(const rbo_grid *g, double dt, double tol, int accel, int aa_start,
                          int aa_budget, const double *T, const double *p, const double *sza,
                          const double *alt, const double *ajl, const double *o3up, double *qj1,
@@ -60,7 +63,7 @@ int rbo_chem_converge_aa(const rbo_grid *g, double dt, double tol, int accel, in
  * step.  One sweep from it measures its residual as a fixed point of the sweep map (the
  * converged mode's test at accept_tol); a box under accept_tol keeps the surrogate's
  * answer and gets iters[box] = 0, every other box goes on sweeping to tol exactly as
- * rbo_chem_converge_aa does from that guess. */
+ * rbo_chem_converge does from that guess (accel > 0 returns -2, as above). */
 int rbo_chem_surrogate(const rbo_grid *g, double dt, double accept_tol, double tol, int accel,
                        int aa_start, int aa_budget, const double *T, const double *p,
                        const double *sza,

@@ -126,15 +126,16 @@ bool any_null(std::initializer_list<const void *> ps)
     return false;
 }
 
-// rb_config.check_finite: count the non-finite values of the state after a step (the
-// whole qj1 and hc fields), report the count in rb_stats and return RB_ENONFINITE when
-// there is any.  It synchronises the stream, so it is skipped inside a graph capture.
-// Newton solve: plain sweeps a box that Newton hands over still gets
+// Newton solve: the most plain sweeps a box that Newton hands back still gets (on top of
+// the sweeps it had before Newton)
 #ifndef RB_NEWTON_RESUME_SWEEPS
 #define RB_NEWTON_RESUME_SWEEPS 100
 #endif
 constexpr int32_t NEWTON_RESUME_SWEEPS = RB_NEWTON_RESUME_SWEEPS;
 
+// rb_config.check_finite: count the non-finite values of the state after a step (the
+// whole qj1 and hc fields), report the count in rb_stats and return RB_ENONFINITE when
+// there is any.  It synchronises the stream, so it is skipped inside a graph capture.
 int finite_check(rb_ctx *ctx, const double *qj1, const double *hc, int64_t field)
 {
     if (!ctx->cfg.check_finite || ctx->capturing) return RB_OK;
@@ -378,6 +379,15 @@ static int converge_impl(rb_ctx *ctx, const rb_grid *g, double dt, double tol, i
     if (!(tol > 0.) || maxiter < 1) return fail(ctx, RB_EINVAL, "tol must be > 0, maxiter >= 1");
     if (accel != 0 && accel != rb::AA_DEPTH)
         return fail(ctx, RB_EINVAL, "accel must be 0 (plain sweeps) or 2 (Anderson depth 2)");
+    // From a guess, the mixing extrapolates some boxes' night NO2 (F:1671-1746; fixed point
+    // NOx - b9 O3 / b12, about 1e-11) down to ~1e-41, under the 1e-30 floor.  The stopping
+    // test compares values there at the floor, so such a box stops while its NO2 still grows
+    // by NOx b12 / (b9 O3) per sweep.  On the c2 grid the learned guess gives ~150 such boxes
+    // of 786,240; from the incoming state, none.
+    if (accel && guess_qj1)
+        return fail(ctx, RB_EINVAL, "accel with a guess is refused: Anderson mixing from a "
+                                    "guess can stop boxes under the 1e-30 floor short of the "
+                                    "fixed point (use accel 0, or rb_chem_newton_dev)");
     HIPCHK(hipSetDevice(ctx->device));
     const int64_t plane = (int64_t)g->nlon * g->nlat;
     const int64_t field = plane * g->niv;

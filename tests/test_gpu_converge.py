@@ -194,3 +194,10 @@ def test_converged_rejects_bad_arguments(step):
         converge_gpu(step, inp, tol=0.0)
     with pytest.raises(RbError):
         converge_gpu(step, inp, maxiter=0)
+    with pytest.raises(RbError):
+        converge_gpu(step, inp, accel=1)
+    # Anderson mixing from a guess is refused (it can leave boxes short of the fixed point
+    # under the 1e-30 floor); from the incoming state it runs
+    with pytest.raises(RbError, match="accel with a guess"):
+        converge_gpu(step, inp, guess=(inp.qj1, inp.hc), accel=2)
+    converge_gpu(step, inp, maxiter=50, accel=2)

@@ -564,3 +564,33 @@ def test_anderson_acceleration_reaches_the_same_fixed_point(table):
                     plain[k][:42] if k == "qj1_chem" else plain[k])
         assert np.percentile(d[:, live].max(axis=0), 99) < 1e-9, k
     assert np.abs(aa["iters"]).mean() < 0.6 * np.abs(plain["iters"]).mean()
+
+
+def test_a_guess_under_the_floor_stops_short(table):
+    """Why Anderson mixing from a guess is refused.  Changes under the 1e-30 floor are not
+    seen by the stopping test, and night NO2 (F:1671-1746) under the floor grows back only
+    by NOx b12 / (b9 O3) per sweep.  So night boxes whose NO2 fixed point is above the
+    floor, started from their fixed point with NO2 set to 1e-40, report convergence with
+    NO2 still under the floor; from the incoming state they reach the fixed point.  (From
+    the learned guess, the mixing drives c2 boxes there.)  The restatement refuses accel
+    with a guess, as the library does."""
+    import cport
+    from reprobus_amd import synth
+
+    inp = synth.make_c1(seed=1, nlon=12, nlat=7)
+    nb = inp.grid.nivbas
+    base = cport.chem_converge(inp, *table, tol=1e-11, maxiter=5000, conserv=False)
+    s = synth.S["no2"]
+    sel = (inp.sza > 95) & (base["hc_chem"][s, :nb] > 1e-20) & (base["iters"] > 0)
+    gq, gh = base["qj1_chem"].copy(), base["hc_chem"].copy()
+    gh[s, :nb][sel] = 1e-40
+    r = cport.chem_converge(inp, *table, tol=1e-11, maxiter=5000, conserv=False,
+                            guess=(gq, gh))
+    short = sel & (r["iters"] > 0) & (r["hc_chem"][s, :nb] < 1e-30)
+    assert sel.sum() > 1000 and 0 < short.sum() < 0.05 * sel.sum()
+    assert r["iters"][short].max() < 20          # stopped early, well short of the fixed point
+    with pytest.raises(RuntimeError):
+        cport.chem_converge(inp, *table, tol=1e-11, maxiter=50, conserv=False,
+                            guess=(gq, gh), accel=2)
+    with pytest.raises(RuntimeError):
+        cport.chem_surrogate(inp, *table, (gq, gh), accel=2)
