@@ -18,6 +18,7 @@ Fortran timed on the host cores.  Prints one JSON line.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--no-extras]
                   [--extras converged,surrogate,day,transport] [--surrogate-steps 96]
+                  [--block B] [--conv-shape BLOCK,CHUNK --newton-shape SWEEPS_FIRST,WAVES]
 """
 from __future__ import annotations
 
@@ -72,6 +73,10 @@ def parse():
     ap.add_argument("--surrogate-steps", type=int, default=96,
                     help="sub-steps of the surrogate run (96 = one day, 960 = ten days)")
     ap.add_argument("--block", type=int, default=0, help="0: autotuned")
+    ap.add_argument("--conv-shape", default="",
+                    help="BLOCK,CHUNK of the converged-mode kernels (default: autotuned)")
+    ap.add_argument("--newton-shape", default="",
+                    help="SWEEPS_FIRST,WAVES of the Newton solve (default: autotuned)")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="after the timed region, one device copy of this many MB (a kernel of "
                          "known traffic, to scale profiler byte counters)")
@@ -412,8 +417,7 @@ def surrogate_section(table, block_conv, refill, dev, nsteps=96, accept_tol=1e-8
         return {"error": "no trained guess weights (reprobus_amd/data/guess_mlp.npz)"}
     sp = _device_state(inp, dev)                    # surrogate path
     fp = _device_state(inp, dev)                    # FP64 path
-    step = ChemistryStep(*table, device=0, block=block_conv, conv_refill=refill,
-                         newton_waves=newton_shape[1])
+    step = ChemistryStep(*table, device=0, block=block_conv, conv_refill=refill)
     stream = torch.cuda.current_stream()
     step.set_stream(stream.cuda_stream)
     net.set_stream(stream.cuda_stream)
@@ -647,7 +651,13 @@ def main():
     want = set() if args.no_extras else set(s for s in args.extras.split(",") if s)
     conv_shape = None
     newton_shape = (60, 1)
-    if want & {"converged", "surrogate"}:
+    if args.conv_shape and args.newton_shape:
+        conv_shape = tuple(int(v) for v in args.conv_shape.split(","))
+        newton_shape = tuple(int(v) for v in args.newton_shape.split(","))
+        tuned["converged"] = {"block": conv_shape[0], "refill": conv_shape[1], "given": True}
+        tuned["newton"] = {"sweeps_first": newton_shape[0], "waves": newton_shape[1],
+                           "given": True}
+    elif want & {"converged", "surrogate"}:
         try:
             cache = os.path.join(tempfile.gettempdir(), "reprobus_amd_tune.json")
             c2 = _device_state(synth.make_c2(1), dev)
