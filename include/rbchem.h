@@ -160,6 +160,31 @@ int rb_altitude_dev(rb_ctx *ctx, const rb_grid *g, double dlon, double dlat,
  * COMMON /depo/ (F:6215-6217); aaa, bbb the niv hybrid coefficients (COMMON /verniv/).
  * Each entry synchronises the stream (it reads back the count of departure points whose
  * cell falls outside the grid, RB_EINVAL when there is any). */
+/* One forcing record of the trajectory code (COMMON /forcm1/ and /forcp1/, F:6197-6203),
+ * device arrays: winds u, v (m/s) and omega w (Pa/s), temperature t (K), nlon*nlat*niv;
+ * lnps the natural log of the surface pressure (Pa), nlon*nlat; day its day number. */
+typedef struct {
+    const double *u, *v, *w, *t, *lnps;
+    double day;
+} rb_forcing;
+/* rb_transport_backtra_dev replaces `call backtra(daynum, dtdyn, nrapp, trajlon, trajlat,
+ * trajpre, trajtem, ntime, nintecmwf)` (F:6142-6144, F:6156-6470): the back-trajectory of
+ * every node over dtdyn seconds ending at day daynum, in nrapp sub-steps of two-pass
+ * midpoint trajectories (in a frame rotated about the y axis poleward of 45 degrees), the
+ * forcing interpolated in time between the records m1 and p1 (instant1/instant2; records
+ * nintecmwf / ntime hours apart).  Outputs, device: the departure points x0, y0, z0, p0 (n
+ * each, COMMON /depo/), their cells idx and weights w (as rb_transport_weights_dev gives
+ * them, for rb_transport_interp_dev), and, with trajlon non-NULL, the positions (deg, deg,
+ * hPa) and temperatures (K) along the trajectory: traj*[(s - 1) n + p] for sub-step slot
+ * s = 1..nrapp, slot nrapp being the arrival nodes, as the model's trajlon(:,:,:,s).  The
+ * four trajectory arrays are all set or all NULL.  Needs nlat >= 5; synchronises the
+ * stream; RB_EINVAL if a point's cell falls outside the grid on the way. */
+int rb_transport_backtra_dev(rb_ctx *ctx, const rb_grid *g, const double *aaa, const double *bbb,
+                             const rb_forcing *m1, const rb_forcing *p1, double daynum,
+                             double dtdyn, int32_t nrapp, int32_t ntime, int32_t nintecmwf,
+                             double *x0, double *y0, double *z0, double *p0, int32_t *idx,
+                             double *w, double *trajlon, double *trajlat, double *trajpre,
+                             double *trajtem);
 /* rb_transport_weights_dev replaces posx/posy/posz and poidx/poidy/poidz as backtra calls
  * them (F:6444-6451, F:7371-7647): idx (3 n int32: ig0, kg0, lg0, 1-based, COMMON
  * /indices/) and w (12 n doubles: alfxm1 .. alfzp2, COMMON /poids/ order), device. */

@@ -68,6 +68,10 @@ def lib(cr_psc: bool = False):
         L.rbo_transport_weights.restype = None
         L.rbo_internew.argtypes = [C.c_int, C.c_int, C.c_int, ip, dp, dp, dp]
         L.rbo_internew.restype = C.c_int64
+        L.rbo_backtra.argtypes = [C.c_int, C.c_int, C.c_int, dp, dp, C.c_void_p, C.c_void_p,
+                                  C.c_double, C.c_double, C.c_int, C.c_int, C.c_int] + \
+            [C.c_void_p] * 10 + [C.c_int]
+        L.rbo_backtra.restype = C.c_int
         L.rbo_troe_constants.argtypes = [dp]
         L.rbo_troe.argtypes = [C.c_double] * 6
         L.rbo_troe.restype = C.c_double
@@ -227,6 +231,43 @@ def transport_weights(tr) -> tuple:
     lib().rbo_transport_weights(tr.nlon, tr.nlat, tr.niv, tr.aaa, tr.bbb, tr.x0.reshape(-1),
                                 tr.y0.reshape(-1), tr.z0.reshape(-1), tr.p0.reshape(-1), idx, w)
     return idx, w
+
+
+class _Forcing(C.Structure):
+    _fields_ = [("u", C.c_void_p), ("v", C.c_void_p), ("w", C.c_void_p), ("t", C.c_void_p),
+                ("lnps", C.c_void_p), ("day", C.c_double)]
+
+
+def _forcing(rec, keep):
+    arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (rec.u, rec.v, rec.w, rec.t, rec.lnps)]
+    keep.extend(arrs)
+    return _Forcing(*[a.ctypes.data for a in arrs], float(rec.day))
+
+
+def backtra(ti, nrapp: int, nthreads: int = 0) -> dict:
+    """rbo_backtra on synth.TrajectoryInputs: departure points x0 y0 z0 p0, cells idx (3, n),
+    weights w (12, n) and the trajectory traj{lon,lat,pre,tem} (nrapp, niv, nlat, nlon)."""
+    L = lib()
+    n = ti.nlon * ti.nlat * ti.niv
+    keep = []
+    m1, p1 = _forcing(ti.m1, keep), _forcing(ti.p1, keep)
+    out = {k: np.zeros(n) for k in ("x0", "y0", "z0", "p0")}
+    idx = np.zeros((3, n), dtype=np.int32)
+    w = np.zeros((12, n))
+    tr = {k: np.zeros((nrapp, ti.niv, ti.nlat, ti.nlon)) for k in ("trajlon", "trajlat", "trajpre", "trajtem")}
+    c = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+    rc = L.rbo_backtra(ti.nlon, ti.nlat, ti.niv, c(ti.aaa), c(ti.bbb), C.byref(m1), C.byref(p1),
+                       float(ti.daynum), float(ti.dtdyn), int(nrapp), int(ti.ntime),
+                       int(ti.nintecmwf), *[out[k].ctypes.data for k in ("x0", "y0", "z0", "p0")],
+                       idx.ctypes.data, w.ctypes.data,
+                       *[tr[k].ctypes.data for k in ("trajlon", "trajlat", "trajpre", "trajtem")],
+                       int(nthreads))
+    if rc < 0:
+        raise RuntimeError(f"rbo_backtra failed: {rc}")
+    out.update(tr)
+    out["outside"] = rc
+    out["idx"], out["w"] = idx, w
+    return out
 
 
 def internew(tr, idx, w, q) -> tuple:

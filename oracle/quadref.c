@@ -9,4 +9,7 @@ double qr_log10(double x) { return (double)log10q((__float128)x); }
 double qr_pow(double x, double y) { return (double)powq((__float128)x, (__float128)y); }
 double qr_atan(double x) { return (double)atanq((__float128)x); }
 double qr_cos(double x) { return (double)cosq((__float128)x); }
+double qr_sin(double x) { return (double)sinq((__float128)x); }
+double qr_asin(double x) { return (double)asinq((__float128)x); }
+double qr_acos(double x) { return (double)acosq((__float128)x); }
 double qr_tanh(double x) { return (double)tanhq((__float128)x); }

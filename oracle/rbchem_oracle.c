@@ -11,6 +11,7 @@
  * Parity: pinned against the flang-built reference (tests/golden fixtures
  * and test_oracle_vs_reference) and against closed forms in the unit tests.
  */
+#define _GNU_SOURCE             /* sincos */
 #include "rbchem_oracle.h"
 
 #include <math.h>
@@ -1837,7 +1838,10 @@ int64_t rbo_internew(int nlon, int nlat, int niv, const int32_t *idx, const doub
     int64_t bad = 0;
     for (int64_t p = 0; p < n; ++p) {
         const int i = idx[p], k = idx[n + p], l = idx[2 * n + p];
-        if (i < 1 || i > nlon || k < 1 || k > nlat - 1 || l < 1 || l > niv) {
+        /* kg0 = nlat (a point on the south-pole row): the stencil reaches one row past
+         * the extended grid, with weight 0 when the point is on the row; that row is read
+         * as row nlat + 1 */
+        if (i < 1 || i > nlon || k < 1 || k > nlat || l < 1 || l > niv) {
             ++bad;
             continue;
         }
@@ -1848,7 +1852,7 @@ int64_t rbo_internew(int nlon, int nlat, int niv, const int32_t *idx, const doub
             if (c == 3 && lc > niv + 1) lc = niv + 1;   /* lp2 = min(l+2, niv+1) */
             const int lv = lc < 1 ? 1 : (lc > niv ? niv : lc);
             for (int b = 0; b < 4; ++b) {
-                const int kk = k + b - 1;
+                const int kk = k + b - 1 > nlat + 1 ? nlat + 1 : k + b - 1;
                 for (int a = 0; a < 4; ++a) {
                     int ii = i + a - 1;
                     if (ii == 0) ii = nlon;
@@ -1882,4 +1886,428 @@ int64_t rbo_internew(int nlon, int nlat, int niv, const int32_t *idx, const doub
 #undef IP
     return bad;
 }
+
+/* ---------------------------------------------------------------- back-trajectories */
+/* The extended grids of the trajectory code, x(0:nlon+2, 0:nlat+1, 0:niv+1) (inivit
+ * F:6557-6722, intert F:7142-7177) and x(0:nlon+2, 0:nlat+1) (interp F:7029-7050). */
+#define EX(i, k, l) ((((size_t)(l)) * (size_t)(nlat + 2) + (size_t)(k)) * (size_t)(nlon + 3) + (size_t)(i))
+#define EX2(i, k) ((size_t)(k) * (size_t)(nlon + 3) + (size_t)(i))
+
+typedef struct {
+    double xpi, xpi2, xpih, conv, dlatlim, rter;
+    double *dlon, *dlat;                       /* dlon(0:nlon+2), dlat(0:nlat+1) */
+    double *slon, *clon, *slat;                /* [i-1], [k-1] */
+    double *slonr, *clonr, *slatr, *clatr;     /* [(k-1) nlon + i-1]; zero where unset */
+    int *ip;                                   /* [i-1] */
+} bgrid_t;
+
+/* initransp, F:6010-6097 (cstes F:6010-6025, grid F:6034-6073, ip F:6089-6091,
+ * dlatlim F:6101); the rotated-frame tables are not set on the row where clat = 1 and
+ * keep the zero the COMMON block starts with. */
+static void bgrid_init(bgrid_t *b, int nlon, int nlat)
+{
+    b->rter = 6371229.;
+    b->xpi = 2. * asin(1.);
+    b->xpi2 = 2. * b->xpi;
+    b->xpih = b->xpi / 2.;
+    b->conv = 180. / b->xpi;
+    b->dlatlim = 45. / b->conv;
+    b->dlon = calloc((size_t)nlon + 3, sizeof(double));
+    b->dlat = calloc((size_t)nlat + 2, sizeof(double));
+    b->slon = calloc((size_t)nlon, sizeof(double));
+    b->clon = calloc((size_t)nlon, sizeof(double));
+    b->slat = calloc((size_t)nlat, sizeof(double));
+    b->slonr = calloc((size_t)nlon * nlat, sizeof(double));
+    b->clonr = calloc((size_t)nlon * nlat, sizeof(double));
+    b->slatr = calloc((size_t)nlon * nlat, sizeof(double));
+    b->clatr = calloc((size_t)nlon * nlat, sizeof(double));
+    b->ip = calloc((size_t)nlon, sizeof(int));
+    const double ddlat = b->xpi / (double)(nlat - 1), ddlon = b->xpi2 / (double)nlon;
+    double *clat = calloc((size_t)nlat, sizeof(double));
+    for (int k = 0; k <= nlat + 1; ++k) b->dlat[k] = b->xpih - ((double)k - 1.) * ddlat;
+    /* the flang build evaluates each of these two loops' sin and cos of one argument with
+     * one sincos call, whose last bit differs from sin's or cos's on ~0.07 % of arguments */
+    for (int k = 1; k <= nlat; ++k) sincos(b->dlat[k], &b->slat[k - 1], &clat[k - 1]);
+    for (int i = 0; i <= nlon + 2; ++i) b->dlon[i] = ddlon * ((double)i - 1.);
+    for (int i = 1; i <= nlon; ++i) sincos(b->dlon[i], &b->slon[i - 1], &b->clon[i - 1]);
+    for (int k = 1; k <= nlat; ++k) {
+        if (clat[k - 1] == 1.) continue;
+        for (int i = 1; i <= nlon; ++i) {
+            const size_t c = (size_t)(k - 1) * nlon + (i - 1);
+            b->slatr[c] = -b->clon[i - 1] * clat[k - 1];
+            b->clatr[c] = sqrt(1. - b->slatr[c] * b->slatr[c]);
+            b->clonr[c] = fmax(-1., fmin(b->slat[k - 1] / b->clatr[c], 1.));
+            const double sl = sqrt(1. - b->clonr[c] * b->clonr[c]);
+            b->slonr[c] = copysign(sl, b->slon[i - 1]);
+        }
+    }
+    for (int i = 1; i <= nlon; ++i) b->ip[i - 1] = (i + nlon / 2 - 1) % nlon + 1;
+    free(clat);
+}
+
+static void bgrid_free(bgrid_t *b)
+{
+    free(b->dlon); free(b->dlat); free(b->slon); free(b->clon); free(b->slat);
+    free(b->slonr); free(b->clonr); free(b->slatr); free(b->clatr); free(b->ip);
+}
+
+/* sin and cos of the trajectory code as separate calls, as the flang build makes them
+ * there (kept out of line so that the compiler does not merge a pair into sincos) */
+__attribute__((noinline)) static double bt_sin(double x) { return sin(x); }
+__attribute__((noinline)) static double bt_cos(double x) { return cos(x); }
+
+/* instant1 / instant2 (F:6473-6554): the weight of the later record at day daynum */
+static double instant_weight(double daynum, double dm1, int ntime, int nintecmwf)
+{
+    double day = daynum;
+    if (day < 1.e-5) day = day + trunc(dm1) + 1.;
+    double cp1 = (day - dm1) / ((double)nintecmwf / (double)(24 * ntime));
+    cp1 = fmin(cp1, 1.);
+    cp1 = fmax(cp1, 0.);
+    return cp1;
+}
+
+/* posx/posy/posz + poidx/poidy/poidz (F:7371-7647) at one point: cell (1-based) and
+ * weights w[12] */
+static void cell_weights(const tgrid_t *t, int niv, const double *aaa, const double *bbb,
+                         double xx, double yy, double zz, double pp, int want_z, int c[3],
+                         double w[12])
+{
+    const int ix = (int)(xx / t->ddlon);
+    const double dx = xx - (double)ix * t->ddlon;
+    const int iy = (int)((t->xpih - yy) / t->ddlat);
+    const double dy = t->xpih - (double)iy * t->ddlat - yy;
+    c[0] = ix + 1;
+    c[1] = iy + 1;
+    {
+        const double dxm1 = dx + t->ddlon, dxp0 = dx, dxp1 = dx - t->ddlon, dxp2 = dx - t->ddlon * 2.;
+        w[0] = t->xcoef[0] * dxp0 * dxp1 * dxp2;
+        w[1] = t->xcoef[1] * dxm1 * dxp1 * dxp2;
+        w[2] = t->xcoef[2] * dxm1 * dxp0 * dxp2;
+        w[3] = t->xcoef[3] * dxm1 * dxp0 * dxp1;
+    }
+    {
+        const double dym1 = dy + t->ddlat, dyp0 = dy, dyp1 = dy - t->ddlat, dyp2 = dy - t->ddlat * 2.;
+        w[4] = t->ycoef[0] * dyp0 * dyp1 * dyp2;
+        w[5] = t->ycoef[1] * dym1 * dyp1 * dyp2;
+        w[6] = t->ycoef[2] * dym1 * dyp0 * dyp2;
+        w[7] = t->ycoef[3] * dym1 * dyp0 * dyp1;
+    }
+    if (!want_z) return;
+    int ll = 1;
+    for (int l = 2; l <= niv - 1; ++l) {
+        const double pres = (aaa[l - 1] + bbb[l - 1] * pp) * 100.;
+        if (zz - pres >= 0.) ll = l;
+    }
+    c[2] = ll;
+    const double prem1 = ll > 1 ? (aaa[ll - 2] + bbb[ll - 2] * pp) * 100. : 5.e-3 * 100.;
+    const double prep0 = (aaa[ll - 1] + bbb[ll - 1] * pp) * 100.;
+    const double prep1 = (aaa[ll] + bbb[ll] * pp) * 100.;
+    const double prep2 = ll < niv - 1 ? (aaa[ll + 1] + bbb[ll + 1] * pp) * 100. : 1200. * 100.;
+    const double dzm1 = zz - prem1, dzp0 = zz - prep0, dzp1 = zz - prep1, dzp2 = zz - prep2;
+    const double dzp2m1 = prep2 - prem1, dzp2p0 = prep2 - prep0, dzp2p1 = prep2 - prep1;
+    const double dzp1m1 = prep1 - prem1, dzp1p0 = prep1 - prep0, dzp0m1 = prep0 - prem1;
+    w[8] = -(dzp0 * dzp1 * dzp2) / (dzp0m1 * dzp1m1 * dzp2m1);
+    w[9] = (dzm1 * dzp1 * dzp2) / (dzp0m1 * dzp1p0 * dzp2p0);
+    w[10] = -(dzm1 * dzp0 * dzp2) / (dzp1m1 * dzp1p0 * dzp2p1);
+    w[11] = (dzm1 * dzp0 * dzp1) / (dzp2m1 * dzp2p0 * dzp2p1);
+}
+
+static int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+/* A cell the extended grid holds: ig0 in 1..nlon, kg0 in 1..nlat.  A point on the
+ * south-pole row (kg0 = nlat) reaches one row past the reference's extended grid, with
+ * weight 0 when the point is on the row; that row is read as row nlat + 1.  Cells outside
+ * are counted (and read clamped, so that every read stays inside the fields). */
+static int cell_inside(const int c[3], int nlon, int nlat)
+{
+    return c[0] >= 1 && c[0] <= nlon && c[1] >= 1 && c[1] <= nlat;
+}
+
+/* the tricubic Lagrange sum of interv (F:6784-6842, no limiter) or intert
+ * (F:7181-7250, limiter at every stage) on an extended field e */
+static double tricubic(int nlon, int nlat, const double *e, const int c0[3], const double w[12],
+                       int limit)
+{
+    const int c[3] = {clampi(c0[0], 1, nlon), c0[1], c0[2]};
+    double z[4];
+    for (int ls = -1; ls <= 2; ++ls) {
+        double y[4];
+        for (int ks = -1; ks <= 2; ++ks) {
+            const int i = c[0], k = clampi(c[1] + ks, 0, nlat + 1), l = c[2] + ls;
+            const double xm1 = e[EX(i - 1, k, l)], xp0 = e[EX(i, k, l)], xp1 = e[EX(i + 1, k, l)],
+                         xp2 = e[EX(i + 2, k, l)];
+            double v = w[0] * xm1 + w[1] * xp0 + w[2] * xp1 + w[3] * xp2;
+            if (limit) v = fmax(fmin(v, fmax(xp0, xp1)), fmin(xp0, xp1));
+            y[ks + 1] = v;
+        }
+        double v = w[4] * y[0] + w[5] * y[1] + w[6] * y[2] + w[7] * y[3];
+        if (limit) v = fmax(fmin(v, fmax(y[1], y[2])), fmin(y[1], y[2]));
+        z[ls + 1] = v;
+    }
+    double v = w[8] * z[0] + w[9] * z[1] + w[10] * z[2] + w[11] * z[3];
+    if (limit) v = fmax(fmin(v, fmax(z[1], z[2])), fmin(z[1], z[2]));
+    return v;
+}
+
+/* interp (F:6982-7108): the surface pressure ex2 (extended) at (xx, yy), bicubic with
+ * the limiter */
+static double interp_ps(const tgrid_t *t, int nlon, int nlat, const double *ex2, double xx,
+                        double yy, int *outside)
+{
+    int c[3];
+    double w[12];
+    cell_weights(t, 0, NULL, NULL, xx, yy, 0., 0., 0, c, w);
+    *outside |= !cell_inside(c, nlon, nlat);
+    c[0] = clampi(c[0], 1, nlon);
+    double y[4];
+    for (int ks = -1; ks <= 2; ++ks) {
+        const int i = c[0], k = clampi(c[1] + ks, 0, nlat + 1);
+        const double xm1 = ex2[EX2(i - 1, k)], xp0 = ex2[EX2(i, k)], xp1 = ex2[EX2(i + 1, k)],
+                     xp2 = ex2[EX2(i + 2, k)];
+        const double v = w[0] * xm1 + w[1] * xp0 + w[2] * xp1 + w[3] * xp2;
+        y[ks + 1] = fmax(fmin(v, fmax(xp0, xp1)), fmin(xp0, xp1));
+    }
+    const double v = w[4] * y[0] + w[5] * y[1] + w[6] * y[2] + w[7] * y[3];
+    return fmax(fmin(v, fmax(y[1], y[2])), fmin(y[1], y[2]));
+}
+
+/* corlat (F:7283-7329) */
+static void corlat1(const bgrid_t *b, double *x, double *y)
+{
+    *x = *x - b->xpi2 * (double)(int)(*x / b->xpi2);
+    if (*x < 0.) *x = *x + b->xpi2;
+    *y = *y - b->xpi2 * (double)(int)(*y / b->xpi2);
+    const double temp0 = b->xpih - fabs(*y);
+    const double temp1 = *x - b->xpi < 0. ? *x + b->xpi : *x - b->xpi;
+    const double temp2 = copysign(b->xpi, *y) - *y;
+    if (temp0 < 0.) {
+        *x = temp1;
+        *y = temp2;
+    }
+}
+
+/* corniv (F:7333-7367) */
+static double corniv1(int niv, const double *aaa, const double *bbb, double z, double p)
+{
+    const double pmin = (aaa[0] + bbb[0] * p) * 100.;
+    const double pmax = (aaa[niv - 1] + bbb[niv - 1] * p) * 100.;
+    return fmin(pmax, fmax(pmin, z));
+}
+
+/* extended copy of a base-grid field (lon fastest): polar rows mirrored through the pole
+ * with sign sg (F:6626-6635), longitudes periodic (F:6648-6660), levels clamped
+ * (F:6690-6700); on the rows of skip (the equator row of u1m, v1m: -1 for none) nothing is
+ * written, as inivit leaves them */
+static void extend3(int nlon, int nlat, int niv, const int *ip, double *e, double sg, int skip)
+{
+    for (int l = 1; l <= niv; ++l)
+        for (int i = 1; i <= nlon; ++i) {
+            e[EX(i, 0, l)] = sg * e[EX(ip[i - 1], 2, l)];
+            e[EX(i, nlat + 1, l)] = sg * e[EX(ip[i - 1], nlat - 1, l)];
+        }
+    for (int l = 1; l <= niv; ++l)
+        for (int k = 0; k <= nlat + 1; ++k) {
+            if (k == skip) continue;
+            e[EX(0, k, l)] = e[EX(nlon, k, l)];
+            e[EX(nlon + 1, k, l)] = e[EX(1, k, l)];
+            e[EX(nlon + 2, k, l)] = e[EX(2, k, l)];
+        }
+    for (int i = 0; i <= nlon + 2; ++i)
+        for (int k = 0; k <= nlat + 1; ++k) {
+            if (k == skip) continue;
+            e[EX(i, k, 0)] = e[EX(i, k, 1)];
+            e[EX(i, k, niv + 1)] = e[EX(i, k, niv)];
+        }
+}
+
+int rbo_backtra(int nlon, int nlat, int niv, const double *aaa, const double *bbb,
+                const rbo_forcing *m1, const rbo_forcing *p1, double daynum, double dtdyn,
+                int nrapp, int ntime, int nintecmwf, double *x0, double *y0, double *z0,
+                double *p0, int32_t *idx, double *w, double *trajlon, double *trajlat,
+                double *trajpre, double *trajtem, int nthreads)
+{
+    if (nlon < 4 || nlat < 5 || niv < 3 || nrapp < 1 || ntime < 1 || nintecmwf < 1) return -1;
+    const size_t n = (size_t)nlon * nlat * niv, plane = (size_t)nlon * nlat;
+    const size_t ne = (size_t)(nlon + 3) * (nlat + 2) * (niv + 2);
+    const tgrid_t t = transport_grid(nlon, nlat);
+    bgrid_t b;
+    bgrid_init(&b, nlon, nlat);
+    double *u1s = calloc(ne, sizeof(double)), *v1s = calloc(ne, sizeof(double));
+    double *w1s = calloc(ne, sizeof(double)), *u1m = calloc(ne, sizeof(double));
+    double *v1m = calloc(ne, sizeof(double)), *te = calloc(ne, sizeof(double));
+    double *pe = calloc((size_t)(nlon + 3) * (nlat + 2), sizeof(double));
+    double *pj = malloc(plane * sizeof(double));
+    const int eq = (nlat - 1) / 2 + 1;          /* the row inivit skips for u1m, v1m */
+    int64_t bad = 0;                            /* points whose cell fell outside, per sub-step */
+    /* dtn, the sub-step of the trajectories (F:6237) */
+    const double dtn = dtdyn / (double)nrapp;
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#endif
+    for (int iter = 1; iter <= nrapp; ++iter) {
+        /* F:6245: the time of this sub-step; instant1/instant2 at it (F:6249-6258) */
+        const double daypre = daynum - dtn * (double)(iter - 1) / 86400.;
+        const double cp1 = instant_weight(daypre, m1->day, ntime, nintecmwf), cm1 = 1. - cp1;
+        for (size_t c = 0; c < plane; ++c) pj[c] = .01 * (cp1 * exp(p1->lnps[c]) + cm1 * exp(m1->lnps[c]));
+        /* inivit (F:6557-6722): the winds of this time, scaled, on the extended grid, and
+         * in the frame rotated about the y axis poleward of the equator row */
+        const double coefh = dtn / b.rter, coefv = dtn;
+        for (int l = 1; l <= niv; ++l)
+            for (int k = 1; k <= nlat; ++k)
+                for (int i = 1; i <= nlon; ++i) {
+                    const size_t s = (size_t)(l - 1) * plane + (size_t)(k - 1) * nlon + (i - 1);
+                    const size_t e = EX(i, k, l);
+                    const double uj = cp1 * p1->u[s] + cm1 * m1->u[s];
+                    const double vj = cp1 * p1->v[s] + cm1 * m1->v[s];
+                    const double tj = cp1 * p1->t[s] + cm1 * m1->t[s];
+                    const double wj = cp1 * p1->w[s] + cm1 * m1->w[s];
+                    u1s[e] = uj * coefh;
+                    v1s[e] = vj * coefh;
+                    w1s[e] = wj * coefv;
+                    te[e] = tj;
+                    if (k != eq) {
+                        const size_t r = (size_t)(k - 1) * nlon + (i - 1);
+                        v1m[e] = (u1s[e] * b.slon[i - 1] + v1s[e] * b.clon[i - 1] * b.slat[k - 1]) / b.clatr[r];
+                        u1m[e] = (u1s[e] * b.clon[i - 1] - v1s[e] * b.slon[i - 1] * b.slat[k - 1] +
+                                  v1m[e] * b.slonr[r] * b.slatr[r]) / b.clonr[r];
+                    }
+                }
+        extend3(nlon, nlat, niv, b.ip, u1s, -1., -1);
+        extend3(nlon, nlat, niv, b.ip, v1s, -1., -1);
+        extend3(nlon, nlat, niv, b.ip, w1s, 1., -1);
+        extend3(nlon, nlat, niv, b.ip, u1m, 1., eq);
+        extend3(nlon, nlat, niv, b.ip, v1m, 1., eq);
+        extend3(nlon, nlat, niv, b.ip, te, 1., -1);         /* intert F:7142-7177 */
+        for (int k = 1; k <= nlat; ++k)                      /* interp F:7029-7050 */
+            for (int i = 1; i <= nlon; ++i) pe[EX2(i, k)] = pj[(size_t)(k - 1) * nlon + (i - 1)];
+        for (int i = 1; i <= nlon; ++i) {
+            pe[EX2(i, 0)] = pe[EX2(b.ip[i - 1], 2)];
+            pe[EX2(i, nlat + 1)] = pe[EX2(b.ip[i - 1], nlat - 1)];
+        }
+        for (int k = 0; k <= nlat + 1; ++k) {
+            pe[EX2(0, k)] = pe[EX2(nlon, k)];
+            pe[EX2(nlon + 1, k)] = pe[EX2(1, k)];
+            pe[EX2(nlon + 2, k)] = pe[EX2(2, k)];
+        }
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
+        for (int64_t q = 0; q < (int64_t)n; ++q) {
+            const int i = (int)(q % nlon) + 1, k = (int)((q / nlon) % nlat) + 1, l = (int)(q / plane) + 1;
+            int outside = 0;
+            if (iter == 1) {                            /* F:6262-6276 */
+                x0[q] = b.dlon[i];
+                y0[q] = b.dlat[k];
+                z0[q] = (aaa[l - 1] + bbb[l - 1] * pj[(size_t)(k - 1) * nlon + (i - 1)]) * 100.;
+                p0[q] = pj[(size_t)(k - 1) * nlon + (i - 1)];
+                if (trajlon) {
+                    const size_t o = (size_t)(nrapp - 1) * n + (size_t)q;
+                    trajlon[o] = x0[q] * 180. / b.xpi;
+                    trajlat[o] = y0[q] * 180. / b.xpi;
+                    trajpre[o] = z0[q] / 100.;
+                    trajtem[o] = te[EX(i, k, l)];
+                }
+            }
+            /* first guess of the midpoint, F:6285-6296; the rotated frame, F:6298-6315 */
+            double xm = x0[q], ym = y0[q], zm = z0[q], pm = p0[q];
+            const int ind = fabs(ym) >= b.dlatlim;
+            double x0r = 0., y0r = 0.;
+            if (ind) {
+                const double sy0r = -bt_cos(x0[q]) * bt_cos(y0[q]);
+                y0r = asin(sy0r);
+                const double cy0r = sqrt(1. - sy0r * sy0r);
+                const double cx0r = cy0r == 0. ? 0. : fmax(-1., fmin(1., bt_sin(y0[q]) / cy0r));
+                x0r = copysign(acos(cx0r), bt_sin(x0[q]));
+            }
+            const double *ue = ind ? u1m : u1s, *ve = ind ? v1m : v1s;
+            int c[3];
+            double wt[12], u2, v2, w2;
+            /* the wind at the first guess (F:6329-6352): the node's own at iter 1 */
+            if (iter == 1) {
+                u2 = ue[EX(i, k, l)];
+                v2 = ve[EX(i, k, l)];
+                w2 = w1s[EX(i, k, l)];
+            } else {                                    /* interv F:6726-6978 */
+                cell_weights(&t, niv, aaa, bbb, xm, ym, zm, pm, 1, c, wt);
+                outside |= !cell_inside(c, nlon, nlat);
+                u2 = tricubic(nlon, nlat, ue, c, wt, 0);
+                v2 = tricubic(nlon, nlat, ve, c, wt, 0);
+                w2 = tricubic(nlon, nlat, w1s, c, wt, 0);
+            }
+            /* the midpoint, F:6356-6384 */
+            if (!ind) {
+                xm = x0[q] - 0.5 * u2 / bt_cos(ym);
+                ym = y0[q] - 0.5 * v2;
+            } else {
+                const double symr = -bt_cos(xm) * bt_cos(ym);
+                const double cymr = sqrt(1. - symr * symr);
+                const double xmr = x0r - 0.5 * u2 / cymr;
+                const double ymr = y0r - 0.5 * v2;
+                const double sym = bt_cos(xmr) * bt_cos(ymr);
+                ym = asin(sym);
+                const double cym = sqrt(1. - sym * sym);
+                const double cxm = cym == 0. ? 0. : fmax(-1., fmin(1., -bt_sin(ymr) / cym));
+                xm = copysign(acos(cxm), bt_sin(xmr));
+            }
+            zm = z0[q] - 0.5 * w2;
+            corlat1(&b, &xm, &ym);                      /* F:6388-6394 */
+            pm = interp_ps(&t, nlon, nlat, pe, xm, ym, &outside);
+            zm = corniv1(niv, aaa, bbb, zm, pm);
+            /* the wind at the midpoint (F:6402) and the departure point, F:6404-6430 */
+            cell_weights(&t, niv, aaa, bbb, xm, ym, zm, pm, 1, c, wt);
+            outside |= !cell_inside(c, nlon, nlat);
+            u2 = tricubic(nlon, nlat, ue, c, wt, 0);
+            v2 = tricubic(nlon, nlat, ve, c, wt, 0);
+            w2 = tricubic(nlon, nlat, w1s, c, wt, 0);
+            double xx = x0[q], yy = y0[q];
+            if (!ind) {
+                xx = xx - u2 / bt_cos(ym);
+                yy = yy - v2;
+            } else {
+                const double symr = -bt_cos(xm) * bt_cos(ym);
+                const double cymr = sqrt(1. - symr * symr);
+                x0r = x0r - u2 / cymr;
+                y0r = y0r - v2;
+                const double sy0 = bt_cos(x0r) * bt_cos(y0r);
+                yy = asin(sy0);
+                const double cy0 = sqrt(1. - sy0 * sy0);
+                const double cx0 = cy0 == 0. ? 0. : fmax(-1., fmin(1., -bt_sin(y0r) / cy0));
+                xx = copysign(acos(cx0), bt_sin(x0r));
+            }
+            double zz = z0[q] - w2;
+            corlat1(&b, &xx, &yy);                      /* F:6434-6440 */
+            const double pp = interp_ps(&t, nlon, nlat, pe, xx, yy, &outside);
+            zz = corniv1(niv, aaa, bbb, zz, pp);
+            x0[q] = xx;
+            y0[q] = yy;
+            z0[q] = zz;
+            p0[q] = pp;
+            /* the departure cell and weights (F:6444-6451), the temperature there (F:6455) */
+            cell_weights(&t, niv, aaa, bbb, xx, yy, zz, pp, 1, c, wt);
+            outside |= !cell_inside(c, nlon, nlat);
+            if (outside) {
+#ifdef _OPENMP
+#pragma omp atomic
+#endif
+                ++bad;
+            }
+            for (int m = 0; m < 3; ++m) idx[(size_t)m * n + q] = c[m];
+            for (int m = 0; m < 12; ++m) w[(size_t)m * n + q] = wt[m];
+            const double tt = tricubic(nlon, nlat, te, c, wt, 1);
+            if (trajlon && iter < nrapp) {              /* F:6459-6469 */
+                const size_t o = (size_t)(nrapp - iter - 1) * n + (size_t)q;
+                trajlon[o] = xx * 180. / b.xpi;
+                trajlat[o] = yy * 180. / b.xpi;
+                trajpre[o] = zz / 100.;
+                trajtem[o] = tt;
+            }
+        }
+    }
+    free(u1s); free(v1s); free(w1s); free(u1m); free(v1m); free(te); free(pe); free(pj);
+    bgrid_free(&b);
+    return bad > 0x7fffffff ? 0x7fffffff : (int)bad;
+}
+#undef EX
+#undef EX2
 #endif /* RBO_OPCOUNT */

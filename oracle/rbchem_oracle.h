@@ -104,12 +104,37 @@ void rbo_altitude(int mlon, int mlat, int mniv, double dlon, double dlat, const 
  * with the grid of initransp (F:6028-6090); aaa, bbb are the niv hybrid coefficients.
  * rbo_internew: one tracer field q interpolated at the departure points into out
  * (internew, F:8339-8618).  Returns the number of points whose indices fall outside the
- * grid (0 for valid departure points; such points are left untouched). */
+ * grid (kg0 above nlat and the like; 0 for valid departure points; such points are left
+ * untouched). */
 void rbo_transport_weights(int nlon, int nlat, int niv, const double *aaa, const double *bbb,
                            const double *x0, const double *y0, const double *z0,
                            const double *p0, int32_t *idx, double *w);
 int64_t rbo_internew(int nlon, int nlat, int niv, const int32_t *idx, const double *w,
                      const double *q, double *out);
+
+/* One forcing record of the trajectory code (COMMON /forcm1/, /forcp1/, F:6197-6203):
+ * winds u, v (m/s) and omega w (Pa/s), temperature t (K) on the (nlon, nlat, niv) grid,
+ * log surface pressure lnps (ln Pa) on (nlon, nlat), and the record's day number. */
+typedef struct {
+    const double *u, *v, *w, *t, *lnps;
+    double day;
+} rbo_forcing;
+
+/* backtra (F:6156-6470): the departure points of every node of the transport grid over
+ * dtdyn seconds in nrapp sub-steps, the forcing interpolated in time between the records
+ * m1 and p1 (instant1/instant2, ntime and nintecmwf as the model's), two-pass midpoint
+ * trajectories in a frame rotated about the y axis poleward of 45 degrees.  Outputs the
+ * departure points x0 y0 (rad) z0 (Pa) p0 (surface pressure there, hPa), their cells and
+ * weights idx[3][n], w[12][n] as rbo_transport_weights gives them, and (trajlon non-NULL)
+ * the positions and temperatures along the trajectory, traj*[s][n] for slot s + 1 =
+ * 1..nrapp (slot nrapp: the arrival nodes).  Returns the number of point-sub-steps whose
+ * cell fell outside the grid on the way (0 for a sound flow; reads stay clamped inside the
+ * fields), or -1 for bad sizes. */
+int rbo_backtra(int nlon, int nlat, int niv, const double *aaa, const double *bbb,
+                const rbo_forcing *m1, const rbo_forcing *p1, double daynum, double dtdyn,
+                int nrapp, int ntime, int nintecmwf, double *x0, double *y0, double *z0,
+                double *p0, int32_t *idx, double *w, double *trajlon, double *trajlat,
+                double *trajpre, double *trajtem, int nthreads);
 
 /* ---- pieces exposed for unit tests ---- */
 /* Troe prefactors of const (F:5927-5948), 18 values in COMMON order. */

@@ -260,3 +260,42 @@ def run_transport_reference(tr) -> tuple:
         return idx, w, q
     finally:
         shutil.rmtree(run, ignore_errors=True)
+
+
+REF_BACKTRA_BIN = os.path.join(HERE, "_ref", "ref_backtra")
+
+
+def run_backtra_reference(ti, nrapp: int) -> dict:
+    """The reference's initransp and backtra (flang build) on synth.TrajectoryInputs of the
+    180 x 91 x 137 grid: departure points x0 y0 z0 p0, cells idx (3, n), weights w (12, n)
+    and the trajectory traj{lon,lat,pre,tem} (nrapp, 137, 91, 180)."""
+    if not os.path.exists(REF_BACKTRA_BIN):
+        raise RuntimeError(f"{REF_BACKTRA_BIN} not built (run `make -C oracle ref`)")
+    assert (ti.nlon, ti.nlat, ti.niv) == (180, 91, 137), "the reference's grid is fixed"
+    n = 180 * 91 * 137
+    base = os.path.join(HERE, "_ref")
+    run = tempfile.mkdtemp(prefix="bt_", dir=base)
+    try:
+        fin, fout = os.path.join(run, "in.bin"), os.path.join(run, "out.bin")
+        with open(fin, "wb") as f:
+            np.array([nrapp, ti.ntime, ti.nintecmwf], dtype=np.int32).tofile(f)
+            np.array([ti.daynum, ti.dtdyn, ti.m1.day, ti.p1.day], dtype=np.float64).tofile(f)
+            for a in (ti.aaa, ti.bbb):
+                np.ascontiguousarray(a, dtype=np.float64).tofile(f)
+            for rec in (ti.m1, ti.p1):
+                for a in (rec.u, rec.v, rec.w, rec.t, rec.lnps):
+                    np.ascontiguousarray(a, dtype=np.float64).tofile(f)
+        r = subprocess.run([REF_BACKTRA_BIN, fin, fout], cwd=run, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"ref_backtra failed ({r.returncode}): {r.stderr[-2000:]}")
+        out = {}
+        with open(fout, "rb") as f:
+            for k in ("x0", "y0", "z0", "p0"):
+                out[k] = np.fromfile(f, dtype=np.float64, count=n)
+            out["idx"] = np.fromfile(f, dtype=np.int32, count=3 * n).reshape(3, n)
+            out["w"] = np.fromfile(f, dtype=np.float64, count=12 * n).reshape(12, n)
+            for k in ("trajlon", "trajlat", "trajpre", "trajtem"):
+                out[k] = np.fromfile(f, dtype=np.float64, count=nrapp * n).reshape(nrapp, 137, 91, 180)
+        return out
+    finally:
+        shutil.rmtree(run, ignore_errors=True)

@@ -23,7 +23,8 @@ EXPORTS = ["rb_init", "rb_load_jtable", "rb_set_stream", "rb_chem_step", "rb_con
            "rb_chem_converge_dev",
            "rb_chem_newton_dev", "rb_chem_surrogate_dev", "rb_sediment",
            "rb_sediment_dev", "rb_zenith", "rb_zenith_dev", "rb_altitude", "rb_altitude_dev",
-           "rb_run_day_dev", "rb_transport_weights_dev", "rb_transport_interp_dev", "rb_transport",
+           "rb_run_day_dev", "rb_transport_backtra_dev", "rb_transport_weights_dev",
+           "rb_transport_interp_dev", "rb_transport",
            "rb_sync", "rb_get_stats", "rb_last_error", "rb_finalize"]
 # and include/rbmlp.h (the learned initial guess)
 MLP_EXPORTS = ["rb_mlp_create", "rb_mlp_destroy", "rb_mlp_set_stream", "rb_mlp_last_error",
@@ -38,6 +39,12 @@ IO_EXPORTS = ["rb_history_write", "rb_history_read", "rb_ecmwf_write_time", "rb_
 class Grid(C.Structure):
     _fields_ = [("nlon", C.c_int32), ("nlat", C.c_int32), ("niv", C.c_int32),
                 ("nivbas", C.c_int32), ("nivhau", C.c_int32)]
+
+
+class Forcing(C.Structure):
+    """rb_forcing: one forcing record of the trajectory code (device pointers)."""
+    _fields_ = [("u", C.c_void_p), ("v", C.c_void_p), ("w", C.c_void_p), ("t", C.c_void_p),
+                ("lnps", C.c_void_p), ("day", C.c_double)]
 
 
 class Config(C.Structure):
@@ -108,6 +115,9 @@ def load() -> C.CDLL:
     L.rb_altitude.argtypes = [P, C.POINTER(Grid), d, d] + [P] * 10
     L.rb_altitude_dev.argtypes = [P, C.POINTER(Grid), d, d] + [P] * 10
     L.rb_run_day_dev.argtypes = [P, C.POINTER(Grid), C.POINTER(DayConfig)] + [P] * 13
+    L.rb_transport_backtra_dev.argtypes = [P, C.POINTER(Grid), P, P, C.POINTER(Forcing),
+                                           C.POINTER(Forcing), d, d, C.c_int32, C.c_int32,
+                                           C.c_int32] + [P] * 10
     L.rb_transport_weights_dev.argtypes = [P, C.POINTER(Grid)] + [P] * 8
     L.rb_transport_interp_dev.argtypes = [P, C.POINTER(Grid), P, P, C.c_int32, P, P]
     L.rb_transport.argtypes = [P, C.POINTER(Grid)] + [P] * 6 + [C.c_int32, P]

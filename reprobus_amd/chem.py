@@ -238,6 +238,31 @@ class ChemistryStep:
                                             *[_ptr(a) for a in arrs]))
 
     # -- semi-Lagrangian transport interpolation -------------------------------------
+    def backtra_device(self, grid: Grid, aaa, bbb, m1: dict, p1: dict, daynum: float,
+                       dtdyn: float, nrapp: int, ntime: int, nintecmwf: int, x0, y0, z0, p0,
+                       idx, w, traj=None) -> None:
+        """Back-trajectories (backtra) of every node of the transport grid: m1, p1 are the
+        forcing records {"u", "v", "w", "t": (niv, nlat, nlon), "lnps": (nlat, nlon),
+        "day": float} as device tensors; outputs the departure points x0..p0 (n), cells idx
+        (3, n) int32 and weights w (12, n), and with traj = (lon, lat, pre, tem) tensors of
+        (nrapp, niv, nlat, nlon) the positions and temperatures along the way."""
+        n = _extents(grid)[0]
+        plane = int(grid.nlon) * int(grid.nlat)
+        niv = int(grid.niv)
+        recs = []
+        for name, r in (("m1", m1), ("p1", p1)):
+            recs.append(_lib.Forcing(*[_ptr(r[k], n, f"{name}.{k}") for k in ("u", "v", "w", "t")],
+                                     _ptr(r["lnps"], plane, f"{name}.lnps"), float(r["day"])))
+        tp = [None] * 4
+        if traj is not None:
+            tp = [_ptr(a, int(nrapp) * n, f"traj[{j}]") for j, a in enumerate(traj)]
+        self._check(self._L.rb_transport_backtra_dev(
+            self._ctx, C.byref(grid), _ptr(aaa, niv, "aaa"), _ptr(bbb, niv, "bbb"),
+            C.byref(recs[0]), C.byref(recs[1]), float(daynum), float(dtdyn), int(nrapp),
+            int(ntime), int(nintecmwf),
+            *[_ptr(a, n, w_) for a, w_ in ((x0, "x0"), (y0, "y0"), (z0, "z0"), (p0, "p0"))],
+            _iptr(idx, 3 * n, "idx"), _ptr(w, 12 * n, "w"), *tp))
+
     def transport_weights_device(self, grid: Grid, aaa, bbb, x0, y0, z0, p0, idx, w) -> None:
         """Cell indices idx (3, n) int32 and Lagrange weights w (12, n) of the departure
         points (device tensors; posx..poidz)."""

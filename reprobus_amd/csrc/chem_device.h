@@ -98,6 +98,39 @@ hipError_t launch_transport_interp(const TransportGrid &t, const int32_t *idx, c
                                    const double *q_in, int64_t sin_, double *q_out, int64_t sout,
                                    int ntr, unsigned long long *bad, hipStream_t s, int group = 0);
 
+// Back-trajectories (backtra): one forcing record on the device (winds and omega, T on
+// the grid; ln surface pressure on the columns)
+struct BtRecord {
+    const double *u, *v, *w, *t, *lnps;
+};
+// initransp's tables on the device: dlon (nlon + 3), dlat (nlat + 2), slon, clon (nlon),
+// slat (nlat), slonr, clonr, slatr, clatr (nlon * nlat; zero on the row where clat = 1)
+struct BtTables {
+    const double *dlon, *dlat, *slon, *clon, *slat, *slonr, *clonr, *slatr, *clatr;
+};
+struct BacktraArgs {
+    TransportGrid t;
+    const double *aaa, *bbb;
+    BtRecord m1, p1;
+    BtTables tb;
+    double cp1, cm1, coefh, coefv;      // this sub-step's time weight, wind scalings
+    double xpi, xpi2, dlatlim;
+    double *f;                          // u1s v1s w1s u1m v1m T (n each), ps (nlon * nlat)
+    double *x0, *y0, *z0, *p0;
+    int32_t *idx;
+    double *w;
+    double *traj[4];                    // lon lat pre tem: slot s at s * n, or all null
+    int iter, nrapp;
+    int64_t n;
+    unsigned long long *bad;            // points whose cell fell outside the grid
+};
+size_t backtra_table_size(int nlon, int nlat);
+// initransp's tables into h (host libm, as the model's), in BtTables order
+void backtra_tables(int nlon, int nlat, double *h);
+BtTables backtra_table_ptrs(int nlon, int nlat, const double *d);
+hipError_t launch_backtra_fields(const BacktraArgs &a, hipStream_t s);
+hipError_t launch_backtra_step(const BacktraArgs &a, hipStream_t s);
+
 Troe troe_constants();
 hipError_t launch_columns(const StepArgs &a, double *col, hipStream_t s);
 hipError_t launch_chem(const StepArgs &a, const double *rec, int block, hipStream_t s);

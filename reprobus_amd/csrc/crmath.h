@@ -7,7 +7,9 @@
 // reference links glibc, whose exp/log/pow/atan/cos/tanh are correctly rounded
 // on essentially every argument, so the aerosol block evaluates these functions
 // to ~100 bits here and rounds once: the step then reproduces the reference in
-// those boxes instead of amplifying the vendor library's last-bit errors.
+// those boxes instead of amplifying the vendor library's last-bit errors.  The
+// back-trajectories (backtra, F:6156-6470) use sin, cos, asin, acos and exp the
+// same way, so that the device's departure points are the reference's.
 //
 // Every routine uses only IEEE add/mul/div/sqrt and fma, so the host build
 // (used by the CPU tests) and the gfx950 build return identical bits.
@@ -255,6 +257,44 @@ CR_HD double cos_cr(double x)
     dd s, c;
     sincos_dd({x, 0.}, s, c);
     return c.hi + c.lo;
+}
+
+CR_HD double sin_cr(double x)
+{
+    if (x != x || x == INFINITY || x == -INFINITY || fabs(x) > 100. || x == 0.) return sin(x);
+    dd s, c;
+    sincos_dd({x, 0.}, s, c);
+    return s.hi + s.lo;
+}
+
+// asin: one Newton step on sin(y) = x from the library's value,
+// y = y0 + (x - sin(y0)) / cos(y0); asin(+-1) = +-pi/2 rounded.
+CR_HD double asin_cr(double x)
+{
+    if (x != x || x == 0.) return x;
+    if (x == 1.) return PIO2_HI;
+    if (x == -1.) return -PIO2_HI;
+    if (fabs(x) > 1.) return asin(x);
+    const double y0 = asin(x);
+    dd s, c;
+    sincos_dd({y0, 0.}, s, c);
+    const dd y = add_d(div(add_d(neg(s), x), c), y0);
+    return y.hi + y.lo;
+}
+
+// acos: one Newton step on cos(y) = x from the library's value,
+// y = y0 + (cos(y0) - x) / sin(y0); acos(1) = 0, acos(-1) = pi rounded.
+CR_HD double acos_cr(double x)
+{
+    if (x != x) return x;
+    if (x == 1.) return 0.;
+    if (x == -1.) return 2. * PIO2_HI;
+    if (fabs(x) > 1.) return acos(x);
+    const double y0 = acos(x);
+    dd s, c;
+    sincos_dd({y0, 0.}, s, c);
+    const dd y = add_d(div(add_d(c, -x), s), y0);
+    return y.hi + y.lo;
 }
 
 // atan: one Newton step on tan(y) = x from the library's value,

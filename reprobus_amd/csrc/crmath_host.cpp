@@ -11,5 +11,8 @@ double rb_cr_pow(double x, double y) { return crm::pow_cr(x, y); }
 double rb_cr_pow10(double y) { return crm::pow10_cr(y); }
 double rb_cr_atan(double x) { return crm::atan_cr(x); }
 double rb_cr_cos(double x) { return crm::cos_cr(x); }
+double rb_cr_sin(double x) { return crm::sin_cr(x); }
+double rb_cr_asin(double x) { return crm::asin_cr(x); }
+double rb_cr_acos(double x) { return crm::acos_cr(x); }
 double rb_cr_tanh(double x) { return crm::tanh_cr(x); }
 }

@@ -41,6 +41,10 @@ struct rb_ctx {
     size_t tw_cap = 0;                       // positions, levels
     double *d_tq = nullptr;                  // transport (host entry): fields in and out
     size_t tq_cap = 0;
+    double *d_bt = nullptr;                  // back-trajectories: fields of a sub-step, tables
+    size_t bt_cap = 0;
+    std::vector<double> bt_tables;           // host tables of the grid bt_grid
+    int bt_grid[2] = {0, 0};
     int n_cu = 0;
     double *noalloc = nullptr;               // this device's copy of `scratch` (rb_init)
     bool capturing = false;                  // inside a graph capture: no event records
@@ -794,6 +798,94 @@ int rb_transport_interp_dev(rb_ctx *ctx, const rb_grid *g, const int32_t *idx, c
     return transport_outside(ctx, "rb_transport_interp");
 }
 
+int rb_transport_backtra_dev(rb_ctx *ctx, const rb_grid *g, const double *aaa, const double *bbb,
+                             const rb_forcing *m1, const rb_forcing *p1, double daynum,
+                             double dtdyn, int32_t nrapp, int32_t ntime, int32_t nintecmwf,
+                             double *x0, double *y0, double *z0, double *p0, int32_t *idx,
+                             double *w, double *trajlon, double *trajlat, double *trajpre,
+                             double *trajtem)
+{
+    if (!ctx) return RB_EINVAL;
+    int rc = check_transport(ctx, g);
+    if (rc) return rc;
+    if (g->nlat < 5) return fail(ctx, RB_EINVAL, "backtra needs nlat >= 5");
+    if (!m1 || !p1) return fail(ctx, RB_EINVAL, "a forcing record is NULL");
+    if (any_null({aaa, bbb, x0, y0, z0, p0, w, m1->u, m1->v, m1->w, m1->t, m1->lnps, p1->u, p1->v,
+                  p1->w, p1->t, p1->lnps}) || !idx)
+        return fail(ctx, RB_EINVAL, "a device array is NULL");
+    const bool traj = trajlon || trajlat || trajpre || trajtem;
+    if (traj && any_null({trajlon, trajlat, trajpre, trajtem}))
+        return fail(ctx, RB_EINVAL, "the trajectory arrays must be all set or all NULL");
+    if (nrapp < 1 || ntime < 1 || nintecmwf < 1)
+        return fail(ctx, RB_EINVAL, "nrapp, ntime and nintecmwf must be >= 1");
+    if (!(dtdyn > 0.) || !std::isfinite(dtdyn) || !std::isfinite(daynum) ||
+        !std::isfinite(m1->day) || !std::isfinite(p1->day))
+        return fail(ctx, RB_EINVAL, "dtdyn must be positive; days finite");
+    HIPCHK(hipSetDevice(ctx->device));
+    const int64_t n = (int64_t)g->nlon * g->nlat * g->niv, plane = (int64_t)g->nlon * g->nlat;
+    const size_t ntab = rb::backtra_table_size(g->nlon, g->nlat);
+    // the fields of a sub-step (6 n + plane) and the grid's tables
+    if ((rc = ensure(ctx, &ctx->d_bt, &ctx->bt_cap, 6 * (size_t)n + (size_t)plane + ntab)))
+        return rc;
+    double *dtab = ctx->d_bt + 6 * (size_t)n + (size_t)plane;
+    if (ctx->bt_grid[0] != g->nlon || ctx->bt_grid[1] != g->nlat) {
+        ctx->bt_tables.assign(ntab, 0.);
+        rb::backtra_tables(g->nlon, g->nlat, ctx->bt_tables.data());
+        ctx->bt_grid[0] = g->nlon;
+        ctx->bt_grid[1] = g->nlat;
+    }
+    // the tables are rewritten on every call: the buffer may have grown since
+    HIPCHK(hipMemcpyAsync(dtab, ctx->bt_tables.data(), ntab * sizeof(double),
+                          hipMemcpyHostToDevice, ctx->stream));
+    rb::BacktraArgs a{};
+    a.t = rb::transport_grid(g->nlon, g->nlat, g->niv);
+    a.aaa = aaa;
+    a.bbb = bbb;
+    a.m1 = {m1->u, m1->v, m1->w, m1->t, m1->lnps};
+    a.p1 = {p1->u, p1->v, p1->w, p1->t, p1->lnps};
+    a.tb = rb::backtra_table_ptrs(g->nlon, g->nlat, dtab);
+    // cstes (F:6010-6025) and dlatlim (F:6101)
+    a.xpi = 2. * asin(1.);
+    a.xpi2 = 2. * a.xpi;
+    a.dlatlim = 45. / (180. / a.xpi);
+    a.f = ctx->d_bt;
+    a.x0 = x0;
+    a.y0 = y0;
+    a.z0 = z0;
+    a.p0 = p0;
+    a.idx = idx;
+    a.w = w;
+    if (traj) {
+        a.traj[0] = trajlon;
+        a.traj[1] = trajlat;
+        a.traj[2] = trajpre;
+        a.traj[3] = trajtem;
+    }
+    a.nrapp = nrapp;
+    a.n = n;
+    a.bad = ctx->d_count;
+    // dtn, the sub-step (F:6237); inivit's scalings (F:6594-6595)
+    const double dtn = dtdyn / (double)nrapp;
+    a.coefh = dtn / 6371229.;
+    a.coefv = dtn;
+    HIPCHK(rb::launch_zero_counter(ctx->d_count, ctx->stream));
+    for (int iter = 1; iter <= nrapp; ++iter) {
+        // F:6245 and instant1/instant2's time weight (F:6495-6504)
+        const double daypre = daynum - dtn * (double)(iter - 1) / 86400.;
+        double day = daypre;
+        if (day < 1.e-5) day = day + std::trunc(m1->day) + 1.;
+        double cp1 = (day - m1->day) / ((double)nintecmwf / (double)(24 * ntime));
+        cp1 = std::fmin(cp1, 1.);
+        cp1 = std::fmax(cp1, 0.);
+        a.cp1 = cp1;
+        a.cm1 = 1. - cp1;
+        a.iter = iter;
+        HIPCHK(rb::launch_backtra_fields(a, ctx->stream));
+        HIPCHK(rb::launch_backtra_step(a, ctx->stream));
+    }
+    return transport_outside(ctx, "rb_transport_backtra");
+}
+
 int rb_transport(rb_ctx *ctx, const rb_grid *g, const double *aaa, const double *bbb,
                  const double *x0, const double *y0, const double *z0, const double *p0,
                  int32_t ntr, double *qj1)
@@ -1013,6 +1105,7 @@ int rb_finalize(rb_ctx *ctx)
     if (ctx->d_spill) (void)hipFree(ctx->d_spill);
     if (ctx->d_tw) (void)hipFree(ctx->d_tw);
     if (ctx->d_tq) (void)hipFree(ctx->d_tq);
+    if (ctx->d_bt) (void)hipFree(ctx->d_bt);
     if (ctx->day_exec) (void)hipGraphExecDestroy(ctx->day_exec);
     if (ctx->day_graph) (void)hipGraphDestroy(ctx->day_graph);
     for (hipEvent_t e : {ctx->ev0, ctx->ev1, ctx->ev_h2d, ctx->ev_d2h})

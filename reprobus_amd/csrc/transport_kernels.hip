@@ -14,6 +14,7 @@
 #include <cstdint>
 
 #include "chem_device.h"
+#include "transport_device.h"
 
 namespace rb {
 
@@ -28,53 +29,14 @@ __global__ void k_transport_weights(TransportGrid t, const double *__restrict__ 
 {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
-    // posx F:7402-7406, posy F:7442-7446
-    const double xx = x0[p], yy = y0[p];
-    const int ix = (int)(xx / t.ddlon);
-    const double dx = xx - (double)ix * t.ddlon;
-    const int iy = (int)((t.xpih - yy) / t.ddlat);
-    const double dy = t.xpih - (double)iy * t.ddlat - yy;
-    // posz F:7485-7494: the last level l in 2..niv-1 whose pressure is <= zz
-    const double zz = z0[p], pp = p0[p];
-    int ll = 1;
-    for (int l = 2; l <= t.niv - 1; ++l) {
-        const double pres = (aaa[l - 1] + bbb[l - 1] * pp) * 100.;
-        if (zz - pres >= 0.) ll = l;
-    }
-    idx[p] = ix + 1;
-    idx[n + p] = iy + 1;
-    idx[2 * n + p] = ll;
-    if (!(ix + 1 >= 1 && ix + 1 <= t.nlon && iy + 1 >= 1 && iy + 1 <= t.nlat - 1)) atomicAdd(bad, 1ull);
-    // poidx F:7529-7536
-    {
-        const double dxm1 = dx + t.ddlon, dxp0 = dx, dxp1 = dx - t.ddlon, dxp2 = dx - t.ddlon * 2.;
-        w[p] = t.xcoef[0] * dxp0 * dxp1 * dxp2;
-        w[n + p] = t.xcoef[1] * dxm1 * dxp1 * dxp2;
-        w[2 * n + p] = t.xcoef[2] * dxm1 * dxp0 * dxp2;
-        w[3 * n + p] = t.xcoef[3] * dxm1 * dxp0 * dxp1;
-    }
-    // poidy F:7571-7578
-    {
-        const double dym1 = dy + t.ddlat, dyp0 = dy, dyp1 = dy - t.ddlat, dyp2 = dy - t.ddlat * 2.;
-        w[4 * n + p] = t.ycoef[0] * dyp0 * dyp1 * dyp2;
-        w[5 * n + p] = t.ycoef[1] * dym1 * dyp1 * dyp2;
-        w[6 * n + p] = t.ycoef[2] * dym1 * dyp0 * dyp2;
-        w[7 * n + p] = t.ycoef[3] * dym1 * dyp0 * dyp1;
-    }
-    // poidz F:7614-7644
-    {
-        const double prem1 = ll > 1 ? (aaa[ll - 2] + bbb[ll - 2] * pp) * 100. : 5.e-3 * 100.;
-        const double prep0 = (aaa[ll - 1] + bbb[ll - 1] * pp) * 100.;
-        const double prep1 = (aaa[ll] + bbb[ll] * pp) * 100.;
-        const double prep2 = ll < t.niv - 1 ? (aaa[ll + 1] + bbb[ll + 1] * pp) * 100. : 1200. * 100.;
-        const double dzm1 = zz - prem1, dzp0 = zz - prep0, dzp1 = zz - prep1, dzp2 = zz - prep2;
-        const double dzp2m1 = prep2 - prem1, dzp2p0 = prep2 - prep0, dzp2p1 = prep2 - prep1;
-        const double dzp1m1 = prep1 - prem1, dzp1p0 = prep1 - prep0, dzp0m1 = prep0 - prem1;
-        w[8 * n + p] = -(dzp0 * dzp1 * dzp2) / (dzp0m1 * dzp1m1 * dzp2m1);
-        w[9 * n + p] = (dzm1 * dzp1 * dzp2) / (dzp0m1 * dzp1p0 * dzp2p0);
-        w[10 * n + p] = -(dzm1 * dzp0 * dzp2) / (dzp1m1 * dzp1p0 * dzp2p1);
-        w[11 * n + p] = (dzm1 * dzp0 * dzp1) / (dzp2m1 * dzp2p0 * dzp2p1);
-    }
+    int c[3];
+    double wt[12];
+    cell_weights(t, aaa, bbb, x0[p], y0[p], z0[p], p0[p], true, c, wt);
+#pragma unroll
+    for (int m = 0; m < 3; ++m) idx[m * n + p] = c[m];
+#pragma unroll
+    for (int m = 0; m < 12; ++m) w[m * n + p] = wt[m];
+    if (!(c[0] >= 1 && c[0] <= t.nlon && c[1] >= 1 && c[1] <= t.nlat)) atomicAdd(bad, 1ull);
 }
 
 // Two doubles at an 8-byte aligned address: one global_load_dwordx4 (multi-dword global
@@ -104,7 +66,10 @@ __global__ void __launch_bounds__(256) k_transport_interp(TransportGrid t, const
     const int64_t p = tile * blockDim.x + threadIdx.x;
     if (p >= n) return;
     const int i = idx[p], k = idx[n + p], l = idx[2 * n + p];
-    if (!(i >= 1 && i <= t.nlon && k >= 1 && k <= t.nlat - 1 && l >= 1 && l <= t.niv)) {
+    // kg0 = nlat: a point on the south-pole row.  Its stencil reaches one row past the
+    // reference's extended grid (x(:, 0:nlat+1)), with weight 0 when the point is on the
+    // row; that row is read as row nlat + 1 here.
+    if (!(i >= 1 && i <= t.nlon && k >= 1 && k <= t.nlat && l >= 1 && l <= t.niv)) {
         if (blockIdx.y == 0) atomicAdd(bad, 1ull);
         return;
     }
@@ -124,7 +89,7 @@ __global__ void __launch_bounds__(256) k_transport_interp(TransportGrid t, const
     int32_t po[4][4];   // [lat offset][lon offset] -> offset in a level plane
 #pragma unroll
     for (int b = 0; b < 4; ++b) {
-        const int kk = k + b - 1;
+        const int kk = k + b - 1 > t.nlat + 1 ? t.nlat + 1 : k + b - 1;
 #pragma unroll
         for (int a = 0; a < 4; ++a) {
             int ii = i + a - 1;

@@ -375,3 +375,66 @@ def make_transport(seed: int = 0, nlon: int = 180, nlat: int = 91, niv: int = 13
     c = lambda a: np.ascontiguousarray(a, dtype=np.float64)
     return TransportInputs(nlon, nlat, niv, c(aaa), c(bbb), c(x0), c(y0), c(z0), c(p0), c(q),
                            np.asarray(jc, dtype=np.int32))
+
+
+# ------------------------------------------------------------------ trajectory forcing
+@dataclass
+class ForcingRecord:
+    """One forcing record of the trajectory code: winds u, v (m/s), omega w (Pa/s) and
+    temperature t (K), numpy order (niv, nlat, nlon); ln of surface pressure (Pa), (nlat,
+    nlon); the record's day number."""
+    u: np.ndarray
+    v: np.ndarray
+    w: np.ndarray
+    t: np.ndarray
+    lnps: np.ndarray
+    day: float
+
+
+@dataclass
+class TrajectoryInputs:
+    """The inputs of backtra: the grid, the hybrid coefficients, two forcing records
+    around the step, and the step's timing (daynum at its end, dtdyn seconds in nrapp
+    sub-steps; records every nintecmwf / ntime hours, as the model's instant1)."""
+    nlon: int
+    nlat: int
+    niv: int
+    aaa: np.ndarray
+    bbb: np.ndarray
+    m1: ForcingRecord
+    p1: ForcingRecord
+    daynum: float
+    dtdyn: float
+    ntime: int
+    nintecmwf: int
+
+
+def make_trajectory_inputs(seed: int = 0, nlon: int = 180, nlat: int = 91, niv: int = 137,
+                           daynum: float = 100.1, dtdyn: float = 3600.0) -> TrajectoryInputs:
+    """Two forcing records 6 h apart (days 100.0 and 100.25) of a smooth synthetic flow: a
+    zonal jet with waves that crosses both poles (so the rotated-frame branch sees strong
+    winds), a meridional wobble, omega proportional to pressure, temperature falling with
+    height; the second record is the first advected a little, with noise."""
+    rng = np.random.default_rng(seed)
+    xpi = 2.0 * np.arcsin(1.0)
+    lon = 2 * xpi / nlon * np.arange(nlon)
+    lat = xpi / 2 - xpi / (nlat - 1) * np.arange(nlat)
+    aaa, bbb = transport_levels(niv)
+    L, LA, LO = np.meshgrid(np.arange(niv), lat, lon, indexing="ij")
+    c = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+
+    def record(phase, day):
+        psol = 1000.0 + 30.0 * np.cos(2 * LA[0]) * np.sin(LO[0] + phase) \
+            + rng.normal(0, 2.0, LO[0].shape)
+        pres = (aaa[:, None, None] + bbb[:, None, None] * psol[None]) * 100.0
+        u = (30.0 + 20.0 * np.cos(3 * LO + phase)) * np.cos(LA) ** 2 * (1 + 0.5 * np.sin(L / 9.0)) \
+            + 15.0 * np.sin(LA) ** 2 * np.cos(LO) + rng.normal(0, 2.0, LO.shape)
+        v = 8.0 * np.sin(2 * LA) * np.cos(2 * LO + phase) - 15.0 * np.sin(LA) * np.sin(LO) \
+            + rng.normal(0, 1.0, LO.shape)
+        w = pres * 2e-6 * (np.sin(LO + LA + phase) + rng.normal(0, 0.1, LO.shape))
+        t = 190.0 + 100.0 * (L + 0.5) / niv + 10.0 * np.cos(LA) * np.cos(LO + phase) \
+            + rng.normal(0, 1.0, LO.shape)
+        return ForcingRecord(c(u), c(v), c(w), c(t), c(np.log(psol * 100.0)), day)
+
+    return TrajectoryInputs(nlon, nlat, niv, c(aaa), c(bbb), record(0.0, 100.0),
+                            record(0.2, 100.25), daynum, dtdyn, 1, 6)

@@ -22,7 +22,7 @@ def libs():
         pytest.fail("build first: __graft_entry__.build()")
     H, Q = C.CDLL(HOST), C.CDLL(QUAD)
     for L, pre in ((H, "rb_cr_"), (Q, "qr_")):
-        for n in ("exp", "log", "log10", "atan", "cos", "tanh"):
+        for n in ("exp", "log", "log10", "atan", "cos", "tanh", "sin", "asin", "acos"):
             f = getattr(L, pre + n)
             f.argtypes, f.restype = [C.c_double], C.c_double
         f = getattr(L, pre + "pow")
@@ -40,6 +40,12 @@ CASES = [
     ("atan", lambda r, n: r.standard_normal(n) * np.exp(r.uniform(-12, 12, n))),
     ("cos", lambda r, n: r.uniform(-7, 7, n)),
     ("tanh", lambda r, n: np.exp(r.uniform(-30, 4, n))),
+    ("sin", lambda r, n: r.uniform(-7, 7, n)),
+    ("sin", lambda r, n: r.standard_normal(n) * np.exp(r.uniform(-30, 0, n))),
+    ("asin", lambda r, n: r.uniform(-1, 1, n)),
+    ("asin", lambda r, n: np.sign(r.standard_normal(n)) * (1 - np.exp(r.uniform(-36, -1, n)))),
+    ("acos", lambda r, n: r.uniform(-1, 1, n)),
+    ("acos", lambda r, n: np.sign(r.standard_normal(n)) * (1 - np.exp(r.uniform(-36, -1, n)))),
 ]
 
 

@@ -115,7 +115,7 @@ def test_departure_points_outside_the_grid_are_rejected(step):
         step.transport_weights_device(grid, *[_dev(a) for a in (tr.aaa, tr.bbb, x0, tr.y0, tr.z0, tr.p0)],
                                       idx, w)
     idx_c, w_c = cport.transport_weights(tr)
-    idx_c[1, 7] = 9                     # kg0 = nlat: the stencil would leave the grid
+    idx_c[1, 7] = 10                    # kg0 = nlat + 1: the stencil would leave the grid
     q_in = _dev(tr.q)
     q_out = torch.zeros_like(q_in)
     with pytest.raises(RbError):

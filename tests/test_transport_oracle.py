@@ -185,6 +185,84 @@ def test_points_outside_the_grid_are_counted():
     tr = synth.make_transport(6, nlon=16, nlat=9, niv=7)
     idx, w = cport.transport_weights(tr)
     idx[0, 3] = 17
-    idx[1, 5] = 9
+    idx[1, 5] = 10                     # kg0 = nlat + 1 (nlat, the south-pole row, is inside)
     out, bad = cport.internew(tr, idx, w, tr.q[0])
     assert bad == 2
+
+
+# ---------------------------------------------------------------- back-trajectories
+@pytest.fixture(scope="module")
+def traj0():
+    return synth.make_trajectory_inputs(0)
+
+
+@pytest.mark.parametrize("nrapp", [1, 3])
+def test_backtra_restatement_matches_reference_build(traj0, nrapp):
+    """rbo_backtra against the reference's backtra (flang build, oracle/ref_backtra.f90) on
+    the 2,244,060-node grid: departure points, cells, weights and every trajectory slot bit
+    for bit (nrapp 1: the first sub-step's node winds; 3: two more with interpolated
+    winds at the first guess)."""
+    ref = refrun.run_backtra_reference(traj0, nrapp)
+    res = cport.backtra(traj0, nrapp)
+    for k in ("x0", "y0", "z0", "p0", "idx", "w", "trajlon", "trajlat", "trajpre", "trajtem"):
+        assert np.array_equal(res[k], ref[k]), k
+    # half the nodes are poleward of 45 degrees: the rotated frame is exercised
+    assert 0.4 < float((np.abs(ref["y0"]) >= np.pi / 4).mean()) < 0.6
+
+
+def _uniform_forcing(ti, u_of_lat):
+    """ti with u = u_of_lat(latitude), v = w = 0, the same record at both times."""
+    niv, nlat, nlon = ti.niv, ti.nlat, ti.nlon
+    lat = np.pi / 2 - np.pi / (nlat - 1) * np.arange(nlat)
+    u = np.broadcast_to(u_of_lat(lat)[None, :, None], (niv, nlat, nlon)).copy()
+    z = np.zeros((niv, nlat, nlon))
+    rec = synth.ForcingRecord(u, z, z.copy(), ti.m1.t, ti.m1.lnps, ti.m1.day)
+    return synth.TrajectoryInputs(nlon, nlat, niv, ti.aaa, ti.bbb, rec,
+                                  synth.ForcingRecord(u, z, z.copy(), ti.m1.t, ti.m1.lnps,
+                                                      ti.p1.day), ti.daynum, ti.dtdyn,
+                                  ti.ntime, ti.nintecmwf)
+
+
+def backtra_closed_forms(run, nrapp=3):
+    """Still air: every departure point is its node, at the node's pressure, with the node's
+    temperature at every slot.  Solid-body rotation about the polar axis (u = U cos(lat),
+    v = w = 0): equatorward of 45 degrees, where the trajectories stay in the geographic
+    frame, each node's air left U dtdyn / R radians of longitude upstream, on its own
+    latitude and level."""
+    ti = synth.make_trajectory_inputs(5, nlon=36, nlat=19, niv=12)
+    n = ti.nlon * ti.nlat * ti.niv
+    xpi = 2 * np.arcsin(1.0)
+    lon = 2 * xpi / ti.nlon * np.arange(ti.nlon)
+    lat = xpi / 2 - xpi / (ti.nlat - 1) * np.arange(ti.nlat)
+    L, LA, LO = np.meshgrid(np.arange(ti.niv), lat, lon, indexing="ij")
+    still = run(_uniform_forcing(ti, lambda la: 0.0 * la), nrapp)
+    assert still.get("outside", 0) == 0       # the south-pole row (kg0 = nlat) is inside
+    dx = np.abs(np.angle(np.exp(1j * (still["x0"] - LO.ravel()))))
+    off_pole = np.abs(LA.ravel()) < xpi / 2 - 1e-9
+    low = np.abs(LA.ravel()) < xpi / 4 - 1e-9
+    assert dx[low].max() < 1e-13
+    # the rotated frame's round trip takes acos near 1: sqrt(2 ulp) ~ 2e-8 rad of longitude
+    assert dx[off_pole].max() < 2e-7
+    assert np.abs(still["y0"] - LA.ravel()).max() < 1e-12
+    # (at a pole every longitude is the same point, and the synthetic T still varies with
+    # longitude there: the pole rows are left out of the temperature check)
+    t_nodes = ti.m1.t.ravel()
+    for s in range(nrapp):
+        tem = still["trajtem"][s].ravel()
+        assert np.abs(tem - t_nodes)[low].max() < 1e-12 * t_nodes.max()
+        assert np.abs(tem - t_nodes)[off_pole].max() < 1e-8 * t_nodes.max()
+        assert np.abs(still["trajlat"][s] - np.degrees(LA)).max() < 1e-10
+    U = 40.0
+    rot = run(_uniform_forcing(ti, lambda la: U * np.cos(la)), nrapp)
+    shift = U * ti.dtdyn / 6371229.0
+    want = np.mod(LO.ravel() - shift, 2 * xpi)
+    err = np.abs(np.angle(np.exp(1j * (rot["x0"] - want))))
+    assert err[low].max() < 1e-12
+    assert np.abs(rot["y0"] - LA.ravel())[low].max() < 1e-13
+    # poleward, the rotated-frame midpoint scheme follows the small circle to second order
+    assert err[~low & off_pole].max() < 1e-4 * shift
+    return n
+
+
+def test_backtra_restatement_closed_forms():
+    backtra_closed_forms(lambda ti, nr: cport.backtra(ti, nr))
