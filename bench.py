@@ -553,7 +553,26 @@ def transport_section(table, dev, ntr=43, reps=5):
     q_out = torch.empty_like(q_in)
     w_ms = _timed(lambda: step.transport_weights_device(grid, *pos, idx, w), reps, stream)
     i_ms = _timed(lambda: step.transport_interp_device(grid, idx, w, q_in, q_out), reps, stream)
+    # the back-trajectories of one dynamical step (1 h in 4 sub-steps, as semilag calls
+    # backtra) from two synthetic forcing records, with the trajectory slots
+    ti = synth.make_trajectory_inputs(0)
+    nrapp = 4
+    rec = lambda r: {"u": t(r.u), "v": t(r.v), "w": t(r.w), "t": t(r.t), "lnps": t(r.lnps),
+                     "day": r.day}
+    m1, p1 = rec(ti.m1), rec(ti.p1)
+    dep = [torch.empty(n, dtype=torch.float64, device=dev) for _ in range(4)]
+    trj = [torch.empty((nrapp, tr.niv, tr.nlat, tr.nlon), dtype=torch.float64, device=dev)
+           for _ in range(4)]
+    bt = lambda: step.backtra_device(grid, pos[0], pos[1], m1, p1, ti.daynum, ti.dtdyn, nrapp,
+                                     ti.ntime, ti.nintecmwf, *dep, idx, w, traj=trj)
+    bt()
+    b_ms = _timed(bt, reps, stream)
+    del m1, p1, dep, trj
     step.close()
+    # per point and sub-step: the fields pass reads 8 forcing values and writes 6 fields; the
+    # trajectory pass reads the 4 position values (after the first sub-step) and writes 4
+    # positions, 3 indices, 12 weights and 4 trajectory values
+    b_backtra = n * nrapp * ((8 + 6) * 8 + (4 + 4 + 12 + 4) * 8 + 3 * 4)
     b_interp = n * (3 * 4 + 12 * 8 + ntr * 2 * 8)
     b_weights = n * (4 * 8 + 3 * 4 + 12 * 8)
     gbs = lambda b, ms: b / (ms / 1e3) / 1e9
@@ -561,6 +580,13 @@ def transport_section(table, dev, ntr=43, reps=5):
                         "a synthetic 1 h wind; inputs resident in HBM",
             "weights_ms": w_ms, "interp_ms": i_ms, "ms": w_ms + i_ms,
             "point_tracers_per_s": n * ntr / ((w_ms + i_ms) / 1e3),
+            "backtra": {"workload": f"back-trajectories of every node over 1 h in {nrapp} "
+                                    "sub-steps between two synthetic 6-hourly forcing records, "
+                                    "with the trajectory slots",
+                        "ms": b_ms, "nrapp": nrapp, "point_substeps_per_s": n * nrapp / (b_ms / 1e3),
+                        "bytes": b_backtra, "achieved_gbs": gbs(b_backtra, b_ms),
+                        "frac": gbs(b_backtra, b_ms) / HBM_PEAK_GBS},
+            "semilag_ms": b_ms + i_ms,
             "roofline": {"bound": "hbm", "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "interp_achieved": gbs(b_interp, i_ms),
                          "interp_frac": gbs(b_interp, i_ms) / HBM_PEAK_GBS,

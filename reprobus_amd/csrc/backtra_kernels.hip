@@ -136,18 +136,64 @@ __device__ __forceinline__ double corniv(const BacktraArgs &a, double z, double 
     return fmin(pmax, fmax(pmin, z));
 }
 
-// the three winds at (xx, yy, zz) in a column of surface pressure pp (interv)
+// the three winds at (xx, yy, zz) in a column of surface pressure pp (interv): the same
+// sums as three tricubic<false> calls, with each stencil node's index worked out once
 __device__ void winds_at(const BacktraArgs &a, const double *ue, const double *ve, double sg,
                          double xx, double yy, double zz, double pp, double &u2, double &v2,
                          double &w2, bool &outside)
 {
+    const TransportGrid &t = a.t;
     int c[3];
     double w[12];
-    cell_weights(a.t, a.aaa, a.bbb, xx, yy, zz, pp, true, c, w);
-    outside |= !(c[0] >= 1 && c[0] <= a.t.nlon && c[1] >= 1 && c[1] <= a.t.nlat);
-    u2 = tricubic<false>(ue, a.t, sg, c, w);
-    v2 = tricubic<false>(ve, a.t, sg, c, w);
-    w2 = tricubic<false>(a.f + F_W1S * a.n, a.t, 1., c, w);
+    cell_weights(t, a.aaa, a.bbb, xx, yy, zz, pp, true, c, w);
+    outside |= !(c[0] >= 1 && c[0] <= t.nlon && c[1] >= 1 && c[1] <= t.nlat);
+    const double *we = a.f + F_W1S * a.n;
+    // the stencil's 4 x 4 columns of a level: offsets in a level plane and the mirror flag
+    int32_t po[4][4];
+    bool mir[4];
+    const int i0 = c[0] < 1 ? 1 : (c[0] > t.nlon ? t.nlon : c[0]);
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+        int k = c[1] + b - 1;
+        k = k < 0 ? 0 : (k > t.nlat + 1 ? t.nlat + 1 : k);
+        mir[b] = k == 0 || k == t.nlat + 1;
+        const int kk = k == 0 ? 2 : (k == t.nlat + 1 ? t.nlat - 1 : k);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int i = i0 + e - 1;
+            int ii = i == 0 ? t.nlon : (i > t.nlon ? i - t.nlon : i);
+            if (mir[b]) ii = (ii + t.nlon / 2 - 1) % t.nlon + 1;     // ip(i), F:6089-6091
+            po[b][e] = (kk - 1) * t.nlon + (ii - 1);
+        }
+    }
+    const int64_t plane = (int64_t)t.nlon * t.nlat;
+    double zu[4], zv[4], zw[4];
+#pragma unroll
+    for (int ls = 0; ls < 4; ++ls) {
+        const int l = c[2] + ls - 1;
+        const int64_t lo = (int64_t)((l < 1 ? 1 : (l > t.niv ? t.niv : l)) - 1) * plane;
+        double yu[4], yv[4], yw[4];
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            double xu[4], xv[4], xw[4];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int64_t o = lo + po[b][e];
+                xu[e] = mir[b] ? sg * ue[o] : ue[o];
+                xv[e] = mir[b] ? sg * ve[o] : ve[o];
+                xw[e] = we[o];
+            }
+            yu[b] = w[0] * xu[0] + w[1] * xu[1] + w[2] * xu[2] + w[3] * xu[3];
+            yv[b] = w[0] * xv[0] + w[1] * xv[1] + w[2] * xv[2] + w[3] * xv[3];
+            yw[b] = w[0] * xw[0] + w[1] * xw[1] + w[2] * xw[2] + w[3] * xw[3];
+        }
+        zu[ls] = w[4] * yu[0] + w[5] * yu[1] + w[6] * yu[2] + w[7] * yu[3];
+        zv[ls] = w[4] * yv[0] + w[5] * yv[1] + w[6] * yv[2] + w[7] * yv[3];
+        zw[ls] = w[4] * yw[0] + w[5] * yw[1] + w[6] * yw[2] + w[7] * yw[3];
+    }
+    u2 = w[8] * zu[0] + w[9] * zu[1] + w[10] * zu[2] + w[11] * zu[3];
+    v2 = w[8] * zv[0] + w[9] * zv[1] + w[10] * zv[2] + w[11] * zv[3];
+    w2 = w[8] * zw[0] + w[9] * zw[1] + w[10] * zw[2] + w[11] * zw[3];
 }
 
 __global__ void __launch_bounds__(256) k_backtra_fields(BacktraArgs a)
@@ -223,13 +269,16 @@ __global__ void __launch_bounds__(256) k_backtra_step(BacktraArgs a, int ntiles)
     // the rotated frame (F:6298-6315)
     double xm = x0, ym = y0, zm = z0, pm = p0;
     const bool ind = fabs(ym) >= a.dlatlim;
-    double x0r = 0., y0r = 0.;
+    double x0r = 0., y0r = 0., sy0r = 0., cy0r = 0.;
     if (ind) {
-        const double sy0r = -crm::cos_cr(x0) * crm::cos_cr(y0);
+        double sx, cx, sy, cy;
+        crm::sincos_cr(x0, sx, cx);
+        crm::sincos_cr(y0, sy, cy);
+        sy0r = -cx * cy;
         y0r = crm::asin_cr(sy0r);
-        const double cy0r = sqrt(1. - sy0r * sy0r);
-        const double cx0r = cy0r == 0. ? 0. : fmax(-1., fmin(1., crm::sin_cr(y0) / cy0r));
-        x0r = copysign(crm::acos_cr(cx0r), crm::sin_cr(x0));
+        cy0r = sqrt(1. - sy0r * sy0r);
+        const double cx0r = cy0r == 0. ? 0. : fmax(-1., fmin(1., sy / cy0r));
+        x0r = copysign(crm::acos_cr(cx0r), sx);
     }
     const double *ue = a.f + (ind ? F_U1M : F_U1S) * a.n;
     const double *ve = a.f + (ind ? F_V1M : F_V1S) * a.n;
@@ -247,15 +296,18 @@ __global__ void __launch_bounds__(256) k_backtra_step(BacktraArgs a, int ntiles)
         xm = x0 - 0.5 * u2 / crm::cos_cr(ym);
         ym = y0 - 0.5 * v2;
     } else {
-        const double symr = -crm::cos_cr(xm) * crm::cos_cr(ym);
-        const double cymr = sqrt(1. - symr * symr);
-        const double xmr = x0r - 0.5 * u2 / cymr;
+        // symr = -cos(xm) cos(ym) at the first guess (xm, ym) = (x0, y0): sy0r, and its
+        // cymr is cy0r
+        const double xmr = x0r - 0.5 * u2 / cy0r;
         const double ymr = y0r - 0.5 * v2;
-        const double sym = crm::cos_cr(xmr) * crm::cos_cr(ymr);
+        double sxr, cxr, syr, cyr;
+        crm::sincos_cr(xmr, sxr, cxr);
+        crm::sincos_cr(ymr, syr, cyr);
+        const double sym = cxr * cyr;
         ym = crm::asin_cr(sym);
         const double cym = sqrt(1. - sym * sym);
-        const double cxm = cym == 0. ? 0. : fmax(-1., fmin(1., -crm::sin_cr(ymr) / cym));
-        xm = copysign(crm::acos_cr(cxm), crm::sin_cr(xmr));
+        const double cxm = cym == 0. ? 0. : fmax(-1., fmin(1., -syr / cym));
+        xm = copysign(crm::acos_cr(cxm), sxr);
     }
     zm = z0 - 0.5 * w2;
     corlat(a, xm, ym);                                          // F:6388-6394
@@ -272,11 +324,14 @@ __global__ void __launch_bounds__(256) k_backtra_step(BacktraArgs a, int ntiles)
         const double cymr = sqrt(1. - symr * symr);
         x0r = x0r - u2 / cymr;
         y0r = y0r - v2;
-        const double sy0 = crm::cos_cr(x0r) * crm::cos_cr(y0r);
+        double sxr, cxr, syr, cyr;
+        crm::sincos_cr(x0r, sxr, cxr);
+        crm::sincos_cr(y0r, syr, cyr);
+        const double sy0 = cxr * cyr;
         yy = crm::asin_cr(sy0);
         const double cy0 = sqrt(1. - sy0 * sy0);
-        const double cx0 = cy0 == 0. ? 0. : fmax(-1., fmin(1., -crm::sin_cr(y0r) / cy0));
-        xx = copysign(crm::acos_cr(cx0), crm::sin_cr(x0r));
+        const double cx0 = cy0 == 0. ? 0. : fmax(-1., fmin(1., -syr / cy0));
+        xx = copysign(crm::acos_cr(cx0), sxr);
     }
     double zz = z0 - w2;
     corlat(a, xx, yy);                                          // F:6434-6440

@@ -267,6 +267,21 @@ CR_HD double sin_cr(double x)
     return s.hi + s.lo;
 }
 
+// sin and cos of one argument from one double-double evaluation (the same bits as
+// sin_cr and cos_cr)
+CR_HD void sincos_cr(double x, double &s, double &c)
+{
+    if (x != x || x == INFINITY || x == -INFINITY || fabs(x) > 100.) {
+        s = sin(x);
+        c = cos(x);
+        return;
+    }
+    dd ss, cc;
+    sincos_dd({x, 0.}, ss, cc);
+    s = x == 0. ? x : ss.hi + ss.lo;
+    c = cc.hi + cc.lo;
+}
+
 // asin: one Newton step on sin(y) = x from the library's value,
 // y = y0 + (x - sin(y0)) / cos(y0); asin(+-1) = +-pi/2 rounded.
 CR_HD double asin_cr(double x)

@@ -14,5 +14,6 @@ double rb_cr_cos(double x) { return crm::cos_cr(x); }
 double rb_cr_sin(double x) { return crm::sin_cr(x); }
 double rb_cr_asin(double x) { return crm::asin_cr(x); }
 double rb_cr_acos(double x) { return crm::acos_cr(x); }
+void rb_cr_sincos(double x, double *s, double *c) { crm::sincos_cr(x, *s, *c); }
 double rb_cr_tanh(double x) { return crm::tanh_cr(x); }
 }

@@ -58,6 +58,20 @@ def test_correctly_rounded_unary(libs, name, gen):
     assert not bad, (name, bad[:5])
 
 
+def test_sincos_is_sin_and_cos(libs):
+    """sincos_cr, which the back-trajectories use, gives sin_cr's and cos_cr's bits."""
+    H, _ = libs
+    H.rb_cr_sincos.argtypes = [C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    H.rb_cr_sincos.restype = None
+    r = np.random.default_rng(5)
+    xs = np.concatenate([r.uniform(-7, 7, 5000), [0.0, -0.0, 1e-300, 150.0, np.pi]])
+    s, c = C.c_double(), C.c_double()
+    for x in xs:
+        H.rb_cr_sincos(float(x), C.byref(s), C.byref(c))
+        assert s.value == H.rb_cr_sin(float(x)) and c.value == H.rb_cr_cos(float(x)), x
+        assert np.signbit(s.value) == np.signbit(H.rb_cr_sin(float(x)))
+
+
 def test_correctly_rounded_pow(libs):
     H, Q = libs
     r = np.random.default_rng(11)
