@@ -185,6 +185,15 @@ int rb_transport_backtra_dev(rb_ctx *ctx, const rb_grid *g, const double *aaa, c
                              double *x0, double *y0, double *z0, double *p0, int32_t *idx,
                              double *w, double *trajlon, double *trajlat, double *trajpre,
                              double *trajtem);
+/* rb_transport_backtra: the same on host arrays (the forcing records' pointers are host
+ * addresses; copies through the context's device buffers): the departure points x0 .. p0
+ * and the trajectory slots trajlon .. trajtem (nrapp n each, all required), for the
+ * model's backtra call with its COMMON arrays; rb_transport then interpolates. */
+int rb_transport_backtra(rb_ctx *ctx, const rb_grid *g, const double *aaa, const double *bbb,
+                         const rb_forcing *m1, const rb_forcing *p1, double daynum, double dtdyn,
+                         int32_t nrapp, int32_t ntime, int32_t nintecmwf, double *x0, double *y0,
+                         double *z0, double *p0, double *trajlon, double *trajlat,
+                         double *trajpre, double *trajtem);
 /* rb_transport_weights_dev replaces posx/posy/posz and poidx/poidy/poidz as backtra calls
  * them (F:6444-6451, F:7371-7647): idx (3 n int32: ig0, kg0, lg0, 1-based, COMMON
  * /indices/) and w (12 n doubles: alfxm1 .. alfzp2, COMMON /poids/ order), device. */

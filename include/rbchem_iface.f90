@@ -19,6 +19,13 @@ module rbchem_iface
                            conv_aa_start = 60, conv_aa_budget = 240, newton_waves = 2
   end type rb_config
 
+  ! rb_forcing: one forcing record of the trajectory code (c_loc of the COMMON /forcm1/ or
+  ! /forcp1/ arrays u, v, w, t and the log surface pressure; the record's day number)
+  type, bind(C) :: rb_forcing
+     type(c_ptr) :: u, v, w, t, lnps
+     real(c_double) :: day
+  end type rb_forcing
+
   ! rbio.h: the history record's header (namexp is 6 characters + NUL)
   type, bind(C) :: rb_history_header
      character(kind=c_char) :: namexp(7)
@@ -91,6 +98,24 @@ module rbchem_iface
        real(c_double), intent(in) :: trajlon(*), trajlat(*), trajpre(*)
        real(c_double), intent(out) :: alt(*)
      end function rb_altitude
+
+     ! semilag's call backtra(daynum, dtdyn, nrapp, trajlon, trajlat, trajpre, trajtem,
+     ! ntime, nintecmwf) (F:6142-6144, F:6156-6470): departure points x0 .. p0 (COMMON
+     ! /depo/) and the trajectory slots 1..nrapp of trajlon .. trajtem
+     integer(c_int) function rb_transport_backtra(ctx, g, aaa, bbb, m1, p1, daynum, dtdyn, nrapp, &
+                                                  ntime, nintecmwf, x0, y0, z0, p0, trajlon, &
+                                                  trajlat, trajpre, trajtem) &
+                                                  bind(C, name="rb_transport_backtra")
+       import :: c_int, c_int32_t, c_ptr, c_double, rb_grid, rb_forcing
+       type(c_ptr), value :: ctx
+       type(rb_grid), intent(in) :: g
+       real(c_double), intent(in) :: aaa(*), bbb(*)
+       type(rb_forcing), intent(in) :: m1, p1
+       real(c_double), value :: daynum, dtdyn
+       integer(c_int32_t), value :: nrapp, ntime, nintecmwf
+       real(c_double), intent(out) :: x0(*), y0(*), z0(*), p0(*)
+       real(c_double), intent(inout) :: trajlon(*), trajlat(*), trajpre(*), trajtem(*)
+     end function rb_transport_backtra
 
      ! semilag after the trajectories (F:6121-6153): posx..poidz at the departure points
      ! x0, y0, z0, p0 (COMMON /depo/), then internew for tracers 1..ntr of qj1, in place

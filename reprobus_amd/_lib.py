@@ -23,7 +23,8 @@ EXPORTS = ["rb_init", "rb_load_jtable", "rb_set_stream", "rb_chem_step", "rb_con
            "rb_chem_converge_dev",
            "rb_chem_newton_dev", "rb_chem_surrogate_dev", "rb_sediment",
            "rb_sediment_dev", "rb_zenith", "rb_zenith_dev", "rb_altitude", "rb_altitude_dev",
-           "rb_run_day_dev", "rb_transport_backtra_dev", "rb_transport_weights_dev",
+           "rb_run_day_dev", "rb_transport_backtra_dev", "rb_transport_backtra",
+           "rb_transport_weights_dev",
            "rb_transport_interp_dev", "rb_transport",
            "rb_sync", "rb_get_stats", "rb_last_error", "rb_finalize"]
 # and include/rbmlp.h (the learned initial guess)
@@ -118,6 +119,9 @@ def load() -> C.CDLL:
     L.rb_transport_backtra_dev.argtypes = [P, C.POINTER(Grid), P, P, C.POINTER(Forcing),
                                            C.POINTER(Forcing), d, d, C.c_int32, C.c_int32,
                                            C.c_int32] + [P] * 10
+    L.rb_transport_backtra.argtypes = [P, C.POINTER(Grid), P, P, C.POINTER(Forcing),
+                                       C.POINTER(Forcing), d, d, C.c_int32, C.c_int32,
+                                       C.c_int32] + [P] * 8
     L.rb_transport_weights_dev.argtypes = [P, C.POINTER(Grid)] + [P] * 8
     L.rb_transport_interp_dev.argtypes = [P, C.POINTER(Grid), P, P, C.c_int32, P, P]
     L.rb_transport.argtypes = [P, C.POINTER(Grid)] + [P] * 6 + [C.c_int32, P]

@@ -43,6 +43,8 @@ struct rb_ctx {
     size_t tq_cap = 0;
     double *d_bt = nullptr;                  // back-trajectories: fields of a sub-step, tables
     size_t bt_cap = 0;
+    double *d_bth = nullptr;                 // back-trajectories (host entry): inputs, outputs
+    size_t bth_cap = 0;
     std::vector<double> bt_tables;           // host tables of the grid bt_grid
     int bt_grid[2] = {0, 0};
     int n_cu = 0;
@@ -886,6 +888,64 @@ int rb_transport_backtra_dev(rb_ctx *ctx, const rb_grid *g, const double *aaa, c
     return transport_outside(ctx, "rb_transport_backtra");
 }
 
+int rb_transport_backtra(rb_ctx *ctx, const rb_grid *g, const double *aaa, const double *bbb,
+                         const rb_forcing *m1, const rb_forcing *p1, double daynum, double dtdyn,
+                         int32_t nrapp, int32_t ntime, int32_t nintecmwf, double *x0, double *y0,
+                         double *z0, double *p0, double *trajlon, double *trajlat,
+                         double *trajpre, double *trajtem)
+{
+    if (!ctx) return RB_EINVAL;
+    int rc = check_transport(ctx, g);
+    if (rc) return rc;
+    if (!m1 || !p1) return fail(ctx, RB_EINVAL, "a forcing record is NULL");
+    if (any_null({aaa, bbb, x0, y0, z0, p0, trajlon, trajlat, trajpre, trajtem, m1->u, m1->v,
+                  m1->w, m1->t, m1->lnps, p1->u, p1->v, p1->w, p1->t, p1->lnps}))
+        return fail(ctx, RB_EINVAL, "a host array is NULL");
+    if (nrapp < 1) return fail(ctx, RB_EINVAL, "nrapp must be >= 1");
+    HIPCHK(hipSetDevice(ctx->device));
+    const size_t n = (size_t)g->nlon * g->nlat * g->niv, plane = (size_t)g->nlon * g->nlat;
+    const size_t B = sizeof(double);
+    // forcing 2 (4 n + plane), levels 2 niv, departure points 4 n, indices 3 n int32,
+    // weights 12 n, trajectory 4 nrapp n
+    const size_t need = 2 * (4 * n + plane) + 2 * (size_t)g->niv + 16 * n + (3 * n + 1) / 2 +
+                        4 * (size_t)nrapp * n;
+    if ((rc = ensure(ctx, &ctx->d_bth, &ctx->bth_cap, need))) return rc;
+    double *q = ctx->d_bth;
+    hipStream_t s = ctx->stream;
+    rb_forcing dm[2];
+    const rb_forcing *hm[2] = {m1, p1};
+    for (int r = 0; r < 2; ++r) {
+        double *f[5];
+        const double *h[5] = {hm[r]->u, hm[r]->v, hm[r]->w, hm[r]->t, hm[r]->lnps};
+        for (int k = 0; k < 5; ++k) {
+            const size_t len = k < 4 ? n : plane;
+            f[k] = q;
+            HIPCHK(hipMemcpyAsync(q, h[k], len * B, hipMemcpyHostToDevice, s));
+            q += len;
+        }
+        dm[r] = {f[0], f[1], f[2], f[3], f[4], hm[r]->day};
+    }
+    double *da = q, *db = q + g->niv;
+    q += 2 * g->niv;
+    HIPCHK(hipMemcpyAsync(da, aaa, g->niv * B, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(db, bbb, g->niv * B, hipMemcpyHostToDevice, s));
+    double *dx = q, *dy = dx + n, *dz = dy + n, *dp = dz + n, *dw = dp + n;
+    int32_t *didx = (int32_t *)(dw + 12 * n);
+    double *dt = dw + 12 * n + (3 * n + 1) / 2;
+    double *dtr[4] = {dt, dt + nrapp * n, dt + 2 * nrapp * n, dt + 3 * nrapp * n};
+    if ((rc = rb_transport_backtra_dev(ctx, g, da, db, &dm[0], &dm[1], daynum, dtdyn, nrapp,
+                                       ntime, nintecmwf, dx, dy, dz, dp, didx, dw, dtr[0], dtr[1],
+                                       dtr[2], dtr[3])))
+        return rc;
+    const std::pair<double *, const double *> out[] = {{x0, dx}, {y0, dy}, {z0, dz}, {p0, dp}};
+    for (auto &o : out) HIPCHK(hipMemcpyAsync(o.first, o.second, n * B, hipMemcpyDeviceToHost, s));
+    double *htr[4] = {trajlon, trajlat, trajpre, trajtem};
+    for (int k = 0; k < 4; ++k)
+        HIPCHK(hipMemcpyAsync(htr[k], dtr[k], nrapp * n * B, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return RB_OK;
+}
+
 int rb_transport(rb_ctx *ctx, const rb_grid *g, const double *aaa, const double *bbb,
                  const double *x0, const double *y0, const double *z0, const double *p0,
                  int32_t ntr, double *qj1)
@@ -1106,6 +1166,7 @@ int rb_finalize(rb_ctx *ctx)
     if (ctx->d_tw) (void)hipFree(ctx->d_tw);
     if (ctx->d_tq) (void)hipFree(ctx->d_tq);
     if (ctx->d_bt) (void)hipFree(ctx->d_bt);
+    if (ctx->d_bth) (void)hipFree(ctx->d_bth);
     if (ctx->day_exec) (void)hipGraphExecDestroy(ctx->day_exec);
     if (ctx->day_graph) (void)hipGraphDestroy(ctx->day_graph);
     for (hipEvent_t e : {ctx->ev0, ctx->ev1, ctx->ev_h2d, ctx->ev_d2h})

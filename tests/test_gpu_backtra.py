@@ -137,3 +137,38 @@ def test_backtra_rejects_bad_arguments(step):
         backtra_gpu(step, bad, 2)
     out = backtra_gpu(step, ti, 2, traj=False)          # the trajectory arrays are optional
     assert np.isfinite(out["x0"]).all()
+
+
+def test_host_entry_equals_device_entry(step):
+    """rb_transport_backtra (host arrays, the Fortran model's entry) gives the device entry's
+    departure points and trajectory bit for bit."""
+    import ctypes as C
+
+    from reprobus_amd import _lib
+    from reprobus_amd.chem import make_grid
+
+    ti = synth.make_trajectory_inputs(3, nlon=36, nlat=19, niv=12)
+    nrapp = 3
+    dev = backtra_gpu(step, ti, nrapp)
+    n = ti.nlon * ti.nlat * ti.niv
+    keep = []
+
+    def rec(r):
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (r.u, r.v, r.w, r.t, r.lnps)]
+        keep.extend(arrs)
+        return _lib.Forcing(*[a.ctypes.data for a in arrs], float(r.day))
+
+    m1, p1 = rec(ti.m1), rec(ti.p1)
+    out = {k: np.zeros(n) for k in ("x0", "y0", "z0", "p0")}
+    tr = {k: np.zeros((nrapp, ti.niv, ti.nlat, ti.nlon)) for k in ("trajlon", "trajlat", "trajpre", "trajtem")}
+    grid = make_grid(ti.nlon, ti.nlat, ti.niv, ti.niv, 1)
+    rc = step._L.rb_transport_backtra(step._ctx, C.byref(grid), ti.aaa.ctypes.data,
+                                      ti.bbb.ctypes.data, C.byref(m1), C.byref(p1), ti.daynum,
+                                      ti.dtdyn, nrapp, ti.ntime, ti.nintecmwf,
+                                      *[out[k].ctypes.data for k in ("x0", "y0", "z0", "p0")],
+                                      *[tr[k].ctypes.data for k in ("trajlon", "trajlat", "trajpre", "trajtem")])
+    assert rc == 0
+    for k in ("x0", "y0", "z0", "p0"):
+        assert np.array_equal(out[k], dev[k]), k
+    for k in tr:
+        assert np.array_equal(tr[k], dev[k]), k
