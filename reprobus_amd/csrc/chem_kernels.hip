@@ -1354,11 +1354,14 @@ __device__ __forceinline__ void rec_load(Box &x, const double *__restrict__ rec,
 // the PSC/aerosol block, densities and rain-out, each stage stored as soon as it is
 // computed, so that no stage's registers outlive it.  At two waves per SIMD (~250 VGPRs)
 // the transcendental chains and table gathers of one wave overlap another's.
+// Boxes box0 .. box0 + nr - 1; the record of box box0 + r at index r (stride nr).
 template <int BLK>
-__global__ void __launch_bounds__(BLK, 2) k_setup(StepArgs a, double *__restrict__ rec)
+__global__ void __launch_bounds__(BLK, 2) k_setup(StepArgs a, double *__restrict__ rec,
+                                                  int64_t box0, int64_t nr)
 {
-    const int64_t b = (int64_t)blockIdx.x * BLK + threadIdx.x;
-    if (b >= a.nbox) return;
+    const int64_t r = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    if (r >= nr) return;
+    const int64_t b = box0 + r;
     const int64_t c = b % a.plane;
     const int k = (int)(b / a.plane);
     const double t = a.T[b], pm = a.p[b], sza = a.sza[b];
@@ -1372,10 +1375,10 @@ __global__ void __launch_bounds__(BLK, 2) k_setup(StepArgs a, double *__restrict
     e.dt = a.dt;
     e.day = sza <= 95.;
     thermal_rates(a.K, t, e.hnm, pm, x.R);
-    rec_store<REC_R, REC_J>(x, rec, a.nbox, b);
+    rec_store<REC_R, REC_J>(x, rec, nr, r);
     table_j(a.ajl, a.o3up, a.alt[b], sza, a.col[b], x.tj);
     x.tj[JNO] = j_no(a.col[a.nbox + b], a.col[2 * a.nbox + b], a.col[b], e.hnm, a.sza[c]);
-    rec_store<REC_J, REC_H>(x, rec, a.nbox, b);
+    rec_store<REC_J, REC_H>(x, rec, nr, r);
 #pragma unroll
     for (int s = 0; s < NLONG; ++s) x.ck[s] = a.qj1[(int64_t)s * a.field + b] * e.hnm;
     Het &h = x.h;
@@ -1402,18 +1405,19 @@ __global__ void __launch_bounds__(BLK, 2) k_setup(StepArgs a, double *__restrict
         e.rain_hno3 = rr; e.rain_hcl = rr; e.rain_hbr = rr; e.rain_h2o2 = rr;
         e.rain_hno4 = 0.; e.rain_ch2o = 0.; e.rain_ch3o2h = 0.;
     }
-    rec_store<REC_HNM, REC_R>(x, rec, a.nbox, b);
-    rec_store<REC_H, NREC>(x, rec, a.nbox, b);
+    rec_store<REC_HNM, REC_R>(x, rec, nr, r);
+    rec_store<REC_H, NREC>(x, rec, nr, r);
 }
 
-// The state of box b from its set-up record (chem3d before its sweep loop, F:800-1338):
-// the iterate from the incoming state (or, with use_guess and a guess, from the guess), the
-// start-of-step state to the LDS column ck0 (stride BLK).
+// The state of box b from its set-up record (chem3d before its sweep loop, F:800-1338),
+// record r of stride nr: the iterate from the incoming state (or, with use_guess and a
+// guess, from the guess), the start-of-step state to the LDS column ck0 (stride BLK).
 template <int BLK>
 __device__ __forceinline__ void box_load(const StepArgs &a, const double *__restrict__ rec,
-                                         int64_t b, Box &x, double *ck0, bool use_guess)
+                                         int64_t nr, int64_t r, int64_t b, Box &x, double *ck0,
+                                         bool use_guess)
 {
-    rec_load(x, rec, a.nbox, b, a.dt);
+    rec_load(x, rec, nr, r, a.dt);
     const double hnm = x.e.hnm;
 #pragma unroll
     for (int s = 0; s < NLONG; ++s) {
@@ -1437,8 +1441,10 @@ __device__ __forceinline__ void box_load(const StepArgs &a, const double *__rest
 // (k_setup) runs as its own launch at two waves per SIMD: in one kernel with the sweeps it
 // ran at the sweeps' one wave per SIMD (the box state needs ~2 KB of registers) and took
 // three quarters of the step.
+// Boxes box0 .. box0 + nr - 1 from records 0 .. nr - 1 (k_setup's range).
 template <int BLK>
-__global__ void __launch_bounds__(BLK) k_chem(StepArgs a, const double *__restrict__ rec)
+__global__ void __launch_bounds__(BLK) k_chem(StepArgs a, const double *__restrict__ rec,
+                                              int64_t box0, int64_t nr)
 {
     // start-of-step state of the block's boxes, species-major: lane-private columns,
     // conflict-free, and off the register file that the sweep needs for rates and J
@@ -1449,10 +1455,11 @@ __global__ void __launch_bounds__(BLK) k_chem(StepArgs a, const double *__restri
     double *tjl = s_ck0 + (NLONG - 1) * BLK + threadIdx.x;
     // last blocks first: k_setup wrote the record in box order, so its most recent part is
     // still in the Infinity Cache when this launch starts (1.4-2.8 % faster than in order)
-    const int64_t b = (int64_t)(gridDim.x - 1 - blockIdx.x) * BLK + threadIdx.x;
-    if (b >= a.nbox) return;
+    const int64_t r = (int64_t)(gridDim.x - 1 - blockIdx.x) * BLK + threadIdx.x;
+    if (r >= nr) return;
+    const int64_t b = box0 + r;
     Box x;
-    box_load<BLK>(a, rec, b, x, ck0, false);
+    box_load<BLK>(a, rec, nr, r, b, x, ck0, false);
 #pragma unroll
     for (int i = 0; i < NJL; ++i) tjl[i * BLK] = x.tj[JL_SET[i]];
     Track notrack{true, true, 0., 0., 0.};
@@ -1714,7 +1721,7 @@ __global__ void __launch_bounds__(BLK) k_chem_conv(StepArgs a, ConvArgs cv)
             }
         }
         if (setup) {
-            box_load<BLK>(a, cv.rec, b, x, ck0, !fallback && !resume);
+            box_load<BLK>(a, cv.rec, a.nbox, b, b, x, ck0, !fallback && !resume);
 #pragma unroll
             for (int i = 0; i < NJL; ++i) tjl[i * BLK] = x.tj[JL_SET[i]];
             active = true;
@@ -1874,7 +1881,7 @@ __global__ void __launch_bounds__(NEWTON_BLK, WAVES) k_chem_newton(StepArgs a, C
             const int64_t b = FROM_SPILL ? in_box[w] : w;
             Box x;
             double ck0[NLONG - 1];
-            box_load<1>(a, cv.rec, b, x, ck0, !FROM_SPILL);
+            box_load<1>(a, cv.rec, a.nbox, b, b, x, ck0, !FROM_SPILL);
             const double fl = 1.e-30 * x.e.hnm;
             int sweeps = 0;
             bool fin = true;
@@ -2096,27 +2103,31 @@ hipError_t launch_columns(const StepArgs &a, double *col, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launch_setup(const StepArgs &a, double *rec, int block, hipStream_t s)
+hipError_t launch_setup(const StepArgs &a, double *rec, int block, hipStream_t s, int64_t box0,
+                        int64_t nr)
 {
     const int blk = block > 0 ? block : 256;
-    const unsigned grid = (unsigned)((a.nbox + blk - 1) / blk);
+    if (nr < 0) nr = a.nbox - box0;
+    const unsigned grid = (unsigned)((nr + blk - 1) / blk);
     switch (blk) {
-    case 64: hipLaunchKernelGGL(k_setup<64>, dim3(grid), dim3(64), 0, s, a, rec); break;
-    case 128: hipLaunchKernelGGL(k_setup<128>, dim3(grid), dim3(128), 0, s, a, rec); break;
-    case 256: hipLaunchKernelGGL(k_setup<256>, dim3(grid), dim3(256), 0, s, a, rec); break;
+    case 64: hipLaunchKernelGGL(k_setup<64>, dim3(grid), dim3(64), 0, s, a, rec, box0, nr); break;
+    case 128: hipLaunchKernelGGL(k_setup<128>, dim3(grid), dim3(128), 0, s, a, rec, box0, nr); break;
+    case 256: hipLaunchKernelGGL(k_setup<256>, dim3(grid), dim3(256), 0, s, a, rec, box0, nr); break;
     default: return hipErrorInvalidConfiguration;
     }
     return hipGetLastError();
 }
 
-hipError_t launch_chem(const StepArgs &a, const double *rec, int block, hipStream_t s)
+hipError_t launch_chem(const StepArgs &a, const double *rec, int block, hipStream_t s,
+                       int64_t box0, int64_t nr)
 {
     const int blk = block > 0 ? block : 256;
-    const unsigned grid = (unsigned)((a.nbox + blk - 1) / blk);
+    if (nr < 0) nr = a.nbox - box0;
+    const unsigned grid = (unsigned)((nr + blk - 1) / blk);
     switch (blk) {
-    case 64: hipLaunchKernelGGL(k_chem<64>, dim3(grid), dim3(64), 0, s, a, rec); break;
-    case 128: hipLaunchKernelGGL(k_chem<128>, dim3(grid), dim3(128), 0, s, a, rec); break;
-    case 256: hipLaunchKernelGGL(k_chem<256>, dim3(grid), dim3(256), 0, s, a, rec); break;
+    case 64: hipLaunchKernelGGL(k_chem<64>, dim3(grid), dim3(64), 0, s, a, rec, box0, nr); break;
+    case 128: hipLaunchKernelGGL(k_chem<128>, dim3(grid), dim3(128), 0, s, a, rec, box0, nr); break;
+    case 256: hipLaunchKernelGGL(k_chem<256>, dim3(grid), dim3(256), 0, s, a, rec, box0, nr); break;
     default: return hipErrorInvalidConfiguration;
     }
     return hipGetLastError();
