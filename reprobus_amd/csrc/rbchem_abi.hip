@@ -2,6 +2,7 @@
 #include "../../include/rbchem.h"
 
 #include <hip/hip_runtime.h>
+#include <rocprofiler-sdk-roctx/roctx.h>
 
 #include <algorithm>
 #include <cmath>
@@ -62,6 +63,15 @@ struct rb_ctx {
 };
 
 namespace {
+
+// A named range around an entry point's launches, for tracers (rocprofv3 --marker-trace);
+// a no-op without one.
+struct TraceRange {
+    explicit TraceRange(const char *name) { roctxRangePush(name); }
+    ~TraceRange() { roctxRangePop(); }
+    TraceRange(const TraceRange &) = delete;
+    TraceRange &operator=(const TraceRange &) = delete;
+};
 
 int fail(rb_ctx *c, int code, const std::string &msg)
 {
@@ -344,6 +354,7 @@ int rb_chem_step_dev(rb_ctx *ctx, const rb_grid *g, double dt, const double *T, 
                      double *surfarea, double *vsed3d, double *parth2osed, double *parthno3sed,
                      int fuse_conserv)
 {
+    TraceRange trace("rb_chem_step_dev");
     if (!ctx) return RB_EINVAL;
     int rc = step_checks(ctx, g, dt, {T, p, sza, alt, qj1, hc, surfarea, vsed3d, parth2osed,
                                       parthno3sed});
@@ -361,6 +372,7 @@ int rb_chem_step_noalloc_dev(rb_ctx *ctx, const rb_grid *g, double dt, const dou
                              double *hc, double *surfarea, double *vsed3d, double *parth2osed,
                              double *parthno3sed, int fuse_conserv)
 {
+    TraceRange trace("rb_chem_step_noalloc_dev");
     if (!ctx) return RB_EINVAL;
     int rc = step_checks(ctx, g, dt, {T, p, sza, alt, qj1, hc, surfarea, vsed3d, parth2osed,
                                       parthno3sed});
@@ -471,6 +483,7 @@ int rb_chem_converge_dev(rb_ctx *ctx, const rb_grid *g, double dt, double tol, i
                          const double *guess_qj1, const double *guess_hc, int32_t *iters,
                          int fuse_conserv)
 {
+    TraceRange trace("rb_chem_converge_dev");
     return converge_impl(ctx, g, dt, tol, maxiter, accel, T, p, sza, alt, qj1, hc, surfarea,
                          vsed3d, parth2osed, parthno3sed, guess_qj1, guess_hc, iters,
                          fuse_conserv, 0.);
@@ -484,6 +497,7 @@ int rb_chem_newton_dev(rb_ctx *ctx, const rb_grid *g, double dt, double tol, int
                        const double *guess_qj1, const double *guess_hc, int32_t *iters,
                        int fuse_conserv)
 {
+    TraceRange trace("rb_chem_newton_dev");
     if (!ctx) return RB_EINVAL;
     int rc = check_grid(ctx, g);
     if (rc) return rc;
@@ -570,6 +584,7 @@ int rb_chem_surrogate_dev(rb_ctx *ctx, const rb_grid *g, double dt, double accep
                           double *parthno3sed, const double *sur_qj1, const double *sur_hc,
                           int32_t *iters, int fuse_conserv)
 {
+    TraceRange trace("rb_chem_surrogate_dev");
     if (!ctx) return RB_EINVAL;
     if (!sur_qj1 || !sur_hc)
         return fail(ctx, RB_EINVAL, "the surrogate's answer (sur_qj1, sur_hc) is NULL");
@@ -582,6 +597,7 @@ int rb_chem_surrogate_dev(rb_ctx *ctx, const rb_grid *g, double dt, double accep
 
 int rb_conserv_floor_dev(rb_ctx *ctx, const rb_grid *g, double *qj1, double *hc)
 {
+    TraceRange trace("rb_conserv_floor_dev");
     if (!ctx) return RB_EINVAL;
     int rc = check_grid(ctx, g);
     if (rc) return rc;
@@ -596,6 +612,7 @@ int rb_chem_step(rb_ctx *ctx, const rb_grid *g, double dt, const double *T, cons
                  const double *sza, const double *alt, double *qj1, double *hc, double *surfarea,
                  double *vsed3d, double *parth2osed, double *parthno3sed)
 {
+    TraceRange trace("rb_chem_step");
     if (!ctx) return RB_EINVAL;
     int rc = check_grid(ctx, g);
     if (rc) return rc;
@@ -665,6 +682,7 @@ int rb_conserv_floor(rb_ctx *ctx, const rb_grid *g, double *qj1, double *hc)
 int rb_zenith_dev(rb_ctx *ctx, const rb_grid *g, double day, const double *trajlon,
                   const double *trajlat, double *sza)
 {
+    TraceRange trace("rb_zenith_dev");
     if (!ctx) return RB_EINVAL;
     int rc = check_grid(ctx, g);
     if (rc) return rc;
@@ -708,6 +726,7 @@ int rb_altitude_dev(rb_ctx *ctx, const rb_grid *g, double dlon, double dlat, con
                     const double *h2o, const double *trajlon, const double *trajlat,
                     const double *trajpre, double *alt)
 {
+    TraceRange trace("rb_altitude_dev");
     if (!ctx) return RB_EINVAL;
     int rc = check_altitude(ctx, g, dlon, dlat);
     if (rc) return rc;
@@ -781,6 +800,7 @@ int rb_transport_weights_dev(rb_ctx *ctx, const rb_grid *g, const double *aaa, c
                              const double *x0, const double *y0, const double *z0,
                              const double *p0, int32_t *idx, double *w)
 {
+    TraceRange trace("rb_transport_weights_dev");
     if (!ctx) return RB_EINVAL;
     int rc = check_transport(ctx, g);
     if (rc) return rc;
@@ -797,6 +817,7 @@ int rb_transport_weights_dev(rb_ctx *ctx, const rb_grid *g, const double *aaa, c
 int rb_transport_interp_dev(rb_ctx *ctx, const rb_grid *g, const int32_t *idx, const double *w,
                             int32_t ntr, const double *q_in, double *q_out)
 {
+    TraceRange trace("rb_transport_interp_dev");
     if (!ctx) return RB_EINVAL;
     int rc = check_transport(ctx, g);
     if (rc) return rc;
@@ -820,6 +841,7 @@ int rb_transport_backtra_dev(rb_ctx *ctx, const rb_grid *g, const double *aaa, c
                              double *w, double *trajlon, double *trajlat, double *trajpre,
                              double *trajtem)
 {
+    TraceRange trace("rb_transport_backtra_dev");
     if (!ctx) return RB_EINVAL;
     int rc = check_transport(ctx, g);
     if (rc) return rc;
@@ -1004,6 +1026,7 @@ int rb_sediment_dev(rb_ctx *ctx, const rb_grid *g, double dt, int32_t nivh2o, co
                     const double *p, double *qj1, const double *vsed3d, const double *parth2osed,
                     const double *parthno3sed)
 {
+    TraceRange trace("rb_sediment_dev");
     if (!ctx) return RB_EINVAL;
     int rc = check_sediment(ctx, g, dt, nivh2o);
     if (rc) return rc;
@@ -1068,6 +1091,7 @@ int rb_run_day_dev(rb_ctx *ctx, const rb_grid *g, const rb_day_config *dc, const
                    double *sza, double *qj1, double *hc, double *surfarea, double *vsed3d,
                    double *parth2osed, double *parthno3sed, int32_t *iters)
 {
+    TraceRange trace("rb_run_day_dev");
     if (!ctx) return RB_EINVAL;
     int rc = check_grid(ctx, g);
     if (rc) return rc;
