@@ -77,6 +77,8 @@ def parse():
                     help="BLOCK,CHUNK of the converged-mode kernels (default: autotuned)")
     ap.add_argument("--newton-shape", default="",
                     help="SWEEPS_FIRST,WAVES of the Newton solve (default: autotuned)")
+    ap.add_argument("--conv-order", type=int, default=0,
+                    help="converged mode's work order: 0 night boxes then day boxes, 1 grid order")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="after the timed region, one device copy of this many MB (a kernel of "
                          "known traffic, to scale profiler byte counters)")
@@ -205,7 +207,7 @@ def _timed(fn, reps, stream):
 
 
 def converged_section(table, block_conv, refill, dev, newton_shape=(60, 1), tol=1e-11,
-                      maxiter=2000, max_newton=20):
+                      maxiter=2000, max_newton=20, order=0):
     """Converged mode on the c2 grid (seed 1, the guess network's evaluation seed): plain
     sweeps, Anderson-accelerated sweeps and the Newton solve (generated sparse LU, after
     newton_shape[0] plain sweeps, newton_shape[1] waves per SIMD), each from the incoming
@@ -222,7 +224,7 @@ def converged_section(table, block_conv, refill, dev, newton_shape=(60, 1), tol=
     grid = make_grid(g.nlon, g.nlat, g.niv, g.nivbas, g.nivhau)
     st = _device_state(inp, dev)
     step = ChemistryStep(*table, device=0, block=block_conv, conv_refill=refill,
-                         newton_waves=newton_shape[1])
+                         newton_waves=newton_shape[1], conv_order=order)
     net = GuessMLP(0)
     if not guess.load(net):
         return {"error": "no trained guess weights (reprobus_amd/data/guess_mlp.npz)"}
@@ -320,9 +322,10 @@ def converged_section(table, block_conv, refill, dev, newton_shape=(60, 1), tol=
     nxt["speedup_newton_over_plain"] = nxt["plain"]["ms"] / nxt["newton"]["ms"]
     out = {"workload": "c2 seed 1 (786,240 boxes, polar vortex), incoming state, dt 900 s",
            "tol": tol, "maxiter": maxiter, "block": block_conv, "refill": refill,
+           "work_order": "night boxes, then day boxes" if order == 0 else "grid order",
            "anderson": {"depth": 2, "from_sweep": step.conv_aa_start,
                         "budget": step.conv_aa_budget},
-           "newton": {"sweeps_first": newton_shape[0], "waves_per_simd": newton_shape[1],
+           "newton_shape": {"sweeps_first": newton_shape[0], "waves_per_simd": newton_shape[1],
                       "max_newton": max_newton, "unknowns": 55,
                       "jacobian_nonzeros_day_night": [399, 278],
                       "lu_nonzeros_day_night": [514, 365]},
@@ -699,7 +702,8 @@ def main():
             extras["converged"] = {"error": repr(exc)}
     if "converged" in want and conv_shape:
         try:
-            extras["converged"] = converged_section(table, *conv_shape, dev, newton_shape)
+            extras["converged"] = converged_section(table, *conv_shape, dev, newton_shape,
+                                                    order=args.conv_order)
         except Exception as exc:
             extras["converged"] = {"error": repr(exc)}
     if "surrogate" in want and conv_shape:

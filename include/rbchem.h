@@ -64,6 +64,11 @@ typedef struct {
     int32_t newton_waves;   /* Newton solve: resident waves per SIMD of its kernel, 1, 2 or 4
                                (0: 2) -- more waves hide the latency of its factor storage,
                                fewer leave each more registers */
+    int32_t conv_order;     /* converged mode (rb_chem_converge_dev, rb_chem_newton_dev,
+                               rb_chem_surrogate_dev): the order its persistent kernels take
+                               boxes in -- 0: night boxes (SZA > 95) first, then day boxes, so
+                               that a wave's lanes mostly run one branch of the sweep; 1: grid
+                               order.  Results are the same either way. */
 } rb_config;
 
 typedef struct {

@@ -16,7 +16,8 @@ module rbchem_iface
 
   type, bind(C) :: rb_config
      integer(c_int32_t) :: niter = 5, check_finite = 0, block = 256, profile = 0, conv_refill = 16, &
-                           conv_aa_start = 60, conv_aa_budget = 240, newton_waves = 2
+                           conv_aa_start = 60, conv_aa_budget = 240, newton_waves = 2, &
+                           conv_order = 0
   end type rb_config
 
   ! rb_forcing: one forcing record of the trajectory code (c_loc of the COMMON /forcm1/ or

@@ -51,7 +51,8 @@ class Forcing(C.Structure):
 class Config(C.Structure):
     _fields_ = [("niter", C.c_int32), ("check_finite", C.c_int32), ("block", C.c_int32),
                 ("profile", C.c_int32), ("conv_refill", C.c_int32), ("conv_aa_start", C.c_int32),
-                ("conv_aa_budget", C.c_int32), ("newton_waves", C.c_int32)]
+                ("conv_aa_budget", C.c_int32), ("newton_waves", C.c_int32),
+                ("conv_order", C.c_int32)]
 
 
 class DayConfig(C.Structure):

@@ -77,13 +77,13 @@ class ChemistryStep:
     def __init__(self, ajl: np.ndarray, o3up: np.ndarray, device: int = 0, niter: int = 5,
                  block: int = 256, check_finite: bool = False, profile: bool = False,
                  conv_refill: int = 16, conv_aa_start: int = 60, conv_aa_budget: int = 240,
-                 newton_waves: int = 2):
+                 newton_waves: int = 2, conv_order: int = 0):
         self._L = _lib.load()
         self._ctx = C.c_void_p()
         self.conv_refill, self.conv_aa_start = conv_refill, conv_aa_start
         self.conv_aa_budget = conv_aa_budget
         cfg = Config(niter, 1 if check_finite else 0, block, 1 if profile else 0, conv_refill,
-                     conv_aa_start, conv_aa_budget, newton_waves)
+                     conv_aa_start, conv_aa_budget, newton_waves, conv_order)
         rc = self._L.rb_init(C.byref(self._ctx), device, C.byref(cfg))
         if rc != 0:
             raise RbError(rc, f"rb_init(device={device}) failed")

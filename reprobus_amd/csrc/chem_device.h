@@ -68,6 +68,8 @@ struct ConvArgs {
                                        // Newton solve's resume phase: NEWTON_RESUMED)
     int32_t max_newton;                // Newton solve: iterations before a box is handed
                                        // to the sweeps
+    const int32_t *order;              // or null: the box at queue position w is order[w]
+                                       // (launch_order_day), else w
 };
 // doubles per box of the set-up record
 constexpr int CONV_RECORD = 141;
@@ -169,6 +171,11 @@ int newton_blocks_per_cu(int waves);
 // anderson_phase)
 int conv_blocks_per_cu(int block, int accel, bool anderson_phase = false);
 hipError_t launch_zero_counter(unsigned long long *p, hipStream_t s, int n = 1);
+// the converged mode's work order: the night boxes (sza > 95, or not a number) of
+// 0 .. nbox - 1 from order[0] on, the day boxes from order[nbox - 1] down; cnt holds 2
+// counters, zero at launch
+hipError_t launch_order_day(const double *sza, int64_t nbox, int32_t *order,
+                            unsigned long long *cnt, hipStream_t s);
 // conserv on boxes below plane*nivbas and the floor on every box, for boxes [start, field)
 // day-dependent terms of zenith (F:5334-5380), computed once per call on the host
 struct ZenithTerms {

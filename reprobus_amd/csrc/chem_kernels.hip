@@ -1711,7 +1711,7 @@ __global__ void __launch_bounds__(BLK) k_chem_conv(StepArgs a, ConvArgs cv)
                 const unsigned long long below = waiting & ((1ull << lane) - 1ull);
                 w = (int64_t)(base + (unsigned long long)__popcll(below));
                 if (w < nwork) {
-                    b = FROM_SPILL ? sp_box[w] : w;
+                    b = FROM_SPILL ? sp_box[w] : (cv.order ? (int64_t)cv.order[w] : w);
                     setup = true;
                     resume = FROM_SPILL;
                     fallback = false;
@@ -1878,7 +1878,7 @@ __global__ void __launch_bounds__(NEWTON_BLK, WAVES) k_chem_newton(StepArgs a, C
         if ((int64_t)base >= nwork) break;
         const int64_t w = (int64_t)base + lane;
         if (w < nwork) {
-            const int64_t b = FROM_SPILL ? in_box[w] : w;
+            const int64_t b = FROM_SPILL ? in_box[w] : (cv.order ? (int64_t)cv.order[w] : w);
             Box x;
             double ck0[NLONG - 1];
             box_load<1>(a, cv.rec, a.nbox, b, b, x, ck0, !FROM_SPILL);
@@ -2244,6 +2244,100 @@ __global__ void k_zero_u64(unsigned long long *p, int n)
 hipError_t launch_zero_counter(unsigned long long *p, hipStream_t s, int n)
 {
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, p, n);
+    return hipGetLastError();
+}
+
+// The converged mode's work order.  The persistent sweep kernels refill a wave's idle lanes
+// from a queue; in grid order the lanes soon hold boxes from all over the grid, day and
+// night mixed, and such a wave issues both branches of the sweep (F:1353-1670 by day,
+// F:1671-1746 by night) on every sweep.  Queueing the night boxes first and the day boxes
+// after keeps the two apart except while the queue passes from one part to the other.
+// Within each part a block's boxes stay together in grid order (the day part reversed), so
+// a wave's loads of the fields still fall on few cache lines.  The day test is k_setup's.
+// A block takes ORDER_CHUNKS chunks of ORDER_BLK boxes: two global atomics per 4,096 boxes
+// (one block per 256 boxes, 3,072 blocks on the c1 grid, spent ~80 us contending for the
+// two counters).
+constexpr int ORDER_BLK = 256, ORDER_CHUNKS = 16;
+
+__global__ void __launch_bounds__(ORDER_BLK) k_order_day(const double *__restrict__ sza,
+                                                         int64_t nbox, int32_t *__restrict__ order,
+                                                         unsigned long long *cnt)
+{
+    constexpr int NW = ORDER_BLK / 64;
+    __shared__ unsigned s_cnt[2][NW];
+    __shared__ unsigned long long s_base[2];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t first = (int64_t)blockIdx.x * ORDER_BLK * ORDER_CHUNKS;
+    // the block's night and day counts, then one claim of each part
+    unsigned cn = 0, cd = 0;
+    for (int i = 0; i < ORDER_CHUNKS; ++i) {
+        const int64_t b = first + (int64_t)i * ORDER_BLK + threadIdx.x;
+        if (b < nbox) {
+            if (sza[b] <= 95.) ++cd;
+            else ++cn;
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        cn += __shfl_down(cn, off);
+        cd += __shfl_down(cd, off);
+    }
+    if (lane == 0) {
+        s_cnt[0][wave] = cn;
+        s_cnt[1][wave] = cd;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned tn = 0, td = 0;
+        for (int w = 0; w < NW; ++w) {
+            tn += s_cnt[0][w];
+            td += s_cnt[1][w];
+        }
+        s_base[0] = atomicAdd(cnt, (unsigned long long)tn);
+        s_base[1] = atomicAdd(cnt + 1, (unsigned long long)td);
+    }
+    __syncthreads();
+    unsigned long long pos[2] = {s_base[0], s_base[1]};
+    // the chunks in order: each box's rank among its part's boxes of the chunk
+    for (int i = 0; i < ORDER_CHUNKS; ++i) {
+        const int64_t b = first + (int64_t)i * ORDER_BLK + threadIdx.x;
+        const bool in = b < nbox;
+        const bool day = in && sza[b] <= 95.;
+        const bool night = in && !day;
+        const unsigned long long mn = __ballot(night), md = __ballot(day);
+        __syncthreads();   // every wave has read the previous chunk's counts
+        if (lane == 0) {
+            s_cnt[0][wave] = (unsigned)__popcll(mn);
+            s_cnt[1][wave] = (unsigned)__popcll(md);
+        }
+        __syncthreads();
+        unsigned before[2] = {0u, 0u}, tot[2] = {0u, 0u};
+        for (int w = 0; w < NW; ++w) {
+            if (w < wave) {
+                before[0] += s_cnt[0][w];
+                before[1] += s_cnt[1][w];
+            }
+            tot[0] += s_cnt[0][w];
+            tot[1] += s_cnt[1][w];
+        }
+        if (in) {
+            const int part = day ? 1 : 0;
+            const unsigned rank = before[part] +
+                                  (unsigned)__popcll((day ? md : mn) & ((1ull << lane) - 1ull));
+            const int64_t p = (int64_t)(pos[part] + rank);
+            order[day ? nbox - 1 - p : p] = (int32_t)b;
+        }
+        pos[0] += tot[0];
+        pos[1] += tot[1];
+    }
+}
+
+hipError_t launch_order_day(const double *sza, int64_t nbox, int32_t *order,
+                            unsigned long long *cnt, hipStream_t s)
+{
+    if (nbox < 1 || nbox > INT32_MAX) return hipErrorInvalidValue;
+    const int64_t per = (int64_t)ORDER_BLK * ORDER_CHUNKS;
+    const unsigned grid = (unsigned)((nbox + per - 1) / per);
+    hipLaunchKernelGGL(k_order_day, dim3(grid), dim3(ORDER_BLK), 0, s, sza, nbox, order, cnt);
     return hipGetLastError();
 }
 

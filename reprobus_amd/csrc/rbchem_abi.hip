@@ -20,7 +20,7 @@ struct rb_ctx {
     int device = 0;
     hipStream_t own = nullptr;
     hipStream_t stream = nullptr;
-    rb_config cfg{5, 0, 256, 0, 16, 60, 240, 2};
+    rb_config cfg{5, 0, 256, 0, 16, 60, 240, 2, 0};
     rb::Troe K{};
     double *d_ajl = nullptr, *d_o3up = nullptr;
     bool table = false;
@@ -29,7 +29,8 @@ struct rb_ctx {
     double *d_stage = nullptr;        // staging for the host-array entry points
     size_t stage_cap = 0;
     unsigned long long *d_count = nullptr;
-    unsigned long long *d_queue = nullptr;   // converged mode: queue heads, spill counts (5)
+    unsigned long long *d_queue = nullptr;   // converged mode: queue heads, spill counts (5),
+                                             // work-order counts (2)
     double *d_work = nullptr;                // altitude: level pressure, Tv, geopotential
     size_t work_cap = 0;
     double *d_aa = nullptr;                  // converged mode: Anderson history per lane
@@ -38,6 +39,8 @@ struct rb_ctx {
     size_t rec_cap = 0;
     double *d_spill = nullptr;               // accelerated mode: boxes between the phases
     size_t spill_cap = 0;
+    int32_t *d_order = nullptr;              // converged mode: the boxes in queue order
+    size_t order_cap = 0;
     double *d_tw = nullptr;                  // transport (host entry): indices, weights,
     size_t tw_cap = 0;                       // positions, levels
     double *d_tq = nullptr;                  // transport (host entry): fields in and out
@@ -102,31 +105,34 @@ int check_grid(rb_ctx *ctx, const rb_grid *g)
     return RB_OK;
 }
 
-int ensure(rb_ctx *ctx, double **buf, size_t *cap, size_t n)
+template <class T>
+int ensure(rb_ctx *ctx, T **buf, size_t *cap, size_t n)
 {
     if (*cap >= n) return RB_OK;
     if (*buf) (void)hipFree(*buf);
     *buf = nullptr;
     *cap = 0;
-    hipError_t e = hipMalloc((void **)buf, n * sizeof(double));
+    hipError_t e = hipMalloc((void **)buf, n * sizeof(T));
     if (e != hipSuccess) {
         // the runtime keeps the failure as the thread's last error, which the next launch
         // check (hipGetLastError) would report: take it here, where it belongs
         (void)hipGetLastError();
         *buf = nullptr;
         return fail(ctx, RB_ENOMEM, std::string("device allocation of ") +
-                                        std::to_string(n * sizeof(double)) + " bytes failed: " +
+                                        std::to_string(n * sizeof(T)) + " bytes failed: " +
                                         hipGetErrorString(e));
     }
     *cap = n;
     return RB_OK;
 }
 
-// scratch of the converged mode for nbox boxes: set-up records; with Anderson acceleration
-// also the per-lane history of the Anderson phase's grid and the spill list
+// scratch of the converged mode for nbox boxes: set-up records and the work order; with
+// Anderson acceleration also the per-lane history of the Anderson phase's grid and the
+// spill list
 int conv_buffers(rb_ctx *ctx, size_t nbox, int accel)
 {
     int rc = ensure(ctx, &ctx->d_rec, &ctx->rec_cap, (size_t)rb::CONV_RECORD * nbox);
+    if (!rc) rc = ensure(ctx, &ctx->d_order, &ctx->order_cap, nbox);
     if (rc || !accel) return rc;
     const int blk = ctx->cfg.block > 0 ? ctx->cfg.block : 256;
     const size_t lanes = (size_t)ctx->n_cu * rb::conv_blocks_per_cu(blk, accel, true) * blk;
@@ -200,7 +206,9 @@ hipError_t noalloc_scratch(int device, double **out)
 // The launches of one chemistry step (column pre-pass, set-up, sweeps, conserv below the
 // chemistry levels, the non-finite check) on caller-supplied work buffers: col (3 nbox
 // doubles) and rec (CONV_RECORD nbox doubles).  profile = false skips the per-launch events
-// (their creation allocates).
+// (their creation allocates).  The boxes go in grid order: the converged mode's night-first
+// order (launch_order_day) left the c1 step's two kernels at 1.017 against 1.021 ms
+// (63 % of the grid-order waves hold day and night boxes) and its launch cost more.
 int step_launch(rb_ctx *ctx, const rb_grid *g, double dt, const double *T, const double *p,
                 const double *sza, const double *alt, double *qj1, double *hc, double *surfarea,
                 double *vsed3d, double *parth2osed, double *parthno3sed, int fuse_conserv,
@@ -308,7 +316,7 @@ int rb_init(rb_ctx **out, int device, const rb_config *cfg)
     ctx->K = rb::troe_constants();
     if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&ctx->own) != hipSuccess ||
         hipMalloc((void **)&ctx->d_count, sizeof(unsigned long long)) != hipSuccess ||
-        hipMalloc((void **)&ctx->d_queue, 5 * sizeof(unsigned long long)) != hipSuccess ||
+        hipMalloc((void **)&ctx->d_queue, 7 * sizeof(unsigned long long)) != hipSuccess ||
         hipDeviceGetAttribute(&ctx->n_cu, hipDeviceAttributeMultiprocessorCount, device) !=
             hipSuccess ||
         hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess ||
@@ -449,11 +457,13 @@ static int converge_impl(rb_ctx *ctx, const rb_grid *g, double dt, double tol, i
     rb::ConvArgs cv{tol, cap, ctx->d_queue, iters, accel, accel ? ctx->d_aa : nullptr,
                     ctx->cfg.conv_refill, ctx->d_rec, ctx->cfg.conv_aa_start,
                     accel ? ctx->d_spill : nullptr, accept_tol};
+    cv.order = ctx->cfg.conv_order == 0 ? ctx->d_order : nullptr;
 
     if (!ctx->capturing) HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
     HIPCHK(rb::launch_columns(a, ctx->d_col, ctx->stream));
     HIPCHK(rb::launch_setup(a, ctx->d_rec, ctx->cfg.block, ctx->stream));
-    HIPCHK(rb::launch_zero_counter(ctx->d_queue, ctx->stream, 3));
+    HIPCHK(rb::launch_zero_counter(ctx->d_queue, ctx->stream, 7));
+    if (cv.order) HIPCHK(rb::launch_order_day(sza, nbox, ctx->d_order, ctx->d_queue + 5, ctx->stream));
     if (ctx->cfg.profile && !ctx->capturing) {
         if (ctx->prof_used == ctx->prof.size()) {
             hipEvent_t e0, e1;
@@ -537,6 +547,7 @@ int rb_chem_newton_dev(rb_ctx *ctx, const rb_grid *g, double dt, double tol, int
     rb::ConvArgs cv{tol, maxiter, ctx->d_queue, iters, 0, nullptr, ctx->cfg.conv_refill,
                     ctx->d_rec, 0, ctx->d_spill, 0.};
     cv.max_newton = max_newton;
+    cv.order = ctx->cfg.conv_order == 0 ? ctx->d_order : nullptr;
     rb::NewtonLaunch nl{};
     nl.sweeps_first = sweeps_first;
     nl.waves = ctx->cfg.newton_waves;
@@ -554,7 +565,8 @@ int rb_chem_newton_dev(rb_ctx *ctx, const rb_grid *g, double dt, double tol, int
     if (!ctx->capturing) HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
     HIPCHK(rb::launch_columns(a, ctx->d_col, ctx->stream));
     HIPCHK(rb::launch_setup(a, ctx->d_rec, blk, ctx->stream));
-    HIPCHK(rb::launch_zero_counter(ctx->d_queue, ctx->stream, 5));
+    HIPCHK(rb::launch_zero_counter(ctx->d_queue, ctx->stream, 7));
+    if (cv.order) HIPCHK(rb::launch_order_day(sza, nbox, ctx->d_order, ctx->d_queue + 5, ctx->stream));
     if (ctx->cfg.profile && !ctx->capturing) {
         if (ctx->prof_used == ctx->prof.size()) {
             hipEvent_t e0, e1;
@@ -1126,7 +1138,7 @@ int rb_run_day_dev(rb_ctx *ctx, const rb_grid *g, const rb_day_config *dc, const
     };
     const void *ptrs[] = {trajlon, trajlat, T, p, alt, sza, qj1, hc, surfarea, vsed3d,
                           parth2osed, parthno3sed, iters, ctx->stream, ctx->d_col, ctx->d_aa,
-                          ctx->d_rec, ctx->d_spill};
+                          ctx->d_rec, ctx->d_spill, ctx->d_order};
     add(g, sizeof *g);
     add(dc, sizeof *dc);
     add(ptrs, sizeof ptrs);
@@ -1200,6 +1212,7 @@ int rb_finalize(rb_ctx *ctx)
     if (ctx->d_aa) (void)hipFree(ctx->d_aa);
     if (ctx->d_rec) (void)hipFree(ctx->d_rec);
     if (ctx->d_spill) (void)hipFree(ctx->d_spill);
+    if (ctx->d_order) (void)hipFree(ctx->d_order);
     if (ctx->d_tw) (void)hipFree(ctx->d_tw);
     if (ctx->d_tq) (void)hipFree(ctx->d_tq);
     if (ctx->d_bt) (void)hipFree(ctx->d_bt);

@@ -40,6 +40,29 @@ def test_struct_layouts_match_header():
     from reprobus_amd import _lib
 
     assert C.sizeof(_lib.Grid) == 5 * 4
-    assert C.sizeof(_lib.Config) == 8 * 4
+    assert C.sizeof(_lib.Config) == 9 * 4
     assert C.sizeof(_lib.Stats) == 8 * 7
     assert C.sizeof(_lib.DayConfig) == 4 * 4 + 3 * 8 + 2 * 4
+
+
+def test_struct_sizes_match_the_compiled_header(tmp_path):
+    """The ctypes mirrors have the sizes the C compiler gives the header's structs."""
+    import ctypes as C
+    import shutil
+    import subprocess
+
+    from reprobus_amd import _lib
+
+    if shutil.which("gcc") is None:
+        pytest.skip("no C compiler")
+    src = tmp_path / "sizes.c"
+    src.write_text('#include <stdio.h>\n#include "rbchem.h"\nint main(void) {\n'
+                   '  printf("%zu %zu %zu %zu\\n", sizeof(rb_grid), sizeof(rb_config),\n'
+                   '         sizeof(rb_stats_t), sizeof(rb_day_config));\n  return 0;\n}\n')
+    exe = tmp_path / "sizes"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), "-o", str(exe), str(src)],
+                   check=True)
+    got = [int(v) for v in subprocess.run([str(exe)], check=True, capture_output=True,
+                                          text=True).stdout.split()]
+    assert got == [C.sizeof(_lib.Grid), C.sizeof(_lib.Config), C.sizeof(_lib.Stats),
+                   C.sizeof(_lib.DayConfig)]

@@ -201,3 +201,44 @@ def test_converged_rejects_bad_arguments(step):
     with pytest.raises(RbError, match="accel with a guess"):
         converge_gpu(step, inp, guess=(inp.qj1, inp.hc), accel=2)
     converge_gpu(step, inp, maxiter=50, accel=2)
+
+
+def test_work_order_changes_nothing(table):
+    """The converged kernels take boxes night first, then day (rb_config.conv_order 0), so
+    that a wave's lanes run one branch of the sweep.  A box's answer and sweep count do not
+    depend on the lane or the time it is swept in: grid order (conv_order 1) gives the same
+    bits, for plain and accelerated sweeps and for the Newton solve."""
+    import torch
+    from reprobus_amd import synth
+    from reprobus_amd.chem import ChemistryStep, make_grid
+
+    inp = synth.make_c2(1, nlon=24, nlat=13)
+    g = inp.grid
+    grid = make_grid(g.nlon, g.nlat, g.niv, g.nivbas, g.nivhau)
+    night = float((inp.sza[:g.nivbas] > 95.).mean())
+    print(f"[order] 24x13x{g.nivbas} c2 subgrid, night fraction {night:.2f}", flush=True)
+    assert 0.1 < night < 0.9
+    outs = {}
+    for order in (0, 1):
+        s = ChemistryStep(*table, device=0, conv_aa_start=AA_START, conv_order=order)
+        try:
+            for how in ("plain", "accel", "newton"):
+                d = _dev(inp)
+                args = [d[k] for k in ("T", "p", "sza", "alt", "qj1", "hc", "surfarea", "vsed3d",
+                                       "parth2osed", "parthno3sed")]
+                s.set_stream(torch.cuda.current_stream().cuda_stream)
+                if how == "newton":
+                    s.newton_device(grid, inp.dt, 1e-11, 20, 2000, *args, iters=d["iters"],
+                                    sweeps_first=30)
+                else:
+                    s.converge_device(grid, inp.dt, TOL, 2000, *args, iters=d["iters"],
+                                      accel=2 if how == "accel" else 0)
+                torch.cuda.synchronize()
+                s.set_stream(None)
+                outs[order, how] = {k: d[k].cpu().numpy() for k in ("qj1", "hc", "iters")}
+        finally:
+            s.close()
+    for how in ("plain", "accel", "newton"):
+        a, b = outs[0, how], outs[1, how]
+        for k in ("qj1", "hc", "iters"):
+            assert np.array_equal(a[k], b[k]), (how, k)
