@@ -1,7 +1,7 @@
 """Training data, training and use of the learned initial guess (reprobus_amd.mlp).
 
 Run ``python -m reprobus_amd.guess --train`` to regenerate ``data/guess_mlp.npz`` (the
-96-step c2 day of seed 0, 42,000 box-steps sampled per step, 12 epochs of Adam at batch
+96-step c2 day of seed 0, 42,000 box-steps sampled per step, 120 epochs of Adam at batch
 8192 and lr 1e-3, on the library's kernels) and ``--eval`` for the per-species error of
 the guess on the seed-1 day; ``--report FILE`` writes both as markdown.
 
@@ -187,7 +187,7 @@ def main(argv=None) -> int:
     ap.add_argument("--seed", type=int, default=0, help="c2 day of the training data")
     ap.add_argument("--steps", type=int, default=96)
     ap.add_argument("--per-step", type=int, default=42000)
-    ap.add_argument("--epochs", type=int, default=12)
+    ap.add_argument("--epochs", type=int, default=120)
     ap.add_argument("--batch", type=int, default=8192)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--tol", type=float, default=1e-11)
