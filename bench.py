@@ -68,7 +68,7 @@ def parse():
     ap.add_argument("--cpu-nrep", type=int, default=8)
     ap.add_argument("--no-extras", action="store_true",
                     help="only the headline step (for profiler runs)")
-    ap.add_argument("--extras", default="converged,surrogate,day,transport",
+    ap.add_argument("--extras", default="converged,surrogate,day,transport,mlp",
                     help="comma-separated sections reported beside the headline")
     ap.add_argument("--surrogate-steps", type=int, default=96,
                     help="sub-steps of the surrogate run (96 = one day, 960 = ten days)")
@@ -338,6 +338,45 @@ def converged_section(table, block_conv, refill, dev, newton_shape=(60, 1), tol=
     step.close()
     net.close()
     return out
+
+
+def mlp_section(dev, batch=8192, nbig=786_240, reps=20):
+    """The learned guess's own kernels: one Adam training step at the training batch (features
+    -> forward -> loss -> backward -> Adam, all BF16 MFMA GEMMs) and the GEMM forward chain
+    over a c1-sized batch; achieved BF16 rates on the GEMMs' algorithmic flops (the 60-, 55-
+    wide layers counted padded to 64, as computed)."""
+    import torch
+
+    from reprobus_amd.mlp import NIN, NOUT, GuessMLP, init_params
+
+    net = GuessMLP(0)
+    stream = torch.cuda.current_stream()
+    net.set_stream(stream.cuda_stream)
+    net.set_params(init_params(0))
+    g = torch.Generator(device=dev)
+    g.manual_seed(0)
+    X = torch.randn((nbig, NIN), generator=g, device=dev, dtype=torch.float32)
+    T = torch.randn((nbig, NOUT), generator=g, device=dev, dtype=torch.float32)
+    net.set_norm(np.zeros(NIN), np.ones(NIN), np.zeros(NOUT), np.ones(NOUT))
+    Y = torch.empty((nbig, NOUT), device=dev, dtype=torch.float32)
+    fwd_flop_row = 2 * (64 * 256 + 2 * 256 * 256 + 256 * 64)
+    # backward: the dX products of layers 2-4 and the dW products of all four
+    train_flop_row = fwd_flop_row + (fwd_flop_row - 2 * 64 * 256) + fwd_flop_row
+    Xb, Tb = X[:batch].contiguous(), T[:batch].contiguous()
+    net.train_step(Xb, Tb, 1e-3)
+    torch.cuda.synchronize()
+    t_ms = _timed(lambda: net.train_step(Xb, Tb, 1e-3), reps, stream)
+    net.forward(X, Y)
+    torch.cuda.synchronize()
+    f_ms = _timed(lambda: net.forward(X, Y), 5, stream)
+    net.close()
+    return {"train_step": {"batch": batch, "ms": t_ms,
+                           "tflops": batch * train_flop_row / (t_ms / 1e3) / 1e12,
+                           "note": "whole Adam step: GEMMs, loss, bias sums, Adam, BF16 copies"},
+            "forward_gemm_chain": {"rows": nbig, "ms": f_ms,
+                                   "tflops": nbig * fwd_flop_row / (f_ms / 1e3) / 1e12,
+                                   "peak_tflops": 2500.0},
+            "flop_per_row": {"forward": fwd_flop_row, "train": train_flop_row}}
 
 
 def _histogram(iters):
@@ -722,6 +761,11 @@ def main():
             extras["transport"] = transport_section(table, dev)
         except Exception as exc:
             extras["transport"] = {"error": repr(exc)}
+    if "mlp" in want:
+        try:
+            extras["mlp"] = mlp_section(dev)
+        except Exception as exc:
+            extras["mlp"] = {"error": repr(exc)}
 
     cpu = None
     if not args.no_cpu_baseline:

@@ -1025,26 +1025,43 @@ __global__ void k_columns(const double *__restrict__ p, const double *__restrict
     const double zcmt_t = 2.120e20, zcmt_j = 287.0 / (1.38e-23 * 9.81), r_o2 = 0.2095;
     const double *o3 = hc + (int64_t)O3 * field, *no = hc + (int64_t)NO * field;
     double o3t = 0., co2 = 0., cno = 0., hprev = 0., nprev = 0., pprev = 0., o3prev = 0.;
-    for (int k = 0; k < nivbas; ++k) {
-        const int64_t b = (int64_t)k * plane + c;
-        const double pk = p[b], hnm = pk / (T[b] * 1.38e-19), ccno = no[b] * hnm, o3k = o3[b];
-        if (k == 0) {
-            o3t = o3k * hnm * 6.8e5;                       // total, F:5170
-            co2 = 6.8e+05 * r_o2 * hnm;                     // jno, J:109
-            cno = 6.8e+05 * ccno;                           // jno, J:110
-        } else {
-            const double dp = fmax((pk - pprev) * 100., 0.);
-            o3t = o3t + zcmt_t * (o3k + o3prev) * .5 * dp;  // F:5176-5181
-            co2 = co2 + zcmt_j * (r_o2 + r_o2) * 0.5 * dp * 1.e-4;
-            cno = cno + zcmt_j * (nprev / hprev + ccno / hnm) * 0.5 * dp * 1.e-4;
+    // the levels in groups of COL_GROUP: a group's loads are issued together, then the
+    // scan runs through it in level order (one memory round trip per group, not per level)
+    constexpr int COL_GROUP = 8;
+    for (int k0 = 0; k0 < nivbas; k0 += COL_GROUP) {
+        double pv[COL_GROUP], tv[COL_GROUP], nv[COL_GROUP], ov[COL_GROUP];
+#pragma unroll
+        for (int u = 0; u < COL_GROUP; ++u) {
+            const int64_t b = (int64_t)(k0 + u < nivbas ? k0 + u : nivbas - 1) * plane + c;
+            pv[u] = p[b];
+            tv[u] = T[b];
+            nv[u] = no[b];
+            ov[u] = o3[b];
         }
-        col[b] = o3t;
-        col[nb + b] = co2;
-        col[2 * nb + b] = cno;
-        hprev = hnm;
-        nprev = ccno;
-        pprev = pk;
-        o3prev = o3k;
+#pragma unroll
+        for (int u = 0; u < COL_GROUP; ++u) {
+            const int k = k0 + u;
+            if (k >= nivbas) break;
+            const int64_t b = (int64_t)k * plane + c;
+            const double pk = pv[u], hnm = pk / (tv[u] * 1.38e-19), ccno = nv[u] * hnm, o3k = ov[u];
+            if (k == 0) {
+                o3t = o3k * hnm * 6.8e5;                       // total, F:5170
+                co2 = 6.8e+05 * r_o2 * hnm;                     // jno, J:109
+                cno = 6.8e+05 * ccno;                           // jno, J:110
+            } else {
+                const double dp = fmax((pk - pprev) * 100., 0.);
+                o3t = o3t + zcmt_t * (o3k + o3prev) * .5 * dp;  // F:5176-5181
+                co2 = co2 + zcmt_j * (r_o2 + r_o2) * 0.5 * dp * 1.e-4;
+                cno = cno + zcmt_j * (nprev / hprev + ccno / hnm) * 0.5 * dp * 1.e-4;
+            }
+            col[b] = o3t;
+            col[nb + b] = co2;
+            col[2 * nb + b] = cno;
+            hprev = hnm;
+            nprev = ccno;
+            pprev = pk;
+            o3prev = o3k;
+        }
     }
 }
 

@@ -50,6 +50,47 @@ __device__ __forceinline__ float gelu_grad(float z)
 // lane fetches its eight operand values with one 16-byte LDS read.
 constexpr int TM = 64, TN = 64, TK = 32, LDS_K = TK + 8;   // +8: spreads rows over banks
 
+// C tile epilogue of the GEMM kernels: lane l holds C rows 4 * (l / 16) + e, e = 0..3,
+// column l % 16 of each of its wave's 2 x 2 tiles
+__device__ __forceinline__ void gemm_epilogue(const GemmArgs &g, const f32x4 (&acc)[2][2],
+                                              int64_t m0, int n0, int wm, int wn, int lane)
+{
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int64_t m = m0 + wm + i * 16 + (lane >> 4) * 4 + e;
+                const int n = n0 + wn + j * 16 + (lane & 15);
+                if (m >= g.M || n >= g.N) continue;
+                float v = acc[i][j][e];
+                switch (g.mode) {
+                case EPI_F32:
+                    if (g.ksplit > 1)
+                        g.part[((int64_t)blockIdx.z * g.M + m) * g.N + n] = v;
+                    else
+                        g.Cf[m * g.ldc + n] = v;
+                    break;
+                case EPI_HIDDEN_FWD: {
+                    v += g.bias[n];
+                    if (g.Cf) g.Cf[m * g.ldc + n] = v;
+                    g.Cb[m * g.ldc + n] = (__bf16)gelu(v);
+                    break;
+                }
+                case EPI_OUT_FWD:
+                    g.Cf[m * g.ldc + n] = (n < g.n_valid) ? v + g.bias[n] : 0.f;
+                    break;
+                case EPI_HIDDEN_BWD: {
+                    v *= gelu_grad(g.Z[m * g.ldc + n]);
+                    g.Cf[m * g.ldc + n] = v;
+                    g.Cb[m * g.ldc + n] = (__bf16)v;
+                    break;
+                }
+                }
+            }
+}
+
 __global__ void __launch_bounds__(256) k_gemm(GemmArgs g)
 {
     __shared__ __attribute__((aligned(16))) __bf16 sA[TM * LDS_K];
@@ -106,41 +147,125 @@ __global__ void __launch_bounds__(256) k_gemm(GemmArgs g)
         __syncthreads();
     }
 
-    // epilogue: lane l holds C rows 4 * (l / 16) + e, e = 0..3, column l % 16 of each tile
+    gemm_epilogue(g, acc, m0, n0, wm, wn, lane);
+}
+
+// The same product with staged operands: 16-byte global loads, 64-deep k slabs double
+// buffered in LDS (one barrier per slab), for operands whose contiguous dimension is k
+// (AK / BK) or m / n (the transposed operands of the gradients: each 16-byte load then
+// carries 8 rows, written to the k-contiguous LDS rows one value at a time).  The tile, the
+// wave layout, the MFMA sequence (k ascending in steps of 32) and the epilogue are k_gemm's,
+// so the sums accumulate in the same order.  Needs 16-byte-aligned chunks: K a multiple of
+// 8, and the strides and the m / n extent of a transposed operand multiples of 8
+// (gemm_staged_ok).
+constexpr int SK = 64, LDS_SK = SK + 8;
+
+template <bool AK, bool BK>
+__global__ void __launch_bounds__(256) k_gemm_staged(GemmArgs g)
+{
+    __shared__ __attribute__((aligned(16))) __bf16 sA[2][TM * LDS_SK];
+    __shared__ __attribute__((aligned(16))) __bf16 sB[2][TN * LDS_SK];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int64_t m0 = (int64_t)blockIdx.y * TM;
+    const int n0 = blockIdx.x * TN;
+    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+    // k_gemm's split of K (multiples of TK): a last slab of a split may be half empty (zero)
+    const int kchunk = ((g.K + g.ksplit - 1) / g.ksplit + TK - 1) / TK * TK;
+    const int kbeg = blockIdx.z * kchunk;
+    const int kend = min(g.K, kbeg + kchunk);
+
+    f32x4 acc[2][2];
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j)
+        for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+    // a slab is 64 x 64 values of each operand = 512 chunks of 8: two per thread
+    bf16x8 ra[2], rb[2];
+    const bf16x8 zero8 = bf16x8{};
+    auto load = [&](int k0) {
 #pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const int64_t m = m0 + wm + i * 16 + (lane >> 4) * 4 + e;
-                const int n = n0 + wn + j * 16 + (lane & 15);
-                if (m >= g.M || n >= g.N) continue;
-                float v = acc[i][j][e];
-                switch (g.mode) {
-                case EPI_F32:
-                    if (g.ksplit > 1)
-                        g.part[((int64_t)blockIdx.z * g.M + m) * g.N + n] = v;
-                    else
-                        g.Cf[m * g.ldc + n] = v;
-                    break;
-                case EPI_HIDDEN_FWD: {
-                    v += g.bias[n];
-                    if (g.Cf) g.Cf[m * g.ldc + n] = v;
-                    g.Cb[m * g.ldc + n] = (__bf16)gelu(v);
-                    break;
-                }
-                case EPI_OUT_FWD:
-                    g.Cf[m * g.ldc + n] = (n < g.n_valid) ? v + g.bias[n] : 0.f;
-                    break;
-                case EPI_HIDDEN_BWD: {
-                    v *= gelu_grad(g.Z[m * g.ldc + n]);
-                    g.Cf[m * g.ldc + n] = v;
-                    g.Cb[m * g.ldc + n] = (__bf16)v;
-                    break;
-                }
-                }
+        for (int h = 0; h < 2; ++h) {
+            const int c = tid + h * 256;
+            const int r = c >> 3, q = (c & 7) * 8;   // r: row of the chunk grid, q: first of 8
+            if (AK) {   // chunk: row m0 + r, k = k0 + q .. + 7
+                const int64_t gm = m0 + r;
+                const int gk = k0 + q;
+                ra[h] = (gm < g.M && gk < kend)
+                            ? *reinterpret_cast<const bf16x8 *>(g.A + gm * g.sam + gk)
+                            : zero8;
+            } else {    // chunk: k = k0 + r, rows m0 + q .. + 7
+                const int64_t gm = m0 + q;
+                const int gk = k0 + r;
+                ra[h] = (gm < g.M && gk < kend)
+                            ? *reinterpret_cast<const bf16x8 *>(g.A + (int64_t)gk * g.sak + gm)
+                            : zero8;
             }
+            if (BK) {   // chunk: column n0 + r, k = k0 + q .. + 7
+                const int gn = n0 + r;
+                const int gk = k0 + q;
+                rb[h] = (gn < g.N && gk < kend)
+                            ? *reinterpret_cast<const bf16x8 *>(g.B + (int64_t)gn * g.sbn + gk)
+                            : zero8;
+            } else {    // chunk: k = k0 + r, columns n0 + q .. + 7
+                const int gn = n0 + q;
+                const int gk = k0 + r;
+                rb[h] = (gn < g.N && gk < kend)
+                            ? *reinterpret_cast<const bf16x8 *>(g.B + (int64_t)gk * g.sbk + gn)
+                            : zero8;
+            }
+        }
+    };
+    auto store = [&](int buf) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int c = tid + h * 256;
+            const int r = c >> 3, q = (c & 7) * 8;
+            if (AK) {
+                *reinterpret_cast<bf16x8 *>(&sA[buf][r * LDS_SK + q]) = ra[h];
+            } else {
+#pragma unroll
+                for (int e = 0; e < 8; ++e) sA[buf][(q + e) * LDS_SK + r] = ra[h][e];
+            }
+            if (BK) {
+                *reinterpret_cast<bf16x8 *>(&sB[buf][r * LDS_SK + q]) = rb[h];
+            } else {
+#pragma unroll
+                for (int e = 0; e < 8; ++e) sB[buf][(q + e) * LDS_SK + r] = rb[h][e];
+            }
+        }
+    };
+
+    const int r16 = lane & 15, kq = (lane >> 4) * 8;
+    int buf = 0;
+    if (kbeg < kend) {
+        load(kbeg);
+        store(0);
+    }
+    __syncthreads();
+    for (int k0 = kbeg; k0 < kend; k0 += SK) {
+        const bool more = k0 + SK < kend;
+        if (more) load(k0 + SK);   // in flight while this slab is multiplied
+#pragma unroll
+        for (int kk = 0; kk < SK; kk += 32) {
+            bf16x8 va[2], vb[2];
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+                va[i] = *reinterpret_cast<const bf16x8 *>(&sA[buf][(wm + i * 16 + r16) * LDS_SK + kk + kq]);
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+                vb[j] = *reinterpret_cast<const bf16x8 *>(&sB[buf][(wn + j * 16 + r16) * LDS_SK + kk + kq]);
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(va[i], vb[j], acc[i][j], 0, 0, 0);
+        }
+        if (more) store(buf ^ 1);
+        __syncthreads();
+        buf ^= 1;
+    }
+    gemm_epilogue(g, acc, m0, n0, wm, wn, lane);
 }
 
 // sums the ksplit partials (M x N each) into C (leading dimension ldc)
@@ -154,6 +279,21 @@ __global__ void k_splitk_reduce(const float *__restrict__ part, int ksplit, int6
     C[(i / N) * ldc + i % N] = acc;
 }
 
+// the operand layouts k_gemm_staged takes: 16-byte chunks along the contiguous dimension,
+// 16-byte aligned (K, strides and transposed extents multiples of 8, aligned bases)
+static bool gemm_staged_ok(const GemmArgs &g, bool &ak, bool &bk)
+{
+    const auto al = [](const void *p) { return ((uintptr_t)p & 15u) == 0; };
+    if (g.K % 8 != 0 || !al(g.A) || !al(g.B)) return false;
+    if (g.sak == 1 && g.sam % 8 == 0) ak = true;
+    else if (g.sam == 1 && g.sak % 8 == 0 && g.M % 8 == 0) ak = false;
+    else return false;
+    if (g.sbk == 1 && g.sbn % 8 == 0) bk = true;
+    else if (g.sbn == 1 && g.sbk % 8 == 0 && g.N % 8 == 0) bk = false;
+    else return false;
+    return true;
+}
+
 hipError_t launch_gemm(const GemmArgs &gin, hipStream_t s)
 {
     GemmArgs g = gin;
@@ -161,7 +301,15 @@ hipError_t launch_gemm(const GemmArgs &gin, hipStream_t s)
     if (g.ksplit < 1 || g.mode != EPI_F32 || !g.part) g.ksplit = 1;
     if (g.ksplit > MAX_KSPLIT) return hipErrorInvalidValue;
     dim3 grid((unsigned)((g.N + TN - 1) / TN), (unsigned)((g.M + TM - 1) / TM), (unsigned)g.ksplit);
-    hipLaunchKernelGGL(k_gemm, grid, dim3(256), 0, s, g);
+    bool ak = false, bk = false;
+    if (gemm_staged_ok(g, ak, bk)) {
+        if (ak && bk) hipLaunchKernelGGL((k_gemm_staged<true, true>), grid, dim3(256), 0, s, g);
+        else if (ak) hipLaunchKernelGGL((k_gemm_staged<true, false>), grid, dim3(256), 0, s, g);
+        else if (bk) hipLaunchKernelGGL((k_gemm_staged<false, true>), grid, dim3(256), 0, s, g);
+        else hipLaunchKernelGGL((k_gemm_staged<false, false>), grid, dim3(256), 0, s, g);
+    } else {
+        hipLaunchKernelGGL(k_gemm, grid, dim3(256), 0, s, g);
+    }
     if (g.ksplit > 1)
         hipLaunchKernelGGL(k_splitk_reduce, dim3((unsigned)((g.M * g.N + 255) / 256)), dim3(256), 0, s,
                            g.part, g.ksplit, g.M, g.N, g.Cf, g.ldc);
