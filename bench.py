@@ -47,13 +47,13 @@ FLOP_PER_BOX = 2201 + 3544 + 595 + 13 + 166
 # Measured HBM bytes per step of the per-box kernels on this workload (k_setup + k_chem),
 # per block size: FETCH_SIZE x 2 + WRITE_SIZE in KiB, the factors from a 1 GiB copy of
 # known traffic (profiles/k_chem_c1_pmc.md).
-TRAFFIC_BYTES = {64: (2 * (47862 + 678354) + 869164 + 520782) * 1024,
-                 128: (2 * (47746 + 684205) + 869164 + 533067) * 1024,
-                 256: (2 * (49534 + 707407) + 869164 + 582207) * 1024}
+TRAFFIC_BYTES = {64: round((2 * (45985.94 + 451234.69) + 365464.75 + 557634.63) * 1024),
+                 128: round((2 * (45851.00 + 457743.44) + 365464.38 + 572992.88) * 1024),
+                 256: round((2 * (45768.75 + 473824.38) + 365468.50 + 606776.63) * 1024)}
 # FP64 VALU instructions per step of the two kernels (SQ_INSTS_VALU_*_F64, same profile,
 # equal for every block size): add + mul + transcendental + 2 x fma, per lane
-FP64_INSTR_FLOP = ((34.400e6 + 24.511e6) + (19.942e6 + 34.126e6) + (2.318e6 + 5.265e6)
-                   + 2 * (44.572e6 + 31.644e6)) * 64
+FP64_INSTR_FLOP = ((27.64e6 + 31.27e6) + (14.75e6 + 39.33e6) + (1.74e6 + 5.84e6)
+                   + 2 * (25.94e6 + 50.28e6)) * 64
 
 
 def parse():

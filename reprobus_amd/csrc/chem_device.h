@@ -74,9 +74,10 @@ struct ConvArgs {
 // doubles per box of the set-up record
 constexpr int CONV_RECORD = 141;
 // the set-up records of boxes box0 .. box0 + nr - 1 (nr < 0: to the last box), record r of
-// box box0 + r at stride nr
+// box box0 + r at stride nr; rates = false leaves the 82 thermal rates out of the record
+// (the fixed-sweep step: launch_chem computes them itself)
 hipError_t launch_setup(const StepArgs &a, double *rec, int block, hipStream_t s,
-                        int64_t box0 = 0, int64_t nr = -1);
+                        int64_t box0 = 0, int64_t nr = -1, bool rates = true);
 // Anderson depths the converged kernel is built for
 constexpr int AA_DEPTH = 2;
 constexpr int AA_NX = 57;
@@ -138,6 +139,7 @@ hipError_t launch_backtra_step(const BacktraArgs &a, hipStream_t s);
 
 Troe troe_constants();
 hipError_t launch_columns(const StepArgs &a, double *col, hipStream_t s);
+// the fixed-sweep step from records written by launch_setup with rates = false
 hipError_t launch_chem(const StepArgs &a, const double *rec, int block, hipStream_t s,
                        int64_t box0 = 0, int64_t nr = -1);
 // plain: one launch of nblocks_sweep blocks; accelerated: the sweep phase on nblocks_sweep

@@ -1241,6 +1241,7 @@ __device__ __forceinline__ void rec_store(const Box &x, double *__restrict__ rec
 
 // The set-up of box b from its record, as the set-up left it (rec_load is exact: the
 // densities are the same products of the air density, the flags small integers).
+template <bool LOAD_R = true>
 __device__ __forceinline__ void rec_load(Box &x, const double *__restrict__ rec, int64_t nbox,
                                          int64_t b, double dt)
 {
@@ -1258,6 +1259,7 @@ __device__ __forceinline__ void rec_load(Box &x, const double *__restrict__ rec,
     const double fl = REC_GET(REC_FLAGS);
     e.day = fl == 1. || fl == 3.;
     x.het_level = fl >= 2.;
+    if (LOAD_R) {
     x.R.a1 = REC_GET(REC_R + 0);
     x.R.a1et = REC_GET(REC_R + 1);
     x.R.a2 = REC_GET(REC_R + 2);
@@ -1340,6 +1342,7 @@ __device__ __forceinline__ void rec_load(Box &x, const double *__restrict__ rec,
     x.R.hk3 = REC_GET(REC_R + 79);
     x.R.hk4 = REC_GET(REC_R + 80);
     x.R.hk5 = REC_GET(REC_R + 81);
+    }
 #pragma unroll
     for (int n = 0; n < NJ; ++n) x.tj[n] = REC_GET(REC_J + n);
     x.h.k[0] = REC_GET(REC_H + 0);
@@ -1372,7 +1375,7 @@ __device__ __forceinline__ void rec_load(Box &x, const double *__restrict__ rec,
 // computed, so that no stage's registers outlive it.  At two waves per SIMD (~250 VGPRs)
 // the transcendental chains and table gathers of one wave overlap another's.
 // Boxes box0 .. box0 + nr - 1; the record of box box0 + r at index r (stride nr).
-template <int BLK>
+template <int BLK, bool STORE_R>
 __global__ void __launch_bounds__(BLK, 2) k_setup(StepArgs a, double *__restrict__ rec,
                                                   int64_t box0, int64_t nr)
 {
@@ -1391,8 +1394,10 @@ __global__ void __launch_bounds__(BLK, 2) k_setup(StepArgs a, double *__restrict
     e.h2 = 5.e-7 * e.hnm;
     e.dt = a.dt;
     e.day = sza <= 95.;
-    thermal_rates(a.K, t, e.hnm, pm, x.R);
-    rec_store<REC_R, REC_J>(x, rec, nr, r);
+    if (STORE_R) {
+        thermal_rates(a.K, t, e.hnm, pm, x.R);
+        rec_store<REC_R, REC_J>(x, rec, nr, r);
+    }
     table_j(a.ajl, a.o3up, a.alt[b], sza, a.col[b], x.tj);
     x.tj[JNO] = j_no(a.col[a.nbox + b], a.col[2 * a.nbox + b], a.col[b], e.hnm, a.sza[c]);
     rec_store<REC_J, REC_H>(x, rec, nr, r);
@@ -1429,12 +1434,15 @@ __global__ void __launch_bounds__(BLK, 2) k_setup(StepArgs a, double *__restrict
 // The state of box b from its set-up record (chem3d before its sweep loop, F:800-1338),
 // record r of stride nr: the iterate from the incoming state (or, with use_guess and a
 // guess, from the guess), the start-of-step state to the LDS column ck0 (stride BLK).
-template <int BLK>
+template <int BLK, bool LOAD_R = true>
 __device__ __forceinline__ void box_load(const StepArgs &a, const double *__restrict__ rec,
                                          int64_t nr, int64_t r, int64_t b, Box &x, double *ck0,
                                          bool use_guess)
 {
-    rec_load(x, rec, nr, r, a.dt);
+    rec_load<LOAD_R>(x, rec, nr, r, a.dt);
+    // without the rates in the record: the set-up's own computation from the same T, p and
+    // air density, so the same bits
+    if (!LOAD_R) thermal_rates(a.K, a.T[b], x.e.hnm, a.p[b], x.R);
     const double hnm = x.e.hnm;
 #pragma unroll
     for (int s = 0; s < NLONG; ++s) {
@@ -1457,7 +1465,10 @@ __device__ __forceinline__ void box_load(const StepArgs &a, const double *__rest
 // lane per box; conserv and the floor fused into the write-back when asked.  The set-up
 // (k_setup) runs as its own launch at two waves per SIMD: in one kernel with the sweeps it
 // ran at the sweeps' one wave per SIMD (the box state needs ~2 KB of registers) and took
-// three quarters of the step.
+// three quarters of the step.  The 82 thermal rates are the exception: this kernel
+// computes them from T and p (the record leaves them out) -- its VALU is idle three
+// quarters of the time while its loads are in flight, and the rates' 82 record fields,
+// written and read back, cost more (the c1 pair at block 64: 0.924 ms against 1.004).
 // Boxes box0 .. box0 + nr - 1 from records 0 .. nr - 1 (k_setup's range).
 template <int BLK>
 __global__ void __launch_bounds__(BLK) k_chem(StepArgs a, const double *__restrict__ rec,
@@ -1476,7 +1487,7 @@ __global__ void __launch_bounds__(BLK) k_chem(StepArgs a, const double *__restri
     if (r >= nr) return;
     const int64_t b = box0 + r;
     Box x;
-    box_load<BLK>(a, rec, nr, r, b, x, ck0, false);
+    box_load<BLK, false>(a, rec, nr, r, b, x, ck0, false);
 #pragma unroll
     for (int i = 0; i < NJL; ++i) tjl[i * BLK] = x.tj[JL_SET[i]];
     Track notrack{true, true, 0., 0., 0.};
@@ -2120,19 +2131,27 @@ hipError_t launch_columns(const StepArgs &a, double *col, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launch_setup(const StepArgs &a, double *rec, int block, hipStream_t s, int64_t box0,
-                        int64_t nr)
+template <bool R>
+static hipError_t launch_setup_t(const StepArgs &a, double *rec, int blk, hipStream_t s,
+                                 int64_t box0, int64_t nr)
 {
-    const int blk = block > 0 ? block : 256;
-    if (nr < 0) nr = a.nbox - box0;
     const unsigned grid = (unsigned)((nr + blk - 1) / blk);
     switch (blk) {
-    case 64: hipLaunchKernelGGL(k_setup<64>, dim3(grid), dim3(64), 0, s, a, rec, box0, nr); break;
-    case 128: hipLaunchKernelGGL(k_setup<128>, dim3(grid), dim3(128), 0, s, a, rec, box0, nr); break;
-    case 256: hipLaunchKernelGGL(k_setup<256>, dim3(grid), dim3(256), 0, s, a, rec, box0, nr); break;
+    case 64: hipLaunchKernelGGL((k_setup<64, R>), dim3(grid), dim3(64), 0, s, a, rec, box0, nr); break;
+    case 128: hipLaunchKernelGGL((k_setup<128, R>), dim3(grid), dim3(128), 0, s, a, rec, box0, nr); break;
+    case 256: hipLaunchKernelGGL((k_setup<256, R>), dim3(grid), dim3(256), 0, s, a, rec, box0, nr); break;
     default: return hipErrorInvalidConfiguration;
     }
     return hipGetLastError();
+}
+
+hipError_t launch_setup(const StepArgs &a, double *rec, int block, hipStream_t s, int64_t box0,
+                        int64_t nr, bool rates)
+{
+    const int blk = block > 0 ? block : 256;
+    if (nr < 0) nr = a.nbox - box0;
+    return rates ? launch_setup_t<true>(a, rec, blk, s, box0, nr)
+                 : launch_setup_t<false>(a, rec, blk, s, box0, nr);
 }
 
 hipError_t launch_chem(const StepArgs &a, const double *rec, int block, hipStream_t s,
