@@ -74,10 +74,10 @@ struct ConvArgs {
 // doubles per box of the set-up record
 constexpr int CONV_RECORD = 141;
 // the set-up records of boxes box0 .. box0 + nr - 1 (nr < 0: to the last box), record r of
-// box box0 + r at stride nr; rates = false leaves the 82 thermal rates out of the record
-// (the fixed-sweep step: launch_chem computes them itself)
+// box box0 + r at stride nr; full = false writes the fixed-sweep step's record, the J-values
+// and the aerosol terms (launch_chem computes the rest itself)
 hipError_t launch_setup(const StepArgs &a, double *rec, int block, hipStream_t s,
-                        int64_t box0 = 0, int64_t nr = -1, bool rates = true);
+                        int64_t box0 = 0, int64_t nr = -1, bool full = true);
 // Anderson depths the converged kernel is built for
 constexpr int AA_DEPTH = 2;
 constexpr int AA_NX = 57;

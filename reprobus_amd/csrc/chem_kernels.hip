@@ -1241,12 +1241,13 @@ __device__ __forceinline__ void rec_store(const Box &x, double *__restrict__ rec
 
 // The set-up of box b from its record, as the set-up left it (rec_load is exact: the
 // densities are the same products of the air density, the flags small integers).
-template <bool LOAD_R = true>
+template <bool LOAD_R = true, bool LOAD_ENV = true, bool LOAD_J = true>
 __device__ __forceinline__ void rec_load(Box &x, const double *__restrict__ rec, int64_t nbox,
                                          int64_t b, double dt)
 {
 #define REC_GET(f) rec[(int64_t)(f) * nbox + b]
     BoxEnv &e = x.e;
+    if (LOAD_ENV) {
     e.hnm = REC_GET(REC_HNM);
     e.n2 = 0.78 * e.hnm;
     e.o2 = 0.21 * e.hnm;
@@ -1259,6 +1260,7 @@ __device__ __forceinline__ void rec_load(Box &x, const double *__restrict__ rec,
     const double fl = REC_GET(REC_FLAGS);
     e.day = fl == 1. || fl == 3.;
     x.het_level = fl >= 2.;
+    }
     if (LOAD_R) {
     x.R.a1 = REC_GET(REC_R + 0);
     x.R.a1et = REC_GET(REC_R + 1);
@@ -1343,8 +1345,10 @@ __device__ __forceinline__ void rec_load(Box &x, const double *__restrict__ rec,
     x.R.hk4 = REC_GET(REC_R + 80);
     x.R.hk5 = REC_GET(REC_R + 81);
     }
+    if (LOAD_J) {
 #pragma unroll
     for (int n = 0; n < NJ; ++n) x.tj[n] = REC_GET(REC_J + n);
+    }
     x.h.k[0] = REC_GET(REC_H + 0);
     x.h.k[1] = REC_GET(REC_H + 1);
     x.h.k[2] = REC_GET(REC_H + 2);
@@ -1375,17 +1379,12 @@ __device__ __forceinline__ void rec_load(Box &x, const double *__restrict__ rec,
 // computed, so that no stage's registers outlive it.  At two waves per SIMD (~250 VGPRs)
 // the transcendental chains and table gathers of one wave overlap another's.
 // Boxes box0 .. box0 + nr - 1; the record of box box0 + r at index r (stride nr).
-template <int BLK, bool STORE_R>
-__global__ void __launch_bounds__(BLK, 2) k_setup(StepArgs a, double *__restrict__ rec,
-                                                  int64_t box0, int64_t nr)
+// The air density and the fixed fractions of it, the day flag and the level flag of box b
+// (chem3d F:800-851: hnm = p / (1.38e-19 T), N2, O2, CO2, H2; day where SZA <= 95;
+// heterogeneous chemistry on levels nivhau..nivbas).
+__device__ __forceinline__ void box_env(const StepArgs &a, int64_t b, double t, double pm,
+                                        Box &x)
 {
-    const int64_t r = (int64_t)blockIdx.x * BLK + threadIdx.x;
-    if (r >= nr) return;
-    const int64_t b = box0 + r;
-    const int64_t c = b % a.plane;
-    const int k = (int)(b / a.plane);
-    const double t = a.T[b], pm = a.p[b], sza = a.sza[b];
-    Box x;
     BoxEnv &e = x.e;
     e.hnm = pm / (t * 1.38e-19);
     e.n2 = 0.78 * e.hnm;
@@ -1393,18 +1392,58 @@ __global__ void __launch_bounds__(BLK, 2) k_setup(StepArgs a, double *__restrict
     e.co2 = 360.e-6 * e.hnm;
     e.h2 = 5.e-7 * e.hnm;
     e.dt = a.dt;
-    e.day = sza <= 95.;
-    if (STORE_R) {
+    e.day = a.sza[b] <= 95.;
+    const int k = (int)(b / a.plane);
+    x.het_level = (k + 1 >= a.nivhau) && (k + 1 <= a.nivbas);
+}
+
+// rain-out above 100 ppmv water (F:1315-1338), from the box's water density
+__device__ __forceinline__ void box_rain(Box &x)
+{
+    BoxEnv &e = x.e;
+    const bool wet = x.ck[H2O] / e.hnm >= 100.e-6;
+    const double rr = wet ? 1.e-6 : 0.;
+    e.rain_hno3 = rr; e.rain_hcl = rr; e.rain_hbr = rr; e.rain_h2o2 = rr;
+    e.rain_hno4 = 0.; e.rain_ch2o = 0.; e.rain_ch3o2h = 0.;
+}
+
+// the 41 J-values of box b (phot F:4996-5149 on the column's O3, jno J:1-249 on the column
+// sums and the column top's SZA)
+__device__ __forceinline__ void box_j(const StepArgs &a, int64_t b, Box &x)
+{
+    const int64_t c = b % a.plane;
+    table_j(a.ajl, a.o3up, a.alt[b], a.sza[b], a.col[b], x.tj);
+    x.tj[JNO] = j_no(a.col[a.nbox + b], a.col[2 * a.nbox + b], a.col[b], x.e.hnm, a.sza[c]);
+}
+
+// What a set-up record holds: everything (the converged mode, whose persistent kernels set
+// a box up at each refill), or what the fixed-sweep step's k_chem does not compute itself:
+// the J-values and the aerosol terms.  (With the J-values computed in k_chem too, the c1
+// pair took 0.910 / 0.948 ms at block 64 / 128 against 0.914 / 0.940: no gain for 320 table
+// gathers more in the kernel that waits on its loads.)
+enum RecMode : int { REC_FULL = 0, REC_STEP = 1 };
+
+template <int BLK, int MODE>
+__global__ void __launch_bounds__(BLK, 2) k_setup(StepArgs a, double *__restrict__ rec,
+                                                  int64_t box0, int64_t nr)
+{
+    constexpr bool FULL = MODE == REC_FULL;
+    const int64_t r = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    if (r >= nr) return;
+    const int64_t b = box0 + r;
+    const double t = a.T[b], pm = a.p[b];
+    Box x;
+    BoxEnv &e = x.e;
+    box_env(a, b, t, pm, x);
+    if (FULL) {
         thermal_rates(a.K, t, e.hnm, pm, x.R);
         rec_store<REC_R, REC_J>(x, rec, nr, r);
     }
-    table_j(a.ajl, a.o3up, a.alt[b], sza, a.col[b], x.tj);
-    x.tj[JNO] = j_no(a.col[a.nbox + b], a.col[2 * a.nbox + b], a.col[b], e.hnm, a.sza[c]);
+    box_j(a, b, x);
     rec_store<REC_J, REC_H>(x, rec, nr, r);
 #pragma unroll
     for (int s = 0; s < NLONG; ++s) x.ck[s] = a.qj1[(int64_t)s * a.field + b] * e.hnm;
     Het &h = x.h;
-    x.het_level = (k + 1 >= a.nivhau) && (k + 1 <= a.nivbas);
     if (x.het_level) {
         psc_and_het(t, pm, e.hnm, x.ck, h);
     } else {
@@ -1420,29 +1459,30 @@ __global__ void __launch_bounds__(BLK, 2) k_setup(StepArgs a, double *__restrict
     e.ph2o = h.ph2o;
     e.phcl = h.phcl;
     e.pbr = 1.;
-    {
-        // rain-out above 100 ppmv water (F:1315-1338)
-        const bool wet = x.ck[H2O] / e.hnm >= 100.e-6;
-        const double rr = wet ? 1.e-6 : 0.;
-        e.rain_hno3 = rr; e.rain_hcl = rr; e.rain_hbr = rr; e.rain_h2o2 = rr;
-        e.rain_hno4 = 0.; e.rain_ch2o = 0.; e.rain_ch3o2h = 0.;
+    if (FULL) {
+        box_rain(x);
+        rec_store<REC_HNM, REC_R>(x, rec, nr, r);
     }
-    rec_store<REC_HNM, REC_R>(x, rec, nr, r);
     rec_store<REC_H, NREC>(x, rec, nr, r);
 }
 
 // The state of box b from its set-up record (chem3d before its sweep loop, F:800-1338),
 // record r of stride nr: the iterate from the incoming state (or, with use_guess and a
 // guess, from the guess), the start-of-step state to the LDS column ck0 (stride BLK).
-template <int BLK, bool LOAD_R = true>
+template <int BLK, int MODE = REC_FULL>
 __device__ __forceinline__ void box_load(const StepArgs &a, const double *__restrict__ rec,
                                          int64_t nr, int64_t r, int64_t b, Box &x, double *ck0,
                                          bool use_guess)
 {
-    rec_load<LOAD_R>(x, rec, nr, r, a.dt);
-    // without the rates in the record: the set-up's own computation from the same T, p and
-    // air density, so the same bits
-    if (!LOAD_R) thermal_rates(a.K, a.T[b], x.e.hnm, a.p[b], x.R);
+    constexpr bool FULL = MODE == REC_FULL;
+    if (!FULL) {
+        // what the step's record leaves out: k_setup's own computations from the same
+        // inputs, so the same bits
+        const double t = a.T[b], pm = a.p[b];
+        box_env(a, b, t, pm, x);
+        thermal_rates(a.K, t, x.e.hnm, pm, x.R);
+    }
+    rec_load<FULL, FULL, true>(x, rec, nr, r, a.dt);
     const double hnm = x.e.hnm;
 #pragma unroll
     for (int s = 0; s < NLONG; ++s) {
@@ -1453,6 +1493,7 @@ __device__ __forceinline__ void box_load(const StepArgs &a, const double *__rest
     for (int s = 0; s < NSHORT; ++s) x.cc[s] = a.hc[(int64_t)s * a.field + b] * hnm;
     x.ch3o = 0.;
     x.hco = 0.;
+    if (!FULL) box_rain(x);
     if (use_guess && a.guess_q) {
 #pragma unroll
         for (int s = 0; s < NLONG - 1; ++s) x.ck[s] = a.guess_q[(int64_t)s * a.field + b] * hnm;
@@ -1465,10 +1506,11 @@ __device__ __forceinline__ void box_load(const StepArgs &a, const double *__rest
 // lane per box; conserv and the floor fused into the write-back when asked.  The set-up
 // (k_setup) runs as its own launch at two waves per SIMD: in one kernel with the sweeps it
 // ran at the sweeps' one wave per SIMD (the box state needs ~2 KB of registers) and took
-// three quarters of the step.  The 82 thermal rates are the exception: this kernel
-// computes them from T and p (the record leaves them out) -- its VALU is idle three
-// quarters of the time while its loads are in flight, and the rates' 82 record fields,
-// written and read back, cost more (the c1 pair at block 64: 0.924 ms against 1.004).
+// three quarters of the step.  The 82 thermal rates, the air density and the flags are
+// the exception: this kernel computes them from T, p and SZA (the step's record leaves
+// them out) -- its VALU is idle three quarters of the time while its loads are in flight,
+// and 85 record fields, written and read back, cost more (the c1 pair at block 64: 0.914 ms
+// against 1.004 with the full record).
 // Boxes box0 .. box0 + nr - 1 from records 0 .. nr - 1 (k_setup's range).
 template <int BLK>
 __global__ void __launch_bounds__(BLK) k_chem(StepArgs a, const double *__restrict__ rec,
@@ -1487,7 +1529,7 @@ __global__ void __launch_bounds__(BLK) k_chem(StepArgs a, const double *__restri
     if (r >= nr) return;
     const int64_t b = box0 + r;
     Box x;
-    box_load<BLK, false>(a, rec, nr, r, b, x, ck0, false);
+    box_load<BLK, REC_STEP>(a, rec, nr, r, b, x, ck0, false);
 #pragma unroll
     for (int i = 0; i < NJL; ++i) tjl[i * BLK] = x.tj[JL_SET[i]];
     Track notrack{true, true, 0., 0., 0.};
@@ -2131,7 +2173,7 @@ hipError_t launch_columns(const StepArgs &a, double *col, hipStream_t s)
     return hipGetLastError();
 }
 
-template <bool R>
+template <int R>
 static hipError_t launch_setup_t(const StepArgs &a, double *rec, int blk, hipStream_t s,
                                  int64_t box0, int64_t nr)
 {
@@ -2146,12 +2188,12 @@ static hipError_t launch_setup_t(const StepArgs &a, double *rec, int blk, hipStr
 }
 
 hipError_t launch_setup(const StepArgs &a, double *rec, int block, hipStream_t s, int64_t box0,
-                        int64_t nr, bool rates)
+                        int64_t nr, bool full)
 {
     const int blk = block > 0 ? block : 256;
     if (nr < 0) nr = a.nbox - box0;
-    return rates ? launch_setup_t<true>(a, rec, blk, s, box0, nr)
-                 : launch_setup_t<false>(a, rec, blk, s, box0, nr);
+    return full ? launch_setup_t<REC_FULL>(a, rec, blk, s, box0, nr)
+                 : launch_setup_t<REC_STEP>(a, rec, blk, s, box0, nr);
 }
 
 hipError_t launch_chem(const StepArgs &a, const double *rec, int block, hipStream_t s,

@@ -236,7 +236,8 @@ int step_launch(rb_ctx *ctx, const rb_grid *g, double dt, const double *T, const
     auto pair = [&]() -> hipError_t {
         for (int64_t b0 = 0; b0 < nbox; b0 += per) {
             const int64_t n = nbox - b0 < per ? nbox - b0 : per;
-            // the record without the thermal rates: k_chem computes them from T and p
+            // the step's record: k_chem computes the thermal rates, the air density and the
+            // flags itself from T, p and SZA
             hipError_t e = rb::launch_setup(a, rec, blk, ctx->stream, b0, n, false);
             if (e == hipSuccess) e = rb::launch_chem(a, rec, blk, ctx->stream, b0, n);
             if (e != hipSuccess) return e;
