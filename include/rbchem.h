@@ -305,7 +305,7 @@ int rb_chem_converge_dev(rb_ctx *ctx, const rb_grid *g, double dt, double tol, i
  *   - Newton: at most max_newton iterations, each step projected onto non-negative species,
  *     until a step moves every species by less than tol relative; then one plain sweep as
  *     the check (every species must change by less than tol; the swept state is the result).
- *   - A box Newton does not solve gets plain sweeps again, at most 100 more (and maxiter in
+ *   - A box Newton does not solve gets plain sweeps again, at most 40 more (and maxiter in
  *     all), then the fixed-sweep answer, as in rb_chem_converge_dev.
  * iters (optional): k (1 <= k < 10000) for a box the first sweeps converged in k sweeps,
  * 10000 + n for a box Newton solved in n iterations, 20000 + k for a box the last sweeps

@@ -695,6 +695,10 @@ __device__ __forceinline__ void track(Track &t, double v, double old)
     const double dv = fabs(v - old), sc = fmax(fabs(v), t.fl);
     t.conv &= dv < t.tol * sc;
     t.fin &= fabs(v) <= 1.7976931348623157e308;
+    // a reciprocal, not a division: off the chain of maxima that bounds a lone lane's sweep
+    // (the largest ratio by cross-multiplication, divided once per sweep, put two
+    // multiplications per species on that chain: the plain c2 solve, bound by its slowest
+    // lane, took 42-45 ms against 37)
     t.dmax = fmax(t.dmax, dv * __builtin_amdgcn_rcp(sc));
 }
 

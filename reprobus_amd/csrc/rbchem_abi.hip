@@ -149,9 +149,11 @@ bool any_null(std::initializer_list<const void *> ps)
 }
 
 // Newton solve: the most plain sweeps a box that Newton hands back still gets (on top of
-// the sweeps it had before Newton)
+// the sweeps it had before Newton).  On the c2 grid (100 sweeps first) the handed-back boxes
+// that converge at all do so within 22 such sweeps; the others (45 of 113) never do, and
+// with a cap of 100 their sweeps set the phase's time (~1 ms).
 #ifndef RB_NEWTON_RESUME_SWEEPS
-#define RB_NEWTON_RESUME_SWEEPS 100
+#define RB_NEWTON_RESUME_SWEEPS 40
 #endif
 constexpr int32_t NEWTON_RESUME_SWEEPS = RB_NEWTON_RESUME_SWEEPS;
 
