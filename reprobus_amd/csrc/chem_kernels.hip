@@ -1932,6 +1932,33 @@ __global__ void __launch_bounds__(BLK) k_chem_conv(StepArgs a, ConvArgs cv)
 // exhausted.
 constexpr int NEWTON_BLK = 64;
 
+// One Newton system of box x at its current iterate: the generated residual and Jacobian
+// into (A, B), the sparse LU of A in place and the solve, B = J^-1 F (day or night system).
+template <class AT, class BT>
+__device__ __forceinline__ void newton_system(const Box &x, const double *ck0, AT &A, BT &B)
+{
+    const Rates &R = x.R;
+    const BoxEnv &e = x.e;
+    const double *ck = x.ck, *cc = x.cc;
+#define TJ(X) x.tj[X]
+#define CK0(S) ck0[S]
+    if (e.day) {
+        RB_NEWTON_DAY_JAC(A, B);
+        RB_NEWTON_DAY_LU(A);
+        RB_NEWTON_DAY_SOLVE(A, B);
+    } else {
+        RB_NEWTON_NIGHT_JAC(A, B);
+        RB_NEWTON_NIGHT_LU(A);
+        RB_NEWTON_NIGHT_SOLVE(A, B);
+    }
+#undef TJ
+#undef CK0
+    (void)R; (void)e; (void)ck; (void)cc;
+}
+
+// A box's factors live in the lane's scratch, WAVES (1, 2, 4) resident waves per SIMD.  (In
+// LDS instead, 569 doubles per box, a CU holds 32 boxes -- half a wave with nothing to hide
+// its latency: the Newton-only c2 solve took 74.5 ms against 23.5 ms.)
 template <int WAVES, bool FROM_SPILL>
 __global__ void __launch_bounds__(NEWTON_BLK, WAVES) k_chem_newton(StepArgs a, ConvArgs cv,
                                                                    NewtonArgs na)
@@ -1975,22 +2002,9 @@ __global__ void __launch_bounds__(NEWTON_BLK, WAVES) k_chem_newton(StepArgs a, C
             bool ok = false;
             double sprev = 1.e300;
             {
-                const Rates &R = x.R;
-                const BoxEnv &e = x.e;
-                const double *ck = x.ck, *cc = x.cc;
-#define TJ(X) x.tj[X]
-#define CK0(S) ck0[S]
                 for (nit = 1; fin && nit <= cv.max_newton; ++nit) {
                     double A[NEWTON_NNZ], B[NEWTON_N];
-                    if (x.e.day) {
-                        RB_NEWTON_DAY_JAC(A, B);
-                        RB_NEWTON_DAY_LU(A);
-                        RB_NEWTON_DAY_SOLVE(A, B);
-                    } else {
-                        RB_NEWTON_NIGHT_JAC(A, B);
-                        RB_NEWTON_NIGHT_LU(A);
-                        RB_NEWTON_NIGHT_SOLVE(A, B);
-                    }
+                    newton_system(x, ck0, A, B);
                     // the full step, each species projected onto x >= 0: a species the step
                     // would take below zero (one heading for a tiny root: night NO2 below its
                     // clamp, HO2 or CH3OOH in weak sunlight) goes to 0 and the next step
@@ -2002,8 +2016,9 @@ __global__ void __launch_bounds__(NEWTON_BLK, WAVES) k_chem_newton(StepArgs a, C
 #pragma unroll
                     for (int i = 0; i < NEWTON_N; ++i) {
                         double &v = i < NEWTON_NLONG ? x.ck[NEWTON_SLOT[i]] : x.cc[NEWTON_SLOT[i]];
-                        fin &= fabs(B[i]) <= 1.7976931348623157e308;
-                        const double nv = fmax(v - B[i], 0.);
+                        const double bi = B[i];
+                        fin &= fabs(bi) <= 1.7976931348623157e308;
+                        const double nv = fmax(v - bi, 0.);
                         smax = fmax(smax, fabs(nv - v) * __builtin_amdgcn_rcp(fmax(nv, fl)));
                         v = nv;
                     }
@@ -2014,9 +2029,6 @@ __global__ void __launch_bounds__(NEWTON_BLK, WAVES) k_chem_newton(StepArgs a, C
                         break;
                     }
                 }
-#undef TJ
-#undef CK0
-                (void)R; (void)e; (void)ck; (void)cc;
             }
             if (ok) {
                 // the check: one plain sweep from the Newton point

@@ -174,3 +174,24 @@ def test_newton_rejects_bad_arguments(step):
                {"sweeps_first": 2000}):
         with pytest.raises(RbError):
             solve_gpu(step, inp, "newton", **kw)
+
+
+def test_launch_shapes_agree(table, c2):
+    """The Newton kernel at 1, 2 and 4 resident waves per SIMD runs the same operations in
+    the same order: the same bits and counts, from the incoming state and after sweeps
+    first."""
+    from reprobus_amd.chem import ChemistryStep
+
+    inp, _ = c2
+    outs = {}
+    for waves in (1, 2, 4):
+        s = ChemistryStep(*table, device=0, newton_waves=waves)
+        try:
+            for sf in (0, 30):
+                outs[waves, sf] = solve_gpu(s, inp, "newton", sweeps_first=sf)
+        finally:
+            s.close()
+    for sf in (0, 30):
+        for waves in (2, 4):
+            for k in ("qj1", "hc", "iters"):
+                assert np.array_equal(outs[1, sf][k], outs[waves, sf][k]), (waves, sf, k)
