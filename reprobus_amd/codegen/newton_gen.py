@@ -616,6 +616,42 @@ def lu_ops(filled: set, order: list[int], slot: dict):
     return lu, sv
 
 
+def lu_rows_ops(filled: set, order: list[int], slot: dict):
+    """The same factorisation row by row (up-looking), with the forward substitution fused
+    in, and the back substitution, as op lists.  Row i takes, for each multiplier column
+    k < i in ascending order, the multiplier a_ik *= 1 / u_kk (read back from the inverted
+    pivot slot), the updates a_ij -= a_ik u_kj of its own entries and b_i -= a_ik b_k; then
+    its pivot is inverted in place:
+      ('scalep', s, sk)    a[s] *= a[sk]           (sk holds 1 / u_kk)
+      ('inv', s)           a[s] = 1 / a[s]
+    Every entry and every b_i receives the same operations in the same order as in
+    lu_ops (the updates of an entry by ascending pivot), so the factors and the solution
+    are the same bits; the working set is one row, where the right-looking order touches
+    every trailing row at each pivot."""
+    pos = {u: k for k, u in enumerate(order)}
+    n = len(order)
+    rows = [[] for _ in range(n)]
+    for (i, j) in filled:
+        rows[pos[i]].append(pos[j])
+    for r in rows:
+        r.sort()
+    S = lambda pi, pj: slot[(order[pi], order[pj])]
+    lu, sv = [], []
+    for i in range(n):
+        for k in (k for k in rows[i] if k < i):
+            lu.append(("scalep", S(i, k), S(k, k)))
+            for j in (j for j in rows[k] if j > k):
+                lu.append(("upd", S(i, j), S(i, k), S(k, j)))
+            lu.append(("fsub", order[i], S(i, k), order[k]))
+        lu.append(("inv", S(i, i)))
+    for i in reversed(range(n)):
+        for j in rows[i]:
+            if j > i:
+                sv.append(("bsub", order[i], S(i, j), order[j]))
+        sv.append(("bmul", order[i], S(i, i)))
+    return lu, sv
+
+
 def render(ops) -> list[str]:
     out = []
     for op in ops:
@@ -624,6 +660,10 @@ def render(ops) -> list[str]:
             out.append(f"const double d{op[1]} = 1. / a[{op[2]}];")
         elif t == "scale":
             out.append(f"a[{op[1]}] *= d{op[2]};")
+        elif t == "scalep":
+            out.append(f"a[{op[1]}] *= a[{op[2]}];")
+        elif t == "inv":
+            out.append(f"a[{op[1]}] = 1. / a[{op[1]}];")
         elif t == "upd":
             out.append(f"a[{op[1]}] -= a[{op[2]}] * a[{op[3]}];")
         elif t == "setinv":
@@ -645,6 +685,10 @@ def interpret(ops, a, b) -> None:
             d[op[1]] = 1.0 / a[op[2]]
         elif t == "scale":
             a[op[1]] *= d[op[2]]
+        elif t == "scalep":
+            a[op[1]] *= a[op[2]]
+        elif t == "inv":
+            a[op[1]] = 1.0 / a[op[1]]
         elif t == "upd":
             a[op[1]] -= a[op[2]] * a[op[3]]
         elif t == "setinv":
@@ -663,7 +707,7 @@ def system(day: bool) -> dict:
     cands = best_order(pat)
     flops, nnz, oname, order, filled = cands[0]
     slot = _slot_map(filled, order)
-    lu, sv = lu_ops(filled, order, slot)
+    lu, sv = lu_rows_ops(filled, order, slot)
     return {"pattern": pat, "order": order, "filled": filled, "slot": slot, "lu": lu,
             "solve": sv, "flops_lu": flops, "order_name": oname,
             "cands": [(c[0], c[1], c[2]) for c in cands]}
@@ -740,7 +784,8 @@ def emit_system(day: bool):
     return {"jac": jac, "lu": "\n".join("    " + l for l in render(sy["lu"])),
             "solve": "\n".join("    " + l for l in render(sy["solve"])),
             "nnz_j": len(sy["pattern"]), "nnz_lu": len(sy["filled"]),
-            "flops_lu": sy["flops_lu"], "flops_solve": len(sy["solve"]),
+            "flops_lu": sy["flops_lu"],
+            "flops_solve": len(sy["solve"]) + sum(1 for op in sy["lu"] if op[0] == "fsub"),
             "order_name": sy["order_name"], "cands": sy["cands"],
             "jac_ops": len(lines) + len(body), "jac_live": max_live(stmts)}
 
@@ -774,11 +819,11 @@ FUNC = """// {what}: b <- residual, a <- Jacobian (factor slots, fill-in zero)
 #define RB_NEWTON_{tag}_JAC(a, b) do {{ \\
 {jac}}} while (0)
 
-// {what}: a <- LU factors (inverted diagonal), in place
-#define RB_NEWTON_{tag}_LU(a) do {{ \\
+// {what}: a <- LU factors (inverted diagonal), in place, row by row; b <- L^-1 b
+#define RB_NEWTON_{tag}_LU(a, b) do {{ \\
 {lu}}} while (0)
 
-// {what}: b <- A^-1 b from the factors
+// {what}: b <- U^-1 b from the factors
 #define RB_NEWTON_{tag}_SOLVE(a, b) do {{ \\
 {solve}}} while (0)
 

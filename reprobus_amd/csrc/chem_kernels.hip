@@ -1944,11 +1944,11 @@ __device__ __forceinline__ void newton_system(const Box &x, const double *ck0, A
 #define CK0(S) ck0[S]
     if (e.day) {
         RB_NEWTON_DAY_JAC(A, B);
-        RB_NEWTON_DAY_LU(A);
+        RB_NEWTON_DAY_LU(A, B);
         RB_NEWTON_DAY_SOLVE(A, B);
     } else {
         RB_NEWTON_NIGHT_JAC(A, B);
-        RB_NEWTON_NIGHT_LU(A);
+        RB_NEWTON_NIGHT_LU(A, B);
         RB_NEWTON_NIGHT_SOLVE(A, B);
     }
 #undef TJ
