@@ -23,8 +23,14 @@ NLON_REF = 180
 DAYNUM = 80.5      # fixed day number: declination near zero, ut = 12 h
 
 
-def available() -> bool:
-    return os.path.exists(REF_BIN)
+def available(niter: int = 5) -> bool:
+    return os.path.exists(ref_bin(niter))
+
+
+def ref_bin(niter: int = 5) -> str:
+    """The harness binary of the reference built with its fixed sweep count niter (F:630):
+    the shipped 5, or a raised count the Makefile builds (NITERS)."""
+    return REF_BIN if niter == 5 else f"{REF_BIN}_n{niter}"
 
 
 def _zenith_terms(day):
@@ -56,14 +62,15 @@ def trajectories_for_sza(sza_deg: np.ndarray, day: float = DAYNUM):
 
 
 def run_reference(inp, nrep: int = 1, day: float = DAYNUM, workdir: str | None = None,
-                  keep: bool = False, dtsedi: float = 0.0) -> dict:
+                  keep: bool = False, dtsedi: float = 0.0, niter: int = 5) -> dict:
     """Run the reference on ``inp`` (a synth.StepInputs with nlat*nlon <= 16380 columns).
     dtsedi > 0 also runs the reference's sedimentation (F:419-423; needs nivbas = 94) and
-    returns qj1 after it as 'qj1_sedi'."""
+    returns qj1 after it as 'qj1_sedi'.  niter selects the build with that many sweeps."""
     from reprobus_amd import jtable
 
-    if not available():
-        raise RuntimeError(f"reference harness not built: {REF_BIN} (run `make -C oracle ref`)")
+    exe = ref_bin(niter)
+    if not os.path.exists(exe):
+        raise RuntimeError(f"reference harness not built: {exe} (run `make -C oracle ref`)")
     g = inp.grid
     ncol, nlev = g.nlat * g.nlon, g.nivbas
     lon, lat = trajectories_for_sza(inp.sza[:nlev])
@@ -83,7 +90,7 @@ def run_reference(inp, nrep: int = 1, day: float = DAYNUM, workdir: str | None =
                 np.ascontiguousarray(a, dtype=np.float64).tofile(f)
             np.ascontiguousarray(inp.qj1[:, :nlev], dtype=np.float64).tofile(f)
             np.ascontiguousarray(inp.hc[:, :nlev], dtype=np.float64).tofile(f)
-        r = subprocess.run([REF_BIN, fin, fout], cwd=run, capture_output=True, text=True)
+        r = subprocess.run([exe, fin, fout], cwd=run, capture_output=True, text=True)
         if r.returncode != 0:
             raise RuntimeError(f"reference harness failed ({r.returncode}): {r.stdout}\n{r.stderr}")
         with open(fout, "rb") as f:

@@ -4,7 +4,8 @@ on small synthetic grids.
 
 Run from the repository root where /root/reference is present:
     make -C oracle ref && python tests/golden/make_golden.py
-(``python tests/golden/make_golden.py wet`` or ``transport`` writes that fixture only.)
+(``python tests/golden/make_golden.py wet``, ``transport`` or ``niter`` writes that fixture
+only; ``niter`` needs the raised-sweep builds, ``make -C oracle ref``.)
 
 Each chemistry fixture stores the step inputs (with the SZA the reference realised,
 ``sza3d``) and the reference outputs after chem3d and after conserv + floor; the
@@ -162,6 +163,41 @@ def main():
     print("wrote inputs_seed11", sza.size, "SZA values,", alt.size, "altitudes")
     ecmwf_fixture()
     transport_fixture()
+    niter_fixtures()
+
+
+def columns_of(inp, cols):
+    """The listed columns (flat index lat * nlon + lon) of any grid, as a 1 x ncol grid."""
+    g = inp.grid
+    ncol = g.nlat * g.nlon
+    out = inp.copy()
+    out.grid = synth.Grid(len(cols), 1, g.niv, g.nivbas, g.nivhau)
+    flat = lambda a: np.ascontiguousarray(a.reshape(a.shape[:-2] + (ncol,))[..., cols][..., None, :])
+    for name in ("T", "p", "sza", "alt", "sad_target"):
+        a = getattr(inp, name)
+        if a is not None:
+            setattr(out, name, flat(a))
+    out.qj1, out.hc = flat(inp.qj1), flat(inp.hc)
+    return out
+
+
+def niter_fixtures():
+    """The reference built with niter (F:630) raised: 60 sweeps on the c0 columns, and 400
+    sweeps on 16 columns of the c2 grid (8 in the polar vortex, liquid PSC and cold
+    heterogeneous chemistry, 8 across the other latitudes, day and night) -- there the
+    sweeps have reached the fixed point the converged mode solves for, except at the few
+    boxes that converge slower or not at all."""
+    c0 = synth.make_c0(0)
+    sub = subset_columns(c0, list(range(16)))
+    save("c0_seed0_16col_n60", sub, refrun.run_reference(sub, niter=60))
+    c2 = synth.make_c2(1, nlon=24, nlat=13)
+    nlon = c2.grid.nlon
+    # rows 0-1 are poleward of 60 S (the vortex); 8 columns there, 8 spread over the rest
+    cols = [0, 5, 11, 17, nlon + 2, nlon + 8, nlon + 14, nlon + 20]
+    cols += [r * nlon + c for r, c in ((3, 1), (4, 7), (6, 13), (7, 19), (9, 4), (10, 10),
+                                        (11, 16), (12, 22))]
+    vx = columns_of(c2, cols)
+    save("c2_seed1_16col_n400", vx, refrun.run_reference(vx, niter=400))
 
 
 def transport_fixture(seed: int = 0, nsamp: int = 4000):
@@ -216,5 +252,7 @@ if __name__ == "__main__":
         transport_fixture()
     elif sys.argv[1:] == ["wet"]:
         wet_fixture()
+    elif sys.argv[1:] == ["niter"]:
+        niter_fixtures()
     else:
         main()

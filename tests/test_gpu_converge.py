@@ -242,3 +242,56 @@ def test_work_order_changes_nothing(table):
         a, b = outs[0, how], outs[1, how]
         for k in ("qj1", "hc", "iters"):
             assert np.array_equal(a[k], b[k]), (how, k)
+
+
+def test_gpu_step_matches_reference_with_niter_raised(table):
+    """The fixed-sweep step at rb_config.niter = 60 against the reference built with niter
+    (F:630) = 60, on the 16 c0 columns (960 boxes), at the 1e-9 bar."""
+    from conftest import load_fixture
+    from reprobus_amd.chem import ChemistryStep
+
+    inp, ref = load_fixture("c0_seed0_16col_n60")
+    s = ChemistryStep(*table, device=0, niter=60)
+    try:
+        out = s.run_inputs(inp)
+    finally:
+        s.close()
+    w = max(float(max_rel_by_species(out[k], ref[k]).max())
+            for k in ("qj1_chem", "hc_chem", "qj1", "hc"))
+    print(f"[niter] GPU 60 sweeps vs reference niter=60: max {w:.2e}", flush=True)
+    assert w < PARITY_TOL
+
+
+@pytest.mark.parametrize("how", ["plain", "newton"])
+def test_gpu_converged_matches_reference_swept_to_its_fixed_point(table, how):
+    """The converged solve (plain sweeps; Newton after 30 sweeps) against the reference
+    built with niter = 400 on 16 c2 columns (8 in the polar vortex), on the boxes where the
+    400 sweeps have reached the fixed point -- those the restatement's converged mode
+    (tol 1e-14) converges within 300 sweeps -- and the GPU solve converged: the same fixed
+    point at the 1e-9 bar.  (A box the sweeps converge more slowly than that is not at its
+    fixed point after 400 of them.)"""
+    import cport
+    from conftest import load_fixture
+    from reprobus_amd.chem import ChemistryStep
+    from test_gpu_newton import solve_gpu
+
+    inp, ref = load_fixture("c2_seed1_16col_n400")
+    rit = cport.chem_converge(inp, *table, tol=1e-14, maxiter=50000, conserv=True)["iters"]
+    s = ChemistryStep(*table, device=0)
+    try:
+        if how == "plain":
+            out = converge_gpu(s, inp, tol=1e-12, maxiter=2000)
+            it = out["iters"]
+            ok = (it > 0) & (rit > 0) & (rit < 300)
+        else:
+            out = solve_gpu(s, inp, "newton", tol=1e-12, sweeps_first=30, fuse=True)
+            it = out["iters"]
+            ok = (it > 0) & (it < 20000) & (rit > 0) & (rit < 300)
+    finally:
+        s.close()
+    d = np.maximum(rel_err(out["qj1"][:42], ref["qj1"][:42]).max(axis=0),
+                   rel_err(out["hc"], ref["hc"]).max(axis=0))
+    print(f"[niter] {how}: {int(ok.sum())} of {ok.size} boxes converged, vs reference "
+          f"niter=400 max {d[ok].max():.2e}", flush=True)
+    assert ok.sum() >= 940
+    assert d[ok].max() < PARITY_TOL
