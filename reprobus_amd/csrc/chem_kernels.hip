@@ -1934,14 +1934,17 @@ constexpr int NEWTON_BLK = 64;
 
 // One Newton system of box x at its current iterate: the generated residual and Jacobian
 // into (A, B), the sparse LU of A in place and the solve, B = J^-1 F (day or night system).
-template <class AT, class BT>
-__device__ __forceinline__ void newton_system(const Box &x, const double *ck0, AT &A, BT &B)
+// ck0 at stride CK0_STRIDE; with JLDS the J-values of JL_SET in tjl (stride CK0_STRIDE), as
+// the sweep reads them
+template <int CK0_STRIDE, bool JLDS, class AT, class BT>
+__device__ __forceinline__ void newton_system(const Box &x, const double *ck0, const double *tjl,
+                                              AT &A, BT &B)
 {
     const Rates &R = x.R;
     const BoxEnv &e = x.e;
     const double *ck = x.ck, *cc = x.cc;
-#define TJ(X) x.tj[X]
-#define CK0(S) ck0[S]
+#define TJ(X) ((JLDS && jl_slot(X) >= 0) ? tjl[jl_slot(X) * CK0_STRIDE] : x.tj[X])
+#define CK0(S) ck0[(S) * CK0_STRIDE]
     if (e.day) {
         RB_NEWTON_DAY_JAC(A, B);
         RB_NEWTON_DAY_LU(A, B);
@@ -1953,7 +1956,7 @@ __device__ __forceinline__ void newton_system(const Box &x, const double *ck0, A
     }
 #undef TJ
 #undef CK0
-    (void)R; (void)e; (void)ck; (void)cc;
+    (void)R; (void)e; (void)ck; (void)cc; (void)tjl;
 }
 
 // A box's factors live in the lane's scratch, WAVES (1, 2, 4) resident waves per SIMD.  (In
@@ -1964,6 +1967,12 @@ __global__ void __launch_bounds__(NEWTON_BLK, WAVES) k_chem_newton(StepArgs a, C
                                                                    NewtonArgs na)
 {
     const int lane = threadIdx.x & (warpSize - 1);
+    // at one wave per SIMD the start-of-step state and NJL J-values live in LDS, as in the
+    // sweep kernels (80 doubles per lane, the CU's 160 KB at four waves): registers for the
+    // Jacobian's temporaries.  At more waves per SIMD the LDS would cap the occupancy.
+    constexpr bool SL = WAVES == 1;
+    constexpr int ST = SL ? NEWTON_BLK : 1;
+    __shared__ double s_st[SL ? (NLONG - 1 + NJL) * NEWTON_BLK : 1];
     const int64_t cap = a.nbox;
     const int64_t *in_box = FROM_SPILL ? (const int64_t *)(na.in + NX * cap) : nullptr;
     const int32_t *in_it = FROM_SPILL ? (const int32_t *)(na.in + (NX + 1) * cap) : nullptr;
@@ -1981,8 +1990,14 @@ __global__ void __launch_bounds__(NEWTON_BLK, WAVES) k_chem_newton(StepArgs a, C
         if (w < nwork) {
             const int64_t b = FROM_SPILL ? in_box[w] : (cv.order ? (int64_t)cv.order[w] : w);
             Box x;
-            double ck0[NLONG - 1];
-            box_load<1>(a, cv.rec, a.nbox, b, b, x, ck0, !FROM_SPILL);
+            double ck0r[SL ? 1 : NLONG - 1];
+            double *ck0 = SL ? s_st + threadIdx.x : ck0r;
+            double *tjl = SL ? s_st + (NLONG - 1) * NEWTON_BLK + threadIdx.x : nullptr;
+            box_load<ST>(a, cv.rec, a.nbox, b, b, x, ck0, !FROM_SPILL);
+            if (SL) {
+#pragma unroll
+                for (int i = 0; i < NJL; ++i) tjl[i * NEWTON_BLK] = x.tj[JL_SET[i]];
+            }
             const double fl = 1.e-30 * x.e.hnm;
             int sweeps = 0;
             bool fin = true;
@@ -1994,7 +2009,7 @@ __global__ void __launch_bounds__(NEWTON_BLK, WAVES) k_chem_newton(StepArgs a, C
                 sweeps = in_it[w];
             } else {
                 Track tr{true, true, cv.tol, fl, 0.};
-                sweep<1, true, false>(x.R, x.tj, x.e, ck0, x.ck, x.cc, x.ch3o, x.hco, tr);
+                sweep<ST, true, SL>(x.R, x.tj, x.e, ck0, x.ck, x.cc, x.ch3o, x.hco, tr, tjl);
                 sweeps = 1;
                 fin = tr.fin;
             }
@@ -2004,7 +2019,7 @@ __global__ void __launch_bounds__(NEWTON_BLK, WAVES) k_chem_newton(StepArgs a, C
             {
                 for (nit = 1; fin && nit <= cv.max_newton; ++nit) {
                     double A[NEWTON_NNZ], B[NEWTON_N];
-                    newton_system(x, ck0, A, B);
+                    newton_system<ST, SL>(x, ck0, tjl, A, B);
                     // the full step, each species projected onto x >= 0: a species the step
                     // would take below zero (one heading for a tiny root: night NO2 below its
                     // clamp, HO2 or CH3OOH in weak sunlight) goes to 0 and the next step
@@ -2033,7 +2048,7 @@ __global__ void __launch_bounds__(NEWTON_BLK, WAVES) k_chem_newton(StepArgs a, C
             if (ok) {
                 // the check: one plain sweep from the Newton point
                 Track tc{true, true, cv.tol, fl, 0.};
-                sweep<1, true, false>(x.R, x.tj, x.e, ck0, x.ck, x.cc, x.ch3o, x.hco, tc);
+                sweep<ST, true, SL>(x.R, x.tj, x.e, ck0, x.ck, x.cc, x.ch3o, x.hco, tc, tjl);
                 ++sweeps;
                 ok = tc.conv && tc.fin;
                 fin = tc.fin;
