@@ -29,7 +29,7 @@ BLOCKS = (64, 128, 256)
 REFILLS = (8, 16, 32)
 # Newton solve: plain sweeps before Newton takes the still unconverged boxes, and resident
 # waves per SIMD of the Newton kernel (the occupancy it runs at)
-SWEEPS_FIRST = (40, 60, 80, 100, 130, 160)
+SWEEPS_FIRST = (20, 40, 60, 80, 100, 130)
 NEWTON_WAVES = (1, 2)
 
 
