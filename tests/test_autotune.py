@@ -52,7 +52,7 @@ def test_newton_choice_and_cache(tmp_path):
 
     def timer(sf, w):
         calls.append((sf, w))
-        return {40: 20.0, 60: 16.0, 80: 16.0, 100: 17.0, 130: 18.0, 160: 19.0}[sf] + \
+        return {20: 24.0, 40: 20.0, 60: 16.0, 80: 16.0, 100: 17.0, 130: 18.0, 160: 19.0}[sf] + \
             (0.5 if w == 2 else 0.0)
 
     r = autotune.tune_newton(None, grid, {}, cache_path=path, timer=timer)
