@@ -295,3 +295,42 @@ def test_gpu_converged_matches_reference_swept_to_its_fixed_point(table, how):
           f"niter=400 max {d[ok].max():.2e}", flush=True)
     assert ok.sum() >= 940
     assert d[ok].max() < PARITY_TOL
+
+
+def test_block_sizes_give_the_same_bits(table):
+    """A box's converged answer and sweep count do not depend on the launch shape: plain,
+    accelerated and Newton solves at blocks 64, 128 and 256 (the shapes the autotuner
+    chooses from) give the same bits on a c2 subgrid."""
+    import torch
+    from reprobus_amd import synth
+    from reprobus_amd.chem import ChemistryStep, make_grid
+
+    inp = synth.make_c2(1, nlon=24, nlat=13)
+    g = inp.grid
+    grid = make_grid(g.nlon, g.nlat, g.niv, g.nivbas, g.nivhau)
+    outs = {}
+    for block in (64, 128, 256):
+        print(f"[blocks] block {block}", flush=True)
+        s = ChemistryStep(*table, device=0, block=block, conv_refill=32, newton_waves=1,
+                          conv_aa_start=AA_START)
+        try:
+            for how in ("plain", "accel", "newton"):
+                d = _dev(inp)
+                args = [d[k] for k in ("T", "p", "sza", "alt", "qj1", "hc", "surfarea", "vsed3d",
+                                       "parth2osed", "parthno3sed")]
+                s.set_stream(torch.cuda.current_stream().cuda_stream)
+                if how == "newton":
+                    s.newton_device(grid, inp.dt, 1e-11, 20, 2000, *args, iters=d["iters"],
+                                    sweeps_first=30)
+                else:
+                    s.converge_device(grid, inp.dt, TOL, 2000, *args, iters=d["iters"],
+                                      accel=2 if how == "accel" else 0)
+                torch.cuda.synchronize()
+                s.set_stream(None)
+                outs[block, how] = {k: d[k].cpu().numpy() for k in ("qj1", "hc", "iters")}
+        finally:
+            s.close()
+    for how in ("plain", "accel", "newton"):
+        for block in (128, 256):
+            for k in ("qj1", "hc", "iters"):
+                assert np.array_equal(outs[64, how][k], outs[block, how][k]), (how, block, k)
