@@ -684,22 +684,32 @@ struct BoxEnv {
 // the distance-to-fixed-point estimate; it only steers when to stop, so the hardware
 // reciprocal (not the IEEE division) forms it.
 struct Track {
-    bool conv, fin;
+    bool conv, fin;            // set at the sweep's end (track_end)
     double tol, fl;
-    double dmax;
+    double dmax;               // the sweep's largest relative change
+    double sum = 0.;           // of the new values: finite iff every one is
 };
 
+// One species' change: the largest |v - old| / max(|v|, fl) of the sweep, and the sum of
+// the new values.  Every species changed by less than tol exactly when the largest change
+// is below tol, and every value is finite exactly when their sum is (the values, number
+// densities below 1e25, cannot overflow it; an infinity or NaN carries through): two
+// operations per species instead of the two compare-and-mask tests of each.
 __device__ __forceinline__ void track(Track &t, double v, double old)
 {
-    // non-short-circuit: a lane mask AND per test, no branch
     const double dv = fabs(v - old), sc = fmax(fabs(v), t.fl);
-    t.conv &= dv < t.tol * sc;
-    t.fin &= fabs(v) <= 1.7976931348623157e308;
     // a reciprocal, not a division: off the chain of maxima that bounds a lone lane's sweep
     // (the largest ratio by cross-multiplication, divided once per sweep, put two
     // multiplications per species on that chain: the plain c2 solve, bound by its slowest
     // lane, took 42-45 ms against 37)
     t.dmax = fmax(t.dmax, dv * __builtin_amdgcn_rcp(sc));
+    t.sum += v;
+}
+
+__device__ __forceinline__ void track_end(Track &t)
+{
+    t.fin = fabs(t.sum) <= 1.7976931348623157e308;
+    t.conv = t.dmax < t.tol;
 }
 
 // Successive plain sweeps the converged mode needs before it may stop: its estimate of
@@ -1014,6 +1024,7 @@ __device__ void sweep(const Rates &R, const double *tj, const BoxEnv &e,
         if (TRACK) track(tr, nk[s], ck[s]);
         ck[s] = nk[s];
     }
+    if (TRACK) track_end(tr);
 #undef CCSET
 #undef TJ
 }
