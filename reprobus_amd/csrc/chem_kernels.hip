@@ -677,24 +677,24 @@ struct BoxEnv {
 
 // The converged mode's fixed-point test, without divisions: a sweep has converged when
 // every long- and short-lived species changed by less than tol relative to its new value
-// (values under the floor fl compared at fl): |v - old| < tol max(|v|, fl); `fin` stays true
-// while every new value is finite.  A NaN fails both.  Each species is set once per sweep,
-// so `old` is its value at the start of the sweep.
-// dmax accumulates the largest relative change of the sweep, |v - old| / max(|v|, fl), for
-// the distance-to-fixed-point estimate; it only steers when to stop, so the hardware
-// reciprocal (not the IEEE division) forms it.
+// (values under the floor fl compared at fl): |v - old| / max(|v|, fl) < tol; `fin` stays
+// true while every change is finite.  A NaN fails both.  Each species is set once per
+// sweep, so `old` is its value at the start of the sweep.
+// dmax is the largest relative change of the sweep, also the input of the distance-to-
+// fixed-point estimate; it only steers when to stop, so the hardware reciprocal (not the
+// IEEE division) forms it.
 struct Track {
     bool conv, fin;            // set at the sweep's end (track_end)
     double tol, fl;
-    double dmax;               // the sweep's largest relative change
-    double sum = 0.;           // of the new values: finite iff every one is
+    double dmax;               // the sweep's largest relative change (set by track_end)
+    uint32_t hmax = 0;         // the largest high word of the relative changes
 };
 
-// One species' change: the largest |v - old| / max(|v|, fl) of the sweep, and the sum of
-// the new values.  Every species changed by less than tol exactly when the largest change
-// is below tol, and every value is finite exactly when their sum is (the values, number
-// densities below 1e25, cannot overflow it; an infinity or NaN carries through): two
-// operations per species instead of the two compare-and-mask tests of each.
+// One species' change r = |v - old| / max(|v|, fl).  For r >= 0 the high 32 bits of the
+// double, read as an unsigned integer, order like r to 2^-20 relative, and every NaN or
+// infinity reads above every finite r: one integer maximum per species keeps both the
+// largest change and whether every change was finite.  A new value that is infinite or NaN
+// makes r NaN (|v - old| / max(|v|, fl) is NaN or inf * 0), and so does a NaN start.
 __device__ __forceinline__ void track(Track &t, double v, double old)
 {
     const double dv = fabs(v - old), sc = fmax(fabs(v), t.fl);
@@ -702,13 +702,17 @@ __device__ __forceinline__ void track(Track &t, double v, double old)
     // (the largest ratio by cross-multiplication, divided once per sweep, put two
     // multiplications per species on that chain: the plain c2 solve, bound by its slowest
     // lane, took 42-45 ms against 37)
-    t.dmax = fmax(t.dmax, dv * __builtin_amdgcn_rcp(sc));
-    t.sum += v;
+    const uint32_t h = (uint32_t)__double2hiint(dv * __builtin_amdgcn_rcp(sc));
+    t.hmax = t.hmax > h ? t.hmax : h;
 }
 
+// The sweep's largest change, its high word with the low word zero (below the true maximum
+// by less than 2^-20 relative: the test against tol and the distance estimate do not depend
+// on such a difference), and the finiteness of every change.
 __device__ __forceinline__ void track_end(Track &t)
 {
-    t.fin = fabs(t.sum) <= 1.7976931348623157e308;
+    t.fin = t.hmax < 0x7ff00000u;
+    t.dmax = __hiloint2double((int)t.hmax, 0);
     t.conv = t.dmax < t.tol;
 }
 
