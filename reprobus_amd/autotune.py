@@ -85,12 +85,16 @@ def choose(timings: dict):
 
 def tune(table, grid, state: dict, mode: str = "plain", candidates=BLOCKS, reps: int = 3,
          cache_path: str | None = None, device: int = 0, tol: float = 1e-11,
-         maxiter: int = 2000, accel: int = 2, refills=REFILLS, timer=None) -> TuneResult:
+         maxiter: int = 2000, accel: int = 2, refills=REFILLS, timer=None,
+         steps: int = 5) -> TuneResult:
     """Pick the launch shape for `mode` ("plain": the fixed-sweep step's block;
     "converged": block and queue chunk of the converged mode with Anderson depth `accel`)
     on `state` (device tensors T, p, sza, alt, qj1, hc, surfarea, vsed3d, parth2osed,
-    parthno3sed, dt).  `timer(block)` (plain) or `timer(block, refill)` (converged) -> ms
-    replaces the GPU timing (tests)."""
+    parthno3sed, dt).  A plain trial times `steps` successive steps (ms per step), as a
+    model run issues them: a lone step after the state's restore starts from cold caches
+    and an idle queue, and its time is noisier than the differences between blocks.
+    `timer(block)` (plain) or `timer(block, refill)` (converged) -> ms replaces the GPU
+    timing (tests)."""
     import torch
 
     from .chem import ChemistryStep
@@ -115,8 +119,9 @@ def tune(table, grid, state: dict, mode: str = "plain", candidates=BLOCKS, reps:
 
             def once():
                 if not conv:
-                    step.step_device(grid, state["dt"], *args, state["qj1"], state["hc"], *outs,
-                                     fuse_conserv=True)
+                    for _ in range(steps):
+                        step.step_device(grid, state["dt"], *args, state["qj1"], state["hc"],
+                                         *outs, fuse_conserv=True)
                 else:
                     step.converge_device(grid, state["dt"], tol, maxiter, *args, state["qj1"],
                                          state["hc"], *outs, accel=accel)
@@ -131,7 +136,7 @@ def tune(table, grid, state: dict, mode: str = "plain", candidates=BLOCKS, reps:
                 e1.record(s)
                 torch.cuda.synchronize()
                 if r:
-                    times.append(e0.elapsed_time(e1))
+                    times.append(e0.elapsed_time(e1) / (1 if conv else steps))
             state["qj1"].copy_(q0)
             state["hc"].copy_(h0)
             torch.cuda.synchronize()
