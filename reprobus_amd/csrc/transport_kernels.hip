@@ -5,10 +5,11 @@
 //
 // One lane per grid point.  The weights kernel reproduces the reference's arithmetic
 // operation for operation (FP64, built with -ffp-contract=off).  The interpolation kernel
-// reads a point's 3 indices and 12 weights once per group of tracers and loops over the
-// group: per tracer the 4 x 4 x 4 stencil (16 rows of four longitudes, neighbouring lanes
-// share most of it through the caches) and one coalesced store.  The reference's extended grid x(0:nlon+2, 0:nlat+1, 0:niv+1)
-// (F:8374-8413) is not built: each stencil index is mapped to the field instead.
+// takes a tile of points per workgroup and, per tracer, copies the part of the field that
+// the tile's 4 x 4 x 4 stencils cover into LDS, each value once; every lane reads its
+// stencil from there and makes one coalesced store.  The reference's extended grid
+// x(0:nlon+2, 0:nlat+1, 0:niv+1) (F:8374-8413) is not built: each stencil index is mapped
+// to the field instead.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -245,18 +246,21 @@ __global__ void __launch_bounds__(TX * TY, 3) k_transport_interp_tiled(
             atomicMax(&s_bnd[5], lg + 2);
         }
         __syncthreads();
+        // no point contributed: the bounds are still their initial values (and their
+        // difference would overflow)
+        const bool any = s_bnd[1] >= s_bnd[0];
         x0 = s_bnd[0];
         y0 = s_bnd[2];
         z0 = s_bnd[4];
-        wxn = s_bnd[1] - x0 + 1;
-        wyn = s_bnd[3] - y0 + 1;
-        wzn = s_bnd[5] - z0 + 1;
+        wxn = any ? s_bnd[1] - x0 + 1 : 0;
+        wyn = any ? s_bnd[3] - y0 + 1 : 0;
+        wzn = any ? s_bnd[5] - z0 + 1 : 0;
     };
     int x0, y0, z0, wxn, wyn, wzn;
     window(good, x0, y0, z0, wxn, wyn, wzn);
     if (wxn <= 0) return;                       // no point of the tile inside the grid
     bool in = good;
-    if (wxn * wyn * wzn > WMAX) {
+    if ((int64_t)wxn * wyn * wzn > WMAX) {
         in = good && abs(u - i) <= RX && abs(kg - k) <= RY && abs(lg - l) <= RZ;
         window(in, x0, y0, z0, wxn, wyn, wzn);
     }

@@ -122,3 +122,47 @@ def test_departure_points_outside_the_grid_are_rejected(step):
         step.transport_interp_device(grid, _dev(idx_c), _dev(w_c), q_in, q_out)
     with pytest.raises(RbError):
         step.transport_interp_device(grid, _dev(idx_c), _dev(w_c), q_in, q_in)   # overlap
+
+
+def test_tiles_without_near_points(step):
+    """The interpolation works on tiles of points: a tile whose every departure cell is far
+    from its arrival point gathers each stencil from the fields, and a tile with no point
+    inside the grid writes nothing and only counts its points; the other tiles are
+    unaffected (bit for bit against the restatement)."""
+    import torch
+    from reprobus_amd._lib import RbError
+    from reprobus_amd.chem import make_grid
+
+    nlon, nlat, niv = 16, 9, 7                     # one tile per level
+    tr = synth.make_transport(5, nlon=nlon, nlat=nlat, niv=niv, jc=(1, 2, 3))
+    grid = make_grid(nlon, nlat, niv, niv, 1)
+    idx_c, w_c = cport.transport_weights(tr)
+    idx_c = idx_c.reshape(3, niv, nlat, nlon).copy()
+    # level 2: every departure cell 3 to 8 longitudes from its arrival point, east or west,
+    # at any latitude and level: the stencils span more than a tile's window can hold, and
+    # no point is near enough for the window of the near points
+    rng = np.random.default_rng(5)
+    i = np.arange(nlon)[None, :]
+    shift = rng.integers(3, 9, (nlat, nlon)) * rng.choice([-1, 1], (nlat, nlon))
+    idx_c[0, 1] = (i + shift) % nlon + 1
+    idx_c[1, 1] = rng.integers(1, nlat + 1, (nlat, nlon))
+    idx_c[2, 1] = rng.integers(1, niv, (nlat, nlon))
+    idx_c[1, 1, 0, :2] = (1, nlat)
+    idx_c[2, 1, 0, :2] = (1, niv - 1)
+    # level 4: every departure cell outside the grid
+    idx_c[1, 3] = nlat + 3
+    idx_c = idx_c.reshape(3, -1)
+    q_in = _dev(tr.q)
+    q_out = torch.full_like(q_in, -7.0)
+    print("[transport] 16x9x7: a level of far cells, a level outside the grid", flush=True)
+    with pytest.raises(RbError):
+        step.transport_interp_device(grid, _dev(idx_c), _dev(w_c), q_in, q_out)
+    torch.cuda.synchronize()
+    q = q_out.cpu().numpy().reshape(len(tr.jc), niv, nlat, nlon)
+    assert np.all(q[:, 3] == -7.0)                 # the points outside the grid: untouched
+    good = np.ones((niv, nlat, nlon), bool)
+    good[3] = False
+    for t in range(len(tr.jc)):
+        ref, _ = cport.internew(tr, idx_c, w_c, tr.q[t])
+        ref = ref.reshape(niv, nlat, nlon)
+        assert np.array_equal(q[t][good], ref[good]), t
