@@ -174,7 +174,7 @@ __global__ void __launch_bounds__(256) k_transport_interp(TransportGrid t, const
 // once per tile into LDS.  Per tile, the window of the extended grid that the tile's
 // stencils cover -- longitudes unwrapped around the tile (a departure cell across the
 // periodic boundary counts from the tile's side), extended rows 0 .. nlat + 2 and levels
-// 0 .. niv + 1 before the index map -- is found by a block reduction.  Each tracer's window
+// 0 .. niv + 2 before the index map -- is found by a block reduction.  Each tracer's window
 // is copied into LDS with coalesced loads (each value once, through interp_point's index
 // map), and each lane reads its 64 stencil values from there.  The copy of tracer tr + 1 is
 // in flight while tracer tr is interpolated (two buffers, one barrier per tracer).
