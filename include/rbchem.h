@@ -272,7 +272,7 @@ int rb_sediment_dev(rb_ctx *ctx, const rb_grid *g, double dt, int32_t nivh2o, co
  *     lies above the floor far under it can stop a box short of the fixed point, because
  *     changes under the floor are not seen; on the c2 grid the learned guess does not (its
  *     answer and the incoming state's differ by more than 1e-9 at 1 box of 786,240, by
- *     6.8e-9).
+ *     1.01e-9).
  *   - iters (device, nlon*nlat*nivbas int32, or NULL) receives the sweep count of each box.
  *   - accel = 2 accelerates the iteration (Anderson mixing of depth 2 over the last three
  *     sweeps, Walker & Ni 2011, from sweep rb_config.conv_aa_start on; a box whose change
