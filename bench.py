@@ -664,7 +664,7 @@ def main():
     block = args.block
     if block == 0:
         cache = os.path.join(tempfile.gettempdir(), "reprobus_amd_tune.json")
-        r = autotune.tune(table, grid, st, mode="plain", cache_path=cache)
+        r = autotune.tune(table, grid, st, mode="plain", cache_path=cache, reps=5, steps=10)
         block = r.block
         tuned["plain"] = {"block": r.block, "ms": r.ms}
 
