@@ -1720,7 +1720,8 @@ __device__ void aa_update(double *ck, double *cc, double *ws, int64_t stride, in
 // MODE CONV_PLAIN: plain sweeps.  With Anderson acceleration (from sweep cv.aa_start) the
 // solve is two launches, so that the wide history code never runs masked in a wave whose
 // other lanes do plain sweeps: MODE CONV_SWEEP sweeps like CONV_PLAIN but hands a box that
-// reaches sweep aa_start unconverged, with its iterate and count, to the spill list;
+// reaches sweep aa_start unconverged, with its iterate and count, to the spill list (at the
+// wave's next serve, so that the sweep loop holds no spill-list code);
 // MODE CONV_ANDERSON takes its boxes from that list and continues them with accelerated
 // sweeps.  The sequence of operations per box is the one-launch sequence.
 //
@@ -1764,6 +1765,9 @@ __global__ void __launch_bounds__(BLK) k_chem_conv(StepArgs a, ConvArgs cv)
     // lane states: active (sweeping), done (holding a result not yet written), restart
     // (to be set up again for the fallback), finished (queue exhausted); none = waiting
     bool active = false, done = false, finished = false, fallback = false, restart = false;
+    // CONV_SWEEP: the box reached sweep aa_start unconverged and goes to the spill list at
+    // the next serve (its lane counts as waiting until then)
+    bool handoff = false;
     int64_t b = 0;
     int it = 0, tried = 0, head = 0, nstored = 0, result = 0;
     // the last two sweeps' largest relative changes and the number of successive plain
@@ -1787,6 +1791,29 @@ __global__ void __launch_bounds__(BLK) k_chem_conv(StepArgs a, ConvArgs cv)
             box_writeback(a, b, x);
             if (cv.iters) cv.iters[b] = result;
             done = false;
+        }
+        if (MODE == CONV_SWEEP) {
+            // the boxes that reached sweep aa_start unconverged: to the spill list, with their
+            // iterate, count and last two changes (one atomic per wave)
+            const unsigned long long gv = __ballot(handoff);
+            if (gv) {
+                const int leader = __ffsll((long long)gv) - 1;
+                unsigned long long base = 0;
+                if (lane == leader) base = atomicAdd(cv.counter + 1, (unsigned long long)__popcll(gv));
+                base = __shfl(base, leader);
+                if (handoff) {
+                    const int64_t s = (int64_t)(base + (unsigned long long)__popcll(gv & ((1ull << lane) - 1ull)));
+#pragma unroll
+                    for (int i = 0; i < NLONG - 1; ++i) cv.spill[i * cap + s] = x.ck[i];
+#pragma unroll
+                    for (int i = 0; i < NSHORT; ++i) cv.spill[(NLONG - 1 + i) * cap + s] = x.cc[i];
+                    sp_box[s] = b;
+                    sp_it[s] = it;
+                    sp_d[s] = d1;
+                    sp_d[cap + s] = d2;
+                    handoff = false;
+                }
+            }
         }
         const unsigned long long waiting = __ballot(!active && !finished && !restart);
         bool setup = restart, resume = false;
@@ -1840,28 +1867,11 @@ __global__ void __launch_bounds__(BLK) k_chem_conv(StepArgs a, ConvArgs cv)
         if (__ballot(active) == 0ull) break;
         // sweep until the wave needs serving
         for (;;) {
-            if (MODE == CONV_SWEEP) {
-                // a box about to take its first accelerated sweep goes to the spill list
-                const bool give = active && !fallback && it + 1 >= cv.aa_start;
-                const unsigned long long gv = __ballot(give);
-                if (gv) {
-                    const int leader = __ffsll((long long)gv) - 1;
-                    unsigned long long base = 0;
-                    if (lane == leader) base = atomicAdd(cv.counter + 1, (unsigned long long)__popcll(gv));
-                    base = __shfl(base, leader);
-                    if (give) {
-                        const int64_t s = (int64_t)(base + (unsigned long long)__popcll(gv & ((1ull << lane) - 1ull)));
-#pragma unroll
-                        for (int i = 0; i < NLONG - 1; ++i) cv.spill[i * cap + s] = x.ck[i];
-#pragma unroll
-                        for (int i = 0; i < NSHORT; ++i) cv.spill[(NLONG - 1 + i) * cap + s] = x.cc[i];
-                        sp_box[s] = b;
-                        sp_it[s] = it;
-                        sp_d[s] = d1;
-                        sp_d[cap + s] = d2;
-                        active = false;
-                    }
-                }
+            if (MODE == CONV_SWEEP && active && !fallback && it + 1 >= cv.aa_start) {
+                // a box about to take its first accelerated sweep leaves the sweeps; the
+                // serve puts it on the spill list
+                handoff = true;
+                active = false;
             }
             if (active) {
                 const bool acc = AAM > 0 && !fallback && it + 1 >= cv.aa_start;
