@@ -77,6 +77,8 @@ def parse():
                     help="BLOCK,CHUNK of the converged-mode kernels (default: autotuned)")
     ap.add_argument("--newton-shape", default="",
                     help="SWEEPS_FIRST,WAVES of the Newton solve (default: autotuned)")
+    ap.add_argument("--max-newton", type=int, default=20,
+                    help="Newton iterations a box gets before it goes back to plain sweeps")
     ap.add_argument("--conv-order", type=int, default=0,
                     help="converged mode's work order: 0 night boxes then day boxes, 1 grid order")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
@@ -755,6 +757,7 @@ def main():
     if "converged" in want and conv_shape:
         try:
             extras["converged"] = converged_section(table, *conv_shape, dev, newton_shape,
+                                                    max_newton=args.max_newton,
                                                     order=args.conv_order)
         except Exception as exc:
             extras["converged"] = {"error": repr(exc)}
