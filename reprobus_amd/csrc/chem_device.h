@@ -96,11 +96,7 @@ hipError_t launch_transport_weights(const TransportGrid &t, const double *aaa, c
                                     const double *x0, const double *y0, const double *z0,
                                     const double *p0, int32_t *idx, double *w,
                                     unsigned long long *bad, hipStream_t s);
-// tracers per workgroup of the gathering interpolation (a grid dimension: a smaller group
-// keeps the cached neighbourhoods of fewer fields live at once, at the cost of reading each
-// point's indices and weights once per group); the tiled one takes every tracer
-constexpr int TRANSPORT_GROUP = 11;
-// values of a tile's window in each of the tiled interpolation's two LDS buffers
+// values of a tile's window in each of the interpolation's two LDS buffers
 constexpr int TRANSPORT_WINDOW = 3200;
 hipError_t launch_transport_interp(const TransportGrid &t, const int32_t *idx, const double *w,
                                    const double *q_in, int64_t sin_, double *q_out, int64_t sout,

@@ -133,44 +133,8 @@ __device__ __forceinline__ void interp_point(const TransportGrid &t, int i, int 
 }
 
 // ntr fields (q_in + t * sin, t < ntr) interpolated at the departure points into
-// q_out + t * sout.  A point whose indices fall outside the grid is counted in *bad and
-// not written.  One lane per point, the stencils gathered from the fields (interp_point):
-// the launch for grids whose field size does not fit the tiled kernel's 32-bit offsets.
-//
-// Launch shape: workgroups go round-robin to the 8 XCDs, so workgroup j computes the
-// points of tile (j mod 8) * per + j / 8 (per = tiles per XCD, the grid rounded up to
-// 8 * per workgroups): each XCD takes one contiguous eighth of the grid, and the
-// neighbouring rows and levels its stencils share stay in that XCD's L2.
-__global__ void __launch_bounds__(256) k_transport_interp(TransportGrid t, const int32_t *__restrict__ idx,
-                                                          const double *__restrict__ w,
-                                                          const double *__restrict__ q_in, int64_t sin_,
-                                                          double *__restrict__ q_out, int64_t sout,
-                                                          int ntr, int64_t n, int group, int ntiles,
-                                                          unsigned long long *__restrict__ bad)
-{
-    const int per = (ntiles + 7) / 8;
-    const int64_t tile = (int64_t)(blockIdx.x % 8) * per + blockIdx.x / 8;
-    const int64_t p = tile * blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    const int i = idx[p], k = idx[n + p], l = idx[2 * n + p];
-    if (!(i >= 1 && i <= t.nlon && k >= 1 && k <= t.nlat && l >= 1 && l <= t.niv)) {
-        if (blockIdx.y == 0) atomicAdd(bad, 1ull);
-        return;
-    }
-    // this block's tracers: [blockIdx.y * group, min(ntr, (blockIdx.y + 1) * group))
-    const int tr0 = (int)blockIdx.y * group;
-    const int tr1 = tr0 + group < ntr ? tr0 + group : ntr;
-    double wx[4], wy[4], wz[4];
-#pragma unroll
-    for (int m = 0; m < 4; ++m) {
-        wx[m] = w[m * n + p];
-        wy[m] = w[(4 + m) * n + p];
-        wz[m] = w[(8 + m) * n + p];
-    }
-    interp_point(t, i, k, l, wx, wy, wz, q_in, sin_, q_out, sout, tr0, tr1, p);
-}
-
-// The same interpolation over tiles of TX x TY points of one level, with the fields read
+// q_out + t * sout; a point whose indices fall outside the grid is counted in *bad and not
+// written.  A workgroup takes a tile of TX x TY points of one level, with the fields read
 // once per tile into LDS.  Per tile, the window of the extended grid that the tile's
 // stencils cover -- longitudes unwrapped around the tile (a departure cell across the
 // periodic boundary counts from the tile's side), extended rows 0 .. nlat + 2 and levels
@@ -182,7 +146,7 @@ __global__ void __launch_bounds__(256) k_transport_interp(TransportGrid t, const
 // If the window of every stencil exceeds WMAX values, the window covers only the points
 // whose departure cell lies within RX, RY, RZ cells of the arrival point (it then fits by
 // construction), and the others -- next to the poles, where a cell spans few kilometres of
-// longitude -- gather from the fields as k_transport_interp does.  The arithmetic is
+// longitude -- gather their stencils from the fields (interp_point).  The arithmetic is
 // interp_point's in every case.
 //
 // Tiles go level-fastest within each XCD's contiguous range (workgroup j takes tile
@@ -192,7 +156,7 @@ __global__ void __launch_bounds__(256) k_transport_interp(TransportGrid t, const
 // The copies wait on memory latency, so occupancy decides the time: WMAX sets the LDS per
 // workgroup (two buffers) and the register bound three workgroups per CU, twelve waves
 // (16 x 16 tiles, WMAX 3200: 1.96 ms for 43 tracers on 180 x 91 x 137 points, against 2.30
-// with two workgroups per CU and the gathering kernel's 3.04).
+// with two workgroups per CU and 3.04 with every stencil gathered from the fields).
 template <int TX, int TY, int WMAX>
 __global__ void __launch_bounds__(TX * TY, 3) k_transport_interp_tiled(
     TransportGrid t, const int32_t *__restrict__ idx, const double *__restrict__ w,
@@ -397,30 +361,20 @@ hipError_t launch_transport_interp(const TransportGrid &t, const int32_t *idx, c
                                    const double *q_in, int64_t sin_, double *q_out, int64_t sout,
                                    int ntr, unsigned long long *bad, hipStream_t s, int group)
 {
+    // field offsets are 32-bit: grids up to 2^31 - 1 points (the entry points check it)
     const int64_t n = (int64_t)t.nlon * t.nlat * t.niv;
-    if (n < INT32_MAX) {
-        // 16 x 16 tiles of one level, every tracer in one workgroup (the tile's set-up --
-        // window, index map, weights -- done once)
-        constexpr int TX = 16, TY = 16;
-        if (group < 1) group = ntr;
-        const unsigned ng = (unsigned)((ntr + group - 1) / group);
-        const int ntx = (t.nlon + TX - 1) / TX, nty = (t.nlat + TY - 1) / TY;
-        const int64_t ntiles = (int64_t)ntx * nty * t.niv;
-        if (ntiles < INT32_MAX / 2) {
-            const unsigned nwg = (unsigned)(((ntiles + 7) / 8) * 8);
-            hipLaunchKernelGGL((k_transport_interp_tiled<TX, TY, TRANSPORT_WINDOW>), dim3(nwg, ng),
-                               dim3(TX * TY), 0, s, t, idx, w, q_in, sin_, q_out, sout, ntr, n,
-                               group, ntx, (int)ntiles, bad);
-            return hipGetLastError();
-        }
-    }
-    if (group < 1) group = TRANSPORT_GROUP;
+    if (n > INT32_MAX) return hipErrorInvalidValue;
+    // 16 x 16 tiles of one level, every tracer in one workgroup (the tile's set-up --
+    // window, index map, weights -- done once)
+    constexpr int TX = 16, TY = 16;
+    if (group < 1) group = ntr;
     const unsigned ng = (unsigned)((ntr + group - 1) / group);
-    // 256-point tiles, the workgroups rounded up to a multiple of the 8 XCDs
-    const int ntiles = (int)((n + 255) / 256);
+    const int ntx = (t.nlon + TX - 1) / TX, nty = (t.nlat + TY - 1) / TY;
+    const int64_t ntiles = (int64_t)ntx * nty * t.niv;
     const unsigned nwg = (unsigned)(((ntiles + 7) / 8) * 8);
-    hipLaunchKernelGGL(k_transport_interp, dim3(nwg, ng), dim3(256), 0, s, t, idx, w, q_in, sin_,
-                       q_out, sout, ntr, n, group, ntiles, bad);
+    hipLaunchKernelGGL((k_transport_interp_tiled<TX, TY, TRANSPORT_WINDOW>), dim3(nwg, ng),
+                       dim3(TX * TY), 0, s, t, idx, w, q_in, sin_, q_out, sout, ntr, n, group,
+                       ntx, (int)ntiles, bad);
     return hipGetLastError();
 }
 
